@@ -1,0 +1,389 @@
+"""Python binding of the MI355X search core (libdmr.so, C ABI in include/dmr.h).
+
+Mirrors the reference's search surface: SearchConfig (src/search.rs:19-80),
+blastp/blastx (search::run + merge_frames, src/search.rs:95-146, 357-396) and
+the drop-in calls seed::build, seed::join_into, ungapped::xdrop,
+gapped::extend/extend_span, gapped::band_for. Every call runs on the GPU
+through libdmr.so; there is no CPU fallback. Importing works without a GPU
+(the library only needs the HIP runtime); creating a Context needs one.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libdmr.so"
+
+DMR_OK = 0
+DMR_E_ARG = -1
+DMR_E_DEVICE = -2
+DMR_E_OOM = -3
+DMR_E_CAPACITY = -4
+DMR_E_UNSUPPORTED = -5
+
+# Symbols declared by include/dmr.h.
+EXPORTS = [
+    "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
+    "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
+    "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend",
+    "dmr_extend_span_batch", "dmr_db_load", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
+    "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
+    "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy",
+]
+
+HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
+              "identities", "positives", "mismatches", "gap_openings", "gaps", "length", "n_ops"]
+
+STAT_FIELDS = ["query_seeds", "runs", "hits", "hit_runs", "segments", "groups", "jobs", "maybe",
+               "selected", "traceback_cells", "ms_seed", "ms_join", "ms_sort_hits",
+               "ms_ungapped", "ms_bands", "ms_sw", "ms_select", "ms_traceback", "ms_total",
+               "ms_host"]
+
+
+class DmrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dmr error {code}: {msg}")
+        self.code = code
+
+
+class SeedEntry(C.Structure):
+    _fields_ = [("key", C.c_uint64), ("seq", C.c_uint32), ("pos", C.c_uint32)]
+
+
+class SeedHit(C.Structure):
+    _fields_ = [("query", C.c_uint32), ("qpos", C.c_uint32), ("subject", C.c_uint32),
+                ("spos", C.c_uint32)]
+
+
+class Segment(C.Structure):
+    _fields_ = [("q_start", C.c_uint64), ("s_start", C.c_uint64), ("len", C.c_uint64),
+                ("score", C.c_int32), ("reserved", C.c_int32)]
+
+
+class HspStats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in
+                ("identities", "positives", "mismatches", "gap_openings", "gaps", "length")]
+
+
+class Aligned(C.Structure):
+    _fields_ = [("found", C.c_int32), ("score", C.c_int32), ("q_begin", C.c_uint64),
+                ("q_end", C.c_uint64), ("s_begin", C.c_uint64), ("s_end", C.c_uint64),
+                ("stats", HspStats), ("n_ops", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class SpanJob(C.Structure):
+    _fields_ = [("q_off", C.c_uint64), ("q_len", C.c_uint64), ("s_off", C.c_uint64),
+                ("s_len", C.c_uint64), ("d0", C.c_int64), ("d1", C.c_int64)]
+
+
+class _Cfg(C.Structure):
+    _fields_ = [("sensitivity", C.c_char_p), ("evalue", C.c_double),
+                ("max_target_seqs", C.c_uint32), ("max_hsps", C.c_uint32),
+                ("index_chunks", C.c_int32), ("gap_open", C.c_int32), ("gap_extend", C.c_int32),
+                ("min_orf", C.c_int32), ("seed_cut", C.c_double)]
+
+
+@dataclasses.dataclass
+class SearchConfig:
+    """search::SearchConfig defaults (src/search.rs:36-48, src/cli.rs:125-140)."""
+    sensitivity: str = "default"
+    evalue: float = 0.001
+    max_target_seqs: int = 25
+    max_hsps: int = 1
+    index_chunks: int = -1
+    gap_open: int = -1
+    gap_extend: int = -1
+    min_orf: int = -1
+    seed_cut: float = -1.0
+
+    def _c(self):
+        self._keep = self.sensitivity.encode()
+        return _Cfg(self._keep, self.evalue, self.max_target_seqs, self.max_hsps,
+                    self.index_chunks, self.gap_open, self.gap_extend, self.min_orf,
+                    self.seed_cut)
+
+
+_lib = None
+
+
+def lib():
+    """Loads libdmr.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run diamond-rust_amd/build.py")
+        L = C.CDLL(str(LIB_PATH))
+        P, U64, I64, U32, I32, D = C.c_void_p, C.c_uint64, C.c_int64, C.c_uint32, C.c_int32, C.c_double
+        sig = {
+            "dmr_last_error": ([], C.c_char_p), "dmr_version": ([], C.c_char_p),
+            "dmr_ctx_create": ([C.c_int, P], C.c_int), "dmr_ctx_destroy": ([P], None),
+            "dmr_device_count": ([P], C.c_int), "dmr_band_for": ([U64], U32),
+            "dmr_evalue": ([I32, I32, I32, U64, U64, U64, P, P], C.c_int),
+            "dmr_seed_build": ([P, P, P, U32, C.c_char_p, C.c_char_p, U32, D, P, U64, P], C.c_int),
+            "dmr_seed_join": ([P, P, U64, P, U64, U64, P, U64, P], C.c_int),
+            "dmr_xdrop": ([P, P, U64, P, U64, U64, U64, I32, P], C.c_int),
+            "dmr_xdrop_batch": ([P, P, U64, P, U64, P, P, U64, I32, P], C.c_int),
+            "dmr_extend_span": ([P, P, U64, P, U64, I64, I64, I32, I32, P, P, U32], C.c_int),
+            "dmr_extend": ([P, P, U64, P, U64, U64, U64, U64, I32, I32, P, P, U32], C.c_int),
+            "dmr_extend_span_batch": ([P, P, U64, P, U64, P, U64, I32, I32, C.c_int, P, P, U64, P, P],
+                                      C.c_int),
+            "dmr_db_load": ([P, P, P, U32], C.c_int),
+            "dmr_db_build_index": ([P, C.c_char_p, I32, D, P], C.c_int),
+            "dmr_blastp": ([P, P, P, U32, P, P], C.c_int),
+            "dmr_blastx": ([P, P, P, U32, P, P], C.c_int),
+            "dmr_result_count": ([P], I64), "dmr_result_hsps": ([P, P, P], C.c_int),
+            "dmr_result_ops": ([P, I64, P, U32, P], C.c_int),
+            "dmr_result_format": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, U64, P], C.c_int),
+            "dmr_result_stats": ([P, P], C.c_int), "dmr_last_dp_cells": ([P, P], C.c_int),
+            "dmr_result_destroy": ([P], None), "dmr_search_config_default": ([P], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != DMR_OK:
+        raise DmrError(rc, lib().dmr_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and len(a) else None
+
+
+def band_for(length: int) -> int:
+    """gapped::band_for (src/gapped.rs:27-32)."""
+    return int(lib().dmr_band_for(length))
+
+
+def evalue(raw: int, qlen: int, slen: int, db_letters: int, gap_open=11, gap_extend=1):
+    e, b = C.c_double(), C.c_double()
+    _check(lib().dmr_evalue(gap_open, gap_extend, raw, qlen, slen, db_letters, C.byref(e), C.byref(b)))
+    return e.value, b.value
+
+
+def pack(seqs):
+    """Concatenated uint8 letters + uint64 offsets (n+1) of a list of arrays."""
+    seqs = [np.asarray(s, dtype=np.uint8) for s in seqs]
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    if seqs:
+        np.cumsum([len(s) for s in seqs], out=off[1:])
+    letters = np.concatenate(seqs) if seqs else np.zeros(0, np.uint8)
+    return np.ascontiguousarray(letters, dtype=np.uint8), off
+
+
+class Result:
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and lib is not None:
+            try:
+                lib().dmr_result_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __len__(self):
+        return int(lib().dmr_result_count(self._h))
+
+    def hsps(self):
+        n = len(self)
+        ints = np.zeros((n, 16), dtype=np.int64)
+        dbl = np.zeros((n, 2), dtype=np.float64)
+        if n:
+            _check(lib().dmr_result_hsps(self._h, ints.ctypes.data, dbl.ctypes.data))
+        out = {k: ints[:, i] for i, k in enumerate(HSP_FIELDS)}
+        out["bits"] = dbl[:, 0]
+        out["evalue"] = dbl[:, 1]
+        return out
+
+    def ops(self, i):
+        n = C.c_uint32()
+        buf = np.zeros(1 << 16, dtype=np.uint32)
+        _check(lib().dmr_result_ops(self._h, i, buf.ctypes.data, len(buf), C.byref(n)))
+        return buf[: n.value].copy()
+
+    def format(self, query_ids, subject_ids, fields="") -> str:
+        q = "\n".join(query_ids).encode()
+        s = "\n".join(subject_ids).encode()
+        f = fields.encode() if isinstance(fields, str) else " ".join(fields).encode()
+        n = C.c_uint64()
+        _check(lib().dmr_result_format(self._h, q, s, f, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(lib().dmr_result_format(self._h, q, s, f, buf, n.value + 1, C.byref(n)))
+        return buf.raw[: n.value].decode()
+
+    def stats(self):
+        v = np.zeros(20, dtype=np.float64)
+        _check(lib().dmr_result_stats(self._h, v.ctypes.data))
+        return dict(zip(STAT_FIELDS, v.tolist()))
+
+
+class Context:
+    """One GPU, its stream, the resident database and seed index."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().dmr_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().dmr_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- database / search ------------------------------------------------
+    def db_load(self, letters, offsets):
+        letters = np.ascontiguousarray(letters, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self._db = (letters, offsets)
+        _check(lib().dmr_db_load(self._h, _ptr(letters), offsets.ctypes.data, len(offsets) - 1))
+
+    def build_index(self, sensitivity="default", index_chunks=-1, seed_cut=-1.0) -> float:
+        ms = C.c_double()
+        _check(lib().dmr_db_build_index(self._h, sensitivity.encode(), index_chunks, seed_cut,
+                                        C.byref(ms)))
+        return ms.value
+
+    def blastp(self, letters, offsets, cfg: SearchConfig | None = None) -> Result:
+        cfg = cfg or SearchConfig()
+        letters = np.ascontiguousarray(letters, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        c = cfg._c()
+        h = C.c_void_p()
+        _check(lib().dmr_blastp(self._h, _ptr(letters), offsets.ctypes.data, len(offsets) - 1,
+                                C.byref(c), C.byref(h)))
+        return Result(h.value)
+
+    def blastx(self, reads, cfg: SearchConfig | None = None) -> Result:
+        cfg = cfg or SearchConfig()
+        data = "".join(reads).encode()
+        off = np.zeros(len(reads) + 1, dtype=np.uint64)
+        np.cumsum([len(r) for r in reads], out=off[1:])
+        c = cfg._c()
+        h = C.c_void_p()
+        buf = C.create_string_buffer(data, len(data) + 1)
+        _check(lib().dmr_blastx(self._h, buf, off.ctypes.data, len(reads), C.byref(c), C.byref(h)))
+        return Result(h.value)
+
+    def last_dp_cells(self) -> int:
+        v = C.c_uint64()
+        _check(lib().dmr_last_dp_cells(self._h, C.byref(v)))
+        return v.value
+
+    # -- drop-in calls ------------------------------------------------------
+    def seed_build(self, seqs, shape_code, reduction_def, base=0, cut=0.0):
+        letters, off = pack(seqs)
+        n = C.c_uint64()
+        cap = max(1, int(len(letters)))
+        out = (SeedEntry * cap)()
+        _check(lib().dmr_seed_build(self._h, _ptr(letters), off.ctypes.data, len(seqs),
+                                    shape_code.encode(), reduction_def.encode(), base, cut,
+                                    out, cap, C.byref(n)))
+        a = np.ctypeslib.as_array(out)[: n.value]
+        return a["key"].copy(), a["seq"].copy(), a["pos"].copy()
+
+    def seed_join(self, q, r, max_expand=4096):
+        """q, r: (keys, seqs, poss) tuples sorted by key. Returns an (n, 4)
+        array of (query, qpos, subject, spos)."""
+        def entries(t):
+            k, s, p = (np.asarray(x) for x in t)
+            arr = (SeedEntry * max(1, len(k)))()
+            a = np.ctypeslib.as_array(arr)
+            a["key"][: len(k)] = k
+            a["seq"][: len(k)] = s
+            a["pos"][: len(k)] = p
+            return arr, len(k)
+        qa, nq = entries(q)
+        ra, nr = entries(r)
+        n = C.c_uint64()
+        rc = lib().dmr_seed_join(self._h, qa, nq, ra, nr, max_expand, None, 0, C.byref(n))
+        if rc not in (DMR_OK, DMR_E_CAPACITY):
+            _check(rc)
+        out = (SeedHit * max(1, n.value))()
+        _check(lib().dmr_seed_join(self._h, qa, nq, ra, nr, max_expand, out, n.value, C.byref(n)))
+        a = np.ctypeslib.as_array(out)[: n.value]
+        return np.stack([a["query"], a["qpos"], a["subject"], a["spos"]], axis=1).astype(np.int64)
+
+    def xdrop(self, query, subject, qa, sa, xdrop):
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        s = np.ascontiguousarray(subject, dtype=np.uint8)
+        o = Segment()
+        _check(lib().dmr_xdrop(self._h, _ptr(q), len(q), _ptr(s), len(s), qa, sa, xdrop, C.byref(o)))
+        return (o.q_start, o.s_start, o.len, o.score)
+
+    def xdrop_batch(self, query, subject, qa, sa, xdrop):
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        s = np.ascontiguousarray(subject, dtype=np.uint8)
+        qa = np.ascontiguousarray(qa, dtype=np.uint32)
+        sa = np.ascontiguousarray(sa, dtype=np.uint32)
+        out = (Segment * max(1, len(qa)))()
+        _check(lib().dmr_xdrop_batch(self._h, _ptr(q), len(q), _ptr(s), len(s), _ptr(qa), _ptr(sa),
+                                     len(qa), xdrop, out))
+        a = np.ctypeslib.as_array(out)[: len(qa)]
+        return np.stack([a["q_start"], a["s_start"], a["len"], a["score"]], axis=1).astype(np.int64)
+
+    def extend_span(self, query, subject, d0, d1, gap_open=11, gap_extend=1):
+        """gapped::extend_span: None or a dict with score, coordinates, stats, ops."""
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        s = np.ascontiguousarray(subject, dtype=np.uint8)
+        out = Aligned()
+        ops = np.zeros(len(q) + len(s) + 1, dtype=np.uint32)
+        _check(lib().dmr_extend_span(self._h, _ptr(q), len(q), _ptr(s), len(s), d0, d1, gap_open,
+                                     gap_extend, C.byref(out), ops.ctypes.data, len(ops)))
+        return _aligned_dict(out, ops[: out.n_ops])
+
+    def extend(self, query, subject, anchor, band, gap_open=11, gap_extend=1):
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        s = np.ascontiguousarray(subject, dtype=np.uint8)
+        out = Aligned()
+        ops = np.zeros(len(q) + len(s) + 1, dtype=np.uint32)
+        _check(lib().dmr_extend(self._h, _ptr(q), len(q), _ptr(s), len(s), anchor[0], anchor[1],
+                                band, gap_open, gap_extend, C.byref(out), ops.ctypes.data, len(ops)))
+        return _aligned_dict(out, ops[: out.n_ops])
+
+    def extend_span_batch(self, qbuf, sbuf, jobs, traceback=True, gap_open=11, gap_extend=1):
+        """jobs: iterable of (q_off, q_len, s_off, s_len, d0, d1)."""
+        qb = np.ascontiguousarray(qbuf, dtype=np.uint8)
+        sb = np.ascontiguousarray(sbuf, dtype=np.uint8)
+        jobs = list(jobs)
+        J = (SpanJob * max(1, len(jobs)))(*[SpanJob(*j) for j in jobs])
+        out = (Aligned * max(1, len(jobs)))()
+        offs = np.zeros(len(jobs) + 1, dtype=np.uint64)
+        need = C.c_uint64()
+        cap = sum(j[1] + j[3] + 1 for j in jobs) if traceback else 0
+        pool = np.zeros(max(1, cap), dtype=np.uint32)
+        _check(lib().dmr_extend_span_batch(self._h, _ptr(qb), len(qb), _ptr(sb), len(sb), J,
+                                           len(jobs), gap_open, gap_extend, 1 if traceback else 0,
+                                           out, pool.ctypes.data, len(pool), offs.ctypes.data,
+                                           C.byref(need)))
+        res = []
+        for i in range(len(jobs)):
+            res.append(_aligned_dict(out[i], pool[offs[i]:offs[i + 1]]) if traceback
+                       else (out[i].score if out[i].found else None))
+        return res
+
+
+def _aligned_dict(a: Aligned, ops):
+    if not a.found:
+        return None
+    st = a.stats
+    return dict(score=a.score, q_begin=a.q_begin, q_end=a.q_end, s_begin=a.s_begin, s_end=a.s_end,
+                identities=st.identities, positives=st.positives, mismatches=st.mismatches,
+                gap_openings=st.gap_openings, gaps=st.gaps, length=st.length,
+                ops=[(int(o) & 3, int(o) >> 2) for o in ops])

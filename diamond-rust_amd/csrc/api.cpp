@@ -1,0 +1,652 @@
+// C ABI (include/dmr.h): drop-in entry points, blastp/blastx drivers,
+// frame merging and outfmt-6 rendering.
+#include "../../include/dmr.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+struct dmr_ctx {
+  std::unique_ptr<dmr::Engine> eng;
+};
+
+struct dmr_result {
+  std::vector<dmr::HspRec> hsps;     // output order
+  std::vector<int> target_of;        // target index of each HSP
+  dmr::TargetSet targets;
+  std::vector<uint64_t> dna_len;     // per query record (blastx)
+  bool translated = false;
+  std::vector<uint32_t> db_offs;     // host copy for subject slices
+  const std::vector<uint8_t>* db_res = nullptr;
+  std::vector<uint8_t> db_res_copy;
+  dmr::StageStats stats;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// Runs f and maps exceptions to status codes (no exception crosses the ABI).
+template <class F>
+int guard(F&& f) {
+  try {
+    return f();
+  } catch (const dmr::Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(DMR_E_OOM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(DMR_E_DEVICE, e.what());
+  }
+}
+
+using dmr::HspRec;
+
+bool hsp_less(const HspRec& a, const HspRec& b) {
+  if (a.score != b.score) return a.score > b.score;
+  if (a.evalue < b.evalue) return true;
+  if (b.evalue < a.evalue) return false;
+  return a.subject < b.subject;
+}
+
+// search::merge_frames (src/search.rs:357-396): per query, all frames'
+// HSPs sorted by cmp_hsp, at most max_hsps per subject, subjects in order of
+// first appearance up to max_targets.
+std::vector<std::vector<HspRec>> merge_frames(std::vector<std::vector<HspRec>>& per_target,
+                                              const dmr::TargetSet& t, size_t nqueries,
+                                              size_t max_hsps, size_t max_targets) {
+  std::vector<std::vector<HspRec>> per(nqueries);
+  for (size_t ti = 0; ti < per_target.size(); ++ti) {
+    size_t q = t.query_of[ti];
+    for (auto& h : per_target[ti]) {
+      h.frame = t.frame_of[ti];
+      h.query = (uint32_t)q;
+      per[q].push_back(std::move(h));
+    }
+  }
+  for (auto& hs : per) {
+    std::stable_sort(hs.begin(), hs.end(), hsp_less);
+    std::vector<uint32_t> order;
+    std::map<uint32_t, std::vector<HspRec>> by;
+    for (auto& h : hs) {
+      auto it = by.find(h.subject);
+      if (it != by.end()) {
+        if (it->second.size() < max_hsps) it->second.push_back(std::move(h));
+      } else if (order.size() < max_targets) {
+        order.push_back(h.subject);
+        by[h.subject].push_back(std::move(h));
+      }
+    }
+    hs.clear();
+    for (uint32_t s : order)
+      for (auto& h : by[s]) hs.push_back(std::move(h));
+  }
+  return per;
+}
+
+// translate.rs: six frames (0-2 forward offsets, 3-5 reverse complement) with
+// the standard code; codons with ambiguous bases become X.
+uint8_t nt_code(char b) {
+  switch (b) {
+    case 'T': case 't': case 'U': case 'u': return 0;
+    case 'C': case 'c': return 1;
+    case 'A': case 'a': return 2;
+    case 'G': case 'g': return 3;
+    default: return 4;
+  }
+}
+char nt_comp(char b) {
+  switch (b) {
+    case 'A': return 'T'; case 'T': return 'A'; case 'C': return 'G'; case 'G': return 'C';
+    case 'U': return 'A'; case 'R': return 'Y'; case 'Y': return 'R'; case 'S': return 'S';
+    case 'W': return 'W'; case 'K': return 'M'; case 'M': return 'K'; case 'B': return 'V';
+    case 'V': return 'B'; case 'D': return 'H'; case 'H': return 'D'; case 'N': return 'N';
+    default: return b;
+  }
+}
+std::vector<uint8_t> translate(const std::string& nt, size_t off) {
+  static const char code[] = "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
+  std::vector<uint8_t> out;
+  for (size_t i = off; i + 3 <= nt.size(); i += 3) {
+    uint8_t a = nt_code(nt[i]), b = nt_code(nt[i + 1]), c = nt_code(nt[i + 2]);
+    out.push_back((a > 3 || b > 3 || c > 3) ? 23 : dmr::encode_letter((uint8_t)code[a * 16 + b * 4 + c]));
+  }
+  return out;
+}
+// --min-orf masking (src/translate.rs:101-121): stop-free runs shorter than
+// min_len become X.
+void mask_orfs(std::vector<uint8_t>& f, size_t min_len) {
+  if (min_len <= 1) return;
+  size_t begin = 0;
+  for (size_t i = 0; i < f.size(); ++i)
+    if (f[i] == 24) {
+      if (i - begin < min_len) std::fill(f.begin() + begin, f.begin() + i, (uint8_t)23);
+      begin = i + 1;
+    }
+  if (f.size() - begin < min_len) std::fill(f.begin() + begin, f.end(), (uint8_t)23);
+}
+size_t default_min_orf(size_t len) { return len < 30 ? 1 : (len < 100 ? 20 : 40); }
+
+dmr::SearchOptions options(const dmr_search_config* c) {
+  dmr::SearchOptions o;
+  if (!c) return o;
+  o.sensitivity = c->sensitivity ? c->sensitivity : "default";
+  o.evalue = c->evalue;
+  o.max_target_seqs = c->max_target_seqs;
+  o.max_hsps = c->max_hsps;
+  o.index_chunks = c->index_chunks;
+  o.seed_cut = c->seed_cut;
+  o.gap_open = c->gap_open == -1 ? 11 : c->gap_open;
+  o.gap_extend = c->gap_extend == -1 ? 1 : c->gap_extend;
+  return o;
+}
+
+void finish(dmr_ctx* ctx, dmr_result* r, const dmr::SearchOptions& o, size_t nqueries) {
+  ctx->eng->load_targets(r->targets);
+  auto per_target = ctx->eng->search_targets(o, &r->stats);
+  auto per = merge_frames(per_target, r->targets, nqueries, o.max_hsps, o.max_target_seqs);
+  for (auto& hs : per)
+    for (auto& h : hs) {
+      int ti = -1;
+      for (size_t t = 0; t < r->targets.query_of.size(); ++t)
+        if (r->targets.query_of[t] == h.query && r->targets.frame_of[t] == h.frame) {
+          ti = (int)t;
+          break;
+        }
+      r->target_of.push_back(ti);
+      r->hsps.push_back(std::move(h));
+    }
+  r->db_offs = ctx->eng->db_offsets_host();
+  r->db_res = &r->db_res_copy;
+  r->db_res_copy = ctx->eng->db_letters_host();
+}
+
+// src/output.rs:235-241
+std::string fmt_double(double x) {
+  char b[64];
+  if (x >= 100.0) std::snprintf(b, sizeof b, "%lld", (long long)std::floor(x));
+  else std::snprintf(b, sizeof b, "%.1f", x);
+  return b;
+}
+// src/output.rs:244-250 (Rust {:.2e}: no exponent sign padding)
+std::string fmt_evalue(double e) {
+  if (!std::isfinite(e) || e <= 0.0) return "0.0";
+  char b[64];
+  std::snprintf(b, sizeof b, "%.2e", e);
+  const char* p = std::strchr(b, 'e');
+  return std::string(b, (size_t)(p - b)) + "e" + std::to_string(std::atoi(p + 1));
+}
+
+std::vector<std::string> split(const char* s, char sep) {
+  std::vector<std::string> out;
+  if (!s) return out;
+  std::string cur;
+  for (const char* p = s; *p; ++p) {
+    if (*p == sep) {
+      if (sep != ' ' || !cur.empty()) out.push_back(cur);
+      cur.clear();
+    } else {
+      cur += *p;
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+void set_aligned(const dmr::SwOut& o, dmr_aligned* a, int64_t q_shift) {
+  std::memset(a, 0, sizeof *a);
+  a->found = o.score > 0 ? 1 : 0;
+  a->score = o.score;
+  if (!a->found) return;
+  a->q_begin = (uint64_t)(q_shift + o.q_begin);
+  a->q_end = (uint64_t)(q_shift + o.best_i);
+  a->s_begin = o.s_begin;
+  a->s_end = o.best_j;
+  a->stats = {o.identities, o.positives, o.mismatches, o.gap_openings, o.gaps, o.length};
+  a->n_ops = o.n_ops;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dmr_last_error(void) { return g_err.c_str(); }
+const char* dmr_version(void) { return "dmr 0.1 (gfx950)"; }
+
+void dmr_search_config_default(dmr_search_config* c) {
+  if (!c) return;
+  c->sensitivity = "default";
+  c->evalue = 0.001;
+  c->max_target_seqs = 25;
+  c->max_hsps = 1;
+  c->index_chunks = -1;
+  c->gap_open = -1;
+  c->gap_extend = -1;
+  c->min_orf = -1;
+  c->seed_cut = -1.0;
+}
+
+int dmr_device_count(int* n) {
+  return guard([&]() -> int {
+    DMR_HIP(hipGetDeviceCount(n));
+    return DMR_OK;
+  });
+}
+
+int dmr_ctx_create(int device, dmr_ctx** out) {
+  if (!out) return fail(DMR_E_ARG, "out is NULL");
+  return guard([&]() -> int {
+    auto c = std::make_unique<dmr_ctx>();
+    c->eng = std::make_unique<dmr::Engine>(device);
+    *out = c.release();
+    return DMR_OK;
+  });
+}
+
+void dmr_ctx_destroy(dmr_ctx* ctx) { delete ctx; }
+
+uint32_t dmr_band_for(uint64_t len) { return dmr::band_for(len); }
+
+int dmr_evalue(int32_t go, int32_t ge, int32_t raw, uint64_t qlen, uint64_t slen, uint64_t db,
+               double* evalue, double* bits) {
+  dmr::GapRow r;
+  if (go == -1) go = 11;
+  if (ge == -1) ge = 1;
+  if (!dmr::blosum62_gap_row(go, ge, &r)) return fail(DMR_E_ARG, "unsupported gap costs");
+  dmr::AlpParams p = dmr::alp_make(r.lambda, r.k, r.alpha, r.alpha_v, r.sigma, go, ge,
+                                   dmr::kB62UngappedAlpha, dmr::kB62UngappedAlphaV);
+  if (evalue) *evalue = dmr::alp_evalue(p, raw, qlen, slen, db);
+  if (bits) *bits = dmr::bit_score(p, raw);
+  return DMR_OK;
+}
+
+int dmr_seed_build(dmr_ctx* ctx, const uint8_t* seqs, const uint64_t* offsets, uint32_t nseq,
+                   const char* shape_code, const char* reduction_def, uint32_t base,
+                   double complexity_cut, dmr_seed_entry* out, uint64_t cap, uint64_t* n_out) {
+  if (!ctx || !offsets || !shape_code || !reduction_def || !n_out) return fail(DMR_E_ARG, "NULL argument");
+  dmr::ShapeDef sh;
+  if (!dmr::parse_shape(shape_code, &sh)) return fail(DMR_E_ARG, "invalid seed shape");
+  dmr::ReductionDef red = dmr::parse_reduction(reduction_def);
+  if (red.size == 0) return fail(DMR_E_ARG, "empty reduction definition");
+  return guard([&]() -> int {
+    std::vector<uint64_t> k;
+    std::vector<uint32_t> s, p;
+    ctx->eng->seed_build(seqs, offsets, nseq, sh, red, base, complexity_cut, &k, &s, &p);
+    *n_out = k.size();
+    if (k.size() > cap) return fail(DMR_E_CAPACITY, "output capacity too small");
+    for (size_t i = 0; i < k.size(); ++i) out[i] = dmr_seed_entry{k[i], s[i], p[i]};
+    return DMR_OK;
+  });
+}
+
+int dmr_seed_join(dmr_ctx* ctx, const dmr_seed_entry* q, uint64_t nq, const dmr_seed_entry* r,
+                  uint64_t nr, uint64_t max_expand, dmr_seed_hit* out, uint64_t cap,
+                  uint64_t* n_out) {
+  if (!ctx || !n_out || (nq && !q) || (nr && !r)) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&]() -> int {
+    std::vector<uint64_t> qk(nq), rk(nr);
+    std::vector<uint32_t> qs(nq), qp(nq), rs(nr), rp(nr);
+    for (uint64_t i = 0; i < nq; ++i) { qk[i] = q[i].key; qs[i] = q[i].seq; qp[i] = q[i].pos; }
+    for (uint64_t i = 0; i < nr; ++i) { rk[i] = r[i].key; rs[i] = r[i].seq; rp[i] = r[i].pos; }
+    std::vector<uint32_t> h;
+    ctx->eng->seed_join(qk.data(), qs.data(), qp.data(), nq, rk.data(), rs.data(), rp.data(), nr,
+                        max_expand, &h);
+    uint64_t n = h.size() / 4;
+    *n_out = n;
+    if (n > cap) return fail(DMR_E_CAPACITY, "output capacity too small");
+    for (uint64_t i = 0; i < n; ++i) out[i] = dmr_seed_hit{h[4 * i], h[4 * i + 1], h[4 * i + 2], h[4 * i + 3]};
+    return DMR_OK;
+  });
+}
+
+int dmr_xdrop_batch(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+                    uint64_t slen, const uint32_t* qa, const uint32_t* sa, uint64_t n, int32_t xdrop,
+                    dmr_segment* out) {
+  if (!ctx || (n && (!qa || !sa || !out))) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&]() -> int {
+    std::vector<dmr::SegRec> s(n);
+    ctx->eng->xdrop_batch(query, qlen, subject, slen, qa, sa, n, xdrop, s.data());
+    for (uint64_t i = 0; i < n; ++i) out[i] = dmr_segment{s[i].q_start, s[i].s_start, s[i].len, s[i].score, 0};
+    return DMR_OK;
+  });
+}
+
+int dmr_xdrop(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+              uint64_t slen, uint64_t qa, uint64_t sa, int32_t xdrop, dmr_segment* out) {
+  uint32_t a = (uint32_t)qa, b = (uint32_t)sa;
+  return dmr_xdrop_batch(ctx, query, qlen, subject, slen, &a, &b, 1, xdrop, out);
+}
+
+int dmr_extend_span_batch(dmr_ctx* ctx, const uint8_t* qbuf, uint64_t qbuf_len, const uint8_t* sbuf,
+                          uint64_t sbuf_len, const dmr_span_job* jobs, uint64_t njobs, int32_t go,
+                          int32_t ge, int traceback, dmr_aligned* out, uint32_t* ops_pool,
+                          uint64_t ops_cap, uint64_t* ops_offsets, uint64_t* ops_needed) {
+  if (!ctx || (njobs && (!jobs || !out))) return fail(DMR_E_ARG, "NULL argument");
+  dmr::GapRow gr;
+  if (!dmr::blosum62_gap_row(go, ge, &gr)) return fail(DMR_E_ARG, "unsupported gap costs");
+  return guard([&]() -> int {
+    std::vector<dmr::Engine::SpanJob> sj(njobs);
+    for (uint64_t i = 0; i < njobs; ++i)
+      sj[i] = {jobs[i].q_off, jobs[i].q_len, jobs[i].s_off, jobs[i].s_len, jobs[i].d0, jobs[i].d1};
+    std::vector<dmr::SwOut> o;
+    std::vector<std::vector<uint32_t>> ops;
+    ctx->eng->extend_span_batch(qbuf, qbuf_len, sbuf, sbuf_len, sj.data(), njobs, go, ge,
+                                traceback != 0, &o, &ops);
+    uint64_t need = 0;
+    for (uint64_t i = 0; i < njobs; ++i) {
+      set_aligned(o[i], &out[i], 0);
+      if (!traceback) {
+        std::memset(&out[i].q_begin, 0, 4 * sizeof(uint64_t) + sizeof(dmr_hsp_stats) + 8);
+      }
+      if (ops_offsets) ops_offsets[i] = need;
+      need += traceback ? ops[i].size() : 0;
+    }
+    if (ops_offsets) ops_offsets[njobs] = need;
+    if (ops_needed) *ops_needed = need;
+    if (traceback) {
+      if (need > ops_cap || (need && !ops_pool)) return fail(DMR_E_CAPACITY, "ops pool too small");
+      uint64_t k = 0;
+      for (auto& v : ops)
+        for (uint32_t x : v) ops_pool[k++] = x;
+    }
+    return DMR_OK;
+  });
+}
+
+int dmr_extend_span(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+                    uint64_t slen, int64_t d0, int64_t d1, int32_t go, int32_t ge, dmr_aligned* out,
+                    uint32_t* ops, uint32_t ops_cap) {
+  if (!ctx || !out) return fail(DMR_E_ARG, "NULL argument");
+  dmr_span_job j{0, qlen, 0, slen, d0, d1};
+  uint64_t need = 0, offs[2];
+  std::vector<uint32_t> tmp(qlen + slen + 1);
+  int rc = dmr_extend_span_batch(ctx, query, qlen, subject, slen, &j, 1, go, ge, 1, out, tmp.data(),
+                                 tmp.size(), offs, &need);
+  if (rc != DMR_OK) return rc;
+  if (need > ops_cap) return fail(DMR_E_CAPACITY, "ops capacity too small");
+  if (need) std::memcpy(ops, tmp.data(), need * 4);
+  return DMR_OK;
+}
+
+int dmr_extend(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+               uint64_t slen, uint64_t aq, uint64_t as, uint64_t band, int32_t go, int32_t ge,
+               dmr_aligned* out, uint32_t* ops, uint32_t ops_cap) {
+  int64_t diag = (int64_t)aq - (int64_t)as;
+  int64_t b = (int64_t)std::max<uint64_t>(band, 1);
+  return dmr_extend_span(ctx, query, qlen, subject, slen, diag - b, diag + b, go, ge, out, ops, ops_cap);
+}
+
+int dmr_db_load(dmr_ctx* ctx, const uint8_t* letters, const uint64_t* offsets, uint32_t nseq) {
+  if (!ctx || !offsets || (offsets[nseq] && !letters)) return fail(DMR_E_ARG, "NULL argument");
+  for (uint32_t i = 0; i < nseq; ++i)
+    if (offsets[i + 1] < offsets[i]) return fail(DMR_E_ARG, "offsets are not monotone");
+  return guard([&]() -> int {
+    ctx->eng->load_db(letters, offsets, nseq);
+    return DMR_OK;
+  });
+}
+
+int dmr_db_build_index(dmr_ctx* ctx, const char* sens, int32_t index_chunks, double seed_cut,
+                       double* build_ms) {
+  if (!ctx) return fail(DMR_E_ARG, "NULL context");
+  return guard([&]() -> int {
+    ctx->eng->build_index(sens ? sens : "default", index_chunks, seed_cut);
+    if (build_ms) *build_ms = ctx->eng->last_index_ms();
+    return DMR_OK;
+  });
+}
+
+int dmr_blastp(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq,
+               const dmr_search_config* cfg, dmr_result** out) {
+  if (!ctx || !q_offsets || !out) return fail(DMR_E_ARG, "NULL argument");
+  dmr::SearchOptions o = options(cfg);
+  dmr::GapRow gr;
+  if (!dmr::blosum62_gap_row(o.gap_open, o.gap_extend, &gr))
+    return fail(DMR_E_ARG, "Gap penalty settings are outside the supported range for BLOSUM62");
+  return guard([&]() -> int {
+    auto r = std::make_unique<dmr_result>();
+    r->targets.offs.push_back(0);
+    for (uint32_t i = 0; i < nq; ++i) {
+      uint64_t a = q_offsets[i], b = q_offsets[i + 1];
+      if (b < a) return fail(DMR_E_ARG, "query offsets are not monotone");
+      if (b == a) continue;  // empty queries have no target (src/workflow.rs:129-132)
+      r->targets.letters.insert(r->targets.letters.end(), queries + a, queries + b);
+      r->targets.offs.push_back((uint32_t)r->targets.letters.size());
+      r->targets.query_of.push_back(i);
+      r->targets.frame_of.push_back(0);
+    }
+    r->dna_len.resize(nq, 0);
+    finish(ctx, r.get(), o, nq);
+    *out = r.release();
+    return DMR_OK;
+  });
+}
+
+int dmr_blastx(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint32_t nreads,
+               const dmr_search_config* cfg, dmr_result** out) {
+  if (!ctx || !r_offsets || !out) return fail(DMR_E_ARG, "NULL argument");
+  dmr::SearchOptions o = options(cfg);
+  dmr::GapRow gr;
+  if (!dmr::blosum62_gap_row(o.gap_open, o.gap_extend, &gr))
+    return fail(DMR_E_ARG, "Gap penalty settings are outside the supported range for BLOSUM62");
+  return guard([&]() -> int {
+    auto r = std::make_unique<dmr_result>();
+    r->translated = true;
+    r->targets.offs.push_back(0);
+    r->dna_len.resize(nreads);
+    for (uint32_t i = 0; i < nreads; ++i) {
+      std::string nt(reads + r_offsets[i], reads + r_offsets[i + 1]);
+      r->dna_len[i] = nt.size();
+      size_t mo = (cfg && cfg->min_orf >= 0) ? (size_t)cfg->min_orf : default_min_orf(nt.size());
+      std::string rc(nt.rbegin(), nt.rend());
+      for (auto& ch : rc) ch = nt_comp(ch);
+      for (int f = 0; f < 6; ++f) {
+        auto fr = translate(f < 3 ? nt : rc, (size_t)(f % 3));
+        mask_orfs(fr, mo);
+        bool any = false;
+        for (auto c : fr) any |= c < 23;
+        if (!any) continue;
+        r->targets.letters.insert(r->targets.letters.end(), fr.begin(), fr.end());
+        r->targets.offs.push_back((uint32_t)r->targets.letters.size());
+        r->targets.query_of.push_back(i);
+        r->targets.frame_of.push_back((uint8_t)f);
+      }
+    }
+    finish(ctx, r.get(), o, nreads);
+    *out = r.release();
+    return DMR_OK;
+  });
+}
+
+int64_t dmr_result_count(const dmr_result* r) { return r ? (int64_t)r->hsps.size() : -1; }
+
+int dmr_result_hsps(const dmr_result* r, int64_t* o, double* d) {
+  if (!r) return fail(DMR_E_ARG, "NULL result");
+  for (size_t i = 0; i < r->hsps.size(); ++i) {
+    const HspRec& h = r->hsps[i];
+    int64_t* x = o + 16 * i;
+    x[0] = h.query; x[1] = h.subject; x[2] = h.frame; x[3] = h.q_begin; x[4] = h.q_end;
+    x[5] = h.s_begin; x[6] = h.s_end; x[7] = h.score; x[8] = h.identities; x[9] = h.positives;
+    x[10] = h.mismatches; x[11] = h.gap_openings; x[12] = h.gaps; x[13] = h.length;
+    x[14] = (int64_t)h.ops.size(); x[15] = 0;
+    d[2 * i] = h.bits;
+    d[2 * i + 1] = h.evalue;
+  }
+  return DMR_OK;
+}
+
+int dmr_result_ops(const dmr_result* r, int64_t i, uint32_t* ops, uint32_t cap, uint32_t* n) {
+  if (!r || i < 0 || (size_t)i >= r->hsps.size()) return fail(DMR_E_ARG, "bad HSP index");
+  const auto& v = r->hsps[i].ops;
+  if (n) *n = (uint32_t)v.size();
+  if (v.size() > cap) return fail(DMR_E_CAPACITY, "ops capacity too small");
+  std::copy(v.begin(), v.end(), ops);
+  return DMR_OK;
+}
+
+int dmr_result_stats(const dmr_result* r, double* o) {
+  if (!r) return fail(DMR_E_ARG, "NULL result");
+  const dmr::StageStats& s = r->stats;
+  double v[20] = {(double)s.query_seeds, (double)s.runs, (double)s.hits, (double)s.hit_runs,
+                  (double)s.segments, (double)s.groups, (double)s.jobs, (double)s.maybe,
+                  (double)s.selected, (double)s.traceback_cells, s.ms_seed, s.ms_join,
+                  s.ms_sort_hits, s.ms_ungapped, s.ms_bands, s.ms_sw, s.ms_select, s.ms_traceback,
+                  s.ms_total, s.ms_host};
+  std::memcpy(o, v, sizeof v);
+  return DMR_OK;
+}
+
+int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells) {
+  if (!ctx || !cells) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&]() -> int {
+    *cells = ctx->eng->count_last_dp_cells();
+    return DMR_OK;
+  });
+}
+
+// Tabular::write (src/output.rs:150-223).
+int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, const char* fields,
+                      char* buf, uint64_t cap, uint64_t* len) {
+  if (!r || !len) return fail(DMR_E_ARG, "NULL argument");
+  auto qid = split(qids, '\n'), sid = split(sids, '\n');
+  auto f = split(fields, ' ');
+  if (f.empty())
+    f = {"qseqid", "sseqid", "pident", "length", "mismatch", "gapopen",
+         "qstart", "qend", "sstart", "send", "evalue", "bitscore"};
+  static const char* known[] = {"qseqid", "qlen", "sseqid", "slen", "qstart", "qend", "sstart",
+                                "send", "qseq", "sseq", "evalue", "bitscore", "score", "length",
+                                "pident", "nident", "mismatch", "positive", "gapopen", "gaps",
+                                "ppos", "qframe", "qcovhsp", "scovhsp", "cigar", "btop", "qtitle",
+                                "stitle", "full_sseq", "qseq_translated", "qstrand"};
+  for (auto& n : f) {
+    bool ok = false;
+    for (auto* k : known) ok |= n == k;
+    if (!ok) return fail(DMR_E_ARG, "Invalid output field: " + n);
+  }
+  std::string out;
+  for (size_t i = 0; i < r->hsps.size(); ++i) {
+    const HspRec& h = r->hsps[i];
+    int ti = r->target_of[i];
+    if (ti < 0) continue;
+    if (h.query >= qid.size() || h.subject >= sid.size())
+      return fail(DMR_E_ARG, "id list shorter than the query or database");
+    const uint8_t* letters = r->targets.letters.data() + r->targets.offs[ti];
+    size_t frame_len = r->targets.offs[ti + 1] - r->targets.offs[ti];
+    const uint8_t* subj = r->db_res->data() + r->db_offs[h.subject];
+    size_t slen = r->db_offs[h.subject + 1] - r->db_offs[h.subject];
+    size_t dna = r->translated ? r->dna_len[h.query] : frame_len;
+    size_t qb = h.q_begin, qe = h.q_end;
+    if (r->translated) {
+      size_t fr = h.frame;
+      if (fr < 3) {
+        qb = fr + 3 * h.q_begin;
+        qe = fr + 3 * h.q_end;
+      } else {
+        size_t o = fr - 3;
+        qb = dna - (o + 3 * h.q_end);
+        qe = dna - (o + 3 * h.q_begin);
+      }
+    }
+    size_t qlen = r->translated ? dna : frame_len;
+    auto aligned = [&](bool query_side) {
+      std::string s;
+      size_t qi = h.q_begin, si = h.s_begin;
+      for (uint32_t op : h.ops)
+        for (uint32_t k = 0; k < (op >> 2); ++k) {
+          uint32_t c = op & 3;
+          if (c == 0) {
+            s += dmr::decode_letter(query_side ? letters[qi] : subj[si]);
+            ++qi;
+            ++si;
+          } else if (c == 1) {
+            s += query_side ? dmr::decode_letter(letters[qi]) : '-';
+            ++qi;
+          } else {
+            s += query_side ? '-' : dmr::decode_letter(subj[si]);
+            ++si;
+          }
+        }
+      return s;
+    };
+    for (size_t k = 0; k < f.size(); ++k) {
+      const std::string& n = f[k];
+      std::string v;
+      if (n == "qseqid" || n == "qtitle") v = qid[h.query];
+      else if (n == "qlen") v = std::to_string(qlen);
+      else if (n == "sseqid" || n == "stitle") v = sid[h.subject];
+      else if (n == "slen") v = std::to_string(slen);
+      else if (n == "qstart") v = std::to_string(qb + 1);
+      else if (n == "qend") v = std::to_string(qe);
+      else if (n == "sstart") v = std::to_string(h.s_begin + 1);
+      else if (n == "send") v = std::to_string(h.s_end);
+      else if (n == "evalue") v = fmt_evalue(h.evalue);
+      else if (n == "bitscore") v = fmt_double(h.bits);
+      else if (n == "score") v = std::to_string(h.score);
+      else if (n == "length") v = std::to_string(h.length);
+      else if (n == "pident") v = fmt_double(h.length ? h.identities * 100.0 / h.length : 0.0);
+      else if (n == "nident") v = std::to_string(h.identities);
+      else if (n == "mismatch") v = std::to_string(h.mismatches);
+      else if (n == "positive") v = std::to_string(h.positives);
+      else if (n == "gapopen") v = std::to_string(h.gap_openings);
+      else if (n == "gaps") v = std::to_string(h.gaps);
+      else if (n == "ppos") v = fmt_double(h.length ? h.positives * 100.0 / h.length : 0.0);
+      else if (n == "qframe") v = std::to_string(h.frame < 3 ? (int)h.frame + 1 : 2 - (int)h.frame);
+      else if (n == "qcovhsp") v = qlen == 0 ? "0.0" : fmt_double((double)(qe - qb) * 100.0 / (double)qlen);
+      else if (n == "scovhsp") v = slen == 0 ? "0.0" : fmt_double((double)(h.s_end - h.s_begin) * 100.0 / (double)slen);
+      else if (n == "cigar") {
+        for (uint32_t op : h.ops) v += std::to_string(op >> 2) + "MID"[op & 3];
+      } else if (n == "btop") {
+        uint32_t run = 0;
+        size_t qi = h.q_begin, si = h.s_begin;
+        auto flush = [&]() {
+          if (run) v += std::to_string(run);
+          run = 0;
+        };
+        for (uint32_t op : h.ops)
+          for (uint32_t t = 0; t < (op >> 2); ++t) {
+            uint32_t c = op & 3;
+            if (c == 0) {
+              char a = dmr::decode_letter(letters[qi++]), b = dmr::decode_letter(subj[si++]);
+              if (a == b) ++run;
+              else { flush(); v += a; v += b; }
+            } else if (c == 1) {
+              flush(); v += dmr::decode_letter(letters[qi++]); v += '-';
+            } else {
+              flush(); v += '-'; v += dmr::decode_letter(subj[si++]);
+            }
+          }
+        flush();
+      } else if (n == "qseq") v = aligned(true);
+      else if (n == "sseq") v = aligned(false);
+      else if (n == "full_sseq") {
+        for (size_t t = 0; t < slen; ++t) v += dmr::decode_letter(subj[t]);
+      } else if (n == "qseq_translated") {
+        size_t e = std::min<size_t>(h.q_end, frame_len), b = std::min<size_t>(h.q_begin, e);
+        for (size_t t = b; t < e; ++t) v += dmr::decode_letter(letters[t]);
+      } else if (n == "qstrand") v = h.frame >= 3 ? "-" : "+";
+      if (k) out += '\t';
+      out += v;
+    }
+    out += '\n';
+  }
+  *len = out.size();
+  if (buf && cap >= out.size()) std::memcpy(buf, out.data(), out.size());
+  else if (buf && cap < out.size()) return fail(DMR_E_CAPACITY, "buffer too small");
+  return DMR_OK;
+}
+
+void dmr_result_destroy(dmr_result* r) { delete r; }
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Kernel launchers of the seed / extension / gapped stages.
+#pragma once
+#include "dmr_alp.hpp"
+#include "dmr_device.hpp"
+
+namespace dmr {
+
+// One DB chunk of the resident seed index (all shapes of the preset).
+struct RefChunkDev {
+  const uint64_t* keys;   // tagged keys, sorted
+  const uint32_t* locs;   // global residue index of each seed
+  const uint32_t* table;  // bucket lower bounds, nbuckets + 1 entries
+  uint64_t n;
+  uint64_t nbuckets;
+  int shift;              // bucket = key >> shift
+  int pad;
+};
+
+// ---- seeds ---------------------------------------------------------------
+void launch_seed_enumerate(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs,
+                           uint64_t r0, uint64_t r1, const SeedParams* P, uint32_t nshapes,
+                           int key_bits, uint32_t* block_counts, uint64_t* block_off,
+                           uint64_t* scan_tmp, uint64_t* keys, uint32_t* locs, hipStream_t st);
+uint32_t seed_enum_blocks(uint64_t r0, uint64_t r1);
+void launch_bucket_table(const uint64_t* keys, uint64_t n, int shift, uint64_t nbuckets,
+                         uint32_t* table, hipStream_t st);
+void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st);
+void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n, uint32_t* out,
+                          hipStream_t st);
+void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_t nruns,
+                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
+                       uint64_t max_expand, uint32_t* counts, uint32_t* rlo, hipStream_t st);
+void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
+                      uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
+                      const uint32_t* counts, const uint32_t* rlo, const uint64_t* hit_off,
+                      const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
+                      const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st);
+
+// ---- ungapped + bands ----------------------------------------------------
+struct SeqSet {
+  const uint8_t* res;
+  const uint32_t* offs;  // n+1
+};
+
+// Key flags over packed hits: 1 where (hit >> shift) changes.
+void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st);
+void launch_xdrop_runs(const uint64_t* hits, const uint32_t* run_start, uint64_t nruns,
+                       uint64_t nhits, HitGeom g, SeqSet q, SeqSet db, const int8_t* matrix32,
+                       int xdrop, int cutoff, SegRec* segs, uint32_t* keep, hipStream_t st);
+void launch_compact_segments(const SegRec* segs, const uint64_t* hits, const uint32_t* keep,
+                             const uint64_t* pos, uint64_t nhits, int group_shift, SegRec* out,
+                             uint64_t* out_gkey, hipStream_t st);
+void launch_bands(const SegRec* segs, const uint64_t* gkey, const uint32_t* group_start,
+                  uint64_t ngroups, uint64_t nsegs, uint32_t bs, SeqSet q, SeqSet db,
+                  int32_t* tmp_d0, int32_t* tmp_d1, uint32_t* tmp_best, SwJob* tmp_jobs,
+                  uint32_t* nbands, hipStream_t st);
+void launch_jobs_compact(const SwJob* tmp_jobs, const uint32_t* group_start, const uint32_t* nbands,
+                         const uint64_t* job_off, uint64_t ngroups, SwJob* jobs, hipStream_t st);
+
+// ---- gapped --------------------------------------------------------------
+// Per job: DP cells (sum of row widths, rows 1..n) and the int32 row stride.
+void launch_job_geometry(const SwJob* jobs, const uint32_t* job_idx, uint64_t njobs, SeqSet db,
+                         uint64_t* cells, uint64_t* width0, uint32_t* stride, hipStream_t st);
+void launch_sw_generic(const SwJob* jobs, const uint32_t* job_idx, uint64_t njobs, SeqSet q,
+                       SeqSet db, const int8_t* matrix32, int gap_open, int gap_extend,
+                       int32_t* scratch, const uint64_t* scratch_off, uint8_t* trace,
+                       const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off,
+                       SwOut* out, bool traceback, hipStream_t st);
+void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
+                          SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
+                          uint32_t* flags, hipStream_t st);
+void launch_u32_times(const uint32_t* in, uint64_t n, uint32_t mul, uint64_t* out, hipStream_t st);
+
+}  // namespace dmr

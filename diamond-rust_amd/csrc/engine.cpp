@@ -1,0 +1,797 @@
+// Host orchestration of the search core (see engine.hpp).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace dmr {
+
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+void launch_gather(const SwJob* jobs, const SwOut* outs, const uint32_t* idx, uint64_t n,
+                   SwJob* jout, SwOut* oout, hipStream_t st);
+void launch_xdrop_pairs(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint64_t slen,
+                        const uint32_t* qa, const uint32_t* sa, uint64_t n, const int8_t* m32,
+                        int xdrop, SegRec* out, hipStream_t st);
+void launch_join_emit_entries(const uint32_t* qs, const uint32_t* qp, const uint32_t* run_start,
+                              uint64_t nruns, uint64_t nq_total, const uint32_t* rs,
+                              const uint32_t* rp, const uint32_t* counts, const uint32_t* rlo,
+                              const uint64_t* hit_off, HitGeom g, uint64_t* hits, hipStream_t st);
+
+namespace {
+
+int bits_for(uint64_t x) { return x == 0 ? 1 : 64 - __builtin_clzll(x); }
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// src/search.rs:273-282 — score desc, E-value asc, subject asc.
+bool hsp_less(const HspRec& a, const HspRec& b) {
+  if (a.score != b.score) return a.score > b.score;
+  if (a.evalue < b.evalue) return true;
+  if (b.evalue < a.evalue) return false;
+  return a.subject < b.subject;
+}
+
+}  // namespace
+
+Engine::Engine(int device) : device_(device) {
+  int n = 0;
+  DMR_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) throw Error(-1, "no such HIP device");
+  DMR_HIP(hipSetDevice(device));
+  DMR_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  for (auto& e : ev_) DMR_HIP(hipEventCreate(&e));
+  // 32x32 score table; letters >= 25 (padding) score -64 against everything.
+  int8_t m[kScoreStride * kScoreStride];
+  for (int a = 0; a < kScoreStride; ++a)
+    for (int b = 0; b < kScoreStride; ++b)
+      m[a * kScoreStride + b] = (a < 25 && b < 25) ? blosum62()[a][b] : (int8_t)-64;
+  DMR_HIP(hipMemcpy(matrix32_.ensure<int8_t>(sizeof m), m, sizeof m, hipMemcpyHostToDevice));
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  if (st_) (void)hipStreamSynchronize(st_);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+template <class T>
+T Engine::read1(const T* dptr) {
+  T v;
+  DMR_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  return v;
+}
+
+void Engine::stage_event(int k) { DMR_HIP(hipEventRecord(ev_[k], st_)); }
+float Engine::stage_ms(int a, int b) {
+  float ms = 0;
+  DMR_HIP(hipEventElapsedTime(&ms, ev_[a], ev_[b]));
+  return ms;
+}
+
+AlpParams Engine::alp(int go, int ge) const {
+  GapRow r;
+  if (!blosum62_gap_row(go, ge, &r)) throw Error(-1, "unsupported gap costs for BLOSUM62");
+  return alp_make(r.lambda, r.k, r.alpha, r.alpha_v, r.sigma, go, ge, kB62UngappedAlpha,
+                  kB62UngappedAlphaV);
+}
+
+void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n) {
+  DMR_HIP(hipSetDevice(device_));
+  uint64_t total = offsets[n];
+  if (total >= (1ull << 32)) throw Error(-5, "database larger than 4G letters");
+  h_db_offs_.resize(n + 1);
+  uint32_t maxl = 0;
+  std::vector<uint32_t> seq_of(total);
+  for (uint32_t i = 0; i < n; ++i) {
+    h_db_offs_[i] = (uint32_t)offsets[i];
+    uint32_t l = (uint32_t)(offsets[i + 1] - offsets[i]);
+    maxl = std::max(maxl, l);
+    for (uint64_t k = offsets[i]; k < offsets[i + 1]; ++k) seq_of[k] = i;
+  }
+  h_db_offs_[n] = (uint32_t)total;
+  h_db_res_.assign(letters, letters + total);
+  DMR_HIP(hipMemcpy(db_res_.ensure<uint8_t>(total + 64), letters, total, hipMemcpyHostToDevice));
+  DMR_HIP(hipMemcpy(db_offs_.ensure<uint32_t>(n + 1), h_db_offs_.data(), (n + 1) * 4,
+                    hipMemcpyHostToDevice));
+  DMR_HIP(hipMemcpy(db_seq_of_.ensure<uint32_t>(total + 1), seq_of.data(), total * 4,
+                    hipMemcpyHostToDevice));
+  db_n_ = n;
+  db_letters_ = total;
+  db_max_len_ = maxl;
+  chunks_.clear();
+  idx_sens_.clear();
+}
+
+void Engine::upload_seed_params(const std::string& sens, double cut, uint32_t* nshapes,
+                                int* key_bits, int* shape_bits) {
+  const auto& codes = shape_codes(sens);
+  ReductionDef red = parse_reduction(reduction_def(sens));
+  SeedParams P;
+  std::memset(&P, 0, sizeof P);
+  P.nshapes = (uint32_t)codes.size();
+  P.red_size = red.size;
+  std::memcpy(P.red_map, red.map, 25);
+  P.cut = cut;
+  P.lnfact[0] = P.lnfact[1] = 0.0;
+  for (int k = 2; k < 33; ++k) P.lnfact[k] = P.lnfact[k - 1] + std::log((double)k);
+  int kb = 1;
+  for (size_t s = 0; s < codes.size(); ++s) {
+    ShapeDef sd;
+    if (!parse_shape(codes[s], &sd)) throw Error(-1, "invalid shape " + codes[s]);
+    P.shapes[s].weight = sd.weight;
+    P.shapes[s].span = sd.span;
+    std::memcpy(P.shapes[s].pos, sd.pos, sizeof sd.pos);
+    // max key = size^weight - 1
+    long double mx = std::pow((long double)red.size, (long double)sd.weight) - 1;
+    if (mx >= 18446744073709551615.0L) throw Error(-5, "seed key does not fit 64 bits");
+    kb = std::max(kb, bits_for((uint64_t)mx));
+  }
+  *nshapes = P.nshapes;
+  *key_bits = kb;
+  *shape_bits = P.nshapes > 1 ? bits_for(P.nshapes - 1) : 0;
+  if (kb + *shape_bits > 64) throw Error(-5, "tagged seed key does not fit 64 bits");
+  DMR_HIP(hipMemcpyAsync(seed_params_.ensure<SeedParams>(1), &P, sizeof P, hipMemcpyHostToDevice, st_));
+}
+
+uint64_t Engine::index_entries() const {
+  uint64_t n = 0;
+  for (auto& c : chunks_) n += c.n;
+  return n;
+}
+
+// Resident index: per DB chunk (src/seed.rs:146-164), all shapes enumerated
+// and sorted by tagged key, plus a bucket table over the top key bits.
+void Engine::build_index(const std::string& sens, int index_chunks, double seed_cut) {
+  DMR_HIP(hipSetDevice(device_));
+  double cutv = (seed_cut >= 0 ? seed_cut : preset_seed_cut(sens));
+  int nch = index_chunks >= 0 ? std::max(index_chunks, 1) : preset_index_chunks(sens);
+  const auto& codes = shape_codes(sens);
+  ShapeDef s0;
+  parse_shape(codes[0], &s0);
+  // cfg.seed_complexity_cut = seed_cut * ln 2 * shapes[0].weight (src/search.rs:102-105)
+  double cut = cutv * 0.6931471805599453 * (double)s0.weight;
+  if (!chunks_.empty() && idx_sens_ == sens && idx_chunks_ == nch && idx_cut_ == cut) return;
+  double t0 = now_ms();
+  upload_seed_params(sens, cut, &nshapes_, &key_bits_, &shape_bits_);
+  const int total_bits = key_bits_ + shape_bits_;
+  // chunks(n, parts)
+  std::vector<std::pair<uint64_t, uint64_t>> ranges;
+  {
+    uint64_t n = db_n_, parts = std::min<uint64_t>(std::max(nch, 1), std::max<uint64_t>(n, 1));
+    uint64_t start = 0;
+    for (uint64_t p = 0; p < parts && n > 0; ++p) {
+      uint64_t end = (p + 1 == parts) ? n : start + n / parts + (p < n % parts ? 1 : 0);
+      ranges.push_back({start, end});
+      start = end;
+    }
+  }
+  chunks_.clear();
+  chunks_.resize(ranges.size());
+  std::vector<RefChunkDev> desc(ranges.size());
+  for (size_t c = 0; c < ranges.size(); ++c) {
+    uint64_t r0 = h_db_offs_[ranges[c].first], r1 = h_db_offs_[ranges[c].second];
+    uint32_t nb = seed_enum_blocks(r0, r1);
+    uint64_t nbk = (uint64_t)nb * nshapes_ + 1;
+    uint32_t* bc = b_counts_.ensure<uint32_t>(nbk);
+    launch_fill_u32(bc + nbk - 1, 1, 0, st_);
+    uint64_t* bo = b_off_.ensure<uint64_t>(nbk);
+    uint64_t* tmp = b_scan_.ensure<uint64_t>(scan_tmp_elems(nbk));
+    // Upper bound on entries: one per residue and shape.
+    uint64_t cap = (r1 - r0) * nshapes_ + 1;
+    IndexChunk& ch = chunks_[c];
+    uint64_t* keys = ch.keys.ensure<uint64_t>(cap);
+    uint32_t* locs = ch.locs.ensure<uint32_t>(cap);
+    launch_seed_enumerate(db_res_.as<uint8_t>(), db_seq_of_.as<uint32_t>(), db_offs_.as<uint32_t>(),
+                          r0, r1, seed_params_.as<SeedParams>(), nshapes_, key_bits_, bc, bo, tmp,
+                          keys, locs, st_);
+    uint64_t n = read1(bo + nbk - 1);
+    ch.n = n;
+    uint64_t* k2 = qkeys2_.ensure<uint64_t>(n + 1);
+    uint32_t* l2 = qlocs2_.ensure<uint32_t>(n + 1);
+    uint32_t* hist = b_hist_.ensure<uint32_t>(radix_hist_elems(n));
+    uint64_t* stmp = b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(n));
+    radix_sort_pairs(keys, locs, k2, l2, n, total_bits, hist, stmp, st_);
+    int tb = std::min(total_bits, 22);
+    ch.shift = total_bits - tb;
+    uint64_t maxkey = (((uint64_t)(nshapes_ - 1)) << key_bits_) | ((key_bits_ >= 64) ? ~0ull : ((1ull << key_bits_) - 1));
+    ch.nbuckets = (maxkey >> ch.shift) + 1;
+    launch_bucket_table(keys, n, ch.shift, ch.nbuckets, ch.table.ensure<uint32_t>(ch.nbuckets + 1), st_);
+    desc[c] = RefChunkDev{keys, locs, ch.table.as<uint32_t>(), n, ch.nbuckets, ch.shift, 0};
+  }
+  DMR_HIP(hipMemcpyAsync(chunk_desc_.ensure<RefChunkDev>(desc.size() + 1), desc.data(),
+                         desc.size() * sizeof(RefChunkDev), hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  idx_sens_ = sens;
+  idx_chunks_ = nch;
+  idx_cut_ = cut;
+  index_ms_ = now_ms() - t0;
+}
+
+void Engine::load_targets(const TargetSet& t) {
+  DMR_HIP(hipSetDevice(device_));
+  tset_ = t;
+  q_n_ = (uint32_t)(t.offs.size() - 1);
+  q_total_ = t.offs.back();
+  q_max_len_ = 0;
+  std::vector<uint32_t> seq_of(q_total_ + 1);
+  for (uint32_t i = 0; i < q_n_; ++i) {
+    q_max_len_ = std::max(q_max_len_, t.offs[i + 1] - t.offs[i]);
+    for (uint32_t k = t.offs[i]; k < t.offs[i + 1]; ++k) seq_of[k] = i;
+  }
+  DMR_HIP(hipMemcpyAsync(q_res_.ensure<uint8_t>(q_total_ + 64), t.letters.data(), q_total_,
+                         hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(q_offs_.ensure<uint32_t>(q_n_ + 1), t.offs.data(), (q_n_ + 1) * 4,
+                         hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(q_seq_of_.ensure<uint32_t>(q_total_ + 1), seq_of.data(), q_total_ * 4,
+                         hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+}
+
+std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt,
+                                                        StageStats* stats) {
+  DMR_HIP(hipSetDevice(device_));
+  StageStats S;
+  double t_start = now_ms();
+  std::vector<std::vector<HspRec>> result(q_n_);
+  if (q_n_ == 0 || db_n_ == 0 || q_total_ == 0) {
+    if (stats) *stats = S;
+    return result;
+  }
+  build_index(opt.sensitivity, opt.index_chunks, opt.seed_cut);
+  const AlpParams P = alp(opt.gap_open, opt.gap_extend);
+  const int xdrop = raw_score(P, 12.3);  // --xdrop 12.3 bits (src/search.rs:51-53)
+  const int cutoff = 19;                  // src/search.rs:164
+  const int total_bits = key_bits_ + shape_bits_;
+  SeqSet qs{q_res_.as<uint8_t>(), q_offs_.as<uint32_t>()};
+  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
+  stage_event(0);
+
+  // ---- query seeds (all shapes, tagged, sorted) ---------------------------
+  uint32_t nb = seed_enum_blocks(0, q_total_);
+  uint64_t nbk = (uint64_t)nb * nshapes_ + 1;
+  uint32_t* bc = b_counts_.ensure<uint32_t>(nbk);
+  launch_fill_u32(bc + nbk - 1, 1, 0, st_);
+  uint64_t* bo = b_off_.ensure<uint64_t>(nbk);
+  uint64_t cap = q_total_ * nshapes_ + 1;
+  uint64_t* qk = qkeys_.ensure<uint64_t>(cap);
+  uint32_t* ql = qlocs_.ensure<uint32_t>(cap);
+  launch_seed_enumerate(q_res_.as<uint8_t>(), q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), 0,
+                        q_total_, seed_params_.as<SeedParams>(), nshapes_, key_bits_, bc, bo,
+                        b_scan_.ensure<uint64_t>(scan_tmp_elems(nbk)), qk, ql, st_);
+  const uint64_t nQ = read1(bo + nbk - 1);
+  S.query_seeds = nQ;
+  {
+    uint64_t* k2 = qkeys2_.ensure<uint64_t>(nQ + 1);
+    uint32_t* l2 = qlocs2_.ensure<uint32_t>(nQ + 1);
+    radix_sort_pairs(qk, ql, k2, l2, nQ, total_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(nQ)),
+                     b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nQ)), st_);
+  }
+  stage_event(1);
+
+  // ---- join: runs of equal query keys x DB chunks --------------------------
+  uint32_t* fl = flags_.ensure<uint32_t>(nQ + 1);
+  launch_run_flags(qk, nQ, fl, st_);
+  launch_fill_u32(fl + nQ, 1, 0, st_);
+  uint64_t* ps = pos_.ensure<uint64_t>(nQ + 1);
+  scan_exclusive_u32_to_u64(fl, ps, nQ + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nQ + 1)), st_);
+  const uint64_t nruns = read1(ps + nQ);
+  uint32_t* rs = runs_.ensure<uint32_t>(nruns + 1);
+  launch_compact_flags(fl, ps, nQ, rs, st_);
+  S.runs = nruns;
+  const uint32_t nch = (uint32_t)chunks_.size();
+  const uint64_t npairs = nruns * nch;
+  uint32_t* cnt = jc_counts_.ensure<uint32_t>(npairs + 1);
+  uint32_t* rlo = jc_rlo_.ensure<uint32_t>(npairs + 1);
+  launch_join_count(qk, rs, nruns, nQ, chunk_desc_.as<RefChunkDev>(), nch, opt.max_expand, cnt, rlo,
+                    st_);
+  launch_fill_u32(cnt + npairs, 1, 0, st_);
+  uint64_t* hoff = jc_off_.ensure<uint64_t>(npairs + 1);
+  scan_exclusive_u32_to_u64(cnt, hoff, npairs + 1,
+                            b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
+  const uint64_t nhits = read1(hoff + npairs);
+  S.hits = nhits;
+  HitGeom g;
+  g.bt = bits_for(q_n_ > 0 ? q_n_ - 1 : 0);
+  g.bs = bits_for(db_n_ > 0 ? db_n_ - 1 : 0);
+  g.bp = bits_for(db_max_len_);
+  g.bd = bits_for((uint64_t)q_max_len_ + db_max_len_);
+  g.diag_bias = db_max_len_;
+  const int hit_bits = (int)(g.bt + g.bs + g.bd + g.bp);
+  if (hit_bits > 64) throw Error(-5, "hit key does not fit 64 bits for this batch");
+  uint64_t* hits = hits_.ensure<uint64_t>(nhits + 1);
+  launch_join_emit(ql, rs, nruns, nQ, chunk_desc_.as<RefChunkDev>(), nch, cnt, rlo, hoff,
+                   q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), db_seq_of_.as<uint32_t>(),
+                   db_offs_.as<uint32_t>(), g, hits, st_);
+  stage_event(2);
+  radix_sort_keys(hits, hits2_.ensure<uint64_t>(nhits + 1), nhits, hit_bits,
+                  b_hist_.ensure<uint32_t>(radix_hist_elems(nhits)),
+                  b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nhits)), st_);
+  stage_event(3);
+
+  // ---- ungapped: runs of (target, subject, diagonal) -----------------------
+  fl = flags_.ensure<uint32_t>(nhits + 1);
+  launch_key_flags(hits, nhits, (int)g.bp, fl, st_);
+  launch_fill_u32(fl + nhits, 1, 0, st_);
+  ps = pos_.ensure<uint64_t>(nhits + 1);
+  scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+  const uint64_t nhr = read1(ps + nhits);
+  S.hit_runs = nhr;
+  uint32_t* hrs = runs_.ensure<uint32_t>(nhr + 1);
+  launch_compact_flags(fl, ps, nhits, hrs, st_);
+  SegRec* sraw = segs_raw_.ensure<SegRec>(nhits + 1);
+  uint32_t* keep = keep_.ensure<uint32_t>(nhits + 1);
+  launch_xdrop_runs(hits, hrs, nhr, nhits, g, qs, dbs, matrix32_.as<int8_t>(), xdrop, cutoff, sraw,
+                    keep, st_);
+  launch_fill_u32(keep + nhits, 1, 0, st_);
+  ps = pos_.ensure<uint64_t>(nhits + 1);
+  scan_exclusive_u32_to_u64(keep, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+  const uint64_t nsegs = read1(ps + nhits);
+  S.segments = nsegs;
+  SegRec* segs = segs_.ensure<SegRec>(nsegs + 1);
+  uint64_t* gk = gkeys_.ensure<uint64_t>(nsegs + 1);
+  launch_compact_segments(sraw, hits, keep, ps, nhits, (int)(g.bd + g.bp), segs, gk, st_);
+  stage_event(4);
+
+  // ---- bands per (target, subject) -----------------------------------------
+  fl = flags_.ensure<uint32_t>(nsegs + 1);
+  launch_key_flags(gk, nsegs, 0, fl, st_);
+  launch_fill_u32(fl + nsegs, 1, 0, st_);
+  ps = pos_.ensure<uint64_t>(nsegs + 1);
+  scan_exclusive_u32_to_u64(fl, ps, nsegs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nsegs + 1)), st_);
+  const uint64_t ngroups = read1(ps + nsegs);
+  S.groups = ngroups;
+  uint32_t* gs = gstart_.ensure<uint32_t>(ngroups + 1);
+  launch_compact_flags(fl, ps, nsegs, gs, st_);
+  uint32_t* nbd = nbands_.ensure<uint32_t>(ngroups + 1);
+  SwJob* tj = tmpjobs_.ensure<SwJob>(nsegs + 1);
+  launch_bands(segs, gk, gs, ngroups, nsegs, g.bs, qs, dbs, bd0_.ensure<int32_t>(nsegs + 1),
+               bd1_.ensure<int32_t>(nsegs + 1), bbest_.ensure<uint32_t>(nsegs + 1), tj, nbd, st_);
+  launch_fill_u32(nbd + ngroups, 1, 0, st_);
+  uint64_t* jo = joboff_.ensure<uint64_t>(ngroups + 1);
+  scan_exclusive_u32_to_u64(nbd, jo, ngroups + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(ngroups + 1)), st_);
+  const uint64_t njobs = read1(jo + ngroups);
+  S.jobs = njobs;
+  SwJob* jobs = jobs_.ensure<SwJob>(njobs + 1);
+  launch_jobs_compact(tj, gs, nbd, jo, ngroups, jobs, st_);
+  stage_event(5);
+
+  // ---- banded DP, score pass -----------------------------------------------
+  uint32_t* stride = stride_.ensure<uint32_t>(njobs + 1);
+  launch_job_geometry(jobs, nullptr, njobs, dbs, nullptr, nullptr, stride, st_);
+  uint64_t* soff = scr_off_.ensure<uint64_t>(njobs + 1);
+  launch_u32_times(stride, njobs, 4, soff, st_);
+  scan_exclusive_u64(soff, soff, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
+  const uint64_t scratch_total = read1(soff + njobs);
+  SwOut* outs = outs_.ensure<SwOut>(njobs + 1);
+  stage_event(6);
+  launch_sw_generic(jobs, nullptr, njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+                    opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
+                    nullptr, nullptr, nullptr, outs, false, st_);
+  stage_event(7);
+  last_njobs_ = njobs;
+
+  // ---- selection by E-value -------------------------------------------------
+  fl = flags_.ensure<uint32_t>(njobs + 1);
+  launch_evalue_filter(jobs, outs, njobs, qs, dbs, P, db_letters_, opt.evalue, fl, st_);
+  launch_fill_u32(fl + njobs, 1, 0, st_);
+  ps = pos_.ensure<uint64_t>(njobs + 1);
+  scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
+  const uint64_t nmaybe = read1(ps + njobs);
+  S.maybe = nmaybe;
+  uint32_t* midx = sel_idx_.ensure<uint32_t>(nmaybe + 1);
+  launch_compact_flags(fl, ps, njobs, midx, st_);
+  std::vector<uint32_t> h_midx(nmaybe);
+  std::vector<SwJob> h_mjobs(nmaybe);
+  std::vector<SwOut> h_mouts(nmaybe);
+  if (nmaybe) {
+    SwJob* gj = tmpjobs_.ensure<SwJob>(std::max<uint64_t>(nmaybe, nsegs) + 1);
+    SwOut* go = sel_outs_.ensure<SwOut>(nmaybe + 1);
+    launch_gather(jobs, outs, midx, nmaybe, gj, go, st_);
+    DMR_HIP(hipMemcpyAsync(h_midx.data(), midx, nmaybe * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_mjobs.data(), gj, nmaybe * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_mouts.data(), go, nmaybe * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+  }
+  stage_event(8);
+  DMR_HIP(hipStreamSynchronize(st_));
+  double th0 = now_ms();
+  // Per group: bands in rank order; the first max_hsps whose exact E-value
+  // passes are kept (src/search.rs:232-255).
+  struct Cand {
+    uint64_t job;  // index into the maybe list
+    HspRec h;
+  };
+  std::vector<std::vector<std::vector<Cand>>> per_target(q_n_);
+  {
+    uint64_t k = 0;
+    while (k < nmaybe) {
+      uint32_t grp = h_mjobs[k].group;
+      std::vector<Cand> hs;
+      for (; k < nmaybe && h_mjobs[k].group == grp; ++k) {
+        if (hs.size() >= opt.max_hsps) continue;
+        const SwJob& j = h_mjobs[k];
+        int32_t sc = h_mouts[k].score;
+        if (sc <= 0) continue;
+        uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
+        uint64_t slen = h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
+        double e = alp_evalue(P, sc, qlen, slen, db_letters_);
+        if (e <= opt.evalue) {
+          Cand c;
+          c.job = k;
+          c.h.query = j.target;
+          c.h.subject = j.subject;
+          c.h.frame = 0;
+          c.h.score = sc;
+          c.h.evalue = e;
+          c.h.bits = bit_score(P, sc);
+          hs.push_back(std::move(c));
+        }
+      }
+      if (!hs.empty()) {
+        std::stable_sort(hs.begin(), hs.end(),
+                         [](const Cand& a, const Cand& b) { return hsp_less(a.h, b.h); });
+        per_target[hs[0].h.query].push_back(std::move(hs));
+      }
+    }
+  }
+  std::vector<uint64_t> sel;  // indices into the maybe list
+  for (auto& found : per_target) {
+    std::stable_sort(found.begin(), found.end(),
+                     [](const std::vector<Cand>& a, const std::vector<Cand>& b) {
+                       return hsp_less(a[0].h, b[0].h);
+                     });
+    if (found.size() > opt.max_target_seqs) found.resize(opt.max_target_seqs);
+    for (auto& hs : found) {
+      if (hs.size() > opt.max_hsps) hs.resize(opt.max_hsps);
+      for (auto& c : hs) sel.push_back(c.job);
+    }
+  }
+  S.ms_host = now_ms() - th0;
+  S.selected = sel.size();
+
+  // ---- traceback of the reported DPs ----------------------------------------
+  std::vector<SwOut> h_souts(sel.size());
+  std::vector<uint32_t> h_ops;
+  std::vector<uint64_t> h_ops_off(sel.size() + 1, 0);
+  stage_event(9);
+  if (!sel.empty()) {
+    const uint64_t ns = sel.size();
+    std::vector<uint32_t> sidx(ns);
+    for (uint64_t i = 0; i < ns; ++i) {
+      sidx[i] = h_midx[sel[i]];
+      const SwJob& j = h_mjobs[sel[i]];
+      uint64_t cap_ops = (uint64_t)(j.q1 - j.q0) + (h_db_offs_[j.subject + 1] - h_db_offs_[j.subject]) + 1;
+      h_ops_off[i + 1] = h_ops_off[i] + cap_ops;
+    }
+    uint32_t* d_sidx = sel_idx_.ensure<uint32_t>(ns + 1);
+    DMR_HIP(hipMemcpyAsync(d_sidx, sidx.data(), ns * 4, hipMemcpyHostToDevice, st_));
+    uint64_t* cells = cells_.ensure<uint64_t>(ns + 1);
+    uint64_t* w0 = width0_.ensure<uint64_t>(ns + 1);
+    uint32_t* sstr = stride_.ensure<uint32_t>(ns + 1);
+    launch_job_geometry(jobs, d_sidx, ns, dbs, cells, w0, sstr, st_);
+    std::vector<uint64_t> hc(ns), hw(ns);
+    std::vector<uint32_t> hs(ns);
+    DMR_HIP(hipMemcpyAsync(hc.data(), cells, ns * 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hw.data(), w0, ns * 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hs.data(), sstr, ns * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    std::vector<uint64_t> toff(ns + 1, 0), so(ns + 1, 0);
+    for (uint64_t i = 0; i < ns; ++i) {
+      toff[i + 1] = toff[i] + hc[i] + hw[i];
+      so[i + 1] = so[i] + 4ull * hs[i];
+      S.traceback_cells += hc[i];
+    }
+    uint64_t* d_toff = trace_off_.ensure<uint64_t>(ns + 1);
+    uint64_t* d_so = scr_off_.ensure<uint64_t>(std::max<uint64_t>(ns, njobs) + 1);
+    uint64_t* d_oo = ops_off_.ensure<uint64_t>(ns + 1);
+    DMR_HIP(hipMemcpyAsync(d_toff, toff.data(), (ns + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(d_so, so.data(), (ns + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(d_oo, h_ops_off.data(), (ns + 1) * 8, hipMemcpyHostToDevice, st_));
+    uint32_t* d_ops = ops_.ensure<uint32_t>(h_ops_off[ns] + 1);
+    SwOut* d_souts = sel_outs_.ensure<SwOut>(ns + 1);
+    launch_sw_generic(jobs, d_sidx, ns, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+                      opt.gap_extend, scratch_.ensure<int32_t>(so[ns] + 1), d_so,
+                      trace_.ensure<uint8_t>(toff[ns] + 1), d_toff, d_ops, d_oo, d_souts, true, st_);
+    h_ops.resize(h_ops_off[ns]);
+    DMR_HIP(hipMemcpyAsync(h_souts.data(), d_souts, ns * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_ops.data(), d_ops, h_ops_off[ns] * 4, hipMemcpyDeviceToHost, st_));
+  }
+  stage_event(10);
+  DMR_HIP(hipStreamSynchronize(st_));
+
+  // ---- assemble HSPs in align_target order ----------------------------------
+  {
+    uint64_t k = 0;
+    for (auto& found : per_target) {
+      for (auto& hs : found) {
+        for (auto& c : hs) {
+          const SwJob& j = h_mjobs[c.job];
+          const SwOut& o = h_souts[k];
+          HspRec h = c.h;
+          h.q_begin = j.q0 + o.q_begin;
+          h.q_end = j.q0 + o.best_i;
+          h.s_begin = o.s_begin;
+          h.s_end = o.best_j;
+          h.identities = o.identities;
+          h.positives = o.positives;
+          h.mismatches = o.mismatches;
+          h.gap_openings = o.gap_openings;
+          h.gaps = o.gaps;
+          h.length = o.length;
+          h.ops.assign(h_ops.begin() + h_ops_off[k], h_ops.begin() + h_ops_off[k] + o.n_ops);
+          if (o.score != c.h.score)
+            throw Error(-2, "traceback score differs from the score pass");
+          result[h.query].push_back(std::move(h));
+          ++k;
+        }
+      }
+    }
+  }
+  S.ms_seed = stage_ms(0, 1);
+  S.ms_join = stage_ms(1, 2);
+  S.ms_sort_hits = stage_ms(2, 3);
+  S.ms_ungapped = stage_ms(3, 4);
+  S.ms_bands = stage_ms(4, 5);
+  S.ms_sw = stage_ms(6, 7);
+  S.ms_select = stage_ms(7, 8);
+  S.ms_traceback = stage_ms(9, 10);
+  S.ms_total = now_ms() - t_start;
+  if (stats) *stats = S;
+  return result;
+}
+
+uint64_t Engine::count_last_dp_cells() {
+  uint64_t n = last_njobs_;
+  if (!n) return 0;
+  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
+  uint64_t* cells = cells_.ensure<uint64_t>(n + 1);
+  launch_job_geometry(jobs_.as<SwJob>(), nullptr, n, dbs, cells, nullptr, nullptr, st_);
+  std::vector<uint64_t> h(n);
+  DMR_HIP(hipMemcpyAsync(h.data(), cells, n * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  return std::accumulate(h.begin(), h.end(), (uint64_t)0);
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in calls on host buffers.
+// ---------------------------------------------------------------------------
+
+// seed::build (src/seed.rs:76-104): entries sorted by (key, seq, pos).
+void Engine::seed_build(const uint8_t* seqs, const uint64_t* offs, uint32_t nseq,
+                        const ShapeDef& shape, const ReductionDef& red, uint32_t base, double cut,
+                        std::vector<uint64_t>* keys, std::vector<uint32_t>* seq,
+                        std::vector<uint32_t>* pos) {
+  DMR_HIP(hipSetDevice(device_));
+  keys->clear();
+  seq->clear();
+  pos->clear();
+  uint64_t total = offs[nseq];
+  if (nseq == 0 || total == 0) return;
+  if (total >= (1ull << 32)) throw Error(-5, "input larger than 4G letters");
+  std::vector<uint32_t> o32(nseq + 1), seq_of(total);
+  for (uint32_t i = 0; i <= nseq; ++i) o32[i] = (uint32_t)offs[i];
+  for (uint32_t i = 0; i < nseq; ++i)
+    for (uint64_t k = offs[i]; k < offs[i + 1]; ++k) seq_of[k] = i;
+  DevBuf res, dof, dso, prm, k0, l0, k1, l1;
+  DMR_HIP(hipMemcpyAsync(res.ensure<uint8_t>(total + 64), seqs, total, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dof.ensure<uint32_t>(nseq + 1), o32.data(), (nseq + 1) * 4, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dso.ensure<uint32_t>(total), seq_of.data(), total * 4, hipMemcpyHostToDevice, st_));
+  SeedParams P;
+  std::memset(&P, 0, sizeof P);
+  P.nshapes = 1;
+  P.red_size = red.size;
+  std::memcpy(P.red_map, red.map, 25);
+  P.cut = cut;
+  for (int k = 2; k < 33; ++k) P.lnfact[k] = P.lnfact[k - 1] + std::log((double)k);
+  P.shapes[0].weight = shape.weight;
+  P.shapes[0].span = shape.span;
+  std::memcpy(P.shapes[0].pos, shape.pos, sizeof shape.pos);
+  DMR_HIP(hipMemcpyAsync(prm.ensure<SeedParams>(1), &P, sizeof P, hipMemcpyHostToDevice, st_));
+  long double mx = std::pow((long double)red.size, (long double)shape.weight) - 1;
+  if (mx >= 18446744073709551615.0L) throw Error(-5, "seed key does not fit 64 bits");
+  int kb = bits_for((uint64_t)mx);
+  uint32_t nb = seed_enum_blocks(0, total);
+  uint64_t nbk = nb + 1ull;
+  uint32_t* bc = b_counts_.ensure<uint32_t>(nbk);
+  launch_fill_u32(bc + nb, 1, 0, st_);
+  uint64_t* bo = b_off_.ensure<uint64_t>(nbk);
+  uint64_t* kk = k0.ensure<uint64_t>(total + 1);
+  uint32_t* ll = l0.ensure<uint32_t>(total + 1);
+  launch_seed_enumerate(res.as<uint8_t>(), dso.as<uint32_t>(), dof.as<uint32_t>(), 0, total,
+                        prm.as<SeedParams>(), 1, kb, bc, bo, b_scan_.ensure<uint64_t>(scan_tmp_elems(nbk)),
+                        kk, ll, st_);
+  uint64_t n = read1(bo + nb);
+  radix_sort_pairs(kk, ll, k1.ensure<uint64_t>(n + 1), l1.ensure<uint32_t>(n + 1), n, kb,
+                   b_hist_.ensure<uint32_t>(radix_hist_elems(n)),
+                   b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(n)), st_);
+  keys->resize(n);
+  std::vector<uint32_t> locs(n);
+  DMR_HIP(hipMemcpyAsync(keys->data(), kk, n * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipMemcpyAsync(locs.data(), ll, n * 4, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  seq->resize(n);
+  pos->resize(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t s = seq_of[locs[i]];
+    (*seq)[i] = base + s;
+    (*pos)[i] = locs[i] - o32[s];
+  }
+}
+
+// seed::join_into (src/seed.rs:110-143) on key-sorted arrays; hits are
+// returned sorted by (query, subject, diagonal, spos).
+void Engine::seed_join(const uint64_t* qk, const uint32_t* qs, const uint32_t* qp, uint64_t nq,
+                       const uint64_t* rk, const uint32_t* rs, const uint32_t* rp, uint64_t nr,
+                       uint64_t max_expand, std::vector<uint32_t>* out4) {
+  DMR_HIP(hipSetDevice(device_));
+  out4->clear();
+  if (!nq || !nr) return;
+  for (uint64_t i = 1; i < nq; ++i)
+    if (qk[i] < qk[i - 1]) throw Error(-1, "query seeds are not sorted by key");
+  for (uint64_t i = 1; i < nr; ++i)
+    if (rk[i] < rk[i - 1]) throw Error(-1, "reference seeds are not sorted by key");
+  uint64_t maxkey = std::max(qk[nq - 1], rk[nr - 1]);
+  uint32_t mq = 0, mqp = 0, ms = 0, msp = 0;
+  for (uint64_t i = 0; i < nq; ++i) { mq = std::max(mq, qs[i]); mqp = std::max(mqp, qp[i]); }
+  for (uint64_t i = 0; i < nr; ++i) { ms = std::max(ms, rs[i]); msp = std::max(msp, rp[i]); }
+  int kb = bits_for(maxkey);
+  int tb = std::min(kb, 22);
+  DevBuf dqk, dqs, dqp, drk, drs, drp, tab, dfl, dps, druns, dcnt, drlo, doff, dh, dh2, desc;
+  DMR_HIP(hipMemcpyAsync(dqk.ensure<uint64_t>(nq), qk, nq * 8, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dqs.ensure<uint32_t>(nq), qs, nq * 4, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dqp.ensure<uint32_t>(nq), qp, nq * 4, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(drk.ensure<uint64_t>(nr), rk, nr * 8, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(drs.ensure<uint32_t>(nr), rs, nr * 4, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(drp.ensure<uint32_t>(nr), rp, nr * 4, hipMemcpyHostToDevice, st_));
+  RefChunkDev ch;
+  ch.keys = drk.as<uint64_t>();
+  ch.locs = nullptr;
+  ch.n = nr;
+  ch.shift = kb - tb;
+  ch.nbuckets = (maxkey >> ch.shift) + 1;
+  ch.table = tab.ensure<uint32_t>(ch.nbuckets + 1);
+  ch.pad = 0;
+  launch_bucket_table(ch.keys, nr, ch.shift, ch.nbuckets, tab.as<uint32_t>(), st_);
+  DMR_HIP(hipMemcpyAsync(desc.ensure<RefChunkDev>(1), &ch, sizeof ch, hipMemcpyHostToDevice, st_));
+  uint32_t* fl = dfl.ensure<uint32_t>(nq + 1);
+  launch_run_flags(dqk.as<uint64_t>(), nq, fl, st_);
+  launch_fill_u32(fl + nq, 1, 0, st_);
+  uint64_t* ps = dps.ensure<uint64_t>(nq + 1);
+  scan_exclusive_u32_to_u64(fl, ps, nq + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nq + 1)), st_);
+  uint64_t nruns = read1(ps + nq);
+  uint32_t* runs = druns.ensure<uint32_t>(nruns + 1);
+  launch_compact_flags(fl, ps, nq, runs, st_);
+  uint32_t* cnt = dcnt.ensure<uint32_t>(nruns + 1);
+  uint32_t* rlo = drlo.ensure<uint32_t>(nruns + 1);
+  launch_join_count(dqk.as<uint64_t>(), runs, nruns, nq, desc.as<RefChunkDev>(), 1, max_expand, cnt,
+                    rlo, st_);
+  launch_fill_u32(cnt + nruns, 1, 0, st_);
+  uint64_t* hoff = doff.ensure<uint64_t>(nruns + 1);
+  scan_exclusive_u32_to_u64(cnt, hoff, nruns + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nruns + 1)), st_);
+  uint64_t nh = read1(hoff + nruns);
+  HitGeom g;
+  g.bt = bits_for(mq);
+  g.bs = bits_for(ms);
+  g.bp = bits_for(msp);
+  g.bd = bits_for((uint64_t)mqp + msp + 1);
+  g.diag_bias = (int64_t)msp;
+  int hb = (int)(g.bt + g.bs + g.bd + g.bp);
+  if (hb > 64) throw Error(-5, "hit key does not fit 64 bits");
+  uint64_t* hits = dh.ensure<uint64_t>(nh + 1);
+  launch_join_emit_entries(dqs.as<uint32_t>(), dqp.as<uint32_t>(), runs, nruns, nq, drs.as<uint32_t>(),
+                           drp.as<uint32_t>(), cnt, rlo, hoff, g, hits, st_);
+  radix_sort_keys(hits, dh2.ensure<uint64_t>(nh + 1), nh, hb, b_hist_.ensure<uint32_t>(radix_hist_elems(nh)),
+                  b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nh)), st_);
+  std::vector<uint64_t> h(nh);
+  DMR_HIP(hipMemcpyAsync(h.data(), hits, nh * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  out4->resize(nh * 4);
+  const uint64_t pm = (1ull << g.bp) - 1, dm = (1ull << g.bd) - 1, sm = (1ull << g.bs) - 1;
+  for (uint64_t i = 0; i < nh; ++i) {
+    uint64_t k = h[i];
+    uint32_t spos = (uint32_t)(k & pm);
+    int64_t diag = (int64_t)((k >> g.bp) & dm) - g.diag_bias;
+    uint32_t subj = (uint32_t)((k >> (g.bp + g.bd)) & sm);
+    uint32_t qry = (uint32_t)(k >> (g.bp + g.bd + g.bs));
+    (*out4)[4 * i] = qry;
+    (*out4)[4 * i + 1] = (uint32_t)(diag + spos);
+    (*out4)[4 * i + 2] = subj;
+    (*out4)[4 * i + 3] = spos;
+  }
+}
+
+// ungapped::xdrop (src/ungapped.rs:38-88) for many seeds of one sequence pair.
+void Engine::xdrop_batch(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint64_t slen,
+                         const uint32_t* qa, const uint32_t* sa, uint64_t n, int xdrop, SegRec* out) {
+  DMR_HIP(hipSetDevice(device_));
+  if (!n) return;
+  for (uint64_t i = 0; i < n; ++i)
+    if (qa[i] > qlen || sa[i] > slen) throw Error(-1, "seed position outside the sequence");
+  DevBuf dq, ds, dqa, dsa, dout;
+  DMR_HIP(hipMemcpyAsync(dq.ensure<uint8_t>(qlen + 1), q, qlen, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(ds.ensure<uint8_t>(slen + 1), s, slen, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dqa.ensure<uint32_t>(n), qa, n * 4, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dsa.ensure<uint32_t>(n), sa, n * 4, hipMemcpyHostToDevice, st_));
+  launch_xdrop_pairs(dq.as<uint8_t>(), qlen, ds.as<uint8_t>(), slen, dqa.as<uint32_t>(),
+                     dsa.as<uint32_t>(), n, matrix32_.as<int8_t>(), xdrop, dout.ensure<SegRec>(n), st_);
+  DMR_HIP(hipMemcpyAsync(out, dout.as<SegRec>(), n * sizeof(SegRec), hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+}
+
+// gapped::extend_span (src/gapped.rs:67-280) for a batch of sequence pairs.
+void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uint8_t* sbuf,
+                               uint64_t sbuf_len, const SpanJob* jobs, uint64_t njobs, int gap_open,
+                               int gap_extend, bool traceback, std::vector<SwOut>* outs,
+                               std::vector<std::vector<uint32_t>>* ops) {
+  DMR_HIP(hipSetDevice(device_));
+  outs->assign(njobs, SwOut{});
+  if (ops) ops->assign(njobs, {});
+  if (!njobs) return;
+  if (qbuf_len >= (1ull << 32) || sbuf_len >= (1ull << 32)) throw Error(-5, "buffers larger than 4G");
+  // Each job becomes (target j, subject j) of a private offsets table.
+  std::vector<uint32_t> qo(2 * njobs), so(2 * njobs);
+  std::vector<SwJob> dj(njobs);
+  for (uint64_t i = 0; i < njobs; ++i) {
+    const SpanJob& J = jobs[i];
+    if (J.q_off + J.qlen > qbuf_len || J.s_off + J.slen > sbuf_len)
+      throw Error(-1, "job outside its buffer");
+    int64_t d0 = std::min(J.d0, J.d1), d1 = std::max(J.d0, J.d1);
+    if (d0 < INT32_MIN / 2 || d1 > INT32_MAX / 2) throw Error(-1, "diagonal out of range");
+    qo[2 * i] = (uint32_t)J.q_off;
+    qo[2 * i + 1] = (uint32_t)(J.q_off + J.qlen);
+    so[2 * i] = (uint32_t)J.s_off;
+    so[2 * i + 1] = (uint32_t)(J.s_off + J.slen);
+    dj[i] = SwJob{(uint32_t)(2 * i), (uint32_t)(2 * i), 0, (uint32_t)J.qlen, (int32_t)d0, (int32_t)d1,
+                  (uint32_t)i, 0};
+  }
+  DevBuf dq, ds, dqo, dso, djobs, dst, dsoff, dscr, dcells, dw0, dtoff, dtr, doo, dops, dout;
+  DMR_HIP(hipMemcpyAsync(dq.ensure<uint8_t>(qbuf_len + 64), qbuf, qbuf_len, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(ds.ensure<uint8_t>(sbuf_len + 64), sbuf, sbuf_len, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dqo.ensure<uint32_t>(2 * njobs), qo.data(), 8 * njobs, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dso.ensure<uint32_t>(2 * njobs), so.data(), 8 * njobs, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(djobs.ensure<SwJob>(njobs), dj.data(), njobs * sizeof(SwJob), hipMemcpyHostToDevice, st_));
+  SeqSet qs{dq.as<uint8_t>(), dqo.as<uint32_t>()}, ss{ds.as<uint8_t>(), dso.as<uint32_t>()};
+  uint64_t* cells = dcells.ensure<uint64_t>(njobs);
+  uint64_t* w0 = dw0.ensure<uint64_t>(njobs);
+  uint32_t* stv = dst.ensure<uint32_t>(njobs);
+  launch_job_geometry(djobs.as<SwJob>(), nullptr, njobs, ss, cells, w0, stv, st_);
+  std::vector<uint64_t> hc(njobs), hw(njobs);
+  std::vector<uint32_t> hs(njobs);
+  DMR_HIP(hipMemcpyAsync(hc.data(), cells, njobs * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipMemcpyAsync(hw.data(), w0, njobs * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipMemcpyAsync(hs.data(), stv, njobs * 4, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  std::vector<uint64_t> soff(njobs + 1, 0), toff(njobs + 1, 0), ooff(njobs + 1, 0);
+  for (uint64_t i = 0; i < njobs; ++i) {
+    soff[i + 1] = soff[i] + 4ull * hs[i];
+    toff[i + 1] = toff[i] + (traceback ? hc[i] + hw[i] : 0);
+    ooff[i + 1] = ooff[i] + (traceback ? jobs[i].qlen + jobs[i].slen + 1 : 0);
+  }
+  DMR_HIP(hipMemcpyAsync(dsoff.ensure<uint64_t>(njobs + 1), soff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(dtoff.ensure<uint64_t>(njobs + 1), toff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(doo.ensure<uint64_t>(njobs + 1), ooff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
+  SwOut* o = dout.ensure<SwOut>(njobs);
+  uint32_t* opsd = dops.ensure<uint32_t>(ooff[njobs] + 1);
+  launch_sw_generic(djobs.as<SwJob>(), nullptr, njobs, qs, ss, matrix32_.as<int8_t>(), gap_open,
+                    gap_extend, dscr.ensure<int32_t>(soff[njobs] + 1), dsoff.as<uint64_t>(),
+                    dtr.ensure<uint8_t>(toff[njobs] + 1), dtoff.as<uint64_t>(), opsd,
+                    doo.as<uint64_t>(), o, traceback, st_);
+  DMR_HIP(hipMemcpyAsync(outs->data(), o, njobs * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+  std::vector<uint32_t> hops(ooff[njobs]);
+  if (traceback && ooff[njobs])
+    DMR_HIP(hipMemcpyAsync(hops.data(), opsd, ooff[njobs] * 4, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  if (ops && traceback)
+    for (uint64_t i = 0; i < njobs; ++i)
+      (*ops)[i].assign(hops.begin() + ooff[i], hops.begin() + ooff[i] + (*outs)[i].n_ops);
+}
+
+}  // namespace dmr

@@ -1,0 +1,201 @@
+// Host orchestration of the MI355X search core.
+//
+// Restates src/search.rs:95-396 around device kernels: presets, resident seed
+// index per DB chunk, query seeding, join, ungapped extension, band merge,
+// banded DP, selection by E-value, traceback, culling and frame merging.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dmr_alp.hpp"
+#include "dmr_device.hpp"
+#include "dmr_kernels.hpp"
+#include "dmr_tables.hpp"
+
+namespace dmr {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define DMR_HIP(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::dmr::Error(e_ == hipErrorOutOfMemory ? -3 : -2,                               \
+                         std::string(#x) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+// Growable device buffer (reallocated only when a larger size is requested).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+  template <class T>
+  T* ensure(size_t n) {
+    size_t bytes = n * sizeof(T) + 256;
+    if (bytes > cap) {
+      release();
+      size_t c = bytes + bytes / 4;
+      DMR_HIP(hipMalloc(&p, c));
+      cap = c;
+    }
+    return static_cast<T*>(p);
+  }
+};
+
+struct SearchOptions {
+  std::string sensitivity = "default";
+  double evalue = 0.001;
+  uint32_t max_target_seqs = 25, max_hsps = 1;
+  int index_chunks = -1;  // <0: preset
+  double seed_cut = -1;   // <0: preset
+  int gap_open = 11, gap_extend = 1;
+  uint64_t max_expand = 4096;
+};
+
+struct HspRec {
+  uint32_t query, subject, frame;
+  uint32_t q_begin, q_end, s_begin, s_end;  // 0-based half-open, frame coordinates
+  int32_t score;
+  double bits, evalue;
+  uint32_t identities, positives, mismatches, gap_openings, gaps, length;
+  std::vector<uint32_t> ops;  // op | count << 2 (0 = M, 1 = I, 2 = D)
+};
+
+struct StageStats {
+  uint64_t query_seeds = 0, runs = 0, hits = 0, hit_runs = 0, segments = 0, groups = 0;
+  uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
+  double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
+         ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
+};
+
+// A batch of query targets (one per protein query, or <= 6 frames per read).
+struct TargetSet {
+  std::vector<uint8_t> letters;
+  std::vector<uint32_t> offs;      // n+1
+  std::vector<uint32_t> query_of;  // record index of each target
+  std::vector<uint8_t> frame_of;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return st_; }
+
+  // Resident database (codes 0..24, offsets n+1).
+  void load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n);
+  uint32_t db_size() const { return db_n_; }
+  uint64_t db_letters() const { return db_letters_; }
+  const std::vector<uint32_t>& db_offsets_host() const { return h_db_offs_; }
+  const std::vector<uint8_t>& db_letters_host() const { return h_db_res_; }
+
+  // Resident seed index for a preset (rebuilt only when the preset changes).
+  void build_index(const std::string& sensitivity, int index_chunks, double seed_cut);
+  double last_index_ms() const { return index_ms_; }
+  uint64_t index_entries() const;
+
+  // Upload a target batch (kept resident until the next call).
+  void load_targets(const TargetSet& t);
+  // Search the resident targets; HSPs per target, ordered as align_target
+  // returns them (src/search.rs:262-270).
+  std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats);
+  // Cells of every DP of the last search (one extra pass, for GCUPS).
+  uint64_t count_last_dp_cells();
+
+  // --- drop-in calls on host buffers -------------------------------------
+  void seed_build(const uint8_t* seqs, const uint64_t* offs, uint32_t nseq, const ShapeDef& shape,
+                  const ReductionDef& red, uint32_t base, double cut, std::vector<uint64_t>* keys,
+                  std::vector<uint32_t>* seq, std::vector<uint32_t>* pos);
+  void seed_join(const uint64_t* qk, const uint32_t* qs, const uint32_t* qp, uint64_t nq,
+                 const uint64_t* rk, const uint32_t* rs, const uint32_t* rp, uint64_t nr,
+                 uint64_t max_expand, std::vector<uint32_t>* out4);
+  void xdrop_batch(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint64_t slen,
+                   const uint32_t* qa, const uint32_t* sa, uint64_t n, int xdrop, SegRec* out);
+  struct SpanJob {
+    uint64_t q_off, qlen, s_off, slen;
+    int64_t d0, d1;
+  };
+  void extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uint8_t* sbuf,
+                         uint64_t sbuf_len, const SpanJob* jobs, uint64_t njobs, int gap_open,
+                         int gap_extend, bool traceback, std::vector<SwOut>* outs,
+                         std::vector<std::vector<uint32_t>>* ops);
+
+  AlpParams alp(int gap_open, int gap_extend) const;
+
+ private:
+  struct IndexChunk {
+    DevBuf keys, locs, table;
+    uint64_t n = 0, nbuckets = 0;
+    int shift = 0;
+  };
+  void upload_seed_params(const std::string& sens, double cut, uint32_t* nshapes, int* key_bits,
+                          int* shape_bits);
+  void stage_event(int k);
+  float stage_ms(int a, int b);
+  template <class T>
+  T read1(const T* dptr);
+
+  int device_ = 0;
+  hipStream_t st_ = nullptr;
+  hipEvent_t ev_[16] = {};
+
+  // database
+  DevBuf db_res_, db_offs_, db_seq_of_;
+  uint32_t db_n_ = 0;
+  uint64_t db_letters_ = 0;
+  uint32_t db_max_len_ = 0;
+  std::vector<uint32_t> h_db_offs_;
+  std::vector<uint8_t> h_db_res_;
+
+  // index
+  std::string idx_sens_;
+  int idx_chunks_ = 0;
+  double idx_cut_ = 0;
+  std::vector<IndexChunk> chunks_;
+  DevBuf chunk_desc_, seed_params_, ref_params_;
+  int key_bits_ = 0, shape_bits_ = 0;
+  uint32_t nshapes_ = 0;
+  double index_ms_ = 0;
+
+  // matrix (32 x 32 int8, padded letters score -64)
+  DevBuf matrix32_;
+
+  // targets
+  TargetSet tset_;
+  DevBuf q_res_, q_offs_, q_seq_of_;
+  uint32_t q_n_ = 0, q_max_len_ = 0;
+  uint64_t q_total_ = 0;
+
+  // per-search scratch
+  DevBuf b_counts_, b_off_, b_scan_, b_hist_;
+  DevBuf qkeys_, qlocs_, qkeys2_, qlocs2_, flags_, pos_, runs_;
+  DevBuf jc_counts_, jc_rlo_, jc_off_;
+  DevBuf hits_, hits2_;
+  DevBuf segs_raw_, keep_, segs_, gkeys_, gstart_;
+  DevBuf bd0_, bd1_, bbest_, tmpjobs_, nbands_, joboff_, jobs_;
+  DevBuf stride_, scr_off_, scratch_, outs_;
+  DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
+  uint64_t last_njobs_ = 0;
+};
+
+}  // namespace dmr

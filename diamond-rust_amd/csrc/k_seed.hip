@@ -1,0 +1,303 @@
+// Spaced-seed enumeration, reference-index bucket tables and the seed join.
+//
+// Enumeration restates Shape::packed + seed_is_complex (src/shape.rs:60-74,
+// src/seed.rs:46-70): a seed at (sequence, pos) exists when the shape fits in
+// the sequence, every sampled letter is canonical (< 20), and the log
+// multinomial of the bucket counts reaches the cut. Keys are base-`size`
+// numbers with the first sampled letter most significant; the shape index is
+// prepended above the key bits so all shapes sort in one pass.
+//
+// The join restates seed::join_into (src/seed.rs:110-143): for every equal
+// key, nq query entries (whole batch) and nr reference entries (one DB
+// chunk) expand to nq*nr hits unless nq*nr > max_expand.
+#include "dmr_device.hpp"
+#include "dmr_kernels.hpp"
+
+namespace dmr {
+
+namespace {
+
+constexpr int kSeedThreads = 256;
+
+// Evaluates the seed at residue `loc` for `shape`; returns true and the
+// tagged key if it is emitted.
+__device__ __forceinline__ bool seed_at(const uint8_t* __restrict__ res,
+                                        const uint32_t* __restrict__ seq_of,
+                                        const uint32_t* __restrict__ offs, uint64_t loc,
+                                        const SeedParams& P, const ShapeDev& sh, uint32_t s,
+                                        int key_bits, uint64_t* key_out) {
+  uint32_t sid = seq_of[loc];
+  uint64_t start = offs[sid], end = offs[sid + 1];
+  if (loc + sh.span > end) return false;
+  (void)start;
+  uint64_t key = 0;
+  // Bucket counts packed as 5-bit fields, 12 buckets per word (counts <= 16).
+  uint64_t c_lo = 0, c_hi = 0;
+  for (uint32_t k = 0; k < sh.weight; ++k) {
+    uint8_t l = res[loc + sh.pos[k]];
+    if (l >= 20) return false;
+    uint32_t b = P.red_map[l];
+    if (b >= 32) return false;
+    key = key * P.red_size + b;
+    if (b < 12) c_lo += 1ull << (5 * b);
+    else c_hi += 1ull << (5 * (b - 12));
+  }
+  if (P.cut > 0.0) {
+    double e = P.lnfact[sh.weight];
+    for (uint32_t g = 0; g < P.red_size && g < 24; ++g) {
+      uint32_t c = g < 12 ? (uint32_t)((c_lo >> (5 * g)) & 31u) : (uint32_t)((c_hi >> (5 * (g - 12))) & 31u);
+      if (c > 0) e -= P.lnfact[c];
+    }
+    if (!(e >= P.cut)) return false;
+  }
+  *key_out = ((uint64_t)s << key_bits) | key;
+  return true;
+}
+
+// grid: (blocks over [r0, r1), nshapes)
+__global__ void k_seed_count(const uint8_t* __restrict__ res, const uint32_t* __restrict__ seq_of,
+                             const uint32_t* __restrict__ offs, uint64_t r0, uint64_t r1,
+                             const SeedParams* __restrict__ Pp, int key_bits,
+                             uint32_t* __restrict__ block_counts) {
+  __shared__ uint32_t cnt;
+  const SeedParams& P = *Pp;
+  const uint32_t s = blockIdx.y;
+  const ShapeDev sh = P.shapes[s];
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint64_t loc = r0 + (uint64_t)blockIdx.x * kSeedThreads + threadIdx.x;
+  uint64_t key;
+  bool ok = loc < r1 && seed_at(res, seq_of, offs, loc, P, sh, s, key_bits, &key);
+  uint64_t m = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&cnt, (uint32_t)__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[(uint64_t)s * gridDim.x + blockIdx.x] = cnt;
+}
+
+__global__ void k_seed_emit(const uint8_t* __restrict__ res, const uint32_t* __restrict__ seq_of,
+                            const uint32_t* __restrict__ offs, uint64_t r0, uint64_t r1,
+                            const SeedParams* __restrict__ Pp, int key_bits,
+                            const uint64_t* __restrict__ block_off, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ locs) {
+  __shared__ uint32_t wave_cnt[kSeedThreads / 64];
+  const SeedParams& P = *Pp;
+  const uint32_t s = blockIdx.y;
+  const ShapeDev sh = P.shapes[s];
+  uint64_t loc = r0 + (uint64_t)blockIdx.x * kSeedThreads + threadIdx.x;
+  uint64_t key = 0;
+  bool ok = loc < r1 && seed_at(res, seq_of, offs, loc, P, sh, s, key_bits, &key);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t m = __ballot(ok);
+  if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (ok) {
+    uint64_t pos = block_off[(uint64_t)s * gridDim.x + blockIdx.x];
+    for (int w = 0; w < wave; ++w) pos += wave_cnt[w];
+    uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    pos += (uint64_t)__popcll(m & lt);
+    keys[pos] = key;
+    locs[pos] = (uint32_t)loc;
+  }
+}
+
+// table[b] = first index whose key >= b << shift (lower bound per bucket).
+__global__ void k_bucket_table(const uint64_t* __restrict__ keys, uint64_t n, int shift,
+                               uint64_t nbuckets, uint32_t* __restrict__ table) {
+  uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbuckets) return;
+  uint64_t target = b << shift;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  table[b] = (uint32_t)lo;
+}
+
+__global__ void k_run_flags(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flags) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_compact_flags(const uint32_t* __restrict__ flags, const uint64_t* __restrict__ pos,
+                                uint64_t n, uint32_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) out[pos[i]] = (uint32_t)i;
+}
+
+__device__ __forceinline__ void ref_range(const uint64_t* __restrict__ rkeys,
+                                          const uint32_t* __restrict__ table, int shift,
+                                          uint64_t key, uint32_t* lo_out, uint32_t* hi_out) {
+  uint64_t b = key >> shift;
+  uint32_t lo = table[b], hi = table[b + 1];
+  // lower_bound(key) and upper_bound(key) inside the bucket.
+  uint32_t a = lo, z = hi;
+  while (a < z) {
+    uint32_t mid = (a + z) >> 1;
+    if (rkeys[mid] < key) a = mid + 1;
+    else z = mid;
+  }
+  uint32_t first = a;
+  z = hi;
+  while (a < z) {
+    uint32_t mid = (a + z) >> 1;
+    if (rkeys[mid] <= key) a = mid + 1;
+    else z = mid;
+  }
+  *lo_out = first;
+  *hi_out = a;
+}
+
+// grid over runs; one thread per (run, chunk).
+__global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t* __restrict__ run_start,
+                             uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
+                             uint32_t nchunks, uint64_t max_expand, uint32_t* __restrict__ counts,
+                             uint32_t* __restrict__ rlo) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nruns * nchunks) return;
+  uint64_t r = t / nchunks;
+  uint32_t c = (uint32_t)(t - r * nchunks);
+  uint64_t s0 = run_start[r];
+  uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
+  uint64_t key = qkeys[s0];
+  const RefChunkDev ch = chunks[c];
+  uint32_t lo = 0, hi = 0;
+  if (ch.n > 0 && (key >> ch.shift) < ch.nbuckets) ref_range(ch.keys, ch.table, ch.shift, key, &lo, &hi);
+  uint64_t nq = s1 - s0, nr = hi - lo;
+  uint64_t cnt = nq * nr;
+  counts[t] = (nr > 0 && cnt <= max_expand) ? (uint32_t)cnt : 0u;
+  rlo[t] = lo;
+}
+
+// One thread per (run, chunk) pair; writes its nq*nr hits as packed keys.
+__global__ void k_join_emit(const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start,
+                            uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
+                            uint32_t nchunks, const uint32_t* __restrict__ counts,
+                            const uint32_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
+                            const uint32_t* __restrict__ q_seq_of, const uint32_t* __restrict__ q_offs,
+                            const uint32_t* __restrict__ db_seq_of, const uint32_t* __restrict__ db_offs,
+                            HitGeom g, uint64_t* __restrict__ hits) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nruns * nchunks) return;
+  uint32_t cnt = counts[t];
+  if (cnt == 0) return;
+  uint64_t r = t / nchunks;
+  uint32_t c = (uint32_t)(t - r * nchunks);
+  uint64_t s0 = run_start[r];
+  uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
+  uint32_t nq = (uint32_t)(s1 - s0), nr = cnt / nq;
+  const uint32_t* rl = chunks[c].locs + rlo[t];
+  uint64_t o = hit_off[t];
+  for (uint32_t a = 0; a < nq; ++a) {
+    uint32_t ql = qlocs[s0 + a];
+    uint32_t tg = q_seq_of[ql];
+    uint32_t qpos = ql - q_offs[tg];
+    for (uint32_t b = 0; b < nr; ++b) {
+      uint32_t rloc = rl[b];
+      uint32_t sj = db_seq_of[rloc];
+      uint32_t spos = rloc - db_offs[sj];
+      uint64_t diag = (uint64_t)((int64_t)qpos - (int64_t)spos + g.diag_bias);
+      uint64_t key = ((((uint64_t)tg << g.bs) | sj) << g.bd | diag) << g.bp | spos;
+      hits[o++] = key;
+    }
+  }
+}
+
+// Join emission for explicit (seq, pos) entry arrays (drop-in seed::join_into).
+__global__ void k_join_emit_entries(const uint32_t* __restrict__ qs, const uint32_t* __restrict__ qp,
+                                    const uint32_t* __restrict__ run_start, uint64_t nruns,
+                                    uint64_t nq_total, const uint32_t* __restrict__ rs,
+                                    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ counts,
+                                    const uint32_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
+                                    HitGeom g, uint64_t* __restrict__ hits) {
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  uint32_t cnt = counts[r];
+  if (cnt == 0) return;
+  uint64_t s0 = run_start[r];
+  uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
+  uint32_t nq = (uint32_t)(s1 - s0), nr = cnt / nq;
+  uint64_t o = hit_off[r];
+  for (uint32_t a = 0; a < nq; ++a) {
+    uint32_t tg = qs[s0 + a], qpos = qp[s0 + a];
+    for (uint32_t b = 0; b < nr; ++b) {
+      uint32_t sj = rs[rlo[r] + b], spos = rp[rlo[r] + b];
+      uint64_t diag = (uint64_t)((int64_t)qpos - (int64_t)spos + g.diag_bias);
+      hits[o++] = ((((uint64_t)tg << g.bs) | sj) << g.bd | diag) << g.bp | spos;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_join_emit_entries(const uint32_t* qs, const uint32_t* qp, const uint32_t* run_start,
+                              uint64_t nruns, uint64_t nq_total, const uint32_t* rs,
+                              const uint32_t* rp, const uint32_t* counts, const uint32_t* rlo,
+                              const uint64_t* hit_off, HitGeom g, uint64_t* hits, hipStream_t st) {
+  if (!nruns) return;
+  hipLaunchKernelGGL(k_join_emit_entries, dim3((unsigned)((nruns + 255) / 256)), dim3(256), 0, st, qs,
+                     qp, run_start, nruns, nq_total, rs, rp, counts, rlo, hit_off, g, hits);
+}
+
+void launch_seed_enumerate(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs,
+                           uint64_t r0, uint64_t r1, const SeedParams* P, uint32_t nshapes,
+                           int key_bits, uint32_t* block_counts, uint64_t* block_off,
+                           uint64_t* scan_tmp, uint64_t* keys, uint32_t* locs, hipStream_t st) {
+  if (r1 <= r0) return;
+  uint32_t nb = (uint32_t)((r1 - r0 + kSeedThreads - 1) / kSeedThreads);
+  dim3 grid(nb, nshapes);
+  hipLaunchKernelGGL(k_seed_count, grid, dim3(kSeedThreads), 0, st, res, seq_of, offs, r0, r1, P,
+                     key_bits, block_counts);
+  scan_exclusive_u32_to_u64(block_counts, block_off, (uint64_t)nb * nshapes + 1, scan_tmp, st);
+  hipLaunchKernelGGL(k_seed_emit, grid, dim3(kSeedThreads), 0, st, res, seq_of, offs, r0, r1, P,
+                     key_bits, block_off, keys, locs);
+}
+
+uint32_t seed_enum_blocks(uint64_t r0, uint64_t r1) {
+  return (uint32_t)((r1 - r0 + kSeedThreads - 1) / kSeedThreads);
+}
+
+void launch_bucket_table(const uint64_t* keys, uint64_t n, int shift, uint64_t nbuckets,
+                         uint32_t* table, hipStream_t st) {
+  uint64_t threads = nbuckets + 1;
+  hipLaunchKernelGGL(k_bucket_table, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, keys,
+                     n, shift, nbuckets, table);
+}
+
+void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_run_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, flags);
+}
+
+void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n, uint32_t* out,
+                          hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_compact_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, pos,
+                     n, out);
+}
+
+void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_t nruns,
+                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
+                       uint64_t max_expand, uint32_t* counts, uint32_t* rlo, hipStream_t st) {
+  uint64_t n = nruns * nchunks;
+  if (!n) return;
+  hipLaunchKernelGGL(k_join_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qkeys,
+                     run_start, nruns, nq_total, chunks, nchunks, max_expand, counts, rlo);
+}
+
+void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
+                      uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
+                      const uint32_t* counts, const uint32_t* rlo, const uint64_t* hit_off,
+                      const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
+                      const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st) {
+  uint64_t n = nruns * nchunks;
+  if (!n) return;
+  hipLaunchKernelGGL(k_join_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs,
+                     run_start, nruns, nq_total, chunks, nchunks, counts, rlo, hit_off, q_seq_of,
+                     q_offs, db_seq_of, db_offs, g, hits);
+}
+
+}  // namespace dmr

@@ -1,0 +1,228 @@
+// Prefix scans and LSD radix sort for the seed and hit arrays.
+//
+// Radix sort: 8-bit digits, one (histogram, scan, scatter) triple per pass.
+// The scatter ranks keys stably inside a 256-thread tile with 64-lane ballot
+// matching (8 ballots give each lane the mask of lanes sharing its digit), so
+// every pass is stable and the final order equals a stable sort by key.
+#include "dmr_device.hpp"
+
+namespace dmr {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total to *total.
+__device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* total) {
+  __shared__ uint64_t wave_sums[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wave_sums[wave] = x;
+  __syncthreads();
+  uint64_t wave_off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    uint64_t s = wave_sums[w];
+    if (w < wave) wave_off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + x - v;
+}
+
+// Exclusive scan of one tile per block; adds block_off[blockIdx] if given.
+template <class T>
+__global__ void k_scan_apply(const T* in, uint64_t* out, uint64_t n, const uint64_t* block_off) {
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? (uint64_t)in[i] : 0;
+    s += v[k];
+  }
+  uint64_t tot;
+  uint64_t run = block_exclusive_scan(s, &tot);
+  if (block_off) run += block_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+}
+
+template <class T>
+__global__ void k_scan_reduce_blocked(const T* in, uint64_t n, uint64_t* sums) {
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint64_t i = base + k;
+    if (i < n) s += (uint64_t)in[i];
+  }
+  uint64_t tot;
+  block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <class T>
+void scan_impl(const T* in, uint64_t* out, uint64_t n, uint64_t* tmp, hipStream_t st) {
+  if (n == 0) return;
+  uint64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(kScanThreads), 0, st, in, out, n,
+                       (const uint64_t*)nullptr);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce_blocked<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, n,
+                     tmp);
+  scan_impl<uint64_t>(tmp, tmp, nb, tmp + nb, st);
+  hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n,
+                     (const uint64_t*)tmp);
+}
+
+// ---- radix sort -----------------------------------------------------------
+
+__global__ void k_radix_hist(const uint64_t* keys, uint64_t n, int shift, uint32_t* hist,
+                             uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int k = 0; k < kSortItems; ++k) {
+    uint64_t i = base + (uint64_t)k * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <bool kValues>
+__global__ void k_radix_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                                uint32_t* vout, uint64_t n, int shift, const uint64_t* offs,
+                                uint32_t nblocks) {
+  __shared__ uint32_t wave_hist[kSortWaves][256];
+  __shared__ uint64_t run[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < kSortWaves; ++w) wave_hist[w][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    uint64_t i = base + (uint64_t)k * kSortThreads + threadIdx.x;
+    bool valid = i < n;
+    uint64_t key = valid ? kin[i] : 0;
+    uint32_t val = (kValues && valid) ? vin[i] : 0;
+    uint32_t d = (uint32_t)(key >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(m & lt_mask);
+    if (valid && rank == 0) wave_hist[wave][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint64_t pos = run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wave_hist[w][d];
+      kout[pos] = key;
+      if (kValues) vout[pos] = val;
+    }
+    __syncthreads();
+    {
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < kSortWaves; ++w) {
+        s += wave_hist[w][threadIdx.x];
+        wave_hist[w][threadIdx.x] = 0;
+      }
+      run[threadIdx.x] += s;
+    }
+    __syncthreads();
+  }
+}
+
+template <bool kValues>
+void radix_impl(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n, int bits,
+                uint32_t* hist, uint64_t* scan_tmp, hipStream_t st) {
+  if (n <= 1 || bits <= 0) return;
+  uint32_t nb = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  uint64_t* offs = scan_tmp;                  // 256 * nb
+  uint64_t* tmp = scan_tmp + 256ull * nb;     // scan scratch
+  int passes = (bits + 7) / 8;
+  uint64_t *ka = k0, *kb = k1;
+  uint32_t *va = v0, *vb = v1;
+  for (int p = 0; p < passes; ++p) {
+    int shift = 8 * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist, nb);
+    scan_impl<uint32_t>(hist, offs, 256ull * nb, tmp, st);
+    hipLaunchKernelGGL(k_radix_scatter<kValues>, dim3(nb), dim3(kSortThreads), 0, st, ka, va, kb,
+                       vb, n, shift, offs, nb);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != k0) {
+    (void)hipMemcpyAsync(k0, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st);
+    if (kValues) (void)hipMemcpyAsync(v0, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+  }
+}
+
+}  // namespace
+
+uint64_t scan_tmp_elems(uint64_t n) {
+  uint64_t total = 0;
+  while (n > (uint64_t)kScanTile) {
+    n = (n + kScanTile - 1) / kScanTile;
+    total += n;
+  }
+  return total + 8;
+}
+
+uint64_t radix_hist_elems(uint64_t n) { return 256ull * ((n + kSortTile - 1) / kSortTile) + 256; }
+
+uint64_t radix_scan_tmp_elems(uint64_t n) {
+  uint64_t h = 256ull * ((n + kSortTile - 1) / kSortTile);
+  return h + scan_tmp_elems(h) + 8;
+}
+
+void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* tmp,
+                        hipStream_t st) {
+  scan_impl<uint64_t>(in, out, n, tmp, st);
+}
+
+void scan_exclusive_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* tmp,
+                               hipStream_t st) {
+  scan_impl<uint32_t>(in, out, n, tmp, st);
+}
+
+void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n, int bits,
+                      uint32_t* hist, uint64_t* scan_tmp, hipStream_t st) {
+  radix_impl<true>(k0, v0, k1, v1, n, bits, hist, scan_tmp, st);
+}
+
+void radix_sort_keys(uint64_t* k0, uint64_t* k1, uint64_t n, int bits, uint32_t* hist,
+                     uint64_t* scan_tmp, hipStream_t st) {
+  radix_impl<false>(k0, nullptr, k1, nullptr, n, bits, hist, scan_tmp, st);
+}
+
+}  // namespace dmr

@@ -1,0 +1,217 @@
+"""Deterministic synthetic protein workloads for tests and the benchmark.
+
+Residues are drawn i.i.d. from the BLOSUM62 background frequencies
+(reference src/matrix.rs:16-19, NCBI letter order) and encoded as uint8
+codes 0..19. Sequence lengths are log-normal and clipped. Homologs are
+planted by substituting residues from the BLOSUM62 conditional distribution
+P(b|a) = p_b * exp(lambda_u * s_ab) (lambda_u solves
+sum_ab p_a p_b exp(lambda_u s_ab) = 1, the method of src/matrix.rs:245-272),
+mixed with identity-keeping or background draws to reach a target identity,
+plus insertions/deletions at a per-site rate with geometric lengths.
+
+Everything is a pure function of the workload name and seed, so the CPU
+reference and the GPU path always see identical bytes.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BACKGROUND = np.array(
+    [0.074, 0.052, 0.045, 0.054, 0.025, 0.034, 0.054, 0.074, 0.026, 0.068,
+     0.099, 0.058, 0.025, 0.047, 0.039, 0.057, 0.051, 0.013, 0.032, 0.073])
+BACKGROUND = BACKGROUND / BACKGROUND.sum()
+
+AA = "ARNDCQEGHILKMFPSTWYVBJZX*"
+
+# BLOSUM62 restricted to the 20 canonical letters (NCBI order).
+_B62 = np.array([
+    [4, -1, -2, -2, 0, -1, -1, 0, -2, -1, -1, -1, -1, -2, -1, 1, 0, -3, -2, 0],
+    [-1, 5, 0, -2, -3, 1, 0, -2, 0, -3, -2, 2, -1, -3, -2, -1, -1, -3, -2, -3],
+    [-2, 0, 6, 1, -3, 0, 0, 0, 1, -3, -3, 0, -2, -3, -2, 1, 0, -4, -2, -3],
+    [-2, -2, 1, 6, -3, 0, 2, -1, -1, -3, -4, -1, -3, -3, -1, 0, -1, -4, -3, -3],
+    [0, -3, -3, -3, 9, -3, -4, -3, -3, -1, -1, -3, -1, -2, -3, -1, -1, -2, -2, -1],
+    [-1, 1, 0, 0, -3, 5, 2, -2, 0, -3, -2, 1, 0, -3, -1, 0, -1, -2, -1, -2],
+    [-1, 0, 0, 2, -4, 2, 5, -2, 0, -3, -3, 1, -2, -3, -1, 0, -1, -3, -2, -2],
+    [0, -2, 0, -1, -3, -2, -2, 6, -2, -4, -4, -2, -3, -3, -2, 0, -2, -2, -3, -3],
+    [-2, 0, 1, -1, -3, 0, 0, -2, 8, -3, -3, -1, -2, -1, -2, -1, -2, -2, 2, -3],
+    [-1, -3, -3, -3, -1, -3, -3, -4, -3, 4, 2, -3, 1, 0, -3, -2, -1, -3, -1, 3],
+    [-1, -2, -3, -4, -1, -2, -3, -4, -3, 2, 4, -2, 2, 0, -3, -2, -1, -2, -1, 1],
+    [-1, 2, 0, -1, -3, 1, 1, -2, -1, -3, -2, 5, -1, -3, -1, 0, -1, -3, -2, -2],
+    [-1, -1, -2, -3, -1, 0, -2, -3, -2, 1, 2, -1, 5, 0, -2, -1, -1, -1, -1, 1],
+    [-2, -3, -3, -3, -2, -3, -3, -3, -1, 0, 0, -3, 0, 6, -4, -2, -2, 1, 3, -1],
+    [-1, -2, -2, -1, -3, -1, -1, -2, -2, -3, -3, -1, -2, -4, 7, -1, -1, -4, -3, -2],
+    [1, -1, 1, 0, -1, 0, 0, 0, -1, -2, -2, 0, -1, -2, -1, 4, 1, -3, -2, -2],
+    [0, -1, 0, -1, -1, -1, -1, -2, -2, -1, -1, -1, -1, -2, -1, 1, 5, -2, -2, 0],
+    [-3, -3, -4, -4, -2, -2, -3, -2, -2, -3, -2, -3, -1, 1, -4, -3, -2, 11, 2, -3],
+    [-2, -2, -2, -3, -2, -1, -2, -3, 2, -1, -1, -2, -1, 3, -3, -2, -2, 2, 7, -1],
+    [0, -3, -3, -3, -1, -2, -2, -3, -3, 3, 1, -2, 1, -1, -2, -2, 0, -3, -1, 4],
+], dtype=np.float64)
+
+
+def solve_lambda_u() -> float:
+    """Ungapped lambda of BLOSUM62 under the background (bisection)."""
+    pp = np.outer(BACKGROUND, BACKGROUND)
+
+    def f(lam):
+        return float((pp * np.exp(lam * _B62)).sum() - 1.0)
+
+    lo, hi = 1e-6, 1.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(mid) > 0:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)
+
+
+LAMBDA_U = solve_lambda_u()
+_COND = BACKGROUND[None, :] * np.exp(LAMBDA_U * _B62)
+_COND = _COND / _COND.sum(axis=1, keepdims=True)
+_COND_CUM = np.cumsum(_COND, axis=1)
+COND_IDENTITY = float((BACKGROUND * np.diag(_COND)).sum())
+BG_IDENTITY = float((BACKGROUND ** 2).sum())
+_BG_CUM = np.cumsum(BACKGROUND)
+
+
+def _residues(rng, n):
+    u = rng.random(n)
+    return np.minimum(np.searchsorted(_BG_CUM, u, side="right"), 19).astype(np.uint8)
+
+
+def _lengths(rng, n, median, sigma, lo, hi):
+    x = np.exp(np.log(median) + sigma * rng.standard_normal(n))
+    return np.clip(np.rint(x), lo, hi).astype(np.int64)
+
+
+def mutate(rng, seq: np.ndarray, identity: float, indel_rate: float) -> np.ndarray:
+    """A homolog of `seq` at roughly `identity` with geometric indels (p=0.5)."""
+    n = len(seq)
+    if identity >= COND_IDENTITY:
+        keep = (identity - COND_IDENTITY) / (1.0 - COND_IDENTITY)
+        bg = 0.0
+    else:
+        keep = 0.0
+        bg = (COND_IDENTITY - identity) / (COND_IDENTITY - BG_IDENTITY)
+    u = rng.random(n)
+    v = rng.random(n)
+    cond = np.minimum((_COND_CUM[seq] < v[:, None]).sum(axis=1), 19).astype(np.uint8)
+    bgr = _residues(rng, n)
+    out = np.where(u < keep, seq, np.where(rng.random(n) < bg, bgr, cond)).astype(np.uint8)
+    # Indels: at each site, with probability indel_rate, insert or delete a
+    # geometric(0.5) number of residues.
+    ev = np.nonzero(rng.random(n) < indel_rate)[0]
+    if len(ev) == 0:
+        return out
+    pieces = []
+    last = 0
+    for p in ev:
+        if p < last:
+            continue
+        L = int(rng.geometric(0.5))
+        pieces.append(out[last:p])
+        if rng.random() < 0.5:
+            pieces.append(_residues(rng, L))
+            last = p
+        else:
+            last = min(n, p + L)
+    pieces.append(out[last:])
+    return np.concatenate(pieces).astype(np.uint8)
+
+
+def _pack(seqs):
+    lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    letters = np.concatenate(seqs).astype(np.uint8) if seqs else np.zeros(0, np.uint8)
+    return letters, off
+
+
+class Workload:
+    """Packed DB + queries: letters (uint8 codes), offsets (uint64, n+1), ids."""
+
+    def __init__(self, name, db_seqs, db_ids, q_seqs, q_ids, meta):
+        self.name = name
+        self.db_letters, self.db_offsets = _pack(db_seqs)
+        self.q_letters, self.q_offsets = _pack(q_seqs)
+        self.db_ids = db_ids
+        self.q_ids = q_ids
+        self.meta = meta
+
+    @property
+    def n_db(self):
+        return len(self.db_offsets) - 1
+
+    @property
+    def n_q(self):
+        return len(self.q_offsets) - 1
+
+    def subset_queries(self, idx):
+        qs = [self.q_letters[self.q_offsets[i]:self.q_offsets[i + 1]] for i in idx]
+        w = Workload.__new__(Workload)
+        w.name = self.name + f"[{len(idx)}q]"
+        w.db_letters, w.db_offsets = self.db_letters, self.db_offsets
+        w.q_letters, w.q_offsets = _pack(qs)
+        w.db_ids = self.db_ids
+        w.q_ids = [self.q_ids[i] for i in idx]
+        w.meta = dict(self.meta, queries=len(idx))
+        return w
+
+
+def protein_db_2p26(seed=20240601, n_queries=1024):
+    """The 7,860-protein (~2.26 Maa) DB shared by the --default (1x497 aa)
+    and --sensitive (1024 queries) workloads, with the planted homologs of
+    both query sets."""
+    rng = np.random.default_rng(seed)
+    n_db = 7860
+    q1 = _residues(rng, 497)
+    qlens = _lengths(rng, n_queries, 300, 0.57, 50, 1000)
+    qs = [_residues(rng, int(L)) for L in qlens]
+    planted = []  # (query set, query index, homolog)
+    for _ in range(7):
+        planted.append((1, 0, mutate(rng, q1, rng.uniform(0.30, 0.60), 0.02)))
+    for qi in range(n_queries):
+        for _ in range(int(rng.integers(0, 11))):
+            planted.append((2, qi, mutate(rng, qs[qi], rng.uniform(0.25, 0.60), 0.02)))
+    n_bg = max(0, n_db - len(planted))
+    # Background lengths are scaled so the DB totals ~2,264,000 letters.
+    planted_letters = sum(len(p[2]) for p in planted)
+    bg_median = max(30.0, (2_264_000 - planted_letters) / max(n_bg, 1) / np.exp(0.57 ** 2 / 2))
+    bg_lens = _lengths(rng, n_bg, bg_median, 0.57, 30, 5000)
+    db = [_residues(rng, int(L)) for L in bg_lens] + [p[2] for p in planted]
+    tags = ["bg"] * n_bg + [f"h{p[0]}q{p[1]}" for p in planted]
+    perm = rng.permutation(len(db))
+    db = [db[i] for i in perm]
+    ids = [f"s{k:06d}_{tags[i]}" for k, i in enumerate(perm)]
+    return db, ids, q1, qs
+
+
+def make(name: str, seed: int = 20240601) -> Workload:
+    """Named workloads:
+    'blastp-default-1x497'   : 1 query of 497 aa vs the 2.26 Maa DB
+    'blastp-sensitive-1024'  : 1024 queries (median 300, 50..1000) vs the same DB
+    'tiny'                   : 24 queries vs 400 proteins (fast parity cases)
+    """
+    if name in ("blastp-default-1x497", "blastp-sensitive-1024"):
+        db, ids, q1, qs = protein_db_2p26(seed)
+        if name == "blastp-default-1x497":
+            return Workload(name, db, ids, [q1], ["query1"],
+                            dict(sensitivity="default", seed=seed, queries=1))
+        return Workload(name, db, ids, qs, [f"q{i:04d}" for i in range(len(qs))],
+                        dict(sensitivity="sensitive", seed=seed, queries=len(qs)))
+    if name.startswith("tiny"):
+        rng = np.random.default_rng(seed)
+        n_q = 24
+        qlens = _lengths(rng, n_q, 200, 0.6, 20, 900)
+        qs = [_residues(rng, int(L)) for L in qlens]
+        planted = []
+        for qi in range(n_q):
+            for _ in range(int(rng.integers(0, 4))):
+                planted.append(mutate(rng, qs[qi], rng.uniform(0.25, 0.8), 0.03))
+        bg = [_residues(rng, int(L)) for L in _lengths(rng, 400 - len(planted), 200, 0.6, 10, 2000)]
+        db = bg + planted
+        perm = rng.permutation(len(db))
+        db = [db[i] for i in perm]
+        return Workload(name, db, [f"t{k:04d}" for k in range(len(db))], qs,
+                        [f"tq{i:02d}" for i in range(n_q)], dict(seed=seed, queries=n_q))
+    raise ValueError(f"unknown workload {name}")

@@ -1,0 +1,206 @@
+/*
+ * dmr.h — C ABI of the MI355X (gfx950) search core.
+ *
+ * Each entry point replaces one call of diamond-rust's search core; the
+ * comment above it names the reference function (file:line in the
+ * diamond-rust repository) whose arguments and results it mirrors. All
+ * buffers are host memory owned by the caller; sizes are element counts.
+ * Functions return DMR_OK or a negative status; dmr_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Residues are NCBI-order letter codes 0..24
+ * (A R N D C Q E G H I L K M F P S T W Y V B J Z X *), as produced by
+ * alphabet::encode (src/alphabet.rs:54-63). Sequence sets are passed as one
+ * concatenated buffer plus nseq+1 offsets.
+ */
+#ifndef DMR_H
+#define DMR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMR_OK 0
+#define DMR_E_ARG (-1)         /* invalid argument (bad shape, gap costs, sizes) */
+#define DMR_E_DEVICE (-2)      /* HIP runtime failure */
+#define DMR_E_OOM (-3)         /* device allocation failed */
+#define DMR_E_CAPACITY (-4)    /* output buffer too small; *n_out holds the size needed */
+#define DMR_E_UNSUPPORTED (-5) /* input outside what this build handles */
+
+typedef struct dmr_ctx dmr_ctx;
+typedef struct dmr_result dmr_result;
+
+/* seed::SeedEntry (src/seed.rs:14-19) */
+typedef struct {
+  uint64_t key;
+  uint32_t seq;
+  uint32_t pos;
+} dmr_seed_entry;
+
+/* seed::SeedHit (src/seed.rs:21-27) */
+typedef struct {
+  uint32_t query, qpos, subject, spos;
+} dmr_seed_hit;
+
+/* ungapped::Segment (src/ungapped.rs:10-16) */
+typedef struct {
+  uint64_t q_start, s_start, len;
+  int32_t score;
+  int32_t reserved;
+} dmr_segment;
+
+/* hsp::HspStats (src/hsp.rs:29-42) */
+typedef struct {
+  uint32_t identities, positives, mismatches, gap_openings, gaps, length;
+} dmr_hsp_stats;
+
+/* gapped::Aligned (src/gapped.rs:34-43); found = 0 stands for None.
+ * Alignment ops are returned separately as uint32 (op | count << 2) with
+ * op 0 = Match, 1 = Ins (gap in subject), 2 = Del (gap in query). */
+typedef struct {
+  int32_t found;
+  int32_t score;
+  uint64_t q_begin, q_end, s_begin, s_end;
+  dmr_hsp_stats stats;
+  uint32_t n_ops;
+  uint32_t reserved;
+} dmr_aligned;
+
+/* One extend_span call for dmr_extend_span_batch: query and subject slices
+ * of the caller's buffers plus the diagonal range [d0, d1]. */
+typedef struct {
+  uint64_t q_off, q_len, s_off, s_len;
+  int64_t d0, d1;
+} dmr_span_job;
+
+/* search::SearchConfig (src/search.rs:19-33) and the blastp/blastx options
+ * the search reads (src/cli.rs:125-269). NULL sensitivity = "default";
+ * index_chunks < 0 and seed_cut < 0 keep the preset; gap costs -1 keep the
+ * matrix default (BLOSUM62 11/1). */
+typedef struct {
+  const char* sensitivity;
+  double evalue;
+  uint32_t max_target_seqs;
+  uint32_t max_hsps;
+  int32_t index_chunks;
+  int32_t gap_open;
+  int32_t gap_extend;
+  int32_t min_orf; /* blastx --min-orf; < 0 = length-dependent default */
+  double seed_cut;
+} dmr_search_config;
+
+/* Fills the reference defaults: default sensitivity, evalue 1e-3, k 25,
+ * max-hsps 1 (src/search.rs:36-48, src/cli.rs:125-140). */
+void dmr_search_config_default(dmr_search_config* cfg);
+
+const char* dmr_last_error(void);
+const char* dmr_version(void);
+
+/* Context: one HIP device, one stream, the resident database and seed index. */
+int dmr_ctx_create(int device, dmr_ctx** out);
+void dmr_ctx_destroy(dmr_ctx* ctx);
+int dmr_device_count(int* n);
+
+/* gapped::band_for (src/gapped.rs:27-32) */
+uint32_t dmr_band_for(uint64_t len);
+
+/* ScoreMatrix::raw_score / bit_score / evalue for BLOSUM62 with the given
+ * gap costs (src/matrix.rs:182-200, src/alp.rs:108-122). */
+int dmr_evalue(int32_t gap_open, int32_t gap_extend, int32_t raw, uint64_t query_len,
+               uint64_t subject_len, uint64_t db_letters, double* evalue, double* bits);
+
+/* seed::build (src/seed.rs:76-104): every seed of seqs (shape code of 0/1,
+ * reduction as DIAMOND group strings, e.g. "A KR EDNQ C G H ILVM FYW P ST"),
+ * sequence numbers offset by base, sorted by (key, seq, pos). */
+int dmr_seed_build(dmr_ctx* ctx, const uint8_t* seqs, const uint64_t* offsets, uint32_t nseq,
+                   const char* shape_code, const char* reduction_def, uint32_t base,
+                   double complexity_cut, dmr_seed_entry* out, uint64_t cap, uint64_t* n_out);
+
+/* seed::join_into (src/seed.rs:110-143) for key-sorted arrays; hits are
+ * returned sorted by (query, subject, qpos - spos, spos), the order
+ * align_target consumes them (src/search.rs:167-178). */
+int dmr_seed_join(dmr_ctx* ctx, const dmr_seed_entry* query, uint64_t nq,
+                  const dmr_seed_entry* reference, uint64_t nr, uint64_t max_expand,
+                  dmr_seed_hit* out, uint64_t cap, uint64_t* n_out);
+
+/* ungapped::xdrop (src/ungapped.rs:38-88). */
+int dmr_xdrop(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+              uint64_t slen, uint64_t qa, uint64_t sa, int32_t xdrop, dmr_segment* out);
+/* The same for n seeds of one sequence pair. */
+int dmr_xdrop_batch(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+                    uint64_t slen, const uint32_t* qa, const uint32_t* sa, uint64_t n,
+                    int32_t xdrop, dmr_segment* out);
+
+/* gapped::extend_span (src/gapped.rs:67-280). ops receives up to ops_cap
+ * run-length ops; out->n_ops is the number produced (DMR_E_CAPACITY when
+ * larger than ops_cap). */
+int dmr_extend_span(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+                    uint64_t slen, int64_t d0, int64_t d1, int32_t gap_open, int32_t gap_extend,
+                    dmr_aligned* out, uint32_t* ops, uint32_t ops_cap);
+
+/* gapped::extend (src/gapped.rs:50-60): band around the anchor diagonal. */
+int dmr_extend(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+               uint64_t slen, uint64_t anchor_q, uint64_t anchor_s, uint64_t band,
+               int32_t gap_open, int32_t gap_extend, dmr_aligned* out, uint32_t* ops,
+               uint32_t ops_cap);
+
+/* Many extend_span calls in one launch. With traceback = 0 only score is
+ * filled (found = score > 0). ops_offsets (njobs + 1) index ops_pool;
+ * *ops_needed receives the pool size required. */
+int dmr_extend_span_batch(dmr_ctx* ctx, const uint8_t* qbuf, uint64_t qbuf_len,
+                          const uint8_t* sbuf, uint64_t sbuf_len, const dmr_span_job* jobs,
+                          uint64_t njobs, int32_t gap_open, int32_t gap_extend, int traceback,
+                          dmr_aligned* out, uint32_t* ops_pool, uint64_t ops_cap,
+                          uint64_t* ops_offsets, uint64_t* ops_needed);
+
+/* Database::open + build (src/db.rs:159-291) as a resident device copy:
+ * encoded letters and nseq + 1 offsets. Replaces any previous database. */
+int dmr_db_load(dmr_ctx* ctx, const uint8_t* letters, const uint64_t* offsets, uint32_t nseq);
+
+/* Builds (or keeps) the resident seed index for a sensitivity preset:
+ * seed::chunks(n, index_chunks) slices x all shapes (src/search.rs:121-133). */
+int dmr_db_build_index(dmr_ctx* ctx, const char* sensitivity, int32_t index_chunks,
+                       double seed_cut, double* build_ms);
+
+/* search::run + merge_frames (src/search.rs:95-146, 357-396) for blastp:
+ * queries are encoded letters. Results stay on the handle. */
+int dmr_blastp(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq,
+               const dmr_search_config* cfg, dmr_result** out);
+
+/* blastx: queries are nucleotide ASCII; six frames with --min-orf masking
+ * (src/translate.rs:54-137, src/workflow.rs:112-127). */
+int dmr_blastx(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint32_t nreads,
+               const dmr_search_config* cfg, dmr_result** out);
+
+/* Number of HSPs (all queries, output order). */
+int64_t dmr_result_count(const dmr_result* r);
+/* 16 int64 per HSP: query, subject, frame, q_begin, q_end, s_begin, s_end,
+ * score, identities, positives, mismatches, gap_openings, gaps, length,
+ * n_ops, 0; and 2 doubles: bitscore, evalue. Coordinates are 0-based
+ * half-open in the frame's amino-acid space. */
+int dmr_result_hsps(const dmr_result* r, int64_t* out16, double* out2);
+/* Alignment ops of HSP i (op | count << 2). */
+int dmr_result_ops(const dmr_result* r, int64_t i, uint32_t* ops, uint32_t cap, uint32_t* n);
+/* outfmt 6 text (Tabular::write, src/output.rs:150-223). query_ids and
+ * subject_ids are newline-separated; fields is a space-separated list of
+ * outfmt-6 field names (NULL or "" = the 12 defaults). *len receives the
+ * byte length; the text is written when buf holds cap >= *len bytes. */
+int dmr_result_format(const dmr_result* r, const char* query_ids, const char* subject_ids,
+                      const char* fields, char* buf, uint64_t cap, uint64_t* len);
+/* Stage counters and GPU times: query_seeds, runs, hits, hit_runs,
+ * segments, groups, jobs, maybe, selected, traceback_cells, ms_seed,
+ * ms_join, ms_sort_hits, ms_ungapped, ms_bands, ms_sw, ms_select,
+ * ms_traceback, ms_total, ms_host (20 doubles). */
+int dmr_result_stats(const dmr_result* r, double* out20);
+/* Cells of every banded DP of the last search on ctx (for GCUPS). */
+int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
+void dmr_result_destroy(dmr_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMR_H */

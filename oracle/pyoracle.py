@@ -1,0 +1,268 @@
+"""ctypes binding of the oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the CPU
+baseline of bench.py as the checker, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", str(HERE), "-j4"], check=True, capture_output=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        P, U64, I64, U32, D = C.c_void_p, C.c_uint64, C.c_int64, C.c_uint32, C.c_double
+        sig = {
+            "orc_encode": ([C.c_char_p, P, C.c_int], C.c_int),
+            "orc_shape_key": ([C.c_char_p, P, C.c_int, C.c_int, C.c_char_p, P], C.c_int),
+            "orc_seed_is_complex": ([P, C.c_char_p, C.c_char_p, D], C.c_int),
+            "orc_shapes_for": ([C.c_char_p, C.c_int, P, C.c_int], C.c_int),
+            "orc_reduction_map": ([C.c_char_p, P], C.c_int),
+            "orc_seed_cut": ([C.c_char_p, D], D),
+            "orc_index_chunks": ([C.c_char_p, C.c_long], C.c_long),
+            "orc_seed_build": ([P, P, U32, C.c_char_p, C.c_char_p, U32, D, P, P, P, I64], I64),
+            "orc_seed_join": ([P, P, P, I64, P, P, P, I64, U32, I64, P, I64], I64),
+            "orc_chunks": ([I64, I64, P], C.c_int),
+            "orc_matrix_score": ([C.c_int, C.c_int], C.c_int),
+            "orc_matrix_ok": ([C.c_int, C.c_int], C.c_int),
+            "orc_xdrop": ([P, I64, P, I64, I64, I64, C.c_int, P], None),
+            "orc_extend_span": ([P, I64, P, I64, I64, I64, C.c_int, C.c_int, P, P, C.c_int, P], C.c_int),
+            "orc_band_for": ([I64], C.c_int),
+            "orc_evalue": ([C.c_int, I64, I64, U64], D),
+            "orc_bit_score": ([C.c_int], D),
+            "orc_raw_score": ([D], C.c_int),
+            "orc_log_erfc": ([D], D),
+            "orc_fmt_double": ([D, P, C.c_int], C.c_int),
+            "orc_fmt_evalue": ([D, P, C.c_int], C.c_int),
+            "orc_six_frames": ([C.c_char_p, P, I64, P, C.c_long], C.c_int),
+            "orc_search": ([P, P, U32, P, P, U32, C.c_char_p, C.c_int, D, C.c_int, C.c_int,
+                            C.c_long, D, C.c_int, C.c_long], P),
+            "orc_result_count": ([P], I64),
+            "orc_result_hsps": ([P, P, P], None),
+            "orc_result_counters": ([P, P], None),
+            "orc_result_outfmt6": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, I64], I64),
+            "orc_result_free": ([P], None),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def encode(s: str) -> np.ndarray:
+    out = np.zeros(len(s) + 1, dtype=np.uint8)
+    n = lib().orc_encode(s.encode(), out.ctypes.data, len(out))
+    return out[:n].copy()
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and len(a) else None
+
+
+def shape_key(code, seq, pos, reduction):
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    k = C.c_uint64()
+    ok = lib().orc_shape_key(code.encode(), _p(seq), len(seq), pos, reduction.encode(), C.byref(k))
+    return k.value if ok else None
+
+
+def seed_is_complex(seq, code, reduction, cut):
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    return bool(lib().orc_seed_is_complex(_p(seq), code.encode(), reduction.encode(), cut))
+
+
+def shapes_for(sens, limit=0):
+    buf = C.create_string_buffer(8192)
+    lib().orc_shapes_for(sens.encode(), limit, buf, len(buf))
+    return [x for x in buf.value.decode().split("\n") if x]
+
+
+def seed_build(seqs, code, reduction, base=0, cut=0.0):
+    seqs = [np.asarray(s, dtype=np.uint8) for s in seqs]
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    np.cumsum([len(s) for s in seqs], out=off[1:])
+    letters = np.concatenate(seqs) if seqs else np.zeros(0, np.uint8)
+    cap = max(1, len(letters))
+    k = np.zeros(cap, np.uint64)
+    s = np.zeros(cap, np.uint32)
+    p = np.zeros(cap, np.uint32)
+    n = lib().orc_seed_build(_p(letters), off.ctypes.data, len(seqs), code.encode(),
+                             reduction.encode(), base, cut, k.ctypes.data, s.ctypes.data,
+                             p.ctypes.data, cap)
+    return k[:n].copy(), s[:n].copy(), p[:n].copy()
+
+
+def seed_join(q, r, ntargets, max_expand=4096):
+    qk, qs, qp = (np.ascontiguousarray(x) for x in q)
+    rk, rs, rp = (np.ascontiguousarray(x) for x in r)
+    qk, rk = qk.astype(np.uint64), rk.astype(np.uint64)
+    qs, qp, rs, rp = (x.astype(np.uint32) for x in (qs, qp, rs, rp))
+    n = lib().orc_seed_join(_p(qk), _p(qs), _p(qp), len(qk), _p(rk), _p(rs), _p(rp), len(rk),
+                            ntargets, max_expand, None, 0)
+    out = np.zeros(4 * max(n, 1), np.uint32)
+    lib().orc_seed_join(_p(qk), _p(qs), _p(qp), len(qk), _p(rk), _p(rs), _p(rp), len(rk),
+                        ntargets, max_expand, out.ctypes.data, n)
+    return out[: 4 * n].reshape(n, 4).astype(np.int64)
+
+
+def chunks(n, parts):
+    out = np.zeros(2 * max(parts, 1) + 2, np.int64)
+    k = lib().orc_chunks(n, parts, out.ctypes.data)
+    return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(k)]
+
+
+def xdrop(q, s, qa, sa, xd):
+    q = np.ascontiguousarray(q, dtype=np.uint8)
+    s = np.ascontiguousarray(s, dtype=np.uint8)
+    o = np.zeros(4, np.int64)
+    lib().orc_xdrop(_p(q), len(q), _p(s), len(s), qa, sa, xd, o.ctypes.data)
+    return tuple(int(x) for x in o)
+
+
+def extend_span(q, s, d0, d1, go=11, ge=1):
+    """None or a dict like the product's extend_span (plus 'cells')."""
+    q = np.ascontiguousarray(q, dtype=np.uint8)
+    s = np.ascontiguousarray(s, dtype=np.uint8)
+    o = np.zeros(12, np.int64)
+    ops = np.zeros(len(q) + len(s) + 2, np.uint32)
+    nops = C.c_int()
+    ok = lib().orc_extend_span(_p(q), len(q), _p(s), len(s), d0, d1, go, ge, o.ctypes.data,
+                               ops.ctypes.data, len(ops), C.byref(nops))
+    if ok <= 0:
+        return None
+    keys = ["score", "q_begin", "q_end", "s_begin", "s_end", "identities", "positives",
+            "mismatches", "gap_openings", "gaps", "length", "cells"]
+    d = {k: int(v) for k, v in zip(keys, o)}
+    d["ops"] = [(int(x) & 3, int(x) >> 2) for x in ops[: nops.value]]
+    return d
+
+
+def dp_cells(q, s, d0, d1):
+    q = np.ascontiguousarray(q, dtype=np.uint8)
+    s = np.ascontiguousarray(s, dtype=np.uint8)
+    o = np.zeros(12, np.int64)
+    ops = np.zeros(len(q) + len(s) + 2, np.uint32)
+    nops = C.c_int()
+    lib().orc_extend_span(_p(q), len(q), _p(s), len(s), d0, d1, 11, 1, o.ctypes.data,
+                          ops.ctypes.data, len(ops), C.byref(nops))
+    return int(o[11])
+
+
+def band_for(n):
+    return lib().orc_band_for(n)
+
+
+def evalue(raw, qlen, slen, db):
+    return lib().orc_evalue(raw, qlen, slen, db)
+
+
+def bit_score(raw):
+    return lib().orc_bit_score(raw)
+
+
+def raw_score(bits):
+    return lib().orc_raw_score(bits)
+
+
+def log_erfc(x):
+    return lib().orc_log_erfc(x)
+
+
+def fmt_double(x):
+    b = C.create_string_buffer(64)
+    lib().orc_fmt_double(x, b, 64)
+    return b.value.decode()
+
+
+def fmt_evalue(x):
+    b = C.create_string_buffer(64)
+    lib().orc_fmt_evalue(x, b, 64)
+    return b.value.decode()
+
+
+def six_frames(nt, min_orf=-1):
+    cap = len(nt) + 8
+    out = np.zeros(cap, np.uint8)
+    lens = np.zeros(6, np.int64)
+    lib().orc_six_frames(nt.encode(), out.ctypes.data, cap, lens.ctypes.data, min_orf)
+    frames, o = [], 0
+    for L in lens:
+        frames.append(out[o:o + L].copy())
+        o += L
+    return frames
+
+
+COUNTER_FIELDS = ["query_seeds", "ref_seeds", "hits", "xdrops", "segments", "bands", "dps",
+                  "dp_cells", "clipped_dps", "band_tie_events", "anchor_tie_events", "t_seed",
+                  "t_join", "t_align"]
+
+
+class SearchResult:
+    def __init__(self, h):
+        self._h = h
+
+    def __del__(self):
+        if self._h:
+            lib().orc_result_free(self._h)
+            self._h = None
+
+    def __len__(self):
+        return int(lib().orc_result_count(self._h))
+
+    def hsps(self):
+        n = len(self)
+        ints = np.zeros((n, 16), np.int64)
+        d = np.zeros((n, 2), np.float64)
+        if n:
+            lib().orc_result_hsps(self._h, ints.ctypes.data, d.ctypes.data)
+        names = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
+                 "identities", "positives", "mismatches", "gap_openings", "gaps", "length", "n_ops"]
+        out = {k: ints[:, i] for i, k in enumerate(names)}
+        out["bits"] = d[:, 0]
+        out["evalue"] = d[:, 1]
+        return out
+
+    def counters(self):
+        v = np.zeros(14, np.float64)
+        lib().orc_result_counters(self._h, v.ctypes.data)
+        return dict(zip(COUNTER_FIELDS, v.tolist()))
+
+    def format(self, qids, sids, fields=""):
+        q = "\n".join(qids).encode()
+        s = "\n".join(sids).encode()
+        f = fields.encode() if isinstance(fields, str) else " ".join(fields).encode()
+        n = lib().orc_result_outfmt6(self._h, q, s, f, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        lib().orc_result_outfmt6(self._h, q, s, f, buf, n + 1)
+        return buf.raw[:n].decode()
+
+
+def search(db_letters, db_offsets, q_letters, q_offsets, sensitivity="default", threads=1,
+           evalue=1e-3, max_target_seqs=25, max_hsps=1, index_chunks=-1, seed_cut=-1.0,
+           translated=False, min_orf=-1):
+    dbl = np.ascontiguousarray(db_letters, dtype=np.uint8)
+    dbo = np.ascontiguousarray(db_offsets, dtype=np.uint64)
+    ql = np.ascontiguousarray(q_letters, dtype=np.uint8)
+    qo = np.ascontiguousarray(q_offsets, dtype=np.uint64)
+    h = lib().orc_search(_p(dbl), dbo.ctypes.data, len(dbo) - 1, _p(ql), qo.ctypes.data,
+                         len(qo) - 1, sensitivity.encode(), threads, evalue, max_target_seqs,
+                         max_hsps, index_chunks, seed_cut, 1 if translated else 0, min_orf)
+    return SearchResult(h)

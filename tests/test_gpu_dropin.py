@@ -1,0 +1,138 @@
+"""GPU parity of the drop-in calls (through the C ABI) against the oracle.
+
+Bit-exact for every field: seeds (key, seq, pos), hits, segments and the
+full gapped::Aligned (score, coordinates, statistics, run-length ops).
+"""
+import numpy as np
+import pytest
+
+from conftest import mutate, rand_seq
+
+pytestmark = pytest.mark.gpu
+
+MURPHY10 = "A KR EDNQ C G H ILVM FYW P ST"
+DIAMOND9 = "A G P CH ST NDE RQK FWY ILMV"
+
+
+def _eq_aligned(a, b):
+    if a is None or b is None:
+        return a is None and b is None
+    keys = ["score", "q_begin", "q_end", "s_begin", "s_end", "identities", "positives",
+            "mismatches", "gap_openings", "gaps", "length", "ops"]
+    return all(a[k] == b[k] for k in keys)
+
+
+def span_cases(seed, n_cases):
+    rng = np.random.default_rng(seed)
+    for _ in range(n_cases):
+        n = int(rng.integers(1, 400))
+        q = rand_seq(rng, n, canonical_only=rng.random() < 0.8)
+        kind = rng.integers(0, 4)
+        if kind == 0:
+            s = q.copy()
+        elif kind == 1:
+            s = mutate(rng, q, float(rng.uniform(0.1, 0.7)), float(rng.uniform(0.0, 0.08)))
+        else:
+            s = rand_seq(rng, int(rng.integers(1, 500)))
+        if len(s) == 0:
+            s = rand_seq(rng, 1)
+        m = len(s)
+        # Bands inside, across and entirely outside the matrix (clamped rows).
+        c = int(rng.integers(-m - 20, n + 20))
+        w = int(rng.choice([0, 1, 5, 12, 30, 64, 200, 2000]))
+        yield q, s, c - w, c + w + int(rng.integers(0, 3))
+
+
+def test_extend_span_single(ctx, orc):
+    for q, s, d0, d1 in span_cases(1, 60):
+        assert _eq_aligned(ctx.extend_span(q, s, d0, d1), orc.extend_span(q, s, d0, d1)), (len(q), len(s), d0, d1)
+
+
+def test_extend_span_reference_cases(ctx, orc):  # src/gapped.rs:299-329
+    q = orc.encode("MKWVTFISLLFLFSSAYSR")
+    a = ctx.extend(q, q, (0, 0), 30)
+    assert a["score"] == sum(orc.lib().orc_matrix_score(int(c), int(c)) for c in q)
+    assert (a["q_begin"], a["q_end"], a["identities"]) == (0, len(q), len(q))
+    s = orc.encode("MKWVTFISLLLFSSAYSR")
+    a = ctx.extend(q, s, (0, 0), 30)
+    assert a["gaps"] > 0 and a["identities"] == len(s)
+    assert ctx.extend(orc.encode("WWWWWWWWWW"), orc.encode("AAAAAAAAAA"), (0, 0), 15) is None
+
+
+@pytest.mark.parametrize("traceback", [True, False])
+def test_extend_span_batch(ctx, orc, traceback):
+    cases = list(span_cases(7, 400))
+    qbuf = np.concatenate([c[0] for c in cases])
+    sbuf = np.concatenate([c[1] for c in cases])
+    jobs, qo, so = [], 0, 0
+    for q, s, d0, d1 in cases:
+        jobs.append((qo, len(q), so, len(s), d0, d1))
+        qo += len(q)
+        so += len(s)
+    got = ctx.extend_span_batch(qbuf, sbuf, jobs, traceback=traceback)
+    for (q, s, d0, d1), g in zip(cases, got):
+        ref = orc.extend_span(q, s, d0, d1)
+        if traceback:
+            assert _eq_aligned(g, ref)
+        else:
+            assert g == (ref["score"] if ref else None)
+
+
+def test_xdrop_batch(ctx, orc):
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        q = rand_seq(rng, int(rng.integers(1, 300)), canonical_only=False)
+        s = mutate(rng, q, 0.3, 0.0) if rng.random() < 0.6 else rand_seq(rng, int(rng.integers(1, 300)))
+        if len(s) == 0:
+            continue
+        qa = rng.integers(0, len(q) + 1, size=50).astype(np.uint32)
+        sa = rng.integers(0, len(s) + 1, size=50).astype(np.uint32)
+        xd = int(rng.integers(1, 40))
+        got = ctx.xdrop_batch(q, s, qa, sa, xd)
+        for k in range(50):
+            assert tuple(int(x) for x in got[k]) == orc.xdrop(q, s, int(qa[k]), int(sa[k]), xd)
+    q = orc.encode("WWWWWWWWWWCCCCCCCCCC")  # src/ungapped.rs:328-337
+    assert ctx.xdrop(q, orc.encode("WWWWWWWWWWDDDDDDDDDD"), 0, 0, 5)[2:] == (10, 110)
+
+
+@pytest.mark.parametrize("code,red,cut", [
+    ("111101110111", MURPHY10, 0.8 * np.log(2) * 10),
+    ("1011110111", MURPHY10, 1.0 * np.log(2) * 8),
+    ("11110111", DIAMOND9, 1.0 * np.log(2) * 7),
+    ("1101", MURPHY10, 0.0),
+    ("1111111111111111", MURPHY10, 0.8 * np.log(2) * 16),
+])
+def test_seed_build(ctx, orc, code, red, cut):
+    rng = np.random.default_rng(5)
+    seqs = [rand_seq(rng, int(rng.integers(0, 300)), canonical_only=rng.random() < 0.7) for _ in range(40)]
+    seqs.append(np.zeros(40, np.uint8))  # homopolymer: low complexity
+    got = ctx.seed_build(seqs, code, red, base=17, cut=cut)
+    exp = orc.seed_build(seqs, code, red, base=17, cut=cut)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g.astype(np.int64), e.astype(np.int64))
+
+
+def test_seed_build_reference_case(ctx, orc):  # src/seed.rs:175-196
+    a = orc.encode("MKWVTFGCPD")
+    k, s, p = ctx.seed_build([a], "1111", MURPHY10, 0, 0.8 * np.log(2) * 4)
+    assert len(k) == 7
+
+
+@pytest.mark.parametrize("cap", [4096, 12, 1])
+def test_seed_join(ctx, orc, cap):
+    rng = np.random.default_rng(9)
+    qs = [rand_seq(rng, int(rng.integers(10, 200))) for _ in range(12)]
+    rs = [mutate(rng, qs[i % 12], 0.25, 0.02) for i in range(30)] + [rand_seq(rng, 150) for _ in range(20)]
+    rs += [np.zeros(60, np.uint8)]  # repeats push some keys over the cap
+    q = orc.seed_build(qs, "110101", MURPHY10, 0, 0.0)
+    r = orc.seed_build(rs, "110101", MURPHY10, 100, 0.0)
+    got = ctx.seed_join(q, r, cap)
+    exp = orc.seed_join(q, r, 12, cap)
+    assert np.array_equal(got, exp)
+
+
+def test_seed_join_cap_boundary(ctx):
+    z = np.zeros
+    q = (z(64, np.uint64), np.arange(64) % 3, np.arange(64))
+    assert len(ctx.seed_join(q, (z(64, np.uint64), np.arange(64), np.arange(64)), 4096)) == 4096
+    assert len(ctx.seed_join(q, (z(65, np.uint64), np.arange(65), np.arange(65)), 4096)) == 0

@@ -1,0 +1,66 @@
+"""GPU blastp against the oracle: byte-identical outfmt-6 text (default
+fields and the extended ones: raw score, CIGAR, BTOP, aligned strings).
+
+The max_expand cap counts query seeds over the whole batch
+(src/search.rs:109-131), so both sides search the same query set in one call.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EXTENDED = ("qseqid sseqid pident length mismatch gapopen qstart qend sstart send evalue "
+            "bitscore score nident positive gaps ppos qlen slen qcovhsp scovhsp cigar btop "
+            "qseq sseq qframe qstrand")
+
+
+def run_both(dmr, ctx, orc, w, sens, **kw):
+    ctx.db_load(w["db_letters"], w["db_offsets"])
+    cfg = dmr.SearchConfig(sensitivity=sens, **kw)
+    res = ctx.blastp(w["q_letters"], w["q_offsets"], cfg)
+    ref = orc.search(w["db_letters"], w["db_offsets"], w["q_letters"], w["q_offsets"],
+                     sensitivity=sens, threads=8, **{k: v for k, v in kw.items()
+                                                     if k in ("evalue", "max_target_seqs", "max_hsps")})
+    return res, ref
+
+
+def workload_dict(w):
+    return dict(db_letters=w.db_letters, db_offsets=w.db_offsets, q_letters=w.q_letters,
+                q_offsets=w.q_offsets, db_ids=w.db_ids, q_ids=w.q_ids)
+
+
+@pytest.mark.parametrize("sens", ["default", "sensitive", "very-sensitive", "fast"])
+def test_tiny_workload(dmr, ctx, orc, synth, sens):
+    w = workload_dict(synth.make("tiny"))
+    res, ref = run_both(dmr, ctx, orc, w, sens)
+    got = res.format(w["q_ids"], w["db_ids"], EXTENDED)
+    exp = ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+    assert len(ref) > 0
+    assert got == exp
+
+
+@pytest.mark.parametrize("kw", [dict(max_hsps=3), dict(max_target_seqs=2), dict(evalue=10.0)])
+def test_tiny_options(dmr, ctx, orc, synth, kw):
+    w = workload_dict(synth.make("tiny"))
+    res, ref = run_both(dmr, ctx, orc, w, "sensitive", **kw)
+    assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+
+
+@pytest.mark.parametrize("sens", ["default", "fast", "sensitive"])
+def test_reference_e2e_data(dmr, ctx, orc, e2e, sens):
+    res, ref = run_both(dmr, ctx, orc, e2e, sens)
+    got = res.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
+    assert got == ref.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
+    assert len(res) == {"default": 4, "fast": 1, "sensitive": 7}[sens]
+
+
+def test_sensitive_batch_subset(dmr, ctx, orc, synth):
+    """48 queries of the 1024-query --sensitive workload against the 2.26 Maa DB."""
+    w = synth.make("blastp-sensitive-1024").subset_queries(list(range(0, 1024, 21)))
+    w = workload_dict(w)
+    res, ref = run_both(dmr, ctx, orc, w, "sensitive")
+    assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+    st = res.stats()
+    c = ref.counters()
+    assert st["hits"] == c["hits"]
+    assert st["jobs"] == c["bands"]
