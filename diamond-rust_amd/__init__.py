@@ -32,7 +32,7 @@ EXPORTS = [
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend",
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
-    "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
+    "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy",
 ]
 
@@ -136,6 +136,9 @@ def lib():
             "dmr_db_build_index": ([P, C.c_char_p, I32, D, P], C.c_int),
             "dmr_blastp": ([P, P, P, U32, P, P], C.c_int),
             "dmr_blastx": ([P, P, P, U32, P, P], C.c_int),
+            "dmr_queries_load": ([P, P, P, U32], C.c_int),
+            "dmr_search_loaded": ([P, P, P], C.c_int),
+            "dmr_blastp_plain": ([P, P, P, U32, P, P], C.c_int),
             "dmr_result_count": ([P], I64), "dmr_result_hsps": ([P, P, P], C.c_int),
             "dmr_result_ops": ([P, I64, P, U32, P], C.c_int),
             "dmr_result_format": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, U64, P], C.c_int),
@@ -268,6 +271,30 @@ class Context:
         h = C.c_void_p()
         _check(lib().dmr_blastp(self._h, _ptr(letters), offsets.ctypes.data, len(offsets) - 1,
                                 C.byref(c), C.byref(h)))
+        return Result(h.value)
+
+    def queries_load(self, letters, offsets):
+        """Uploads a blastp query batch that stays resident for search_loaded."""
+        letters = np.ascontiguousarray(letters, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        _check(lib().dmr_queries_load(self._h, _ptr(letters), offsets.ctypes.data, len(offsets) - 1))
+
+    def search_loaded(self, cfg: SearchConfig | None = None) -> Result:
+        cfg = cfg or SearchConfig()
+        c = cfg._c()
+        h = C.c_void_p()
+        _check(lib().dmr_search_loaded(self._h, C.byref(c), C.byref(h)))
+        return Result(h.value)
+
+    def blastp_plain(self, letters, offsets, cfg: SearchConfig | None = None) -> Result:
+        """The per-call drop-in form (one kernel launch per reference call)."""
+        cfg = cfg or SearchConfig()
+        letters = np.ascontiguousarray(letters, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        c = cfg._c()
+        h = C.c_void_p()
+        _check(lib().dmr_blastp_plain(self._h, _ptr(letters), offsets.ctypes.data, len(offsets) - 1,
+                                      C.byref(c), C.byref(h)))
         return Result(h.value)
 
     def blastx(self, reads, cfg: SearchConfig | None = None) -> Result:

@@ -15,6 +15,9 @@
 
 struct dmr_ctx {
   std::unique_ptr<dmr::Engine> eng;
+  dmr::TargetSet loaded;  // resident query batch (dmr_queries_load)
+  uint32_t loaded_nq = 0;
+  bool has_loaded = false;
 };
 
 struct dmr_result {
@@ -23,9 +26,8 @@ struct dmr_result {
   dmr::TargetSet targets;
   std::vector<uint64_t> dna_len;     // per query record (blastx)
   bool translated = false;
-  std::vector<uint32_t> db_offs;     // host copy for subject slices
-  const std::vector<uint8_t>* db_res = nullptr;
-  std::vector<uint8_t> db_res_copy;
+  std::shared_ptr<const std::vector<uint32_t>> db_offs;  // host copy of the database
+  std::shared_ptr<const std::vector<uint8_t>> db_res;
   dmr::StageStats stats;
 };
 
@@ -153,24 +155,42 @@ dmr::SearchOptions options(const dmr_search_config* c) {
   return o;
 }
 
-void finish(dmr_ctx* ctx, dmr_result* r, const dmr::SearchOptions& o, size_t nqueries) {
-  ctx->eng->load_targets(r->targets);
-  auto per_target = ctx->eng->search_targets(o, &r->stats);
+void assemble(dmr_ctx* ctx, dmr_result* r, std::vector<std::vector<HspRec>>& per_target,
+              const dmr::SearchOptions& o, size_t nqueries) {
   auto per = merge_frames(per_target, r->targets, nqueries, o.max_hsps, o.max_target_seqs);
+  std::map<std::pair<uint32_t, uint32_t>, int> tix;
+  for (size_t t = 0; t < r->targets.query_of.size(); ++t)
+    tix[{r->targets.query_of[t], r->targets.frame_of[t]}] = (int)t;
   for (auto& hs : per)
     for (auto& h : hs) {
-      int ti = -1;
-      for (size_t t = 0; t < r->targets.query_of.size(); ++t)
-        if (r->targets.query_of[t] == h.query && r->targets.frame_of[t] == h.frame) {
-          ti = (int)t;
-          break;
-        }
-      r->target_of.push_back(ti);
+      auto it = tix.find({h.query, h.frame});
+      r->target_of.push_back(it == tix.end() ? -1 : it->second);
       r->hsps.push_back(std::move(h));
     }
-  r->db_offs = ctx->eng->db_offsets_host();
-  r->db_res = &r->db_res_copy;
-  r->db_res_copy = ctx->eng->db_letters_host();
+  r->db_offs = ctx->eng->db_offsets_shared();
+  r->db_res = ctx->eng->db_letters_shared();
+}
+
+void finish(dmr_ctx* ctx, dmr_result* r, const dmr::SearchOptions& o, size_t nqueries,
+            bool upload = true) {
+  if (upload) ctx->eng->load_targets(r->targets);
+  auto per_target = ctx->eng->search_targets(o, &r->stats);
+  assemble(ctx, r, per_target, o, nqueries);
+}
+
+dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq) {
+  dmr::TargetSet t;
+  t.offs.push_back(0);
+  for (uint32_t i = 0; i < nq; ++i) {
+    uint64_t a = q_offsets[i], b = q_offsets[i + 1];
+    if (b < a) throw dmr::Error(DMR_E_ARG, "query offsets are not monotone");
+    if (b == a) continue;  // empty queries have no target (src/workflow.rs:129-132)
+    t.letters.insert(t.letters.end(), queries + a, queries + b);
+    t.offs.push_back((uint32_t)t.letters.size());
+    t.query_of.push_back(i);
+    t.frame_of.push_back(0);
+  }
+  return t;
 }
 
 // src/output.rs:235-241
@@ -417,18 +437,55 @@ int dmr_blastp(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, 
     return fail(DMR_E_ARG, "Gap penalty settings are outside the supported range for BLOSUM62");
   return guard([&]() -> int {
     auto r = std::make_unique<dmr_result>();
-    r->targets.offs.push_back(0);
-    for (uint32_t i = 0; i < nq; ++i) {
-      uint64_t a = q_offsets[i], b = q_offsets[i + 1];
-      if (b < a) return fail(DMR_E_ARG, "query offsets are not monotone");
-      if (b == a) continue;  // empty queries have no target (src/workflow.rs:129-132)
-      r->targets.letters.insert(r->targets.letters.end(), queries + a, queries + b);
-      r->targets.offs.push_back((uint32_t)r->targets.letters.size());
-      r->targets.query_of.push_back(i);
-      r->targets.frame_of.push_back(0);
-    }
+    r->targets = protein_targets(queries, q_offsets, nq);
     r->dna_len.resize(nq, 0);
     finish(ctx, r.get(), o, nq);
+    *out = r.release();
+    return DMR_OK;
+  });
+}
+
+int dmr_queries_load(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq) {
+  if (!ctx || !q_offsets) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&]() -> int {
+    ctx->loaded = protein_targets(queries, q_offsets, nq);
+    ctx->loaded_nq = nq;
+    ctx->eng->load_targets(ctx->loaded);
+    ctx->has_loaded = true;
+    return DMR_OK;
+  });
+}
+
+int dmr_search_loaded(dmr_ctx* ctx, const dmr_search_config* cfg, dmr_result** out) {
+  if (!ctx || !out) return fail(DMR_E_ARG, "NULL argument");
+  if (!ctx->has_loaded) return fail(DMR_E_ARG, "no query batch loaded");
+  dmr::SearchOptions o = options(cfg);
+  dmr::GapRow gr;
+  if (!dmr::blosum62_gap_row(o.gap_open, o.gap_extend, &gr))
+    return fail(DMR_E_ARG, "Gap penalty settings are outside the supported range for BLOSUM62");
+  return guard([&]() -> int {
+    auto r = std::make_unique<dmr_result>();
+    r->targets = ctx->loaded;
+    r->dna_len.resize(ctx->loaded_nq, 0);
+    finish(ctx, r.get(), o, ctx->loaded_nq, false);
+    *out = r.release();
+    return DMR_OK;
+  });
+}
+
+int dmr_blastp_plain(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq,
+                     const dmr_search_config* cfg, dmr_result** out) {
+  if (!ctx || !q_offsets || !out) return fail(DMR_E_ARG, "NULL argument");
+  dmr::SearchOptions o = options(cfg);
+  dmr::GapRow gr;
+  if (!dmr::blosum62_gap_row(o.gap_open, o.gap_extend, &gr))
+    return fail(DMR_E_ARG, "Gap penalty settings are outside the supported range for BLOSUM62");
+  return guard([&]() -> int {
+    auto r = std::make_unique<dmr_result>();
+    r->targets = protein_targets(queries, q_offsets, nq);
+    r->dna_len.resize(nq, 0);
+    auto per_target = dmr::search_plain(*ctx->eng, r->targets, o, &r->stats);
+    assemble(ctx, r.get(), per_target, o, nq);
     *out = r.release();
     return DMR_OK;
   });
@@ -544,8 +601,8 @@ int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, c
       return fail(DMR_E_ARG, "id list shorter than the query or database");
     const uint8_t* letters = r->targets.letters.data() + r->targets.offs[ti];
     size_t frame_len = r->targets.offs[ti + 1] - r->targets.offs[ti];
-    const uint8_t* subj = r->db_res->data() + r->db_offs[h.subject];
-    size_t slen = r->db_offs[h.subject + 1] - r->db_offs[h.subject];
+    const uint8_t* subj = r->db_res->data() + (*r->db_offs)[h.subject];
+    size_t slen = (*r->db_offs)[h.subject + 1] - (*r->db_offs)[h.subject];
     size_t dna = r->translated ? r->dna_len[h.query] : frame_len;
     size_t qb = h.q_begin, qe = h.q_end;
     if (r->translated) {

@@ -98,7 +98,8 @@ void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n
     for (uint64_t k = offsets[i]; k < offsets[i + 1]; ++k) seq_of[k] = i;
   }
   h_db_offs_[n] = (uint32_t)total;
-  h_db_res_.assign(letters, letters + total);
+  h_db_offs_sp_ = std::make_shared<const std::vector<uint32_t>>(h_db_offs_);
+  h_db_res_sp_ = std::make_shared<const std::vector<uint8_t>>(letters, letters + total);
   DMR_HIP(hipMemcpy(db_res_.ensure<uint8_t>(total + 64), letters, total, hipMemcpyHostToDevice));
   DMR_HIP(hipMemcpy(db_offs_.ensure<uint32_t>(n + 1), h_db_offs_.data(), (n + 1) * 4,
                     hipMemcpyHostToDevice));

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -107,7 +108,9 @@ class Engine {
   uint32_t db_size() const { return db_n_; }
   uint64_t db_letters() const { return db_letters_; }
   const std::vector<uint32_t>& db_offsets_host() const { return h_db_offs_; }
-  const std::vector<uint8_t>& db_letters_host() const { return h_db_res_; }
+  const std::vector<uint8_t>& db_letters_host() const { return *h_db_res_sp_; }
+  std::shared_ptr<const std::vector<uint32_t>> db_offsets_shared() const { return h_db_offs_sp_; }
+  std::shared_ptr<const std::vector<uint8_t>> db_letters_shared() const { return h_db_res_sp_; }
 
   // Resident seed index for a preset (rebuilt only when the preset changes).
   void build_index(const std::string& sensitivity, int index_chunks, double seed_cut);
@@ -165,7 +168,8 @@ class Engine {
   uint64_t db_letters_ = 0;
   uint32_t db_max_len_ = 0;
   std::vector<uint32_t> h_db_offs_;
-  std::vector<uint8_t> h_db_res_;
+  std::shared_ptr<const std::vector<uint32_t>> h_db_offs_sp_;
+  std::shared_ptr<const std::vector<uint8_t>> h_db_res_sp_;
 
   // index
   std::string idx_sens_;
@@ -197,5 +201,10 @@ class Engine {
   DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
   uint64_t last_njobs_ = 0;
 };
+
+// search::run restated on the host with one drop-in call (one kernel launch)
+// per seed::build / seed::join_into / ungapped::xdrop / gapped::extend_span.
+std::vector<std::vector<HspRec>> search_plain(Engine& eng, const TargetSet& t,
+                                              const SearchOptions& opt, StageStats* stats);
 
 }  // namespace dmr

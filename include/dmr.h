@@ -175,6 +175,20 @@ int dmr_blastp(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, 
 int dmr_blastx(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint32_t nreads,
                const dmr_search_config* cfg, dmr_result** out);
 
+/* Uploads a blastp query batch and keeps it resident on the device, so
+ * dmr_search_loaded can run without any host-to-device copy of the inputs
+ * (the same search as dmr_blastp). */
+int dmr_queries_load(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq);
+int dmr_search_loaded(dmr_ctx* ctx, const dmr_search_config* cfg, dmr_result** out);
+
+/* The plain drop-in form: search::run restated on the host exactly as the
+ * reference structures it (src/search.rs:95-271), with each seed::build,
+ * seed::join_into, ungapped::xdrop and gapped::extend_span call replaced by
+ * one call of the entry points above (one kernel launch per call). Same
+ * results as dmr_blastp; kept as the per-call baseline of the batched form. */
+int dmr_blastp_plain(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq,
+                     const dmr_search_config* cfg, dmr_result** out);
+
 /* Number of HSPs (all queries, output order). */
 int64_t dmr_result_count(const dmr_result* r);
 /* 16 int64 per HSP: query, subject, frame, q_begin, q_end, s_begin, s_end,
