@@ -3,6 +3,8 @@
 #include "dmr_alp.hpp"
 #include "dmr_device.hpp"
 
+#include <vector>
+
 namespace dmr {
 
 // One DB chunk of the resident seed index (all shapes of the preset).
@@ -65,7 +67,19 @@ void launch_sw_generic(const SwJob* jobs, const uint32_t* job_idx, uint64_t njob
                        SeqSet db, const int8_t* matrix32, int gap_open, int gap_extend,
                        int32_t* scratch, const uint64_t* scratch_off, uint8_t* trace,
                        const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off,
-                       SwOut* out, bool traceback, hipStream_t st);
+                       SwOut* out, bool traceback, hipStream_t st, bool scatter = false);
+
+// Packed int16 score pass (k_sw16.hip).
+int pair_table_entries();
+void fill_pair_table(const int8_t (&m)[25][25], std::vector<int>* out);
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits,
+                     uint64_t* keys, uint32_t* idx, hipStream_t st);
+void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
+                       const uint32_t* lane_off, uint32_t nruns, uint2* lanes, hipStream_t st);
+int sw16_width(int cls);
+void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
+                 const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
+                 uint32_t* redo, uint32_t* nredo, int grid, hipStream_t st);
 void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);

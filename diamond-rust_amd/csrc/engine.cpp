@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -52,6 +53,14 @@ Engine::Engine(int device) : device_(device) {
     for (int b = 0; b < kScoreStride; ++b)
       m[a * kScoreStride + b] = (a < 25 && b < 25) ? blosum62()[a][b] : (int8_t)-64;
   DMR_HIP(hipMemcpy(matrix32_.ensure<int8_t>(sizeof m), m, sizeof m, hipMemcpyHostToDevice));
+  std::vector<int> pt;
+  fill_pair_table(blosum62(), &pt);
+  DMR_HIP(hipMemcpy(pair_tab_.ensure<int>(pt.size()), pt.data(), pt.size() * 4, hipMemcpyHostToDevice));
+  hipDeviceProp_t prop;
+  DMR_HIP(hipGetDeviceProperties(&prop, device));
+  cu_count_ = prop.multiProcessorCount;
+  const char* fg = std::getenv("DMR_FORCE_GENERIC");
+  force_generic_ = fg && fg[0] == '1';
 }
 
 Engine::~Engine() {
@@ -366,17 +375,9 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   stage_event(5);
 
   // ---- banded DP, score pass -----------------------------------------------
-  uint32_t* stride = stride_.ensure<uint32_t>(njobs + 1);
-  launch_job_geometry(jobs, nullptr, njobs, dbs, nullptr, nullptr, stride, st_);
-  uint64_t* soff = scr_off_.ensure<uint64_t>(njobs + 1);
-  launch_u32_times(stride, njobs, 4, soff, st_);
-  scan_exclusive_u64(soff, soff, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
-  const uint64_t scratch_total = read1(soff + njobs);
   SwOut* outs = outs_.ensure<SwOut>(njobs + 1);
   stage_event(6);
-  launch_sw_generic(jobs, nullptr, njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
-                    opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
-                    nullptr, nullptr, nullptr, outs, false, st_);
+  score_pass(jobs, njobs, (int)g.bt, qs, dbs, opt, outs, &S);
   stage_event(7);
   last_njobs_ = njobs;
 
@@ -547,6 +548,103 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   S.ms_total = now_ms() - t_start;
   if (stats) *stats = S;
   return result;
+}
+
+// Score pass over all DP jobs: jobs of the standard band widths go to the
+// packed int16 kernel (two DPs of one query per lane), the rest (merged or
+// clipped bands, other gap costs) and any int16-saturated DP to the exact
+// int32 kernel.
+void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
+                        const SearchOptions& opt, SwOut* outs, StageStats* S) {
+  if (!njobs) return;
+  const bool fast = opt.gap_open == 11 && opt.gap_extend == 1 && !force_generic_;
+  uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
+  uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
+  launch_classify(jobs, njobs, qs, dbs, target_bits, keys, idx, st_);
+  const int key_bits = 3 + target_bits + 14;
+  radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
+                   njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
+                   b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_);
+  // Runs of equal (class, target).
+  uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
+  launch_key_flags(keys, njobs, 14, fl, st_);
+  launch_fill_u32(fl + njobs, 1, 0, st_);
+  uint64_t* ps = pos_.ensure<uint64_t>(njobs + 1);
+  scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
+  const uint64_t nruns = read1(ps + njobs);
+  uint32_t* rs = runs_.ensure<uint32_t>(nruns + 1);
+  launch_compact_flags(fl, ps, njobs, rs, st_);
+  std::vector<uint32_t> h_rs(nruns + 1);
+  DMR_HIP(hipMemcpyAsync(h_rs.data(), rs, nruns * 4, hipMemcpyDeviceToHost, st_));
+  std::vector<uint64_t> h_keys(nruns);
+  // keys of the run starts: gather on the host side via a second small copy
+  DMR_HIP(hipStreamSynchronize(st_));
+  h_rs[nruns] = (uint32_t)njobs;
+  for (uint64_t r = 0; r < nruns; ++r)
+    DMR_HIP(hipMemcpyAsync(&h_keys[r], keys + h_rs[r], 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  const int cls_shift = target_bits + 14;
+  std::vector<uint32_t> run_start, run_len, lane_off;
+  uint32_t class_lanes[5] = {0, 0, 0, 0, 0}, class_lane0[5] = {0, 0, 0, 0, 0};
+  uint64_t generic_from = njobs;
+  uint32_t nl = 0;
+  for (uint64_t r = 0; r < nruns; ++r) {
+    int cls = (int)(h_keys[r] >> cls_shift);
+    if (cls >= 5 || !fast) {
+      generic_from = std::min<uint64_t>(generic_from, h_rs[r]);
+      continue;
+    }
+    if (class_lanes[cls] == 0) class_lane0[cls] = nl;
+    uint32_t len = h_rs[r + 1] - h_rs[r];
+    run_start.push_back(h_rs[r]);
+    run_len.push_back(len);
+    lane_off.push_back(nl);
+    nl += (len + 1) / 2;
+    class_lanes[cls] += (len + 1) / 2;
+  }
+  if (!fast) generic_from = 0;
+  uint32_t* nredo = sw_counters_.ensure<uint32_t>(8);
+  launch_fill_u32(nredo, 8, 0, st_);
+  uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
+  if (nl) {
+    const uint32_t nr = (uint32_t)run_start.size();
+    uint32_t* d_rs = sw_runs_.ensure<uint32_t>(3ull * nr + 3);
+    DMR_HIP(hipMemcpyAsync(d_rs, run_start.data(), nr * 4, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(d_rs + nr, run_len.data(), nr * 4, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(d_rs + 2 * nr, lane_off.data(), nr * 4, hipMemcpyHostToDevice, st_));
+    uint2* lanes = sw_lanes_.ensure<uint2>(nl + 1);
+    launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, lanes, st_);
+    int grid = 2 * cu_count_;
+    for (int c = 0; c < 5; ++c) {
+      if (!class_lanes[c]) continue;
+      DMR_HIP(hipEventRecord(ev_[11 + (c > 3 ? 3 : c > 2 ? 2 : c > 1 ? 1 : 0)], st_));
+      launch_sw16(c, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs, pair_tab_.as<int>(),
+                  matrix32_.as<int8_t>(), nredo + 1 + c, outs, redo, nredo, grid, st_);
+    }
+  }
+  S->sw16_lanes = nl;
+  // Exact int32 path: class-5 jobs (contiguous at the end of the sort) plus
+  // the DPs that saturated int16.
+  const uint64_t ngen = njobs - generic_from;
+  const uint32_t nr16 = read1(nredo);
+  S->sw_generic = ngen + nr16;
+  S->sw_redo = nr16;
+  const uint64_t nlist = ngen + nr16;
+  if (nlist) {
+    uint32_t* list = sw_list_.ensure<uint32_t>(nlist + 1);
+    if (ngen)
+      DMR_HIP(hipMemcpyAsync(list, idx + generic_from, ngen * 4, hipMemcpyDeviceToDevice, st_));
+    if (nr16) DMR_HIP(hipMemcpyAsync(list + ngen, redo, nr16 * 4, hipMemcpyDeviceToDevice, st_));
+    uint32_t* stride = stride_.ensure<uint32_t>(nlist + 1);
+    launch_job_geometry(jobs, list, nlist, dbs, nullptr, nullptr, stride, st_);
+    uint64_t* soff = scr_off_.ensure<uint64_t>(nlist + 1);
+    launch_u32_times(stride, nlist, 4, soff, st_);
+    scan_exclusive_u64(soff, soff, nlist + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nlist + 1)), st_);
+    const uint64_t scratch_total = read1(soff + nlist);
+    launch_sw_generic(jobs, list, nlist, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+                      opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
+                      nullptr, nullptr, nullptr, outs, false, st_, /*scatter=*/true);
+  }
 }
 
 uint64_t Engine::count_last_dp_cells() {
@@ -781,10 +879,23 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
   DMR_HIP(hipMemcpyAsync(doo.ensure<uint64_t>(njobs + 1), ooff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
   SwOut* o = dout.ensure<SwOut>(njobs);
   uint32_t* opsd = dops.ensure<uint32_t>(ooff[njobs] + 1);
-  launch_sw_generic(djobs.as<SwJob>(), nullptr, njobs, qs, ss, matrix32_.as<int8_t>(), gap_open,
-                    gap_extend, dscr.ensure<int32_t>(soff[njobs] + 1), dsoff.as<uint64_t>(),
-                    dtr.ensure<uint8_t>(toff[njobs] + 1), dtoff.as<uint64_t>(), opsd,
-                    doo.as<uint64_t>(), o, traceback, st_);
+  if (traceback) {
+    launch_sw_generic(djobs.as<SwJob>(), nullptr, njobs, qs, ss, matrix32_.as<int8_t>(), gap_open,
+                      gap_extend, dscr.ensure<int32_t>(soff[njobs] + 1), dsoff.as<uint64_t>(),
+                      dtr.ensure<uint8_t>(toff[njobs] + 1), dtoff.as<uint64_t>(), opsd,
+                      doo.as<uint64_t>(), o, true, st_);
+  } else {
+    // Score only: the same score pass as the batched search (packed int16
+    // kernel for the standard band widths, exact int32 for the rest).
+    SearchOptions so;
+    so.gap_open = gap_open;
+    so.gap_extend = gap_extend;
+    StageStats S;
+    int tb = 1;
+    while ((1ull << tb) <= 2 * njobs) ++tb;
+    score_pass(djobs.as<SwJob>(), njobs, tb, qs, ss, so, o, &S);
+    last_dropin_stats_ = S;
+  }
   DMR_HIP(hipMemcpyAsync(outs->data(), o, njobs * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
   std::vector<uint32_t> hops(ooff[njobs]);
   if (traceback && ooff[njobs])

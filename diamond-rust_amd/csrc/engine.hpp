@@ -83,6 +83,7 @@ struct HspRec {
 struct StageStats {
   uint64_t query_seeds = 0, runs = 0, hits = 0, hit_runs = 0, segments = 0, groups = 0;
   uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
+  uint64_t sw16_lanes = 0, sw_generic = 0, sw_redo = 0;
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
 };
@@ -153,6 +154,8 @@ class Engine {
   };
   void upload_seed_params(const std::string& sens, double cut, uint32_t* nshapes, int* key_bits,
                           int* shape_bits);
+  void score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
+                  const SearchOptions& opt, SwOut* outs, StageStats* S);
   void stage_event(int k);
   float stage_ms(int a, int b);
   template <class T>
@@ -199,7 +202,14 @@ class Engine {
   DevBuf bd0_, bd1_, bbest_, tmpjobs_, nbands_, joboff_, jobs_;
   DevBuf stride_, scr_off_, scratch_, outs_;
   DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
+  DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_runs_, sw_lanes_, sw_counters_, sw_redo_, sw_list_;
+  DevBuf pair_tab_;
+  int cu_count_ = 256;
+  bool force_generic_ = false;
   uint64_t last_njobs_ = 0;
+
+ public:
+  StageStats last_dropin_stats_;
 };
 
 // search::run restated on the host with one drop-in call (one kernel launch)

@@ -1,0 +1,304 @@
+// Banded SW score pass in packed int16: two DPs per lane, band in registers.
+//
+// Each lane carries two DPs of the same query (lo/hi halves of every VGPR)
+// with the same band width W (a compile-time constant), so every recurrence
+// step is one packed instruction for two cells. The band of a row lives in
+// registers indexed by the in-row offset t (column j = i - d1 + t), and the
+// dependencies of gapped::extend_span (src/gapped.rs:110-199) become
+//   D(i,t) = H(i-1,t) + s,  E(i,t) = max(H(i-1,t+1) - o, E(i-1,t+1) - e),
+//   F(i,t) = max(H(i,t-1) - o, F(i,t-1) - e),  H = max(0, D, E, F),
+// with o = open + extend. G = H - o is stored instead of H (the score table
+// holds s + o), which saves one subtraction per cell pair.
+//
+// Exactness against the row-range rules of the reference:
+// - Columns j <= 0 and rows above the band read a padding letter scoring -64:
+//   those cells stay at H = 0, which is what the reference computes there
+//   (column 0 is always 0; E/F values of -o or NEG never change H).
+// - Columns j > m (past the subject) and query rows past n read the padding
+//   letter too: such cells only feed cells outside the DP and stay strictly
+//   below some in-range H, so they never change the maximum.
+// - The column after the band (t = W) is a wall: its G and E stay NEG, so the
+//   rightmost cell of a row gets no E from outside the band.
+// - Rows past m + d1 hold a single cell at column m (the clamped rows of the
+//   reference); they are scored after the main loop from the values captured
+//   at row m + d1, and can only matter when the best score is below 11.
+// Saturating adds detect scores >= 32767; those DPs are re-run in int32.
+#include "dmr_kernels.hpp"
+
+namespace dmr {
+
+namespace {
+
+typedef short v2s __attribute__((ext_vector_type(2)));
+
+constexpr int kCodes = 26;                  // 25 letters + padding
+constexpr int kPairRow = kCodes * kCodes;   // (sA, sB) pairs for one query letter
+constexpr int kPairTab = kCodes * kPairRow;
+constexpr short kNeg16 = -16384;
+constexpr int kOpen = 12;                   // gap open + extend of the 16-bit tables
+
+__device__ __forceinline__ v2s as_v2s(int x) { return __builtin_bit_cast(v2s, x); }
+__device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ v2s vadds(v2s a, v2s b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ v2s vsubs(v2s a, v2s b) { return __builtin_elementwise_sub_sat(a, b); }
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ int letter_at(const uint8_t* s, int m, int p) {
+  return (p >= 0 && p < m) ? (int)s[p] : 25;
+}
+
+// Scalar tail of the clamped rows below the band: cells (i, m) for
+// i = istar+1..n starting from H(istar, m), E(istar, m).
+__device__ int tail_best(const int8_t* __restrict__ m32, const uint8_t* qs, int n, int istar,
+                         int h, int e, int slast, int best) {
+  for (int i = istar + 1; i <= n; ++i) {
+    int ev = max(h - kOpen, e - 1);
+    int d = m32[qs[i - 1] * kScoreStride + slast];
+    int hv = max(0, max(d, ev));
+    best = max(best, hv);
+    h = hv;
+    e = ev;
+  }
+  return best;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, uint32_t nlanes,
+                                              const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
+                                              const int* __restrict__ pair_tab,
+                                              const int8_t* __restrict__ m32,
+                                              uint32_t* __restrict__ counter,
+                                              SwOut* __restrict__ outs,
+                                              uint32_t* __restrict__ redo, uint32_t* __restrict__ nredo) {
+  __shared__ int T[kPairTab];
+  for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const v2s NEG2 = {kNeg16, kNeg16};
+  const v2s ONE2 = {1, 1};
+  const v2s OPEN2 = {kOpen, kOpen};
+  const v2s ZERO2 = {0, 0};
+  for (;;) {
+    uint32_t task = 0;
+    if (lane == 0) task = atomicAdd(counter, 1u);
+    task = __shfl(task, 0, 64);
+    if ((uint64_t)task * 64 >= nlanes) break;
+    const uint32_t L = task * 64 + lane;
+    const bool active = L < nlanes;
+    uint2 ln = active ? lanes[L] : make_uint2(~0u, ~0u);
+    const bool hasA = ln.x != ~0u, hasB = ln.y != ~0u;
+    SwJob ja = {}, jb = {};
+    if (hasA) ja = jobs[ln.x];
+    if (hasB) jb = jobs[ln.y];
+    const uint8_t* qs = q.res + (hasA ? q.offs[ja.target] : 0);
+    const int n = hasA ? (int)(q.offs[ja.target + 1] - q.offs[ja.target]) : 0;
+    const uint8_t* sA = db.res + (hasA ? db.offs[ja.subject] : 0);
+    const uint8_t* sB = db.res + (hasB ? db.offs[jb.subject] : 0);
+    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - db.offs[ja.subject]) : 0;
+    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - db.offs[jb.subject]) : 0;
+    const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
+    const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
+    const int istarA = mA + d1A, istarB = mB + d1B;
+    int rlo = 1 << 30, rhi = -(1 << 30);
+    if (hasA) {
+      rlo = max(1, d0A + 1);
+      rhi = min(n, istarA);
+    }
+    if (hasB) {
+      rlo = min(rlo, max(1, d0B + 1));
+      rhi = max(rhi, min(n, istarB));
+    }
+    const int lo = wave_min(rlo), hi = wave_max(rhi);
+
+    // Window of (sA, sB) letter pairs for the current row, as byte offsets
+    // into a score-table row, two 16-bit entries per register: entry t is
+    // column j = i - d1 + t (t even: low half, t odd: high half).
+    constexpr int NR = (W + 1) / 2;
+    v2s G[W + 1], E[W + 1];
+    uint32_t R[NR];
+#pragma unroll
+    for (int t = 0; t <= W; ++t) {
+      G[t] = t < W ? v2s{-kOpen, -kOpen} : NEG2;
+      E[t] = NEG2;
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      uint32_t lo16 = 4u * (letter_at(sA, mA, lo - d1A + 2 * k - 1) * kCodes +
+                            letter_at(sB, mB, lo - d1B + 2 * k - 1));
+      uint32_t hi16 = 2 * k + 1 < W ? 4u * (letter_at(sA, mA, lo - d1A + 2 * k) * kCodes +
+                                            letter_at(sB, mB, lo - d1B + 2 * k))
+                                    : 0u;
+      R[k] = lo16 | (hi16 << 16);
+    }
+    v2s best = ZERO2;
+    int capHA = 0, capEA = kNeg16, capHB = 0, capEB = kNeg16;
+    for (int i = lo; i <= hi; ++i) {
+      if (i > lo) {
+        // Shift the window by one entry and append the pair entering at t = W-1.
+        static_assert(W % 2 == 1, "band widths are odd");
+#pragma unroll
+        for (int k = 0; k < NR - 1; ++k) R[k] = __builtin_amdgcn_alignbit(R[k + 1], R[k], 16);
+        R[NR - 1] = 4u * (letter_at(sA, mA, i - d0A - 1) * kCodes + letter_at(sB, mB, i - d0B - 1));
+      }
+      const int qi = (i <= n) ? (int)qs[i - 1] : 25;
+      const char* Tq = reinterpret_cast<const char*>(T) + qi * (4 * kPairRow);
+      v2s F = NEG2, Gl = NEG2;
+#pragma unroll
+      for (int t = 0; t < W; ++t) {
+        const uint32_t off = (t & 1) ? (R[t >> 1] >> 16) : (R[t >> 1] & 0xFFFFu);
+        const v2s sc = as_v2s(*reinterpret_cast<const int*>(Tq + off));
+        const v2s D = vadds(G[t], sc);
+        const v2s Ev = vmax(G[t + 1], vsubs(E[t + 1], ONE2));
+        const v2s Fv = vmax(Gl, vsubs(F, ONE2));
+        const v2s H = vmax(vmax(D, Ev), vmax(Fv, ZERO2));
+        best = vmax(best, H);
+        Gl = vsubs(H, OPEN2);
+        G[t] = Gl;
+        E[t] = Ev;
+        F = Fv;
+      }
+      if (i == istarA) {
+        capHA = (int)G[0].x + kOpen;
+        capEA = E[0].x;
+      }
+      if (i == istarB) {
+        capHB = (int)G[0].y + kOpen;
+        capEB = E[0].y;
+      }
+    }
+    int bA = best.x, bB = best.y;
+    if (hasA) {
+      if (bA < 11 && istarA < n && istarA >= 1) bA = tail_best(m32, qs, n, istarA, capHA, capEA, sA[mA - 1], bA);
+      if (bA >= 32767) {
+        redo[atomicAdd(nredo, 1u)] = ln.x;
+      } else {
+        outs[ln.x].score = bA;
+        outs[ln.x].flags = 0;
+      }
+    }
+    if (hasB) {
+      if (bB < 11 && istarB < n && istarB >= 1) bB = tail_best(m32, qs, n, istarB, capHB, capEB, sB[mB - 1], bB);
+      if (bB >= 32767) {
+        redo[atomicAdd(nredo, 1u)] = ln.y;
+      } else {
+        outs[ln.y].score = bB;
+        outs[ln.y].flags = 0;
+      }
+    }
+  }
+}
+
+// Band-width classes served by k_sw16 (2 * band + 1 for each band_for value).
+__device__ __forceinline__ int width_class(int w) {
+  switch (w) {
+    case 25: return 0;
+    case 33: return 1;
+    case 61: return 2;
+    case 81: return 3;
+    case 129: return 4;
+    default: return 5;
+  }
+}
+
+// Sort key per job: class | target | rows; class 5 = exact int32 path.
+__global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSet q, SeqSet db,
+                           int target_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= njobs) return;
+  const SwJob j = jobs[k];
+  const int64_t qlen = q.offs[j.target + 1] - q.offs[j.target];
+  const int64_t m = db.offs[j.subject + 1] - db.offs[j.subject];
+  const int64_t n = (int64_t)j.q1 - j.q0;
+  const int64_t w = (int64_t)j.d1 - j.d0 + 1;
+  int cls = width_class((int)(w > 1000 ? 1000 : w));
+  const int64_t rlo = j.d0 + 1 > 1 ? j.d0 + 1 : 1;
+  const int64_t istar = m + j.d1;
+  const int64_t rhi = istar < n ? istar : n;
+  if (j.q0 != 0 || n != qlen || rlo > rhi || istar < 1 || m < 1 || n > 16383 || m > 16383) cls = 5;
+  const uint64_t r = (uint64_t)(rhi < 0 ? 0 : (rhi > 16383 ? 16383 : rhi));
+  keys[k] = ((uint64_t)cls << (target_bits + 14)) | ((uint64_t)j.target << 14) | r;
+  idx[k] = (uint32_t)k;
+}
+
+// One thread per (class, target) run of sorted jobs: pairs consecutive jobs.
+__global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint32_t* __restrict__ run_start,
+                             const uint32_t* __restrict__ run_len, const uint32_t* __restrict__ lane_off,
+                             uint32_t nruns, uint2* __restrict__ lanes) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  uint32_t s = run_start[r], len = run_len[r], o = lane_off[r];
+  for (uint32_t k = 0; k < len; k += 2)
+    lanes[o + k / 2] = make_uint2(sorted_idx[s + k], k + 1 < len ? sorted_idx[s + k + 1] : ~0u);
+}
+
+inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+int pair_table_entries() { return kPairTab; }
+
+// Score table for the packed kernel: entry (qa, sA, sB) holds
+// (score(qa, sA) + o, score(qa, sB) + o) as two int16; letter 25 is padding.
+void fill_pair_table(const int8_t (&m)[25][25], std::vector<int>* out) {
+  out->assign(kPairTab, 0);
+  auto sc = [&](int a, int b) { return (a < 25 && b < 25) ? (int)m[a][b] : -64; };
+  for (int qa = 0; qa < kCodes; ++qa)
+    for (int a = 0; a < kCodes; ++a)
+      for (int b = 0; b < kCodes; ++b) {
+        int lo = sc(qa, a) + kOpen, hi = sc(qa, b) + kOpen;
+        (*out)[qa * kPairRow + a * kCodes + b] = (int)((uint32_t)(uint16_t)(int16_t)lo |
+                                                       ((uint32_t)(uint16_t)(int16_t)hi << 16));
+      }
+}
+
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits,
+                     uint64_t* keys, uint32_t* idx, hipStream_t st) {
+  if (!njobs) return;
+  hipLaunchKernelGGL(k_classify, dim3(nblk(njobs, 256)), dim3(256), 0, st, jobs, njobs, q, db,
+                     target_bits, keys, idx);
+}
+
+void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
+                       const uint32_t* lane_off, uint32_t nruns, uint2* lanes, hipStream_t st) {
+  if (!nruns) return;
+  hipLaunchKernelGGL(k_make_lanes, dim3(nblk(nruns, 128)), dim3(128), 0, st, sorted_idx, run_start,
+                     run_len, lane_off, nruns, lanes);
+}
+
+static const int kWidths[5] = {25, 33, 61, 81, 129};
+int sw16_width(int cls) { return kWidths[cls]; }
+
+void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
+                 const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
+                 uint32_t* redo, uint32_t* nredo, int grid, hipStream_t st) {
+  if (!nlanes) return;
+  unsigned tasks = nblk(nlanes, 64);
+  unsigned blocks = std::min<unsigned>((unsigned)grid, nblk(tasks, 4));
+  switch (cls) {
+#define DMR_SW16(c, w)                                                                              \
+  case c:                                                                                           \
+    hipLaunchKernelGGL(k_sw16<w>, dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,       \
+                       pair_tab, m32, counter, outs, redo, nredo);                                  \
+    break;
+    DMR_SW16(0, 25)
+    DMR_SW16(1, 33)
+    DMR_SW16(2, 61)
+    DMR_SW16(3, 81)
+    DMR_SW16(4, 129)
+#undef DMR_SW16
+    default:
+      break;
+  }
+}
+
+}  // namespace dmr

@@ -136,3 +136,65 @@ def test_seed_join_cap_boundary(ctx):
     q = (z(64, np.uint64), np.arange(64) % 3, np.arange(64))
     assert len(ctx.seed_join(q, (z(64, np.uint64), np.arange(64), np.arange(64)), 4096)) == 4096
     assert len(ctx.seed_join(q, (z(65, np.uint64), np.arange(65), np.arange(65)), 4096)) == 0
+
+
+def _score_batch(ctx, cases):
+    qbuf = np.concatenate([c[0] for c in cases])
+    sbuf = np.concatenate([c[1] for c in cases])
+    jobs, qo, so = [], 0, 0
+    for q, s, d0, d1 in cases:
+        jobs.append((qo, len(q), so, len(s), d0, d1))
+        qo += len(q)
+        so += len(s)
+    return ctx.extend_span_batch(qbuf, sbuf, jobs, traceback=False)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_int16_score_pass_band_widths(ctx, orc, seed):
+    """Score-only batches run the packed int16 kernel for the standard band
+    widths 25/33/61/81/129 (2 * band_for + 1); every geometry must match."""
+    rng = np.random.default_rng(100 + seed)
+    cases = []
+    for _ in range(600):
+        band = int(rng.choice([12, 16, 30, 40, 64]))
+        n = int(rng.integers(1, 1200 if rng.random() < 0.2 else 350))
+        q = rand_seq(rng, n, canonical_only=rng.random() < 0.9)
+        r = rng.random()
+        s = q.copy() if r < 0.15 else (mutate(rng, q, float(rng.uniform(0.1, 0.7)), 0.05) if r < 0.6
+                                       else rand_seq(rng, int(rng.integers(1, 800))))
+        if len(s) == 0:
+            s = rand_seq(rng, 2)
+        c = int(rng.integers(-len(s) - band, n + band))
+        cases.append((q, s, c - band, c + band))
+    got = _score_batch(ctx, cases)
+    for (q, s, d0, d1), g in zip(cases, got):
+        ref = orc.extend_span(q, s, d0, d1)
+        assert g == (ref["score"] if ref else None), (len(q), len(s), d0, d1)
+
+
+def test_int16_tail_rows_below_band(ctx, orc):
+    """Rows past m + d1 keep one cell at column m (the reference clamps them
+    there); with an in-band best below 11 that column decides the score."""
+    enc = orc.encode
+    cases = []
+    for k in (20, 50, 99):
+        q = enc("D" * k + "W" + "D" * (120 - k))
+        cases.append((q, enc("AAW"), -12, 12))       # W = 25
+        cases.append((q, enc("AAAAW"), -20, 20))     # W = 41: exact int32 path
+        cases.append((q, enc("AW"), 5 - 30, 5 + 30))  # W = 61
+    got = _score_batch(ctx, cases)
+    for (q, s, d0, d1), g in zip(cases, got):
+        ref = orc.extend_span(q, s, d0, d1)
+        assert g == (ref["score"] if ref else None) and g == 11
+
+
+def test_int16_saturation_reruns_exact(ctx, orc):
+    """Self-alignments scoring above 32767 saturate int16 and are re-scored in int32."""
+    rng = np.random.default_rng(3)
+    hi = np.array([17, 4, 8, 18, 14, 13], np.uint8)  # W C H Y P F: self-scores 11, 9, 8, 7, 7, 6
+    q = hi[rng.integers(0, len(hi), size=4500)]
+    cases = [(q, q.copy(), -64, 64), (q[:300], q[:300].copy(), -64, 64)]
+    got = _score_batch(ctx, cases)
+    ref = [orc.extend_span(*c)["score"] for c in cases]
+    assert ref[0] > 32767
+    assert got == ref
