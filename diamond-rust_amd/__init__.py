@@ -32,7 +32,8 @@ EXPORTS = [
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend",
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
-    "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
+    "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
+    "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy",
 ]
 
@@ -143,6 +144,7 @@ def lib():
             "dmr_result_ops": ([P, I64, P, U32, P], C.c_int),
             "dmr_result_format": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, U64, P], C.c_int),
             "dmr_result_stats": ([P, P], C.c_int), "dmr_last_dp_cells": ([P, P], C.c_int),
+            "dmr_last_class_profile": ([P, P, P, P], C.c_int),
             "dmr_result_destroy": ([P], None), "dmr_search_config_default": ([P], None),
         }
         for name, (args, res) in sig.items():
@@ -307,6 +309,16 @@ class Context:
         buf = C.create_string_buffer(data, len(data) + 1)
         _check(lib().dmr_blastx(self._h, buf, off.ctypes.data, len(reads), C.byref(c), C.byref(h)))
         return Result(h.value)
+
+    def class_profile(self):
+        """Per band-width class (25, 33, 61, 81, 129, int32 rest) of the last
+        search: jobs, DP cells, score-kernel ms."""
+        j = np.zeros(6, np.uint64)
+        c = np.zeros(6, np.uint64)
+        m = np.zeros(6, np.float64)
+        _check(lib().dmr_last_class_profile(self._h, j.ctypes.data, c.ctypes.data, m.ctypes.data))
+        names = ["w25", "w33", "w61", "w81", "w129", "int32"]
+        return {n: dict(jobs=int(j[i]), cells=int(c[i]), ms=float(m[i])) for i, n in enumerate(names)}
 
     def last_dp_cells(self) -> int:
         v = C.c_uint64()

@@ -704,6 +704,14 @@ int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, c
   return DMR_OK;
 }
 
+int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs6, uint64_t* cells6, double* ms6) {
+  if (!ctx || !jobs6 || !cells6 || !ms6) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&]() -> int {
+    ctx->eng->last_class_profile(jobs6, cells6, ms6);
+    return DMR_OK;
+  });
+}
+
 void dmr_result_destroy(dmr_result* r) { delete r; }
 
 }  // extern "C"

@@ -557,6 +557,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
 void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
                         const SearchOptions& opt, SwOut* outs, StageStats* S) {
   if (!njobs) return;
+  last_target_bits_ = target_bits;
   const bool fast = opt.gap_open == 11 && opt.gap_extend == 1 && !force_generic_;
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
@@ -617,9 +618,10 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     int grid = 2 * cu_count_;
     for (int c = 0; c < 5; ++c) {
       if (!class_lanes[c]) continue;
-      DMR_HIP(hipEventRecord(ev_[11 + (c > 3 ? 3 : c > 2 ? 2 : c > 1 ? 1 : 0)], st_));
+      DMR_HIP(hipEventRecord(ev_[16 + 2 * c], st_));
       launch_sw16(c, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs, pair_tab_.as<int>(),
                   matrix32_.as<int8_t>(), nredo + 1 + c, outs, redo, nredo, grid, st_);
+      DMR_HIP(hipEventRecord(ev_[17 + 2 * c], st_));
     }
   }
   S->sw16_lanes = nl;
@@ -627,6 +629,10 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   // the DPs that saturated int16.
   const uint64_t ngen = njobs - generic_from;
   const uint32_t nr16 = read1(nredo);
+  for (int c = 0; c < 6; ++c) S->ms_class[c] = 0;
+  for (int c = 0; c < 5; ++c)
+    if (nl && class_lanes[c]) S->ms_class[c] = stage_ms(16 + 2 * c, 17 + 2 * c);
+  last_generic_from_ = generic_from;
   S->sw_generic = ngen + nr16;
   S->sw_redo = nr16;
   const uint64_t nlist = ngen + nr16;
@@ -641,9 +647,38 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     launch_u32_times(stride, nlist, 4, soff, st_);
     scan_exclusive_u64(soff, soff, nlist + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nlist + 1)), st_);
     const uint64_t scratch_total = read1(soff + nlist);
+    DMR_HIP(hipEventRecord(ev_[26], st_));
     launch_sw_generic(jobs, list, nlist, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                       opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
                       nullptr, nullptr, nullptr, outs, false, st_, /*scatter=*/true);
+    DMR_HIP(hipEventRecord(ev_[27], st_));
+    DMR_HIP(hipEventSynchronize(ev_[27]));
+    S->ms_class[5] = stage_ms(26, 27);
+  }
+  for (int c = 0; c < 6; ++c) last_ms_class_[c] = S->ms_class[c];
+}
+
+void Engine::last_class_profile(uint64_t jobs[6], uint64_t cells[6], double ms[6]) {
+  for (int c = 0; c < 6; ++c) {
+    jobs[c] = cells[c] = 0;
+    ms[c] = last_ms_class_[c];
+  }
+  const uint64_t n = last_njobs_;
+  if (!n) return;
+  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
+  uint64_t* c = cells_.ensure<uint64_t>(n + 1);
+  launch_job_geometry(jobs_.as<SwJob>(), sw_idx_.as<uint32_t>(), n, dbs, c, nullptr, nullptr, st_);
+  std::vector<uint64_t> hc(n), hk(n);
+  DMR_HIP(hipMemcpyAsync(hc.data(), c, n * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipMemcpyAsync(hk.data(), sw_keys_.as<uint64_t>(), n * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  const int shift = last_target_bits_ + 14;
+  for (uint64_t k = 0; k < n; ++k) {
+    int cls = (int)(hk[k] >> shift);
+    if (k >= last_generic_from_) cls = 5;
+    if (cls > 5) cls = 5;
+    jobs[cls]++;
+    cells[cls] += hc[k];
   }
 }
 

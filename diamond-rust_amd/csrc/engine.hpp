@@ -84,6 +84,7 @@ struct StageStats {
   uint64_t query_seeds = 0, runs = 0, hits = 0, hit_runs = 0, segments = 0, groups = 0;
   uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
   uint64_t sw16_lanes = 0, sw_generic = 0, sw_redo = 0;
+  double ms_class[6] = {0, 0, 0, 0, 0, 0};  // score kernel time per width class (5 = int32)
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
 };
@@ -125,6 +126,9 @@ class Engine {
   std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats);
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
+  // Per width class (25, 33, 61, 81, 129, other/int32) of the last search:
+  // jobs, DP cells and score-kernel milliseconds.
+  void last_class_profile(uint64_t jobs[6], uint64_t cells[6], double ms[6]);
 
   // --- drop-in calls on host buffers -------------------------------------
   void seed_build(const uint8_t* seqs, const uint64_t* offs, uint32_t nseq, const ShapeDef& shape,
@@ -163,7 +167,7 @@ class Engine {
 
   int device_ = 0;
   hipStream_t st_ = nullptr;
-  hipEvent_t ev_[16] = {};
+  hipEvent_t ev_[32] = {};
 
   // database
   DevBuf db_res_, db_offs_, db_seq_of_;
@@ -206,7 +210,9 @@ class Engine {
   DevBuf pair_tab_;
   int cu_count_ = 256;
   bool force_generic_ = false;
-  uint64_t last_njobs_ = 0;
+  uint64_t last_njobs_ = 0, last_generic_from_ = 0;
+  int last_target_bits_ = 0;
+  double last_ms_class_[6] = {0, 0, 0, 0, 0, 0};
 
  public:
   StageStats last_dropin_stats_;
