@@ -103,9 +103,7 @@ def main():
     gcups = cells / (sw / 1e3) / 1e9
     # Roofline of the dominant score kernel (the width class with the most
     # kernel time): algorithmic cells of its jobs x 11 int16 ops / its time.
-    dom = max((k for k in prof if k != "int32"), key=lambda k: prof[k]["ms"])
-    if prof["int32"]["ms"] > prof[dom]["ms"]:
-        dom = "int32"
+    dom = max(prof, key=lambda k: prof[k]["ms"])
     dcells, dms = prof[dom]["cells"], max(prof[dom]["ms"], 1e-9)
     achieved = dcells * OPS_PER_CELL / (dms / 1e3) / 1e12
     p50, p99 = (float(np.percentile(np.array(steps) * 1e3, q)) for q in (50, 99))
@@ -128,8 +126,10 @@ def main():
         "dp_cells": int(cells),
         "hsps": text.count("\n"),
         "roofline": {"bound": "valu",
-                     "kernel": ("k_sw_generic (int32 score pass)" if dom == "int32"
-                                else f"k_sw16<{dom[1:]}> (packed int16 score pass, band width {dom[1:]})"),
+                     "kernel": (f"k_sw16<{dom[1:]}> (packed int16 score pass, band width {dom[1:]})"
+                                if dom.startswith("w") else
+                                f"k_sw_wave ({dom}, int32 score pass)" if dom.startswith("wave")
+                                else "k_sw_generic (row-exact int32 score pass)"),
                      "achieved": round(achieved, 3), "peak": INT16_VALU_PEAK_TOPS,
                      "unit": "Tops/s int16", "frac": round(achieved / INT16_VALU_PEAK_TOPS, 4),
                      "traffic": None,

@@ -311,13 +311,15 @@ class Context:
         return Result(h.value)
 
     def class_profile(self):
-        """Per band-width class (25, 33, 61, 81, 129, int32 rest) of the last
-        search: jobs, DP cells, score-kernel ms."""
-        j = np.zeros(6, np.uint64)
-        c = np.zeros(6, np.uint64)
-        m = np.zeros(6, np.float64)
+        """Per score-kernel class of the last search: packed int16 by band width
+        (w25..w129), wave-per-DP int32 by width bound (wave64..wave512), and the
+        row-exact int32 kernel: jobs, DP cells, kernel ms."""
+        j = np.zeros(10, np.uint64)
+        c = np.zeros(10, np.uint64)
+        m = np.zeros(10, np.float64)
         _check(lib().dmr_last_class_profile(self._h, j.ctypes.data, c.ctypes.data, m.ctypes.data))
-        names = ["w25", "w33", "w61", "w81", "w129", "int32"]
+        names = ["w25", "w33", "w61", "w81", "w129", "wave64", "wave128", "wave256", "wave512",
+                 "rowexact"]
         return {n: dict(jobs=int(j[i]), cells=int(c[i]), ms=float(m[i])) for i, n in enumerate(names)}
 
     def last_dp_cells(self) -> int:

@@ -18,7 +18,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libdmr.so"
 ARCH = os.environ.get("DMR_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["k_sort.hip", "k_seed.hip", "k_extend.hip", "k_sw.hip", "k_sw16.hip", "plain.cpp",
+SOURCES = ["k_sort.hip", "k_seed.hip", "k_extend.hip", "k_sw.hip", "k_sw16.hip", "k_swwave.hip", "plain.cpp",
            "engine.cpp", "api.cpp"]
 
 COMMON = [

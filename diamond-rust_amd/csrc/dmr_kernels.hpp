@@ -84,5 +84,14 @@ void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, 
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);
 void launch_u32_times(const uint32_t* in, uint64_t n, uint32_t mul, uint64_t* out, hipStream_t st);
+void launch_gather_u64(const uint64_t* in, const uint32_t* idx, uint64_t n, uint64_t* out, hipStream_t st);
+
+// Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width <= 64*K.
+int swwave_k_for(int64_t width);
+void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
+                    SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
+                    uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
+                    uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
+                    int grid, hipStream_t st);
 
 }  // namespace dmr

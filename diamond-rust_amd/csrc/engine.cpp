@@ -460,51 +460,19 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   S.selected = sel.size();
 
   // ---- traceback of the reported DPs ----------------------------------------
-  std::vector<SwOut> h_souts(sel.size());
-  std::vector<uint32_t> h_ops;
-  std::vector<uint64_t> h_ops_off(sel.size() + 1, 0);
+  std::vector<SwOut> h_souts;
+  std::vector<std::vector<uint32_t>> h_sops;
   stage_event(9);
-  if (!sel.empty()) {
-    const uint64_t ns = sel.size();
-    std::vector<uint32_t> sidx(ns);
-    for (uint64_t i = 0; i < ns; ++i) {
-      sidx[i] = h_midx[sel[i]];
-      const SwJob& j = h_mjobs[sel[i]];
-      uint64_t cap_ops = (uint64_t)(j.q1 - j.q0) + (h_db_offs_[j.subject + 1] - h_db_offs_[j.subject]) + 1;
-      h_ops_off[i + 1] = h_ops_off[i] + cap_ops;
+  {
+    std::vector<uint32_t> list(sel.size()), slen(sel.size());
+    std::vector<SwJob> hj(sel.size());
+    for (size_t i = 0; i < sel.size(); ++i) {
+      list[i] = h_midx[sel[i]];
+      hj[i] = h_mjobs[sel[i]];
+      slen[i] = h_db_offs_[hj[i].subject + 1] - h_db_offs_[hj[i].subject];
     }
-    uint32_t* d_sidx = sel_idx_.ensure<uint32_t>(ns + 1);
-    DMR_HIP(hipMemcpyAsync(d_sidx, sidx.data(), ns * 4, hipMemcpyHostToDevice, st_));
-    uint64_t* cells = cells_.ensure<uint64_t>(ns + 1);
-    uint64_t* w0 = width0_.ensure<uint64_t>(ns + 1);
-    uint32_t* sstr = stride_.ensure<uint32_t>(ns + 1);
-    launch_job_geometry(jobs, d_sidx, ns, dbs, cells, w0, sstr, st_);
-    std::vector<uint64_t> hc(ns), hw(ns);
-    std::vector<uint32_t> hs(ns);
-    DMR_HIP(hipMemcpyAsync(hc.data(), cells, ns * 8, hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(hw.data(), w0, ns * 8, hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(hs.data(), sstr, ns * 4, hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipStreamSynchronize(st_));
-    std::vector<uint64_t> toff(ns + 1, 0), so(ns + 1, 0);
-    for (uint64_t i = 0; i < ns; ++i) {
-      toff[i + 1] = toff[i] + hc[i] + hw[i];
-      so[i + 1] = so[i] + 4ull * hs[i];
-      S.traceback_cells += hc[i];
-    }
-    uint64_t* d_toff = trace_off_.ensure<uint64_t>(ns + 1);
-    uint64_t* d_so = scr_off_.ensure<uint64_t>(std::max<uint64_t>(ns, njobs) + 1);
-    uint64_t* d_oo = ops_off_.ensure<uint64_t>(ns + 1);
-    DMR_HIP(hipMemcpyAsync(d_toff, toff.data(), (ns + 1) * 8, hipMemcpyHostToDevice, st_));
-    DMR_HIP(hipMemcpyAsync(d_so, so.data(), (ns + 1) * 8, hipMemcpyHostToDevice, st_));
-    DMR_HIP(hipMemcpyAsync(d_oo, h_ops_off.data(), (ns + 1) * 8, hipMemcpyHostToDevice, st_));
-    uint32_t* d_ops = ops_.ensure<uint32_t>(h_ops_off[ns] + 1);
-    SwOut* d_souts = sel_outs_.ensure<SwOut>(ns + 1);
-    launch_sw_generic(jobs, d_sidx, ns, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
-                      opt.gap_extend, scratch_.ensure<int32_t>(so[ns] + 1), d_so,
-                      trace_.ensure<uint8_t>(toff[ns] + 1), d_toff, d_ops, d_oo, d_souts, true, st_);
-    h_ops.resize(h_ops_off[ns]);
-    DMR_HIP(hipMemcpyAsync(h_souts.data(), d_souts, ns * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(h_ops.data(), d_ops, h_ops_off[ns] * 4, hipMemcpyDeviceToHost, st_));
+    traceback_pass(jobs, list, hj, slen, qs, dbs, opt.gap_open, opt.gap_extend, &h_souts, &h_sops,
+                   &S.traceback_cells);
   }
   stage_event(10);
   DMR_HIP(hipStreamSynchronize(st_));
@@ -528,7 +496,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
           h.gap_openings = o.gap_openings;
           h.gaps = o.gaps;
           h.length = o.length;
-          h.ops.assign(h_ops.begin() + h_ops_off[k], h_ops.begin() + h_ops_off[k] + o.n_ops);
+          h.ops = h_sops[k];
           if (o.score != c.h.score)
             throw Error(-2, "traceback score differs from the score pass");
           result[h.query].push_back(std::move(h));
@@ -556,17 +524,18 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
 // int32 kernel.
 void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
                         const SearchOptions& opt, SwOut* outs, StageStats* S) {
+  for (int c = 0; c < kClasses; ++c) S->ms_class[c] = 0;
   if (!njobs) return;
   last_target_bits_ = target_bits;
   const bool fast = opt.gap_open == 11 && opt.gap_extend == 1 && !force_generic_;
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
   launch_classify(jobs, njobs, qs, dbs, target_bits, keys, idx, st_);
-  const int key_bits = 3 + target_bits + 14;
+  const int key_bits = 4 + target_bits + 14;
   radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
                    njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
                    b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_);
-  // Runs of equal (class, target).
+  // Runs of equal (class, target): their start positions and keys.
   uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
   launch_key_flags(keys, njobs, 14, fl, st_);
   launch_fill_u32(fl + njobs, 1, 0, st_);
@@ -575,26 +544,27 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   const uint64_t nruns = read1(ps + njobs);
   uint32_t* rs = runs_.ensure<uint32_t>(nruns + 1);
   launch_compact_flags(fl, ps, njobs, rs, st_);
+  uint64_t* rk = sw_keys2_.as<uint64_t>();
+  launch_gather_u64(keys, rs, nruns, rk, st_);
   std::vector<uint32_t> h_rs(nruns + 1);
-  DMR_HIP(hipMemcpyAsync(h_rs.data(), rs, nruns * 4, hipMemcpyDeviceToHost, st_));
   std::vector<uint64_t> h_keys(nruns);
-  // keys of the run starts: gather on the host side via a second small copy
+  DMR_HIP(hipMemcpyAsync(h_rs.data(), rs, nruns * 4, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipMemcpyAsync(h_keys.data(), rk, nruns * 8, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
   h_rs[nruns] = (uint32_t)njobs;
-  for (uint64_t r = 0; r < nruns; ++r)
-    DMR_HIP(hipMemcpyAsync(&h_keys[r], keys + h_rs[r], 8, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipStreamSynchronize(st_));
   const int cls_shift = target_bits + 14;
+  // Class ranges in the sorted order; lane pairs for the int16 classes.
+  uint64_t cbeg[kClasses], cend[kClasses];
+  for (int c = 0; c < kClasses; ++c) cbeg[c] = cend[c] = 0;
   std::vector<uint32_t> run_start, run_len, lane_off;
   uint32_t class_lanes[5] = {0, 0, 0, 0, 0}, class_lane0[5] = {0, 0, 0, 0, 0};
-  uint64_t generic_from = njobs;
   uint32_t nl = 0;
   for (uint64_t r = 0; r < nruns; ++r) {
     int cls = (int)(h_keys[r] >> cls_shift);
-    if (cls >= 5 || !fast) {
-      generic_from = std::min<uint64_t>(generic_from, h_rs[r]);
-      continue;
-    }
+    if (cls >= kClasses) cls = kClasses - 1;
+    if (cend[cls] == 0) cbeg[cls] = h_rs[r];
+    cend[cls] = h_rs[r + 1];
+    if (cls >= 5 || !fast) continue;
     if (class_lanes[cls] == 0) class_lane0[cls] = nl;
     uint32_t len = h_rs[r + 1] - h_rs[r];
     run_start.push_back(h_rs[r]);
@@ -603,10 +573,11 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     nl += (len + 1) / 2;
     class_lanes[cls] += (len + 1) / 2;
   }
-  if (!fast) generic_from = 0;
-  uint32_t* nredo = sw_counters_.ensure<uint32_t>(8);
-  launch_fill_u32(nredo, 8, 0, st_);
+  uint32_t* cnt = sw_counters_.ensure<uint32_t>(32);
+  launch_fill_u32(cnt, 32, 0, st_);
   uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
+  uint32_t* fallback = sw_fallback_.ensure<uint32_t>(njobs + 1);
+  const int grid = 2 * cu_count_;
   if (nl) {
     const uint32_t nr = (uint32_t)run_start.size();
     uint32_t* d_rs = sw_runs_.ensure<uint32_t>(3ull * nr + 3);
@@ -615,71 +586,234 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     DMR_HIP(hipMemcpyAsync(d_rs + 2 * nr, lane_off.data(), nr * 4, hipMemcpyHostToDevice, st_));
     uint2* lanes = sw_lanes_.ensure<uint2>(nl + 1);
     launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, lanes, st_);
-    int grid = 2 * cu_count_;
     for (int c = 0; c < 5; ++c) {
       if (!class_lanes[c]) continue;
-      DMR_HIP(hipEventRecord(ev_[16 + 2 * c], st_));
+      DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
       launch_sw16(c, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs, pair_tab_.as<int>(),
-                  matrix32_.as<int8_t>(), nredo + 1 + c, outs, redo, nredo, grid, st_);
-      DMR_HIP(hipEventRecord(ev_[17 + 2 * c], st_));
+                  matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, grid, st_);
+      DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
     }
   }
+  // Wave-per-DP int32 kernel: classes 5-8 (and every width class when the
+  // gap costs have no int16 table).
+  for (int c = 5; c < 9; ++c) {
+    const int K = 1 << (c - 5);
+    std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    if (cend[c] > cbeg[c]) ranges.push_back({cbeg[c], cend[c]});
+    if (!fast)
+      for (int w = 0; w < 5; ++w)
+        if (cend[w] > cbeg[w] && swwave_k_for(sw16_width(w)) == K) ranges.push_back({cbeg[w], cend[w]});
+    uint64_t total = 0;
+    for (auto& rg : ranges) total += rg.second - rg.first;
+    if (!total) continue;
+    uint32_t* list = sw_klist_[c - 5].ensure<uint32_t>(total + 1);
+    uint64_t o = 0;
+    for (auto& rg : ranges) {
+      DMR_HIP(hipMemcpyAsync(list + o, idx + rg.first, (rg.second - rg.first) * 4, hipMemcpyDeviceToDevice, st_));
+      o += rg.second - rg.first;
+    }
+    DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
+    launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
+                   opt.gap_open, opt.gap_extend, cnt + 6 + (c - 5), outs, nullptr, nullptr, nullptr,
+                   nullptr, fallback, cnt + 10, grid, st_);
+    DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
+  }
+  // int16-saturated DPs: re-scored by the int32 wave kernel (widths <= 129).
+  const uint32_t nr16 = read1(cnt);
+  if (nr16)
+    launch_sw_wave(4, false, jobs, redo, nr16, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+                   opt.gap_extend, cnt + 11, outs, nullptr, nullptr, nullptr, nullptr, fallback,
+                   cnt + 10, grid, st_);
+  const uint32_t nfb = read1(cnt + 10);
+  for (int c = 0; c < 9; ++c)
+    if ((c < 5 && nl && class_lanes[c]) || (c >= 5 && swk_launched(c, cbeg, cend, fast)))
+      S->ms_class[c] = stage_ms(12 + 2 * c, 13 + 2 * c);
+  // Row-exact int32 kernel: bands wider than 512 and the DPs whose score may
+  // come from the clamped rows below the band.
+  const uint64_t nwide = cend[9] > cbeg[9] ? cend[9] - cbeg[9] : 0;
+  const uint64_t nlist = nwide + nfb;
   S->sw16_lanes = nl;
-  // Exact int32 path: class-5 jobs (contiguous at the end of the sort) plus
-  // the DPs that saturated int16.
-  const uint64_t ngen = njobs - generic_from;
-  const uint32_t nr16 = read1(nredo);
-  for (int c = 0; c < 6; ++c) S->ms_class[c] = 0;
-  for (int c = 0; c < 5; ++c)
-    if (nl && class_lanes[c]) S->ms_class[c] = stage_ms(16 + 2 * c, 17 + 2 * c);
-  last_generic_from_ = generic_from;
-  S->sw_generic = ngen + nr16;
   S->sw_redo = nr16;
-  const uint64_t nlist = ngen + nr16;
+  S->sw_generic = nlist;
+  last_cbeg_.assign(cbeg, cbeg + kClasses);
+  last_cend_.assign(cend, cend + kClasses);
   if (nlist) {
     uint32_t* list = sw_list_.ensure<uint32_t>(nlist + 1);
-    if (ngen)
-      DMR_HIP(hipMemcpyAsync(list, idx + generic_from, ngen * 4, hipMemcpyDeviceToDevice, st_));
-    if (nr16) DMR_HIP(hipMemcpyAsync(list + ngen, redo, nr16 * 4, hipMemcpyDeviceToDevice, st_));
+    if (nwide) DMR_HIP(hipMemcpyAsync(list, idx + cbeg[9], nwide * 4, hipMemcpyDeviceToDevice, st_));
+    if (nfb) DMR_HIP(hipMemcpyAsync(list + nwide, fallback, nfb * 4, hipMemcpyDeviceToDevice, st_));
     uint32_t* stride = stride_.ensure<uint32_t>(nlist + 1);
     launch_job_geometry(jobs, list, nlist, dbs, nullptr, nullptr, stride, st_);
     uint64_t* soff = scr_off_.ensure<uint64_t>(nlist + 1);
     launch_u32_times(stride, nlist, 4, soff, st_);
     scan_exclusive_u64(soff, soff, nlist + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nlist + 1)), st_);
     const uint64_t scratch_total = read1(soff + nlist);
-    DMR_HIP(hipEventRecord(ev_[26], st_));
+    DMR_HIP(hipEventRecord(ev_[30], st_));
     launch_sw_generic(jobs, list, nlist, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                       opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
                       nullptr, nullptr, nullptr, outs, false, st_, /*scatter=*/true);
-    DMR_HIP(hipEventRecord(ev_[27], st_));
-    DMR_HIP(hipEventSynchronize(ev_[27]));
-    S->ms_class[5] = stage_ms(26, 27);
+    DMR_HIP(hipEventRecord(ev_[31], st_));
+    DMR_HIP(hipEventSynchronize(ev_[31]));
+    S->ms_class[9] = stage_ms(30, 31);
   }
-  for (int c = 0; c < 6; ++c) last_ms_class_[c] = S->ms_class[c];
+  for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S->ms_class[c];
 }
 
-void Engine::last_class_profile(uint64_t jobs[6], uint64_t cells[6], double ms[6]) {
-  for (int c = 0; c < 6; ++c) {
+bool Engine::swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, bool fast) const {
+  if (cend[c] > cbeg[c]) return true;
+  if (fast) return false;
+  for (int w = 0; w < 5; ++w)
+    if (cend[w] > cbeg[w] && swwave_k_for(sw16_width(w)) == (1 << (c - 5))) return true;
+  return false;
+}
+
+// Traceback DPs for jobs[list[k]] (hjobs[k] is the host copy, slen[k] the
+// subject length). Wave-per-DP kernel by band width; the row-exact kernel for
+// bands wider than 512 and for DPs whose best cell may sit in the clamped
+// rows below the band.
+void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& list,
+                            const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen,
+                            SeqSet qs, SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
+                            std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out) {
+  const uint64_t ns = list.size();
+  outs->assign(ns, SwOut{});
+  ops->assign(ns, {});
+  if (!ns) return;
+  std::vector<std::vector<uint64_t>> group(5);  // K = 1, 2, 4, 8; generic
+  for (uint64_t k = 0; k < ns; ++k) {
+    const SwJob& j = hjobs[k];
+    int K = swwave_k_for((int64_t)j.d1 - j.d0 + 1);
+    int g = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4;
+    group[g].push_back(k);
+  }
+  uint32_t* cnt = tb_counters_.ensure<uint32_t>(16);
+  launch_fill_u32(cnt, 16, 0, st_);
+  const int grid = 2 * cu_count_;
+  std::vector<uint64_t> fallback_items;
+  uint64_t cells = 0;
+  for (int g = 0; g < 4; ++g) {
+    const auto& items = group[g];
+    if (items.empty()) continue;
+    const uint64_t ni = items.size();
+    std::vector<uint32_t> jl(ni);
+    std::vector<uint64_t> toff(ni + 1, 0), ooff(ni + 1, 0);
+    for (uint64_t t = 0; t < ni; ++t) {
+      const SwJob& j = hjobs[items[t]];
+      jl[t] = list[items[t]];
+      const int64_t n = (int64_t)j.q1 - j.q0, m = slen[items[t]];
+      const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
+      const int64_t W = (int64_t)j.d1 - j.d0 + 1;
+      toff[t + 1] = toff[t] + (uint64_t)(rhi >= rlo ? (rhi - rlo + 1) * W : 0);
+      ooff[t + 1] = ooff[t] + (uint64_t)(n + m + 1);
+      cells += (uint64_t)(rhi >= rlo ? (rhi - rlo + 1) * W : 0);
+    }
+    uint32_t* dl = tb_list_[g].ensure<uint32_t>(ni + 1);
+    uint64_t* dt = tb_toff_[g].ensure<uint64_t>(ni + 1);
+    uint64_t* dop = tb_ooff_[g].ensure<uint64_t>(ni + 1);
+    DMR_HIP(hipMemcpyAsync(dl, jl.data(), ni * 4, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dt, toff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dop, ooff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, st_));
+    uint8_t* tr = tb_trace_[g].ensure<uint8_t>(toff[ni] + 1);
+    uint32_t* dops = tb_ops_[g].ensure<uint32_t>(ooff[ni] + 1);
+    SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
+    uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
+    launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
+                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + 8 + g, grid, st_);
+    tb_host_ooff_[g] = std::move(ooff);
+  }
+  DMR_HIP(hipStreamSynchronize(st_));
+  uint32_t hfb[4] = {0, 0, 0, 0};
+  DMR_HIP(hipMemcpy(hfb, cnt + 8, 16, hipMemcpyDeviceToHost));
+  for (int g = 0; g < 4; ++g) {
+    const auto& items = group[g];
+    if (items.empty()) continue;
+    const uint64_t ni = items.size();
+    std::vector<SwOut> ho(ni);
+    std::vector<uint32_t> hops(tb_host_ooff_[g][ni]);
+    DMR_HIP(hipMemcpyAsync(ho.data(), tb_out_[g].as<SwOut>(), ni * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hops.data(), tb_ops_[g].as<uint32_t>(), hops.size() * 4, hipMemcpyDeviceToHost, st_));
+    std::vector<uint32_t> fbt(hfb[g]);
+    if (hfb[g]) DMR_HIP(hipMemcpyAsync(fbt.data(), tb_fb_[g].as<uint32_t>(), hfb[g] * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    std::vector<char> is_fb(ni, 0);
+    for (uint32_t t : fbt) {
+      is_fb[t] = 1;
+      fallback_items.push_back(items[t]);
+    }
+    for (uint64_t t = 0; t < ni; ++t) {
+      if (is_fb[t]) continue;
+      (*outs)[items[t]] = ho[t];
+      const uint64_t o = tb_host_ooff_[g][t];
+      (*ops)[items[t]].assign(hops.begin() + o, hops.begin() + o + ho[t].n_ops);
+    }
+  }
+  for (uint64_t k : group[4]) fallback_items.push_back(k);
+  if (!fallback_items.empty()) {
+    const uint64_t nf = fallback_items.size();
+    std::vector<uint32_t> jl(nf);
+    for (uint64_t t = 0; t < nf; ++t) jl[t] = list[fallback_items[t]];
+    uint32_t* dl = tb_list_[4].ensure<uint32_t>(nf + 1);
+    DMR_HIP(hipMemcpyAsync(dl, jl.data(), nf * 4, hipMemcpyHostToDevice, st_));
+    uint64_t* dc = cells_.ensure<uint64_t>(nf + 1);
+    uint64_t* dw = width0_.ensure<uint64_t>(nf + 1);
+    uint32_t* dsr = stride_.ensure<uint32_t>(nf + 1);
+    launch_job_geometry(djobs, dl, nf, dbs, dc, dw, dsr, st_);
+    std::vector<uint64_t> hc(nf), hw(nf);
+    std::vector<uint32_t> hs(nf);
+    DMR_HIP(hipMemcpyAsync(hc.data(), dc, nf * 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hw.data(), dw, nf * 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hs.data(), dsr, nf * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    std::vector<uint64_t> toff(nf + 1, 0), so(nf + 1, 0), ooff(nf + 1, 0);
+    for (uint64_t t = 0; t < nf; ++t) {
+      const SwJob& j = hjobs[fallback_items[t]];
+      toff[t + 1] = toff[t] + hc[t] + hw[t];
+      so[t + 1] = so[t] + 4ull * hs[t];
+      ooff[t + 1] = ooff[t] + (uint64_t)(j.q1 - j.q0) + slen[fallback_items[t]] + 1;
+      cells += hc[t];
+    }
+    uint64_t* dto = tb_toff_[4].ensure<uint64_t>(nf + 1);
+    uint64_t* dso = scr_off_.ensure<uint64_t>(nf + 1);
+    uint64_t* doo = tb_ooff_[4].ensure<uint64_t>(nf + 1);
+    DMR_HIP(hipMemcpyAsync(dto, toff.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dso, so.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(doo, ooff.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st_));
+    uint32_t* dops = tb_ops_[4].ensure<uint32_t>(ooff[nf] + 1);
+    SwOut* dout = tb_out_[4].ensure<SwOut>(nf + 1);
+    launch_sw_generic(djobs, dl, nf, qs, dbs, matrix32_.as<int8_t>(), go, ge,
+                      scratch_.ensure<int32_t>(so[nf] + 1), dso, tb_trace_[4].ensure<uint8_t>(toff[nf] + 1),
+                      dto, dops, doo, dout, true, st_);
+    std::vector<SwOut> ho(nf);
+    std::vector<uint32_t> hops(ooff[nf]);
+    DMR_HIP(hipMemcpyAsync(ho.data(), dout, nf * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hops.data(), dops, ooff[nf] * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    for (uint64_t t = 0; t < nf; ++t) {
+      (*outs)[fallback_items[t]] = ho[t];
+      (*ops)[fallback_items[t]].assign(hops.begin() + ooff[t], hops.begin() + ooff[t] + ho[t].n_ops);
+    }
+  }
+  if (cells_out) *cells_out = cells;
+}
+
+void Engine::last_class_profile(uint64_t jobs[kClasses], uint64_t cells[kClasses],
+                                double ms[kClasses]) {
+  for (int c = 0; c < kClasses; ++c) {
     jobs[c] = cells[c] = 0;
     ms[c] = last_ms_class_[c];
   }
   const uint64_t n = last_njobs_;
-  if (!n) return;
+  if (!n || last_cbeg_.size() != (size_t)kClasses) return;
   SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
   uint64_t* c = cells_.ensure<uint64_t>(n + 1);
   launch_job_geometry(jobs_.as<SwJob>(), sw_idx_.as<uint32_t>(), n, dbs, c, nullptr, nullptr, st_);
-  std::vector<uint64_t> hc(n), hk(n);
+  std::vector<uint64_t> hc(n);
   DMR_HIP(hipMemcpyAsync(hc.data(), c, n * 8, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipMemcpyAsync(hk.data(), sw_keys_.as<uint64_t>(), n * 8, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
-  const int shift = last_target_bits_ + 14;
-  for (uint64_t k = 0; k < n; ++k) {
-    int cls = (int)(hk[k] >> shift);
-    if (k >= last_generic_from_) cls = 5;
-    if (cls > 5) cls = 5;
-    jobs[cls]++;
-    cells[cls] += hc[k];
-  }
+  for (int k = 0; k < kClasses; ++k)
+    for (uint64_t p = last_cbeg_[k]; p < last_cend_[k]; ++p) {
+      jobs[k]++;
+      cells[k] += hc[p];
+    }
 }
 
 uint64_t Engine::count_last_dp_cells() {
@@ -915,10 +1049,17 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
   SwOut* o = dout.ensure<SwOut>(njobs);
   uint32_t* opsd = dops.ensure<uint32_t>(ooff[njobs] + 1);
   if (traceback) {
-    launch_sw_generic(djobs.as<SwJob>(), nullptr, njobs, qs, ss, matrix32_.as<int8_t>(), gap_open,
-                      gap_extend, dscr.ensure<int32_t>(soff[njobs] + 1), dsoff.as<uint64_t>(),
-                      dtr.ensure<uint8_t>(toff[njobs] + 1), dtoff.as<uint64_t>(), opsd,
-                      doo.as<uint64_t>(), o, true, st_);
+    std::vector<uint32_t> list(njobs), slen(njobs);
+    for (uint64_t i = 0; i < njobs; ++i) {
+      list[i] = (uint32_t)i;
+      slen[i] = (uint32_t)jobs[i].slen;
+    }
+    std::vector<SwOut> ho;
+    std::vector<std::vector<uint32_t>> hops;
+    traceback_pass(djobs.as<SwJob>(), list, dj, slen, qs, ss, gap_open, gap_extend, &ho, &hops, nullptr);
+    *outs = ho;
+    if (ops) *ops = hops;
+    return;
   } else {
     // Score only: the same score pass as the batched search (packed int16
     // kernel for the standard band widths, exact int32 for the rest).

@@ -84,7 +84,9 @@ struct StageStats {
   uint64_t query_seeds = 0, runs = 0, hits = 0, hit_runs = 0, segments = 0, groups = 0;
   uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
   uint64_t sw16_lanes = 0, sw_generic = 0, sw_redo = 0;
-  double ms_class[6] = {0, 0, 0, 0, 0, 0};  // score kernel time per width class (5 = int32)
+  // Score-kernel time per class: 0-4 packed int16 (widths 25/33/61/81/129),
+  // 5-8 wave-per-DP int32 (K = 1/2/4/8), 9 row-exact int32.
+  double ms_class[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
 };
@@ -126,9 +128,15 @@ class Engine {
   std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats);
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
-  // Per width class (25, 33, 61, 81, 129, other/int32) of the last search:
-  // jobs, DP cells and score-kernel milliseconds.
-  void last_class_profile(uint64_t jobs[6], uint64_t cells[6], double ms[6]);
+  static constexpr int kClasses = 10;
+  // Per score-kernel class of the last search (see StageStats::ms_class):
+  // jobs, DP cells and kernel milliseconds.
+  void last_class_profile(uint64_t jobs[kClasses], uint64_t cells[kClasses], double ms[kClasses]);
+  // Traceback DPs for djobs[list[k]] (host copies hjobs[k], subject lengths slen[k]).
+  void traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& list,
+                      const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
+                      SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
+                      std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out);
 
   // --- drop-in calls on host buffers -------------------------------------
   void seed_build(const uint8_t* seqs, const uint64_t* offs, uint32_t nseq, const ShapeDef& shape,
@@ -158,6 +166,7 @@ class Engine {
   };
   void upload_seed_params(const std::string& sens, double cut, uint32_t* nshapes, int* key_bits,
                           int* shape_bits);
+  bool swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, bool fast) const;
   void score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
                   const SearchOptions& opt, SwOut* outs, StageStats* S);
   void stage_event(int k);
@@ -207,12 +216,15 @@ class Engine {
   DevBuf stride_, scr_off_, scratch_, outs_;
   DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
   DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_runs_, sw_lanes_, sw_counters_, sw_redo_, sw_list_;
-  DevBuf pair_tab_;
+  DevBuf pair_tab_, sw_fallback_, sw_klist_[4];
+  DevBuf tb_counters_, tb_list_[5], tb_toff_[5], tb_ooff_[5], tb_trace_[5], tb_ops_[5], tb_out_[5], tb_fb_[5];
+  std::vector<uint64_t> tb_host_ooff_[5];
+  std::vector<uint64_t> last_cbeg_, last_cend_;
   int cu_count_ = 256;
   bool force_generic_ = false;
-  uint64_t last_njobs_ = 0, last_generic_from_ = 0;
+  uint64_t last_njobs_ = 0;
   int last_target_bits_ = 0;
-  double last_ms_class_[6] = {0, 0, 0, 0, 0, 0};
+  double last_ms_class_[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
  public:
   StageStats last_dropin_stats_;

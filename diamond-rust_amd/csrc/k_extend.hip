@@ -85,6 +85,12 @@ __global__ void k_gather(const SwJob* __restrict__ jobs, const SwOut* __restrict
   oout[i] = outs[idx[i]];
 }
 
+__global__ void k_gather_u64(const uint64_t* __restrict__ in, const uint32_t* __restrict__ idx,
+                             uint64_t n, uint64_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -268,6 +274,12 @@ inline unsigned nblk(uint64_t n, unsigned t = 256) { return (unsigned)((n + t - 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
   if (!n) return;
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n)), dim3(256), 0, st, p, n, v);
+}
+
+void launch_gather_u64(const uint64_t* in, const uint32_t* idx, uint64_t n, uint64_t* out,
+                       hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(256), 0, st, in, idx, n, out);
 }
 
 void launch_gather(const SwJob* jobs, const SwOut* outs, const uint32_t* idx, uint64_t n,
