@@ -25,6 +25,9 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
     # Packed FP32 instructions are kept out of every kernel.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Wave-wide atomics with per-lane values (task fetches) use a DPP scan
+    # instead of a 64-step scalar loop.
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
     f"-I{CSRC}", f"-I{HERE.parent / 'include'}",
 ]

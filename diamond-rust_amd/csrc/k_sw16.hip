@@ -89,9 +89,10 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
   const v2s OPEN2 = {kOpen, kOpen};
   const v2s ZERO2 = {0, 0};
   for (;;) {
-    uint32_t task = 0;
-    if (lane == 0) task = atomicAdd(counter, 1u);
-    task = __shfl(task, 0, 64);
+    // Branch-free task fetch (every lane joins the atomic, lane 0 adds 1):
+    // the task lives in a scalar register and the loop exit is uniform.
+    const uint32_t task =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
     if ((uint64_t)task * 64 >= nlanes) break;
     const uint32_t L = task * 64 + lane;
     const bool active = L < nlanes;
@@ -118,7 +119,8 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
       rlo = min(rlo, max(1, d0B + 1));
       rhi = max(rhi, min(n, istarB));
     }
-    const int lo = wave_min(rlo), hi = wave_max(rhi);
+    const int lo = __builtin_amdgcn_readfirstlane(wave_min(rlo));
+    const int hi = __builtin_amdgcn_readfirstlane(wave_max(rhi));
 
     // Window of (sA, sB) letter pairs for the current row, as byte offsets
     // into a score-table row, two 16-bit entries per register: entry t is

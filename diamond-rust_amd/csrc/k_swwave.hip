@@ -49,6 +49,12 @@ __device__ __forceinline__ int scan_max(int v, int lane) {
   return v;
 }
 
+// Marks a value that is equal in every lane as wave-uniform.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 struct RowRange {
   int d0, d1, m;
   __device__ __forceinline__ int lo(int i) const { return min(max(i - d1, 0), m); }
@@ -73,17 +79,24 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
   const int lane = threadIdx.x & 63;
   const int oc = gap_open + gap_extend, ge = gap_extend;
   for (;;) {
-    uint32_t task = 0;
-    if (lane == 0) task = atomicAdd(counter, 1u);
-    task = __shfl(task, 0, 64);
+    // Task fetch without a lane-divergent branch: every lane takes part in
+    // the atomic and only lane 0 adds 1, so lane 0's old value is this
+    // wave's task. The task and everything derived from it are held in
+    // scalar registers, which keeps the loop's exit and the row loop uniform.
+    // (A `lane == 0` branch here, correlated with the one that writes the
+    // result, lets the optimizer thread lanes 1-63 from the end of one
+    // iteration into the next while lane 0 is inactive.)
+    const uint32_t task = uniform(atomicAdd(counter, lane == 0 ? 1u : 0u));
     if (task >= njobs) break;
-    const uint32_t jid = list[task];
-    const SwJob J = jobs[jid];
-    const uint8_t* qs = q.res + q.offs[J.target] + J.q0;
-    const int n = (int)(J.q1 - J.q0);
-    const uint8_t* ss = db.res + db.offs[J.subject];
-    const int m = (int)(db.offs[J.subject + 1] - db.offs[J.subject]);
-    const int d0 = J.d0, d1 = J.d1;
+    const uint32_t jid = uniform(list[task]);
+    const uint32_t tgt = uniform(jobs[jid].target), sub = uniform(jobs[jid].subject);
+    const uint32_t q0 = uniform(jobs[jid].q0), q1 = uniform(jobs[jid].q1);
+    const int d0 = uniform(jobs[jid].d0), d1 = uniform(jobs[jid].d1);
+    const uint8_t* qs = q.res + uniform(q.offs[tgt]) + q0;
+    const int n = (int)(q1 - q0);
+    const uint32_t sb = uniform(db.offs[sub]);
+    const uint8_t* ss = db.res + sb;
+    const int m = (int)(uniform(db.offs[sub + 1]) - sb);
     const int W = d1 - d0 + 1;
     const int rlo = max(1, d0 + 1), rhi = min(n, m + d1);
     const int istar = m + d1;
@@ -190,13 +203,13 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
               (uint8_t)(choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3));
       }
       if (kTrace) {
-        int rb = wmax(rowbest);
+        const int rb = uniform(wmax(rowbest));
         if (rb > best) {
           int tf = 1 << 30;
 #pragma unroll
           for (int k = K - 1; k >= 0; --k)
             if (valid[k] && Hp[k] == rb) tf = lane * K + k;
-          tf = wmin(tf);
+          tf = uniform(wmin(tf));
           best = rb;
           bi = i;
           bj = i - d1 + tf;
@@ -205,7 +218,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         lbest = max(lbest, rowbest);
       }
     }
-    if (!kTrace) best = wmax(lbest);
+    if (!kTrace) best = uniform(wmax(lbest));
     // Clamped rows below the band (single cell at column m) can only matter
     // when the best score is below 11: leave those DPs to the row-exact kernel.
     const bool need_exact = best < 11 && istar < n;
