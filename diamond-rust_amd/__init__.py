@@ -37,6 +37,10 @@ EXPORTS = [
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy",
 ]
 
+# Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
+SCORE_CLASSES = ["w25", "w33", "w61", "w81", "w129", "wave64", "wave128", "wave256", "wave512",
+                 "wave1024", "wave2048", "rowexact"]
+
 HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
               "identities", "positives", "mismatches", "gap_openings", "gaps", "length", "n_ops"]
 
@@ -312,14 +316,13 @@ class Context:
 
     def class_profile(self):
         """Per score-kernel class of the last search: packed int16 by band width
-        (w25..w129), wave-per-DP int32 by width bound (wave64..wave512), and the
+        (w25..w129), wave-per-DP int32 by width bound (wave64..wave2048), and the
         row-exact int32 kernel: jobs, DP cells, kernel ms."""
-        j = np.zeros(10, np.uint64)
-        c = np.zeros(10, np.uint64)
-        m = np.zeros(10, np.float64)
+        names = SCORE_CLASSES
+        j = np.zeros(len(names), np.uint64)
+        c = np.zeros(len(names), np.uint64)
+        m = np.zeros(len(names), np.float64)
         _check(lib().dmr_last_class_profile(self._h, j.ctypes.data, c.ctypes.data, m.ctypes.data))
-        names = ["w25", "w33", "w61", "w81", "w129", "wave64", "wave128", "wave256", "wave512",
-                 "rowexact"]
         return {n: dict(jobs=int(j[i]), cells=int(c[i]), ms=float(m[i])) for i, n in enumerate(names)}
 
     def last_dp_cells(self) -> int:

@@ -704,10 +704,11 @@ int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, c
   return DMR_OK;
 }
 
-int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs10, uint64_t* cells10, double* ms10) {
-  if (!ctx || !jobs10 || !cells10 || !ms10) return fail(DMR_E_ARG, "NULL argument");
+int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms) {
+  static_assert(DMR_SCORE_CLASSES == dmr::Engine::kClasses, "class count");
+  if (!ctx || !jobs || !cells || !ms) return fail(DMR_E_ARG, "NULL argument");
   return guard([&]() -> int {
-    ctx->eng->last_class_profile(jobs10, cells10, ms10);
+    ctx->eng->last_class_profile(jobs, cells, ms);
     return DMR_OK;
   });
 }

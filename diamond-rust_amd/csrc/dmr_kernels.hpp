@@ -86,7 +86,10 @@ void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, 
 void launch_u32_times(const uint32_t* in, uint64_t n, uint32_t mul, uint64_t* out, hipStream_t st);
 void launch_gather_u64(const uint64_t* in, const uint32_t* idx, uint64_t n, uint64_t* out, hipStream_t st);
 
-// Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width <= 64*K.
+// Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width
+// <= 64*K, K a power of two up to kMaxWaveK (0 = wider than 64*kMaxWaveK).
+constexpr int kMaxWaveK = 32;
+constexpr int kWaveKinds = 6;  // K = 1, 2, 4, 8, 16, 32
 int swwave_k_for(int64_t width);
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,

@@ -594,9 +594,9 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
       DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
     }
   }
-  // Wave-per-DP int32 kernel: classes 5-8 (and every width class when the
+  // Wave-per-DP int32 kernel: classes 5-10 (and every width class when the
   // gap costs have no int16 table).
-  for (int c = 5; c < 9; ++c) {
+  for (int c = 5; c < 5 + kWaveKinds; ++c) {
     const int K = 1 << (c - 5);
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     if (cend[c] > cbeg[c]) ranges.push_back({cbeg[c], cend[c]});
@@ -614,23 +614,23 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     }
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
-                   opt.gap_open, opt.gap_extend, cnt + 6 + (c - 5), outs, nullptr, nullptr, nullptr,
-                   nullptr, fallback, cnt + 10, grid, st_);
+                   opt.gap_open, opt.gap_extend, cnt + 8 + (c - 5), outs, nullptr, nullptr, nullptr,
+                   nullptr, fallback, cnt + 16, grid, st_);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
   }
   // int16-saturated DPs: re-scored by the int32 wave kernel (widths <= 129).
   const uint32_t nr16 = read1(cnt);
   if (nr16)
     launch_sw_wave(4, false, jobs, redo, nr16, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
-                   opt.gap_extend, cnt + 11, outs, nullptr, nullptr, nullptr, nullptr, fallback,
-                   cnt + 10, grid, st_);
-  const uint32_t nfb = read1(cnt + 10);
-  for (int c = 0; c < 9; ++c)
+                   opt.gap_extend, cnt + 17, outs, nullptr, nullptr, nullptr, nullptr, fallback,
+                   cnt + 16, grid, st_);
+  const uint32_t nfb = read1(cnt + 16);
+  for (int c = 0; c < kRowExact; ++c)
     if ((c < 5 && nl && class_lanes[c]) || (c >= 5 && swk_launched(c, cbeg, cend, fast)))
       S->ms_class[c] = stage_ms(12 + 2 * c, 13 + 2 * c);
-  // Row-exact int32 kernel: bands wider than 512 and the DPs whose score may
-  // come from the clamped rows below the band.
-  const uint64_t nwide = cend[9] > cbeg[9] ? cend[9] - cbeg[9] : 0;
+  // Row-exact int32 kernel: bands wider than 64 * kMaxWaveK and the DPs whose
+  // band never reaches the last in-band row but whose clamped rows may score.
+  const uint64_t nwide = cend[kRowExact] > cbeg[kRowExact] ? cend[kRowExact] - cbeg[kRowExact] : 0;
   const uint64_t nlist = nwide + nfb;
   S->sw16_lanes = nl;
   S->sw_redo = nr16;
@@ -639,7 +639,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   last_cend_.assign(cend, cend + kClasses);
   if (nlist) {
     uint32_t* list = sw_list_.ensure<uint32_t>(nlist + 1);
-    if (nwide) DMR_HIP(hipMemcpyAsync(list, idx + cbeg[9], nwide * 4, hipMemcpyDeviceToDevice, st_));
+    if (nwide) DMR_HIP(hipMemcpyAsync(list, idx + cbeg[kRowExact], nwide * 4, hipMemcpyDeviceToDevice, st_));
     if (nfb) DMR_HIP(hipMemcpyAsync(list + nwide, fallback, nfb * 4, hipMemcpyDeviceToDevice, st_));
     uint32_t* stride = stride_.ensure<uint32_t>(nlist + 1);
     launch_job_geometry(jobs, list, nlist, dbs, nullptr, nullptr, stride, st_);
@@ -647,13 +647,13 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     launch_u32_times(stride, nlist, 4, soff, st_);
     scan_exclusive_u64(soff, soff, nlist + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nlist + 1)), st_);
     const uint64_t scratch_total = read1(soff + nlist);
-    DMR_HIP(hipEventRecord(ev_[30], st_));
+    DMR_HIP(hipEventRecord(ev_[12 + 2 * kRowExact], st_));
     launch_sw_generic(jobs, list, nlist, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                       opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
                       nullptr, nullptr, nullptr, outs, false, st_, /*scatter=*/true);
-    DMR_HIP(hipEventRecord(ev_[31], st_));
-    DMR_HIP(hipEventSynchronize(ev_[31]));
-    S->ms_class[9] = stage_ms(30, 31);
+    DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
+    DMR_HIP(hipEventSynchronize(ev_[13 + 2 * kRowExact]));
+    S->ms_class[kRowExact] = stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
   }
   for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S->ms_class[c];
 }
@@ -678,19 +678,21 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
   outs->assign(ns, SwOut{});
   ops->assign(ns, {});
   if (!ns) return;
-  std::vector<std::vector<uint64_t>> group(5);  // K = 1, 2, 4, 8; generic
+  // Group g < kWaveKinds: wave kernel with K = 2^g; the last group: row-exact.
+  std::vector<std::vector<uint64_t>> group(kTbGroups);
   for (uint64_t k = 0; k < ns; ++k) {
     const SwJob& j = hjobs[k];
-    int K = swwave_k_for((int64_t)j.d1 - j.d0 + 1);
-    int g = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4;
+    const int K = swwave_k_for((int64_t)j.d1 - j.d0 + 1);
+    int g = kWaveKinds;
+    if (K) g = __builtin_ctz((unsigned)K);
     group[g].push_back(k);
   }
-  uint32_t* cnt = tb_counters_.ensure<uint32_t>(16);
-  launch_fill_u32(cnt, 16, 0, st_);
+  uint32_t* cnt = tb_counters_.ensure<uint32_t>(2 * kWaveKinds);
+  launch_fill_u32(cnt, 2 * kWaveKinds, 0, st_);
   const int grid = 2 * cu_count_;
   std::vector<uint64_t> fallback_items;
   uint64_t cells = 0;
-  for (int g = 0; g < 4; ++g) {
+  for (int g = 0; g < kWaveKinds; ++g) {
     const auto& items = group[g];
     if (items.empty()) continue;
     const uint64_t ni = items.size();
@@ -717,13 +719,13 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
     SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + 8 + g, grid, st_);
+                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, grid, st_);
     tb_host_ooff_[g] = std::move(ooff);
   }
   DMR_HIP(hipStreamSynchronize(st_));
-  uint32_t hfb[4] = {0, 0, 0, 0};
-  DMR_HIP(hipMemcpy(hfb, cnt + 8, 16, hipMemcpyDeviceToHost));
-  for (int g = 0; g < 4; ++g) {
+  uint32_t hfb[kWaveKinds] = {};
+  DMR_HIP(hipMemcpy(hfb, cnt + kWaveKinds, sizeof(hfb), hipMemcpyDeviceToHost));
+  for (int g = 0; g < kWaveKinds; ++g) {
     const auto& items = group[g];
     if (items.empty()) continue;
     const uint64_t ni = items.size();
@@ -746,12 +748,12 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
       (*ops)[items[t]].assign(hops.begin() + o, hops.begin() + o + ho[t].n_ops);
     }
   }
-  for (uint64_t k : group[4]) fallback_items.push_back(k);
+  for (uint64_t k : group[kWaveKinds]) fallback_items.push_back(k);
   if (!fallback_items.empty()) {
     const uint64_t nf = fallback_items.size();
     std::vector<uint32_t> jl(nf);
     for (uint64_t t = 0; t < nf; ++t) jl[t] = list[fallback_items[t]];
-    uint32_t* dl = tb_list_[4].ensure<uint32_t>(nf + 1);
+    uint32_t* dl = tb_list_[kWaveKinds].ensure<uint32_t>(nf + 1);
     DMR_HIP(hipMemcpyAsync(dl, jl.data(), nf * 4, hipMemcpyHostToDevice, st_));
     uint64_t* dc = cells_.ensure<uint64_t>(nf + 1);
     uint64_t* dw = width0_.ensure<uint64_t>(nf + 1);
@@ -771,16 +773,16 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
       ooff[t + 1] = ooff[t] + (uint64_t)(j.q1 - j.q0) + slen[fallback_items[t]] + 1;
       cells += hc[t];
     }
-    uint64_t* dto = tb_toff_[4].ensure<uint64_t>(nf + 1);
+    uint64_t* dto = tb_toff_[kWaveKinds].ensure<uint64_t>(nf + 1);
     uint64_t* dso = scr_off_.ensure<uint64_t>(nf + 1);
-    uint64_t* doo = tb_ooff_[4].ensure<uint64_t>(nf + 1);
+    uint64_t* doo = tb_ooff_[kWaveKinds].ensure<uint64_t>(nf + 1);
     DMR_HIP(hipMemcpyAsync(dto, toff.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st_));
     DMR_HIP(hipMemcpyAsync(dso, so.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st_));
     DMR_HIP(hipMemcpyAsync(doo, ooff.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st_));
-    uint32_t* dops = tb_ops_[4].ensure<uint32_t>(ooff[nf] + 1);
-    SwOut* dout = tb_out_[4].ensure<SwOut>(nf + 1);
+    uint32_t* dops = tb_ops_[kWaveKinds].ensure<uint32_t>(ooff[nf] + 1);
+    SwOut* dout = tb_out_[kWaveKinds].ensure<SwOut>(nf + 1);
     launch_sw_generic(djobs, dl, nf, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                      scratch_.ensure<int32_t>(so[nf] + 1), dso, tb_trace_[4].ensure<uint8_t>(toff[nf] + 1),
+                      scratch_.ensure<int32_t>(so[nf] + 1), dso, tb_trace_[kWaveKinds].ensure<uint8_t>(toff[nf] + 1),
                       dto, dops, doo, dout, true, st_);
     std::vector<SwOut> ho(nf);
     std::vector<uint32_t> hops(ooff[nf]);

@@ -85,8 +85,8 @@ struct StageStats {
   uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
   uint64_t sw16_lanes = 0, sw_generic = 0, sw_redo = 0;
   // Score-kernel time per class: 0-4 packed int16 (widths 25/33/61/81/129),
-  // 5-8 wave-per-DP int32 (K = 1/2/4/8), 9 row-exact int32.
-  double ms_class[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // 5-10 wave-per-DP int32 (K = 1/2/4/8/16/32), 11 row-exact int32.
+  double ms_class[12] = {};
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
 };
@@ -128,7 +128,8 @@ class Engine {
   std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats);
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
-  static constexpr int kClasses = 10;
+  static constexpr int kClasses = 12;  // k_classify in k_sw16.hip
+  static constexpr int kRowExact = kClasses - 1;
   // Per score-kernel class of the last search (see StageStats::ms_class):
   // jobs, DP cells and kernel milliseconds.
   void last_class_profile(uint64_t jobs[kClasses], uint64_t cells[kClasses], double ms[kClasses]);
@@ -176,7 +177,7 @@ class Engine {
 
   int device_ = 0;
   hipStream_t st_ = nullptr;
-  hipEvent_t ev_[32] = {};
+  hipEvent_t ev_[40] = {};
 
   // database
   DevBuf db_res_, db_offs_, db_seq_of_;
@@ -216,15 +217,18 @@ class Engine {
   DevBuf stride_, scr_off_, scratch_, outs_;
   DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
   DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_runs_, sw_lanes_, sw_counters_, sw_redo_, sw_list_;
-  DevBuf pair_tab_, sw_fallback_, sw_klist_[4];
-  DevBuf tb_counters_, tb_list_[5], tb_toff_[5], tb_ooff_[5], tb_trace_[5], tb_ops_[5], tb_out_[5], tb_fb_[5];
-  std::vector<uint64_t> tb_host_ooff_[5];
+  DevBuf pair_tab_, sw_fallback_, sw_klist_[kWaveKinds];
+  // Traceback groups: one per wave kernel K, then the row-exact kernel.
+  static constexpr int kTbGroups = kWaveKinds + 1;
+  DevBuf tb_counters_, tb_list_[kTbGroups], tb_toff_[kTbGroups], tb_ooff_[kTbGroups],
+      tb_trace_[kTbGroups], tb_ops_[kTbGroups], tb_out_[kTbGroups], tb_fb_[kTbGroups];
+  std::vector<uint64_t> tb_host_ooff_[kTbGroups];
   std::vector<uint64_t> last_cbeg_, last_cend_;
   int cu_count_ = 256;
   bool force_generic_ = false;
   uint64_t last_njobs_ = 0;
   int last_target_bits_ = 0;
-  double last_ms_class_[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double last_ms_class_[kClasses] = {};
 
  public:
   StageStats last_dropin_stats_;

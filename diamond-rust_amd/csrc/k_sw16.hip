@@ -213,9 +213,9 @@ __device__ __forceinline__ int width_class(int w) {
 }
 
 // Sort key per job: class | target | rows. Classes 0-4: packed int16 kernel
-// for band widths 25/33/61/81/129; 5-8: wave-per-DP int32 kernel with
-// K = 1/2/4/8 offsets per lane (widths up to 64/128/256/512); 9: row-exact
-// int32 kernel (wider bands).
+// for band widths 25/33/61/81/129; 5-10: wave-per-DP int32 kernel with
+// K = 1..32 offsets per lane (widths up to 64 * K); 11: row-exact int32
+// kernel (wider bands).
 __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSet q, SeqSet db,
                            int target_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
   uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,7 +230,14 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
   const int64_t istar = m + j.d1;
   const int64_t rhi = istar < n ? istar : n;
   if (j.q0 != 0 || n != qlen || rlo > rhi || istar < 1 || m < 1 || n > 16383 || m > 16383) cls = 5;
-  if (cls == 5) cls = w <= 64 ? 5 : w <= 128 ? 6 : w <= 256 ? 7 : w <= 512 ? 8 : 9;
+  if (cls == 5) {
+    cls = 5 + kWaveKinds;
+    for (int c = 0; c < kWaveKinds; ++c)
+      if (w <= (64ll << c)) {
+        cls = 5 + c;
+        break;
+      }
+  }
   const uint64_t r = (uint64_t)(rhi < 0 ? 0 : (rhi > 16383 ? 16383 : rhi));
   keys[k] = ((uint64_t)cls << (target_bits + 14)) | ((uint64_t)j.target << 14) | r;
   idx[k] = (uint32_t)k;

@@ -1,5 +1,5 @@
 // Banded SW with one DP per wavefront: exact int32, any band width up to
-// 64*K, optional trace bytes and traceback.
+// 64*K (K = 1..32, so up to 2048), optional trace bytes and traceback.
 //
 // Lane l owns band offsets t = l*K .. l*K+K-1 (column j = i - d1 + t of row
 // i). Rows are processed in order; within a row, D and E only read the
@@ -218,10 +218,32 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         lbest = max(lbest, rowbest);
       }
     }
-    if (!kTrace) best = uniform(wmax(lbest));
-    // Clamped rows below the band (single cell at column m) can only matter
-    // when the best score is below 11: leave those DPs to the row-exact kernel.
-    const bool need_exact = best < 11 && istar < n;
+    // Clamped rows below the band: rows istar+1..n hold the single cell
+    // (i, m), whose diagonal predecessor lies outside the previous row's
+    // range (it counts as 0) and which has no horizontal source. Their H is
+    // at most max(11, H(istar, m) - o), so they can only raise a best score
+    // below 11, the largest matrix entry. Score mode finishes them here from
+    // H and E of (istar, m), held by offset 0 after row istar; trace mode and
+    // bands that never reach row istar go to the row-exact kernel.
+    bool need_exact = best < 11 && istar < n;
+    if (!kTrace) {
+      best = uniform(wmax(lbest));
+      need_exact = best < 11 && istar < n;
+      if (need_exact && m >= 1 && istar >= 1 && rlo <= rhi) {
+        int h = uniform(Hp[0]), e = uniform(Ep[0]);
+        const int sl = ss[m - 1];
+        for (int i = istar + 1; i <= n; ++i) {
+          int ev = kNeg;
+          if (h > kNeg / 2 && h - oc > ev) ev = h - oc;
+          if (e > kNeg / 2 && e - ge > ev) ev = e - ge;
+          const int hv = max(0, max((int)M[qs[i - 1] * kScoreStride + sl], ev));
+          best = max(best, hv);
+          h = hv;
+          e = ev;
+        }
+        need_exact = false;
+      }
+    }
     if (lane == 0) {
       if (need_exact) {
         fallback[atomicAdd(nfallback, 1u)] = kTrace ? task : jid;
@@ -310,10 +332,8 @@ inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t
 }  // namespace
 
 int swwave_k_for(int64_t width) {
-  if (width <= 64) return 1;
-  if (width <= 128) return 2;
-  if (width <= 256) return 4;
-  if (width <= 512) return 8;
+  for (int K = 1; K <= kMaxWaveK; K <<= 1)
+    if (width <= 64 * K) return K;
   return 0;
 }
 
@@ -334,6 +354,8 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
       case 2: DMR_SWW(2, true); break;
       case 4: DMR_SWW(4, true); break;
       case 8: DMR_SWW(8, true); break;
+      case 16: DMR_SWW(16, true); break;
+      case 32: DMR_SWW(32, true); break;
       default: break;
     }
   } else {
@@ -342,6 +364,8 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
       case 2: DMR_SWW(2, false); break;
       case 4: DMR_SWW(4, false); break;
       case 8: DMR_SWW(8, false); break;
+      case 16: DMR_SWW(16, false); break;
+      case 32: DMR_SWW(32, false); break;
       default: break;
     }
   }

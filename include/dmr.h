@@ -211,12 +211,14 @@ int dmr_result_format(const dmr_result* r, const char* query_ids, const char* su
 int dmr_result_stats(const dmr_result* r, double* out20);
 /* Cells of every banded DP of the last search on ctx (for GCUPS). */
 int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
-/* Score-pass profile of the last search on ctx, per kernel class (10
- * entries): 0-4 packed int16 kernel for band widths 25/33/61/81/129, 5-8
- * wave-per-DP int32 kernel for widths up to 64/128/256/512, 9 row-exact int32
- * kernel (wider bands and DPs scored from the clamped rows): DP jobs, DP
- * cells and kernel milliseconds. */
-int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs10, uint64_t* cells10, double* ms10);
+/* Score-pass profile of the last search on ctx, per kernel class
+ * (DMR_SCORE_CLASSES entries): 0-4 packed int16 kernel for band widths
+ * 25/33/61/81/129, 5-10 wave-per-DP int32 kernel for widths up to
+ * 64/128/256/512/1024/2048, 11 row-exact int32 kernel (wider bands, and
+ * bands that end before their clamped rows): DP jobs, DP cells and kernel
+ * milliseconds. Diagnostics only; no counterpart in the reference. */
+#define DMR_SCORE_CLASSES 12
+int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
 void dmr_result_destroy(dmr_result* r);
 
 #ifdef __cplusplus

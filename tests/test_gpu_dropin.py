@@ -39,7 +39,7 @@ def span_cases(seed, n_cases):
         m = len(s)
         # Bands inside, across and entirely outside the matrix (clamped rows).
         c = int(rng.integers(-m - 20, n + 20))
-        w = int(rng.choice([0, 1, 5, 12, 30, 64, 200, 2000]))
+        w = int(rng.choice([0, 1, 5, 12, 30, 64, 200, 300, 700, 2000]))
         yield q, s, c - w, c + w + int(rng.integers(0, 3))
 
 
