@@ -79,12 +79,24 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, co
 int sw16_width(int cls);
 void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
                  const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
-                 uint32_t* redo, uint32_t* nredo, int grid, hipStream_t st);
+                 uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);
 void launch_u32_times(const uint32_t* in, uint64_t n, uint32_t mul, uint64_t* out, hipStream_t st);
 void launch_gather_u64(const uint64_t* in, const uint32_t* idx, uint64_t n, uint64_t* out, hipStream_t st);
+
+// Persistent kernels (k_sw16, k_sw_wave) launch as many blocks as can be
+// resident on cu_count CUs at once (from the occupancy of the instantiation),
+// capped by the work available.
+template <class Kernel>
+unsigned resident_blocks(Kernel kernel, int threads, int cu_count) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (unsigned)per_cu * (unsigned)(cu_count > 0 ? cu_count : 1);
+}
 
 // Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width
 // <= 64*K, K a power of two up to kMaxWaveK (0 = wider than 64*kMaxWaveK).
@@ -95,6 +107,6 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int grid, hipStream_t st);
+                    int cu_count, hipStream_t st);
 
 }  // namespace dmr

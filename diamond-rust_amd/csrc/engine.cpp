@@ -577,7 +577,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   launch_fill_u32(cnt, 32, 0, st_);
   uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
   uint32_t* fallback = sw_fallback_.ensure<uint32_t>(njobs + 1);
-  const int grid = 2 * cu_count_;
+  const int cus = cu_count_;
   if (nl) {
     const uint32_t nr = (uint32_t)run_start.size();
     uint32_t* d_rs = sw_runs_.ensure<uint32_t>(3ull * nr + 3);
@@ -590,7 +590,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
       if (!class_lanes[c]) continue;
       DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
       launch_sw16(c, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs, pair_tab_.as<int>(),
-                  matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, grid, st_);
+                  matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, cus, st_);
       DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
     }
   }
@@ -615,7 +615,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
                    opt.gap_open, opt.gap_extend, cnt + 8 + (c - 5), outs, nullptr, nullptr, nullptr,
-                   nullptr, fallback, cnt + 16, grid, st_);
+                   nullptr, fallback, cnt + 16, cus, st_);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
   }
   // int16-saturated DPs: re-scored by the int32 wave kernel (widths <= 129).
@@ -623,7 +623,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   if (nr16)
     launch_sw_wave(4, false, jobs, redo, nr16, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                    opt.gap_extend, cnt + 17, outs, nullptr, nullptr, nullptr, nullptr, fallback,
-                   cnt + 16, grid, st_);
+                   cnt + 16, cus, st_);
   const uint32_t nfb = read1(cnt + 16);
   for (int c = 0; c < kRowExact; ++c)
     if ((c < 5 && nl && class_lanes[c]) || (c >= 5 && swk_launched(c, cbeg, cend, fast)))
@@ -689,7 +689,7 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
   }
   uint32_t* cnt = tb_counters_.ensure<uint32_t>(2 * kWaveKinds);
   launch_fill_u32(cnt, 2 * kWaveKinds, 0, st_);
-  const int grid = 2 * cu_count_;
+  const int cus = cu_count_;
   std::vector<uint64_t> fallback_items;
   uint64_t cells = 0;
   for (int g = 0; g < kWaveKinds; ++g) {
@@ -719,7 +719,7 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
     SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, grid, st_);
+                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, st_);
     tb_host_ooff_[g] = std::move(ooff);
   }
   DMR_HIP(hipStreamSynchronize(st_));

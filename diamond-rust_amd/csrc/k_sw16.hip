@@ -293,16 +293,17 @@ int sw16_width(int cls) { return kWidths[cls]; }
 
 void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
                  const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
-                 uint32_t* redo, uint32_t* nredo, int grid, hipStream_t st) {
+                 uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!nlanes) return;
   unsigned tasks = nblk(nlanes, 64);
-  unsigned blocks = std::min<unsigned>((unsigned)grid, nblk(tasks, 4));
   switch (cls) {
 #define DMR_SW16(c, w)                                                                              \
-  case c:                                                                                           \
+  case c: {                                                                                         \
+    static const unsigned resident = resident_blocks(k_sw16<w>, 256, cu_count);                    \
+    const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));                           \
     hipLaunchKernelGGL(k_sw16<w>, dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,       \
                        pair_tab, m32, counter, outs, redo, nredo);                                  \
-    break;
+  } break;
     DMR_SW16(0, 25)
     DMR_SW16(1, 33)
     DMR_SW16(2, 61)

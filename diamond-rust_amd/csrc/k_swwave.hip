@@ -29,25 +29,44 @@ namespace {
 constexpr int32_t kNeg = INT32_MIN / 4;
 constexpr int kPad = 25;
 
-__device__ __forceinline__ int wmax(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+// Cross-lane moves in the VALU (DPP, GFX9 encodings) instead of the LDS
+// crossbar: they sit on every row's critical path.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+constexpr int kQuadIdentity = 0xE4;  // quad_perm [0,1,2,3]: a plain VALU copy
+
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ int dpp(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, kCtrl, kRowMask, 0xf, false);
+}
+// Lane l receives lane l+1's v; lane 63 receives `edge`.
+__device__ __forceinline__ int from_next(int v, int edge) { return dpp<kWaveShl1>(edge, v); }
+// Lane l receives lane l-1's v; lane 0 receives `edge`.
+__device__ __forceinline__ int from_prev(int v, int edge) { return dpp<kWaveShr1>(edge, v); }
+// Inclusive prefix maximum / minimum over the 64 lanes. The identity of the
+// operation as the DPP `old` value lets each step fold into one max/min.
+__device__ __forceinline__ int scan_max(int v) {
+  constexpr int kId = INT32_MIN;
+  v = max(v, dpp<kRowShr1>(kId, v));
+  v = max(v, dpp<kRowShr2>(kId, v));
+  v = max(v, dpp<kRowShr4>(kId, v));
+  v = max(v, dpp<kRowShr8>(kId, v));
+  v = max(v, dpp<kRowBcast15, 0xa>(kId, v));
+  v = max(v, dpp<kRowBcast31, 0xc>(kId, v));
   return v;
 }
-__device__ __forceinline__ int wmin(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+__device__ __forceinline__ int scan_min(int v) {
+  constexpr int kId = INT32_MAX;
+  v = min(v, dpp<kRowShr1>(kId, v));
+  v = min(v, dpp<kRowShr2>(kId, v));
+  v = min(v, dpp<kRowShr4>(kId, v));
+  v = min(v, dpp<kRowShr8>(kId, v));
+  v = min(v, dpp<kRowBcast15, 0xa>(kId, v));
+  v = min(v, dpp<kRowBcast31, 0xc>(kId, v));
   return v;
 }
-// Inclusive prefix maximum over the 64 lanes.
-__device__ __forceinline__ int scan_max(int v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int u = __shfl_up(v, o, 64);
-    if (lane >= o) v = max(v, u);
-  }
-  return v;
-}
+__device__ __forceinline__ int wmax(int v) { return __builtin_amdgcn_readlane(scan_max(v), 63); }
+__device__ __forceinline__ int wmin(int v) { return __builtin_amdgcn_readlane(scan_min(v), 63); }
 
 // Marks a value that is equal in every lane as wave-uniform.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -60,6 +79,10 @@ struct RowRange {
   __device__ __forceinline__ int lo(int i) const { return min(max(i - d1, 0), m); }
   __device__ __forceinline__ int hi(int i) const { return max(min(max(i - d0, 0), m), lo(i)); }
 };
+
+__device__ __forceinline__ int letter_at(const uint8_t* ss, int m, int j) {
+  return (j >= 1 && j <= m) ? (int)ss[j - 1] : kPad;
+}
 
 template <int K, bool kTrace>
 __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
@@ -101,7 +124,11 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     const int rlo = max(1, d0 + 1), rhi = min(n, m + d1);
     const int istar = m + d1;
 
-    int Hp[K], Ep[K];
+    // Band offset t = lane*K + k is column j = i - d1 + t of row i. Offsets
+    // past the band (t >= W) hold H = E = NEG: the wall right of the band.
+    // S[k] is the subject letter of column j (padding letter outside 1..m);
+    // it moves one offset left per row, so each row loads one new letter.
+    int Hp[K], Ep[K], S[K];
     bool valid[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -109,107 +136,121 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
       valid[k] = t < W;
       Hp[k] = valid[k] ? 0 : kNeg;
       Ep[k] = kNeg;
+      S[k] = letter_at(ss, m, rlo - d1 + t);
     }
     int best = 0, lbest = 0, bi = 0, bj = 0;
     uint8_t* tr = kTrace ? trace + trace_off[task] : nullptr;
-    for (int i = rlo; i <= rhi; ++i) {
-      const int qrow = qs[i - 1] * kScoreStride;
-      // E sources of the last owned offset: next lane, previous row.
-      int nH = __shfl_down(Hp[0], 1, 64), nE = __shfl_down(Ep[0], 1, 64);
-      if (lane == 63) {
-        nH = kNeg;
-        nE = kNeg;
-      }
+    // Per-row letters in blocks of 64 rows, one row per lane: the query
+    // letter of row i and the subject letter entering the window after row i
+    // (column i + 1 - d1 + 64K - 1). The next block is loaded a block ahead,
+    // so no row waits on memory.
+    // The loads are unconditional (clamped addresses, padding selected
+    // afterwards) and each block is moved out of the load registers by a
+    // VALU copy at the block start, so the row loop itself never waits on
+    // memory. Needs rlo <= rhi and m >= 1, which the row loop implies below.
+    auto qletters = [&](int r0) { return (int)qs[min(r0 + lane, rhi) - 1]; };
+    auto eletters = [&](int r0) {
+      const int j = r0 + lane + 64 * K - d1;
+      const int v = ss[min(max(j, 1), m) - 1];
+      return (j >= 1 && j <= m) ? v : kPad;
+    };
+    const bool rows = rlo <= rhi && m >= 1;
+    int qn = rows ? qletters(rlo) : 0, en = rows ? eletters(rlo) : 0;
+    for (int blk = rlo; rows && blk <= rhi; blk += 64) {
+     const int qv = dpp<kQuadIdentity>(0, qn), ev = dpp<kQuadIdentity>(0, en);
+     qn = qletters(blk + 64);
+     en = eletters(blk + 64);
+     const int rend = min(rhi, blk + 63);
+     for (int i = blk; i <= rend; ++i) {
+      const int qrow = __builtin_amdgcn_readlane(qv, i - blk) * kScoreStride;
+      const int edge = __builtin_amdgcn_readlane(ev, i - blk);
+      // E sources of the last owned offset: the next lane's first offset in
+      // the previous row (NEG beyond lane 63).
+      const int nH = from_next(Hp[0], kNeg), nE = from_next(Ep[0], kNeg);
       int Hq[K], E[K], D[K];
       uint32_t eext = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const int t = lane * K + k;
-        const int j = i - d1 + t;
-        const int s = (j >= 1 && j <= m) ? (int)ss[j - 1] : kPad;
-        D[k] = Hp[k] + M[qrow + s];
+        D[k] = Hp[k] + M[qrow + S[k]];
         const int hs = k + 1 < K ? Hp[k + 1] : nH;
         const int es = k + 1 < K ? Ep[k + 1] : nE;
-        // Reference E: defined only when the source lies in the previous row.
-        int e = kNeg;
-        bool ex = false;
-        if (hs > kNeg / 2 && hs - oc > e) e = hs - oc;
-        if (es > kNeg / 2 && es - ge > e) {
-          e = es - ge;
-          ex = true;
-        }
-        E[k] = e;
-        eext |= (uint32_t)ex << k;
-        Hq[k] = max(0, max(D[k], e));
+        // E opens from H or extends E of (i-1, j); with no source in the
+        // band both terms are below NEG/2 and E stays undefined in effect
+        // (it never wins a max against D >= -64 or 0). The extension bit is
+        // only read on cells whose E is defined.
+        const int eo = hs - oc, ee = es - ge;
+        E[k] = max(eo, ee);
+        eext |= (uint32_t)(ee > eo) << k;
+        Hq[k] = max(max(D[k], E[k]), 0);
       }
-      // Horizontal gaps: exclusive prefix over lanes, then the lane's chain.
-      int L = kNeg;
+      // Horizontal gaps: F(t) = max_{t'<t} H'(t') - o - e*(t-1-t'), with
+      // H' = max(0, D, E). Per lane the best source for the cell after its
+      // last offset, then an exclusive prefix maximum across lanes.
+      int L = Hq[0] - oc - ge * (K - 1);
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        int c = valid[k] ? Hq[k] - oc - ge * (K - 1 - k) : kNeg;
-        L = max(L - 0, c);
-      }
+      for (int k = 1; k < K; ++k) L = max(L, Hq[k] - oc - ge * (K - 1 - k));
       const int stepK = ge * K;
-      int Y = L > kNeg / 2 ? L + stepK * lane : kNeg;
-      int inc = scan_max(Y, lane);
-      int exc = __shfl_up(inc, 1, 64);
-      int Fin = (lane == 0 || exc <= kNeg / 2) ? kNeg : exc - stepK * (lane - 1);
-      // Left neighbour's last H' for the first offset's extension bit.
+      const int inc = scan_max(L + stepK * lane);
+      const int exc = from_prev(inc, kNeg);
       int F[K];
+      F[0] = exc - stepK * (lane - 1);
       uint32_t fext = 0;
-      int prevHq = __shfl_up(Hq[K - 1], 1, 64);
-      // F of the left neighbour's last cell is max(prevHq_prev - oc, ...); its
-      // value is recovered from Fin: Fin = max(prevHq - oc, Fprev - ge).
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if (k == 0) {
-          F[0] = Fin;
-        } else {
-          int fo = Hq[k - 1] - oc;
-          int fe = F[k - 1] > kNeg / 2 ? F[k - 1] - ge : kNeg;
-          F[k] = max(fo, fe);
-          fext |= (uint32_t)(fe > fo) << k;
-        }
+      for (int k = 1; k < K; ++k) {
+        const int fo = Hq[k - 1] - oc, fe = F[k - 1] - ge;
+        F[k] = max(fo, fe);
+        fext |= (uint32_t)(fe > fo) << k;
       }
-      // Extension bit of the first owned cell needs F of the left cell.
       {
-        int lastF = __shfl_up(F[K - 1], 1, 64);
-        int fo = prevHq - oc;
-        int fe = lastF > kNeg / 2 ? lastF - ge : kNeg;
-        if (lane > 0 && fe > fo) fext |= 1u;
+        // Extension bit of the first owned cell: F and H' of the left cell.
+        const int fo = from_prev(Hq[K - 1], 1 << 29) - oc;
+        const int fe = from_prev(F[K - 1], kNeg) - ge;
+        fext |= (uint32_t)(fe > fo);
       }
       int rowbest = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        int hv = 0, choice = 0;
-        if (D[k] > hv) {
-          hv = D[k];
-          choice = 1;
+        int hv;
+        if (kTrace) {
+          // Strict priority D > E > F from 0 (src/gapped.rs:171-184).
+          int choice = 0;
+          hv = 0;
+          if (D[k] > hv) {
+            hv = D[k];
+            choice = 1;
+          }
+          if (E[k] > hv) {
+            hv = E[k];
+            choice = 2;
+          }
+          if (F[k] > hv) {
+            hv = F[k];
+            choice = 3;
+          }
+          if (valid[k])
+            tr[(uint64_t)(i - rlo) * W + lane * K + k] =
+                (uint8_t)(choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3));
+        } else {
+          hv = max(Hq[k], F[k]);
         }
-        if (E[k] > hv) {
-          hv = E[k];
-          choice = 2;
-        }
-        if (F[k] > hv) {
-          hv = F[k];
-          choice = 3;
-        }
-        if (!valid[k]) hv = kNeg;
-        Hp[k] = hv;
+        Hp[k] = valid[k] ? hv : kNeg;
         Ep[k] = valid[k] ? E[k] : kNeg;
-        rowbest = max(rowbest, hv);
-        if (kTrace && valid[k])
-          tr[(uint64_t)(i - rlo) * W + lane * K + k] =
-              (uint8_t)(choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3));
+        rowbest = max(rowbest, Hp[k]);
       }
+      // Subject letters for the next row (the next lane's first letter is
+      // read before the shift overwrites it).
+      const int sn = from_next(S[0], edge);
+#pragma unroll
+      for (int k = 0; k + 1 < K; ++k) S[k] = S[k + 1];
+      S[K - 1] = sn;
       if (kTrace) {
-        const int rb = uniform(wmax(rowbest));
+        const int rb = wmax(rowbest);
         if (rb > best) {
           int tf = 1 << 30;
 #pragma unroll
           for (int k = K - 1; k >= 0; --k)
             if (valid[k] && Hp[k] == rb) tf = lane * K + k;
-          tf = uniform(wmin(tf));
+          tf = wmin(tf);
           best = rb;
           bi = i;
           bj = i - d1 + tf;
@@ -217,6 +258,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
       } else {
         lbest = max(lbest, rowbest);
       }
+     }
     }
     // Clamped rows below the band: rows istar+1..n hold the single cell
     // (i, m), whose diagonal predecessor lies outside the previous row's
@@ -227,7 +269,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     // bands that never reach row istar go to the row-exact kernel.
     bool need_exact = best < 11 && istar < n;
     if (!kTrace) {
-      best = uniform(wmax(lbest));
+      best = wmax(lbest);
       need_exact = best < 11 && istar < n;
       if (need_exact && m >= 1 && istar >= 1 && rlo <= rhi) {
         int h = uniform(Hp[0]), e = uniform(Ep[0]);
@@ -341,13 +383,16 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int grid, hipStream_t st) {
+                    int cu_count, hipStream_t st) {
   if (!njobs) return;
-  unsigned blocks = std::min<unsigned>((unsigned)grid, nblk(njobs, 4));
 #define DMR_SWW(k, tr)                                                                             \
-  hipLaunchKernelGGL((k_sw_wave<k, tr>), dim3(blocks), dim3(256), 0, st, jobs, list, njobs, q, db,  \
-                     m32, gap_open, gap_extend, counter, outs, trace, trace_off, ops, ops_off,     \
-                     fallback, nfallback)
+  do {                                                                                             \
+    static const unsigned resident = resident_blocks(k_sw_wave<k, tr>, 256, cu_count);            \
+    const unsigned blocks = std::min<unsigned>(resident, nblk(njobs, 4));                          \
+    hipLaunchKernelGGL((k_sw_wave<k, tr>), dim3(blocks), dim3(256), 0, st, jobs, list, njobs, q,   \
+                       db, m32, gap_open, gap_extend, counter, outs, trace, trace_off, ops,        \
+                       ops_off, fallback, nfallback);                                              \
+  } while (0)
   if (traceback) {
     switch (K) {
       case 1: DMR_SWW(1, true); break;
