@@ -48,6 +48,18 @@ def load(name, path, pkg=False):
     return mod
 
 
+def hbm_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 PMC passes on this
+    workload (profiles/hbm_traffic.json), or None when not measured."""
+    try:
+        d = json.loads((ROOT / "profiles" / "hbm_traffic.json").read_text())
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload:
+        return None
+    return d.get("kernels", {}).get(kernel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +118,10 @@ def main():
     dom = max(prof, key=lambda k: prof[k]["ms"])
     dcells, dms = prof[dom]["cells"], max(prof[dom]["ms"], 1e-9)
     achieved = dcells * OPS_PER_CELL / (dms / 1e3) / 1e12
+    kname = (f"k_sw16<{dom[1:]}>" if dom.startswith("w") else
+             f"k_sw_wave<{int(dom[4:]) // 64}, false>" if dom.startswith("wave") else
+             "k_sw_generic<false>")
+    traffic = hbm_traffic(kname, w.name)
     p50, p99 = (float(np.percentile(np.array(steps) * 1e3, q)) for q in (50, 99))
     log(f"[bench] {qps:.1f} queries/s, {ms:.1f} ms/step, SW {sw:.2f} ms for {cells} cells "
         f"= {gcups:.1f} GCUPS")
@@ -113,7 +129,8 @@ def main():
     out = {
         "metric": METRIC, "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int16" if dom.startswith("w") and not dom.startswith("wave") else "int32",
         "data": "synthetic (seeded generator diamond-rust_amd/synth.py)",
         "config": {"workload": f"{w.name}: blastp --sensitive, {w.n_q} queries vs {w.n_db} "
                                f"proteins / {int(w.db_offsets[-1])} letters",
@@ -126,13 +143,12 @@ def main():
         "dp_cells": int(cells),
         "hsps": text.count("\n"),
         "roofline": {"bound": "valu",
-                     "kernel": (f"k_sw16<{dom[1:]}> (packed int16 score pass, band width {dom[1:]})"
-                                if dom.startswith("w") else
-                                f"k_sw_wave ({dom}, int32 score pass)" if dom.startswith("wave")
-                                else "k_sw_generic (row-exact int32 score pass)"),
+                     "kernel": kname,
                      "achieved": round(achieved, 3), "peak": INT16_VALU_PEAK_TOPS,
                      "unit": "Tops/s int16", "frac": round(achieved / INT16_VALU_PEAK_TOPS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                     "copy-calibrated; profiles/hbm_traffic.json)",
                      "gcups": round(dcells / (dms / 1e3) / 1e9, 2),
                      "basis": f"{OPS_PER_CELL} int16 ops per reference DP cell x cells of the "
                               "launch / launch time (HIP events on the engine stream)"},

@@ -20,8 +20,8 @@
 // - The column after the band (t = W) is a wall: its G and E stay NEG, so the
 //   rightmost cell of a row gets no E from outside the band.
 // - Rows past m + d1 hold a single cell at column m (the clamped rows of the
-//   reference); they are scored after the main loop from the values captured
-//   at row m + d1, and can only matter when the best score is below 11.
+//   reference). Their H is at most max(11, H(m + d1, m) - o), so they only
+//   matter when the best score is below 11: such DPs are re-run in int32.
 // Saturating adds detect scores >= 32767; those DPs are re-run in int32.
 #include "dmr_kernels.hpp"
 
@@ -53,23 +53,11 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-__device__ __forceinline__ int letter_at(const uint8_t* s, int m, int p) {
-  return (p >= 0 && p < m) ? (int)s[p] : 25;
-}
-
-// Scalar tail of the clamped rows below the band: cells (i, m) for
-// i = istar+1..n starting from H(istar, m), E(istar, m).
-__device__ int tail_best(const int8_t* __restrict__ m32, const uint8_t* qs, int n, int istar,
-                         int h, int e, int slast, int best) {
-  for (int i = istar + 1; i <= n; ++i) {
-    int ev = max(h - kOpen, e - 1);
-    int d = m32[qs[i - 1] * kScoreStride + slast];
-    int hv = max(0, max(d, ev));
-    best = max(best, hv);
-    h = hv;
-    e = ev;
-  }
-  return best;
+// Letter p of the sequence at res + off (length m), padding outside 0..m-1.
+// Sequences are addressed by 32-bit offsets from the kernel-argument base,
+// which keeps per-lane state to one register per sequence.
+__device__ __forceinline__ int letter_at(const uint8_t* res, uint32_t off, int m, int p) {
+  return (p >= 0 && p < m) ? (int)res[off + (uint32_t)p] : 25;
 }
 
 template <int W>
@@ -101,26 +89,30 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     SwJob ja = {}, jb = {};
     if (hasA) ja = jobs[ln.x];
     if (hasB) jb = jobs[ln.y];
-    const uint8_t* qs = q.res + (hasA ? q.offs[ja.target] : 0);
-    const int n = hasA ? (int)(q.offs[ja.target + 1] - q.offs[ja.target]) : 0;
-    const uint8_t* sA = db.res + (hasA ? db.offs[ja.subject] : 0);
-    const uint8_t* sB = db.res + (hasB ? db.offs[jb.subject] : 0);
-    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - db.offs[ja.subject]) : 0;
-    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - db.offs[jb.subject]) : 0;
+    const uint32_t qo = hasA ? q.offs[ja.target] : 0;
+    const int n = hasA ? (int)(q.offs[ja.target + 1] - qo) : 0;
+    const uint32_t ao = hasA ? db.offs[ja.subject] : 0, bo = hasB ? db.offs[jb.subject] : 0;
+    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - ao) : 0;
+    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - bo) : 0;
     const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
     const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
-    const int istarA = mA + d1A, istarB = mB + d1B;
-    int rlo = 1 << 30, rhi = -(1 << 30);
+    // Rows of this lane: the union of its two DPs' row ranges, rows 1..m+d1
+    // (the clamped rows below, one cell at column m each, are left to the
+    // exact re-run when they can matter). Each lane runs its own rows (row
+    // i = lo + r at step r); the wave steps as often as its longest lane
+    // needs. Before its first row a lane would only see padding cells (H = 0,
+    // E = -o), which the initial state reproduces after one row; after its
+    // last row it sees padding cells, which stay below an earlier H.
+    int lo = 0, hi = -1;
     if (hasA) {
-      rlo = max(1, d0A + 1);
-      rhi = min(n, istarA);
+      lo = max(1, d0A + 1);
+      hi = min(n, mA + d1A);
     }
     if (hasB) {
-      rlo = min(rlo, max(1, d0B + 1));
-      rhi = max(rhi, min(n, istarB));
+      lo = min(lo, max(1, d0B + 1));
+      hi = max(hi, min(n, mB + d1B));
     }
-    const int lo = __builtin_amdgcn_readfirstlane(wave_min(rlo));
-    const int hi = __builtin_amdgcn_readfirstlane(wave_max(rhi));
+    const int steps = __builtin_amdgcn_readfirstlane(wave_max(hi - lo + 1));
 
     // Window of (sA, sB) letter pairs for the current row, as byte offsets
     // into a score-table row, two 16-bit entries per register: entry t is
@@ -135,30 +127,72 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-      uint32_t lo16 = 4u * (letter_at(sA, mA, lo - d1A + 2 * k - 1) * kCodes +
-                            letter_at(sB, mB, lo - d1B + 2 * k - 1));
-      uint32_t hi16 = 2 * k + 1 < W ? 4u * (letter_at(sA, mA, lo - d1A + 2 * k) * kCodes +
-                                            letter_at(sB, mB, lo - d1B + 2 * k))
+      uint32_t lo16 = 4u * (letter_at(db.res, ao, mA, lo - d1A + 2 * k - 1) * kCodes +
+                            letter_at(db.res, bo, mB, lo - d1B + 2 * k - 1));
+      uint32_t hi16 = 2 * k + 1 < W ? 4u * (letter_at(db.res, ao, mA, lo - d1A + 2 * k) * kCodes +
+                                            letter_at(db.res, bo, mB, lo - d1B + 2 * k))
                                     : 0u;
       R[k] = lo16 | (hi16 << 16);
     }
+    // Per-row letters: the query letter of row i and the letter pair entering
+    // the window at t = W-1 (subject positions i - d0 - 1).
+    auto pair_at = [&](int i) {
+      return (uint32_t)(4 * (letter_at(db.res, ao, mA, i - d0A - 1) * kCodes +
+                             letter_at(db.res, bo, mB, i - d0B - 1)));
+    };
+    // Widths up to 81 load them one row ahead with clamped, always in-bounds
+    // addresses; the raw bytes are only read by a VALU copy at the end of the
+    // row (padding is selected after it), so the loads are unconditional and
+    // no row waits on memory. W = 129 has no registers to spare for this (it
+    // would spill to scratch) and loads them at the start of each row.
+    constexpr bool kPrefetch = W < 129;
+    auto qraw = [&](int i) { return (int)q.res[qo + (uint32_t)(max(min(i, n), 1) - 1)]; };
+    auto sraw = [&](uint32_t off, int m, int p) {
+      return (int)db.res[off + (uint32_t)min(max(p, 0), max(m - 1, 0))];
+    };
+    auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
+    int qcur = lo <= n ? qraw(lo) : 25;
+    uint32_t pnext = 0;
     v2s best = ZERO2;
-    int capHA = 0, capEA = kNeg16, capHB = 0, capEB = kNeg16;
-    for (int i = lo; i <= hi; ++i) {
-      if (i > lo) {
+    for (int r = 0; r < steps; ++r) {
+      const int i = lo + r;
+      if (r > 0) {
         // Shift the window by one entry and append the pair entering at t = W-1.
         static_assert(W % 2 == 1, "band widths are odd");
 #pragma unroll
         for (int k = 0; k < NR - 1; ++k) R[k] = __builtin_amdgcn_alignbit(R[k + 1], R[k], 16);
-        R[NR - 1] = 4u * (letter_at(sA, mA, i - d0A - 1) * kCodes + letter_at(sB, mB, i - d0B - 1));
+        if (kPrefetch) {
+          R[NR - 1] = pnext;
+        } else {
+          qcur = i <= n ? (int)q.res[qo + (uint32_t)(i - 1)] : 25;
+          R[NR - 1] = pair_at(i);
+        }
       }
-      const int qi = (i <= n) ? (int)qs[i - 1] : 25;
-      const char* Tq = reinterpret_cast<const char*>(T) + qi * (4 * kPairRow);
+      int rq = 0, ra = 0, rb = 0;
+      if (kPrefetch) {  // raw bytes of row i + 1
+        rq = qraw(i + 1);
+        ra = sraw(ao, mA, i - d0A);
+        rb = sraw(bo, mB, i - d0B);
+      }
+      const char* Tq = reinterpret_cast<const char*>(T) + qcur * (4 * kPairRow);
+      auto score = [&](int t) {
+        const uint32_t off = (t & 1) ? (R[t >> 1] >> 16) : (R[t >> 1] & 0xFFFFu);
+        return *reinterpret_cast<const int*>(Tq + off);
+      };
+      // Score-table reads run kAhead columns ahead of their use, with a
+      // scheduling fence after each so the compiler cannot sink them back to
+      // their use: the LDS latency overlaps the recurrence even at one wave
+      // per SIMD.
+      constexpr int kAhead = 4;
+      int sq[kAhead];
+#pragma unroll
+      for (int t = 0; t < kAhead; ++t) sq[t] = score(t);
       v2s F = NEG2, Gl = NEG2;
 #pragma unroll
       for (int t = 0; t < W; ++t) {
-        const uint32_t off = (t & 1) ? (R[t >> 1] >> 16) : (R[t >> 1] & 0xFFFFu);
-        const v2s sc = as_v2s(*reinterpret_cast<const int*>(Tq + off));
+        const v2s sc = as_v2s(sq[t % kAhead]);
+        if (t + kAhead < W) sq[t % kAhead] = score(t + kAhead);
+        __builtin_amdgcn_sched_barrier(0);
         const v2s D = vadds(G[t], sc);
         const v2s Ev = vmax(G[t + 1], vsubs(E[t + 1], ONE2));
         const v2s Fv = vmax(Gl, vsubs(F, ONE2));
@@ -169,19 +203,20 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         E[t] = Ev;
         F = Fv;
       }
-      if (i == istarA) {
-        capHA = (int)G[0].x + kOpen;
-        capEA = E[0].x;
-      }
-      if (i == istarB) {
-        capHB = (int)G[0].y + kOpen;
-        capEB = E[0].y;
+      if (kPrefetch) {  // letters of row i + 1, padding selected as in letter_at
+        const int cq = copy(rq), ca = copy(ra), cb = copy(rb);
+        qcur = i + 1 <= n ? cq : 25;
+        const int la = (i - d0A >= 0 && i - d0A < mA) ? ca : 25;
+        const int lb = (i - d0B >= 0 && i - d0B < mB) ? cb : 25;
+        pnext = (uint32_t)(4 * (la * kCodes + lb));
       }
     }
-    int bA = best.x, bB = best.y;
+    // Re-run in int32 (exact) when int16 saturated, or when the score is
+    // below 11 and the band reaches past the subject's end: the clamped rows
+    // there could then hold the maximum.
+    const int bA = best.x, bB = best.y;
     if (hasA) {
-      if (bA < 11 && istarA < n && istarA >= 1) bA = tail_best(m32, qs, n, istarA, capHA, capEA, sA[mA - 1], bA);
-      if (bA >= 32767) {
+      if (bA >= 32767 || (bA < 11 && mA + d1A < n)) {
         redo[atomicAdd(nredo, 1u)] = ln.x;
       } else {
         outs[ln.x].score = bA;
@@ -189,8 +224,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
       }
     }
     if (hasB) {
-      if (bB < 11 && istarB < n && istarB >= 1) bB = tail_best(m32, qs, n, istarB, capHB, capEB, sB[mB - 1], bB);
-      if (bB >= 32767) {
+      if (bB >= 32767 || (bB < 11 && mB + d1B < n)) {
         redo[atomicAdd(nredo, 1u)] = ln.y;
       } else {
         outs[ln.y].score = bB;
