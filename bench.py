@@ -97,12 +97,16 @@ def main():
         txt = r.format(qids, sids)
         log(f"[bench] warmup {k}: {r.stats()['ms_total']:.1f} ms, {len(r)} HSPs")
     steps, sw_ms, stats = [], [], None
+    split = {"search_call": [], "format": []}
     text = None
     for k in range(a.steps):
         t0 = time.perf_counter()
         r = ctx.search_loaded(cfg)
+        t1 = time.perf_counter()
         text = r.format(qids, sids)
         dt = time.perf_counter() - t0
+        split["search_call"].append((t1 - t0) * 1e3)
+        split["format"].append((time.perf_counter() - t1) * 1e3)
         steps.append(dt)
         stats = r.stats()
         sw_ms.append(stats["ms_sw"])
@@ -139,6 +143,7 @@ def main():
         "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3), "of": "one 1024-query batch"},
         "index_build_ms": round(index_ms, 2),
         "stage_ms": {k[3:]: round(v, 3) for k, v in stats.items() if k.startswith("ms_")},
+        "step_split_ms": {k: round(float(np.mean(v)), 3) for k, v in split.items()},
         "counts": {k: int(v) for k, v in stats.items() if not k.startswith("ms_")},
         "dp_cells": int(cells),
         "hsps": text.count("\n"),

@@ -3,6 +3,7 @@
 #include "../../include/dmr.h"
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,9 @@ struct dmr_result {
   std::shared_ptr<const std::vector<uint32_t>> db_offs;  // host copy of the database
   std::shared_ptr<const std::vector<uint8_t>> db_res;
   dmr::StageStats stats;
+  // Text of the last dmr_result_format call and the inputs it was made from,
+  // so the usual size-then-fill call pair renders once.
+  mutable std::string fmt_key, fmt_text;
 };
 
 namespace {
@@ -194,19 +198,32 @@ dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets
 }
 
 // src/output.rs:235-241
+// src/output.rs:235-242: values >= 100 print as their floor, others with one
+// decimal ({:.1}, the exact binary value rounded half to even, as printf's
+// %.1f; std::to_chars with a precision is specified to match printf).
 std::string fmt_double(double x) {
   char b[64];
-  if (x >= 100.0) std::snprintf(b, sizeof b, "%lld", (long long)std::floor(x));
-  else std::snprintf(b, sizeof b, "%.1f", x);
-  return b;
+  std::to_chars_result r;
+  if (x >= 100.0) r = std::to_chars(b, b + sizeof b, (long long)std::floor(x));
+  else r = std::to_chars(b, b + sizeof b, x, std::chars_format::fixed, 1);
+  return std::string(b, r.ptr);
 }
 // src/output.rs:244-250 (Rust {:.2e}: no exponent sign padding)
 std::string fmt_evalue(double e) {
   if (!std::isfinite(e) || e <= 0.0) return "0.0";
   char b[64];
-  std::snprintf(b, sizeof b, "%.2e", e);
-  const char* p = std::strchr(b, 'e');
-  return std::string(b, (size_t)(p - b)) + "e" + std::to_string(std::atoi(p + 1));
+  const auto r = std::to_chars(b, b + sizeof b, e, std::chars_format::scientific, 2);
+  const char* end = r.ptr;
+  const char* p = std::find(static_cast<const char*>(b), end, 'e');
+  std::string out(b, (size_t)(p - b));
+  out += 'e';
+  const char* x = p + 1;
+  const bool neg = *x == '-';
+  if (*x == '-' || *x == '+') ++x;
+  while (x + 1 < end && *x == '0') ++x;
+  if (neg) out += '-';
+  out.append(x, (size_t)(end - x));
+  return out;
 }
 
 std::vector<std::string> split(const char* s, char sep) {
@@ -574,25 +591,54 @@ int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells) {
 }
 
 // Tabular::write (src/output.rs:150-223).
-int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, const char* fields,
-                      char* buf, uint64_t cap, uint64_t* len) {
-  if (!r || !len) return fail(DMR_E_ARG, "NULL argument");
-  auto qid = split(qids, '\n'), sid = split(sids, '\n');
-  auto f = split(fields, ' ');
-  if (f.empty())
-    f = {"qseqid", "sseqid", "pident", "length", "mismatch", "gapopen",
-         "qstart", "qend", "sstart", "send", "evalue", "bitscore"};
-  static const char* known[] = {"qseqid", "qlen", "sseqid", "slen", "qstart", "qend", "sstart",
-                                "send", "qseq", "sseq", "evalue", "bitscore", "score", "length",
-                                "pident", "nident", "mismatch", "positive", "gapopen", "gaps",
-                                "ppos", "qframe", "qcovhsp", "scovhsp", "cigar", "btop", "qtitle",
-                                "stitle", "full_sseq", "qseq_translated", "qstrand"};
-  for (auto& n : f) {
-    bool ok = false;
-    for (auto* k : known) ok |= n == k;
-    if (!ok) return fail(DMR_E_ARG, "Invalid output field: " + n);
+namespace {
+
+enum Field {
+  kQseqid, kQlen, kSseqid, kSlen, kQstart, kQend, kSstart, kSend, kQseq, kSseq, kEvalue,
+  kBitscore, kScore, kLength, kPident, kNident, kMismatch, kPositive, kGapopen, kGaps, kPpos,
+  kQframe, kQcovhsp, kScovhsp, kCigar, kBtop, kQtitle, kStitle, kFullSseq, kQseqTranslated,
+  kQstrand, kFieldCount
+};
+const char* const kFieldNames[kFieldCount] = {
+    "qseqid", "qlen", "sseqid", "slen", "qstart", "qend", "sstart", "send", "qseq", "sseq",
+    "evalue", "bitscore", "score", "length", "pident", "nident", "mismatch", "positive",
+    "gapopen", "gaps", "ppos", "qframe", "qcovhsp", "scovhsp", "cigar", "btop", "qtitle",
+    "stitle", "full_sseq", "qseq_translated", "qstrand"};
+
+// Start and length of every line of a newline-separated list (no copies).
+std::vector<std::pair<const char*, size_t>> lines_of(const char* s) {
+  std::vector<std::pair<const char*, size_t>> out;
+  if (!s) return out;
+  const char* b = s;
+  for (const char* p = s;; ++p) {
+    if (*p == '\n' || *p == 0) {
+      if (*p == '\n' || p > b) out.push_back({b, (size_t)(p - b)});
+      if (*p == 0) break;
+      b = p + 1;
+    }
   }
-  std::string out;
+  return out;
+}
+
+int render(const dmr_result* r, const char* qids, const char* sids, const char* fields,
+           std::string* text) {
+  const auto qid = lines_of(qids), sid = lines_of(sids);
+  std::vector<int> f;
+  for (const std::string& n : split(fields, ' ')) {
+    int code = -1;
+    for (int k = 0; k < kFieldCount; ++k)
+      if (n == kFieldNames[k]) code = k;
+    if (code < 0) return fail(DMR_E_ARG, "Invalid output field: " + n);
+    f.push_back(code);
+  }
+  if (f.empty())
+    f = {kQseqid, kSseqid, kPident, kLength, kMismatch, kGapopen,
+         kQstart, kQend, kSstart, kSend, kEvalue, kBitscore};
+  std::string& out = *text;
+  out.clear();
+  out.reserve(r->hsps.size() * 96);
+  char nb[32];
+  auto num = [&](long long x) { out.append(nb, std::to_chars(nb, nb + sizeof nb, x).ptr); };
   for (size_t i = 0; i < r->hsps.size(); ++i) {
     const HspRec& h = r->hsps[i];
     int ti = r->target_of[i];
@@ -618,90 +664,134 @@ int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, c
     }
     size_t qlen = r->translated ? dna : frame_len;
     auto aligned = [&](bool query_side) {
-      std::string s;
       size_t qi = h.q_begin, si = h.s_begin;
       for (uint32_t op : h.ops)
         for (uint32_t k = 0; k < (op >> 2); ++k) {
           uint32_t c = op & 3;
           if (c == 0) {
-            s += dmr::decode_letter(query_side ? letters[qi] : subj[si]);
+            out += dmr::decode_letter(query_side ? letters[qi] : subj[si]);
             ++qi;
             ++si;
           } else if (c == 1) {
-            s += query_side ? dmr::decode_letter(letters[qi]) : '-';
+            out += query_side ? dmr::decode_letter(letters[qi]) : '-';
             ++qi;
           } else {
-            s += query_side ? '-' : dmr::decode_letter(subj[si]);
+            out += query_side ? '-' : dmr::decode_letter(subj[si]);
             ++si;
           }
         }
-      return s;
     };
     for (size_t k = 0; k < f.size(); ++k) {
-      const std::string& n = f[k];
-      std::string v;
-      if (n == "qseqid" || n == "qtitle") v = qid[h.query];
-      else if (n == "qlen") v = std::to_string(qlen);
-      else if (n == "sseqid" || n == "stitle") v = sid[h.subject];
-      else if (n == "slen") v = std::to_string(slen);
-      else if (n == "qstart") v = std::to_string(qb + 1);
-      else if (n == "qend") v = std::to_string(qe);
-      else if (n == "sstart") v = std::to_string(h.s_begin + 1);
-      else if (n == "send") v = std::to_string(h.s_end);
-      else if (n == "evalue") v = fmt_evalue(h.evalue);
-      else if (n == "bitscore") v = fmt_double(h.bits);
-      else if (n == "score") v = std::to_string(h.score);
-      else if (n == "length") v = std::to_string(h.length);
-      else if (n == "pident") v = fmt_double(h.length ? h.identities * 100.0 / h.length : 0.0);
-      else if (n == "nident") v = std::to_string(h.identities);
-      else if (n == "mismatch") v = std::to_string(h.mismatches);
-      else if (n == "positive") v = std::to_string(h.positives);
-      else if (n == "gapopen") v = std::to_string(h.gap_openings);
-      else if (n == "gaps") v = std::to_string(h.gaps);
-      else if (n == "ppos") v = fmt_double(h.length ? h.positives * 100.0 / h.length : 0.0);
-      else if (n == "qframe") v = std::to_string(h.frame < 3 ? (int)h.frame + 1 : 2 - (int)h.frame);
-      else if (n == "qcovhsp") v = qlen == 0 ? "0.0" : fmt_double((double)(qe - qb) * 100.0 / (double)qlen);
-      else if (n == "scovhsp") v = slen == 0 ? "0.0" : fmt_double((double)(h.s_end - h.s_begin) * 100.0 / (double)slen);
-      else if (n == "cigar") {
-        for (uint32_t op : h.ops) v += std::to_string(op >> 2) + "MID"[op & 3];
-      } else if (n == "btop") {
-        uint32_t run = 0;
-        size_t qi = h.q_begin, si = h.s_begin;
-        auto flush = [&]() {
-          if (run) v += std::to_string(run);
-          run = 0;
-        };
-        for (uint32_t op : h.ops)
-          for (uint32_t t = 0; t < (op >> 2); ++t) {
-            uint32_t c = op & 3;
-            if (c == 0) {
-              char a = dmr::decode_letter(letters[qi++]), b = dmr::decode_letter(subj[si++]);
-              if (a == b) ++run;
-              else { flush(); v += a; v += b; }
-            } else if (c == 1) {
-              flush(); v += dmr::decode_letter(letters[qi++]); v += '-';
-            } else {
-              flush(); v += '-'; v += dmr::decode_letter(subj[si++]);
-            }
-          }
-        flush();
-      } else if (n == "qseq") v = aligned(true);
-      else if (n == "sseq") v = aligned(false);
-      else if (n == "full_sseq") {
-        for (size_t t = 0; t < slen; ++t) v += dmr::decode_letter(subj[t]);
-      } else if (n == "qseq_translated") {
-        size_t e = std::min<size_t>(h.q_end, frame_len), b = std::min<size_t>(h.q_begin, e);
-        for (size_t t = b; t < e; ++t) v += dmr::decode_letter(letters[t]);
-      } else if (n == "qstrand") v = h.frame >= 3 ? "-" : "+";
       if (k) out += '\t';
-      out += v;
+      switch (f[k]) {
+        case kQseqid: case kQtitle: out.append(qid[h.query].first, qid[h.query].second); break;
+        case kQlen: num((long long)qlen); break;
+        case kSseqid: case kStitle: out.append(sid[h.subject].first, sid[h.subject].second); break;
+        case kSlen: num((long long)slen); break;
+        case kQstart: num((long long)qb + 1); break;
+        case kQend: num((long long)qe); break;
+        case kSstart: num((long long)h.s_begin + 1); break;
+        case kSend: num((long long)h.s_end); break;
+        case kEvalue: out += fmt_evalue(h.evalue); break;
+        case kBitscore: out += fmt_double(h.bits); break;
+        case kScore: num(h.score); break;
+        case kLength: num(h.length); break;
+        case kPident: out += fmt_double(h.length ? h.identities * 100.0 / h.length : 0.0); break;
+        case kNident: num(h.identities); break;
+        case kMismatch: num(h.mismatches); break;
+        case kPositive: num(h.positives); break;
+        case kGapopen: num(h.gap_openings); break;
+        case kGaps: num(h.gaps); break;
+        case kPpos: out += fmt_double(h.length ? h.positives * 100.0 / h.length : 0.0); break;
+        case kQframe: num(h.frame < 3 ? (int)h.frame + 1 : 2 - (int)h.frame); break;
+        case kQcovhsp:
+          out += qlen == 0 ? std::string("0.0") : fmt_double((double)(qe - qb) * 100.0 / (double)qlen);
+          break;
+        case kScovhsp:
+          out += slen == 0 ? std::string("0.0")
+                           : fmt_double((double)(h.s_end - h.s_begin) * 100.0 / (double)slen);
+          break;
+        case kCigar:
+          for (uint32_t op : h.ops) {
+            num(op >> 2);
+            out += "MID"[op & 3];
+          }
+          break;
+        case kBtop: {
+          uint32_t run = 0;
+          size_t qi = h.q_begin, si = h.s_begin;
+          auto flush = [&]() {
+            if (run) num(run);
+            run = 0;
+          };
+          for (uint32_t op : h.ops)
+            for (uint32_t t = 0; t < (op >> 2); ++t) {
+              uint32_t c = op & 3;
+              if (c == 0) {
+                char a = dmr::decode_letter(letters[qi++]), b = dmr::decode_letter(subj[si++]);
+                if (a == b) {
+                  ++run;
+                } else {
+                  flush();
+                  out += a;
+                  out += b;
+                }
+              } else if (c == 1) {
+                flush();
+                out += dmr::decode_letter(letters[qi++]);
+                out += '-';
+              } else {
+                flush();
+                out += '-';
+                out += dmr::decode_letter(subj[si++]);
+              }
+            }
+          flush();
+          break;
+        }
+        case kQseq: aligned(true); break;
+        case kSseq: aligned(false); break;
+        case kFullSseq:
+          for (size_t t = 0; t < slen; ++t) out += dmr::decode_letter(subj[t]);
+          break;
+        case kQseqTranslated: {
+          size_t e = std::min<size_t>(h.q_end, frame_len), b = std::min<size_t>(h.q_begin, e);
+          for (size_t t = b; t < e; ++t) out += dmr::decode_letter(letters[t]);
+          break;
+        }
+        case kQstrand: out += h.frame >= 3 ? "-" : "+"; break;
+        default: break;
+      }
     }
     out += '\n';
   }
-  *len = out.size();
-  if (buf && cap >= out.size()) std::memcpy(buf, out.data(), out.size());
-  else if (buf && cap < out.size()) return fail(DMR_E_CAPACITY, "buffer too small");
   return DMR_OK;
+}
+
+}  // namespace
+
+int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, const char* fields,
+                      char* buf, uint64_t cap, uint64_t* len) {
+  if (!r || !len) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&]() -> int {
+    // The inputs as one key: the text is rendered once per distinct request.
+    std::string key;
+    for (const char* s : {qids, sids, fields}) {
+      key += s ? s : "";
+      key += '\0';
+    }
+    if (key != r->fmt_key || r->fmt_key.empty()) {
+      r->fmt_key.clear();
+      const int rc = render(r, qids, sids, fields, &r->fmt_text);
+      if (rc != DMR_OK) return rc;
+      r->fmt_key = std::move(key);
+    }
+    const std::string& out = r->fmt_text;
+    *len = out.size();
+    if (buf && cap >= out.size()) std::memcpy(buf, out.data(), out.size());
+    else if (buf && cap < out.size()) return fail(DMR_E_CAPACITY, "buffer too small");
+    return DMR_OK;
+  });
 }
 
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms) {
