@@ -75,7 +75,8 @@ void fill_pair_table(const int8_t (&m)[25][25], std::vector<int>* out);
 void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits,
                      uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
-                       const uint32_t* lane_off, uint32_t nruns, uint2* lanes, hipStream_t st);
+                       const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
+                       hipStream_t st);
 int sw16_width(int cls);
 void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
                  const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,

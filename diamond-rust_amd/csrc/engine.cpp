@@ -47,6 +47,9 @@ Engine::Engine(int device) : device_(device) {
   DMR_HIP(hipSetDevice(device));
   DMR_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
   for (auto& e : ev_) DMR_HIP(hipEventCreate(&e));
+  for (auto& s : aux_) DMR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  DMR_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  for (auto& e : join_ev_) DMR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // 32x32 score table; letters >= 25 (padding) score -64 against everything.
   int8_t m[kScoreStride * kScoreStride];
   for (int a = 0; a < kScoreStride; ++a)
@@ -66,9 +69,28 @@ Engine::Engine(int device) : device_(device) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (st_) (void)hipStreamSynchronize(st_);
+  for (auto& s : aux_)
+    if (s) (void)hipStreamSynchronize(s);
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : join_ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  for (auto& s : aux_)
+    if (s) (void)hipStreamDestroy(s);
   if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Engine::fork() {
+  DMR_HIP(hipEventRecord(fork_ev_, st_));
+  for (auto& s : aux_) DMR_HIP(hipStreamWaitEvent(s, fork_ev_, 0));
+}
+
+void Engine::join() {
+  for (int k = 0; k < kAux; ++k) {
+    DMR_HIP(hipEventRecord(join_ev_[k], aux_[k]));
+    DMR_HIP(hipStreamWaitEvent(st_, join_ev_[k], 0));
+  }
 }
 
 template <class T>
@@ -578,20 +600,32 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
   uint32_t* fallback = sw_fallback_.ensure<uint32_t>(njobs + 1);
   const int cus = cu_count_;
+  uint2* lanes = nullptr;
   if (nl) {
     const uint32_t nr = (uint32_t)run_start.size();
     uint32_t* d_rs = sw_runs_.ensure<uint32_t>(3ull * nr + 3);
     DMR_HIP(hipMemcpyAsync(d_rs, run_start.data(), nr * 4, hipMemcpyHostToDevice, st_));
     DMR_HIP(hipMemcpyAsync(d_rs + nr, run_len.data(), nr * 4, hipMemcpyHostToDevice, st_));
     DMR_HIP(hipMemcpyAsync(d_rs + 2 * nr, lane_off.data(), nr * 4, hipMemcpyHostToDevice, st_));
-    uint2* lanes = sw_lanes_.ensure<uint2>(nl + 1);
-    launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, lanes, st_);
-    for (int c = 0; c < 5; ++c) {
+    lanes = sw_lanes_.ensure<uint2>(nl + 1);
+    launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, nl, lanes, st_);
+  }
+  // The class kernels are independent (distinct jobs and counters; the redo
+  // and fallback lists are filled through atomics): they run side by side on
+  // the auxiliary streams, so the small classes use the issue slots and
+  // registers the widest packed class leaves free. The widest class gets a
+  // stream of its own.
+  fork();
+  int next_aux = 1;
+  auto class_stream = [&](int c) { return c == 4 ? aux_[0] : aux_[1 + (next_aux++ - 1) % (kAux - 1)]; };
+  if (nl) {
+    for (int c = 4; c >= 0; --c) {
       if (!class_lanes[c]) continue;
-      DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
+      hipStream_t s = class_stream(c);
+      DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
       launch_sw16(c, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs, pair_tab_.as<int>(),
-                  matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, cus, st_);
-      DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
+                  matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, cus, s);
+      DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
     }
   }
   // Wave-per-DP int32 kernel: classes 5-10 (and every width class when the
@@ -607,17 +641,19 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     for (auto& rg : ranges) total += rg.second - rg.first;
     if (!total) continue;
     uint32_t* list = sw_klist_[c - 5].ensure<uint32_t>(total + 1);
+    hipStream_t s = class_stream(c);
     uint64_t o = 0;
     for (auto& rg : ranges) {
-      DMR_HIP(hipMemcpyAsync(list + o, idx + rg.first, (rg.second - rg.first) * 4, hipMemcpyDeviceToDevice, st_));
+      DMR_HIP(hipMemcpyAsync(list + o, idx + rg.first, (rg.second - rg.first) * 4, hipMemcpyDeviceToDevice, s));
       o += rg.second - rg.first;
     }
-    DMR_HIP(hipEventRecord(ev_[12 + 2 * c], st_));
+    DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
                    opt.gap_open, opt.gap_extend, cnt + 8 + (c - 5), outs, nullptr, nullptr, nullptr,
-                   nullptr, fallback, cnt + 16, cus, st_);
-    DMR_HIP(hipEventRecord(ev_[13 + 2 * c], st_));
+                   nullptr, fallback, cnt + 16, cus, s);
+    DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   }
+  join();
   // int16-saturated DPs: re-scored by the int32 wave kernel (widths <= 129).
   const uint32_t nr16 = read1(cnt);
   if (nr16)
@@ -689,39 +725,60 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
   }
   uint32_t* cnt = tb_counters_.ensure<uint32_t>(2 * kWaveKinds);
   launch_fill_u32(cnt, 2 * kWaveKinds, 0, st_);
+  // The groups run side by side on the auxiliary streams.
+  fork();
   const int cus = cu_count_;
   std::vector<uint64_t> fallback_items;
   uint64_t cells = 0;
   for (int g = 0; g < kWaveKinds; ++g) {
-    const auto& items = group[g];
+    auto& items = group[g];
     if (items.empty()) continue;
     const uint64_t ni = items.size();
-    std::vector<uint32_t> jl(ni);
-    std::vector<uint64_t> toff(ni + 1, 0), ooff(ni + 1, 0);
+    // Rows of a job's band and its cells; the longest jobs start first so
+    // they do not finish last.
+    auto rows_of = [&](uint64_t k) {
+      const SwJob& j = hjobs[k];
+      const int64_t n = (int64_t)j.q1 - j.q0, m = slen[k];
+      const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
+      return rhi >= rlo ? rhi - rlo + 1 : 0;
+    };
+    std::stable_sort(items.begin(), items.end(), [&](uint64_t a, uint64_t b) {
+      return rows_of(a) * ((int64_t)hjobs[a].d1 - hjobs[a].d0 + 1) >
+             rows_of(b) * ((int64_t)hjobs[b].d1 - hjobs[b].d0 + 1);
+    });
+    std::vector<uint32_t>& jl = tb_host_jl_[g];
+    std::vector<uint64_t>& toff = tb_host_toff_[g];
+    std::vector<uint64_t>& ooff = tb_host_ooff_[g];
+    jl.assign(ni, 0);
+    toff.assign(ni + 1, 0);
+    ooff.assign(ni + 1, 0);
+    const int64_t stride = 64ll << g;  // trace row: 64 * K bytes
     for (uint64_t t = 0; t < ni; ++t) {
       const SwJob& j = hjobs[items[t]];
       jl[t] = list[items[t]];
       const int64_t n = (int64_t)j.q1 - j.q0, m = slen[items[t]];
-      const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
-      const int64_t W = (int64_t)j.d1 - j.d0 + 1;
-      toff[t + 1] = toff[t] + (uint64_t)(rhi >= rlo ? (rhi - rlo + 1) * W : 0);
+      const int64_t rows = rows_of(items[t]);
+      toff[t + 1] = toff[t] + (uint64_t)(rows * stride);
       ooff[t + 1] = ooff[t] + (uint64_t)(n + m + 1);
-      cells += (uint64_t)(rhi >= rlo ? (rhi - rlo + 1) * W : 0);
+      cells += (uint64_t)(rows * ((int64_t)j.d1 - j.d0 + 1));
     }
+    // The two largest groups (K = 8, K = 4) on streams of their own.
+    static const int kStreamOf[kWaveKinds] = {2, 2, 1, 0, 0, 1};
+    hipStream_t s = aux_[kStreamOf[g]];
     uint32_t* dl = tb_list_[g].ensure<uint32_t>(ni + 1);
     uint64_t* dt = tb_toff_[g].ensure<uint64_t>(ni + 1);
     uint64_t* dop = tb_ooff_[g].ensure<uint64_t>(ni + 1);
-    DMR_HIP(hipMemcpyAsync(dl, jl.data(), ni * 4, hipMemcpyHostToDevice, st_));
-    DMR_HIP(hipMemcpyAsync(dt, toff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, st_));
-    DMR_HIP(hipMemcpyAsync(dop, ooff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dl, jl.data(), ni * 4, hipMemcpyHostToDevice, s));
+    DMR_HIP(hipMemcpyAsync(dt, toff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, s));
+    DMR_HIP(hipMemcpyAsync(dop, ooff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, s));
     uint8_t* tr = tb_trace_[g].ensure<uint8_t>(toff[ni] + 1);
     uint32_t* dops = tb_ops_[g].ensure<uint32_t>(ooff[ni] + 1);
     SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, st_);
-    tb_host_ooff_[g] = std::move(ooff);
+                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, s);
   }
+  join();
   DMR_HIP(hipStreamSynchronize(st_));
   uint32_t hfb[kWaveKinds] = {};
   DMR_HIP(hipMemcpy(hfb, cnt + kWaveKinds, sizeof(hfb), hipMemcpyDeviceToHost));

@@ -178,6 +178,16 @@ class Engine {
   int device_ = 0;
   hipStream_t st_ = nullptr;
   hipEvent_t ev_[40] = {};
+  // Auxiliary streams (default priority) for kernels that may run side by
+  // side: fork() makes them wait for the work queued on st_, join() makes st_
+  // wait for everything queued on them.
+  // Three, so that with the engine stream each has a hardware queue of its
+  // own (HIP gives a process four by default).
+  static constexpr int kAux = 3;
+  hipStream_t aux_[kAux] = {};
+  hipEvent_t fork_ev_ = nullptr, join_ev_[kAux] = {};
+  void fork();
+  void join();
 
   // database
   DevBuf db_res_, db_offs_, db_seq_of_;
@@ -222,7 +232,10 @@ class Engine {
   static constexpr int kTbGroups = kWaveKinds + 1;
   DevBuf tb_counters_, tb_list_[kTbGroups], tb_toff_[kTbGroups], tb_ooff_[kTbGroups],
       tb_trace_[kTbGroups], tb_ops_[kTbGroups], tb_out_[kTbGroups], tb_fb_[kTbGroups];
-  std::vector<uint64_t> tb_host_ooff_[kTbGroups];
+  // Host copies of each group's job list and offsets; they stay alive until
+  // the copies queued from them have run.
+  std::vector<uint64_t> tb_host_ooff_[kTbGroups], tb_host_toff_[kTbGroups];
+  std::vector<uint32_t> tb_host_jl_[kTbGroups];
   std::vector<uint64_t> last_cbeg_, last_cend_;
   int cu_count_ = 256;
   bool force_generic_ = false;

@@ -277,15 +277,21 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
   idx[k] = (uint32_t)k;
 }
 
-// One thread per (class, target) run of sorted jobs: pairs consecutive jobs.
+// One thread per lane: lane L belongs to the (class, target) run whose
+// lane_off range holds it (binary search) and pairs two consecutive jobs.
 __global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint32_t* __restrict__ run_start,
                              const uint32_t* __restrict__ run_len, const uint32_t* __restrict__ lane_off,
-                             uint32_t nruns, uint2* __restrict__ lanes) {
-  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nruns) return;
-  uint32_t s = run_start[r], len = run_len[r], o = lane_off[r];
-  for (uint32_t k = 0; k < len; k += 2)
-    lanes[o + k / 2] = make_uint2(sorted_idx[s + k], k + 1 < len ? sorted_idx[s + k + 1] : ~0u);
+                             uint32_t nruns, uint32_t nlanes, uint2* __restrict__ lanes) {
+  const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= nlanes) return;
+  uint32_t lo = 0, hi = nruns;  // last run with lane_off <= L
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (lane_off[mid] <= L) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t k = 2 * (L - lane_off[lo]), s = run_start[lo], len = run_len[lo];
+  lanes[L] = make_uint2(sorted_idx[s + k], k + 1 < len ? sorted_idx[s + k + 1] : ~0u);
 }
 
 inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -316,10 +322,11 @@ void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int
 }
 
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
-                       const uint32_t* lane_off, uint32_t nruns, uint2* lanes, hipStream_t st) {
-  if (!nruns) return;
-  hipLaunchKernelGGL(k_make_lanes, dim3(nblk(nruns, 128)), dim3(128), 0, st, sorted_idx, run_start,
-                     run_len, lane_off, nruns, lanes);
+                       const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
+                       hipStream_t st) {
+  if (!nruns || !nlanes) return;
+  hipLaunchKernelGGL(k_make_lanes, dim3(nblk(nlanes, 256)), dim3(256), 0, st, sorted_idx, run_start,
+                     run_len, lane_off, nruns, nlanes, lanes);
 }
 
 static const int kWidths[5] = {25, 33, 61, 81, 129};

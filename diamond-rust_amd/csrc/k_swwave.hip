@@ -208,6 +208,9 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         fext |= (uint32_t)(fe > fo);
       }
       int rowbest = 0;
+      uint32_t tw[(K + 3) / 4];  // trace bytes of the lane's offsets, packed
+#pragma unroll
+      for (int w = 0; w < (K + 3) / 4; ++w) tw[w] = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         int hv;
@@ -227,15 +230,24 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
             hv = F[k];
             choice = 3;
           }
-          if (valid[k])
-            tr[(uint64_t)(i - rlo) * W + lane * K + k] =
-                (uint8_t)(choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3));
+          const uint32_t byte = (uint32_t)choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3);
+          tw[k / 4] |= byte << (8 * (k % 4));
         } else {
           hv = max(Hq[k], F[k]);
         }
         Hp[k] = valid[k] ? hv : kNeg;
         Ep[k] = valid[k] ? E[k] : kNeg;
         rowbest = max(rowbest, Hp[k]);
+      }
+      if (kTrace) {
+        // Row i of the trace is 64K bytes (offset t at byte t): every lane
+        // stores its K bytes at once, and a wave writes whole lines.
+        uint8_t* row = tr + (uint64_t)(i - rlo) * (64 * K) + lane * K;
+        if (K == 1) *row = (uint8_t)tw[0];
+        else if (K == 2) *reinterpret_cast<uint16_t*>(row) = (uint16_t)tw[0];
+        else
+#pragma unroll
+          for (int w = 0; w < (K + 3) / 4; ++w) reinterpret_cast<uint32_t*>(row)[w] = tw[w];
       }
       // Subject letters for the next row (the next lane's first letter is
       // read before the shift overwrites it).
@@ -244,8 +256,10 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
       for (int k = 0; k + 1 < K; ++k) S[k] = S[k + 1];
       S[K - 1] = sn;
       if (kTrace) {
-        const int rb = wmax(rowbest);
-        if (rb > best) {
+        // The wave-wide first maximum is only updated on rows where some
+        // lane beats the current best.
+        if (__ballot(rowbest > best) != 0) {
+          const int rb = wmax(rowbest);
           int tf = 1 << 30;
 #pragma unroll
           for (int k = K - 1; k >= 0; --k)
@@ -305,7 +319,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
           bool in_gap = false;
           while (i > 0 && j > 0) {
             if (j < R.lo(i) || j > R.hi(i) || i < rlo) break;
-            const uint8_t tv = tr[(uint64_t)(i - rlo) * W + (j - i + d1)];
+            const uint8_t tv = tr[(uint64_t)(i - rlo) * (64 * K) + (j - i + d1)];
             uint32_t code;
             if (state == 0) {
               const uint8_t c = tv & 3;
