@@ -78,8 +78,10 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, co
                        const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
                        hipStream_t st);
 int sw16_width(int cls);
-void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                 const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
+// Packed int16 score pass of width class cls (k_sw16.hip); split = two lanes
+// per DP pair (half band each).
+void launch_sw16(int cls, bool split, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q,
+                 SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
                  uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,

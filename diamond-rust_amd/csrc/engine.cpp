@@ -64,6 +64,10 @@ Engine::Engine(int device) : device_(device) {
   cu_count_ = prop.multiProcessorCount;
   const char* fg = std::getenv("DMR_FORCE_GENERIC");
   force_generic_ = fg && fg[0] == '1';
+  // Width classes whose packed kernel runs split (two lanes per DP pair):
+  // bit c for class c; DMR_SW16_SPLIT overrides the default (81 and 129).
+  const char* sp = std::getenv("DMR_SW16_SPLIT");
+  split_mask_ = sp ? (unsigned)std::strtoul(sp, nullptr, 0) : 0x18u;
 }
 
 Engine::~Engine() {
@@ -623,8 +627,8 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
       if (!class_lanes[c]) continue;
       hipStream_t s = class_stream(c);
       DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
-      launch_sw16(c, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs, pair_tab_.as<int>(),
-                  matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, cus, s);
+      launch_sw16(c, ((split_mask_ >> c) & 1u) != 0, lanes + class_lane0[c], class_lanes[c], jobs, qs,
+                  dbs, pair_tab_.as<int>(), matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, cus, s);
       DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
     }
   }

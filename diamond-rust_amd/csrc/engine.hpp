@@ -239,6 +239,7 @@ class Engine {
   std::vector<uint64_t> last_cbeg_, last_cend_;
   int cu_count_ = 256;
   bool force_generic_ = false;
+  unsigned split_mask_ = 0x18u;
   uint64_t last_njobs_ = 0;
   int last_target_bits_ = 0;
   double last_ms_class_[kClasses] = {};

@@ -23,6 +23,8 @@
 //   reference). Their H is at most max(11, H(m + d1, m) - o), so they only
 //   matter when the best score is below 11: such DPs are re-run in int32.
 // Saturating adds detect scores >= 32767; those DPs are re-run in int32.
+#include <type_traits>
+
 #include "dmr_kernels.hpp"
 
 namespace dmr {
@@ -234,6 +236,230 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
   }
 }
 
+// Split-band variant: each DP pair is carried by P lanes (P = 2 or 4, one
+// lane group), lane p of the group holding band offsets p*H .. p*H+H-1 with
+// H = ceil((W + 1) / P); offsets >= W are the wall right of the band (and
+// dead padding past it) and stay NEG. A lane keeps H instead of W offsets, so
+// the state fits far fewer registers and more waves share a SIMD.
+//
+// Within a row the halves of the band depend on each other only through the
+// horizontal gap F entering a lane's first offset. Every lane first computes
+// its offsets with no incoming F; then F_in of lane p is
+//   F_in(p) = max_{q<p} (F_out(q) - (p-1-q) * H),
+// F_out(q) being the F a lane passes to the offset after its last, an
+// exclusive prefix maximum over the group. The lane is then corrected with
+// F_true(t0+u) = max(F_local(t0+u), F_in - u): by induction H_true = max(H_local,
+// F_in - u), and a gap reopened from a cell the chain raised (H - o) never
+// beats the chain's own extension (- e). Only G = H - o and the best score
+// carry to the next row, so the correction is G = max(G, F_in - o - u) and
+// best = max(best, F_in); it is skipped on rows where no lane's F_in is
+// positive (it cannot beat H >= 0 then). E crosses lanes from the previous
+// row: a lane's last offset reads the next lane's first.
+template <int W, int P>
+struct SplitShape {
+  static constexpr int H = (W + 1 + P - 1) / P;  // offsets per lane
+  static constexpr int NR = (H + 1) / 2;         // window registers
+  static constexpr int kDeadFrom = W - (P - 1) * H;  // first dead offset of the last lane
+};
+
+template <int W, int P, int kThreads>
+__global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes,
+                                                   const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
+                                                   const int* __restrict__ pair_tab,
+                                                   uint32_t* __restrict__ counter,
+                                                   SwOut* __restrict__ outs,
+                                                   uint32_t* __restrict__ redo,
+                                                   uint32_t* __restrict__ nredo) {
+  static_assert(W % 2 == 1, "band widths are odd");
+  static_assert(P == 2 || P == 4, "lane groups of 2 or 4");
+  constexpr int H = SplitShape<W, P>::H;
+  constexpr int NR = SplitShape<W, P>::NR;
+  constexpr int kDeadFrom = SplitShape<W, P>::kDeadFrom;
+  static_assert(kDeadFrom >= 1 && kDeadFrom <= H, "the wall lies in the last lane");
+  __shared__ int T[kPairTab];
+  for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int p = lane & (P - 1);
+  const bool last = p == P - 1;
+  const int t0 = p * H;  // band offset of this lane's first entry
+  const v2s NEG2 = {kNeg16, kNeg16};
+  const v2s ONE2 = {1, 1};
+  const v2s OPEN2 = {kOpen, kOpen};
+  const v2s ZERO2 = {0, 0};
+  // Moves within a lane group (quad_perm): from the next lane, from the
+  // previous lane, from two lanes back, and the group-wide maximum.
+  constexpr int kFromNext = P == 4 ? 0xF9 : 0xF5;  // [1,2,3,3] / [1,1,3,3]
+  constexpr int kFromPrev = P == 4 ? 0x90 : 0xA0;  // [0,0,1,2] / [0,0,2,2]
+  auto dppi = [](int v, auto ctrl) { return __builtin_amdgcn_update_dpp(v, v, decltype(ctrl)::value, 0xf, 0xf, false); };
+  auto mv = [&](v2s v, auto ctrl) { return as_v2s(dppi(__builtin_bit_cast(int, v), ctrl)); };
+  using NextC = std::integral_constant<int, kFromNext>;
+  using PrevC = std::integral_constant<int, kFromPrev>;
+  using Back2C = std::integral_constant<int, 0x40>;   // [0,0,0,1]
+  using SwapC = std::integral_constant<int, 0xB1>;    // [1,0,3,2]
+  using Swap2C = std::integral_constant<int, 0x4E>;   // [2,3,0,1]
+  auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
+  for (;;) {
+    const uint32_t task =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
+    if ((uint64_t)task * (64 / P) >= nlanes) break;
+    const uint32_t L = task * (64 / P) + lane / P;
+    const bool active = L < nlanes;
+    uint2 ln = active ? lanes[L] : make_uint2(~0u, ~0u);
+    const bool hasA = ln.x != ~0u, hasB = ln.y != ~0u;
+    SwJob ja = {}, jb = {};
+    if (hasA) ja = jobs[ln.x];
+    if (hasB) jb = jobs[ln.y];
+    const uint32_t qo = hasA ? q.offs[ja.target] : 0;
+    const int n = hasA ? (int)(q.offs[ja.target + 1] - qo) : 0;
+    const uint32_t ao = hasA ? db.offs[ja.subject] : 0, bo = hasB ? db.offs[jb.subject] : 0;
+    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - ao) : 0;
+    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - bo) : 0;
+    const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
+    const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
+    // Rows as in k_sw16 (all lanes of a group have the same range).
+    int lo = 0, hi = -1;
+    if (hasA) {
+      lo = max(1, d0A + 1);
+      hi = min(n, mA + d1A);
+    }
+    if (hasB) {
+      lo = min(lo, max(1, d0B + 1));
+      hi = max(hi, min(n, mB + d1B));
+    }
+    const int steps = __builtin_amdgcn_readfirstlane(wave_max(hi - lo + 1));
+
+    // G/E of this lane's offsets; entry H is the E source past the last one
+    // (the next lane's first offset, or the wall for the last lane).
+    v2s G[H + 1], E[H + 1];
+    uint32_t R[NR];
+#pragma unroll
+    for (int u = 0; u <= H; ++u) {
+      const bool dead = u == H || (last && u >= kDeadFrom);
+      G[u] = dead ? NEG2 : v2s{-kOpen, -kOpen};
+      E[u] = NEG2;
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int t = t0 + 2 * k;
+      uint32_t lo16 = 4u * (letter_at(db.res, ao, mA, lo - d1A + t - 1) * kCodes +
+                            letter_at(db.res, bo, mB, lo - d1B + t - 1));
+      uint32_t hi16 = 2 * k + 1 < H ? 4u * (letter_at(db.res, ao, mA, lo - d1A + t) * kCodes +
+                                            letter_at(db.res, bo, mB, lo - d1B + t))
+                                    : 0u;
+      R[k] = lo16 | (hi16 << 16);
+    }
+    // Letters of the next row, loaded one row ahead (clamped, unconditional
+    // loads; padding selected after a VALU copy at the end of the row): the
+    // query letter and the pair entering the last lane at offset P*H-1.
+    auto qraw = [&](int i) { return (int)q.res[qo + (uint32_t)(max(min(i, n), 1) - 1)]; };
+    auto sraw = [&](uint32_t off, int m, int pos) {
+      return (int)db.res[off + (uint32_t)min(max(pos, 0), max(m - 1, 0))];
+    };
+    int qcur = lo <= n ? qraw(lo) : 25;
+    uint32_t pnext = 0;
+    v2s best = ZERO2;
+    for (int r = 0; r < steps; ++r) {
+      const int i = lo + r;
+      if (r > 0) {
+        // Shift the window by one entry; the entry entering at H-1 is the
+        // next lane's old first entry, or the new pair for the last lane.
+        const uint32_t first = (uint32_t)dppi((int)R[0], NextC{}) & 0xFFFFu;
+#pragma unroll
+        for (int k = 0; k < NR - 1; ++k) R[k] = __builtin_amdgcn_alignbit(R[k + 1], R[k], 16);
+        const uint32_t entering = last ? pnext : first;
+        if (H & 1) R[NR - 1] = entering;                             // entry H-1: low half
+        else R[NR - 1] = (R[NR - 1] >> 16) | (entering << 16);  // H-2 <- H-1, H-1 high
+      }
+      // Raw letters of row i + 1: subject positions i + 1 - d1 + P*H - 2.
+      const int pos = i - 1 + P * H;
+      const int rq = qraw(i + 1), ra = sraw(ao, mA, pos - d1A), rb = sraw(bo, mB, pos - d1B);
+      {
+        // E source past the lane: the next lane's first offset, previous row.
+        const v2s g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
+        G[H] = last ? NEG2 : g0;
+        E[H] = last ? NEG2 : e0;
+      }
+      const char* Tq = reinterpret_cast<const char*>(T) + qcur * (4 * kPairRow);
+      auto score = [&](int u) {
+        const uint32_t off = (u & 1) ? (R[u >> 1] >> 16) : (R[u >> 1] & 0xFFFFu);
+        return *reinterpret_cast<const int*>(Tq + off);
+      };
+      constexpr int kAhead = H < 4 ? H : 4;
+      int sq[kAhead];
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) sq[u] = score(u);
+      v2s F = NEG2, Gl = NEG2;
+#pragma unroll
+      for (int u = 0; u < H; ++u) {
+        const v2s sc = as_v2s(sq[u % kAhead]);
+        if (u + kAhead < H) sq[u % kAhead] = score(u + kAhead);
+        __builtin_amdgcn_sched_barrier(0);
+        const v2s D = vadds(G[u], sc);
+        const v2s Ev = vmax(G[u + 1], vsubs(E[u + 1], ONE2));
+        const v2s Fv = vmax(Gl, vsubs(F, ONE2));
+        const v2s Hh = vmax(vmax(D, Ev), vmax(Fv, ZERO2));
+        best = vmax(best, Hh);
+        Gl = vsubs(Hh, OPEN2);
+        G[u] = Gl;
+        E[u] = Ev;
+        F = Fv;
+      }
+      // F entering each lane: exclusive prefix maximum of F_out(q) + q*H over
+      // the group, minus (p-1)*H.
+      const v2s fout = vmax(Gl, vsubs(F, ONE2));
+      v2s y = vadds(fout, v2s{(short)(p * H), (short)(p * H)});
+      y = vmax(y, mv(y, PrevC{}));
+      if (P == 4) y = vmax(y, mv(y, Back2C{}));
+      v2s fin = vsubs(mv(y, PrevC{}), v2s{(short)((p - 1) * H), (short)((p - 1) * H)});
+      if (p == 0) fin = NEG2;
+      best = vmax(best, fin);
+      if (__ballot(max((int)fin.x, (int)fin.y) > 0) != 0) {
+        v2s fc = vsubs(fin, OPEN2);
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+          G[u] = vmax(G[u], fc);
+          fc = vsubs(fc, ONE2);
+        }
+      }
+      if (last) {
+#pragma unroll
+        for (int u = kDeadFrom; u < H; ++u) {
+          G[u] = NEG2;
+          E[u] = NEG2;
+        }
+      }
+      // Letters of row i + 1, padding selected as in letter_at.
+      const int cq = copy(rq), ca = copy(ra), cb = copy(rb);
+      qcur = i + 1 <= n ? cq : 25;
+      const int la = (pos - d1A >= 0 && pos - d1A < mA) ? ca : 25;
+      const int lb = (pos - d1B >= 0 && pos - d1B < mB) ? cb : 25;
+      pnext = (uint32_t)(4 * (la * kCodes + lb));
+    }
+    best = vmax(best, mv(best, SwapC{}));
+    if (P == 4) best = vmax(best, mv(best, Swap2C{}));
+    const int bA = best.x, bB = best.y;
+    if (p == 0) {
+      if (hasA) {
+        if (bA >= 32767 || (bA < 11 && mA + d1A < n)) {
+          redo[atomicAdd(nredo, 1u)] = ln.x;
+        } else {
+          outs[ln.x].score = bA;
+          outs[ln.x].flags = 0;
+        }
+      }
+      if (hasB) {
+        if (bB >= 32767 || (bB < 11 && mB + d1B < n)) {
+          redo[atomicAdd(nredo, 1u)] = ln.y;
+        } else {
+          outs[ln.y].score = bB;
+          outs[ln.y].flags = 0;
+        }
+      }
+    }
+  }
+}
+
 // Band-width classes served by k_sw16 (2 * band + 1 for each band_for value).
 __device__ __forceinline__ int width_class(int w) {
   switch (w) {
@@ -332,19 +558,27 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, co
 static const int kWidths[5] = {25, 33, 61, 81, 129};
 int sw16_width(int cls) { return kWidths[cls]; }
 
-void launch_sw16(int cls, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                 const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
+void launch_sw16(int cls, bool split, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q,
+                 SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
                  uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!nlanes) return;
-  unsigned tasks = nblk(nlanes, 64);
+  // A wave carries 64 DP pairs (one per lane) or, split, 16 (four lanes each).
+  const unsigned tasks = nblk(nlanes, split ? 16 : 64);
   switch (cls) {
 #define DMR_SW16(c, w)                                                                              \
-  case c: {                                                                                         \
-    static const unsigned resident = resident_blocks(k_sw16<w>, 256, cu_count);                    \
-    const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));                           \
-    hipLaunchKernelGGL(k_sw16<w>, dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,       \
-                       pair_tab, m32, counter, outs, redo, nredo);                                  \
-  } break;
+  case c:                                                                                           \
+    if (split) {                                                                                    \
+      static const unsigned resident = resident_blocks(k_sw16p<w, 4, 512>, 512, cu_count);         \
+      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));                         \
+      hipLaunchKernelGGL((k_sw16p<w, 4, 512>), dim3(blocks), dim3(512), 0, st, lanes, nlanes, jobs, \
+                         q, db, pair_tab, counter, outs, redo, nredo);                              \
+    } else {                                                                                        \
+      static const unsigned resident = resident_blocks(k_sw16<w>, 256, cu_count);                  \
+      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));                         \
+      hipLaunchKernelGGL(k_sw16<w>, dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,     \
+                         pair_tab, m32, counter, outs, redo, nredo);                                \
+    }                                                                                               \
+    break;
     DMR_SW16(0, 25)
     DMR_SW16(1, 33)
     DMR_SW16(2, 61)
