@@ -803,6 +803,12 @@ int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double
   });
 }
 
+int dmr_score_split_mask(dmr_ctx* ctx, uint32_t* mask) {
+  if (!ctx || !mask) return fail(DMR_E_ARG, "NULL argument");
+  *mask = ctx->eng->split_mask();
+  return DMR_OK;
+}
+
 void dmr_result_destroy(dmr_result* r) { delete r; }
 
 }  // extern "C"

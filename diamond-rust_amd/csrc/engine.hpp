@@ -128,6 +128,7 @@ class Engine {
   std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats);
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
+  unsigned split_mask() const { return split_mask_; }
   static constexpr int kClasses = 12;  // k_classify in k_sw16.hip
   static constexpr int kRowExact = kClasses - 1;
   // Per score-kernel class of the last search (see StageStats::ms_class):

@@ -219,6 +219,10 @@ int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
  * milliseconds. Diagnostics only; no counterpart in the reference. */
 #define DMR_SCORE_CLASSES 12
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
+/* Layout of the packed int16 classes 0-4 on ctx: bit c set = class c runs
+ * split over four lanes per DP pair (k_sw16p), clear = one lane (k_sw16).
+ * Diagnostics only; no counterpart in the reference. */
+int dmr_score_split_mask(dmr_ctx* ctx, uint32_t* mask);
 void dmr_result_destroy(dmr_result* r);
 
 #ifdef __cplusplus
