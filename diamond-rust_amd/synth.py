@@ -186,6 +186,81 @@ def protein_db_2p26(seed=20240601, n_queries=1024):
     return db, ids, q1, qs
 
 
+# Standard genetic code, amino acid -> synonymous codons (reverse translation
+# of planted proteins into reads).
+_CODONS = {
+    "A": ["GCT", "GCC", "GCA", "GCG"], "R": ["CGT", "CGC", "CGA", "CGG", "AGA", "AGG"],
+    "N": ["AAT", "AAC"], "D": ["GAT", "GAC"], "C": ["TGT", "TGC"], "Q": ["CAA", "CAG"],
+    "E": ["GAA", "GAG"], "G": ["GGT", "GGC", "GGA", "GGG"], "H": ["CAT", "CAC"],
+    "I": ["ATT", "ATC", "ATA"], "L": ["TTA", "TTG", "CTT", "CTC", "CTA", "CTG"],
+    "K": ["AAA", "AAG"], "M": ["ATG"], "F": ["TTT", "TTC"], "P": ["CCT", "CCC", "CCA", "CCG"],
+    "S": ["TCT", "TCC", "TCA", "TCG", "AGT", "AGC"], "T": ["ACT", "ACC", "ACA", "ACG"],
+    "W": ["TGG"], "Y": ["TAT", "TAC"], "V": ["GTT", "GTC", "GTA", "GTG"],
+}
+_COMP = str.maketrans("ACGT", "TGCA")
+
+
+class ReadWorkload:
+    """Nucleotide reads (ASCII) against a protein DB, for blastx."""
+
+    def __init__(self, name, db: Workload, reads, read_ids, meta):
+        self.name = name
+        self.db_letters, self.db_offsets, self.db_ids = db.db_letters, db.db_offsets, db.db_ids
+        self.n_db = db.n_db
+        self.reads = reads
+        self.read_ids = read_ids
+        self.meta = meta
+
+
+def _reads(rng, db_seqs, n_reads, lo_nt, hi_nt, planted_frac):
+    """Reads of lo_nt..hi_nt bases: reverse-translated segments of DB
+    proteins (random synonymous codons, 1% base substitutions, 0.1% single-base
+    indels, either strand) or random bases."""
+    reads = []
+    for _ in range(n_reads):
+        n_nt = int(rng.integers(lo_nt, hi_nt + 1))
+        if rng.random() < planted_frac:
+            prot = db_seqs[int(rng.integers(0, len(db_seqs)))]
+            n_aa = min(len(prot), max(1, n_nt // 3))
+            a = int(rng.integers(0, len(prot) - n_aa + 1))
+            seq = "".join(rng.choice(_CODONS[AA[c]]) for c in prot[a:a + n_aa])
+            out = []
+            for ch in seq:
+                r = rng.random()
+                if r < 0.0005:
+                    continue  # deletion
+                if r < 0.001:
+                    out.append("ACGT"[int(rng.integers(0, 4))])  # insertion
+                out.append("ACGT"[int(rng.integers(0, 4))] if rng.random() < 0.01 else ch)
+            seq = "".join(out)
+            if rng.random() < 0.5:
+                seq = seq.translate(_COMP)[::-1]
+        else:
+            seq = "".join(rng.choice(list("ACGT"), size=n_nt))
+        reads.append(seq)
+    return reads
+
+
+def make_reads(name: str, seed: int = 20240601) -> ReadWorkload:
+    """Named blastx workloads:
+    'blastx-tiny'  : 40 reads of 60..900 bases vs the 'tiny' 400-protein DB
+    'blastx-2048'  : 2048 reads of 150..3000 bases vs the 2.26 Maa DB
+    """
+    rng = np.random.default_rng(seed + 7)
+    if name == "blastx-tiny":
+        db = make("tiny", seed)
+        db_seqs = [db.db_letters[db.db_offsets[i]:db.db_offsets[i + 1]] for i in range(db.n_db)]
+        reads = _reads(rng, db_seqs, 40, 60, 900, 0.7)
+    elif name == "blastx-2048":
+        db = make("blastp-sensitive-1024", seed)
+        db_seqs = [db.db_letters[db.db_offsets[i]:db.db_offsets[i + 1]] for i in range(db.n_db)]
+        reads = _reads(rng, db_seqs, 2048, 150, 3000, 0.5)
+    else:
+        raise ValueError(f"unknown read workload {name}")
+    return ReadWorkload(name, db, reads, [f"r{i:05d}" for i in range(len(reads))],
+                        dict(seed=seed, reads=len(reads)))
+
+
 def make(name: str, seed: int = 20240601) -> Workload:
     """Named workloads:
     'blastp-default-1x497'   : 1 query of 497 aa vs the 2.26 Maa DB

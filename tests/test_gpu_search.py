@@ -64,3 +64,21 @@ def test_sensitive_batch_subset(dmr, ctx, orc, synth):
     c = ref.counters()
     assert st["hits"] == c["hits"]
     assert st["jobs"] == c["bands"]
+
+
+@pytest.mark.parametrize("sens", ["default", "sensitive"])
+def test_blastx_reads(dmr, ctx, orc, synth, sens):
+    """blastx: six frames per read, --min-orf masking, nucleotide coordinates
+    and frames in the output (src/workflow.rs:112-127, src/translate.rs:54-137),
+    against the oracle on reads from both strands."""
+    w = synth.make_reads("blastx-tiny")
+    ctx.db_load(w.db_letters, w.db_offsets)
+    res = ctx.blastx(w.reads, dmr.SearchConfig(sensitivity=sens))
+    nt = np.frombuffer("".join(w.reads).encode(), dtype=np.uint8)
+    off = np.zeros(len(w.reads) + 1, dtype=np.uint64)
+    np.cumsum([len(r) for r in w.reads], out=off[1:])
+    ref = orc.search(w.db_letters, w.db_offsets, nt, off, sensitivity=sens, threads=8, translated=True)
+    assert len(ref) > 0
+    exp = ref.format(w.read_ids, w.db_ids, EXTENDED)
+    assert "\t-" in exp and "\t+" in exp  # hits on both strands
+    assert res.format(w.read_ids, w.db_ids, EXTENDED) == exp
