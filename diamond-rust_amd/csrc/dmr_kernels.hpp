@@ -101,6 +101,10 @@ unsigned resident_blocks(Kernel kernel, int threads, int cu_count) {
   return (unsigned)per_cu * (unsigned)(cu_count > 0 ? cu_count : 1);
 }
 
+// Score-pass sort key of a DP job: class | target | row field, the row field
+// being (row span / 32, first row / 32), 9 bits each.
+constexpr int kRowKeyBits = 18;
+
 // Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width
 // <= 64*K, K a power of two up to kMaxWaveK (0 = wider than 64*kMaxWaveK).
 constexpr int kMaxWaveK = 32;

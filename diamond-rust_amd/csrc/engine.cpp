@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -66,6 +67,8 @@ Engine::Engine(int device) : device_(device) {
   force_generic_ = fg && fg[0] == '1';
   // Width classes whose packed kernel runs split (two lanes per DP pair):
   // bit c for class c; DMR_SW16_SPLIT overrides the default (81 and 129).
+  const char* lr = std::getenv("DMR_LANE_REPORT");
+  lane_report_ = lr && lr[0] == '1';
   const char* sp = std::getenv("DMR_SW16_SPLIT");
   split_mask_ = sp ? (unsigned)std::strtoul(sp, nullptr, 0) : 0x18u;
 }
@@ -557,13 +560,13 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
   launch_classify(jobs, njobs, qs, dbs, target_bits, keys, idx, st_);
-  const int key_bits = 4 + target_bits + 14;
+  const int key_bits = 4 + target_bits + kRowKeyBits;
   radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
                    njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
                    b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_);
   // Runs of equal (class, target): their start positions and keys.
   uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
-  launch_key_flags(keys, njobs, 14, fl, st_);
+  launch_key_flags(keys, njobs, kRowKeyBits, fl, st_);
   launch_fill_u32(fl + njobs, 1, 0, st_);
   uint64_t* ps = pos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
@@ -578,7 +581,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   DMR_HIP(hipMemcpyAsync(h_keys.data(), rk, nruns * 8, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
   h_rs[nruns] = (uint32_t)njobs;
-  const int cls_shift = target_bits + 14;
+  const int cls_shift = target_bits + kRowKeyBits;
   // Class ranges in the sorted order; lane pairs for the int16 classes.
   uint64_t cbeg[kClasses], cend[kClasses];
   for (int c = 0; c < kClasses; ++c) cbeg[c] = cend[c] = 0;
@@ -613,6 +616,8 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     DMR_HIP(hipMemcpyAsync(d_rs + 2 * nr, lane_off.data(), nr * 4, hipMemcpyHostToDevice, st_));
     lanes = sw_lanes_.ensure<uint2>(nl + 1);
     launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, nl, lanes, st_);
+    if (lane_report_ && qs.offs == q_offs_.as<uint32_t>())
+      report_lanes(jobs, njobs, lanes, nl, class_lane0, class_lanes);
   }
   // The class kernels are independent (distinct jobs and counters; the redo
   // and fallback lists are filled through atomics): they run side by side on
@@ -696,6 +701,57 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     S->ms_class[kRowExact] = stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
   }
   for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S->ms_class[c];
+}
+
+// Diagnostics (DMR_LANE_REPORT=1): per packed class, the band cells of its
+// DPs against the cell slots its lanes and waves step through (a lane runs
+// the union of its two DPs' rows; a wave runs as many rows as its longest
+// lane or lane group).
+void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
+                          const uint32_t* class_lane0, const uint32_t* class_lanes) {
+  std::vector<SwJob> hj(njobs);
+  std::vector<uint2> hl(nl);
+  DMR_HIP(hipMemcpyAsync(hj.data(), jobs, njobs * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipMemcpyAsync(hl.data(), lanes, nl * sizeof(uint2), hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  auto rows = [&](uint32_t k, int64_t* lo, int64_t* hi) {
+    const SwJob& j = hj[k];
+    const int64_t n = (int64_t)j.q1 - j.q0;
+    const int64_t m = (int64_t)h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
+    *lo = std::max<int64_t>(1, (int64_t)j.d0 + 1);
+    *hi = std::min<int64_t>(n, m + j.d1);
+  };
+  for (int c = 0; c < 5; ++c) {
+    if (!class_lanes[c]) continue;
+    const int64_t W = sw16_width(c);
+    const int P = ((split_mask_ >> c) & 1u) ? 4 : 1;
+    const int per_wave = 64 / P;
+    uint64_t band = 0, pair_slots = 0, wave_slots = 0;
+    int64_t wave_max = 0, in_wave = 0;
+    for (uint32_t t = 0; t < class_lanes[c]; ++t) {
+      const uint2 ln = hl[class_lane0[c] + t];
+      int64_t lo = INT64_MAX, hi = INT64_MIN, a, b;
+      for (uint32_t k : {ln.x, ln.y}) {
+        if (k == ~0u) continue;
+        rows(k, &a, &b);
+        band += (uint64_t)std::max<int64_t>(0, b - a + 1) * W;
+        lo = std::min(lo, a);
+        hi = std::max(hi, b);
+      }
+      const int64_t span = std::max<int64_t>(0, hi - lo + 1);
+      pair_slots += (uint64_t)span * W * 2;
+      wave_max = std::max(wave_max, span);
+      if (++in_wave == per_wave || t + 1 == class_lanes[c]) {
+        wave_slots += (uint64_t)wave_max * W * 2 * per_wave;
+        wave_max = 0;
+        in_wave = 0;
+      }
+    }
+    std::fprintf(stderr, "[lanes] class %d (W %lld, %s): %u pairs, band cells %.3g, pair slots %.3g (x%.2f), "
+                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P == 4 ? "split" : "full", class_lanes[c],
+                 (double)band, (double)pair_slots, (double)pair_slots / std::max<double>(1, band),
+                 (double)wave_slots, (double)wave_slots / std::max<double>(1, band));
+  }
 }
 
 bool Engine::swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, bool fast) const {

@@ -241,6 +241,9 @@ class Engine {
   int cu_count_ = 256;
   bool force_generic_ = false;
   unsigned split_mask_ = 0x18u;
+  bool lane_report_ = false;
+  void report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
+                    const uint32_t* class_lane0, const uint32_t* class_lanes);
   uint64_t last_njobs_ = 0;
   int last_target_bits_ = 0;
   double last_ms_class_[kClasses] = {};

@@ -78,6 +78,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
   const v2s ONE2 = {1, 1};
   const v2s OPEN2 = {kOpen, kOpen};
   const v2s ZERO2 = {0, 0};
+  const v2s ZOPQ = as_v2s(__builtin_amdgcn_update_dpp(0, 0, 0xE4, 0xf, 0xf, false));
   for (;;) {
     // Branch-free task fetch (every lane joins the atomic, lane 0 adds 1):
     // the task lives in a scalar register and the loop exit is uniform.
@@ -197,8 +198,11 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         __builtin_amdgcn_sched_barrier(0);
         const v2s D = vadds(G[t], sc);
         const v2s Ev = vmax(G[t + 1], vsubs(E[t + 1], ONE2));
+        // max(D, E, 0) is off the F chain that runs along the row (opaque
+        // zero: the max tree is not reassociated back onto the chain).
+        const v2s X0 = vmax(vmax(D, Ev), ZOPQ);
         const v2s Fv = vmax(Gl, vsubs(F, ONE2));
-        const v2s H = vmax(vmax(D, Ev), vmax(Fv, ZERO2));
+        const v2s H = vmax(X0, Fv);
         best = vmax(best, H);
         Gl = vsubs(H, OPEN2);
         G[t] = Gl;
@@ -287,6 +291,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   const v2s ONE2 = {1, 1};
   const v2s OPEN2 = {kOpen, kOpen};
   const v2s ZERO2 = {0, 0};
+  const v2s ZOPQ = as_v2s(__builtin_amdgcn_update_dpp(0, 0, 0xE4, 0xf, 0xf, false));
   // Moves within a lane group (quad_perm): from the next lane, from the
   // previous lane, from two lanes back, and the group-wide maximum.
   constexpr int kFromNext = P == 4 ? 0xF9 : 0xF5;  // [1,2,3,3] / [1,1,3,3]
@@ -397,8 +402,12 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         __builtin_amdgcn_sched_barrier(0);
         const v2s D = vadds(G[u], sc);
         const v2s Ev = vmax(G[u + 1], vsubs(E[u + 1], ONE2));
+        // max(D, E, 0) does not depend on F: only max(., F) and the G
+        // subtraction sit on the chain that runs along the row. (The zero is
+        // opaque so the max tree is not reassociated back onto the chain.)
+        const v2s X0 = vmax(vmax(D, Ev), ZOPQ);
         const v2s Fv = vmax(Gl, vsubs(F, ONE2));
-        const v2s Hh = vmax(vmax(D, Ev), vmax(Fv, ZERO2));
+        const v2s Hh = vmax(X0, Fv);
         best = vmax(best, Hh);
         Gl = vsubs(Hh, OPEN2);
         G[u] = Gl;
@@ -475,7 +484,10 @@ __device__ __forceinline__ int width_class(int w) {
 // Sort key per job: class | target | rows. Classes 0-4: packed int16 kernel
 // for band widths 25/33/61/81/129; 5-10: wave-per-DP int32 kernel with
 // K = 1..32 offsets per lane (widths up to 64 * K); 11: row-exact int32
-// kernel (wider bands).
+// kernel (wider bands). Within a target the jobs are ordered by row span,
+// then first row (32-row buckets): the packed kernels pair consecutive jobs
+// in a lane, which runs the union of their rows, and a wave runs as many
+// rows as its longest lane, so neighbours should start and end alike.
 __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSet q, SeqSet db,
                            int target_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
   uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -498,8 +510,11 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
         break;
       }
   }
-  const uint64_t r = (uint64_t)(rhi < 0 ? 0 : (rhi > 16383 ? 16383 : rhi));
-  keys[k] = ((uint64_t)cls << (target_bits + 14)) | ((uint64_t)j.target << 14) | r;
+  const int64_t span = rhi - rlo + 1;
+  const uint64_t sb = (uint64_t)(span < 0 ? 0 : (span > 16383 ? 16383 : span)) >> 5;
+  const uint64_t lb = (uint64_t)(rlo > 16383 ? 16383 : rlo) >> 5;
+  keys[k] = ((uint64_t)cls << (target_bits + kRowKeyBits)) | ((uint64_t)j.target << kRowKeyBits) |
+            (sb << 9) | lb;
   idx[k] = (uint32_t)k;
 }
 
