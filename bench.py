@@ -12,7 +12,7 @@ renders the outfmt-6 text.
 Prints one JSON line on stdout; progress goes to stderr.
   value        queries/s (whole job, one GPU)
   gcups        banded-DP cells / score-kernel time
-  roofline     the score kernel against the int16 VALU peak
+  roofline     the score kernel against the packed 16-bit VALU peak
   cpu_baseline the oracle (CPU restatement of the reference path, all host
                threads up to 16) on a bounded query sample
   plain_form   the per-call drop-in form on a small sample
@@ -31,7 +31,7 @@ import numpy as np
 
 ROOT = pathlib.Path(__file__).resolve().parent
 METRIC = "queries/s + gapped GCUPS at --sensitive, bit-exact outfmt6; p50/p99 latency"
-INT16_VALU_PEAK_TOPS = 78.6   # 256 CU x 64 lanes x 2 (packed) x 2.4 GHz
+PACKED16_VALU_PEAK_TOPS = 78.6   # 256 CU x 64 lanes x 2 (packed int16/f16) x 2.4 GHz
 OPS_PER_CELL = 11             # E 3, F 3, H 1 add + 3 max, best 1
 
 
@@ -118,7 +118,7 @@ def main():
     sw = float(np.mean(sw_ms))
     gcups = cells / (sw / 1e3) / 1e9
     # Roofline of the dominant score kernel (the class with the most DP
-    # cells): algorithmic cells of its jobs x 11 int16 ops / its launch time.
+    # cells): algorithmic cells of its jobs x 11 16-bit ops / its launch time.
     # The classes run concurrently on separate streams, so a launch's time
     # includes sharing the chip; the score pass as a whole is reported too.
     dom = max(prof, key=lambda k: prof[k]["cells"])
@@ -134,7 +134,7 @@ def main():
         "metric": METRIC, "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int16" if dom.startswith("w") and not dom.startswith("wave") else "int32",
+        "dtype": ("f16" if "F16" in kname else "int16") if kname.startswith("k_sw16") else "int32",
         "data": "synthetic (seeded generator diamond-rust_amd/synth.py)",
         "config": {"workload": f"{w.name}: blastp --sensitive, {w.n_q} queries vs {w.n_db} "
                                f"proteins / {int(w.db_offsets[-1])} letters",
@@ -149,20 +149,20 @@ def main():
         "hsps": text.count("\n"),
         "roofline": {"bound": "valu",
                      "kernel": kname,
-                     "achieved": round(achieved, 3), "peak": INT16_VALU_PEAK_TOPS,
-                     "unit": "Tops/s int16", "frac": round(achieved / INT16_VALU_PEAK_TOPS, 4),
+                     "achieved": round(achieved, 3), "peak": PACKED16_VALU_PEAK_TOPS,
+                     "unit": "Tops/s (16-bit lanes)", "frac": round(achieved / PACKED16_VALU_PEAK_TOPS, 4),
                      "traffic": traffic,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                                      "copy-calibrated; profiles/hbm_traffic.json)",
                      "gcups": round(dcells / (dms / 1e3) / 1e9, 2),
-                     "basis": f"{OPS_PER_CELL} int16 ops per reference DP cell x cells of the "
+                     "basis": f"{OPS_PER_CELL} 16-bit ops per reference DP cell x cells of the "
                               "launch / launch time (HIP events on the engine stream)"},
         "score_pass_classes": prof,
         "roofline_score_pass": {
             "bound": "valu", "achieved": round(cells * OPS_PER_CELL / (sw / 1e3) / 1e12, 3),
-            "peak": INT16_VALU_PEAK_TOPS, "unit": "Tops/s int16",
-            "frac": round(cells * OPS_PER_CELL / (sw / 1e3) / 1e12 / INT16_VALU_PEAK_TOPS, 4),
-            "basis": "all banded-DP cells x 11 int16 ops / score-pass wall time (all classes, "
+            "peak": PACKED16_VALU_PEAK_TOPS, "unit": "Tops/s (16-bit lanes)",
+            "frac": round(cells * OPS_PER_CELL / (sw / 1e3) / 1e12 / PACKED16_VALU_PEAK_TOPS, 4),
+            "basis": "all banded-DP cells x 11 16-bit ops / score-pass wall time (all classes, "
                      "concurrent streams joined)"},
     }
 

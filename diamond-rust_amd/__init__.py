@@ -34,21 +34,23 @@ EXPORTS = [
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
     "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
     "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
-    "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy", "dmr_score_split_mask",
+    "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy", "dmr_score_layout",
 ]
 
 # Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
 SCORE_CLASSES = ["w25", "w33", "w61", "w81", "w129", "wave64", "wave128", "wave256", "wave512",
                  "wave1024", "wave2048", "rowexact"]
 
-def _class_kernel(i: int, name: str, split_mask: int) -> str:
-    """Kernel (template instance, as rocprofv3 names it) that scores class i."""
+def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
+    """Kernel that scores class i: its template instance as rocprofv3 names it,
+    without the dmr namespaces."""
     if name.startswith("wave"):
         return f"k_sw_wave<{int(name[4:]) // 64}, false>"
     if name == "rowexact":
         return "k_sw_generic<false>"
     w = int(name[1:])
-    return f"k_sw16p<{w}, 4, 512>" if (split_mask >> i) & 1 else f"k_sw16<{w}>"
+    tr = "F16" if f16 else "I16"
+    return f"k_sw16p<{w}, 4, 512, {tr}>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
 
 
 HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
@@ -159,7 +161,7 @@ def lib():
             "dmr_result_format": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, U64, P], C.c_int),
             "dmr_result_stats": ([P, P], C.c_int), "dmr_last_dp_cells": ([P, P], C.c_int),
             "dmr_last_class_profile": ([P, P, P, P], C.c_int),
-            "dmr_score_split_mask": ([P, P], C.c_int),
+            "dmr_score_layout": ([P, P, P], C.c_int),
             "dmr_result_destroy": ([P], None), "dmr_search_config_default": ([P], None),
         }
         for name, (args, res) in sig.items():
@@ -326,7 +328,7 @@ class Context:
         return Result(h.value)
 
     def class_profile(self):
-        """Per score-kernel class of the last search: packed int16 by band width
+        """Per score-kernel class of the last search: packed 16-bit by band width
         (w25..w129), wave-per-DP int32 by width bound (wave64..wave2048), and the
         row-exact int32 kernel: jobs, DP cells, kernel ms."""
         names = SCORE_CLASSES
@@ -334,10 +336,17 @@ class Context:
         c = np.zeros(len(names), np.uint64)
         m = np.zeros(len(names), np.float64)
         _check(lib().dmr_last_class_profile(self._h, j.ctypes.data, c.ctypes.data, m.ctypes.data))
-        mask = C.c_uint32()
-        _check(lib().dmr_score_split_mask(self._h, C.byref(mask)))
+        mask, f16 = self.score_layout()
         return {n: dict(jobs=int(j[i]), cells=int(c[i]), ms=float(m[i]),
-                        kernel=_class_kernel(i, n, mask.value)) for i, n in enumerate(names)}
+                        kernel=_class_kernel(i, n, mask, f16)) for i, n in enumerate(names)}
+
+    def score_layout(self):
+        """(split mask, f16) of the packed score classes: bit c set = class c
+        split over four lanes per DP pair; f16 True = pairs in f16, False =
+        saturating int16."""
+        mask, f16 = C.c_uint32(), C.c_uint32()
+        _check(lib().dmr_score_layout(self._h, C.byref(mask), C.byref(f16)))
+        return mask.value, bool(f16.value)
 
     def last_dp_cells(self) -> int:
         v = C.c_uint64()

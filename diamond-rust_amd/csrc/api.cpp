@@ -590,6 +590,8 @@ int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells) {
   });
 }
 
+}  // extern "C"
+
 // Tabular::write (src/output.rs:150-223).
 namespace {
 
@@ -770,6 +772,8 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
 
 }  // namespace
 
+extern "C" {
+
 int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, const char* fields,
                       char* buf, uint64_t cap, uint64_t* len) {
   if (!r || !len) return fail(DMR_E_ARG, "NULL argument");
@@ -803,9 +807,10 @@ int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double
   });
 }
 
-int dmr_score_split_mask(dmr_ctx* ctx, uint32_t* mask) {
-  if (!ctx || !mask) return fail(DMR_E_ARG, "NULL argument");
-  *mask = ctx->eng->split_mask();
+int dmr_score_layout(dmr_ctx* ctx, uint32_t* split_mask, uint32_t* f16) {
+  if (!ctx) return fail(DMR_E_ARG, "NULL argument");
+  if (split_mask) *split_mask = ctx->eng->split_mask();
+  if (f16) *f16 = ctx->eng->packed_f16() ? 1u : 0u;
   return DMR_OK;
 }
 

@@ -71,18 +71,21 @@ void launch_sw_generic(const SwJob* jobs, const uint32_t* job_idx, uint64_t njob
 
 // Packed int16 score pass (k_sw16.hip).
 int pair_table_entries();
-void fill_pair_table(const int8_t (&m)[25][25], std::vector<int>* out);
+// Pair-score table of the packed kernels: entry (qa, sA, sB) holds
+// (score(qa, sA) + o, score(qa, sB) + o) as two int16 or, f16, two halves.
+void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out);
 void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits,
                      uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
                        const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
                        hipStream_t st);
 int sw16_width(int cls);
-// Packed int16 score pass of width class cls (k_sw16.hip); split = two lanes
-// per DP pair (half band each).
-void launch_sw16(int cls, bool split, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q,
-                 SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
-                 uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
+// Packed score pass of width class cls (k_sw16.hip): split = four lanes per
+// DP pair (quarter bands each); f16 = half-precision arithmetic (pair_tab
+// must then be the f16 table).
+void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
+                 SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
+                 SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);

@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
+#include <thread>
 
 namespace dmr {
 
@@ -58,8 +60,10 @@ Engine::Engine(int device) : device_(device) {
       m[a * kScoreStride + b] = (a < 25 && b < 25) ? blosum62()[a][b] : (int8_t)-64;
   DMR_HIP(hipMemcpy(matrix32_.ensure<int8_t>(sizeof m), m, sizeof m, hipMemcpyHostToDevice));
   std::vector<int> pt;
-  fill_pair_table(blosum62(), &pt);
+  fill_pair_table(blosum62(), false, &pt);
   DMR_HIP(hipMemcpy(pair_tab_.ensure<int>(pt.size()), pt.data(), pt.size() * 4, hipMemcpyHostToDevice));
+  fill_pair_table(blosum62(), true, &pt);
+  DMR_HIP(hipMemcpy(pair_tab16_.ensure<int>(pt.size()), pt.data(), pt.size() * 4, hipMemcpyHostToDevice));
   hipDeviceProp_t prop;
   DMR_HIP(hipGetDeviceProperties(&prop, device));
   cu_count_ = prop.multiProcessorCount;
@@ -69,6 +73,9 @@ Engine::Engine(int device) : device_(device) {
   // bit c for class c; DMR_SW16_SPLIT overrides the default (81 and 129).
   const char* lr = std::getenv("DMR_LANE_REPORT");
   lane_report_ = lr && lr[0] == '1';
+  // Packed score pass in f16 (default) or int16: DMR_SW16_F16=0 selects int16.
+  const char* hf = std::getenv("DMR_SW16_F16");
+  use_f16_ = !(hf && hf[0] == '0');
   const char* sp = std::getenv("DMR_SW16_SPLIT");
   split_mask_ = sp ? (unsigned)std::strtoul(sp, nullptr, 0) : 0x18u;
 }
@@ -106,6 +113,24 @@ T Engine::read1(const T* dptr) {
   DMR_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
   return v;
+}
+
+// Runs f(0..n-1) on up to 8 host threads (contiguous slices).
+static void parallel_for(uint64_t n, const std::function<void(uint64_t)>& f) {
+  const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const uint64_t per = 512;  // below this, one thread
+  const unsigned nt = (unsigned)std::min<uint64_t>(hw, (n + per - 1) / per);
+  if (nt <= 1) {
+    for (uint64_t k = 0; k < n; ++k) f(k);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t]() {
+      const uint64_t a = n * t / nt, b = n * (t + 1) / nt;
+      for (uint64_t k = a; k < b; ++k) f(k);
+    });
+  for (auto& x : th) x.join();
 }
 
 void Engine::stage_event(int k) { DMR_HIP(hipEventRecord(ev_[k], st_)); }
@@ -434,6 +459,17 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   stage_event(8);
   DMR_HIP(hipStreamSynchronize(st_));
   double th0 = now_ms();
+  // Exact E-values of the candidates (glibc f64, as the reference), computed
+  // in parallel; the selection below reads them in order.
+  std::vector<double> h_ev(nmaybe, 0.0);
+  parallel_for(nmaybe, [&](uint64_t k) {
+    const SwJob& j = h_mjobs[k];
+    const int32_t sc = h_mouts[k].score;
+    if (sc <= 0) return;
+    const uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
+    const uint64_t slen = h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
+    h_ev[k] = alp_evalue(P, sc, qlen, slen, db_letters_);
+  });
   // Per group: bands in rank order; the first max_hsps whose exact E-value
   // passes are kept (src/search.rs:232-255).
   struct Cand {
@@ -451,9 +487,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
         const SwJob& j = h_mjobs[k];
         int32_t sc = h_mouts[k].score;
         if (sc <= 0) continue;
-        uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
-        uint64_t slen = h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
-        double e = alp_evalue(P, sc, qlen, slen, db_letters_);
+        const double e = h_ev[k];
         if (e <= opt.evalue) {
           Cand c;
           c.job = k;
@@ -632,8 +666,9 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
       if (!class_lanes[c]) continue;
       hipStream_t s = class_stream(c);
       DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
-      launch_sw16(c, ((split_mask_ >> c) & 1u) != 0, lanes + class_lane0[c], class_lanes[c], jobs, qs,
-                  dbs, pair_tab_.as<int>(), matrix32_.as<int8_t>(), cnt + 1 + c, outs, redo, cnt, cus, s);
+      launch_sw16(c, ((split_mask_ >> c) & 1u) != 0, use_f16_, lanes + class_lane0[c], class_lanes[c], jobs,
+                  qs, dbs, (use_f16_ ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(),
+                  cnt + 1 + c, outs, redo, cnt, cus, s);
       DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
     }
   }

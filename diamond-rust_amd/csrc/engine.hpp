@@ -129,6 +129,7 @@ class Engine {
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
   unsigned split_mask() const { return split_mask_; }
+  bool packed_f16() const { return use_f16_; }
   static constexpr int kClasses = 12;  // k_classify in k_sw16.hip
   static constexpr int kRowExact = kClasses - 1;
   // Per score-kernel class of the last search (see StageStats::ms_class):
@@ -228,7 +229,7 @@ class Engine {
   DevBuf stride_, scr_off_, scratch_, outs_;
   DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
   DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_runs_, sw_lanes_, sw_counters_, sw_redo_, sw_list_;
-  DevBuf pair_tab_, sw_fallback_, sw_klist_[kWaveKinds];
+  DevBuf pair_tab_, pair_tab16_, sw_fallback_, sw_klist_[kWaveKinds];
   // Traceback groups: one per wave kernel K, then the row-exact kernel.
   static constexpr int kTbGroups = kWaveKinds + 1;
   DevBuf tb_counters_, tb_list_[kTbGroups], tb_toff_[kTbGroups], tb_ooff_[kTbGroups],
@@ -242,6 +243,7 @@ class Engine {
   bool force_generic_ = false;
   unsigned split_mask_ = 0x18u;
   bool lane_report_ = false;
+  bool use_f16_ = true;
   void report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
                     const uint32_t* class_lane0, const uint32_t* class_lanes);
   uint64_t last_njobs_ = 0;

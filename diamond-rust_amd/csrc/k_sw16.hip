@@ -22,7 +22,8 @@
 // - Rows past m + d1 hold a single cell at column m (the clamped rows of the
 //   reference). Their H is at most max(11, H(m + d1, m) - o), so they only
 //   matter when the best score is below 11: such DPs are re-run in int32.
-// Saturating adds detect scores >= 32767; those DPs are re-run in int32.
+// Saturating adds detect scores >= 32767 (int16; 2000 in the f16 variant);
+// those DPs are re-run in int32.
 #include <type_traits>
 
 #include "dmr_kernels.hpp"
@@ -31,18 +32,44 @@ namespace dmr {
 
 namespace {
 
-typedef short v2s __attribute__((ext_vector_type(2)));
-
 constexpr int kCodes = 26;                  // 25 letters + padding
 constexpr int kPairRow = kCodes * kCodes;   // (sA, sB) pairs for one query letter
 constexpr int kPairTab = kCodes * kPairRow;
 constexpr short kNeg16 = -16384;
 constexpr int kOpen = 12;                   // gap open + extend of the 16-bit tables
 
-__device__ __forceinline__ v2s as_v2s(int x) { return __builtin_bit_cast(v2s, x); }
-__device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
-__device__ __forceinline__ v2s vadds(v2s a, v2s b) { return __builtin_elementwise_add_sat(a, b); }
-__device__ __forceinline__ v2s vsubs(v2s a, v2s b) { return __builtin_elementwise_sub_sat(a, b); }
+// Arithmetic of the packed kernels, two DPs per register.
+// I16: int16 with saturating adds; a DP whose best reaches 32767 is re-run in
+//   int32.
+// F16: half precision. Every value the recurrence can produce is an integer
+//   in [-o - 1, best] (or below -16000 for the NEG sentinels, where rounding
+//   is harmless), and integers up to 2048 are exact in f16, so a DP whose
+//   best stays below 2000 is scored exactly; the rest are re-run in int32.
+//   f16 has a three-input max (v_pk_maximum3_f16), which int16 lacks.
+struct I16 {
+  typedef short V __attribute__((ext_vector_type(2)));
+  static constexpr int kRedo = 32767;
+  static __device__ __forceinline__ V bits(int x) { return __builtin_bit_cast(V, x); }
+  static __device__ __forceinline__ int raw(V v) { return __builtin_bit_cast(int, v); }
+  static __device__ __forceinline__ V splat(int x) { return V{(short)x, (short)x}; }
+  static __device__ __forceinline__ V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
+  static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
+  static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_max(a, b); }
+  static __device__ __forceinline__ int lo(V v) { return v.x; }
+  static __device__ __forceinline__ int hi(V v) { return v.y; }
+};
+struct F16 {
+  typedef _Float16 V __attribute__((ext_vector_type(2)));
+  static constexpr int kRedo = 2000;
+  static __device__ __forceinline__ V bits(int x) { return __builtin_bit_cast(V, x); }
+  static __device__ __forceinline__ int raw(V v) { return __builtin_bit_cast(int, v); }
+  static __device__ __forceinline__ V splat(int x) { return V{(_Float16)x, (_Float16)x}; }
+  static __device__ __forceinline__ V add(V a, V b) { return a + b; }
+  static __device__ __forceinline__ V sub(V a, V b) { return a - b; }
+  static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_maximum(a, b); }
+  static __device__ __forceinline__ int lo(V v) { return (int)v.x; }
+  static __device__ __forceinline__ int hi(V v) { return (int)v.y; }
+};
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -62,7 +89,7 @@ __device__ __forceinline__ int letter_at(const uint8_t* res, uint32_t off, int m
   return (p >= 0 && p < m) ? (int)res[off + (uint32_t)p] : 25;
 }
 
-template <int W>
+template <int W, class Tr>
 __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, uint32_t nlanes,
                                               const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                               const int* __restrict__ pair_tab,
@@ -74,11 +101,12 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
   for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const v2s NEG2 = {kNeg16, kNeg16};
-  const v2s ONE2 = {1, 1};
-  const v2s OPEN2 = {kOpen, kOpen};
-  const v2s ZERO2 = {0, 0};
-  const v2s ZOPQ = as_v2s(__builtin_amdgcn_update_dpp(0, 0, 0xE4, 0xf, 0xf, false));
+  using V = typename Tr::V;
+  const V NEG2 = Tr::splat(kNeg16);
+  const V ONE2 = Tr::splat(1);
+  const V OPEN2 = Tr::splat(kOpen);
+  const V ZERO2 = Tr::splat(0);
+  const V ZOPQ = Tr::bits(__builtin_amdgcn_update_dpp(Tr::raw(ZERO2), Tr::raw(ZERO2), 0xE4, 0xf, 0xf, false));
   for (;;) {
     // Branch-free task fetch (every lane joins the atomic, lane 0 adds 1):
     // the task lives in a scalar register and the loop exit is uniform.
@@ -121,11 +149,11 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     // into a score-table row, two 16-bit entries per register: entry t is
     // column j = i - d1 + t (t even: low half, t odd: high half).
     constexpr int NR = (W + 1) / 2;
-    v2s G[W + 1], E[W + 1];
+    V G[W + 1], E[W + 1];
     uint32_t R[NR];
 #pragma unroll
     for (int t = 0; t <= W; ++t) {
-      G[t] = t < W ? v2s{-kOpen, -kOpen} : NEG2;
+      G[t] = t < W ? Tr::splat(-kOpen) : NEG2;
       E[t] = NEG2;
     }
 #pragma unroll
@@ -156,7 +184,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
     int qcur = lo <= n ? qraw(lo) : 25;
     uint32_t pnext = 0;
-    v2s best = ZERO2;
+    V best = ZERO2;
     for (int r = 0; r < steps; ++r) {
       const int i = lo + r;
       if (r > 0) {
@@ -190,21 +218,21 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
       int sq[kAhead];
 #pragma unroll
       for (int t = 0; t < kAhead; ++t) sq[t] = score(t);
-      v2s F = NEG2, Gl = NEG2;
+      V F = NEG2, Gl = NEG2;
 #pragma unroll
       for (int t = 0; t < W; ++t) {
-        const v2s sc = as_v2s(sq[t % kAhead]);
+        const V sc = Tr::bits(sq[t % kAhead]);
         if (t + kAhead < W) sq[t % kAhead] = score(t + kAhead);
         __builtin_amdgcn_sched_barrier(0);
-        const v2s D = vadds(G[t], sc);
-        const v2s Ev = vmax(G[t + 1], vsubs(E[t + 1], ONE2));
+        const V D = Tr::add(G[t], sc);
+        const V Ev = Tr::max(G[t + 1], Tr::sub(E[t + 1], ONE2));
         // max(D, E, 0) is off the F chain that runs along the row (opaque
         // zero: the max tree is not reassociated back onto the chain).
-        const v2s X0 = vmax(vmax(D, Ev), ZOPQ);
-        const v2s Fv = vmax(Gl, vsubs(F, ONE2));
-        const v2s H = vmax(X0, Fv);
-        best = vmax(best, H);
-        Gl = vsubs(H, OPEN2);
+        const V X0 = Tr::max(Tr::max(D, Ev), ZOPQ);
+        const V Fv = Tr::max(Gl, Tr::sub(F, ONE2));
+        const V H = Tr::max(X0, Fv);
+        best = Tr::max(best, H);
+        Gl = Tr::sub(H, OPEN2);
         G[t] = Gl;
         E[t] = Ev;
         F = Fv;
@@ -220,9 +248,9 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     // Re-run in int32 (exact) when int16 saturated, or when the score is
     // below 11 and the band reaches past the subject's end: the clamped rows
     // there could then hold the maximum.
-    const int bA = best.x, bB = best.y;
+    const int bA = Tr::lo(best), bB = Tr::hi(best);
     if (hasA) {
-      if (bA >= 32767 || (bA < 11 && mA + d1A < n)) {
+      if (bA >= Tr::kRedo || (bA < 11 && mA + d1A < n)) {
         redo[atomicAdd(nredo, 1u)] = ln.x;
       } else {
         outs[ln.x].score = bA;
@@ -230,7 +258,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
       }
     }
     if (hasB) {
-      if (bB >= 32767 || (bB < 11 && mB + d1B < n)) {
+      if (bB >= Tr::kRedo || (bB < 11 && mB + d1B < n)) {
         redo[atomicAdd(nredo, 1u)] = ln.y;
       } else {
         outs[ln.y].score = bB;
@@ -266,7 +294,7 @@ struct SplitShape {
   static constexpr int kDeadFrom = W - (P - 1) * H;  // first dead offset of the last lane
 };
 
-template <int W, int P, int kThreads>
+template <int W, int P, int kThreads, class Tr>
 __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes,
                                                    const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                                    const int* __restrict__ pair_tab,
@@ -287,17 +315,18 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   const int p = lane & (P - 1);
   const bool last = p == P - 1;
   const int t0 = p * H;  // band offset of this lane's first entry
-  const v2s NEG2 = {kNeg16, kNeg16};
-  const v2s ONE2 = {1, 1};
-  const v2s OPEN2 = {kOpen, kOpen};
-  const v2s ZERO2 = {0, 0};
-  const v2s ZOPQ = as_v2s(__builtin_amdgcn_update_dpp(0, 0, 0xE4, 0xf, 0xf, false));
+  using V = typename Tr::V;
+  const V NEG2 = Tr::splat(kNeg16);
+  const V ONE2 = Tr::splat(1);
+  const V OPEN2 = Tr::splat(kOpen);
+  const V ZERO2 = Tr::splat(0);
+  const V ZOPQ = Tr::bits(__builtin_amdgcn_update_dpp(Tr::raw(ZERO2), Tr::raw(ZERO2), 0xE4, 0xf, 0xf, false));
   // Moves within a lane group (quad_perm): from the next lane, from the
   // previous lane, from two lanes back, and the group-wide maximum.
   constexpr int kFromNext = P == 4 ? 0xF9 : 0xF5;  // [1,2,3,3] / [1,1,3,3]
   constexpr int kFromPrev = P == 4 ? 0x90 : 0xA0;  // [0,0,1,2] / [0,0,2,2]
   auto dppi = [](int v, auto ctrl) { return __builtin_amdgcn_update_dpp(v, v, decltype(ctrl)::value, 0xf, 0xf, false); };
-  auto mv = [&](v2s v, auto ctrl) { return as_v2s(dppi(__builtin_bit_cast(int, v), ctrl)); };
+  auto mv = [&](V v, auto ctrl) { return Tr::bits(dppi(Tr::raw(v), ctrl)); };
   using NextC = std::integral_constant<int, kFromNext>;
   using PrevC = std::integral_constant<int, kFromPrev>;
   using Back2C = std::integral_constant<int, 0x40>;   // [0,0,0,1]
@@ -336,12 +365,12 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
 
     // G/E of this lane's offsets; entry H is the E source past the last one
     // (the next lane's first offset, or the wall for the last lane).
-    v2s G[H + 1], E[H + 1];
+    V G[H + 1], E[H + 1];
     uint32_t R[NR];
 #pragma unroll
     for (int u = 0; u <= H; ++u) {
       const bool dead = u == H || (last && u >= kDeadFrom);
-      G[u] = dead ? NEG2 : v2s{-kOpen, -kOpen};
+      G[u] = dead ? NEG2 : Tr::splat(-kOpen);
       E[u] = NEG2;
     }
 #pragma unroll
@@ -363,7 +392,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     };
     int qcur = lo <= n ? qraw(lo) : 25;
     uint32_t pnext = 0;
-    v2s best = ZERO2;
+    V best = ZERO2;
     for (int r = 0; r < steps; ++r) {
       const int i = lo + r;
       if (r > 0) {
@@ -381,7 +410,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       const int rq = qraw(i + 1), ra = sraw(ao, mA, pos - d1A), rb = sraw(bo, mB, pos - d1B);
       {
         // E source past the lane: the next lane's first offset, previous row.
-        const v2s g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
+        const V g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
         G[H] = last ? NEG2 : g0;
         E[H] = last ? NEG2 : e0;
       }
@@ -394,41 +423,41 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       int sq[kAhead];
 #pragma unroll
       for (int u = 0; u < kAhead; ++u) sq[u] = score(u);
-      v2s F = NEG2, Gl = NEG2;
+      V F = NEG2, Gl = NEG2;
 #pragma unroll
       for (int u = 0; u < H; ++u) {
-        const v2s sc = as_v2s(sq[u % kAhead]);
+        const V sc = Tr::bits(sq[u % kAhead]);
         if (u + kAhead < H) sq[u % kAhead] = score(u + kAhead);
         __builtin_amdgcn_sched_barrier(0);
-        const v2s D = vadds(G[u], sc);
-        const v2s Ev = vmax(G[u + 1], vsubs(E[u + 1], ONE2));
+        const V D = Tr::add(G[u], sc);
+        const V Ev = Tr::max(G[u + 1], Tr::sub(E[u + 1], ONE2));
         // max(D, E, 0) does not depend on F: only max(., F) and the G
         // subtraction sit on the chain that runs along the row. (The zero is
         // opaque so the max tree is not reassociated back onto the chain.)
-        const v2s X0 = vmax(vmax(D, Ev), ZOPQ);
-        const v2s Fv = vmax(Gl, vsubs(F, ONE2));
-        const v2s Hh = vmax(X0, Fv);
-        best = vmax(best, Hh);
-        Gl = vsubs(Hh, OPEN2);
+        const V X0 = Tr::max(Tr::max(D, Ev), ZOPQ);
+        const V Fv = Tr::max(Gl, Tr::sub(F, ONE2));
+        const V Hh = Tr::max(X0, Fv);
+        best = Tr::max(best, Hh);
+        Gl = Tr::sub(Hh, OPEN2);
         G[u] = Gl;
         E[u] = Ev;
         F = Fv;
       }
       // F entering each lane: exclusive prefix maximum of F_out(q) + q*H over
       // the group, minus (p-1)*H.
-      const v2s fout = vmax(Gl, vsubs(F, ONE2));
-      v2s y = vadds(fout, v2s{(short)(p * H), (short)(p * H)});
-      y = vmax(y, mv(y, PrevC{}));
-      if (P == 4) y = vmax(y, mv(y, Back2C{}));
-      v2s fin = vsubs(mv(y, PrevC{}), v2s{(short)((p - 1) * H), (short)((p - 1) * H)});
+      const V fout = Tr::max(Gl, Tr::sub(F, ONE2));
+      V y = Tr::add(fout, Tr::splat(p * H));
+      y = Tr::max(y, mv(y, PrevC{}));
+      if (P == 4) y = Tr::max(y, mv(y, Back2C{}));
+      V fin = Tr::sub(mv(y, PrevC{}), Tr::splat((p - 1) * H));
       if (p == 0) fin = NEG2;
-      best = vmax(best, fin);
-      if (__ballot(max((int)fin.x, (int)fin.y) > 0) != 0) {
-        v2s fc = vsubs(fin, OPEN2);
+      best = Tr::max(best, fin);
+      if (__ballot(max(Tr::lo(fin), Tr::hi(fin)) > 0) != 0) {
+        V fc = Tr::sub(fin, OPEN2);
 #pragma unroll
         for (int u = 0; u < H; ++u) {
-          G[u] = vmax(G[u], fc);
-          fc = vsubs(fc, ONE2);
+          G[u] = Tr::max(G[u], fc);
+          fc = Tr::sub(fc, ONE2);
         }
       }
       if (last) {
@@ -445,12 +474,12 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       const int lb = (pos - d1B >= 0 && pos - d1B < mB) ? cb : 25;
       pnext = (uint32_t)(4 * (la * kCodes + lb));
     }
-    best = vmax(best, mv(best, SwapC{}));
-    if (P == 4) best = vmax(best, mv(best, Swap2C{}));
-    const int bA = best.x, bB = best.y;
+    best = Tr::max(best, mv(best, SwapC{}));
+    if (P == 4) best = Tr::max(best, mv(best, Swap2C{}));
+    const int bA = Tr::lo(best), bB = Tr::hi(best);
     if (p == 0) {
       if (hasA) {
-        if (bA >= 32767 || (bA < 11 && mA + d1A < n)) {
+        if (bA >= Tr::kRedo || (bA < 11 && mA + d1A < n)) {
           redo[atomicAdd(nredo, 1u)] = ln.x;
         } else {
           outs[ln.x].score = bA;
@@ -458,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         }
       }
       if (hasB) {
-        if (bB >= 32767 || (bB < 11 && mB + d1B < n)) {
+        if (bB >= Tr::kRedo || (bB < 11 && mB + d1B < n)) {
           redo[atomicAdd(nredo, 1u)] = ln.y;
         } else {
           outs[ln.y].score = bB;
@@ -543,15 +572,26 @@ int pair_table_entries() { return kPairTab; }
 
 // Score table for the packed kernel: entry (qa, sA, sB) holds
 // (score(qa, sA) + o, score(qa, sB) + o) as two int16; letter 25 is padding.
-void fill_pair_table(const int8_t (&m)[25][25], std::vector<int>* out) {
+// IEEE half-precision bits of a small integer (|v| < 2048, exact).
+static uint16_t half_bits(int v) {
+  if (v == 0) return 0;
+  const uint16_t sign = v < 0 ? 0x8000u : 0u;
+  unsigned a = (unsigned)(v < 0 ? -v : v);
+  int e = 0;
+  while ((a >> e) > 1) ++e;                           // a = 1.f * 2^e
+  const unsigned frac = (a << (10 - e)) & 0x3FFu;     // e <= 10: exact
+  return (uint16_t)(sign | (unsigned)((e + 15) << 10) | frac);
+}
+
+void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out) {
   out->assign(kPairTab, 0);
   auto sc = [&](int a, int b) { return (a < 25 && b < 25) ? (int)m[a][b] : -64; };
+  auto enc = [&](int v) { return f16 ? (uint32_t)half_bits(v) : (uint32_t)(uint16_t)(int16_t)v; };
   for (int qa = 0; qa < kCodes; ++qa)
     for (int a = 0; a < kCodes; ++a)
       for (int b = 0; b < kCodes; ++b) {
-        int lo = sc(qa, a) + kOpen, hi = sc(qa, b) + kOpen;
-        (*out)[qa * kPairRow + a * kCodes + b] = (int)((uint32_t)(uint16_t)(int16_t)lo |
-                                                       ((uint32_t)(uint16_t)(int16_t)hi << 16));
+        const int lo = sc(qa, a) + kOpen, hi = sc(qa, b) + kOpen;
+        (*out)[qa * kPairRow + a * kCodes + b] = (int)(enc(lo) | (enc(hi) << 16));
       }
 }
 
@@ -573,36 +613,48 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, co
 static const int kWidths[5] = {25, 33, 61, 81, 129};
 int sw16_width(int cls) { return kWidths[cls]; }
 
-void launch_sw16(int cls, bool split, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q,
-                 SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs,
-                 uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
+template <int W, class Tr>
+void launch_width(bool split, unsigned tasks, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
+                  SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
+                  SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
+  if (split) {
+    static const unsigned resident = resident_blocks(k_sw16p<W, 4, 512, Tr>, 512, cu_count);
+    const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));
+    hipLaunchKernelGGL((k_sw16p<W, 4, 512, Tr>), dim3(blocks), dim3(512), 0, st, lanes, nlanes, jobs,
+                       q, db, pair_tab, counter, outs, redo, nredo);
+  } else {
+    static const unsigned resident = resident_blocks(k_sw16<W, Tr>, 256, cu_count);
+    const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));
+    hipLaunchKernelGGL((k_sw16<W, Tr>), dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,
+                       pair_tab, m32, counter, outs, redo, nredo);
+  }
+}
+
+void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
+                 SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
+                 SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!nlanes) return;
   // A wave carries 64 DP pairs (one per lane) or, split, 16 (four lanes each).
   const unsigned tasks = nblk(nlanes, split ? 16 : 64);
-  switch (cls) {
-#define DMR_SW16(c, w)                                                                              \
-  case c:                                                                                           \
-    if (split) {                                                                                    \
-      static const unsigned resident = resident_blocks(k_sw16p<w, 4, 512>, 512, cu_count);         \
-      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));                         \
-      hipLaunchKernelGGL((k_sw16p<w, 4, 512>), dim3(blocks), dim3(512), 0, st, lanes, nlanes, jobs, \
-                         q, db, pair_tab, counter, outs, redo, nredo);                              \
-    } else {                                                                                        \
-      static const unsigned resident = resident_blocks(k_sw16<w>, 256, cu_count);                  \
-      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));                         \
-      hipLaunchKernelGGL(k_sw16<w>, dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,     \
-                         pair_tab, m32, counter, outs, redo, nredo);                                \
-    }                                                                                               \
+#define DMR_SW16(c, w)                                                                             \
+  case c:                                                                                          \
+    if (f16)                                                                                       \
+      launch_width<w, F16>(split, tasks, lanes, nlanes, jobs, q, db, pair_tab, m32, counter, outs, \
+                           redo, nredo, cu_count, st);                                             \
+    else                                                                                           \
+      launch_width<w, I16>(split, tasks, lanes, nlanes, jobs, q, db, pair_tab, m32, counter, outs, \
+                           redo, nredo, cu_count, st);                                             \
     break;
+  switch (cls) {
     DMR_SW16(0, 25)
     DMR_SW16(1, 33)
     DMR_SW16(2, 61)
     DMR_SW16(3, 81)
     DMR_SW16(4, 129)
-#undef DMR_SW16
     default:
       break;
   }
+#undef DMR_SW16
 }
 
 }  // namespace dmr

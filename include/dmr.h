@@ -219,10 +219,12 @@ int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
  * milliseconds. Diagnostics only; no counterpart in the reference. */
 #define DMR_SCORE_CLASSES 12
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
-/* Layout of the packed int16 classes 0-4 on ctx: bit c set = class c runs
- * split over four lanes per DP pair (k_sw16p), clear = one lane (k_sw16).
- * Diagnostics only; no counterpart in the reference. */
-int dmr_score_split_mask(dmr_ctx* ctx, uint32_t* mask);
+/* Layout of the packed classes 0-4 on ctx: bit c of *split_mask set = class
+ * c runs split over four lanes per DP pair (k_sw16p), clear = one lane
+ * (k_sw16); *f16 = 1 when the packed kernels compute in f16 pairs, 0 in
+ * saturating int16 pairs. Either may be NULL. Diagnostics only; no
+ * counterpart in the reference. */
+int dmr_score_layout(dmr_ctx* ctx, uint32_t* split_mask, uint32_t* f16);
 void dmr_result_destroy(dmr_result* r);
 
 #ifdef __cplusplus

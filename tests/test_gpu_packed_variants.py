@@ -1,11 +1,15 @@
-"""The packed int16 score pass in both layouts against the oracle.
+"""The packed 16-bit score pass in both layouts and both arithmetics against
+the oracle.
 
 Each width class runs either with one lane per DP pair (the whole band in a
 lane's registers) or split over four lanes (quarter bands, with the
 horizontal-gap correction across lanes). DMR_SW16_SPLIT selects the layout
-per class when a context is created (bit c = class c); both extremes are
-checked on drop-in batches (one DP per lane) and on a batched search (two DPs
-of the same query per lane).
+per class when a context is created (bit c = class c). The pairs compute in
+f16 (exact for integers up to 2048; DPs scoring 2000 or more are re-scored in
+int32) or, with DMR_SW16_F16=0, in saturating int16 (re-scored from 32767).
+Every combination is checked on drop-in batches (one DP per lane), on scores
+around the f16 limit, and on a batched search (two DPs of the same query per
+lane).
 """
 import numpy as np
 import pytest
@@ -16,13 +20,17 @@ from test_gpu_search import EXTENDED, run_both, workload_dict
 
 pytestmark = pytest.mark.gpu
 
-MASKS = [0x00, 0x1F]
+LAYOUTS = [(0x00, "1"), (0x1F, "1"), (0x00, "0"), (0x1F, "0")]
 
 
-@pytest.fixture(params=MASKS, ids=["full", "split"])
+@pytest.fixture(params=LAYOUTS, ids=["full-f16", "split-f16", "full-i16", "split-i16"])
 def layout_ctx(request, dmr, monkeypatch):
-    monkeypatch.setenv("DMR_SW16_SPLIT", str(request.param))
+    mask, f16 = request.param
+    monkeypatch.setenv("DMR_SW16_SPLIT", str(mask))
+    monkeypatch.setenv("DMR_SW16_F16", f16)
     c = dmr.Context(0)
+    got = c.score_layout()
+    assert got == (mask, f16 == "1"), got
     yield c
     c.close()
 
@@ -45,6 +53,28 @@ def test_band_widths_both_layouts(layout_ctx, orc):
     for (q, s, d0, d1), g in zip(cases, got):
         ref = orc.extend_span(q, s, d0, d1)
         assert g == (ref["score"] if ref else None), (len(q), len(s), d0, d1)
+
+
+def test_scores_around_f16_limit(layout_ctx, orc):
+    """Self and near-self alignments scoring 1700..2500 in every packed class:
+    below 2000 the f16 pass stands, from 2000 on it is re-scored exactly."""
+    rng = np.random.default_rng(5)
+    hi = np.array([17, 4, 8, 18, 14, 13], np.uint8)  # W C H Y P F: self-scores 11, 9, 8, 7, 7, 6
+    cases = []
+    for n in range(200, 320, 4):
+        q = hi[rng.integers(0, len(hi), size=n)]
+        for band in (12, 16, 30, 40, 64):
+            s = q.copy() if rng.random() < 0.5 else mutate(rng, q, 0.03, 0.01)
+            c = int(rng.integers(-3, 4))
+            cases.append((q, s, c - band, c + band))
+    got = _score_batch(layout_ctx, cases)
+    ref = [orc.extend_span(*c) for c in cases]
+    ref = [r["score"] if r else None for r in ref]
+    scores = [r for r in ref if r is not None]
+    assert min(scores) < 1990 and max(scores) > 2060
+    assert sum(1990 <= r <= 2060 for r in scores) >= 5
+    for c, g, r in zip(cases, got, ref):
+        assert g == r, (len(c[0]), len(c[1]), c[2], c[3], r)
 
 
 def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
