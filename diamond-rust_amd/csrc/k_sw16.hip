@@ -10,6 +10,14 @@
 // with o = open + extend. G = H - o is stored instead of H (the score table
 // holds s + o), which saves one subtraction per cell pair.
 //
+// Row frames: the values of row step r are held plus r (G plus r + 1), so a
+// vertical move gains exactly the gap extension and E needs no decrement:
+//   E^(r,t) = max(G^(r-1,t+1), E^(r-1,t+1)),  D^ = G^(r-1,t) + s + o,
+//   F^(r,t) = max(G^(r,t-1), F^(r,t-1)) - 1,  H^ = max(r, D^, E^, F^),
+//   G^ = H^ - o + 1,
+// with the zero of the local alignment at r. Every kFrame rows the frame is
+// moved back by kFrame (all carried values and r), which bounds the offset.
+//
 // Exactness against the row-range rules of the reference:
 // - Columns j <= 0 and rows above the band read a padding letter scoring -64:
 //   those cells stay at H = 0, which is what the reference computes there
@@ -22,8 +30,8 @@
 // - Rows past m + d1 hold a single cell at column m (the clamped rows of the
 //   reference). Their H is at most max(11, H(m + d1, m) - o), so they only
 //   matter when the best score is below 11: such DPs are re-run in int32.
-// Saturating adds detect scores >= 32767 (int16; 2000 in the f16 variant);
-// those DPs are re-run in int32.
+// DPs scoring kRedo or more (int16 saturation, f16 exactness) are re-run in
+// int32.
 #include <type_traits>
 
 #include "dmr_kernels.hpp"
@@ -37,36 +45,46 @@ constexpr int kPairRow = kCodes * kCodes;   // (sA, sB) pairs for one query lett
 constexpr int kPairTab = kCodes * kPairRow;
 constexpr short kNeg16 = -16384;
 constexpr int kOpen = 12;                   // gap open + extend of the 16-bit tables
+constexpr int kFrame = 64;                  // rows between frame moves
 
-// Arithmetic of the packed kernels, two DPs per register.
-// I16: int16 with saturating adds; a DP whose best reaches 32767 is re-run in
-//   int32.
+// Arithmetic of the packed kernels, two DPs per register. A value in the
+// frame of its row is the true value plus at most kFrame, and no value exceeds
+// the running best H of its DP in that frame.
+// I16: int16 with saturating adds. Saturation (at 32767) leaves the best at
+//   32767 - kFrame - 1 or more after the frame is taken off, so DPs from 32000
+//   on are re-run in int32.
 // F16: half precision. Every value the recurrence can produce is an integer
-//   in [-o - 1, best] (or below -16000 for the NEG sentinels, where rounding
-//   is harmless), and integers up to 2048 are exact in f16, so a DP whose
-//   best stays below 2000 is scored exactly; the rest are re-run in int32.
-//   f16 has a three-input max (v_pk_maximum3_f16), which int16 lacks.
+//   in [-o - 1, best + kFrame] (or below -16000 for the NEG sentinels, where
+//   rounding is harmless), and integers up to 2048 are exact in f16. Every
+//   value is at most the frame's running best, so the first value that
+//   rounds leaves that best at 2048 or more and the final best at
+//   2048 - kFrame or more: a DP whose result is below 1900 was scored
+//   exactly; the rest are re-run in int32. max and max3 become
+//   v_pk_maximum3_f16 (IEEE maximum: unlike maxNum it needs no canonicalized
+//   operands; no NaN ever arises).
 struct I16 {
   typedef short V __attribute__((ext_vector_type(2)));
-  static constexpr int kRedo = 32767;
+  static constexpr int kRedo = 32000;
   static __device__ __forceinline__ V bits(int x) { return __builtin_bit_cast(V, x); }
   static __device__ __forceinline__ int raw(V v) { return __builtin_bit_cast(int, v); }
   static __device__ __forceinline__ V splat(int x) { return V{(short)x, (short)x}; }
   static __device__ __forceinline__ V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
   static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
   static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_max(a, b); }
+  static __device__ __forceinline__ V max3(V a, V b, V c) { return max(max(a, b), c); }
   static __device__ __forceinline__ int lo(V v) { return v.x; }
   static __device__ __forceinline__ int hi(V v) { return v.y; }
 };
 struct F16 {
   typedef _Float16 V __attribute__((ext_vector_type(2)));
-  static constexpr int kRedo = 2000;
+  static constexpr int kRedo = 1900;
   static __device__ __forceinline__ V bits(int x) { return __builtin_bit_cast(V, x); }
   static __device__ __forceinline__ int raw(V v) { return __builtin_bit_cast(int, v); }
   static __device__ __forceinline__ V splat(int x) { return V{(_Float16)x, (_Float16)x}; }
   static __device__ __forceinline__ V add(V a, V b) { return a + b; }
   static __device__ __forceinline__ V sub(V a, V b) { return a - b; }
   static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_maximum(a, b); }
+  static __device__ __forceinline__ V max3(V a, V b, V c) { return max(max(a, b), c); }
   static __device__ __forceinline__ int lo(V v) { return (int)v.x; }
   static __device__ __forceinline__ int hi(V v) { return (int)v.y; }
 };
@@ -104,9 +122,9 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
   using V = typename Tr::V;
   const V NEG2 = Tr::splat(kNeg16);
   const V ONE2 = Tr::splat(1);
-  const V OPEN2 = Tr::splat(kOpen);
+  const V GOFF = Tr::splat(1 - kOpen);
+  const V FRAME2 = Tr::splat(kFrame);
   const V ZERO2 = Tr::splat(0);
-  const V ZOPQ = Tr::bits(__builtin_amdgcn_update_dpp(Tr::raw(ZERO2), Tr::raw(ZERO2), 0xE4, 0xf, 0xf, false));
   for (;;) {
     // Branch-free task fetch (every lane joins the atomic, lane 0 adds 1):
     // the task lives in a scalar register and the loop exit is uniform.
@@ -184,7 +202,8 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
     int qcur = lo <= n ? qraw(lo) : 25;
     uint32_t pnext = 0;
-    V best = ZERO2;
+    V best = ZERO2;  // in the frame of the current row
+    V Zr = ZERO2;    // frame offset of the current row
     for (int r = 0; r < steps; ++r) {
       const int i = lo + r;
       if (r > 0) {
@@ -224,15 +243,14 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         const V sc = Tr::bits(sq[t % kAhead]);
         if (t + kAhead < W) sq[t % kAhead] = score(t + kAhead);
         __builtin_amdgcn_sched_barrier(0);
+        // max(D, E, 0) is off the F chain that runs along the row.
         const V D = Tr::add(G[t], sc);
-        const V Ev = Tr::max(G[t + 1], Tr::sub(E[t + 1], ONE2));
-        // max(D, E, 0) is off the F chain that runs along the row (opaque
-        // zero: the max tree is not reassociated back onto the chain).
-        const V X0 = Tr::max(Tr::max(D, Ev), ZOPQ);
-        const V Fv = Tr::max(Gl, Tr::sub(F, ONE2));
+        const V Ev = Tr::max(G[t + 1], E[t + 1]);
+        const V X0 = Tr::max3(D, Ev, Zr);
+        const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
         const V H = Tr::max(X0, Fv);
         best = Tr::max(best, H);
-        Gl = Tr::sub(H, OPEN2);
+        Gl = Tr::add(H, GOFF);
         G[t] = Gl;
         E[t] = Ev;
         F = Fv;
@@ -244,7 +262,20 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         const int lb = (i - d0B >= 0 && i - d0B < mB) ? cb : 25;
         pnext = (uint32_t)(4 * (la * kCodes + lb));
       }
+      // Frame of the next row; every kFrame rows the frame moves back.
+      best = Tr::add(best, ONE2);
+      Zr = Tr::add(Zr, ONE2);
+      if ((r & (kFrame - 1)) == kFrame - 1) {
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+          G[t] = Tr::sub(G[t], FRAME2);
+          E[t] = Tr::sub(E[t], FRAME2);
+        }
+        best = Tr::sub(best, FRAME2);
+        Zr = ZERO2;
+      }
     }
+    best = Tr::sub(best, Zr);
     // Re-run in int32 (exact) when int16 saturated, or when the score is
     // below 11 and the band reaches past the subject's end: the clamped rows
     // there could then hold the maximum.
@@ -318,9 +349,9 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   using V = typename Tr::V;
   const V NEG2 = Tr::splat(kNeg16);
   const V ONE2 = Tr::splat(1);
-  const V OPEN2 = Tr::splat(kOpen);
+  const V GOFF = Tr::splat(1 - kOpen);
+  const V FRAME2 = Tr::splat(kFrame);
   const V ZERO2 = Tr::splat(0);
-  const V ZOPQ = Tr::bits(__builtin_amdgcn_update_dpp(Tr::raw(ZERO2), Tr::raw(ZERO2), 0xE4, 0xf, 0xf, false));
   // Moves within a lane group (quad_perm): from the next lane, from the
   // previous lane, from two lanes back, and the group-wide maximum.
   constexpr int kFromNext = P == 4 ? 0xF9 : 0xF5;  // [1,2,3,3] / [1,1,3,3]
@@ -392,7 +423,8 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     };
     int qcur = lo <= n ? qraw(lo) : 25;
     uint32_t pnext = 0;
-    V best = ZERO2;
+    V best = ZERO2;  // in the frame of the current row
+    V Zr = ZERO2;    // frame offset of the current row
     for (int r = 0; r < steps; ++r) {
       const int i = lo + r;
       if (r > 0) {
@@ -429,35 +461,45 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         const V sc = Tr::bits(sq[u % kAhead]);
         if (u + kAhead < H) sq[u % kAhead] = score(u + kAhead);
         __builtin_amdgcn_sched_barrier(0);
+        // max(D, E, 0) does not depend on F: only max(., F), the F
+        // decrement and the G offset sit on the chain along the row.
         const V D = Tr::add(G[u], sc);
-        const V Ev = Tr::max(G[u + 1], Tr::sub(E[u + 1], ONE2));
-        // max(D, E, 0) does not depend on F: only max(., F) and the G
-        // subtraction sit on the chain that runs along the row. (The zero is
-        // opaque so the max tree is not reassociated back onto the chain.)
-        const V X0 = Tr::max(Tr::max(D, Ev), ZOPQ);
-        const V Fv = Tr::max(Gl, Tr::sub(F, ONE2));
+        const V Ev = Tr::max(G[u + 1], E[u + 1]);
+        const V X0 = Tr::max3(D, Ev, Zr);
+        const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
         const V Hh = Tr::max(X0, Fv);
         best = Tr::max(best, Hh);
-        Gl = Tr::sub(Hh, OPEN2);
+        Gl = Tr::add(Hh, GOFF);
         G[u] = Gl;
         E[u] = Ev;
         F = Fv;
       }
-      // F entering each lane: exclusive prefix maximum of F_out(q) + q*H over
-      // the group, minus (p-1)*H.
-      const V fout = Tr::max(Gl, Tr::sub(F, ONE2));
-      V y = Tr::add(fout, Tr::splat(p * H));
+      // F entering each lane: exclusive prefix maximum of
+      // F_out(q) - (P-1-q)*H over the group, plus (P-p)*H. (Every operand
+      // stays at or below some F of the row, which the f16 bound needs.)
+      const V fout = Tr::sub(Tr::max(Gl, F), ONE2);
+      V y = Tr::sub(fout, Tr::splat((P - 1 - p) * H));
       y = Tr::max(y, mv(y, PrevC{}));
       if (P == 4) y = Tr::max(y, mv(y, Back2C{}));
-      V fin = Tr::sub(mv(y, PrevC{}), Tr::splat((p - 1) * H));
+      V fin = Tr::add(mv(y, PrevC{}), Tr::splat((P - p) * H));
       if (p == 0) fin = NEG2;
       best = Tr::max(best, fin);
-      if (__ballot(max(Tr::lo(fin), Tr::hi(fin)) > 0) != 0) {
-        V fc = Tr::sub(fin, OPEN2);
+      // The correction can only raise offsets u < F_in (F_in taken out of the
+      // frame): from there on F_in - o - u is at most -o, the least G. It runs
+      // in chunks of kChunk offsets while some lane of the wave still needs it.
+      {
+        const V fu = Tr::sub(fin, Zr);
+        const int fl = max(Tr::lo(fu), Tr::hi(fu));
+        V fc = Tr::add(fin, GOFF);
+        constexpr int kChunk = 8;
 #pragma unroll
-        for (int u = 0; u < H; ++u) {
-          G[u] = Tr::max(G[u], fc);
-          fc = Tr::sub(fc, ONE2);
+        for (int c = 0; c < H; c += kChunk) {
+          if (__ballot(fl > c) == 0) break;
+#pragma unroll
+          for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
+            G[u] = Tr::max(G[u], fc);
+            fc = Tr::sub(fc, ONE2);
+          }
         }
       }
       if (last) {
@@ -473,7 +515,20 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       const int la = (pos - d1A >= 0 && pos - d1A < mA) ? ca : 25;
       const int lb = (pos - d1B >= 0 && pos - d1B < mB) ? cb : 25;
       pnext = (uint32_t)(4 * (la * kCodes + lb));
+      // Frame of the next row; every kFrame rows the frame moves back.
+      best = Tr::add(best, ONE2);
+      Zr = Tr::add(Zr, ONE2);
+      if ((r & (kFrame - 1)) == kFrame - 1) {
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+          G[u] = Tr::sub(G[u], FRAME2);
+          E[u] = Tr::sub(E[u], FRAME2);
+        }
+        best = Tr::sub(best, FRAME2);
+        Zr = ZERO2;
+      }
     }
+    best = Tr::sub(best, Zr);
     best = Tr::max(best, mv(best, SwapC{}));
     if (P == 4) best = Tr::max(best, mv(best, Swap2C{}));
     const int bA = Tr::lo(best), bB = Tr::hi(best);

@@ -5,8 +5,8 @@ Each width class runs either with one lane per DP pair (the whole band in a
 lane's registers) or split over four lanes (quarter bands, with the
 horizontal-gap correction across lanes). DMR_SW16_SPLIT selects the layout
 per class when a context is created (bit c = class c). The pairs compute in
-f16 (exact for integers up to 2048; DPs scoring 2000 or more are re-scored in
-int32) or, with DMR_SW16_F16=0, in saturating int16 (re-scored from 32767).
+f16 (exact for integers up to 2048; DPs scoring 1900 or more are re-scored in
+int32) or, with DMR_SW16_F16=0, in saturating int16 (re-scored from 32000).
 Every combination is checked on drop-in batches (one DP per lane), on scores
 around the f16 limit, and on a batched search (two DPs of the same query per
 lane).
@@ -56,8 +56,9 @@ def test_band_widths_both_layouts(layout_ctx, orc):
 
 
 def test_scores_around_f16_limit(layout_ctx, orc):
-    """Self and near-self alignments scoring 1700..2500 in every packed class:
-    below 2000 the f16 pass stands, from 2000 on it is re-scored exactly."""
+    """Self and near-self alignments scoring 1500..2550 in every packed class:
+    below 1900 the f16 pass stands, from 1900 on it is re-scored exactly; the
+    largest stay far enough past 2048 in the row frames that f16 rounds."""
     rng = np.random.default_rng(5)
     hi = np.array([17, 4, 8, 18, 14, 13], np.uint8)  # W C H Y P F: self-scores 11, 9, 8, 7, 7, 6
     cases = []
@@ -71,8 +72,8 @@ def test_scores_around_f16_limit(layout_ctx, orc):
     ref = [orc.extend_span(*c) for c in cases]
     ref = [r["score"] if r else None for r in ref]
     scores = [r for r in ref if r is not None]
-    assert min(scores) < 1990 and max(scores) > 2060
-    assert sum(1990 <= r <= 2060 for r in scores) >= 5
+    assert min(scores) < 1850 and max(scores) > 2200
+    assert sum(1850 <= r <= 1950 for r in scores) >= 10
     for c, g, r in zip(cases, got, ref):
         assert g == r, (len(c[0]), len(c[1]), c[2], c[3], r)
 
