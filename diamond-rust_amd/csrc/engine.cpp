@@ -73,6 +73,10 @@ Engine::Engine(int device) : device_(device) {
   // bit c for class c; DMR_SW16_SPLIT overrides the default (81 and 129).
   const char* lr = std::getenv("DMR_LANE_REPORT");
   lane_report_ = lr && lr[0] == '1';
+  // DMR_SCORE_SERIAL=1 runs the score classes one after another on the engine
+  // stream (per-class kernel times without sharing the chip).
+  const char* ss = std::getenv("DMR_SCORE_SERIAL");
+  score_serial_ = ss && ss[0] == '1';
   // Packed score pass in f16 (default) or int16: DMR_SW16_F16=0 selects int16.
   const char* hf = std::getenv("DMR_SW16_F16");
   use_f16_ = !(hf && hf[0] == '0');
@@ -648,8 +652,20 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     DMR_HIP(hipMemcpyAsync(d_rs, run_start.data(), nr * 4, hipMemcpyHostToDevice, st_));
     DMR_HIP(hipMemcpyAsync(d_rs + nr, run_len.data(), nr * 4, hipMemcpyHostToDevice, st_));
     DMR_HIP(hipMemcpyAsync(d_rs + 2 * nr, lane_off.data(), nr * 4, hipMemcpyHostToDevice, st_));
+    uint2* paired = sw_lanes2_.ensure<uint2>(nl + 1);
+    launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, nl, paired, st_);
+    // Lanes of each class in order of row span, longest first: a wave (or a
+    // block of lane groups) then steps lanes of about the same length, and
+    // the longest start first. Classes stay contiguous (class is the major
+    // key), so class_lane0/class_lanes still hold.
+    uint64_t* lk = sw_keys_.as<uint64_t>();
+    uint32_t* li = sw_idx2_.as<uint32_t>();
+    launch_lane_keys(paired, nl, jobs, qs, dbs, lk, li, st_);
+    radix_sort_pairs(lk, li, sw_keys2_.as<uint64_t>(), sw_lane_idx_.ensure<uint32_t>(nl + 1), nl,
+                     3 + kLaneSpanBits, b_hist_.ensure<uint32_t>(radix_hist_elems(nl)),
+                     b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nl)), st_);
     lanes = sw_lanes_.ensure<uint2>(nl + 1);
-    launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, nl, lanes, st_);
+    launch_gather_lanes(paired, li, nl, lanes, st_);
     if (lane_report_ && qs.offs == q_offs_.as<uint32_t>())
       report_lanes(jobs, njobs, lanes, nl, class_lane0, class_lanes);
   }
@@ -660,7 +676,10 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   // stream of its own.
   fork();
   int next_aux = 1;
-  auto class_stream = [&](int c) { return c == 4 ? aux_[0] : aux_[1 + (next_aux++ - 1) % (kAux - 1)]; };
+  auto class_stream = [&](int c) {
+    if (score_serial_) return st_;
+    return c == 4 ? aux_[0] : aux_[1 + (next_aux++ - 1) % (kAux - 1)];
+  };
   if (nl) {
     for (int c = 4; c >= 0; --c) {
       if (!class_lanes[c]) continue;

@@ -619,6 +619,35 @@ __global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint
   lanes[L] = make_uint2(sorted_idx[s + k], k + 1 < len ? sorted_idx[s + k + 1] : ~0u);
 }
 
+// Sort key per lane: class | (row span, longest first). The rows a lane runs
+// are the union of its two DPs' row ranges, as in the packed kernels.
+__global__ void k_lane_keys(const uint2* __restrict__ lanes, uint32_t nlanes, const SwJob* __restrict__ jobs,
+                            SeqSet q, SeqSet db, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= nlanes) return;
+  const uint2 ln = lanes[L];
+  int lo = INT32_MAX, hi = 0, cls = 0;
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t k = h ? ln.y : ln.x;
+    if (k == ~0u) continue;
+    const SwJob j = jobs[k];
+    const int n = (int)(q.offs[j.target + 1] - q.offs[j.target]);
+    const int m = (int)(db.offs[j.subject + 1] - db.offs[j.subject]);
+    lo = min(lo, max(1, j.d0 + 1));
+    hi = max(hi, min(n, m + j.d1));
+    cls = width_class(j.d1 - j.d0 + 1);
+  }
+  const int span = min(max(hi - lo + 1, 0), (int)kLaneSpanMax);
+  keys[L] = ((uint64_t)cls << kLaneSpanBits) | (uint64_t)(kLaneSpanMax - span);
+  idx[L] = L;
+}
+
+__global__ void k_gather_lanes(const uint2* __restrict__ in, const uint32_t* __restrict__ idx, uint32_t n,
+                               uint2* __restrict__ out) {
+  const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L < n) out[L] = in[idx[L]];
+}
+
 inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -663,6 +692,18 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, co
   if (!nruns || !nlanes) return;
   hipLaunchKernelGGL(k_make_lanes, dim3(nblk(nlanes, 256)), dim3(256), 0, st, sorted_idx, run_start,
                      run_len, lane_off, nruns, nlanes, lanes);
+}
+
+void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
+                      uint64_t* keys, uint32_t* idx, hipStream_t st) {
+  if (!nlanes) return;
+  hipLaunchKernelGGL(k_lane_keys, dim3(nblk(nlanes, 256)), dim3(256), 0, st, lanes, nlanes, jobs, q, db,
+                     keys, idx);
+}
+
+void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_lanes, dim3(nblk(n, 256)), dim3(256), 0, st, in, idx, n, out);
 }
 
 static const int kWidths[5] = {25, 33, 61, 81, 129};
