@@ -38,8 +38,8 @@ EXPORTS = [
 ]
 
 # Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
-SCORE_CLASSES = ["w25", "w33", "w61", "w81", "w129", "wave64", "wave128", "wave256", "wave512",
-                 "wave1024", "wave2048", "rowexact"]
+SCORE_CLASSES = ["w25", "w33", "w61", "w81", "w129", "m193", "m256", "wave64", "wave128", "wave256",
+                 "wave512", "wave1024", "wave2048", "rowexact"]
 
 def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
     """Kernel that scores class i: its template instance as rocprofv3 names it,
@@ -48,9 +48,11 @@ def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
         return f"k_sw_wave<{int(name[4:]) // 64}, false>"
     if name == "rowexact":
         return "k_sw_generic<false>"
-    w = int(name[1:])
     tr = "F16" if f16 else "I16"
-    return f"k_sw16p<{w}, 4, 512, {tr}>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
+    if name.startswith("m"):  # merged bands: sixteen lanes per pair, per-DP walls
+        return f"k_sw16p<{257 if name == 'm256' else 193}, 16, 512, {tr}, true>"
+    w = int(name[1:])
+    return f"k_sw16p<{w}, 4, 512, {tr}, false>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
 
 
 HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
@@ -329,8 +331,9 @@ class Context:
 
     def class_profile(self):
         """Per score-kernel class of the last search: packed 16-bit by band width
-        (w25..w129), wave-per-DP int32 by width bound (wave64..wave2048), and the
-        row-exact int32 kernel: jobs, DP cells, kernel ms."""
+        (w25..w129, merged bands m193/m256), wave-per-DP int32 by width bound
+        (wave64..wave2048), and the row-exact int32 kernel: jobs, DP cells,
+        kernel ms."""
         names = SCORE_CLASSES
         j = np.zeros(len(names), np.uint64)
         c = np.zeros(len(names), np.uint64)

@@ -88,8 +88,9 @@ void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, Se
                       uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st);
 // Packed score pass of width class cls (k_sw16.hip): split = four lanes per
-// DP pair (quarter bands each); f16 = half-precision arithmetic (pair_tab
-// must then be the f16 table).
+// DP pair (quarter bands each) for classes 0-4; classes 5-6 (merged bands)
+// always run on sixteen lanes per pair with per-DP walls; f16 = half-precision
+// arithmetic (pair_tab must then be the f16 table).
 void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
                  SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
                  SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
@@ -119,6 +120,7 @@ constexpr int kRowKeyBits = 18;
 // <= 64*K, K a power of two up to kMaxWaveK (0 = wider than 64*kMaxWaveK).
 constexpr int kMaxWaveK = 32;
 constexpr int kWaveKinds = 6;  // K = 1, 2, 4, 8, 16, 32
+constexpr int kPackedClasses = 7;  // packed 16-bit score classes (k_sw16.hip)
 int swwave_k_for(int64_t width);
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,

@@ -624,14 +624,14 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   uint64_t cbeg[kClasses], cend[kClasses];
   for (int c = 0; c < kClasses; ++c) cbeg[c] = cend[c] = 0;
   std::vector<uint32_t> run_start, run_len, lane_off;
-  uint32_t class_lanes[5] = {0, 0, 0, 0, 0}, class_lane0[5] = {0, 0, 0, 0, 0};
+  uint32_t class_lanes[kPackedClasses] = {}, class_lane0[kPackedClasses] = {};
   uint32_t nl = 0;
   for (uint64_t r = 0; r < nruns; ++r) {
     int cls = (int)(h_keys[r] >> cls_shift);
     if (cls >= kClasses) cls = kClasses - 1;
     if (cend[cls] == 0) cbeg[cls] = h_rs[r];
     cend[cls] = h_rs[r + 1];
-    if (cls >= 5 || !fast) continue;
+    if (cls >= kPackedClasses || !fast) continue;
     if (class_lanes[cls] == 0) class_lane0[cls] = nl;
     uint32_t len = h_rs[r + 1] - h_rs[r];
     run_start.push_back(h_rs[r]);
@@ -681,7 +681,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     return c == 4 ? aux_[0] : aux_[1 + (next_aux++ - 1) % (kAux - 1)];
   };
   if (nl) {
-    for (int c = 4; c >= 0; --c) {
+    for (int c = kPackedClasses - 1; c >= 0; --c) {
       if (!class_lanes[c]) continue;
       hipStream_t s = class_stream(c);
       DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
@@ -693,17 +693,17 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   }
   // Wave-per-DP int32 kernel: classes 5-10 (and every width class when the
   // gap costs have no int16 table).
-  for (int c = 5; c < 5 + kWaveKinds; ++c) {
-    const int K = 1 << (c - 5);
+  for (int c = kPackedClasses; c < kPackedClasses + kWaveKinds; ++c) {
+    const int K = 1 << (c - kPackedClasses);
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     if (cend[c] > cbeg[c]) ranges.push_back({cbeg[c], cend[c]});
     if (!fast)
-      for (int w = 0; w < 5; ++w)
+      for (int w = 0; w < kPackedClasses; ++w)
         if (cend[w] > cbeg[w] && swwave_k_for(sw16_width(w)) == K) ranges.push_back({cbeg[w], cend[w]});
     uint64_t total = 0;
     for (auto& rg : ranges) total += rg.second - rg.first;
     if (!total) continue;
-    uint32_t* list = sw_klist_[c - 5].ensure<uint32_t>(total + 1);
+    uint32_t* list = sw_klist_[c - kPackedClasses].ensure<uint32_t>(total + 1);
     hipStream_t s = class_stream(c);
     uint64_t o = 0;
     for (auto& rg : ranges) {
@@ -712,7 +712,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     }
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
-                   opt.gap_open, opt.gap_extend, cnt + 8 + (c - 5), outs, nullptr, nullptr, nullptr,
+                   opt.gap_open, opt.gap_extend, cnt + 8 + (c - kPackedClasses), outs, nullptr, nullptr, nullptr,
                    nullptr, fallback, cnt + 16, cus, s);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   }
@@ -725,7 +725,8 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
                    cnt + 16, cus, st_);
   const uint32_t nfb = read1(cnt + 16);
   for (int c = 0; c < kRowExact; ++c)
-    if ((c < 5 && nl && class_lanes[c]) || (c >= 5 && swk_launched(c, cbeg, cend, fast)))
+    if ((c < kPackedClasses && nl && class_lanes[c]) ||
+        (c >= kPackedClasses && swk_launched(c, cbeg, cend, fast)))
       S->ms_class[c] = stage_ms(12 + 2 * c, 13 + 2 * c);
   // Row-exact int32 kernel: bands wider than 64 * kMaxWaveK and the DPs whose
   // band never reaches the last in-band row but whose clamped rows may score.
@@ -775,10 +776,10 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
     *lo = std::max<int64_t>(1, (int64_t)j.d0 + 1);
     *hi = std::min<int64_t>(n, m + j.d1);
   };
-  for (int c = 0; c < 5; ++c) {
+  for (int c = 0; c < kPackedClasses; ++c) {
     if (!class_lanes[c]) continue;
     const int64_t W = sw16_width(c);
-    const int P = ((split_mask_ >> c) & 1u) ? 4 : 1;
+    const int P = c >= 5 ? 16 : (((split_mask_ >> c) & 1u) ? 4 : 1);
     const int per_wave = 64 / P;
     uint64_t band = 0, pair_slots = 0, wave_slots = 0;
     int64_t wave_max = 0, in_wave = 0;
@@ -802,7 +803,8 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
       }
     }
     std::fprintf(stderr, "[lanes] class %d (W %lld, %s): %u pairs, band cells %.3g, pair slots %.3g (x%.2f), "
-                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P == 4 ? "split" : "full", class_lanes[c],
+                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P == 16 ? "split16" : (P == 4 ? "split" : "full"),
+                 class_lanes[c],
                  (double)band, (double)pair_slots, (double)pair_slots / std::max<double>(1, band),
                  (double)wave_slots, (double)wave_slots / std::max<double>(1, band));
   }
@@ -811,8 +813,8 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
 bool Engine::swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, bool fast) const {
   if (cend[c] > cbeg[c]) return true;
   if (fast) return false;
-  for (int w = 0; w < 5; ++w)
-    if (cend[w] > cbeg[w] && swwave_k_for(sw16_width(w)) == (1 << (c - 5))) return true;
+  for (int w = 0; w < kPackedClasses; ++w)
+    if (cend[w] > cbeg[w] && swwave_k_for(sw16_width(w)) == (1 << (c - kPackedClasses))) return true;
   return false;
 }
 

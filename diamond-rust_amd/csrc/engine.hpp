@@ -130,7 +130,7 @@ class Engine {
   uint64_t count_last_dp_cells();
   unsigned split_mask() const { return split_mask_; }
   bool packed_f16() const { return use_f16_; }
-  static constexpr int kClasses = 12;  // k_classify in k_sw16.hip
+  static constexpr int kClasses = kPackedClasses + kWaveKinds + 1;  // k_classify in k_sw16.hip
   static constexpr int kRowExact = kClasses - 1;
   // Per score-kernel class of the last search (see StageStats::ms_class):
   // jobs, DP cells and kernel milliseconds.

@@ -68,6 +68,7 @@ struct I16 {
   static __device__ __forceinline__ V bits(int x) { return __builtin_bit_cast(V, x); }
   static __device__ __forceinline__ int raw(V v) { return __builtin_bit_cast(int, v); }
   static __device__ __forceinline__ V splat(int x) { return V{(short)x, (short)x}; }
+  static __device__ __forceinline__ V pair(int a, int b) { return V{(short)a, (short)b}; }
   static __device__ __forceinline__ V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
   static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
   static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_max(a, b); }
@@ -81,6 +82,7 @@ struct F16 {
   static __device__ __forceinline__ V bits(int x) { return __builtin_bit_cast(V, x); }
   static __device__ __forceinline__ int raw(V v) { return __builtin_bit_cast(int, v); }
   static __device__ __forceinline__ V splat(int x) { return V{(_Float16)x, (_Float16)x}; }
+  static __device__ __forceinline__ V pair(int a, int b) { return V{(_Float16)a, (_Float16)b}; }
   static __device__ __forceinline__ V add(V a, V b) { return a + b; }
   static __device__ __forceinline__ V sub(V a, V b) { return a - b; }
   static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_maximum(a, b); }
@@ -318,6 +320,15 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
 // best = max(best, F_in); it is skipped on rows where no lane's F_in is
 // positive (it cannot beat H >= 0 then). E crosses lanes from the previous
 // row: a lane's last offset reads the next lane's first.
+//
+// With kWall the kernel serves any band width up to W (merged bands): each
+// DP of a lane has its own right wall at its width W'. Cells at offsets
+// t >= W' take G = H + kWallOff instead of H - o + 1 (a per-offset addend,
+// so the wall costs registers, not instructions): their G stays far below any
+// score, E there never rises above it (E only takes G from the row above,
+// one offset right), so the last band cell reads no value from past the
+// wall, and their H (at most a horizontal gap out of the band) never exceeds
+// the best in the band. The F_in correction adds the same per-offset addend.
 template <int W, int P>
 struct SplitShape {
   static constexpr int H = (W + 1 + P - 1) / P;  // offsets per lane
@@ -325,7 +336,9 @@ struct SplitShape {
   static constexpr int kDeadFrom = W - (P - 1) * H;  // first dead offset of the last lane
 };
 
-template <int W, int P, int kThreads, class Tr>
+constexpr int kWallOff = -30000;
+
+template <int W, int P, int kThreads, class Tr, bool kWall>
 __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes,
                                                    const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                                    const int* __restrict__ pair_tab,
@@ -334,11 +347,11 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
                                                    uint32_t* __restrict__ redo,
                                                    uint32_t* __restrict__ nredo) {
   static_assert(W % 2 == 1, "band widths are odd");
-  static_assert(P == 2 || P == 4, "lane groups of 2 or 4");
+  static_assert(P == 4 || P == 8 || P == 16, "lane groups of 4, 8 or 16");
   constexpr int H = SplitShape<W, P>::H;
   constexpr int NR = SplitShape<W, P>::NR;
   constexpr int kDeadFrom = SplitShape<W, P>::kDeadFrom;
-  static_assert(kDeadFrom >= 1 && kDeadFrom <= H, "the wall lies in the last lane");
+  static_assert(kWall || (kDeadFrom >= 1 && kDeadFrom <= H), "the wall lies in the last lane");
   __shared__ int T[kPairTab];
   for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
   __syncthreads();
@@ -352,17 +365,25 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   const V GOFF = Tr::splat(1 - kOpen);
   const V FRAME2 = Tr::splat(kFrame);
   const V ZERO2 = Tr::splat(0);
-  // Moves within a lane group (quad_perm): from the next lane, from the
-  // previous lane, from two lanes back, and the group-wide maximum.
-  constexpr int kFromNext = P == 4 ? 0xF9 : 0xF5;  // [1,2,3,3] / [1,1,3,3]
-  constexpr int kFromPrev = P == 4 ? 0x90 : 0xA0;  // [0,0,1,2] / [0,0,2,2]
+  // Moves within a lane group: from the next lane and from the previous
+  // lane (quad_perm [1,2,3,3] / [0,0,1,2] for groups of 4, row_shl:1 /
+  // row_shr:1 for groups of 8 and 16; a lane whose source lies outside its
+  // group receives a value it does not use), and the swaps of the final
+  // reduction.
+  constexpr int kFromNext = P == 4 ? 0xF9 : 0x101;
+  constexpr int kFromPrev = P == 4 ? 0x90 : 0x111;
   auto dppi = [](int v, auto ctrl) { return __builtin_amdgcn_update_dpp(v, v, decltype(ctrl)::value, 0xf, 0xf, false); };
   auto mv = [&](V v, auto ctrl) { return Tr::bits(dppi(Tr::raw(v), ctrl)); };
   using NextC = std::integral_constant<int, kFromNext>;
   using PrevC = std::integral_constant<int, kFromPrev>;
   using Back2C = std::integral_constant<int, 0x40>;   // [0,0,0,1]
+  using Shr2C = std::integral_constant<int, 0x112>;   // row_shr:2
+  using Shr4C = std::integral_constant<int, 0x114>;   // row_shr:4
+  using Shr8C = std::integral_constant<int, 0x118>;   // row_shr:8
+  using RowMirrorC = std::integral_constant<int, 0x140>;  // row_mirror
   using SwapC = std::integral_constant<int, 0xB1>;    // [1,0,3,2]
   using Swap2C = std::integral_constant<int, 0x4E>;   // [2,3,0,1]
+  using MirrorC = std::integral_constant<int, 0x141>; // row_half_mirror
   auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
   for (;;) {
     const uint32_t task =
@@ -398,10 +419,19 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     // (the next lane's first offset, or the wall for the last lane).
     V G[H + 1], E[H + 1];
     uint32_t R[NR];
+    // kWall: per offset, the addend that turns H into G (1 - o in the band,
+    // kWallOff past the DP's own width), as two halves.
+    V GO[kWall ? H : 1];
+    const int liveA = kWall ? (d1A - d0A + 1) - t0 : 0, liveB = kWall ? (d1B - d0B + 1) - t0 : 0;
 #pragma unroll
     for (int u = 0; u <= H; ++u) {
-      const bool dead = u == H || (last && u >= kDeadFrom);
-      G[u] = dead ? NEG2 : Tr::splat(-kOpen);
+      if (kWall && u < H) {
+        GO[u] = Tr::pair(u < liveA ? 1 - kOpen : kWallOff, u < liveB ? 1 - kOpen : kWallOff);
+        G[u] = Tr::pair(u < liveA ? -kOpen : kNeg16, u < liveB ? -kOpen : kNeg16);
+      } else {
+        const bool dead = u == H || (last && u >= kDeadFrom);
+        G[u] = dead ? NEG2 : Tr::splat(-kOpen);
+      }
       E[u] = NEG2;
     }
 #pragma unroll
@@ -469,7 +499,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
         const V Hh = Tr::max(X0, Fv);
         best = Tr::max(best, Hh);
-        Gl = Tr::add(Hh, GOFF);
+        Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : GOFF);
         G[u] = Gl;
         E[u] = Ev;
         F = Fv;
@@ -479,30 +509,46 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       // stays at or below some F of the row, which the f16 bound needs.)
       const V fout = Tr::sub(Tr::max(Gl, F), ONE2);
       V y = Tr::sub(fout, Tr::splat((P - 1 - p) * H));
-      y = Tr::max(y, mv(y, PrevC{}));
-      if (P == 4) y = Tr::max(y, mv(y, Back2C{}));
+      if (P == 4) {  // quad_perm sources stay inside the quad
+        y = Tr::max(y, mv(y, PrevC{}));
+        y = Tr::max(y, mv(y, Back2C{}));
+      } else if (P == 8) {  // row shifts reach into the other half-row: masked
+        const V s1 = mv(y, PrevC{});
+        y = p >= 1 ? Tr::max(y, s1) : y;
+        const V s2 = mv(y, Shr2C{});
+        y = p >= 2 ? Tr::max(y, s2) : y;
+        const V s4 = mv(y, Shr4C{});
+        y = p >= 4 ? Tr::max(y, s4) : y;
+      } else {  // the group is a whole row: a lane without source keeps its own
+        y = Tr::max(y, mv(y, PrevC{}));
+        y = Tr::max(y, mv(y, Shr2C{}));
+        y = Tr::max(y, mv(y, Shr4C{}));
+        y = Tr::max(y, mv(y, Shr8C{}));
+      }
       V fin = Tr::add(mv(y, PrevC{}), Tr::splat((P - p) * H));
       if (p == 0) fin = NEG2;
       best = Tr::max(best, fin);
       // The correction can only raise offsets u < F_in (F_in taken out of the
       // frame): from there on F_in - o - u is at most -o, the least G. It runs
-      // in chunks of kChunk offsets while some lane of the wave still needs it.
+      // in chunks of kChunk offsets while some lane of the wave still needs it
+      // (with walls: inside its DPs' bands).
       {
         const V fu = Tr::sub(fin, Zr);
-        const int fl = max(Tr::lo(fu), Tr::hi(fu));
-        V fc = Tr::add(fin, GOFF);
+        const int fl = kWall ? max(min(Tr::lo(fu), liveA), min(Tr::hi(fu), liveB))
+                             : max(Tr::lo(fu), Tr::hi(fu));
+        V fc = kWall ? fin : Tr::add(fin, GOFF);
         constexpr int kChunk = 8;
 #pragma unroll
         for (int c = 0; c < H; c += kChunk) {
           if (__ballot(fl > c) == 0) break;
 #pragma unroll
           for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
-            G[u] = Tr::max(G[u], fc);
+            G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0]) : fc);
             fc = Tr::sub(fc, ONE2);
           }
         }
       }
-      if (last) {
+      if (last && !kWall) {
 #pragma unroll
         for (int u = kDeadFrom; u < H; ++u) {
           G[u] = NEG2;
@@ -530,7 +576,9 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     }
     best = Tr::sub(best, Zr);
     best = Tr::max(best, mv(best, SwapC{}));
-    if (P == 4) best = Tr::max(best, mv(best, Swap2C{}));
+    best = Tr::max(best, mv(best, Swap2C{}));
+    if (P >= 8) best = Tr::max(best, mv(best, MirrorC{}));
+    if (P == 16) best = Tr::max(best, mv(best, RowMirrorC{}));
     const int bA = Tr::lo(best), bB = Tr::hi(best);
     if (p == 0) {
       if (hasA) {
@@ -553,22 +601,24 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   }
 }
 
-// Band-width classes served by k_sw16 (2 * band + 1 for each band_for value).
-__device__ __forceinline__ int width_class(int w) {
+// Packed class of a band width: 0-4 the widths 2 * band + 1 of the band_for
+// values (25/33/61/81/129), 5-6 merged bands of widths 130..193 and
+// 194..256 (kernels with per-DP walls); -1 for any other width.
+__device__ __forceinline__ int packed_class(int w) {
   switch (w) {
     case 25: return 0;
     case 33: return 1;
     case 61: return 2;
     case 81: return 3;
     case 129: return 4;
-    default: return 5;
+    default: return w > 129 && w <= 193 ? 5 : (w > 193 && w <= 256 ? 6 : -1);
   }
 }
 
-// Sort key per job: class | target | rows. Classes 0-4: packed int16 kernel
-// for band widths 25/33/61/81/129; 5-10: wave-per-DP int32 kernel with
-// K = 1..32 offsets per lane (widths up to 64 * K); 11: row-exact int32
-// kernel (wider bands). Within a target the jobs are ordered by row span,
+// Sort key per job: class | target | rows. Classes 0-6: packed 16-bit
+// kernels (packed_class); 7-12: wave-per-DP int32 kernel with K = 1..32
+// offsets per lane (widths up to 64 * K); 13: row-exact int32 kernel (wider
+// bands). Within a target the jobs are ordered by row span,
 // then first row (32-row buckets): the packed kernels pair consecutive jobs
 // in a lane, which runs the union of their rows, and a wave runs as many
 // rows as its longest lane, so neighbours should start and end alike.
@@ -581,16 +631,16 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
   const int64_t m = db.offs[j.subject + 1] - db.offs[j.subject];
   const int64_t n = (int64_t)j.q1 - j.q0;
   const int64_t w = (int64_t)j.d1 - j.d0 + 1;
-  int cls = width_class((int)(w > 1000 ? 1000 : w));
+  int cls = packed_class((int)(w > 1000 ? 1000 : w));
   const int64_t rlo = j.d0 + 1 > 1 ? j.d0 + 1 : 1;
   const int64_t istar = m + j.d1;
   const int64_t rhi = istar < n ? istar : n;
-  if (j.q0 != 0 || n != qlen || rlo > rhi || istar < 1 || m < 1 || n > 16383 || m > 16383) cls = 5;
-  if (cls == 5) {
-    cls = 5 + kWaveKinds;
+  if (j.q0 != 0 || n != qlen || rlo > rhi || istar < 1 || m < 1 || n > 16383 || m > 16383) cls = -1;
+  if (cls < 0) {
+    cls = kPackedClasses + kWaveKinds;
     for (int c = 0; c < kWaveKinds; ++c)
       if (w <= (64ll << c)) {
-        cls = 5 + c;
+        cls = kPackedClasses + c;
         break;
       }
   }
@@ -635,7 +685,7 @@ __global__ void k_lane_keys(const uint2* __restrict__ lanes, uint32_t nlanes, co
     const int m = (int)(db.offs[j.subject + 1] - db.offs[j.subject]);
     lo = min(lo, max(1, j.d0 + 1));
     hi = max(hi, min(n, m + j.d1));
-    cls = width_class(j.d1 - j.d0 + 1);
+    cls = packed_class(j.d1 - j.d0 + 1);
   }
   const int span = min(max(hi - lo + 1, 0), (int)kLaneSpanMax);
   keys[L] = ((uint64_t)cls << kLaneSpanBits) | (uint64_t)(kLaneSpanMax - span);
@@ -706,23 +756,30 @@ void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2
   hipLaunchKernelGGL(k_gather_lanes, dim3(nblk(n, 256)), dim3(256), 0, st, in, idx, n, out);
 }
 
-static const int kWidths[5] = {25, 33, 61, 81, 129};
+static const int kWidths[kPackedClasses] = {25, 33, 61, 81, 129, 193, 257};
 int sw16_width(int cls) { return kWidths[cls]; }
 
 template <int W, class Tr>
 void launch_width(bool split, unsigned tasks, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
                   SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
                   SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
-  if (split) {
-    static const unsigned resident = resident_blocks(k_sw16p<W, 4, 512, Tr>, 512, cu_count);
+  if constexpr (W > 129) {  // merged bands: groups of 16 lanes, per-DP walls
+    static const unsigned resident = resident_blocks(k_sw16p<W, 16, 512, Tr, true>, 512, cu_count);
     const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));
-    hipLaunchKernelGGL((k_sw16p<W, 4, 512, Tr>), dim3(blocks), dim3(512), 0, st, lanes, nlanes, jobs,
-                       q, db, pair_tab, counter, outs, redo, nredo);
+    hipLaunchKernelGGL((k_sw16p<W, 16, 512, Tr, true>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
+                       jobs, q, db, pair_tab, counter, outs, redo, nredo);
   } else {
-    static const unsigned resident = resident_blocks(k_sw16<W, Tr>, 256, cu_count);
-    const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));
-    hipLaunchKernelGGL((k_sw16<W, Tr>), dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,
-                       pair_tab, m32, counter, outs, redo, nredo);
+    if (split) {
+      static const unsigned resident = resident_blocks(k_sw16p<W, 4, 512, Tr, false>, 512, cu_count);
+      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));
+      hipLaunchKernelGGL((k_sw16p<W, 4, 512, Tr, false>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
+                         jobs, q, db, pair_tab, counter, outs, redo, nredo);
+    } else {
+      static const unsigned resident = resident_blocks(k_sw16<W, Tr>, 256, cu_count);
+      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));
+      hipLaunchKernelGGL((k_sw16<W, Tr>), dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db,
+                         pair_tab, m32, counter, outs, redo, nredo);
+    }
   }
 }
 
@@ -730,8 +787,9 @@ void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nla
                  SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
                  SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!nlanes) return;
-  // A wave carries 64 DP pairs (one per lane) or, split, 16 (four lanes each).
-  const unsigned tasks = nblk(nlanes, split ? 16 : 64);
+  // A wave carries 64 DP pairs (one per lane) or, split, 16 (four lanes
+  // each); the merged-band classes carry 4 (a row of 16 lanes each).
+  const unsigned tasks = nblk(nlanes, cls >= 5 ? 4 : (split ? 16 : 64));
 #define DMR_SW16(c, w)                                                                             \
   case c:                                                                                          \
     if (f16)                                                                                       \
@@ -747,6 +805,8 @@ void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nla
     DMR_SW16(2, 61)
     DMR_SW16(3, 81)
     DMR_SW16(4, 129)
+    DMR_SW16(5, 193)
+    DMR_SW16(6, 257)
     default:
       break;
   }

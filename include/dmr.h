@@ -212,14 +212,15 @@ int dmr_result_stats(const dmr_result* r, double* out20);
 /* Cells of every banded DP of the last search on ctx (for GCUPS). */
 int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
 /* Score-pass profile of the last search on ctx, per kernel class
- * (DMR_SCORE_CLASSES entries): 0-4 packed int16 kernel for band widths
- * 25/33/61/81/129, 5-10 wave-per-DP int32 kernel for widths up to
- * 64/128/256/512/1024/2048, 11 row-exact int32 kernel (wider bands, and
+ * (DMR_SCORE_CLASSES entries): 0-4 packed 16-bit kernel for band widths
+ * 25/33/61/81/129, 5-6 packed kernel for merged bands of widths 130..193 and
+ * 194..256, 7-12 wave-per-DP int32 kernel for widths up to
+ * 64/128/256/512/1024/2048, 13 row-exact int32 kernel (wider bands, and
  * bands that end before their clamped rows): DP jobs, DP cells and kernel
  * milliseconds. Diagnostics only; no counterpart in the reference. */
-#define DMR_SCORE_CLASSES 12
+#define DMR_SCORE_CLASSES 14
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
-/* Layout of the packed classes 0-4 on ctx: bit c of *split_mask set = class
+/* Layout of the packed classes 0-4 on ctx (5-6 always use sixteen lanes): bit c of *split_mask set = class
  * c runs split over four lanes per DP pair (k_sw16p), clear = one lane
  * (k_sw16); *f16 = 1 when the packed kernels compute in f16 pairs, 0 in
  * saturating int16 pairs. Either may be NULL. Diagnostics only; no

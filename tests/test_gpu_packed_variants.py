@@ -1,5 +1,5 @@
 """The packed 16-bit score pass in both layouts and both arithmetics against
-the oracle.
+the oracle, and the merged-band classes (sixteen lanes per pair, per-DP walls).
 
 Each width class runs either with one lane per DP pair (the whole band in a
 lane's registers) or split over four lanes (quarter bands, with the
@@ -78,6 +78,32 @@ def test_scores_around_f16_limit(layout_ctx, orc):
         assert g == r, (len(c[0]), len(c[1]), c[2], c[3], r)
 
 
+def test_merged_band_widths(layout_ctx, orc):
+    """Bands of widths 130..256 (merged bands) run the sixteen-lane kernels
+    with a wall at each DP's own width (classes m193/m256); 129 and 257+ stay
+    on their own kernels. Every width near a class edge is included."""
+    rng = np.random.default_rng(91)
+    edges = [129, 130, 131, 192, 193, 194, 195, 255, 256, 257, 258]
+    cases = []
+    for k in range(360):
+        w = edges[k] if k < len(edges) else int(rng.integers(130, 257))
+        n = int(rng.integers(1, 1000 if rng.random() < 0.2 else 400))
+        q = rand_seq(rng, n, canonical_only=rng.random() < 0.9)
+        r = rng.random()
+        s = q.copy() if r < 0.2 else (mutate(rng, q, float(rng.uniform(0.1, 0.6)), 0.08) if r < 0.7
+                                      else rand_seq(rng, int(rng.integers(1, 900))))
+        if len(s) == 0:
+            s = rand_seq(rng, 2)
+        d0 = int(rng.integers(-len(s) - w, n + 1))
+        cases.append((q, s, d0, d0 + w - 1))
+    assert sum(129 < c[3] - c[2] + 1 <= 193 for c in cases) > 50
+    assert sum(193 < c[3] - c[2] + 1 <= 256 for c in cases) > 50
+    got = _score_batch(layout_ctx, cases)
+    for (q, s, d0, d1), g in zip(cases, got):
+        ref = orc.extend_span(q, s, d0, d1)
+        assert g == (ref["score"] if ref else None), (len(q), len(s), d0, d1)
+
+
 def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
     """24 queries of the 1024-query --sensitive workload: many DPs per query,
     so lanes carry two DPs."""
@@ -85,4 +111,6 @@ def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
     w = workload_dict(w)
     res, ref = run_both(dmr, layout_ctx, orc, w, "sensitive")
     assert len(ref) > 0
+    prof = layout_ctx.class_profile()
+    assert prof["m193"]["jobs"] > 0 and prof["m256"]["jobs"] > 0
     assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
