@@ -122,6 +122,12 @@ constexpr int kMaxWaveK = 32;
 constexpr int kWaveKinds = 6;  // K = 1, 2, 4, 8, 16, 32
 constexpr int kPackedClasses = 7;  // packed 16-bit score classes (k_sw16.hip)
 int swwave_k_for(int64_t width);
+// Packs the run-length ops of n traceback DPs (capacity slots at ops_off)
+// into packed, at pack_off (n + 1 offsets; pack_off[n] = total words).
+// sizes: n + 1 words of scratch; scan_tmp: scan_tmp_elems(n + 1).
+void launch_ops_pack(const SwOut* outs, const uint32_t* ops, const uint64_t* ops_off, uint32_t n,
+                     uint32_t* sizes, uint64_t* pack_off, uint64_t* scan_tmp, uint32_t* packed,
+                     hipStream_t st);
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,

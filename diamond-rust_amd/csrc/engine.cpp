@@ -841,33 +841,40 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
   }
   uint32_t* cnt = tb_counters_.ensure<uint32_t>(2 * kWaveKinds);
   launch_fill_u32(cnt, 2 * kWaveKinds, 0, st_);
-  // The groups run side by side on the auxiliary streams.
-  fork();
   const int cus = cu_count_;
   std::vector<uint64_t> fallback_items;
   uint64_t cells = 0;
+  // Per group, its job list and the trace and ops capacity offsets, all in
+  // one pinned block that goes to the device in one copy: [list u32 x ni,
+  // padded to 8 bytes][trace offsets u64 x ni+1][ops offsets u64 x ni+1].
+  uint64_t in_off[kWaveKinds] = {}, in_words = 0;  // in u64 units
+  for (int g = 0; g < kWaveKinds; ++g) {
+    const uint64_t ni = group[g].size();
+    if (!ni) continue;
+    in_off[g] = in_words;
+    in_words += (ni + 1) / 2 + 2 * (ni + 1);
+  }
+  uint64_t* pin_in = tb_pin_in_.ensure<uint64_t>(in_words + 1);
+  auto rows_of = [&](uint64_t k) {
+    const SwJob& j = hjobs[k];
+    const int64_t n = (int64_t)j.q1 - j.q0, m = slen[k];
+    const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
+    return rhi >= rlo ? rhi - rlo + 1 : 0;
+  };
   for (int g = 0; g < kWaveKinds; ++g) {
     auto& items = group[g];
-    if (items.empty()) continue;
     const uint64_t ni = items.size();
+    if (!ni) continue;
     // Rows of a job's band and its cells; the longest jobs start first so
     // they do not finish last.
-    auto rows_of = [&](uint64_t k) {
-      const SwJob& j = hjobs[k];
-      const int64_t n = (int64_t)j.q1 - j.q0, m = slen[k];
-      const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
-      return rhi >= rlo ? rhi - rlo + 1 : 0;
-    };
     std::stable_sort(items.begin(), items.end(), [&](uint64_t a, uint64_t b) {
       return rows_of(a) * ((int64_t)hjobs[a].d1 - hjobs[a].d0 + 1) >
              rows_of(b) * ((int64_t)hjobs[b].d1 - hjobs[b].d0 + 1);
     });
-    std::vector<uint32_t>& jl = tb_host_jl_[g];
-    std::vector<uint64_t>& toff = tb_host_toff_[g];
-    std::vector<uint64_t>& ooff = tb_host_ooff_[g];
-    jl.assign(ni, 0);
-    toff.assign(ni + 1, 0);
-    ooff.assign(ni + 1, 0);
+    uint32_t* jl = reinterpret_cast<uint32_t*>(pin_in + in_off[g]);
+    uint64_t* toff = pin_in + in_off[g] + (ni + 1) / 2;
+    uint64_t* ooff = toff + ni + 1;
+    toff[0] = ooff[0] = 0;
     const int64_t stride = 64ll << g;  // trace row: 64 * K bytes
     for (uint64_t t = 0; t < ni; ++t) {
       const SwJob& j = hjobs[items[t]];
@@ -878,47 +885,104 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
       ooff[t + 1] = ooff[t] + (uint64_t)(n + m + 1);
       cells += (uint64_t)(rows * ((int64_t)j.d1 - j.d0 + 1));
     }
-    // The two largest groups (K = 8, K = 4) on streams of their own.
-    static const int kStreamOf[kWaveKinds] = {2, 2, 1, 0, 0, 1};
-    hipStream_t s = aux_[kStreamOf[g]];
-    uint32_t* dl = tb_list_[g].ensure<uint32_t>(ni + 1);
-    uint64_t* dt = tb_toff_[g].ensure<uint64_t>(ni + 1);
-    uint64_t* dop = tb_ooff_[g].ensure<uint64_t>(ni + 1);
-    DMR_HIP(hipMemcpyAsync(dl, jl.data(), ni * 4, hipMemcpyHostToDevice, s));
-    DMR_HIP(hipMemcpyAsync(dt, toff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, s));
-    DMR_HIP(hipMemcpyAsync(dop, ooff.data(), (ni + 1) * 8, hipMemcpyHostToDevice, s));
-    uint8_t* tr = tb_trace_[g].ensure<uint8_t>(toff[ni] + 1);
-    uint32_t* dops = tb_ops_[g].ensure<uint32_t>(ooff[ni] + 1);
+  }
+  uint64_t* din = tb_in_.ensure<uint64_t>(in_words + 1);
+  DMR_HIP(hipMemcpyAsync(din, pin_in, in_words * 8, hipMemcpyHostToDevice, st_));
+  // The groups run side by side on the auxiliary streams, widest first (its
+  // DPs run longest).
+  fork();
+  for (int g = kWaveKinds - 1; g >= 0; --g) {
+    const uint64_t ni = group[g].size();
+    if (!ni) continue;
+    // The two largest groups (K = 8, K = 4) on streams of their own, the
+    // widest two (few DPs) on the engine stream, K = 1, 2 on the third.
+    static const int kStreamOf[kWaveKinds] = {2, 2, 1, 0, -1, -1};
+    hipStream_t s = kStreamOf[g] < 0 ? st_ : aux_[kStreamOf[g]];
+    const uint32_t* dl = reinterpret_cast<const uint32_t*>(din + in_off[g]);
+    const uint64_t* dt = din + in_off[g] + (ni + 1) / 2;
+    const uint64_t* dop = dt + ni + 1;
+    const uint64_t* htoff = pin_in + in_off[g] + (ni + 1) / 2;
+    const uint64_t* hooff = htoff + ni + 1;
+    uint8_t* tr = tb_trace_[g].ensure<uint8_t>(htoff[ni] + 1);
+    uint32_t* dops = tb_ops_[g].ensure<uint32_t>(hooff[ni] + 1);
     SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
                    cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, s);
   }
   join();
+  // Results: the ops of each group packed on the device (n_ops words per
+  // DP), then outputs, pack offsets, packed ops and fallback lists through
+  // pinned memory: two rounds of copies, one wait each.
+  uint64_t out_base[kWaveKinds] = {}, poff_base[kWaveKinds] = {}, nout = 0;
+  for (int g = 0; g < kWaveKinds; ++g) {
+    const uint64_t ni = group[g].size();
+    if (!ni) continue;
+    out_base[g] = nout;
+    poff_base[g] = nout + g;
+    nout += ni;
+  }
+  SwOut* pin_out = tb_pin_out_.ensure<SwOut>(nout + 1);
+  uint64_t* pin_poff = tb_pin_poff_.ensure<uint64_t>(nout + kWaveKinds + 1);
+  uint32_t* pin_fb = tb_pin_fb_.ensure<uint32_t>(nout + kWaveKinds + 1);
+  DMR_HIP(hipMemcpyAsync(pin_fb, cnt + kWaveKinds, kWaveKinds * 4, hipMemcpyDeviceToHost, st_));
+  for (int g = 0; g < kWaveKinds; ++g) {
+    const uint64_t ni = group[g].size();
+    if (!ni) continue;
+    const uint64_t* dop = tb_in_.as<uint64_t>() + in_off[g] + (ni + 1) / 2 + ni + 1;
+    const uint64_t cap = (pin_in + in_off[g] + (ni + 1) / 2 + ni + 1)[ni];
+    uint64_t* poff = tb_poff_[g].ensure<uint64_t>(ni + 1);
+    launch_ops_pack(tb_out_[g].as<SwOut>(), tb_ops_[g].as<uint32_t>(), dop, (uint32_t)ni,
+                    tb_sizes_.ensure<uint32_t>(ni + 1), poff,
+                    b_scan_.ensure<uint64_t>(scan_tmp_elems(ni + 1)), tb_pack_[g].ensure<uint32_t>(cap + 1),
+                    st_);
+    DMR_HIP(hipMemcpyAsync(pin_out + out_base[g], tb_out_[g].as<SwOut>(), ni * sizeof(SwOut),
+                           hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(pin_poff + poff_base[g], poff, (ni + 1) * 8, hipMemcpyDeviceToHost, st_));
+  }
   DMR_HIP(hipStreamSynchronize(st_));
-  uint32_t hfb[kWaveKinds] = {};
-  DMR_HIP(hipMemcpy(hfb, cnt + kWaveKinds, sizeof(hfb), hipMemcpyDeviceToHost));
+  uint32_t nfb[kWaveKinds];
+  std::copy(pin_fb, pin_fb + kWaveKinds, nfb);
+  uint64_t ops_base[kWaveKinds] = {}, nops = 0, fb_base[kWaveKinds] = {}, nfbt = 0;
+  for (int g = 0; g < kWaveKinds; ++g) {
+    const uint64_t ni = group[g].size();
+    if (!ni) continue;
+    ops_base[g] = nops;
+    nops += pin_poff[poff_base[g] + ni];
+    fb_base[g] = nfbt;
+    nfbt += nfb[g];
+  }
+  uint32_t* pin_ops = tb_pin_ops_.ensure<uint32_t>(nops + 1);
+  uint32_t* pin_fbl = tb_pin_fb_.ensure<uint32_t>(nfbt + 1);  // counts already read
+  for (int g = 0; g < kWaveKinds; ++g) {
+    const uint64_t ni = group[g].size();
+    if (!ni) continue;
+    const uint64_t words = pin_poff[poff_base[g] + ni];
+    if (words)
+      DMR_HIP(hipMemcpyAsync(pin_ops + ops_base[g], tb_pack_[g].as<uint32_t>(), words * 4,
+                             hipMemcpyDeviceToHost, st_));
+    if (nfb[g])
+      DMR_HIP(hipMemcpyAsync(pin_fbl + fb_base[g], tb_fb_[g].as<uint32_t>(), nfb[g] * 4,
+                             hipMemcpyDeviceToHost, st_));
+  }
+  DMR_HIP(hipStreamSynchronize(st_));
   for (int g = 0; g < kWaveKinds; ++g) {
     const auto& items = group[g];
-    if (items.empty()) continue;
     const uint64_t ni = items.size();
-    std::vector<SwOut> ho(ni);
-    std::vector<uint32_t> hops(tb_host_ooff_[g][ni]);
-    DMR_HIP(hipMemcpyAsync(ho.data(), tb_out_[g].as<SwOut>(), ni * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(hops.data(), tb_ops_[g].as<uint32_t>(), hops.size() * 4, hipMemcpyDeviceToHost, st_));
-    std::vector<uint32_t> fbt(hfb[g]);
-    if (hfb[g]) DMR_HIP(hipMemcpyAsync(fbt.data(), tb_fb_[g].as<uint32_t>(), hfb[g] * 4, hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipStreamSynchronize(st_));
+    if (!ni) continue;
     std::vector<char> is_fb(ni, 0);
-    for (uint32_t t : fbt) {
+    for (uint32_t f = 0; f < nfb[g]; ++f) {
+      const uint32_t t = pin_fbl[fb_base[g] + f];
       is_fb[t] = 1;
       fallback_items.push_back(items[t]);
     }
+    const uint64_t* po = pin_poff + poff_base[g];
     for (uint64_t t = 0; t < ni; ++t) {
       if (is_fb[t]) continue;
-      (*outs)[items[t]] = ho[t];
-      const uint64_t o = tb_host_ooff_[g][t];
-      (*ops)[items[t]].assign(hops.begin() + o, hops.begin() + o + ho[t].n_ops);
+      const SwOut& o = pin_out[out_base[g] + t];
+      (*outs)[items[t]] = o;
+      const uint32_t* src = pin_ops + ops_base[g] + po[t];
+      (*ops)[items[t]].assign(src, src + o.n_ops);
     }
   }
   for (uint64_t k : group[kWaveKinds]) fallback_items.push_back(k);

@@ -61,6 +61,33 @@ struct DevBuf {
   }
 };
 
+// Pinned host memory (hipHostMalloc), grown like DevBuf: device-to-host and
+// host-to-device copies through it run as DMA without staging.
+struct PinBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  PinBuf() = default;
+  PinBuf(const PinBuf&) = delete;
+  PinBuf& operator=(const PinBuf&) = delete;
+  ~PinBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* ensure(size_t n) {
+    size_t bytes = n * sizeof(T) + 256;
+    if (bytes > cap) {
+      release();
+      size_t c = bytes + bytes / 4;
+      DMR_HIP(hipHostMalloc(&p, c, hipHostMallocDefault));
+      cap = c;
+    }
+    return static_cast<T*>(p);
+  }
+};
+
 struct SearchOptions {
   std::string sensitivity = "default";
   double evalue = 0.001;
@@ -84,9 +111,10 @@ struct StageStats {
   uint64_t query_seeds = 0, runs = 0, hits = 0, hit_runs = 0, segments = 0, groups = 0;
   uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
   uint64_t sw16_lanes = 0, sw_generic = 0, sw_redo = 0;
-  // Score-kernel time per class: 0-4 packed int16 (widths 25/33/61/81/129),
-  // 5-10 wave-per-DP int32 (K = 1/2/4/8/16/32), 11 row-exact int32.
-  double ms_class[12] = {};
+  // Score-kernel time per class: 0-4 packed 16-bit (widths 25/33/61/81/129),
+  // 5-6 packed 16-bit merged bands, 7-12 wave-per-DP int32 (K = 1..32), 13
+  // row-exact int32.
+  double ms_class[kPackedClasses + kWaveKinds + 1] = {};
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
 };
@@ -179,7 +207,8 @@ class Engine {
 
   int device_ = 0;
   hipStream_t st_ = nullptr;
-  hipEvent_t ev_[40] = {};
+  // 0-11 stage boundaries, 12 + 2c / 13 + 2c around score class c.
+  hipEvent_t ev_[12 + 2 * kClasses] = {};
   // Auxiliary streams (default priority) for kernels that may run side by
   // side: fork() makes them wait for the work queued on st_, join() makes st_
   // wait for everything queued on them.
@@ -234,11 +263,11 @@ class Engine {
   // Traceback groups: one per wave kernel K, then the row-exact kernel.
   static constexpr int kTbGroups = kWaveKinds + 1;
   DevBuf tb_counters_, tb_list_[kTbGroups], tb_toff_[kTbGroups], tb_ooff_[kTbGroups],
-      tb_trace_[kTbGroups], tb_ops_[kTbGroups], tb_out_[kTbGroups], tb_fb_[kTbGroups];
+      tb_trace_[kTbGroups], tb_ops_[kTbGroups], tb_out_[kTbGroups], tb_fb_[kTbGroups], tb_in_,
+      tb_sizes_, tb_poff_[kTbGroups], tb_pack_[kTbGroups];
+  PinBuf tb_pin_in_, tb_pin_out_, tb_pin_poff_, tb_pin_ops_, tb_pin_fb_;
   // Host copies of each group's job list and offsets; they stay alive until
   // the copies queued from them have run.
-  std::vector<uint64_t> tb_host_ooff_[kTbGroups], tb_host_toff_[kTbGroups];
-  std::vector<uint32_t> tb_host_jl_[kTbGroups];
   std::vector<uint64_t> last_cbeg_, last_cend_;
   int cu_count_ = 256;
   bool force_generic_ = false;

@@ -303,6 +303,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     if (lane == 0) {
       if (need_exact) {
         fallback[atomicAdd(nfallback, 1u)] = kTrace ? task : jid;
+        if (kTrace) outs[task] = SwOut{};  // no ops: the row-exact kernel redoes this DP
       } else if (!kTrace) {
         outs[jid].score = best;
         outs[jid].flags = 0;
@@ -391,6 +392,43 @@ int swwave_k_for(int64_t width) {
   for (int K = 1; K <= kMaxWaveK; K <<= 1)
     if (width <= 64 * K) return K;
   return 0;
+}
+
+// Run-length ops of traceback DPs, packed: sizes[t] = n_ops of DP t
+// (sizes[n] = 0, so an exclusive scan gives the packed offsets), then each
+// DP's ops move from its capacity slot (ops_off, n + 1 offsets) to its packed
+// slot, one wave per DP. A count never exceeds the DP's capacity slot.
+__device__ __forceinline__ uint32_t ops_count(const SwOut* outs, const uint64_t* ops_off, uint32_t t) {
+  const uint64_t cap = ops_off[t + 1] - ops_off[t];
+  const uint32_t c = outs[t].n_ops;
+  return c < cap ? c : (uint32_t)cap;
+}
+
+__global__ void k_ops_sizes(const SwOut* __restrict__ outs, const uint64_t* __restrict__ ops_off, uint32_t n,
+                            uint32_t* __restrict__ sizes) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) sizes[t] = ops_count(outs, ops_off, t);
+  else if (t == n) sizes[n] = 0;
+}
+
+__global__ void k_ops_pack(const SwOut* __restrict__ outs, const uint32_t* __restrict__ ops,
+                           const uint64_t* __restrict__ ops_off, const uint64_t* __restrict__ pack_off,
+                           uint32_t n, uint32_t* __restrict__ packed) {
+  const uint32_t t = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (t >= n) return;
+  const uint32_t cnt = ops_count(outs, ops_off, t);
+  const uint32_t* src = ops + ops_off[t];
+  uint32_t* dst = packed + pack_off[t];
+  for (uint32_t k = threadIdx.x & 63; k < cnt; k += 64) dst[k] = src[k];
+}
+
+void launch_ops_pack(const SwOut* outs, const uint32_t* ops, const uint64_t* ops_off, uint32_t n,
+                     uint32_t* sizes, uint64_t* pack_off, uint64_t* scan_tmp, uint32_t* packed,
+                     hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_ops_sizes, dim3(nblk(n + 1, 256)), dim3(256), 0, st, outs, ops_off, n, sizes);
+  scan_exclusive_u32_to_u64(sizes, pack_off, n + 1, scan_tmp, st);
+  hipLaunchKernelGGL(k_ops_pack, dim3(nblk(n, 4)), dim3(256), 0, st, outs, ops, ops_off, pack_off, n, packed);
 }
 
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
