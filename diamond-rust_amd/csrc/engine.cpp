@@ -538,8 +538,10 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
       hj[i] = h_mjobs[sel[i]];
       slen[i] = h_db_offs_[hj[i].subject + 1] - h_db_offs_[hj[i].subject];
     }
+    // The score pass's best scores (outs, by job) let each traceback DP stop
+    // at the row of its first maximum.
     traceback_pass(jobs, list, hj, slen, qs, dbs, opt.gap_open, opt.gap_extend, &h_souts, &h_sops,
-                   &S.traceback_cells);
+                   &S.traceback_cells, outs);
   }
   stage_event(10);
   DMR_HIP(hipStreamSynchronize(st_));
@@ -821,11 +823,13 @@ bool Engine::swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, boo
 // Traceback DPs for jobs[list[k]] (hjobs[k] is the host copy, slen[k] the
 // subject length). Wave-per-DP kernel by band width; the row-exact kernel for
 // bands wider than 512 and for DPs whose best cell may sit in the clamped
-// rows below the band.
+// rows below the band. known (optional, device): the score pass's outputs by
+// job; with them each wave-kernel DP stops at the row of its first maximum.
 void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& list,
                             const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen,
                             SeqSet qs, SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
-                            std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out) {
+                            std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out,
+                            const SwOut* known) {
   const uint64_t ns = list.size();
   outs->assign(ns, SwOut{});
   ops->assign(ns, {});
@@ -908,7 +912,7 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
     SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, s);
+                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, s, known);
   }
   join();
   // Results: the ops of each group packed on the device (n_ops words per

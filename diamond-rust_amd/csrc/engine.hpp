@@ -167,7 +167,8 @@ class Engine {
   void traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& list,
                       const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
                       SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
-                      std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out);
+                      std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out,
+                      const SwOut* known = nullptr);
 
   // --- drop-in calls on host buffers -------------------------------------
   void seed_build(const uint8_t* seqs, const uint64_t* offs, uint32_t nseq, const ShapeDef& shape,

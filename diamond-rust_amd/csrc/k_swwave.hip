@@ -95,7 +95,8 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
                                                  uint32_t* __restrict__ ops,
                                                  const uint64_t* __restrict__ ops_off,
                                                  uint32_t* __restrict__ fallback,
-                                                 uint32_t* __restrict__ nfallback) {
+                                                 uint32_t* __restrict__ nfallback,
+                                                 const SwOut* __restrict__ known) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
   for (int k = threadIdx.x; k < kScoreStride * kScoreStride; k += blockDim.x) M[k] = m32[k];
   __syncthreads();
@@ -156,7 +157,12 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     };
     const bool rows = rlo <= rhi && m >= 1;
     int qn = rows ? qletters(rlo) : 0, en = rows ? eletters(rlo) : 0;
-    for (int blk = rlo; rows && blk <= rhi; blk += 64) {
+    // Trace mode with the DP's best score known (from the score pass): the
+    // first cell that reaches it is the first maximum in row-major order, so
+    // the rows after its row cannot change the result and are not run.
+    const int stop_at = (kTrace && known) ? uniform(known[jid].score) : INT32_MAX;
+    bool stop = false;
+    for (int blk = rlo; rows && blk <= rhi && !stop; blk += 64) {
      const int qv = dpp<kQuadIdentity>(0, qn), ev = dpp<kQuadIdentity>(0, en);
      qn = qletters(blk + 64);
      en = eletters(blk + 64);
@@ -268,6 +274,10 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
           best = rb;
           bi = i;
           bj = i - d1 + tf;
+        }
+        if (best >= stop_at) {
+          stop = true;
+          break;
         }
       } else {
         lbest = max(lbest, rowbest);
@@ -435,7 +445,7 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int cu_count, hipStream_t st) {
+                    int cu_count, hipStream_t st, const SwOut* known) {
   if (!njobs) return;
 #define DMR_SWW(k, tr)                                                                             \
   do {                                                                                             \
@@ -443,7 +453,7 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
     const unsigned blocks = std::min<unsigned>(resident, nblk(njobs, 4));                          \
     hipLaunchKernelGGL((k_sw_wave<k, tr>), dim3(blocks), dim3(256), 0, st, jobs, list, njobs, q,   \
                        db, m32, gap_open, gap_extend, counter, outs, trace, trace_off, ops,        \
-                       ops_off, fallback, nfallback);                                              \
+                       ops_off, fallback, nfallback, known);                                       \
   } while (0)
   if (traceback) {
     switch (K) {
