@@ -156,7 +156,7 @@ def main():
                                      "copy-calibrated; profiles/hbm_traffic.json)",
                      "gcups": round(dcells / (dms / 1e3) / 1e9, 2),
                      "basis": f"{OPS_PER_CELL} 16-bit ops per reference DP cell x cells of the "
-                              "launch / launch time (HIP events on the engine stream)"},
+                              "launch / launch time (HIP events on the stream it runs on)"},
         "score_pass_classes": prof,
         "roofline_score_pass": {
             "bound": "valu", "achieved": round(cells * OPS_PER_CELL / (sw / 1e3) / 1e12, 3),
