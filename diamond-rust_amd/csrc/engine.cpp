@@ -673,29 +673,24 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   }
   // The class kernels are independent (distinct jobs and counters; the redo
   // and fallback lists are filled through atomics): they run side by side on
-  // the auxiliary streams, so the small classes use the issue slots and
-  // registers the widest packed class leaves free. The widest class gets a
-  // stream of its own.
+  // the engine stream and the three auxiliary streams. The classes of few,
+  // long DPs (wide wave classes, merged bands) are latency-bound and start
+  // first, so they run under the widest packed class, which has a stream of
+  // its own; the other throughput classes queue behind them.
   fork();
-  int next_aux = 1;
-  auto class_stream = [&](int c) {
-    if (score_serial_) return st_;
-    return c == 4 ? aux_[0] : aux_[1 + (next_aux++ - 1) % (kAux - 1)];
+  auto launch_packed = [&](int c, hipStream_t s) {
+    if (!nl || !class_lanes[c]) return;
+    DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
+    launch_sw16(c, ((split_mask_ >> c) & 1u) != 0, use_f16_, lanes + class_lane0[c], class_lanes[c], jobs,
+                qs, dbs, (use_f16_ ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(),
+                cnt + 1 + c, outs, redo, cnt, cus, s);
+    DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   };
-  if (nl) {
-    for (int c = kPackedClasses - 1; c >= 0; --c) {
-      if (!class_lanes[c]) continue;
-      hipStream_t s = class_stream(c);
-      DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
-      launch_sw16(c, ((split_mask_ >> c) & 1u) != 0, use_f16_, lanes + class_lane0[c], class_lanes[c], jobs,
-                  qs, dbs, (use_f16_ ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(),
-                  cnt + 1 + c, outs, redo, cnt, cus, s);
-      DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
-    }
-  }
-  // Wave-per-DP int32 kernel: classes 5-10 (and every width class when the
-  // gap costs have no int16 table).
-  for (int c = kPackedClasses; c < kPackedClasses + kWaveKinds; ++c) {
+  // Wave-per-DP int32 kernel: classes 7-12 (and every width class when the
+  // gap costs have no 16-bit table). A class's jobs are one range of the
+  // sorted order, used in place; only the fallback for other gap costs
+  // concatenates ranges.
+  auto launch_wave = [&](int c, hipStream_t s) {
     const int K = 1 << (c - kPackedClasses);
     std::vector<std::pair<uint64_t, uint64_t>> ranges;
     if (cend[c] > cbeg[c]) ranges.push_back({cbeg[c], cend[c]});
@@ -704,19 +699,36 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
         if (cend[w] > cbeg[w] && swwave_k_for(sw16_width(w)) == K) ranges.push_back({cbeg[w], cend[w]});
     uint64_t total = 0;
     for (auto& rg : ranges) total += rg.second - rg.first;
-    if (!total) continue;
-    uint32_t* list = sw_klist_[c - kPackedClasses].ensure<uint32_t>(total + 1);
-    hipStream_t s = class_stream(c);
-    uint64_t o = 0;
-    for (auto& rg : ranges) {
-      DMR_HIP(hipMemcpyAsync(list + o, idx + rg.first, (rg.second - rg.first) * 4, hipMemcpyDeviceToDevice, s));
-      o += rg.second - rg.first;
+    if (!total) return;
+    const uint32_t* list = idx + ranges[0].first;
+    if (ranges.size() > 1) {
+      uint32_t* cat = sw_klist_[c - kPackedClasses].ensure<uint32_t>(total + 1);
+      uint64_t o = 0;
+      for (auto& rg : ranges) {
+        DMR_HIP(hipMemcpyAsync(cat + o, idx + rg.first, (rg.second - rg.first) * 4, hipMemcpyDeviceToDevice, s));
+        o += rg.second - rg.first;
+      }
+      list = cat;
     }
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
                    opt.gap_open, opt.gap_extend, cnt + 8 + (c - kPackedClasses), outs, nullptr, nullptr, nullptr,
                    nullptr, fallback, cnt + 16, cus, s);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
+  };
+  static_assert(kPackedClasses == 7 && kWaveKinds == 6, "the launch plan names every class");
+  const int W0 = kPackedClasses;  // first wave class (K = 1)
+  if (score_serial_) {
+    for (int c = kPackedClasses - 1; c >= 0; --c) launch_packed(c, st_);
+    for (int c = W0; c < W0 + kWaveKinds; ++c) launch_wave(c, st_);
+  } else {
+    for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
+    launch_packed(6, aux_[2]);
+    launch_packed(5, aux_[2]);
+    launch_packed(4, aux_[0]);
+    launch_packed(3, aux_[1]);
+    for (int c = 2; c >= 0; --c) launch_packed(c, aux_[2]);
+    for (int c = W0 + 2; c >= W0; --c) launch_wave(c, aux_[1]);  // K = 4, 2, 1
   }
   join();
   // int16-saturated DPs: re-scored by the int32 wave kernel (widths <= 129).
