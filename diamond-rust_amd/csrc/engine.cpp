@@ -462,6 +462,27 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   }
   stage_event(8);
   DMR_HIP(hipStreamSynchronize(st_));
+
+  // ---- traceback, launched before the selection -------------------------------
+  // The reported DPs are among the candidates with a positive score (H6), so
+  // all of those are traced while the host computes the exact E-values and
+  // selects; the few that are not selected are dropped at assembly. The
+  // score pass's best scores (outs, by job) let each traceback DP stop at the
+  // row of its first maximum.
+  std::vector<uint32_t> tb_list, tb_slen;
+  std::vector<SwJob> tb_jobs;
+  std::vector<uint64_t> tb_pos(nmaybe, ~0ull);  // maybe index -> traced index
+  for (uint64_t k = 0; k < nmaybe; ++k) {
+    if (h_mouts[k].score <= 0) continue;
+    tb_pos[k] = tb_list.size();
+    tb_list.push_back(h_midx[k]);
+    tb_jobs.push_back(h_mjobs[k]);
+    tb_slen.push_back(h_db_offs_[h_mjobs[k].subject + 1] - h_db_offs_[h_mjobs[k].subject]);
+  }
+  stage_event(9);
+  TbRun tb_run;
+  traceback_launch(&tb_run, jobs, tb_list, tb_jobs, tb_slen, qs, dbs, opt.gap_open, opt.gap_extend, outs);
+
   double th0 = now_ms();
   // Exact E-values of the candidates (glibc f64, as the reference), computed
   // in parallel; the selection below reads them in order.
@@ -526,34 +547,20 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   S.ms_host = now_ms() - th0;
   S.selected = sel.size();
 
-  // ---- traceback of the reported DPs ----------------------------------------
-  std::vector<SwOut> h_souts;
-  std::vector<std::vector<uint32_t>> h_sops;
-  stage_event(9);
-  {
-    std::vector<uint32_t> list(sel.size()), slen(sel.size());
-    std::vector<SwJob> hj(sel.size());
-    for (size_t i = 0; i < sel.size(); ++i) {
-      list[i] = h_midx[sel[i]];
-      hj[i] = h_mjobs[sel[i]];
-      slen[i] = h_db_offs_[hj[i].subject + 1] - h_db_offs_[hj[i].subject];
-    }
-    // The score pass's best scores (outs, by job) let each traceback DP stop
-    // at the row of its first maximum.
-    traceback_pass(jobs, list, hj, slen, qs, dbs, opt.gap_open, opt.gap_extend, &h_souts, &h_sops,
-                   &S.traceback_cells, outs);
-  }
+  std::vector<SwOut> h_touts;
+  std::vector<std::vector<uint32_t>> h_tops;
+  traceback_collect(&tb_run, &h_touts, &h_tops, &S.traceback_cells);
   stage_event(10);
   DMR_HIP(hipStreamSynchronize(st_));
 
   // ---- assemble HSPs in align_target order ----------------------------------
   {
-    uint64_t k = 0;
     for (auto& found : per_target) {
       for (auto& hs : found) {
         for (auto& c : hs) {
           const SwJob& j = h_mjobs[c.job];
-          const SwOut& o = h_souts[k];
+          const uint64_t k = tb_pos[c.job];  // selected DPs have a positive score
+          const SwOut& o = h_touts[k];
           HspRec h = c.h;
           h.q_begin = j.q0 + o.q_begin;
           h.q_end = j.q0 + o.best_i;
@@ -565,11 +572,10 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
           h.gap_openings = o.gap_openings;
           h.gaps = o.gaps;
           h.length = o.length;
-          h.ops = h_sops[k];
+          h.ops = std::move(h_tops[k]);
           if (o.score != c.h.score)
             throw Error(-2, "traceback score differs from the score pass");
           result[h.query].push_back(std::move(h));
-          ++k;
         }
       }
     }
@@ -842,12 +848,32 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
                             SeqSet qs, SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
                             std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out,
                             const SwOut* known) {
+  TbRun run;
+  traceback_launch(&run, djobs, list, hjobs, slen, qs, dbs, go, ge, known);
+  traceback_collect(&run, outs, ops, cells_out);
+}
+
+// First half of traceback_pass: every wave-kernel DP launched, its ops packed
+// and the first round of result copies queued. The host may work until
+// traceback_collect; list, hjobs and slen must stay alive until then.
+void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<uint32_t>& list,
+                              const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen,
+                              SeqSet qs, SeqSet dbs, int go, int ge, const SwOut* known) {
   const uint64_t ns = list.size();
-  outs->assign(ns, SwOut{});
-  ops->assign(ns, {});
+  *run = TbRun{};
+  run->djobs = djobs;
+  run->list = &list;
+  run->hjobs = &hjobs;
+  run->slen = &slen;
+  run->qs = qs;
+  run->dbs = dbs;
+  run->go = go;
+  run->ge = ge;
+  run->ns = ns;
   if (!ns) return;
   // Group g < kWaveKinds: wave kernel with K = 2^g; the last group: row-exact.
-  std::vector<std::vector<uint64_t>> group(kTbGroups);
+  std::vector<std::vector<uint64_t>>& group = run->group;
+  group.assign(kTbGroups, {});
   for (uint64_t k = 0; k < ns; ++k) {
     const SwJob& j = hjobs[k];
     const int K = swwave_k_for((int64_t)j.d1 - j.d0 + 1);
@@ -858,12 +884,12 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
   uint32_t* cnt = tb_counters_.ensure<uint32_t>(2 * kWaveKinds);
   launch_fill_u32(cnt, 2 * kWaveKinds, 0, st_);
   const int cus = cu_count_;
-  std::vector<uint64_t> fallback_items;
   uint64_t cells = 0;
   // Per group, its job list and the trace and ops capacity offsets, all in
   // one pinned block that goes to the device in one copy: [list u32 x ni,
   // padded to 8 bytes][trace offsets u64 x ni+1][ops offsets u64 x ni+1].
-  uint64_t in_off[kWaveKinds] = {}, in_words = 0;  // in u64 units
+  uint64_t* in_off = run->in_off;
+  uint64_t in_words = 0;  // in u64 units
   for (int g = 0; g < kWaveKinds; ++g) {
     const uint64_t ni = group[g].size();
     if (!ni) continue;
@@ -929,8 +955,11 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
   join();
   // Results: the ops of each group packed on the device (n_ops words per
   // DP), then outputs, pack offsets, packed ops and fallback lists through
-  // pinned memory: two rounds of copies, one wait each.
-  uint64_t out_base[kWaveKinds] = {}, poff_base[kWaveKinds] = {}, nout = 0;
+  // pinned memory: two rounds of copies, one wait each (the first queued
+  // here, the rest in traceback_collect).
+  uint64_t* out_base = run->out_base;
+  uint64_t* poff_base = run->poff_base;
+  uint64_t nout = 0;
   for (int g = 0; g < kWaveKinds; ++g) {
     const uint64_t ni = group[g].size();
     if (!ni) continue;
@@ -956,6 +985,32 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
                            hipMemcpyDeviceToHost, st_));
     DMR_HIP(hipMemcpyAsync(pin_poff + poff_base[g], poff, (ni + 1) * 8, hipMemcpyDeviceToHost, st_));
   }
+  run->cells = cells;
+}
+
+// Second half of traceback_pass: waits for the wave-kernel DPs, brings their
+// ops back and runs the row-exact fallback.
+void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
+                               std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out) {
+  const uint64_t ns = run->ns;
+  outs->assign(ns, SwOut{});
+  ops->assign(ns, {});
+  if (cells_out) *cells_out = 0;
+  if (!ns) return;
+  const std::vector<uint32_t>& list = *run->list;
+  const std::vector<SwJob>& hjobs = *run->hjobs;
+  const std::vector<uint32_t>& slen = *run->slen;
+  const std::vector<std::vector<uint64_t>>& group = run->group;
+  const uint64_t* out_base = run->out_base;
+  const uint64_t* poff_base = run->poff_base;
+  const SwJob* djobs = run->djobs;
+  const SeqSet qs = run->qs, dbs = run->dbs;
+  const int go = run->go, ge = run->ge;
+  uint64_t cells = run->cells;
+  std::vector<uint64_t> fallback_items;
+  SwOut* pin_out = tb_pin_out_.as<SwOut>();
+  uint64_t* pin_poff = tb_pin_poff_.as<uint64_t>();
+  uint32_t* pin_fb = tb_pin_fb_.as<uint32_t>();
   DMR_HIP(hipStreamSynchronize(st_));
   uint32_t nfb[kWaveKinds];
   std::copy(pin_fb, pin_fb + kWaveKinds, nfb);

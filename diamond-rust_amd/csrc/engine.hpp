@@ -86,6 +86,8 @@ struct PinBuf {
     }
     return static_cast<T*>(p);
   }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
 };
 
 struct SearchOptions {
@@ -169,6 +171,23 @@ class Engine {
                       SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
                       std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out,
                       const SwOut* known = nullptr);
+  // A traceback between its launch and the collection of its results.
+  struct TbRun {
+    const SwJob* djobs = nullptr;
+    const std::vector<uint32_t>* list = nullptr;
+    const std::vector<SwJob>* hjobs = nullptr;
+    const std::vector<uint32_t>* slen = nullptr;
+    SeqSet qs{}, dbs{};
+    int go = 0, ge = 0;
+    uint64_t ns = 0, cells = 0;
+    std::vector<std::vector<uint64_t>> group;
+    uint64_t in_off[kWaveKinds] = {}, out_base[kWaveKinds] = {}, poff_base[kWaveKinds] = {};
+  };
+  void traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<uint32_t>& list,
+                        const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
+                        SeqSet dbs, int go, int ge, const SwOut* known);
+  void traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector<std::vector<uint32_t>>* ops,
+                         uint64_t* cells_out);
 
   // --- drop-in calls on host buffers -------------------------------------
   void seed_build(const uint8_t* seqs, const uint64_t* offs, uint32_t nseq, const ShapeDef& shape,

@@ -84,6 +84,128 @@ __device__ __forceinline__ int letter_at(const uint8_t* ss, int m, int j) {
   return (j >= 1 && j <= m) ? (int)ss[j - 1] : kPad;
 }
 
+// Traceback walk of one DP by a whole wave (gapped::traceback,
+// src/gapped.rs:205-279): every lane runs the same state machine on
+// wave-uniform values, and the trace bytes, query letters and subject letters
+// it reads come from a wave-private LDS cache filled a block of kRows rows at
+// a time by all lanes (coalesced), so a step costs one LDS read instead of a
+// dependent global load. Lane 0 writes the run-length ops (op | count << 2,
+// 0 = M, 1 = I, 2 = D, in path order) and the caller's statistics in o.
+// Blocks have four waves (256 threads).
+template <int K>
+__device__ void trace_walk(int lane, int wave, const uint8_t* tr, int rlo, int d0, int d1, int m,
+                           const uint8_t* qs, const uint8_t* ss, const int8_t* M, int bi, int bj,
+                           uint32_t* op, SwOut& o) {
+  constexpr int kRowBytes = 64 * K;       // one trace row
+  constexpr int kRows = 64 / K;           // rows per block: 4 KB of trace
+  constexpr int kCols = kRows + 64 * K;   // subject letters those rows' bands cover
+  __shared__ uint32_t tcache[4][kRows * kRowBytes / 4];
+  __shared__ uint8_t qcache[4][kRows];
+  __shared__ uint8_t scache[4][kCols];
+  const uint8_t* tc = reinterpret_cast<const uint8_t*>(tcache[wave]);
+  const RowRange R{d0, d1, m};
+  int r0 = INT32_MAX, c0 = 1;  // first cached row and column
+  // Rows r0..i of the trace (all written: i <= bi), their query letters and
+  // the subject letters of columns max(1, r0 - d1) .. min(m, i - d0).
+  // All loads of a block are issued before its LDS writes (clamped, always
+  // in-bounds addresses), so a block costs one memory latency. Column
+  // bounds: c0 <= m (r0 <= istar = m + d1) and c1 >= c0 (r0 >= d0 + 1).
+  constexpr int kTW = kRows * kRowBytes / 4 / 64;  // trace words per lane
+  constexpr int kSW = (kCols + 63) / 64;           // subject letters per lane
+  auto refill = [&](int i) {
+    r0 = max(rlo, i - kRows + 1);
+    const int words = (i - r0 + 1) * (kRowBytes / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(tr + (uint64_t)(r0 - rlo) * kRowBytes);
+    uint32_t tv[kTW];
+#pragma unroll
+    for (int k = 0; k < kTW; ++k) tv[k] = src[min(lane + 64 * k, words - 1)];
+    const int nq = i - r0 + 1;  // <= kRows <= 64
+    const int qv = qs[r0 - 1 + min(lane, nq - 1)];
+    c0 = max(1, r0 - d1);
+    const int ns = min(m, i - d0) - c0 + 1;
+    int sv[kSW];
+#pragma unroll
+    for (int k = 0; k < kSW; ++k) sv[k] = ss[c0 - 1 + min(lane + 64 * k, ns - 1)];
+#pragma unroll
+    for (int k = 0; k < kTW; ++k)
+      if (lane + 64 * k < words) tcache[wave][lane + 64 * k] = tv[k];
+    if (lane < nq) qcache[wave][lane] = (uint8_t)qv;
+#pragma unroll
+    for (int k = 0; k < kSW; ++k)
+      if (lane + 64 * k < ns) scache[wave][lane + 64 * k] = (uint8_t)sv[k];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");  // the cache is read by every lane
+  };
+  uint32_t nr = 0, cur = 3, cnt = 0;
+  int state = 0, i = bi, j = bj;
+  bool in_gap = false;
+  auto emit = [&](uint32_t v) {
+    if (lane == 0) op[nr] = v;
+    ++nr;
+  };
+  while (i > 0 && j > 0) {
+    if (j < R.lo(i) || j > R.hi(i) || i < rlo) break;
+    if (i < r0) refill(i);
+    const int tv = uniform((int)tc[(i - r0) * kRowBytes + (j - i + d1)]);
+    uint32_t code;
+    if (state == 0) {
+      const int c = tv & 3;
+      if (c == 0) break;
+      if (c == 2) {
+        state = 1;
+        continue;
+      }
+      if (c == 3) {
+        state = 2;
+        continue;
+      }
+      const int x = uniform((int)qcache[wave][i - r0]), y = uniform((int)scache[wave][j - c0]);
+      o.length++;
+      if (x == y) o.identities++;
+      else o.mismatches++;
+      if (M[x * kScoreStride + y] > 0) o.positives++;
+      in_gap = false;
+      code = 0;
+      --i;
+      --j;
+    } else {
+      o.length++;
+      o.gaps++;
+      if (!in_gap) {
+        o.gap_openings++;
+        in_gap = true;
+      }
+      if (state == 1) {
+        code = 1;
+        if (((tv >> 2) & 1) == 0) state = 0;
+        --i;
+      } else {
+        code = 2;
+        if (((tv >> 3) & 1) == 0) state = 0;
+        --j;
+      }
+    }
+    if (code == cur) {
+      ++cnt;
+    } else {
+      if (cnt) emit(cur | (cnt << 2));
+      cur = code;
+      cnt = 1;
+    }
+  }
+  if (cnt) emit(cur | (cnt << 2));
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");  // lane 0's ops are read by every lane
+  // The walk runs backwards: reverse the ops, one pair per lane.
+  for (uint32_t a = lane; a < nr / 2; a += 64) {
+    const uint32_t b = nr - 1 - a, x = op[a], y = op[b];
+    op[a] = y;
+    op[b] = x;
+  }
+  o.n_ops = nr;
+  o.q_begin = (uint32_t)i;
+  o.s_begin = (uint32_t)j;
+  if (nr == 0) o.score = 0;
+}
+
 template <int K, bool kTrace>
 __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
                                                  const uint32_t* __restrict__ list, uint32_t njobs,
@@ -310,6 +432,17 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         need_exact = false;
       }
     }
+    if constexpr (kTrace) {
+      if (!need_exact) {  // the whole wave walks the trace back
+        SwOut o = {};
+        o.score = best;
+        o.best_i = (uint32_t)bi;
+        o.best_j = (uint32_t)bj;
+        if (best > 0)
+          trace_walk<K>(lane, threadIdx.x / 64, tr, rlo, d0, d1, m, qs, ss, M, bi, bj, ops + ops_off[task], o);
+        if (lane == 0) outs[task] = o;
+      }
+    }
     if (lane == 0) {
       if (need_exact) {
         fallback[atomicAdd(nfallback, 1u)] = kTrace ? task : jid;
@@ -317,78 +450,6 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
       } else if (!kTrace) {
         outs[jid].score = best;
         outs[jid].flags = 0;
-      } else {
-        SwOut o = {};
-        o.score = best;
-        o.best_i = (uint32_t)bi;
-        o.best_j = (uint32_t)bj;
-        if (best > 0) {
-          const RowRange R{d0, d1, m};
-          uint32_t* op = ops + ops_off[task];
-          uint32_t nr = 0, cur = 3, cnt = 0;
-          int state = 0, i = bi, j = bj;
-          bool in_gap = false;
-          while (i > 0 && j > 0) {
-            if (j < R.lo(i) || j > R.hi(i) || i < rlo) break;
-            const uint8_t tv = tr[(uint64_t)(i - rlo) * (64 * K) + (j - i + d1)];
-            uint32_t code;
-            if (state == 0) {
-              const uint8_t c = tv & 3;
-              if (c == 0) break;
-              if (c == 2) {
-                state = 1;
-                continue;
-              }
-              if (c == 3) {
-                state = 2;
-                continue;
-              }
-              const uint8_t x = qs[i - 1], y = ss[j - 1];
-              o.length++;
-              if (x == y) o.identities++;
-              else o.mismatches++;
-              if (M[x * kScoreStride + y] > 0) o.positives++;
-              in_gap = false;
-              code = 0;
-              --i;
-              --j;
-            } else {
-              o.length++;
-              o.gaps++;
-              if (!in_gap) {
-                o.gap_openings++;
-                in_gap = true;
-              }
-              if (state == 1) {
-                code = 1;
-                if (((tv >> 2) & 1) == 0) state = 0;
-                --i;
-              } else {
-                code = 2;
-                if (((tv >> 3) & 1) == 0) state = 0;
-                --j;
-              }
-            }
-            if (code == cur) {
-              ++cnt;
-            } else {
-              if (cnt) op[nr++] = cur | (cnt << 2);
-              cur = code;
-              cnt = 1;
-            }
-          }
-          if (cnt) op[nr++] = cur | (cnt << 2);
-          for (uint32_t a = 0, b = nr ? nr - 1 : 0; a < b; ++a, --b) {
-            uint32_t x = op[a];
-            op[a] = op[b];
-            op[b] = x;
-          }
-          o.n_ops = nr;
-          o.q_begin = (uint32_t)i;
-          o.s_begin = (uint32_t)j;
-          if (nr == 0) o.score = 0;
-        }
-        outs[task] = o;
       }
     }
   }
