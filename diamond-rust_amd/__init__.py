@@ -52,7 +52,8 @@ def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
     if name.startswith("m"):  # merged bands: sixteen lanes per pair, per-DP walls
         return f"k_sw16p<{257 if name == 'm256' else 193}, 16, 512, {tr}, true>"
     w = int(name[1:])
-    return f"k_sw16p<{w}, 4, 512, {tr}, false>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
+    lanes = 2 if w == 81 else 4
+    return f"k_sw16p<{w}, {lanes}, 512, {tr}, false>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
 
 
 HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
@@ -345,8 +346,8 @@ class Context:
 
     def score_layout(self):
         """(split mask, f16) of the packed score classes: bit c set = class c
-        split over four lanes per DP pair; f16 True = pairs in f16, False =
-        saturating int16."""
+        split over several lanes per DP pair (two for w81, four otherwise);
+        f16 True = pairs in f16, False = saturating int16."""
         mask, f16 = C.c_uint32(), C.c_uint32()
         _check(lib().dmr_score_layout(self._h, C.byref(mask), C.byref(f16)))
         return mask.value, bool(f16.value)

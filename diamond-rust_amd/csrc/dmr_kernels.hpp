@@ -80,6 +80,8 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, co
                        const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
                        hipStream_t st);
 int sw16_width(int cls);
+// Lanes per DP pair of class cls when it runs split (always for 5-6).
+int sw16_split_lanes(int cls);
 // Lane order of the packed classes: key = class << kLaneSpanBits | (kLaneSpanMax
 // - row span), so a sort puts each class's longest lanes first.
 constexpr int kLaneSpanBits = 13;
@@ -87,8 +89,8 @@ constexpr uint32_t kLaneSpanMax = (1u << kLaneSpanBits) - 1;
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
                       uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st);
-// Packed score pass of width class cls (k_sw16.hip): split = four lanes per
-// DP pair (quarter bands each) for classes 0-4; classes 5-6 (merged bands)
+// Packed score pass of width class cls (k_sw16.hip): split = sw16_split_lanes
+// lanes per DP pair (part bands each) for classes 0-4; classes 5-6 (merged bands)
 // always run on sixteen lanes per pair with per-DP walls; f16 = half-precision
 // arithmetic (pair_tab must then be the f16 table).
 void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,

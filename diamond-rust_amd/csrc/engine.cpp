@@ -799,7 +799,7 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
   for (int c = 0; c < kPackedClasses; ++c) {
     if (!class_lanes[c]) continue;
     const int64_t W = sw16_width(c);
-    const int P = c >= 5 ? 16 : (((split_mask_ >> c) & 1u) ? 4 : 1);
+    const int P = (c >= 5 || ((split_mask_ >> c) & 1u)) ? sw16_split_lanes(c) : 1;
     const int per_wave = 64 / P;
     uint64_t band = 0, pair_slots = 0, wave_slots = 0;
     int64_t wave_max = 0, in_wave = 0;
@@ -823,7 +823,7 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
       }
     }
     std::fprintf(stderr, "[lanes] class %d (W %lld, %s): %u pairs, band cells %.3g, pair slots %.3g (x%.2f), "
-                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P == 16 ? "split16" : (P == 4 ? "split" : "full"),
+                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P == 1 ? "full" : (P == 2 ? "split2" : (P == 4 ? "split4" : "split16")),
                  class_lanes[c],
                  (double)band, (double)pair_slots, (double)pair_slots / std::max<double>(1, band),
                  (double)wave_slots, (double)wave_slots / std::max<double>(1, band));

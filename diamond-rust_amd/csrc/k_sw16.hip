@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
                                                    uint32_t* __restrict__ redo,
                                                    uint32_t* __restrict__ nredo) {
   static_assert(W % 2 == 1, "band widths are odd");
-  static_assert(P == 4 || P == 8 || P == 16, "lane groups of 4, 8 or 16");
+  static_assert(P == 2 || P == 4 || P == 8 || P == 16, "lane groups of 2, 4, 8 or 16");
   constexpr int H = SplitShape<W, P>::H;
   constexpr int NR = SplitShape<W, P>::NR;
   constexpr int kDeadFrom = SplitShape<W, P>::kDeadFrom;
@@ -366,12 +366,12 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   const V FRAME2 = Tr::splat(kFrame);
   const V ZERO2 = Tr::splat(0);
   // Moves within a lane group: from the next lane and from the previous
-  // lane (quad_perm [1,2,3,3] / [0,0,1,2] for groups of 4, row_shl:1 /
-  // row_shr:1 for groups of 8 and 16; a lane whose source lies outside its
-  // group receives a value it does not use), and the swaps of the final
-  // reduction.
-  constexpr int kFromNext = P == 4 ? 0xF9 : 0x101;
-  constexpr int kFromPrev = P == 4 ? 0x90 : 0x111;
+  // lane (quad_perm [1,1,3,3] / [0,0,2,2] for pairs, [1,2,3,3] / [0,0,1,2]
+  // for groups of 4, row_shl:1 / row_shr:1 for groups of 8 and 16; a lane
+  // whose source lies outside its group receives a value it does not use),
+  // and the swaps of the final reduction.
+  constexpr int kFromNext = P == 2 ? 0xF5 : (P == 4 ? 0xF9 : 0x101);
+  constexpr int kFromPrev = P == 2 ? 0xA0 : (P == 4 ? 0x90 : 0x111);
   auto dppi = [](int v, auto ctrl) { return __builtin_amdgcn_update_dpp(v, v, decltype(ctrl)::value, 0xf, 0xf, false); };
   auto mv = [&](V v, auto ctrl) { return Tr::bits(dppi(Tr::raw(v), ctrl)); };
   using NextC = std::integral_constant<int, kFromNext>;
@@ -509,7 +509,8 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       // stays at or below some F of the row, which the f16 bound needs.)
       const V fout = Tr::sub(Tr::max(Gl, F), ONE2);
       V y = Tr::sub(fout, Tr::splat((P - 1 - p) * H));
-      if (P == 4) {  // quad_perm sources stay inside the quad
+      if (P == 2) {  // one source lane: the exclusive prefix is its value
+      } else if (P == 4) {  // quad_perm sources stay inside the quad
         y = Tr::max(y, mv(y, PrevC{}));
         y = Tr::max(y, mv(y, Back2C{}));
       } else if (P == 8) {  // row shifts reach into the other half-row: masked
@@ -576,7 +577,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     }
     best = Tr::sub(best, Zr);
     best = Tr::max(best, mv(best, SwapC{}));
-    best = Tr::max(best, mv(best, Swap2C{}));
+    if (P >= 4) best = Tr::max(best, mv(best, Swap2C{}));
     if (P >= 8) best = Tr::max(best, mv(best, MirrorC{}));
     if (P == 16) best = Tr::max(best, mv(best, RowMirrorC{}));
     const int bA = Tr::lo(best), bB = Tr::hi(best);
@@ -757,6 +758,12 @@ void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2
 }
 
 static const int kWidths[kPackedClasses] = {25, 33, 61, 81, 129, 193, 257};
+// Lanes per DP pair of the split layout for band width W: two for W = 81
+// (41 offsets per lane still fit the registers without spilling, and the
+// per-row work is spread over twice the offsets), four otherwise.
+template <int W>
+constexpr int kSplitLanes = W == 81 ? 2 : 4;
+int sw16_split_lanes(int cls) { return cls >= 5 ? 16 : (kWidths[cls] == 81 ? 2 : 4); }
 int sw16_width(int cls) { return kWidths[cls]; }
 
 template <int W, class Tr>
@@ -770,9 +777,10 @@ void launch_width(bool split, unsigned tasks, const uint2* lanes, uint32_t nlane
                        jobs, q, db, pair_tab, counter, outs, redo, nredo);
   } else {
     if (split) {
-      static const unsigned resident = resident_blocks(k_sw16p<W, 4, 512, Tr, false>, 512, cu_count);
+      constexpr int P = kSplitLanes<W>;
+      static const unsigned resident = resident_blocks(k_sw16p<W, P, 512, Tr, false>, 512, cu_count);
       const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));
-      hipLaunchKernelGGL((k_sw16p<W, 4, 512, Tr, false>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
+      hipLaunchKernelGGL((k_sw16p<W, P, 512, Tr, false>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
                          jobs, q, db, pair_tab, counter, outs, redo, nredo);
     } else {
       static const unsigned resident = resident_blocks(k_sw16<W, Tr>, 256, cu_count);
@@ -787,9 +795,9 @@ void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nla
                  SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
                  SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!nlanes) return;
-  // A wave carries 64 DP pairs (one per lane) or, split, 16 (four lanes
+  // A wave carries 64 DP pairs (one per lane) or, split, 64 / P (P lanes
   // each); the merged-band classes carry 4 (a row of 16 lanes each).
-  const unsigned tasks = nblk(nlanes, cls >= 5 ? 4 : (split ? 16 : 64));
+  const unsigned tasks = nblk(nlanes, 64 / (cls >= 5 || split ? sw16_split_lanes(cls) : 1));
 #define DMR_SW16(c, w)                                                                             \
   case c:                                                                                          \
     if (f16)                                                                                       \

@@ -2,7 +2,7 @@
 the oracle, and the merged-band classes (sixteen lanes per pair, per-DP walls).
 
 Each width class runs either with one lane per DP pair (the whole band in a
-lane's registers) or split over four lanes (quarter bands, with the
+lane's registers) or split over two or four lanes (part bands, with the
 horizontal-gap correction across lanes). DMR_SW16_SPLIT selects the layout
 per class when a context is created (bit c = class c). The pairs compute in
 f16 (exact for integers up to 2048; DPs scoring 1900 or more are re-scored in
