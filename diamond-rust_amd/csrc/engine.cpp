@@ -77,6 +77,10 @@ Engine::Engine(int device) : device_(device) {
   // stream (per-class kernel times without sharing the chip).
   const char* ss = std::getenv("DMR_SCORE_SERIAL");
   score_serial_ = ss && ss[0] == '1';
+  // DMR_SCORE_PLAN (0-2, default 0) selects how the class kernels share the
+  // streams (see score_pass).
+  const char* plan = std::getenv("DMR_SCORE_PLAN");
+  score_plan_ = plan ? std::atoi(plan) : 0;
   // Packed score pass in f16 (default) or int16: DMR_SW16_F16=0 selects int16.
   const char* hf = std::getenv("DMR_SW16_F16");
   use_f16_ = !(hf && hf[0] == '0');
@@ -681,8 +685,8 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   // and fallback lists are filled through atomics): they run side by side on
   // the engine stream and the three auxiliary streams. The classes of few,
   // long DPs (wide wave classes, merged bands) are latency-bound and start
-  // first, so they run under the widest packed class, which has a stream of
-  // its own; the other throughput classes queue behind them.
+  // first on streams of their own; the VALU-bound packed classes follow one
+  // another on a third (DMR_SCORE_PLAN selects other arrangements).
   fork();
   auto launch_packed = [&](int c, hipStream_t s) {
     if (!nl || !class_lanes[c]) return;
@@ -727,13 +731,27 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   if (score_serial_) {
     for (int c = kPackedClasses - 1; c >= 0; --c) launch_packed(c, st_);
     for (int c = W0; c < W0 + kWaveKinds; ++c) launch_wave(c, st_);
-  } else {
+  } else if (score_plan_ == 2) {
+    // Every throughput class on a stream of its own beside the others.
     for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
     launch_packed(6, aux_[2]);
     launch_packed(5, aux_[2]);
     launch_packed(4, aux_[0]);
     launch_packed(3, aux_[1]);
     for (int c = 2; c >= 0; --c) launch_packed(c, aux_[2]);
+    for (int c = W0 + 2; c >= W0; --c) launch_wave(c, aux_[1]);  // K = 4, 2, 1
+  } else {
+    // The throughput-bound packed classes one after another on one stream
+    // (plan 0, the default: the smaller classes first, then the widest, which
+    // then has the chip nearly to itself; plan 1: widest first); the
+    // latency-bound classes (few, long DPs) beside them on the other streams.
+    for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
+    launch_packed(6, aux_[2]);
+    launch_packed(5, aux_[2]);
+    if (score_plan_ == 1)
+      for (int c = 4; c >= 0; --c) launch_packed(c, aux_[0]);
+    else
+      for (int c : {3, 2, 1, 0, 4}) launch_packed(c, aux_[0]);
     for (int c = W0 + 2; c >= W0; --c) launch_wave(c, aux_[1]);  // K = 4, 2, 1
   }
   join();

@@ -294,6 +294,7 @@ class Engine {
   unsigned split_mask_ = 0x18u;
   bool lane_report_ = false;
   bool score_serial_ = false;
+  int score_plan_ = 0;
   bool use_f16_ = true;
   void report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
                     const uint32_t* class_lane0, const uint32_t* class_lanes);
