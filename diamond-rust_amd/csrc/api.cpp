@@ -2,12 +2,13 @@
 // frame merging and outfmt-6 rendering.
 #include "../../include/dmr.h"
 
+#include "host_pool.hpp"
+
 #include <algorithm>
 #include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -83,21 +84,26 @@ std::vector<std::vector<HspRec>> merge_frames(std::vector<std::vector<HspRec>>& 
     }
   }
   for (auto& hs : per) {
+    if (hs.empty() || (hs.size() == 1 && max_hsps >= 1 && max_targets >= 1)) continue;
     std::stable_sort(hs.begin(), hs.end(), hsp_less);
+    // Subjects in order of first appearance (few per query: a linear scan),
+    // each keeping its first max_hsps HSPs.
     std::vector<uint32_t> order;
-    std::map<uint32_t, std::vector<HspRec>> by;
+    std::vector<std::vector<HspRec>> by;
     for (auto& h : hs) {
-      auto it = by.find(h.subject);
-      if (it != by.end()) {
-        if (it->second.size() < max_hsps) it->second.push_back(std::move(h));
+      size_t k = 0;
+      while (k < order.size() && order[k] != h.subject) ++k;
+      if (k < order.size()) {
+        if (by[k].size() < max_hsps) by[k].push_back(std::move(h));
       } else if (order.size() < max_targets) {
         order.push_back(h.subject);
-        by[h.subject].push_back(std::move(h));
+        by.emplace_back();
+        by.back().push_back(std::move(h));
       }
     }
     hs.clear();
-    for (uint32_t s : order)
-      for (auto& h : by[s]) hs.push_back(std::move(h));
+    for (auto& b : by)
+      for (auto& h : b) hs.push_back(std::move(h));
   }
   return per;
 }
@@ -162,13 +168,19 @@ dmr::SearchOptions options(const dmr_search_config* c) {
 void assemble(dmr_ctx* ctx, dmr_result* r, std::vector<std::vector<HspRec>>& per_target,
               const dmr::SearchOptions& o, size_t nqueries) {
   auto per = merge_frames(per_target, r->targets, nqueries, o.max_hsps, o.max_target_seqs);
-  std::map<std::pair<uint32_t, uint32_t>, int> tix;
-  for (size_t t = 0; t < r->targets.query_of.size(); ++t)
-    tix[{r->targets.query_of[t], r->targets.frame_of[t]}] = (int)t;
+  // Target of each (query, frame): frames are 0..5.
+  std::vector<int> tix(nqueries * 6, -1);
+  for (size_t t = 0; t < r->targets.query_of.size(); ++t) {
+    const size_t q = r->targets.query_of[t], f = r->targets.frame_of[t];
+    if (q < nqueries && f < 6) tix[q * 6 + f] = (int)t;
+  }
+  size_t total = 0;
+  for (auto& hs : per) total += hs.size();
+  r->target_of.reserve(total);
+  r->hsps.reserve(total);
   for (auto& hs : per)
     for (auto& h : hs) {
-      auto it = tix.find({h.query, h.frame});
-      r->target_of.push_back(it == tix.end() ? -1 : it->second);
+      r->target_of.push_back(h.query < nqueries && h.frame < 6 ? tix[(size_t)h.query * 6 + h.frame] : -1);
       r->hsps.push_back(std::move(h));
     }
   r->db_offs = ctx->eng->db_offsets_shared();
@@ -622,6 +634,10 @@ std::vector<std::pair<const char*, size_t>> lines_of(const char* s) {
   return out;
 }
 
+void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, size_t>>& qid,
+                  const std::vector<std::pair<const char*, size_t>>& sid, const std::vector<int>& f,
+                  size_t i0, size_t i1, std::string* text);
+
 int render(const dmr_result* r, const char* qids, const char* sids, const char* fields,
            std::string* text) {
   const auto qid = lines_of(qids), sid = lines_of(sids);
@@ -636,17 +652,39 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
   if (f.empty())
     f = {kQseqid, kSseqid, kPident, kLength, kMismatch, kGapopen,
          kQstart, kQend, kSstart, kSend, kEvalue, kBitscore};
+  for (size_t i = 0; i < r->hsps.size(); ++i)
+    if (r->target_of[i] >= 0 && (r->hsps[i].query >= qid.size() || r->hsps[i].subject >= sid.size()))
+      return fail(DMR_E_ARG, "id list shorter than the query or database");
+  // Lines are independent: blocks of kBlock HSPs render in parallel into
+  // their own strings, joined in order.
+  constexpr size_t kBlock = 512;
+  const size_t nh = r->hsps.size(), nblocks = (nh + kBlock - 1) / kBlock;
+  std::vector<std::string> parts(nblocks);
+  dmr::host_parallel_for(nblocks, 1, [&](uint64_t b0, uint64_t b1) {
+    for (uint64_t blk = b0; blk < b1; ++blk)
+      render_lines(r, qid, sid, f, blk * kBlock, std::min(nh, (size_t)(blk + 1) * kBlock), &parts[blk]);
+  });
   std::string& out = *text;
   out.clear();
-  out.reserve(r->hsps.size() * 96);
+  size_t total = 0;
+  for (auto& pt : parts) total += pt.size();
+  out.reserve(total);
+  for (auto& pt : parts) out += pt;
+  return DMR_OK;
+}
+
+// outfmt-6 lines of HSPs [i0, i1) (ids already checked) appended to out.
+void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, size_t>>& qid,
+                  const std::vector<std::pair<const char*, size_t>>& sid, const std::vector<int>& f,
+                  size_t i0, size_t i1, std::string* text) {
+  std::string& out = *text;
+  out.reserve((i1 - i0) * 96);
   char nb[32];
   auto num = [&](long long x) { out.append(nb, std::to_chars(nb, nb + sizeof nb, x).ptr); };
-  for (size_t i = 0; i < r->hsps.size(); ++i) {
+  for (size_t i = i0; i < i1; ++i) {
     const HspRec& h = r->hsps[i];
     int ti = r->target_of[i];
     if (ti < 0) continue;
-    if (h.query >= qid.size() || h.subject >= sid.size())
-      return fail(DMR_E_ARG, "id list shorter than the query or database");
     const uint8_t* letters = r->targets.letters.data() + r->targets.offs[ti];
     size_t frame_len = r->targets.offs[ti + 1] - r->targets.offs[ti];
     const uint8_t* subj = r->db_res->data() + (*r->db_offs)[h.subject];
@@ -767,7 +805,6 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
     }
     out += '\n';
   }
-  return DMR_OK;
 }
 
 }  // namespace

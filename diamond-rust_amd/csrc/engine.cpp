@@ -1,6 +1,8 @@
 // Host orchestration of the search core (see engine.hpp).
 #include "engine.hpp"
 
+#include "host_pool.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -9,7 +11,6 @@
 #include <cstring>
 #include <functional>
 #include <numeric>
-#include <thread>
 
 namespace dmr {
 
@@ -121,24 +122,6 @@ T Engine::read1(const T* dptr) {
   DMR_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
   return v;
-}
-
-// Runs f(0..n-1) on up to 8 host threads (contiguous slices).
-static void parallel_for(uint64_t n, const std::function<void(uint64_t)>& f) {
-  const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-  const uint64_t per = 512;  // below this, one thread
-  const unsigned nt = (unsigned)std::min<uint64_t>(hw, (n + per - 1) / per);
-  if (nt <= 1) {
-    for (uint64_t k = 0; k < n; ++k) f(k);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < nt; ++t)
-    th.emplace_back([&, t]() {
-      const uint64_t a = n * t / nt, b = n * (t + 1) / nt;
-      for (uint64_t k = a; k < b; ++k) f(k);
-    });
-  for (auto& x : th) x.join();
 }
 
 void Engine::stage_event(int k) { DMR_HIP(hipEventRecord(ev_[k], st_)); }
@@ -491,13 +474,15 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   // Exact E-values of the candidates (glibc f64, as the reference), computed
   // in parallel; the selection below reads them in order.
   std::vector<double> h_ev(nmaybe, 0.0);
-  parallel_for(nmaybe, [&](uint64_t k) {
-    const SwJob& j = h_mjobs[k];
-    const int32_t sc = h_mouts[k].score;
-    if (sc <= 0) return;
-    const uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
-    const uint64_t slen = h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
-    h_ev[k] = alp_evalue(P, sc, qlen, slen, db_letters_);
+  host_parallel_for(nmaybe, 256, [&](uint64_t b, uint64_t e) {
+    for (uint64_t k = b; k < e; ++k) {
+      const SwJob& j = h_mjobs[k];
+      const int32_t sc = h_mouts[k].score;
+      if (sc <= 0) continue;
+      const uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
+      const uint64_t slen = h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
+      h_ev[k] = alp_evalue(P, sc, qlen, slen, db_letters_);
+    }
   });
   // Per group: bands in rank order; the first max_hsps whose exact E-value
   // passes are kept (src/search.rs:232-255).
