@@ -3,7 +3,8 @@
 // Radix sort: 8-bit digits, one (histogram, scan, scatter) triple per pass.
 // The scatter ranks keys stably inside a 256-thread tile with 64-lane ballot
 // matching (8 ballots give each lane the mask of lanes sharing its digit), so
-// every pass is stable and the final order equals a stable sort by key.
+// every pass is stable and the final order equals a stable sort by key; the
+// tile is staged in LDS by digit so its writes are coalesced.
 #include "dmr_device.hpp"
 
 namespace dmr {
@@ -114,51 +115,85 @@ __global__ void k_radix_hist(const uint64_t* keys, uint64_t n, int shift, uint32
   hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// One tile of kSortTile keys per block. Keys are ranked stably by digit in
+// tile order (k, wave, lane), which is the input order: per item, 8 ballots
+// give each lane the mask of lanes sharing its digit, and per-wave digit
+// counts order the waves. The tile is then laid out in LDS by digit and
+// written back in that order, so consecutive threads write consecutive
+// positions of one digit's run (coalesced), at the tile's global offset for
+// that digit (offs, from the scanned histograms).
 template <bool kValues>
-__global__ void k_radix_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
-                                uint32_t* vout, uint64_t n, int shift, const uint64_t* offs,
-                                uint32_t nblocks) {
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin,
+                                                                uint64_t* kout, uint32_t* vout, uint64_t n,
+                                                                int shift, const uint64_t* offs,
+                                                                uint32_t nblocks) {
   __shared__ uint32_t wave_hist[kSortWaves][256];
-  __shared__ uint64_t run[256];
+  __shared__ uint32_t count[256];       // keys of each digit ranked so far
+  __shared__ uint32_t local_base[256];  // first LDS slot of each digit
+  __shared__ uint64_t run[256];         // global position of the tile's first key of each digit
+  __shared__ uint64_t skey[kSortTile];
+  __shared__ uint32_t sval[kValues ? kSortTile : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+  count[threadIdx.x] = 0;
 #pragma unroll
   for (int w = 0; w < kSortWaves; ++w) wave_hist[w][threadIdx.x] = 0;
   __syncthreads();
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  uint64_t key[kSortItems];
+  uint32_t val[kSortItems], pos[kSortItems];
+#pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    uint64_t i = base + (uint64_t)k * kSortThreads + threadIdx.x;
-    bool valid = i < n;
-    uint64_t key = valid ? kin[i] : 0;
-    uint32_t val = (kValues && valid) ? vin[i] : 0;
-    uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const uint64_t i = base + (uint64_t)k * kSortThreads + threadIdx.x;
+    const bool valid = i < n;
+    key[k] = valid ? kin[i] : 0;
+    val[k] = (kValues && valid) ? vin[i] : 0;
+    const uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      uint64_t bb = __ballot((d >> b) & 1u);
+      const uint64_t bb = __ballot((d >> b) & 1u);
       m &= ((d >> b) & 1u) ? bb : ~bb;
     }
-    uint32_t rank = (uint32_t)__popcll(m & lt_mask);
+    const uint32_t rank = (uint32_t)__popcll(m & lt_mask);
     if (valid && rank == 0) wave_hist[wave][d] = (uint32_t)__popcll(m);
     __syncthreads();
-    if (valid) {
-      uint64_t pos = run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wave_hist[w][d];
-      kout[pos] = key;
-      if (kValues) vout[pos] = val;
-    }
+    uint32_t p = count[d] + rank;
+    for (int w = 0; w < wave; ++w) p += wave_hist[w][d];
+    pos[k] = p;
     __syncthreads();
     {
-      uint32_t s = 0;
+      uint32_t sum = 0;
 #pragma unroll
       for (int w = 0; w < kSortWaves; ++w) {
-        s += wave_hist[w][threadIdx.x];
+        sum += wave_hist[w][threadIdx.x];
         wave_hist[w][threadIdx.x] = 0;
       }
-      run[threadIdx.x] += s;
+      count[threadIdx.x] += sum;
     }
     __syncthreads();
+  }
+  uint64_t tile_n;
+  local_base[threadIdx.x] = (uint32_t)block_exclusive_scan(count[threadIdx.x], &tile_n);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    if (base + (uint64_t)k * kSortThreads + threadIdx.x >= n) continue;
+    const uint32_t slot = local_base[(uint32_t)(key[k] >> shift) & 255u] + pos[k];
+    skey[slot] = key[k];
+    if (kValues) sval[slot] = val[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t idx = (uint32_t)k * kSortThreads + threadIdx.x;
+    if (idx >= tile_n) break;
+    const uint64_t kv = skey[idx];
+    const uint32_t d = (uint32_t)(kv >> shift) & 255u;
+    const uint64_t g = run[d] + (idx - local_base[d]);
+    kout[g] = kv;
+    if (kValues) vout[g] = sval[idx];
   }
 }
 
