@@ -825,9 +825,8 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
         in_wave = 0;
       }
     }
-    std::fprintf(stderr, "[lanes] class %d (W %lld, %s): %u pairs, band cells %.3g, pair slots %.3g (x%.2f), "
-                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P == 1 ? "full" : (P == 2 ? "split2" : (P == 4 ? "split4" : "split16")),
-                 class_lanes[c],
+    std::fprintf(stderr, "[lanes] class %d (W %lld, %d lanes per pair): %u pairs, band cells %.3g, pair slots %.3g (x%.2f), "
+                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P, class_lanes[c],
                  (double)band, (double)pair_slots, (double)pair_slots / std::max<double>(1, band),
                  (double)wave_slots, (double)wave_slots / std::max<double>(1, band));
   }

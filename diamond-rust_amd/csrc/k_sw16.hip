@@ -763,17 +763,23 @@ static const int kWidths[kPackedClasses] = {25, 33, 61, 81, 129, 193, 257};
 // per-row work is spread over twice the offsets), four otherwise.
 template <int W>
 constexpr int kSplitLanes = W == 81 ? 2 : 4;
-int sw16_split_lanes(int cls) { return cls >= 5 ? 16 : (kWidths[cls] == 81 ? 2 : 4); }
+// Lanes per DP pair of the merged-band classes (width bound W).
+template <int W>
+constexpr int kMergedLanes = W == 257 ? 8 : 16;
+int sw16_split_lanes(int cls) {
+  return cls == 6 ? 8 : (cls == 5 ? 16 : (kWidths[cls] == 81 ? 2 : 4));
+}
 int sw16_width(int cls) { return kWidths[cls]; }
 
 template <int W, class Tr>
 void launch_width(bool split, unsigned tasks, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
                   SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
                   SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
-  if constexpr (W > 129) {  // merged bands: groups of 16 lanes, per-DP walls
-    static const unsigned resident = resident_blocks(k_sw16p<W, 16, 512, Tr, true>, 512, cu_count);
+  if constexpr (W > 129) {  // merged bands: groups of kMergedLanes<W> lanes, per-DP walls
+    constexpr int P = kMergedLanes<W>;
+    static const unsigned resident = resident_blocks(k_sw16p<W, P, 512, Tr, true>, 512, cu_count);
     const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));
-    hipLaunchKernelGGL((k_sw16p<W, 16, 512, Tr, true>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
+    hipLaunchKernelGGL((k_sw16p<W, P, 512, Tr, true>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
                        jobs, q, db, pair_tab, counter, outs, redo, nredo);
   } else {
     if (split) {
@@ -796,7 +802,7 @@ void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nla
                  SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!nlanes) return;
   // A wave carries 64 DP pairs (one per lane) or, split, 64 / P (P lanes
-  // each); the merged-band classes carry 4 (a row of 16 lanes each).
+  // each; the merged-band classes always split).
   const unsigned tasks = nblk(nlanes, 64 / (cls >= 5 || split ? sw16_split_lanes(cls) : 1));
 #define DMR_SW16(c, w)                                                                             \
   case c:                                                                                          \

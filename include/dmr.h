@@ -220,7 +220,7 @@ int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
  * milliseconds. Diagnostics only; no counterpart in the reference. */
 #define DMR_SCORE_CLASSES 14
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
-/* Layout of the packed classes 0-4 on ctx (5-6 always use sixteen lanes):
+/* Layout of the packed classes 0-4 on ctx (5-6 always split, 16/8 lanes):
  * bit c of *split_mask set = class c runs split over several lanes per DP
  * pair (k_sw16p: two for width 81, four otherwise), clear = one lane
  * (k_sw16); *f16 = 1 when the packed kernels compute in f16 pairs, 0 in

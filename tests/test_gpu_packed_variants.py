@@ -1,5 +1,5 @@
 """The packed 16-bit score pass in both layouts and both arithmetics against
-the oracle, and the merged-band classes (sixteen lanes per pair, per-DP walls).
+the oracle, and the merged-band classes (16 or 8 lanes per pair, per-DP walls).
 
 Each width class runs either with one lane per DP pair (the whole band in a
 lane's registers) or split over two or four lanes (part bands, with the
@@ -79,7 +79,7 @@ def test_scores_around_f16_limit(layout_ctx, orc):
 
 
 def test_merged_band_widths(layout_ctx, orc):
-    """Bands of widths 130..256 (merged bands) run the sixteen-lane kernels
+    """Bands of widths 130..256 (merged bands) run the split kernels
     with a wall at each DP's own width (classes m193/m256); 129 and 257+ stay
     on their own kernels. Every width near a class edge is included."""
     rng = np.random.default_rng(91)
