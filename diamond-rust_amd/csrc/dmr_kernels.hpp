@@ -42,7 +42,20 @@ void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t
 struct SeqSet {
   const uint8_t* res;
   const uint32_t* offs;  // n+1
+  // Padded letters for the packed score kernels (null where unused): letter
+  // codes times 4, sequence k starting at pad[pad_offs[k]], with the padding
+  // code (25 * 4) at pad[pad_offs[k] - 1] and pad[pad_offs[k] + len_k], and
+  // at pad[0] and pad[1] (base 1 serves as an empty sequence).
+  const uint8_t* pad = nullptr;
+  const uint32_t* pad_offs = nullptr;
 };
+
+// Bytes of the padded letters of n sequences of `total` letters laid out one
+// after another: two padding codes, then each sequence followed by one.
+inline uint64_t pad_letters_bytes(uint64_t n, uint64_t total) { return total + n + 2; }
+// Builds that layout from s.res/s.offs (offs non-decreasing from 0): pad_offs[k]
+// = offs[k] + k + 2.
+void launch_pad_letters(SeqSet s, uint32_t n, uint8_t* pad, uint32_t* pad_offs, hipStream_t st);
 
 // Key flags over packed hits: 1 where (hit >> shift) changes.
 void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st);

@@ -141,7 +141,7 @@ AlpParams Engine::alp(int go, int ge) const {
 void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n) {
   DMR_HIP(hipSetDevice(device_));
   uint64_t total = offsets[n];
-  if (total >= (1ull << 32)) throw Error(-5, "database larger than 4G letters");
+  if (total + n + 2 >= (1ull << 32)) throw Error(-5, "database larger than 4G letters");
   h_db_offs_.resize(n + 1);
   uint32_t maxl = 0;
   std::vector<uint32_t> seq_of(total);
@@ -159,6 +159,10 @@ void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n
                     hipMemcpyHostToDevice));
   DMR_HIP(hipMemcpy(db_seq_of_.ensure<uint32_t>(total + 1), seq_of.data(), total * 4,
                     hipMemcpyHostToDevice));
+  launch_pad_letters(SeqSet{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()}, n,
+                     db_pad_.ensure<uint8_t>(pad_letters_bytes(n, total)), db_pad_offs_.ensure<uint32_t>(n + 1),
+                     st_);
+  DMR_HIP(hipStreamSynchronize(st_));
   db_n_ = n;
   db_letters_ = total;
   db_max_len_ = maxl;
@@ -276,6 +280,7 @@ void Engine::load_targets(const TargetSet& t) {
   tset_ = t;
   q_n_ = (uint32_t)(t.offs.size() - 1);
   q_total_ = t.offs.back();
+  if ((uint64_t)q_total_ + q_n_ + 2 >= (1ull << 32)) throw Error(-5, "queries larger than 4G letters");
   q_max_len_ = 0;
   std::vector<uint32_t> seq_of(q_total_ + 1);
   for (uint32_t i = 0; i < q_n_; ++i) {
@@ -288,6 +293,9 @@ void Engine::load_targets(const TargetSet& t) {
                          hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(q_seq_of_.ensure<uint32_t>(q_total_ + 1), seq_of.data(), q_total_ * 4,
                          hipMemcpyHostToDevice, st_));
+  launch_pad_letters(SeqSet{q_res_.as<uint8_t>(), q_offs_.as<uint32_t>()}, q_n_,
+                     q_pad_.ensure<uint8_t>(pad_letters_bytes(q_n_, q_total_)),
+                     q_pad_offs_.ensure<uint32_t>(q_n_ + 1), st_);
   DMR_HIP(hipStreamSynchronize(st_));
 }
 
@@ -306,8 +314,9 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   const int xdrop = raw_score(P, 12.3);  // --xdrop 12.3 bits (src/search.rs:51-53)
   const int cutoff = 19;                  // src/search.rs:164
   const int total_bits = key_bits_ + shape_bits_;
-  SeqSet qs{q_res_.as<uint8_t>(), q_offs_.as<uint32_t>()};
-  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
+  SeqSet qs{q_res_.as<uint8_t>(), q_offs_.as<uint32_t>(), q_pad_.as<uint8_t>(), q_pad_offs_.as<uint32_t>()};
+  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>(), db_pad_.as<uint8_t>(),
+             db_pad_offs_.as<uint32_t>()};
   stage_event(0);
 
   // ---- query seeds (all shapes, tagged, sorted) ---------------------------
@@ -592,6 +601,8 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   if (!njobs) return;
   last_target_bits_ = target_bits;
   const bool fast = opt.gap_open == 11 && opt.gap_extend == 1 && !force_generic_;
+  if (fast && (!qs.pad || !qs.pad_offs || !dbs.pad || !dbs.pad_offs))
+    throw Error(-2, "score pass without padded letters");
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
   launch_classify(jobs, njobs, qs, dbs, target_bits, keys, idx, st_);
@@ -1333,12 +1344,40 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
                   (uint32_t)i, 0};
   }
   DevBuf dq, ds, dqo, dso, djobs, dst, dsoff, dscr, dcells, dw0, dtoff, dtr, doo, dops, dout;
+  DevBuf dqp, dsp, dqpo, dspo;
   DMR_HIP(hipMemcpyAsync(dq.ensure<uint8_t>(qbuf_len + 64), qbuf, qbuf_len, hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(ds.ensure<uint8_t>(sbuf_len + 64), sbuf, sbuf_len, hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(dqo.ensure<uint32_t>(2 * njobs), qo.data(), 8 * njobs, hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(dso.ensure<uint32_t>(2 * njobs), so.data(), 8 * njobs, hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(djobs.ensure<SwJob>(njobs), dj.data(), njobs * sizeof(SwJob), hipMemcpyHostToDevice, st_));
   SeqSet qs{dq.as<uint8_t>(), dqo.as<uint32_t>()}, ss{ds.as<uint8_t>(), dso.as<uint32_t>()};
+  if (!traceback) {
+    // Padded letters (times 4) of each job's query and subject for the packed
+    // score kernels: entry 2i of the bases points at job i's copy, entry
+    // 2i + 1 at the empty sequence at base 1.
+    std::vector<uint8_t> qpad(2, 4 * 25), spad(2, 4 * 25);
+    std::vector<uint32_t> qpo(2 * njobs, 1u), spo(2 * njobs, 1u);
+    auto append = [](std::vector<uint8_t>* pad, const uint8_t* src, uint64_t len) {
+      const uint64_t base = pad->size();
+      pad->resize(base + len + 1);
+      for (uint64_t k = 0; k < len; ++k) (*pad)[base + k] = (uint8_t)(4 * src[k]);
+      (*pad)[base + len] = 4 * 25;
+      return base;
+    };
+    for (uint64_t i = 0; i < njobs; ++i) {
+      qpo[2 * i] = (uint32_t)append(&qpad, qbuf + jobs[i].q_off, jobs[i].qlen);
+      spo[2 * i] = (uint32_t)append(&spad, sbuf + jobs[i].s_off, jobs[i].slen);
+    }
+    if (qpad.size() >= (1ull << 32) || spad.size() >= (1ull << 32)) throw Error(-5, "jobs larger than 4G letters");
+    DMR_HIP(hipMemcpyAsync(dqp.ensure<uint8_t>(qpad.size()), qpad.data(), qpad.size(), hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dsp.ensure<uint8_t>(spad.size()), spad.data(), spad.size(), hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dqpo.ensure<uint32_t>(2 * njobs), qpo.data(), 8 * njobs, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dspo.ensure<uint32_t>(2 * njobs), spo.data(), 8 * njobs, hipMemcpyHostToDevice, st_));
+    qs.pad = dqp.as<uint8_t>();
+    qs.pad_offs = dqpo.as<uint32_t>();
+    ss.pad = dsp.as<uint8_t>();
+    ss.pad_offs = dspo.as<uint32_t>();
+  }
   uint64_t* cells = dcells.ensure<uint64_t>(njobs);
   uint64_t* w0 = dw0.ensure<uint64_t>(njobs);
   uint32_t* stv = dst.ensure<uint32_t>(njobs);

@@ -242,6 +242,7 @@ class Engine {
 
   // database
   DevBuf db_res_, db_offs_, db_seq_of_;
+  DevBuf db_pad_, db_pad_offs_;  // padded letters of the packed score kernels
   uint32_t db_n_ = 0;
   uint64_t db_letters_ = 0;
   uint32_t db_max_len_ = 0;
@@ -265,6 +266,7 @@ class Engine {
   // targets
   TargetSet tset_;
   DevBuf q_res_, q_offs_, q_seq_of_;
+  DevBuf q_pad_, q_pad_offs_;
   uint32_t q_n_ = 0, q_max_len_ = 0;
   uint64_t q_total_ = 0;
 

@@ -73,6 +73,9 @@ struct I16 {
   static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
   static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_max(a, b); }
   static __device__ __forceinline__ V max3(V a, V b, V c) { return max(max(a, b), c); }
+  static __device__ __forceinline__ V min(V a, V b) { return __builtin_elementwise_min(a, b); }
+  typedef short S;  // one half
+  static __device__ __forceinline__ S hmax(V v) { return v.x > v.y ? v.x : v.y; }
   static __device__ __forceinline__ int lo(V v) { return v.x; }
   static __device__ __forceinline__ int hi(V v) { return v.y; }
 };
@@ -87,6 +90,9 @@ struct F16 {
   static __device__ __forceinline__ V sub(V a, V b) { return a - b; }
   static __device__ __forceinline__ V max(V a, V b) { return __builtin_elementwise_maximum(a, b); }
   static __device__ __forceinline__ V max3(V a, V b, V c) { return max(max(a, b), c); }
+  static __device__ __forceinline__ V min(V a, V b) { return __builtin_elementwise_minimum(a, b); }
+  typedef _Float16 S;  // one half
+  static __device__ __forceinline__ S hmax(V v) { return __builtin_elementwise_maximum(v.x, v.y); }
   static __device__ __forceinline__ int lo(V v) { return (int)v.x; }
   static __device__ __forceinline__ int hi(V v) { return (int)v.y; }
 };
@@ -102,11 +108,37 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Letter p of the sequence at res + off (length m), padding outside 0..m-1.
+// Letter (times 4) at position p of a sequence of length m starting at
+// pad[base] in a padded letter set: positions outside 0..m-1 clamp to the
+// padding code on either side, so one clamp replaces the range tests.
 // Sequences are addressed by 32-bit offsets from the kernel-argument base,
 // which keeps per-lane state to one register per sequence.
-__device__ __forceinline__ int letter_at(const uint8_t* res, uint32_t off, int m, int p) {
-  return (p >= 0 && p < m) ? (int)res[off + (uint32_t)p] : 25;
+__device__ __forceinline__ int pad_at(const uint8_t* pad, uint32_t base, int m, int p) {
+  return (int)pad[base + (uint32_t)min(max(p, -1), m)];
+}
+// Byte offset of the score-table row of a query letter (times 4). The empty
+// asm makes the value opaque, so each cell's table address stays one add of
+// a window half to it (the compiler would otherwise fold the multiply into a
+// multiply-add per cell, after extracting the half).
+__device__ __forceinline__ uint32_t row_base(int q4) {
+  uint32_t b = __umul24((uint32_t)q4, (uint32_t)kPairRow);
+  asm("" : "+v"(b));
+  return b;
+}
+// Byte offset of the letter pair (a, b) in a score-table row, from the
+// letters times 4: 4 * (a * kCodes + b), one 24-bit multiply-add.
+__device__ __forceinline__ uint32_t pair_off4(int a4, int b4) {
+  static_assert(kCodes == 26, "the multiplier is written out below");
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, 26, %2" : "=v"(r) : "v"(a4), "v"(b4));
+  return r;
+}
+// A loaded letter, settled at this point: the load completes here (at the
+// end of the row that issued it) instead of being moved to its use in the
+// next row, so no row waits on memory.
+__device__ __forceinline__ int settle(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
 }
 
 template <int W, class Tr>
@@ -140,11 +172,12 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     SwJob ja = {}, jb = {};
     if (hasA) ja = jobs[ln.x];
     if (hasB) jb = jobs[ln.y];
-    const uint32_t qo = hasA ? q.offs[ja.target] : 0;
-    const int n = hasA ? (int)(q.offs[ja.target + 1] - qo) : 0;
-    const uint32_t ao = hasA ? db.offs[ja.subject] : 0, bo = hasB ? db.offs[jb.subject] : 0;
-    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - ao) : 0;
-    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - bo) : 0;
+    const int n = hasA ? (int)(q.offs[ja.target + 1] - q.offs[ja.target]) : 0;
+    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - db.offs[ja.subject]) : 0;
+    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - db.offs[jb.subject]) : 0;
+    // Padded-letter bases (base 1: an empty sequence for an absent DP).
+    const uint32_t qp = hasA ? q.pad_offs[ja.target] : 1u;
+    const uint32_t ap = hasA ? db.pad_offs[ja.subject] : 1u, bp = hasB ? db.pad_offs[jb.subject] : 1u;
     const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
     const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
     // Rows of this lane: the union of its two DPs' row ranges, rows 1..m+d1
@@ -178,31 +211,27 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-      uint32_t lo16 = 4u * (letter_at(db.res, ao, mA, lo - d1A + 2 * k - 1) * kCodes +
-                            letter_at(db.res, bo, mB, lo - d1B + 2 * k - 1));
-      uint32_t hi16 = 2 * k + 1 < W ? 4u * (letter_at(db.res, ao, mA, lo - d1A + 2 * k) * kCodes +
-                                            letter_at(db.res, bo, mB, lo - d1B + 2 * k))
+      uint32_t lo16 = pair_off4(pad_at(db.pad, ap, mA, lo - d1A + 2 * k - 1),
+                                pad_at(db.pad, bp, mB, lo - d1B + 2 * k - 1));
+      uint32_t hi16 = 2 * k + 1 < W ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + 2 * k),
+                                                pad_at(db.pad, bp, mB, lo - d1B + 2 * k))
                                     : 0u;
       R[k] = lo16 | (hi16 << 16);
     }
     // Per-row letters: the query letter of row i and the letter pair entering
     // the window at t = W-1 (subject positions i - d0 - 1).
+    // Query letter of row i (padding past n), times 4.
+    auto qletter = [&](int i) { return (int)q.pad[qp + (uint32_t)min(i - 1, n)]; };
     auto pair_at = [&](int i) {
-      return (uint32_t)(4 * (letter_at(db.res, ao, mA, i - d0A - 1) * kCodes +
-                             letter_at(db.res, bo, mB, i - d0B - 1)));
+      return pair_off4(pad_at(db.pad, ap, mA, i - d0A - 1), pad_at(db.pad, bp, mB, i - d0B - 1));
     };
-    // Widths up to 81 load them one row ahead with clamped, always in-bounds
-    // addresses; the raw bytes are only read by a VALU copy at the end of the
-    // row (padding is selected after it), so the loads are unconditional and
-    // no row waits on memory. W = 129 has no registers to spare for this (it
-    // would spill to scratch) and loads them at the start of each row.
+    // Widths up to 81 load them one row ahead (the clamped addresses are
+    // always in bounds); the raw bytes are only read by a VALU copy at the
+    // end of the row, so the loads are unconditional and no row waits on
+    // memory. W = 129 has no registers to spare for this (it would spill to
+    // scratch) and loads them at the start of each row.
     constexpr bool kPrefetch = W < 129;
-    auto qraw = [&](int i) { return (int)q.res[qo + (uint32_t)(max(min(i, n), 1) - 1)]; };
-    auto sraw = [&](uint32_t off, int m, int p) {
-      return (int)db.res[off + (uint32_t)min(max(p, 0), max(m - 1, 0))];
-    };
-    auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
-    int qcur = lo <= n ? qraw(lo) : 25;
+    int qcur = qletter(lo);
     uint32_t pnext = 0;
     V best = ZERO2;  // in the frame of the current row
     V Zr = ZERO2;    // frame offset of the current row
@@ -216,17 +245,17 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         if (kPrefetch) {
           R[NR - 1] = pnext;
         } else {
-          qcur = i <= n ? (int)q.res[qo + (uint32_t)(i - 1)] : 25;
+          qcur = qletter(i);
           R[NR - 1] = pair_at(i);
         }
       }
       int rq = 0, ra = 0, rb = 0;
       if (kPrefetch) {  // raw bytes of row i + 1
-        rq = qraw(i + 1);
-        ra = sraw(ao, mA, i - d0A);
-        rb = sraw(bo, mB, i - d0B);
+        rq = qletter(i + 1);
+        ra = pad_at(db.pad, ap, mA, i - d0A);
+        rb = pad_at(db.pad, bp, mB, i - d0B);
       }
-      const char* Tq = reinterpret_cast<const char*>(T) + qcur * (4 * kPairRow);
+      const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
       auto score = [&](int t) {
         const uint32_t off = (t & 1) ? (R[t >> 1] >> 16) : (R[t >> 1] & 0xFFFFu);
         return *reinterpret_cast<const int*>(Tq + off);
@@ -257,12 +286,9 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         E[t] = Ev;
         F = Fv;
       }
-      if (kPrefetch) {  // letters of row i + 1, padding selected as in letter_at
-        const int cq = copy(rq), ca = copy(ra), cb = copy(rb);
-        qcur = i + 1 <= n ? cq : 25;
-        const int la = (i - d0A >= 0 && i - d0A < mA) ? ca : 25;
-        const int lb = (i - d0B >= 0 && i - d0B < mB) ? cb : 25;
-        pnext = (uint32_t)(4 * (la * kCodes + lb));
+      if (kPrefetch) {  // letters of row i + 1
+        qcur = settle(rq);
+        pnext = pair_off4(settle(ra), settle(rb));
       }
       // Frame of the next row; every kFrame rows the frame moves back.
       best = Tr::add(best, ONE2);
@@ -304,7 +330,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
 // Split-band variant: each DP pair is carried by P lanes (P = 2 or 4, one
 // lane group), lane p of the group holding band offsets p*H .. p*H+H-1 with
 // H = ceil((W + 1) / P); offsets >= W are the wall right of the band (and
-// dead padding past it) and stay NEG. A lane keeps H instead of W offsets, so
+// dead padding past it) and stay far below any score. A lane keeps H instead of W offsets, so
 // the state fits far fewer registers and more waves share a SIMD.
 //
 // Within a row the halves of the band depend on each other only through the
@@ -384,7 +410,6 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   using SwapC = std::integral_constant<int, 0xB1>;    // [1,0,3,2]
   using Swap2C = std::integral_constant<int, 0x4E>;   // [2,3,0,1]
   using MirrorC = std::integral_constant<int, 0x141>; // row_half_mirror
-  auto copy = [](int v) { return __builtin_amdgcn_update_dpp(0, v, 0xE4, 0xf, 0xf, false); };
   for (;;) {
     const uint32_t task =
         (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
@@ -396,11 +421,12 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     SwJob ja = {}, jb = {};
     if (hasA) ja = jobs[ln.x];
     if (hasB) jb = jobs[ln.y];
-    const uint32_t qo = hasA ? q.offs[ja.target] : 0;
-    const int n = hasA ? (int)(q.offs[ja.target + 1] - qo) : 0;
-    const uint32_t ao = hasA ? db.offs[ja.subject] : 0, bo = hasB ? db.offs[jb.subject] : 0;
-    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - ao) : 0;
-    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - bo) : 0;
+    const int n = hasA ? (int)(q.offs[ja.target + 1] - q.offs[ja.target]) : 0;
+    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - db.offs[ja.subject]) : 0;
+    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - db.offs[jb.subject]) : 0;
+    // Padded-letter bases (base 1: an empty sequence for an absent DP).
+    const uint32_t qp = hasA ? q.pad_offs[ja.target] : 1u;
+    const uint32_t ap = hasA ? db.pad_offs[ja.subject] : 1u, bp = hasB ? db.pad_offs[jb.subject] : 1u;
     const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
     const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
     // Rows as in k_sw16 (all lanes of a group have the same range).
@@ -423,6 +449,13 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
     // kWallOff past the DP's own width), as two halves.
     V GO[kWall ? H : 1];
     const int liveA = kWall ? (d1A - d0A + 1) - t0 : 0, liveB = kWall ? (d1B - d0B + 1) - t0 : 0;
+    const V LIVE2 = Tr::pair(liveA, liveB);
+    // Without kWall the last lane's offsets from kDeadFrom on lie past the
+    // band: their G takes the addend kWallOff instead of 1 - o, which keeps
+    // them (and the E they feed) far below any score, as a per-DP wall does;
+    // DADJ moves the F_in correction there by the same amount.
+    const V GOD = last ? Tr::splat(kWallOff) : GOFF;
+    const V DADJ = last ? Tr::splat(kWallOff - (1 - kOpen)) : ZERO2;
 #pragma unroll
     for (int u = 0; u <= H; ++u) {
       if (kWall && u < H) {
@@ -437,21 +470,17 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int t = t0 + 2 * k;
-      uint32_t lo16 = 4u * (letter_at(db.res, ao, mA, lo - d1A + t - 1) * kCodes +
-                            letter_at(db.res, bo, mB, lo - d1B + t - 1));
-      uint32_t hi16 = 2 * k + 1 < H ? 4u * (letter_at(db.res, ao, mA, lo - d1A + t) * kCodes +
-                                            letter_at(db.res, bo, mB, lo - d1B + t))
+      uint32_t lo16 = pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t - 1),
+                                pad_at(db.pad, bp, mB, lo - d1B + t - 1));
+      uint32_t hi16 = 2 * k + 1 < H ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t),
+                                                pad_at(db.pad, bp, mB, lo - d1B + t))
                                     : 0u;
       R[k] = lo16 | (hi16 << 16);
     }
     // Letters of the next row, loaded one row ahead (clamped, unconditional
-    // loads; padding selected after a VALU copy at the end of the row): the
-    // query letter and the pair entering the last lane at offset P*H-1.
-    auto qraw = [&](int i) { return (int)q.res[qo + (uint32_t)(max(min(i, n), 1) - 1)]; };
-    auto sraw = [&](uint32_t off, int m, int pos) {
-      return (int)db.res[off + (uint32_t)min(max(pos, 0), max(m - 1, 0))];
-    };
-    int qcur = lo <= n ? qraw(lo) : 25;
+    // loads, read by a VALU copy at the end of the row): the query letter and
+    // the pair entering the last lane at offset P*H-1.
+    int qcur = (int)q.pad[qp + (uint32_t)min(lo - 1, n)];
     uint32_t pnext = 0;
     V best = ZERO2;  // in the frame of the current row
     V Zr = ZERO2;    // frame offset of the current row
@@ -469,14 +498,15 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       }
       // Raw letters of row i + 1: subject positions i + 1 - d1 + P*H - 2.
       const int pos = i - 1 + P * H;
-      const int rq = qraw(i + 1), ra = sraw(ao, mA, pos - d1A), rb = sraw(bo, mB, pos - d1B);
+      const int rq = (int)q.pad[qp + (uint32_t)min(i, n)];
+      const int ra = pad_at(db.pad, ap, mA, pos - d1A), rb = pad_at(db.pad, bp, mB, pos - d1B);
       {
         // E source past the lane: the next lane's first offset, previous row.
         const V g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
         G[H] = last ? NEG2 : g0;
         E[H] = last ? NEG2 : e0;
       }
-      const char* Tq = reinterpret_cast<const char*>(T) + qcur * (4 * kPairRow);
+      const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
       auto score = [&](int u) {
         const uint32_t off = (u & 1) ? (R[u >> 1] >> 16) : (R[u >> 1] & 0xFFFFu);
         return *reinterpret_cast<const int*>(Tq + off);
@@ -499,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
         const V Hh = Tr::max(X0, Fv);
         best = Tr::max(best, Hh);
-        Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : GOFF);
+        Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : (u >= kDeadFrom ? GOD : GOFF));
         G[u] = Gl;
         E[u] = Ev;
         F = Fv;
@@ -535,33 +565,24 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       // (with walls: inside its DPs' bands).
       {
         const V fu = Tr::sub(fin, Zr);
-        const int fl = kWall ? max(min(Tr::lo(fu), liveA), min(Tr::hi(fu), liveB))
-                             : max(Tr::lo(fu), Tr::hi(fu));
+        using S = typename Tr::S;
+        const S fl = Tr::hmax(kWall ? Tr::min(fu, LIVE2) : fu);
         V fc = kWall ? fin : Tr::add(fin, GOFF);
         constexpr int kChunk = 8;
 #pragma unroll
         for (int c = 0; c < H; c += kChunk) {
-          if (__ballot(fl > c) == 0) break;
+          if (__ballot(fl > (S)c) == 0) break;
 #pragma unroll
           for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
-            G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0]) : fc);
+            G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0])
+                                       : (u >= kDeadFrom ? Tr::add(fc, DADJ) : fc));
             fc = Tr::sub(fc, ONE2);
           }
         }
       }
-      if (last && !kWall) {
-#pragma unroll
-        for (int u = kDeadFrom; u < H; ++u) {
-          G[u] = NEG2;
-          E[u] = NEG2;
-        }
-      }
-      // Letters of row i + 1, padding selected as in letter_at.
-      const int cq = copy(rq), ca = copy(ra), cb = copy(rb);
-      qcur = i + 1 <= n ? cq : 25;
-      const int la = (pos - d1A >= 0 && pos - d1A < mA) ? ca : 25;
-      const int lb = (pos - d1B >= 0 && pos - d1B < mB) ? cb : 25;
-      pnext = (uint32_t)(4 * (la * kCodes + lb));
+      // Letters of row i + 1.
+      qcur = settle(rq);
+      pnext = pair_off4(settle(ra), settle(rb));
       // Frame of the next row; every kFrame rows the frame moves back.
       best = Tr::add(best, ONE2);
       Zr = Tr::add(Zr, ONE2);
@@ -699,6 +720,20 @@ __global__ void k_gather_lanes(const uint2* __restrict__ in, const uint32_t* __r
   if (L < n) out[L] = in[idx[L]];
 }
 
+// One block per sequence: its letters times 4 at offs[k] + k + 2 and the
+// padding code before it (and after the last).
+__global__ void k_pad_letters(SeqSet s, uint32_t n, uint8_t* __restrict__ pad, uint32_t* __restrict__ pad_offs) {
+  const uint32_t k = blockIdx.x;
+  const uint32_t b = s.offs[k], len = s.offs[k + 1] - b, base = b + k + 2;
+  for (uint32_t p = threadIdx.x; p < len; p += blockDim.x) pad[base + p] = (uint8_t)(4 * s.res[b + p]);
+  if (threadIdx.x == 0) {
+    pad_offs[k] = base;
+    pad[base - 1] = 4 * 25;
+    if (k == 0) pad[0] = pad[1] = 4 * 25;
+    if (k + 1 == n) pad[base + len] = 4 * 25;
+  }
+}
+
 inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -755,6 +790,11 @@ void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, Se
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st) {
   if (!n) return;
   hipLaunchKernelGGL(k_gather_lanes, dim3(nblk(n, 256)), dim3(256), 0, st, in, idx, n, out);
+}
+
+void launch_pad_letters(SeqSet s, uint32_t n, uint8_t* pad, uint32_t* pad_offs, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pad_letters, dim3(n), dim3(256), 0, st, s, n, pad, pad_offs);
 }
 
 static const int kWidths[kPackedClasses] = {25, 33, 61, 81, 129, 193, 257};
