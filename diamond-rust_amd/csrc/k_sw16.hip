@@ -12,11 +12,14 @@
 //
 // Row frames: the values of row step r are held plus r (G plus r + 1), so a
 // vertical move gains exactly the gap extension and E needs no decrement:
-//   E^(r,t) = max(G^(r-1,t+1), E^(r-1,t+1)),  D^ = G^(r-1,t) + s + o,
-//   F^(r,t) = max(G^(r,t-1), F^(r,t-1)) - 1,  H^ = max(r, D^, E^, F^),
+//   E^(r,t) = max(r, G^(r-1,t+1), E^(r-1,t+1)),  D^ = G^(r-1,t) + s + o,
+//   F^(r,t) = max(G^(r,t-1), F^(r,t-1)) - 1,     H^ = max(D^, E^, F^),
 //   G^ = H^ - o + 1,
-// with the zero of the local alignment at r. Every kFrame rows the frame is
-// moved back by kFrame (all carried values and r), which bounds the offset.
+// with the zero of the local alignment at r. The zero rides on E: carrying
+// E' = max(E, 0) instead of E changes no H (H = max(0, D, E, F) =
+// max(D, E', F)) and no later E' (max(E', 0) - e <= max(E - e, 0)), and
+// makes H one three-way max. Every kFrame rows the frame is moved back by
+// kFrame (all carried values and r), which bounds the offset.
 //
 // Exactness against the row-range rules of the reference:
 // - Columns j <= 0 and rows above the band read a padding letter scoring -64:
@@ -274,12 +277,11 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
         const V sc = Tr::bits(sq[t % kAhead]);
         if (t + kAhead < W) sq[t % kAhead] = score(t + kAhead);
         __builtin_amdgcn_sched_barrier(0);
-        // max(D, E, 0) is off the F chain that runs along the row.
+        // D and E (with the zero) are off the F chain that runs along the row.
         const V D = Tr::add(G[t], sc);
-        const V Ev = Tr::max(G[t + 1], E[t + 1]);
-        const V X0 = Tr::max3(D, Ev, Zr);
+        const V Ev = Tr::max3(G[t + 1], E[t + 1], Zr);
         const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
-        const V H = Tr::max(X0, Fv);
+        const V H = Tr::max3(D, Ev, Fv);
         best = Tr::max(best, H);
         Gl = Tr::add(H, GOFF);
         G[t] = Gl;
@@ -395,11 +397,15 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
   // lane (quad_perm [1,1,3,3] / [0,0,2,2] for pairs, [1,2,3,3] / [0,0,1,2]
   // for groups of 4, row_shl:1 / row_shr:1 for groups of 8 and 16; a lane
   // whose source lies outside its group receives a value it does not use),
-  // and the swaps of the final reduction.
+  // and the swaps of the final reduction. mv leaves a lane without source
+  // undefined (no copy of the old value is needed); mvk keeps its own value.
   constexpr int kFromNext = P == 2 ? 0xF5 : (P == 4 ? 0xF9 : 0x101);
   constexpr int kFromPrev = P == 2 ? 0xA0 : (P == 4 ? 0x90 : 0x111);
-  auto dppi = [](int v, auto ctrl) { return __builtin_amdgcn_update_dpp(v, v, decltype(ctrl)::value, 0xf, 0xf, false); };
+  auto dppi = [](int v, auto ctrl) { return __builtin_amdgcn_mov_dpp(v, decltype(ctrl)::value, 0xf, 0xf, false); };
   auto mv = [&](V v, auto ctrl) { return Tr::bits(dppi(Tr::raw(v), ctrl)); };
+  auto mvk = [](V v, auto ctrl) {
+    return Tr::bits(__builtin_amdgcn_update_dpp(Tr::raw(v), Tr::raw(v), decltype(ctrl)::value, 0xf, 0xf, false));
+  };
   using NextC = std::integral_constant<int, kFromNext>;
   using PrevC = std::integral_constant<int, kFromPrev>;
   using Back2C = std::integral_constant<int, 0x40>;   // [0,0,0,1]
@@ -521,13 +527,12 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         const V sc = Tr::bits(sq[u % kAhead]);
         if (u + kAhead < H) sq[u % kAhead] = score(u + kAhead);
         __builtin_amdgcn_sched_barrier(0);
-        // max(D, E, 0) does not depend on F: only max(., F), the F
-        // decrement and the G offset sit on the chain along the row.
+        // D and E (with the zero) do not depend on F: only max(Gl, F), the
+        // F decrement, the max3 and the G offset sit on the chain along the row.
         const V D = Tr::add(G[u], sc);
-        const V Ev = Tr::max(G[u + 1], E[u + 1]);
-        const V X0 = Tr::max3(D, Ev, Zr);
+        const V Ev = Tr::max3(G[u + 1], E[u + 1], Zr);
         const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
-        const V Hh = Tr::max(X0, Fv);
+        const V Hh = Tr::max3(D, Ev, Fv);
         best = Tr::max(best, Hh);
         Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : (u >= kDeadFrom ? GOD : GOFF));
         G[u] = Gl;
@@ -551,10 +556,10 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         const V s4 = mv(y, Shr4C{});
         y = p >= 4 ? Tr::max(y, s4) : y;
       } else {  // the group is a whole row: a lane without source keeps its own
-        y = Tr::max(y, mv(y, PrevC{}));
-        y = Tr::max(y, mv(y, Shr2C{}));
-        y = Tr::max(y, mv(y, Shr4C{}));
-        y = Tr::max(y, mv(y, Shr8C{}));
+        y = Tr::max(y, mvk(y, PrevC{}));
+        y = Tr::max(y, mvk(y, Shr2C{}));
+        y = Tr::max(y, mvk(y, Shr4C{}));
+        y = Tr::max(y, mvk(y, Shr8C{}));
       }
       V fin = Tr::add(mv(y, PrevC{}), Tr::splat((P - p) * H));
       if (p == 0) fin = NEG2;
