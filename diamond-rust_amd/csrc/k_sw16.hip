@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
 template <int W, int P>
 struct SplitShape {
   static constexpr int H = (W + 1 + P - 1) / P;  // offsets per lane
-  static constexpr int NR = (H + 1) / 2;         // window registers
+  static constexpr int NR = H / 2 + 1;           // window registers (entries 0..H)
   static constexpr int kDeadFrom = W - (P - 1) * H;  // first dead offset of the last lane
 };
 
@@ -476,32 +476,28 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int t = t0 + 2 * k;
-      uint32_t lo16 = pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t - 1),
-                                pad_at(db.pad, bp, mB, lo - d1B + t - 1));
+      uint32_t lo16 = 2 * k < H ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t - 1),
+                                            pad_at(db.pad, bp, mB, lo - d1B + t - 1))
+                                : 0u;
       uint32_t hi16 = 2 * k + 1 < H ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t),
                                                 pad_at(db.pad, bp, mB, lo - d1B + t))
                                     : 0u;
       R[k] = lo16 | (hi16 << 16);
     }
     // Letters of the next row, loaded one row ahead (clamped, unconditional
-    // loads, read by a VALU copy at the end of the row): the query letter and
-    // the pair entering the last lane at offset P*H-1.
+    // loads, settled at the end of the row): the query letter and the pair
+    // entering the last lane at offset P*H-1.
     int qcur = (int)q.pad[qp + (uint32_t)min(lo - 1, n)];
     uint32_t pnext = 0;
     V best = ZERO2;  // in the frame of the current row
     V Zr = ZERO2;    // frame offset of the current row
-    for (int r = 0; r < steps; ++r) {
+    // One row step. The window holds the entries of an even row (entry e at
+    // R[e / 2], half e % 2) plus, at entry H, the one entering the next row;
+    // the odd row (S = 1) reads entry u as entry u + 1 of that layout, so the
+    // window moves by whole registers once every two rows.
+    auto row = [&](int r, auto Sc) {
+      constexpr int S = decltype(Sc)::value;
       const int i = lo + r;
-      if (r > 0) {
-        // Shift the window by one entry; the entry entering at H-1 is the
-        // next lane's old first entry, or the new pair for the last lane.
-        const uint32_t first = (uint32_t)dppi((int)R[0], NextC{}) & 0xFFFFu;
-#pragma unroll
-        for (int k = 0; k < NR - 1; ++k) R[k] = __builtin_amdgcn_alignbit(R[k + 1], R[k], 16);
-        const uint32_t entering = last ? pnext : first;
-        if (H & 1) R[NR - 1] = entering;                             // entry H-1: low half
-        else R[NR - 1] = (R[NR - 1] >> 16) | (entering << 16);  // H-2 <- H-1, H-1 high
-      }
       // Raw letters of row i + 1: subject positions i + 1 - d1 + P*H - 2.
       const int pos = i - 1 + P * H;
       const int rq = (int)q.pad[qp + (uint32_t)min(i, n)];
@@ -514,7 +510,8 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       }
       const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
       auto score = [&](int u) {
-        const uint32_t off = (u & 1) ? (R[u >> 1] >> 16) : (R[u >> 1] & 0xFFFFu);
+        const int e = u + S;
+        const uint32_t off = (e & 1) ? (R[e >> 1] >> 16) : (R[e >> 1] & 0xFFFFu);
         return *reinterpret_cast<const int*>(Tq + off);
       };
       constexpr int kAhead = H < 4 ? H : 4;
@@ -570,13 +567,13 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       // (with walls: inside its DPs' bands).
       {
         const V fu = Tr::sub(fin, Zr);
-        using S = typename Tr::S;
-        const S fl = Tr::hmax(kWall ? Tr::min(fu, LIVE2) : fu);
+        using Sc16 = typename Tr::S;
+        const Sc16 fl = Tr::hmax(kWall ? Tr::min(fu, LIVE2) : fu);
         V fc = kWall ? fin : Tr::add(fin, GOFF);
         constexpr int kChunk = 8;
 #pragma unroll
         for (int c = 0; c < H; c += kChunk) {
-          if (__ballot(fl > (S)c) == 0) break;
+          if (__ballot(fl > (Sc16)c) == 0) break;
 #pragma unroll
           for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
             G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0])
@@ -588,10 +585,11 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       // Letters of row i + 1.
       qcur = settle(rq);
       pnext = pair_off4(settle(ra), settle(rb));
-      // Frame of the next row; every kFrame rows the frame moves back.
+      // Frame of the next row; every kFrame rows the frame moves back (an
+      // even row never ends a frame).
       best = Tr::add(best, ONE2);
       Zr = Tr::add(Zr, ONE2);
-      if ((r & (kFrame - 1)) == kFrame - 1) {
+      if (S == 1 && (r & (kFrame - 1)) == kFrame - 1) {
 #pragma unroll
         for (int u = 0; u < H; ++u) {
           G[u] = Tr::sub(G[u], FRAME2);
@@ -600,6 +598,24 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         best = Tr::sub(best, FRAME2);
         Zr = ZERO2;
       }
+    };
+    // Entry e of the window gets the pair offset x.
+    auto put = [&](int e, uint32_t x) {
+      if (e & 1) R[e >> 1] = (R[e >> 1] & 0xFFFFu) | (x << 16);
+      else R[e >> 1] = x;
+    };
+    static_assert(kFrame % 2 == 0, "frames end on odd rows");
+    for (int r = 0; r < steps; r += 2) {
+      // The next lane's entries 0 and 1 of row r: the entries entering at
+      // H-1 in rows r+1 and r+2 (the last lane takes new pairs instead).
+      const uint32_t nx = (uint32_t)dppi((int)R[0], NextC{});
+      row(r, std::integral_constant<int, 0>{});
+      if (r + 1 >= steps) break;
+      put(H, last ? pnext : (nx & 0xFFFFu));
+      row(r + 1, std::integral_constant<int, 1>{});
+#pragma unroll
+      for (int k = 0; k + 1 < NR; ++k) R[k] = R[k + 1];
+      put(H - 1, last ? pnext : (nx >> 16));
     }
     best = Tr::sub(best, Zr);
     best = Tr::max(best, mv(best, SwapC{}));
