@@ -74,6 +74,20 @@ bool hsp_less(const HspRec& a, const HspRec& b) {
 std::vector<std::vector<HspRec>> merge_frames(std::vector<std::vector<HspRec>>& per_target,
                                               const dmr::TargetSet& t, size_t nqueries,
                                               size_t max_hsps, size_t max_targets) {
+  // One target per query (protein queries): the search already returns each
+  // target's HSPs in this order (subjects by their best HSP, each subject's
+  // HSPs sorted, both limits applied per target), so the lists move as they
+  // are.
+  bool single = per_target.size() == nqueries && t.query_of.size() == nqueries;
+  for (size_t ti = 0; single && ti < nqueries; ++ti) single = t.query_of[ti] == ti && t.frame_of[ti] == 0;
+  if (single) {
+    for (size_t ti = 0; ti < nqueries; ++ti)
+      for (auto& h : per_target[ti]) {
+        h.frame = 0;
+        h.query = (uint32_t)ti;
+      }
+    return std::move(per_target);
+  }
   std::vector<std::vector<HspRec>> per(nqueries);
   for (size_t ti = 0; ti < per_target.size(); ++ti) {
     size_t q = t.query_of[ti];

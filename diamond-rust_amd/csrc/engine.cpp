@@ -445,16 +445,21 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   S.maybe = nmaybe;
   uint32_t* midx = sel_idx_.ensure<uint32_t>(nmaybe + 1);
   launch_compact_flags(fl, ps, njobs, midx, st_);
-  std::vector<uint32_t> h_midx(nmaybe);
-  std::vector<SwJob> h_mjobs(nmaybe);
-  std::vector<SwOut> h_mouts(nmaybe);
+  // The candidates (job index, job, score-pass output) come back through
+  // pinned memory and are read in place (valid until the next search).
+  const uint64_t pin_job = (nmaybe * 4 + 15) / 16 * 16;
+  const uint64_t pin_out = pin_job + (nmaybe * sizeof(SwJob) + 15) / 16 * 16;
+  uint8_t* spin = sel_pin_.ensure<uint8_t>(pin_out + nmaybe * sizeof(SwOut) + 16);
+  uint32_t* h_midx = reinterpret_cast<uint32_t*>(spin);
+  SwJob* h_mjobs = reinterpret_cast<SwJob*>(spin + pin_job);
+  SwOut* h_mouts = reinterpret_cast<SwOut*>(spin + pin_out);
   if (nmaybe) {
     SwJob* gj = tmpjobs_.ensure<SwJob>(std::max<uint64_t>(nmaybe, nsegs) + 1);
     SwOut* go = sel_outs_.ensure<SwOut>(nmaybe + 1);
     launch_gather(jobs, outs, midx, nmaybe, gj, go, st_);
-    DMR_HIP(hipMemcpyAsync(h_midx.data(), midx, nmaybe * 4, hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(h_mjobs.data(), gj, nmaybe * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(h_mouts.data(), go, nmaybe * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_midx, midx, nmaybe * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_mjobs, gj, nmaybe * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_mouts, go, nmaybe * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
   }
   stage_event(8);
   DMR_HIP(hipStreamSynchronize(st_));
@@ -898,51 +903,71 @@ void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<
   launch_fill_u32(cnt, 2 * kWaveKinds, 0, st_);
   const int cus = cu_count_;
   uint64_t cells = 0;
-  // Per group, its job list and the trace and ops capacity offsets, all in
-  // one pinned block that goes to the device in one copy: [list u32 x ni,
-  // padded to 8 bytes][trace offsets u64 x ni+1][ops offsets u64 x ni+1].
+  // Tasks of all wave groups in group order (group g from out_base[g]).
+  uint64_t* out_base = run->out_base;
+  uint64_t nout = 0;
+  for (int g = 0; g < kWaveKinds; ++g) {
+    out_base[g] = nout;
+    nout += group[g].size();
+  }
+  run->nout = nout;
+  // One pinned block goes to the device in one copy: per group [list u32 x
+  // ni, padded to 8 bytes][trace offsets u64 x ni+1], then the ops offsets of
+  // all groups' tasks (u64 x nout+1, one array: ops and packed ops are laid
+  // out over all groups).
   uint64_t* in_off = run->in_off;
   uint64_t in_words = 0;  // in u64 units
   for (int g = 0; g < kWaveKinds; ++g) {
     const uint64_t ni = group[g].size();
     if (!ni) continue;
     in_off[g] = in_words;
-    in_words += (ni + 1) / 2 + 2 * (ni + 1);
+    in_words += (ni + 1) / 2 + (ni + 1);
   }
+  run->ooff_at = in_words;
+  in_words += nout + 1;
   uint64_t* pin_in = tb_pin_in_.ensure<uint64_t>(in_words + 1);
-  auto rows_of = [&](uint64_t k) {
-    const SwJob& j = hjobs[k];
-    const int64_t n = (int64_t)j.q1 - j.q0, m = slen[k];
-    const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
-    return rhi >= rlo ? rhi - rlo + 1 : 0;
-  };
+  uint64_t* ooff = pin_in + run->ooff_at;
+  ooff[0] = 0;
   for (int g = 0; g < kWaveKinds; ++g) {
     auto& items = group[g];
     const uint64_t ni = items.size();
     if (!ni) continue;
-    // Rows of a job's band and its cells; the longest jobs start first so
-    // they do not finish last.
-    std::stable_sort(items.begin(), items.end(), [&](uint64_t a, uint64_t b) {
-      return rows_of(a) * ((int64_t)hjobs[a].d1 - hjobs[a].d0 + 1) >
-             rows_of(b) * ((int64_t)hjobs[b].d1 - hjobs[b].d0 + 1);
+    // Rows and cells of each job's band; the longest jobs start first so
+    // they do not finish last (sorted on precomputed keys, ties by index).
+    std::vector<std::pair<uint64_t, uint64_t>> order(ni);  // (cells, item)
+    for (uint64_t t = 0; t < ni; ++t) {
+      const SwJob& j = hjobs[items[t]];
+      const int64_t n = (int64_t)j.q1 - j.q0, m = slen[items[t]];
+      const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
+      const int64_t rows = rhi >= rlo ? rhi - rlo + 1 : 0;
+      order[t] = {(uint64_t)(rows * ((int64_t)j.d1 - j.d0 + 1)), items[t]};
+    }
+    std::sort(order.begin(), order.end(), [](const std::pair<uint64_t, uint64_t>& a,
+                                             const std::pair<uint64_t, uint64_t>& b) {
+      return a.first != b.first ? a.first > b.first : a.second < b.second;
     });
     uint32_t* jl = reinterpret_cast<uint32_t*>(pin_in + in_off[g]);
     uint64_t* toff = pin_in + in_off[g] + (ni + 1) / 2;
-    uint64_t* ooff = toff + ni + 1;
-    toff[0] = ooff[0] = 0;
+    toff[0] = 0;
     const int64_t stride = 64ll << g;  // trace row: 64 * K bytes
     for (uint64_t t = 0; t < ni; ++t) {
+      items[t] = order[t].second;
       const SwJob& j = hjobs[items[t]];
       jl[t] = list[items[t]];
       const int64_t n = (int64_t)j.q1 - j.q0, m = slen[items[t]];
-      const int64_t rows = rows_of(items[t]);
+      const int64_t w = (int64_t)j.d1 - j.d0 + 1;
+      const int64_t rows = w > 0 ? (int64_t)(order[t].first / (uint64_t)w) : 0;
       toff[t + 1] = toff[t] + (uint64_t)(rows * stride);
-      ooff[t + 1] = ooff[t] + (uint64_t)(n + m + 1);
-      cells += (uint64_t)(rows * ((int64_t)j.d1 - j.d0 + 1));
+      const uint64_t T = out_base[g] + t;
+      ooff[T + 1] = ooff[T] + (uint64_t)(n + m + 1);
+      cells += order[t].first;
     }
   }
   uint64_t* din = tb_in_.ensure<uint64_t>(in_words + 1);
   DMR_HIP(hipMemcpyAsync(din, pin_in, in_words * 8, hipMemcpyHostToDevice, st_));
+  const uint64_t* dooff = din + run->ooff_at;
+  SwOut* dout_all = tb_out_all_.ensure<SwOut>(nout + 1);
+  uint32_t* dops_all = tb_ops_all_.ensure<uint32_t>(ooff[nout] + 1);
   // The groups run side by side on the auxiliary streams, widest first (its
   // DPs run longest).
   fork();
@@ -955,48 +980,29 @@ void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<
     hipStream_t s = kStreamOf[g] < 0 ? st_ : aux_[kStreamOf[g]];
     const uint32_t* dl = reinterpret_cast<const uint32_t*>(din + in_off[g]);
     const uint64_t* dt = din + in_off[g] + (ni + 1) / 2;
-    const uint64_t* dop = dt + ni + 1;
     const uint64_t* htoff = pin_in + in_off[g] + (ni + 1) / 2;
-    const uint64_t* hooff = htoff + ni + 1;
     uint8_t* tr = tb_trace_[g].ensure<uint8_t>(htoff[ni] + 1);
-    uint32_t* dops = tb_ops_[g].ensure<uint32_t>(hooff[ni] + 1);
-    SwOut* dout = tb_out_[g].ensure<SwOut>(ni + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                   cnt + g, dout, tr, dt, dops, dop, fb, cnt + kWaveKinds + g, cus, s, known);
+                   cnt + g, dout_all + out_base[g], tr, dt, dops_all, dooff + out_base[g], fb,
+                   cnt + kWaveKinds + g, cus, s, known);
   }
   join();
-  // Results: the ops of each group packed on the device (n_ops words per
-  // DP), then outputs, pack offsets, packed ops and fallback lists through
-  // pinned memory: two rounds of copies, one wait each (the first queued
-  // here, the rest in traceback_collect).
-  uint64_t* out_base = run->out_base;
-  uint64_t* poff_base = run->poff_base;
-  uint64_t nout = 0;
-  for (int g = 0; g < kWaveKinds; ++g) {
-    const uint64_t ni = group[g].size();
-    if (!ni) continue;
-    out_base[g] = nout;
-    poff_base[g] = nout + g;
-    nout += ni;
-  }
+  // Results: the ops of all groups packed on the device (n_ops words per
+  // DP, one array), then outputs, pack offsets, packed ops and fallback
+  // lists through pinned memory: two rounds of copies, one wait each (the
+  // first queued here, the rest in traceback_collect).
   SwOut* pin_out = tb_pin_out_.ensure<SwOut>(nout + 1);
-  uint64_t* pin_poff = tb_pin_poff_.ensure<uint64_t>(nout + kWaveKinds + 1);
+  uint64_t* pin_poff = tb_pin_poff_.ensure<uint64_t>(nout + 1);
   uint32_t* pin_fb = tb_pin_fb_.ensure<uint32_t>(nout + kWaveKinds + 1);
   DMR_HIP(hipMemcpyAsync(pin_fb, cnt + kWaveKinds, kWaveKinds * 4, hipMemcpyDeviceToHost, st_));
-  for (int g = 0; g < kWaveKinds; ++g) {
-    const uint64_t ni = group[g].size();
-    if (!ni) continue;
-    const uint64_t* dop = tb_in_.as<uint64_t>() + in_off[g] + (ni + 1) / 2 + ni + 1;
-    const uint64_t cap = (pin_in + in_off[g] + (ni + 1) / 2 + ni + 1)[ni];
-    uint64_t* poff = tb_poff_[g].ensure<uint64_t>(ni + 1);
-    launch_ops_pack(tb_out_[g].as<SwOut>(), tb_ops_[g].as<uint32_t>(), dop, (uint32_t)ni,
-                    tb_sizes_.ensure<uint32_t>(ni + 1), poff,
-                    b_scan_.ensure<uint64_t>(scan_tmp_elems(ni + 1)), tb_pack_[g].ensure<uint32_t>(cap + 1),
+  if (nout) {
+    uint64_t* poff = tb_poff_all_.ensure<uint64_t>(nout + 1);
+    launch_ops_pack(dout_all, dops_all, dooff, (uint32_t)nout, tb_sizes_.ensure<uint32_t>(nout + 1), poff,
+                    b_scan_.ensure<uint64_t>(scan_tmp_elems(nout + 1)), tb_pack_all_.ensure<uint32_t>(ooff[nout] + 1),
                     st_);
-    DMR_HIP(hipMemcpyAsync(pin_out + out_base[g], tb_out_[g].as<SwOut>(), ni * sizeof(SwOut),
-                           hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(pin_poff + poff_base[g], poff, (ni + 1) * 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(pin_out, dout_all, nout * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(pin_poff, poff, (nout + 1) * 8, hipMemcpyDeviceToHost, st_));
   }
   run->cells = cells;
 }
@@ -1015,7 +1021,7 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
   const std::vector<uint32_t>& slen = *run->slen;
   const std::vector<std::vector<uint64_t>>& group = run->group;
   const uint64_t* out_base = run->out_base;
-  const uint64_t* poff_base = run->poff_base;
+  const uint64_t nout = run->nout;
   const SwJob* djobs = run->djobs;
   const SeqSet qs = run->qs, dbs = run->dbs;
   const int go = run->go, ge = run->ge;
@@ -1027,28 +1033,20 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
   DMR_HIP(hipStreamSynchronize(st_));
   uint32_t nfb[kWaveKinds];
   std::copy(pin_fb, pin_fb + kWaveKinds, nfb);
-  uint64_t ops_base[kWaveKinds] = {}, nops = 0, fb_base[kWaveKinds] = {}, nfbt = 0;
+  uint64_t fb_base[kWaveKinds] = {}, nfbt = 0;
   for (int g = 0; g < kWaveKinds; ++g) {
-    const uint64_t ni = group[g].size();
-    if (!ni) continue;
-    ops_base[g] = nops;
-    nops += pin_poff[poff_base[g] + ni];
     fb_base[g] = nfbt;
-    nfbt += nfb[g];
+    nfbt += group[g].empty() ? 0 : nfb[g];
   }
+  const uint64_t nops = nout ? pin_poff[nout] : 0;
   uint32_t* pin_ops = tb_pin_ops_.ensure<uint32_t>(nops + 1);
   uint32_t* pin_fbl = tb_pin_fb_.ensure<uint32_t>(nfbt + 1);  // counts already read
-  for (int g = 0; g < kWaveKinds; ++g) {
-    const uint64_t ni = group[g].size();
-    if (!ni) continue;
-    const uint64_t words = pin_poff[poff_base[g] + ni];
-    if (words)
-      DMR_HIP(hipMemcpyAsync(pin_ops + ops_base[g], tb_pack_[g].as<uint32_t>(), words * 4,
-                             hipMemcpyDeviceToHost, st_));
-    if (nfb[g])
+  if (nops)
+    DMR_HIP(hipMemcpyAsync(pin_ops, tb_pack_all_.as<uint32_t>(), nops * 4, hipMemcpyDeviceToHost, st_));
+  for (int g = 0; g < kWaveKinds; ++g)
+    if (!group[g].empty() && nfb[g])
       DMR_HIP(hipMemcpyAsync(pin_fbl + fb_base[g], tb_fb_[g].as<uint32_t>(), nfb[g] * 4,
                              hipMemcpyDeviceToHost, st_));
-  }
   DMR_HIP(hipStreamSynchronize(st_));
   for (int g = 0; g < kWaveKinds; ++g) {
     const auto& items = group[g];
@@ -1060,12 +1058,12 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
       is_fb[t] = 1;
       fallback_items.push_back(items[t]);
     }
-    const uint64_t* po = pin_poff + poff_base[g];
     for (uint64_t t = 0; t < ni; ++t) {
       if (is_fb[t]) continue;
-      const SwOut& o = pin_out[out_base[g] + t];
+      const uint64_t T = out_base[g] + t;
+      const SwOut& o = pin_out[T];
       (*outs)[items[t]] = o;
-      const uint32_t* src = pin_ops + ops_base[g] + po[t];
+      const uint32_t* src = pin_ops + pin_poff[T];
       (*ops)[items[t]].assign(src, src + o.n_ops);
     }
   }

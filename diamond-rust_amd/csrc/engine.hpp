@@ -181,7 +181,12 @@ class Engine {
     int go = 0, ge = 0;
     uint64_t ns = 0, cells = 0;
     std::vector<std::vector<uint64_t>> group;
-    uint64_t in_off[kWaveKinds] = {}, out_base[kWaveKinds] = {}, poff_base[kWaveKinds] = {};
+    // Per wave-kernel group: its block in the pinned input and its first
+    // task in the all-group order (outputs, ops offsets and packed ops are
+    // laid out over all groups in that order); nout tasks in all, ooff_at:
+    // the all-group ops offsets (nout + 1) in the pinned input.
+    uint64_t in_off[kWaveKinds] = {}, out_base[kWaveKinds] = {};
+    uint64_t nout = 0, ooff_at = 0;
   };
   void traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<uint32_t>& list,
                         const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
@@ -288,6 +293,8 @@ class Engine {
       tb_trace_[kTbGroups], tb_ops_[kTbGroups], tb_out_[kTbGroups], tb_fb_[kTbGroups], tb_in_,
       tb_sizes_, tb_poff_[kTbGroups], tb_pack_[kTbGroups];
   PinBuf tb_pin_in_, tb_pin_out_, tb_pin_poff_, tb_pin_ops_, tb_pin_fb_;
+  DevBuf tb_out_all_, tb_ops_all_, tb_poff_all_, tb_pack_all_;
+  PinBuf sel_pin_;  // selection candidates (index, job, score-pass output)
   // Host copies of each group's job list and offsets; they stay alive until
   // the copies queued from them have run.
   std::vector<uint64_t> last_cbeg_, last_cend_;
