@@ -53,7 +53,8 @@ def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
         return f"k_sw16p<257, 8, 512, {tr}, true>" if name == "m256" else f"k_sw16p<193, 16, 512, {tr}, true>"
     w = int(name[1:])
     lanes = 2 if w == 81 else 4
-    return f"k_sw16p<{w}, {lanes}, 512, {tr}, false>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
+    threads = 768 if w == 129 else 512
+    return f"k_sw16p<{w}, {lanes}, {threads}, {tr}, false>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
 
 
 HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",

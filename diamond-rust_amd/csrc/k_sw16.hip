@@ -366,8 +366,18 @@ struct SplitShape {
 
 constexpr int kWallOff = -30000;
 
+// Waves per SIMD the register allocation is held to: three for the width-129
+// split kernel (168 registers; the few values that then leave registers are
+// per-DP setup, outside the row loop), what the code needs otherwise. A
+// block holds the 70 KB score table, so one block per CU carries those three
+// waves per SIMD: 768 threads.
+template <int W, bool kWall>
+constexpr int kMinWaves = (W == 129 && !kWall) ? 3 : 1;
+template <int W, bool kWall>
+constexpr int kSplitThreads = kMinWaves<W, kWall> == 3 ? 768 : 512;
+
 template <int W, int P, int kThreads, class Tr, bool kWall>
-__global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<W, kWall>))) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes,
                                                    const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                                    const int* __restrict__ pair_tab,
                                                    uint32_t* __restrict__ counter,
@@ -518,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
       int sq[kAhead];
 #pragma unroll
       for (int u = 0; u < kAhead; ++u) sq[u] = score(u);
-      V F = NEG2, Gl = NEG2;
+      V F = NEG2, Gl = NEG2, Hodd = ZERO2;
 #pragma unroll
       for (int u = 0; u < H; ++u) {
         const V sc = Tr::bits(sq[u % kAhead]);
@@ -530,7 +540,11 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
         const V Ev = Tr::max3(G[u + 1], E[u + 1], Zr);
         const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
         const V Hh = Tr::max3(D, Ev, Fv);
-        best = Tr::max(best, Hh);
+        // The best score takes cells two at a time as they come (one max3
+        // per two cells, and no cell value stays live to the end of the row).
+        if (u & 1) best = Tr::max3(best, Hodd, Hh);
+        else if (u + 1 < H) Hodd = Hh;
+        else best = Tr::max(best, Hh);
         Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : (u >= kDeadFrom ? GOD : GOFF));
         G[u] = Gl;
         E[u] = Ev;
@@ -578,6 +592,10 @@ __global__ __launch_bounds__(kThreads) void k_sw16p(const uint2* __restrict__ la
           for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
             G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0])
                                        : (u >= kDeadFrom ? Tr::add(fc, DADJ) : fc));
+            // Settled here: left to itself the compiler applies every chunk's
+            // maxima after the last test, which keeps the whole ramp live
+            // (about 30 registers more).
+            asm volatile("" : "+v"(G[u]));
             fc = Tr::sub(fc, ONE2);
           }
         }
@@ -845,9 +863,10 @@ void launch_width(bool split, unsigned tasks, const uint2* lanes, uint32_t nlane
   } else {
     if (split) {
       constexpr int P = kSplitLanes<W>;
-      static const unsigned resident = resident_blocks(k_sw16p<W, P, 512, Tr, false>, 512, cu_count);
-      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 8));
-      hipLaunchKernelGGL((k_sw16p<W, P, 512, Tr, false>), dim3(blocks), dim3(512), 0, st, lanes, nlanes,
+      constexpr int T = kSplitThreads<W, false>;
+      static const unsigned resident = resident_blocks(k_sw16p<W, P, T, Tr, false>, T, cu_count);
+      const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, T / 64));
+      hipLaunchKernelGGL((k_sw16p<W, P, T, Tr, false>), dim3(blocks), dim3(T), 0, st, lanes, nlanes,
                          jobs, q, db, pair_tab, counter, outs, redo, nredo);
     } else {
       static const unsigned resident = resident_blocks(k_sw16<W, Tr>, 256, cu_count);
