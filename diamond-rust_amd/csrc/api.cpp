@@ -669,14 +669,16 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
   for (size_t i = 0; i < r->hsps.size(); ++i)
     if (r->target_of[i] >= 0 && (r->hsps[i].query >= qid.size() || r->hsps[i].subject >= sid.size()))
       return fail(DMR_E_ARG, "id list shorter than the query or database");
-  // Lines are independent: blocks of kBlock HSPs render in parallel into
-  // their own strings, joined in order.
-  constexpr size_t kBlock = 512;
-  const size_t nh = r->hsps.size(), nblocks = (nh + kBlock - 1) / kBlock;
+  // Lines are independent: one block of lines per host worker (at least 64
+  // lines each) renders into its own string; the blocks are joined in order.
+  const size_t nh = r->hsps.size();
+  const size_t nblocks = std::max<size_t>(1, std::min<size_t>(dmr::host_threads(), (nh + 63) / 64));
+  const size_t block = (nh + nblocks - 1) / nblocks;
   std::vector<std::string> parts(nblocks);
   dmr::host_parallel_for(nblocks, 1, [&](uint64_t b0, uint64_t b1) {
     for (uint64_t blk = b0; blk < b1; ++blk)
-      render_lines(r, qid, sid, f, blk * kBlock, std::min(nh, (size_t)(blk + 1) * kBlock), &parts[blk]);
+      render_lines(r, qid, sid, f, std::min(nh, blk * block), std::min(nh, (size_t)(blk + 1) * block),
+                   &parts[blk]);
   });
   std::string& out = *text;
   out.clear();

@@ -89,9 +89,16 @@ int pair_table_entries();
 void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out);
 void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits,
                      uint64_t* keys, uint32_t* idx, hipStream_t st);
-void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
-                       const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
+// Lane pairing of the sorted score-pass jobs (see k_lane_heads): head flags
+// and class ranges, then lanes at the heads' scan positions, then the lanes
+// per class. summary holds 2 * (nclasses + npacked) + 1 words.
+void launch_lane_heads(const uint64_t* keys, const uint32_t* flags, const uint64_t* pos, const uint32_t* run_start,
+                       uint64_t n, int cls_shift, int npacked, int nclasses, uint32_t* heads, uint32_t* summary,
                        hipStream_t st);
+void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const uint32_t* heads,
+                       const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st);
+void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
+                         hipStream_t st);
 int sw16_width(int cls);
 // Lanes per DP pair of class cls when it runs split (always for 5-6).
 int sw16_split_lanes(int cls);

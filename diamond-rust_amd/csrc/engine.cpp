@@ -615,44 +615,45 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
                    njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
                    b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_);
-  // Runs of equal (class, target): their start positions and keys.
+  // Runs of equal (class, target) and their starts; lane heads (even
+  // positions of a run in a packed class), their scan, the lanes and the
+  // per-class summary, all on the device; one small read brings the class
+  // ranges and lane counts back.
   uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
   launch_key_flags(keys, njobs, kRowKeyBits, fl, st_);
   launch_fill_u32(fl + njobs, 1, 0, st_);
   uint64_t* ps = pos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
-  const uint64_t nruns = read1(ps + njobs);
-  uint32_t* rs = runs_.ensure<uint32_t>(nruns + 1);
+  uint32_t* rs = runs_.ensure<uint32_t>(njobs + 1);
   launch_compact_flags(fl, ps, njobs, rs, st_);
-  uint64_t* rk = sw_keys2_.as<uint64_t>();
-  launch_gather_u64(keys, rs, nruns, rk, st_);
-  std::vector<uint32_t> h_rs(nruns + 1);
-  std::vector<uint64_t> h_keys(nruns);
-  DMR_HIP(hipMemcpyAsync(h_rs.data(), rs, nruns * 4, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipMemcpyAsync(h_keys.data(), rk, nruns * 8, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipStreamSynchronize(st_));
-  h_rs[nruns] = (uint32_t)njobs;
   const int cls_shift = target_bits + kRowKeyBits;
-  // Class ranges in the sorted order; lane pairs for the int16 classes.
+  const int npacked = fast ? kPackedClasses : 0;
+  constexpr int kSummary = 2 * (kClasses + kPackedClasses) + 1;
+  uint32_t* summ = sw_summary_.ensure<uint32_t>(kSummary);
+  launch_fill_u32(summ, kSummary, 0, st_);
+  uint32_t* heads = sw_heads_.ensure<uint32_t>(njobs + 1);
+  launch_lane_heads(keys, fl, ps, rs, njobs, cls_shift, npacked, kClasses, heads, summ, st_);
+  launch_fill_u32(heads + njobs, 1, 0, st_);
+  uint64_t* hpos = sw_hpos_.ensure<uint64_t>(njobs + 1);
+  scan_exclusive_u32_to_u64(heads, hpos, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
+  uint2* paired = sw_lanes2_.ensure<uint2>(njobs + 1);
+  launch_make_lanes(idx, fl, heads, hpos, njobs, paired, st_);
+  launch_lane_summary(hpos, njobs, npacked, kClasses, summ, st_);
+  uint32_t* h_summ = sw_pin_summary_.ensure<uint32_t>(kSummary);
+  DMR_HIP(hipMemcpyAsync(h_summ, summ, kSummary * 4, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  // Class ranges in the sorted order; lanes of the packed classes.
   uint64_t cbeg[kClasses], cend[kClasses];
-  for (int c = 0; c < kClasses; ++c) cbeg[c] = cend[c] = 0;
-  std::vector<uint32_t> run_start, run_len, lane_off;
-  uint32_t class_lanes[kPackedClasses] = {}, class_lane0[kPackedClasses] = {};
-  uint32_t nl = 0;
-  for (uint64_t r = 0; r < nruns; ++r) {
-    int cls = (int)(h_keys[r] >> cls_shift);
-    if (cls >= kClasses) cls = kClasses - 1;
-    if (cend[cls] == 0) cbeg[cls] = h_rs[r];
-    cend[cls] = h_rs[r + 1];
-    if (cls >= kPackedClasses || !fast) continue;
-    if (class_lanes[cls] == 0) class_lane0[cls] = nl;
-    uint32_t len = h_rs[r + 1] - h_rs[r];
-    run_start.push_back(h_rs[r]);
-    run_len.push_back(len);
-    lane_off.push_back(nl);
-    nl += (len + 1) / 2;
-    class_lanes[cls] += (len + 1) / 2;
+  for (int c = 0; c < kClasses; ++c) {
+    cbeg[c] = h_summ[c];
+    cend[c] = h_summ[kClasses + c];
   }
+  uint32_t class_lanes[kPackedClasses] = {}, class_lane0[kPackedClasses] = {};
+  for (int c = 0; c < npacked; ++c) {
+    class_lane0[c] = h_summ[2 * kClasses + c];
+    class_lanes[c] = h_summ[2 * kClasses + npacked + c];
+  }
+  const uint32_t nl = h_summ[2 * (kClasses + npacked)];
   uint32_t* cnt = sw_counters_.ensure<uint32_t>(32);
   launch_fill_u32(cnt, 32, 0, st_);
   uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
@@ -660,13 +661,6 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   const int cus = cu_count_;
   uint2* lanes = nullptr;
   if (nl) {
-    const uint32_t nr = (uint32_t)run_start.size();
-    uint32_t* d_rs = sw_runs_.ensure<uint32_t>(3ull * nr + 3);
-    DMR_HIP(hipMemcpyAsync(d_rs, run_start.data(), nr * 4, hipMemcpyHostToDevice, st_));
-    DMR_HIP(hipMemcpyAsync(d_rs + nr, run_len.data(), nr * 4, hipMemcpyHostToDevice, st_));
-    DMR_HIP(hipMemcpyAsync(d_rs + 2 * nr, lane_off.data(), nr * 4, hipMemcpyHostToDevice, st_));
-    uint2* paired = sw_lanes2_.ensure<uint2>(nl + 1);
-    launch_make_lanes(idx, d_rs, d_rs + nr, d_rs + 2 * nr, nr, nl, paired, st_);
     // Lanes of each class in order of row span, longest first: a wave (or a
     // block of lane groups) then steps lanes of about the same length, and
     // the longest start first. Classes stay contiguous (class is the major

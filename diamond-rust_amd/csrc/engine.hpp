@@ -284,8 +284,9 @@ class Engine {
   DevBuf bd0_, bd1_, bbest_, tmpjobs_, nbands_, joboff_, jobs_;
   DevBuf stride_, scr_off_, scratch_, outs_;
   DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
-  DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_runs_, sw_lanes_, sw_lanes2_, sw_lane_idx_, sw_counters_,
-      sw_redo_, sw_list_;
+  DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_lanes_, sw_lanes2_, sw_lane_idx_, sw_counters_,
+      sw_redo_, sw_list_, sw_summary_, sw_heads_, sw_hpos_;
+  PinBuf sw_pin_summary_;
   DevBuf pair_tab_, pair_tab16_, sw_fallback_, sw_klist_[kWaveKinds];
   // Traceback groups: one per wave kernel K, then the row-exact kernel.
   static constexpr int kTbGroups = kWaveKinds + 1;

@@ -81,6 +81,8 @@ HostPool& pool() {
 
 }  // namespace
 
+unsigned host_threads() { return pool().threads(); }
+
 void host_parallel_for(uint64_t n, uint64_t min_per_part,
                        const std::function<void(uint64_t, uint64_t)>& f) {
   if (n == 0) return;

@@ -15,4 +15,7 @@ namespace dmr {
 void host_parallel_for(uint64_t n, uint64_t min_per_part,
                        const std::function<void(uint64_t, uint64_t)>& f);
 
+// Threads that host_parallel_for runs on (workers and the caller).
+unsigned host_threads();
+
 }  // namespace dmr

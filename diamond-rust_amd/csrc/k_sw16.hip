@@ -713,21 +713,48 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
   idx[k] = (uint32_t)k;
 }
 
-// One thread per lane: lane L belongs to the (class, target) run whose
-// lane_off range holds it (binary search) and pairs two consecutive jobs.
-__global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint32_t* __restrict__ run_start,
-                             const uint32_t* __restrict__ run_len, const uint32_t* __restrict__ lane_off,
-                             uint32_t nruns, uint32_t nlanes, uint2* __restrict__ lanes) {
-  const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
-  if (L >= nlanes) return;
-  uint32_t lo = 0, hi = nruns;  // last run with lane_off <= L
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (lane_off[mid] <= L) lo = mid;
-    else hi = mid;
+// One thread per sorted job. A job heads a lane when it sits at an even
+// position of its (class, target) run (run r of job i is pos[i] + flags[i] - 1,
+// starting at run_start[r]) in a packed class (class < npacked); the class
+// ranges of the sorted order go to the summary: summary[c] = first job of
+// class c, summary[nclasses + c] = one past its last (absent classes keep 0).
+__global__ void k_lane_heads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
+                             const uint64_t* __restrict__ pos, const uint32_t* __restrict__ run_start,
+                             uint64_t n, int cls_shift, int npacked, int nclasses,
+                             uint32_t* __restrict__ heads, uint32_t* __restrict__ summary) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  auto cls_of = [&](uint64_t k) { return (int)min<uint64_t>(keys[k] >> cls_shift, (uint64_t)(nclasses - 1)); };
+  const int c = cls_of(i);
+  const uint64_t r = pos[i] + flags[i] - 1;
+  heads[i] = (c < npacked && ((i - run_start[r]) & 1) == 0) ? 1u : 0u;
+  if (i == 0 || cls_of(i - 1) != c) summary[c] = (uint32_t)i;
+  if (i + 1 == n || cls_of(i + 1) != c) summary[nclasses + c] = (uint32_t)(i + 1);
+}
+
+// One thread per sorted job: a lane head pairs itself with the next job of
+// its run (none at the run's end) into lane hpos[i].
+__global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint32_t* __restrict__ flags,
+                             const uint32_t* __restrict__ heads, const uint64_t* __restrict__ hpos, uint64_t n,
+                             uint2* __restrict__ lanes) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !heads[i]) return;
+  lanes[hpos[i]] = make_uint2(sorted_idx[i], i + 1 < n && flags[i + 1] == 0 ? sorted_idx[i + 1] : ~0u);
+}
+
+// Lanes per packed class from the class ranges: summary[2 * nclasses + c] =
+// first lane of class c, summary[2 * nclasses + npacked + c] = its lane
+// count, summary[2 * (nclasses + npacked)] = all lanes.
+__global__ void k_lane_summary(const uint64_t* __restrict__ hpos, uint64_t n, int npacked, int nclasses,
+                               uint32_t* __restrict__ summary) {
+  const int c = threadIdx.x;
+  if (c < npacked) {
+    const uint32_t b = summary[c], e = summary[nclasses + c];
+    const bool any = e > b;
+    summary[2 * nclasses + c] = any ? (uint32_t)hpos[b] : 0u;
+    summary[2 * nclasses + npacked + c] = any ? (uint32_t)(hpos[e] - hpos[b]) : 0u;
   }
-  const uint32_t k = 2 * (L - lane_off[lo]), s = run_start[lo], len = run_len[lo];
-  lanes[L] = make_uint2(sorted_idx[s + k], k + 1 < len ? sorted_idx[s + k + 1] : ~0u);
+  if (c == 0) summary[2 * (nclasses + npacked)] = (uint32_t)hpos[n];
 }
 
 // Sort key per lane: class | (row span, longest first). The rows a lane runs
@@ -811,12 +838,24 @@ void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int
                      target_bits, keys, idx);
 }
 
-void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* run_start, const uint32_t* run_len,
-                       const uint32_t* lane_off, uint32_t nruns, uint32_t nlanes, uint2* lanes,
+void launch_lane_heads(const uint64_t* keys, const uint32_t* flags, const uint64_t* pos, const uint32_t* run_start,
+                       uint64_t n, int cls_shift, int npacked, int nclasses, uint32_t* heads, uint32_t* summary,
                        hipStream_t st) {
-  if (!nruns || !nlanes) return;
-  hipLaunchKernelGGL(k_make_lanes, dim3(nblk(nlanes, 256)), dim3(256), 0, st, sorted_idx, run_start,
-                     run_len, lane_off, nruns, nlanes, lanes);
+  if (!n) return;
+  hipLaunchKernelGGL(k_lane_heads, dim3(nblk(n, 256)), dim3(256), 0, st, keys, flags, pos, run_start, n,
+                     cls_shift, npacked, nclasses, heads, summary);
+}
+
+void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const uint32_t* heads,
+                       const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_make_lanes, dim3(nblk(n, 256)), dim3(256), 0, st, sorted_idx, flags, heads, hpos, n,
+                     lanes);
+}
+
+void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_lane_summary, dim3(1), dim3(64), 0, st, hpos, n, npacked, nclasses, summary);
 }
 
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
