@@ -50,7 +50,7 @@ def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
         return "k_sw_generic<false>"
     tr = "F16" if f16 else "I16"
     if name.startswith("m"):  # merged bands: 8 (m256) or 16 (m193) lanes per pair, per-DP walls
-        return f"k_sw16p<257, 8, 512, {tr}, true>" if name == "m256" else f"k_sw16p<193, 16, 512, {tr}, true>"
+        return f"k_sw16p<257, 8, 512, {tr}, true>" if name == "m256" else f"k_sw16p<193, 8, 512, {tr}, true>"
     w = int(name[1:])
     lanes = 2 if w == 81 else 4
     threads = 768 if w == 129 else 512

@@ -883,9 +883,9 @@ template <int W>
 constexpr int kSplitLanes = W == 81 ? 2 : 4;
 // Lanes per DP pair of the merged-band classes (width bound W).
 template <int W>
-constexpr int kMergedLanes = W == 257 ? 8 : 16;
+constexpr int kMergedLanes = 8;
 int sw16_split_lanes(int cls) {
-  return cls == 6 ? 8 : (cls == 5 ? 16 : (kWidths[cls] == 81 ? 2 : 4));
+  return cls >= 5 ? 8 : (kWidths[cls] == 81 ? 2 : 4);
 }
 int sw16_width(int cls) { return kWidths[cls]; }
 
