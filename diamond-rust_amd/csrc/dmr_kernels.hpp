@@ -155,6 +155,7 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int cu_count, hipStream_t st, const SwOut* known = nullptr);
+                    int cu_count, hipStream_t st, const SwOut* known = nullptr,
+                    const uint32_t* njobs_dev = nullptr);  // njobs_dev: a device-side count (njobs bounds it)
 
 }  // namespace dmr

@@ -750,13 +750,17 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     for (int c = W0 + 2; c >= W0; --c) launch_wave(c, aux_[1]);  // K = 4, 2, 1
   }
   join();
-  // int16-saturated DPs: re-scored by the int32 wave kernel (widths <= 129).
-  const uint32_t nr16 = read1(cnt);
-  if (nr16)
-    launch_sw_wave(4, false, jobs, redo, nr16, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
-                   opt.gap_extend, cnt + 17, outs, nullptr, nullptr, nullptr, nullptr, fallback,
-                   cnt + 16, cus, st_);
-  const uint32_t nfb = read1(cnt + 16);
+  // DPs at the 16-bit limit (f16 1900, int16 32000, or a best below 11 with
+  // clamped rows that may score): re-scored by the int32 wave kernel (widths
+  // <= 256). It reads their count on the device, so no host read precedes it;
+  // one read afterwards brings both counts.
+  launch_sw_wave(4, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+                 opt.gap_extend, cnt + 17, outs, nullptr, nullptr, nullptr, nullptr, fallback, cnt + 16, cus,
+                 st_, nullptr, cnt);
+  uint32_t* h_cnt = sw_pin_summary_.ensure<uint32_t>(32);
+  DMR_HIP(hipMemcpyAsync(h_cnt, cnt, 17 * 4, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  const uint32_t nr16 = h_cnt[0], nfb = h_cnt[16];
   for (int c = 0; c < kRowExact; ++c)
     if ((c < kPackedClasses && nl && class_lanes[c]) ||
         (c >= kPackedClasses && swk_launched(c, cbeg, cend, fast)))

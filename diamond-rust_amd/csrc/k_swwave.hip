@@ -218,7 +218,8 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
                                                  const uint64_t* __restrict__ ops_off,
                                                  uint32_t* __restrict__ fallback,
                                                  uint32_t* __restrict__ nfallback,
-                                                 const SwOut* __restrict__ known) {
+                                                 const SwOut* __restrict__ known,
+                                                 const uint32_t* __restrict__ njobs_dev) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
   for (int k = threadIdx.x; k < kScoreStride * kScoreStride; k += blockDim.x) M[k] = m32[k];
   __syncthreads();
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     // result, lets the optimizer thread lanes 1-63 from the end of one
     // iteration into the next while lane 0 is inactive.)
     const uint32_t task = uniform(atomicAdd(counter, lane == 0 ? 1u : 0u));
-    if (task >= njobs) break;
+    if (task >= njobs || (njobs_dev && task >= uniform(*njobs_dev))) break;
     const uint32_t jid = uniform(list[task]);
     const uint32_t tgt = uniform(jobs[jid].target), sub = uniform(jobs[jid].subject);
     const uint32_t q0 = uniform(jobs[jid].q0), q1 = uniform(jobs[jid].q1);
@@ -506,7 +507,7 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int cu_count, hipStream_t st, const SwOut* known) {
+                    int cu_count, hipStream_t st, const SwOut* known, const uint32_t* njobs_dev) {
   if (!njobs) return;
 #define DMR_SWW(k, tr)                                                                             \
   do {                                                                                             \
@@ -514,7 +515,7 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
     const unsigned blocks = std::min<unsigned>(resident, nblk(njobs, 4));                          \
     hipLaunchKernelGGL((k_sw_wave<k, tr>), dim3(blocks), dim3(256), 0, st, jobs, list, njobs, q,   \
                        db, m32, gap_open, gap_extend, counter, outs, trace, trace_off, ops,        \
-                       ops_off, fallback, nfallback, known);                                       \
+                       ops_off, fallback, nfallback, known, njobs_dev);                            \
   } while (0)
   if (traceback) {
     switch (K) {
