@@ -24,6 +24,7 @@ struct dmr_ctx {
 
 struct dmr_result {
   std::vector<dmr::HspRec> hsps;     // output order
+  std::vector<std::shared_ptr<const dmr::OpsStore>> ops_stores;  // where the HSPs' ops live
   std::vector<int> target_of;        // target index of each HSP
   dmr::TargetSet targets;
   std::vector<uint64_t> dna_len;     // per query record (blastx)
@@ -204,7 +205,9 @@ void assemble(dmr_ctx* ctx, dmr_result* r, std::vector<std::vector<HspRec>>& per
 void finish(dmr_ctx* ctx, dmr_result* r, const dmr::SearchOptions& o, size_t nqueries,
             bool upload = true) {
   if (upload) ctx->eng->load_targets(r->targets);
-  auto per_target = ctx->eng->search_targets(o, &r->stats);
+  std::shared_ptr<const dmr::OpsStore> store;
+  auto per_target = ctx->eng->search_targets(o, &r->stats, &store);
+  r->ops_stores.push_back(std::move(store));
   assemble(ctx, r, per_target, o, nqueries);
 }
 
@@ -527,7 +530,9 @@ int dmr_blastp_plain(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_off
     auto r = std::make_unique<dmr_result>();
     r->targets = protein_targets(queries, q_offsets, nq);
     r->dna_len.resize(nq, 0);
-    auto per_target = dmr::search_plain(*ctx->eng, r->targets, o, &r->stats);
+    std::shared_ptr<const dmr::OpsStore> store;
+    auto per_target = dmr::search_plain(*ctx->eng, r->targets, o, &r->stats, &store);
+    r->ops_stores.push_back(std::move(store));
     assemble(ctx, r.get(), per_target, o, nq);
     *out = r.release();
     return DMR_OK;

@@ -14,6 +14,43 @@
 
 namespace dmr {
 
+PinPool::~PinPool() {
+  for (auto& b : free_) (void)hipHostFree(b.first);
+}
+
+void* PinPool::take(size_t bytes, size_t* cap) {
+  {
+    std::lock_guard<std::mutex> g(m_);
+    size_t best = free_.size();
+    for (size_t k = 0; k < free_.size(); ++k)
+      if (free_[k].second >= bytes && (best == free_.size() || free_[k].second < free_[best].second)) best = k;
+    if (best < free_.size()) {
+      auto b = free_[best];
+      free_.erase(free_.begin() + (long)best);
+      *cap = b.second;
+      return b.first;
+    }
+  }
+  const size_t c = bytes + bytes / 4 + 4096;
+  void* p = nullptr;
+  DMR_HIP(hipHostMalloc(&p, c, hipHostMallocDefault));
+  *cap = c;
+  return p;
+}
+
+void PinPool::give(void* p, size_t cap) {
+  std::lock_guard<std::mutex> g(m_);
+  free_.push_back({p, cap});
+}
+
+OpsStore::OpsStore(std::shared_ptr<PinPool> pool, size_t words) : pool_(std::move(pool)) {
+  blk_ = pool_->take(words * sizeof(uint32_t), &cap_);
+}
+
+OpsStore::~OpsStore() {
+  if (blk_) pool_->give(blk_, cap_);
+}
+
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 void launch_gather(const SwJob* jobs, const SwOut* outs, const uint32_t* idx, uint64_t n,
                    SwJob* jout, SwOut* oout, hipStream_t st);
@@ -299,9 +336,10 @@ void Engine::load_targets(const TargetSet& t) {
   DMR_HIP(hipStreamSynchronize(st_));
 }
 
-std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt,
-                                                        StageStats* stats) {
+std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt, StageStats* stats,
+                                                        std::shared_ptr<const OpsStore>* ops_store) {
   DMR_HIP(hipSetDevice(device_));
+  *ops_store = std::make_shared<OpsStore>();
   StageStats S;
   double t_start = now_ms();
   std::vector<std::vector<HspRec>> result(q_n_);
@@ -551,8 +589,10 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   S.selected = sel.size();
 
   std::vector<SwOut> h_touts;
-  std::vector<std::vector<uint32_t>> h_tops;
-  traceback_collect(&tb_run, &h_touts, &h_tops, &S.traceback_cells);
+  std::vector<OpsRef> h_tops;
+  std::shared_ptr<OpsStore> tb_store;
+  traceback_collect(&tb_run, &h_touts, &h_tops, &tb_store, &S.traceback_cells);
+  *ops_store = tb_store;
   stage_event(10);
   DMR_HIP(hipStreamSynchronize(st_));
 
@@ -575,7 +615,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
           h.gap_openings = o.gap_openings;
           h.gaps = o.gaps;
           h.length = o.length;
-          h.ops = std::move(h_tops[k]);
+          h.ops = h_tops[k];
           if (o.score != c.h.score)
             throw Error(-2, "traceback score differs from the score pass");
           result[h.query].push_back(std::move(h));
@@ -866,7 +906,11 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
                             const SwOut* known) {
   TbRun run;
   traceback_launch(&run, djobs, list, hjobs, slen, qs, dbs, go, ge, known);
-  traceback_collect(&run, outs, ops, cells_out);
+  std::vector<OpsRef> refs;
+  std::shared_ptr<OpsStore> store;
+  traceback_collect(&run, outs, &refs, &store, cells_out);
+  ops->resize(refs.size());
+  for (size_t k = 0; k < refs.size(); ++k) (*ops)[k].assign(refs[k].begin(), refs[k].end());
 }
 
 // First half of traceback_pass: every wave-kernel DP launched, its ops packed
@@ -1007,11 +1051,12 @@ void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<
 
 // Second half of traceback_pass: waits for the wave-kernel DPs, brings their
 // ops back and runs the row-exact fallback.
-void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
-                               std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out) {
+void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector<OpsRef>* ops,
+                               std::shared_ptr<OpsStore>* store_out, uint64_t* cells_out) {
   const uint64_t ns = run->ns;
   outs->assign(ns, SwOut{});
-  ops->assign(ns, {});
+  ops->assign(ns, OpsRef{});
+  *store_out = std::make_shared<OpsStore>();
   if (cells_out) *cells_out = 0;
   if (!ns) return;
   const std::vector<uint32_t>& list = *run->list;
@@ -1037,7 +1082,10 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
     nfbt += group[g].empty() ? 0 : nfb[g];
   }
   const uint64_t nops = nout ? pin_poff[nout] : 0;
-  uint32_t* pin_ops = tb_pin_ops_.ensure<uint32_t>(nops + 1);
+  // The packed ops go straight into the pinned block the results keep.
+  auto store = std::make_shared<OpsStore>(ops_pool_, nops + 1);
+  *store_out = store;
+  uint32_t* pin_ops = store->block();
   uint32_t* pin_fbl = tb_pin_fb_.ensure<uint32_t>(nfbt + 1);  // counts already read
   if (nops)
     DMR_HIP(hipMemcpyAsync(pin_ops, tb_pack_all_.as<uint32_t>(), nops * 4, hipMemcpyDeviceToHost, st_));
@@ -1061,8 +1109,7 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
       const uint64_t T = out_base[g] + t;
       const SwOut& o = pin_out[T];
       (*outs)[items[t]] = o;
-      const uint32_t* src = pin_ops + pin_poff[T];
-      (*ops)[items[t]].assign(src, src + o.n_ops);
+      (*ops)[items[t]] = OpsRef{pin_ops + pin_poff[T], o.n_ops};
     }
   }
   for (uint64_t k : group[kWaveKinds]) fallback_items.push_back(k);
@@ -1106,9 +1153,10 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs,
     DMR_HIP(hipMemcpyAsync(ho.data(), dout, nf * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
     DMR_HIP(hipMemcpyAsync(hops.data(), dops, ooff[nf] * 4, hipMemcpyDeviceToHost, st_));
     DMR_HIP(hipStreamSynchronize(st_));
+    store->extra = std::move(hops);
     for (uint64_t t = 0; t < nf; ++t) {
       (*outs)[fallback_items[t]] = ho[t];
-      (*ops)[fallback_items[t]].assign(hops.begin() + ooff[t], hops.begin() + ooff[t] + ho[t].n_ops);
+      (*ops)[fallback_items[t]] = OpsRef{store->extra.data() + ooff[t], ho[t].n_ops};
     }
   }
   if (cells_out) *cells_out = cells;

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -100,13 +101,60 @@ struct SearchOptions {
   uint64_t max_expand = 4096;
 };
 
+// Run-length alignment ops of one HSP (op | count << 2; 0 = M, 1 = I, 2 = D),
+// inside an OpsStore that the holder of the HSP list keeps alive.
+struct OpsRef {
+  const uint32_t* p = nullptr;
+  uint32_t n = 0;
+  const uint32_t* begin() const { return p; }
+  const uint32_t* end() const { return p + n; }
+  size_t size() const { return n; }
+};
+
+// Pinned host blocks, reused: a search's traceback ops are copied from the
+// device straight into one block, which its result keeps (OpsStore) and gives
+// back when it goes, so results cost no host copy, allocation or page fault
+// per HSP. Thread-safe (results may be destroyed on any thread).
+class PinPool {
+ public:
+  ~PinPool();
+  // A block of at least `bytes`; *cap receives its size.
+  void* take(size_t bytes, size_t* cap);
+  void give(void* p, size_t cap);
+
+ private:
+  std::mutex m_;
+  std::vector<std::pair<void*, size_t>> free_;
+};
+
+// The ops of a list of HSPs: one pinned block from a PinPool (the packed
+// traceback ops) and a host vector for ops produced on the host side.
+class OpsStore {
+ public:
+  OpsStore() = default;
+  OpsStore(std::shared_ptr<PinPool> pool, size_t words);
+  OpsStore(const OpsStore&) = delete;
+  OpsStore& operator=(const OpsStore&) = delete;
+  ~OpsStore();
+  uint32_t* block() { return static_cast<uint32_t*>(blk_); }
+  std::vector<uint32_t> extra;  // not resized once HSPs point into it
+  // Per-HSP vectors (host-side producers); moving the outer vector keeps
+  // every inner buffer in place.
+  std::vector<std::vector<uint32_t>> parts;
+
+ private:
+  std::shared_ptr<PinPool> pool_;
+  void* blk_ = nullptr;
+  size_t cap_ = 0;
+};
+
 struct HspRec {
   uint32_t query, subject, frame;
   uint32_t q_begin, q_end, s_begin, s_end;  // 0-based half-open, frame coordinates
   int32_t score;
   double bits, evalue;
   uint32_t identities, positives, mismatches, gap_openings, gaps, length;
-  std::vector<uint32_t> ops;  // op | count << 2 (0 = M, 1 = I, 2 = D)
+  OpsRef ops;
 };
 
 struct StageStats {
@@ -155,7 +203,9 @@ class Engine {
   void load_targets(const TargetSet& t);
   // Search the resident targets; HSPs per target, ordered as align_target
   // returns them (src/search.rs:262-270).
-  std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats);
+  // The HSPs' ops live in *ops_store, which the caller keeps with them.
+  std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats,
+                                                  std::shared_ptr<const OpsStore>* ops_store);
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
   unsigned split_mask() const { return split_mask_; }
@@ -191,7 +241,8 @@ class Engine {
   void traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<uint32_t>& list,
                         const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
                         SeqSet dbs, int go, int ge, const SwOut* known);
-  void traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector<std::vector<uint32_t>>* ops,
+  void traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector<OpsRef>* ops,
+                         std::shared_ptr<OpsStore>* store,
                          uint64_t* cells_out);
 
   // --- drop-in calls on host buffers -------------------------------------
@@ -293,7 +344,8 @@ class Engine {
   DevBuf tb_counters_, tb_list_[kTbGroups], tb_toff_[kTbGroups], tb_ooff_[kTbGroups],
       tb_trace_[kTbGroups], tb_ops_[kTbGroups], tb_out_[kTbGroups], tb_fb_[kTbGroups], tb_in_,
       tb_sizes_, tb_poff_[kTbGroups], tb_pack_[kTbGroups];
-  PinBuf tb_pin_in_, tb_pin_out_, tb_pin_poff_, tb_pin_ops_, tb_pin_fb_;
+  PinBuf tb_pin_in_, tb_pin_out_, tb_pin_poff_, tb_pin_fb_;
+  std::shared_ptr<PinPool> ops_pool_ = std::make_shared<PinPool>();
   DevBuf tb_out_all_, tb_ops_all_, tb_poff_all_, tb_pack_all_;
   PinBuf sel_pin_;  // selection candidates (index, job, score-pass output)
   // Host copies of each group's job list and offsets; they stay alive until
@@ -318,7 +370,9 @@ class Engine {
 
 // search::run restated on the host with one drop-in call (one kernel launch)
 // per seed::build / seed::join_into / ungapped::xdrop / gapped::extend_span.
+// The HSPs' ops live in *ops_store, which the caller keeps with them.
 std::vector<std::vector<HspRec>> search_plain(Engine& eng, const TargetSet& t,
-                                              const SearchOptions& opt, StageStats* stats);
+                                              const SearchOptions& opt, StageStats* stats,
+                                              std::shared_ptr<const OpsStore>* ops_store);
 
 }  // namespace dmr

@@ -27,8 +27,11 @@ bool hsp_less(const HspRec& a, const HspRec& b) {
 }  // namespace
 
 std::vector<std::vector<HspRec>> search_plain(Engine& eng, const TargetSet& t,
-                                              const SearchOptions& opt, StageStats* stats) {
+                                              const SearchOptions& opt, StageStats* stats,
+                                              std::shared_ptr<const OpsStore>* ops_store) {
   auto t0 = std::chrono::steady_clock::now();
+  auto store = std::make_shared<OpsStore>();
+  *ops_store = store;
   const uint32_t nt = (uint32_t)t.offs.size() - 1;
   std::vector<std::vector<HspRec>> out(nt);
   const uint32_t ndb = eng.db_size();
@@ -165,7 +168,8 @@ std::vector<std::vector<HspRec>> search_plain(Engine& eng, const TargetSet& t,
         h.gap_openings = o[0].gap_openings;
         h.gaps = o[0].gaps;
         h.length = o[0].length;
-        h.ops = ops[0];
+        store->parts.push_back(std::move(ops[0]));
+        h.ops = OpsRef{store->parts.back().data(), (uint32_t)store->parts.back().size()};
         hsps.push_back(std::move(h));
       }
       if (!hsps.empty()) {
