@@ -49,7 +49,7 @@ def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
     if name == "rowexact":
         return "k_sw_generic<false>"
     tr = "F16" if f16 else "I16"
-    if name.startswith("m"):  # merged bands: 8 (m256) or 16 (m193) lanes per pair, per-DP walls
+    if name.startswith("m"):  # merged bands: 8 lanes per DP pair, per-DP walls
         return f"k_sw16p<257, 8, 512, {tr}, true>" if name == "m256" else f"k_sw16p<193, 8, 512, {tr}, true>"
     w = int(name[1:])
     lanes = 2 if w == 81 else 4
@@ -246,7 +246,7 @@ class Result:
         _check(lib().dmr_result_format(self._h, q, s, f, None, 0, C.byref(n)))
         buf = C.create_string_buffer(n.value + 1)
         _check(lib().dmr_result_format(self._h, q, s, f, buf, n.value + 1, C.byref(n)))
-        return buf.raw[: n.value].decode()
+        return C.string_at(buf, n.value).decode()
 
     def stats(self):
         v = np.zeros(20, dtype=np.float64)

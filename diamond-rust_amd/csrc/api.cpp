@@ -34,7 +34,12 @@ struct dmr_result {
   dmr::StageStats stats;
   // Text of the last dmr_result_format call and the inputs it was made from,
   // so the usual size-then-fill call pair renders once.
-  mutable std::string fmt_key, fmt_text;
+  // Last rendered text, as one block of lines per host thread, and the
+  // inputs it was rendered from (id lists and fields, compared on the next call).
+  mutable std::string fmt_q, fmt_s, fmt_f;
+  mutable std::vector<std::string> fmt_parts;
+  mutable uint64_t fmt_len = 0;
+  mutable bool fmt_valid = false;
 };
 
 namespace {
@@ -642,14 +647,14 @@ const char* const kFieldNames[kFieldCount] = {
 std::vector<std::pair<const char*, size_t>> lines_of(const char* s) {
   std::vector<std::pair<const char*, size_t>> out;
   if (!s) return out;
+  const size_t len = std::strlen(s);
   const char* b = s;
-  for (const char* p = s;; ++p) {
-    if (*p == '\n' || *p == 0) {
-      if (*p == '\n' || p > b) out.push_back({b, (size_t)(p - b)});
-      if (*p == 0) break;
-      b = p + 1;
-    }
+  const char* end = s + len;
+  while (const char* p = static_cast<const char*>(std::memchr(b, '\n', (size_t)(end - b)))) {
+    out.push_back({b, (size_t)(p - b)});
+    b = p + 1;
   }
+  if (end > b) out.push_back({b, (size_t)(end - b)});
   return out;
 }
 
@@ -658,7 +663,7 @@ void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, 
                   size_t i0, size_t i1, std::string* text);
 
 int render(const dmr_result* r, const char* qids, const char* sids, const char* fields,
-           std::string* text) {
+           std::vector<std::string>* parts_out) {
   const auto qid = lines_of(qids), sid = lines_of(sids);
   std::vector<int> f;
   for (const std::string& n : split(fields, ' ')) {
@@ -679,18 +684,13 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
   const size_t nh = r->hsps.size();
   const size_t nblocks = std::max<size_t>(1, std::min<size_t>(dmr::host_threads(), (nh + 63) / 64));
   const size_t block = (nh + nblocks - 1) / nblocks;
-  std::vector<std::string> parts(nblocks);
+  std::vector<std::string>& parts = *parts_out;
+  parts.assign(nblocks, std::string());
   dmr::host_parallel_for(nblocks, 1, [&](uint64_t b0, uint64_t b1) {
     for (uint64_t blk = b0; blk < b1; ++blk)
       render_lines(r, qid, sid, f, std::min(nh, blk * block), std::min(nh, (size_t)(blk + 1) * block),
                    &parts[blk]);
   });
-  std::string& out = *text;
-  out.clear();
-  size_t total = 0;
-  for (auto& pt : parts) total += pt.size();
-  out.reserve(total);
-  for (auto& pt : parts) out += pt;
   return DMR_OK;
 }
 
@@ -836,22 +836,30 @@ int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, c
                       char* buf, uint64_t cap, uint64_t* len) {
   if (!r || !len) return fail(DMR_E_ARG, "NULL argument");
   return guard([&]() -> int {
-    // The inputs as one key: the text is rendered once per distinct request.
-    std::string key;
-    for (const char* s : {qids, sids, fields}) {
-      key += s ? s : "";
-      key += '\0';
-    }
-    if (key != r->fmt_key || r->fmt_key.empty()) {
-      r->fmt_key.clear();
-      const int rc = render(r, qids, sids, fields, &r->fmt_text);
+    // The text is rendered once per distinct request (the size query and
+    // the fill of one request render once) and kept as its blocks.
+    auto same = [](const std::string& kept, const char* s) { return kept == (s ? s : ""); };
+    if (!r->fmt_valid || !same(r->fmt_q, qids) || !same(r->fmt_s, sids) || !same(r->fmt_f, fields)) {
+      r->fmt_valid = false;
+      const int rc = render(r, qids, sids, fields, &r->fmt_parts);
       if (rc != DMR_OK) return rc;
-      r->fmt_key = std::move(key);
+      r->fmt_q = qids ? qids : "";
+      r->fmt_s = sids ? sids : "";
+      r->fmt_f = fields ? fields : "";
+      r->fmt_len = 0;
+      for (const auto& pt : r->fmt_parts) r->fmt_len += pt.size();
+      r->fmt_valid = true;
     }
-    const std::string& out = r->fmt_text;
-    *len = out.size();
-    if (buf && cap >= out.size()) std::memcpy(buf, out.data(), out.size());
-    else if (buf && cap < out.size()) return fail(DMR_E_CAPACITY, "buffer too small");
+    *len = r->fmt_len;
+    if (buf && cap >= r->fmt_len) {
+      char* o = buf;
+      for (const auto& pt : r->fmt_parts) {
+        std::memcpy(o, pt.data(), pt.size());
+        o += pt.size();
+      }
+    } else if (buf && cap < r->fmt_len) {
+      return fail(DMR_E_CAPACITY, "buffer too small");
+    }
     return DMR_OK;
   });
 }
