@@ -82,3 +82,27 @@ def test_blastx_reads(dmr, ctx, orc, synth, sens):
     exp = ref.format(w.read_ids, w.db_ids, EXTENDED)
     assert "\t-" in exp and "\t+" in exp  # hits on both strands
     assert res.format(w.read_ids, w.db_ids, EXTENDED) == exp
+
+
+def test_results_keep_their_ops(dmr, synth):
+    """A result's alignment ops live in a pinned block that the result holds:
+    later searches on the same context, results being destroyed (their blocks
+    reused) and the context itself going away leave every live result's text
+    and ops unchanged."""
+    w = workload_dict(synth.make("tiny"))
+    c = dmr.Context(0)
+    c.db_load(w["db_letters"], w["db_offsets"])
+    run = lambda sens: c.blastp(w["q_letters"], w["q_offsets"], dmr.SearchConfig(sensitivity=sens))
+    fmt = lambda r: r.format(w["q_ids"], w["db_ids"], EXTENDED)
+    a, b = run("default"), run("sensitive")
+    assert len(a) > 0 and len(b) > 0
+    text_b = fmt(b)
+    ops_b = [list(b.ops(i)) for i in range(len(b))]
+    del a  # its block goes back to the pool; the next search takes it
+    c2 = run("default")
+    text_c = fmt(c2)
+    assert fmt(b) == text_b
+    c.close()
+    assert fmt(b) == text_b and fmt(c2) == text_c
+    assert [list(b.ops(i)) for i in range(len(b))] == ops_b
+    assert all(len(o) > 0 for o in ops_b)
