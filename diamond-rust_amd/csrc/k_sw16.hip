@@ -663,8 +663,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 }
 
 // Packed class of a band width: 0-4 the widths 2 * band + 1 of the band_for
-// values (25/33/61/81/129), 5-6 merged bands of widths 130..193 and
-// 194..256 (kernels with per-DP walls); -1 for any other width.
+// values (25/33/61/81/129), 5-6 merged bands of widths 62..193 (other than
+// 81 and 129) and 194..256 (kernels with per-DP walls); -1 for any other
+// width.
 __device__ __forceinline__ int packed_class(int w) {
   switch (w) {
     case 25: return 0;
@@ -672,7 +673,7 @@ __device__ __forceinline__ int packed_class(int w) {
     case 61: return 2;
     case 81: return 3;
     case 129: return 4;
-    default: return w > 129 && w <= 193 ? 5 : (w > 193 && w <= 256 ? 6 : -1);
+    default: return w > 61 && w <= 193 ? 5 : (w > 193 && w <= 256 ? 6 : -1);
   }
 }
 

@@ -213,7 +213,7 @@ int dmr_result_stats(const dmr_result* r, double* out20);
 int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
 /* Score-pass profile of the last search on ctx, per kernel class
  * (DMR_SCORE_CLASSES entries): 0-4 packed 16-bit kernel for band widths
- * 25/33/61/81/129, 5-6 packed kernel for merged bands of widths 130..193 and
+ * 25/33/61/81/129, 5-6 packed kernel for merged bands of widths 62..193 and
  * 194..256, 7-12 wave-per-DP int32 kernel for widths up to
  * 64/128/256/512/1024/2048, 13 row-exact int32 kernel (wider bands, and
  * bands that end before their clamped rows): DP jobs, DP cells and kernel

@@ -79,14 +79,16 @@ def test_scores_around_f16_limit(layout_ctx, orc):
 
 
 def test_merged_band_widths(layout_ctx, orc):
-    """Bands of widths 130..256 (merged bands) run the split kernels
-    with a wall at each DP's own width (classes m193/m256); 129 and 257+ stay
-    on their own kernels. Every width near a class edge is included."""
+    """Bands of widths 62..256 other than 81 and 129 (merged bands) run the
+    split kernels with a wall at each DP's own width (classes m193/m256);
+    61, 81, 129 and 257+ stay on their own kernels. Every width near a class
+    edge is included."""
     rng = np.random.default_rng(91)
-    edges = [129, 130, 131, 192, 193, 194, 195, 255, 256, 257, 258]
+    edges = [60, 61, 62, 63, 64, 65, 80, 81, 82, 100, 127, 128, 129, 130, 131, 192, 193, 194, 195, 255,
+             256, 257, 258]
     cases = []
-    for k in range(360):
-        w = edges[k] if k < len(edges) else int(rng.integers(130, 257))
+    for k in range(420):
+        w = edges[k] if k < len(edges) else int(rng.integers(62, 257))
         n = int(rng.integers(1, 1000 if rng.random() < 0.2 else 400))
         q = rand_seq(rng, n, canonical_only=rng.random() < 0.9)
         r = rng.random()
@@ -96,6 +98,7 @@ def test_merged_band_widths(layout_ctx, orc):
             s = rand_seq(rng, 2)
         d0 = int(rng.integers(-len(s) - w, n + 1))
         cases.append((q, s, d0, d0 + w - 1))
+    assert sum(61 < c[3] - c[2] + 1 <= 128 for c in cases) > 50
     assert sum(129 < c[3] - c[2] + 1 <= 193 for c in cases) > 50
     assert sum(193 < c[3] - c[2] + 1 <= 256 for c in cases) > 50
     got = _score_batch(layout_ctx, cases)
