@@ -73,7 +73,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
-              "(the multi-GPU split is described in DESIGN.md, not run).", file=sys.stderr)
+              "(the multi-GPU split is a design only, not run).", file=sys.stderr)
         sys.exit(2)
 
     dmr = load("diamond_rust_amd", ROOT / "diamond-rust_amd" / "__init__.py", pkg=True)
