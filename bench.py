@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported "
               "(the multi-GPU split is a design only, not run).", file=sys.stderr)
         sys.exit(2)
 
