@@ -28,7 +28,7 @@ extern "C" {
 #define DMR_E_DEVICE (-2)      /* HIP runtime failure */
 #define DMR_E_OOM (-3)         /* device allocation failed */
 #define DMR_E_CAPACITY (-4)    /* output buffer too small; *n_out holds the size needed */
-#define DMR_E_UNSUPPORTED (-5) /* input outside what this build handles */
+#define DMR_E_UNSUPPORTED (-5) /* input outside what the library handles */
 
 typedef struct dmr_ctx dmr_ctx;
 typedef struct dmr_result dmr_result;

@@ -3,7 +3,7 @@
 // Each function restates the behaviour of the reference function cited above
 // it (paths are relative to the reference repository). The product never
 // links this file; tests, the smoke check and the CPU-baseline leg of the
-// benchmark load it through the C API at the bottom as the checker.
+// benchmark load it through the C API at the bottom, to compare results with.
 #include "oracle.hpp"
 
 #include <algorithm>

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. Nothing in the product links, includes or calls
 // this code; only tests/, the smoke check and bench.py's CPU-baseline leg do,
-// and only as the checker. Every function cites the reference file:line whose
+// and only to compare results with. Every function cites the reference file:line whose
 // behaviour it restates (paths relative to the reference repository).
 //
 // Parity unpinned (nothing in the reference's sources or tests fixes them):

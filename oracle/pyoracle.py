@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the CPU
-baseline of bench.py as the checker, never by the product.
+baseline of bench.py, to compare results with; never by the product.
 """
 from __future__ import annotations
 
