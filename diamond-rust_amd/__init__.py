@@ -31,7 +31,7 @@ EXPORTS = [
     "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend",
-    "dmr_extend_span_batch", "dmr_db_load", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
+    "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
     "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
     "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy", "dmr_score_layout",
@@ -154,6 +154,8 @@ def lib():
             "dmr_extend_span_batch": ([P, P, U64, P, U64, P, U64, I32, I32, C.c_int, P, P, U64, P, P],
                                       C.c_int),
             "dmr_db_load": ([P, P, P, U32], C.c_int),
+            "dmr_db_load_file": ([P, C.c_char_p, P], C.c_int),
+            "dmr_db_ids": ([P, P, U64, P], C.c_int),
             "dmr_db_build_index": ([P, C.c_char_p, I32, D, P], C.c_int),
             "dmr_blastp": ([P, P, P, U32, P, P], C.c_int),
             "dmr_blastx": ([P, P, P, U32, P, P], C.c_int),
@@ -279,6 +281,23 @@ class Context:
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         self._db = (letters, offsets)
         _check(lib().dmr_db_load(self._h, _ptr(letters), offsets.ctypes.data, len(offsets) - 1))
+
+    def db_load_file(self, path) -> int:
+        """Database::open of a DRDMND01 file (src/db.rs:159-224); returns its sequence count."""
+        n = C.c_uint32()
+        _check(lib().dmr_db_load_file(self._h, os.fsencode(path), C.byref(n)))
+        self._db = None
+        return n.value
+
+    def db_ids(self) -> str:
+        """Ids of the file-loaded database, one per line (each followed by a newline)."""
+        need = C.c_uint64()
+        rc = lib().dmr_db_ids(self._h, None, 0, C.byref(need))
+        if rc not in (DMR_OK, DMR_E_CAPACITY):
+            _check(rc)
+        buf = C.create_string_buffer(max(1, need.value))
+        _check(lib().dmr_db_ids(self._h, buf, need.value, C.byref(need)))
+        return buf.raw[:need.value].decode()
 
     def build_index(self, sensitivity="default", index_chunks=-1, seed_cut=-1.0) -> float:
         ms = C.c_double()

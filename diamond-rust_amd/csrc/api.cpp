@@ -4,7 +4,13 @@
 
 #include "host_pool.hpp"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cerrno>
 #include <charconv>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +26,9 @@ struct dmr_ctx {
   dmr::TargetSet loaded;  // resident query batch (dmr_queries_load)
   uint32_t loaded_nq = 0;
   bool has_loaded = false;
+  // Ids of the resident database, one per line (dmr_db_load_file only).
+  std::string db_ids;
+  bool db_has_ids = false;
 };
 
 struct dmr_result {
@@ -63,6 +72,29 @@ int guard(F&& f) {
   } catch (const std::exception& e) {
     return fail(DMR_E_DEVICE, e.what());
   }
+}
+
+// Index of the first letter code outside the 25-letter alphabet
+// (src/alphabet.rs:15-16), n when there is none.
+uint64_t first_bad_letter(const uint8_t* p, uint64_t n) {
+  constexpr uint64_t kBlock = 4096;
+  for (uint64_t b = 0; b < n; b += kBlock) {
+    const uint64_t e = std::min(n, b + kBlock);
+    uint8_t m = 0;
+    for (uint64_t i = b; i < e; ++i) m = std::max(m, p[i]);
+    if (m >= dmr::kAlphabet)
+      for (uint64_t i = b; i < e; ++i)
+        if (p[i] >= dmr::kAlphabet) return i;
+  }
+  return n;
+}
+
+void check_letters(const uint8_t* p, uint64_t n, const char* what) {
+  const uint64_t i = n && p ? first_bad_letter(p, n) : n;
+  if (i < n)
+    throw dmr::Error(DMR_E_ARG, std::string(what) + ": letter code " + std::to_string(p[i]) +
+                                    " at position " + std::to_string(i) +
+                                    " is outside the 25-letter alphabet");
 }
 
 using dmr::HspRec;
@@ -223,6 +255,7 @@ dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets
     uint64_t a = q_offsets[i], b = q_offsets[i + 1];
     if (b < a) throw dmr::Error(DMR_E_ARG, "query offsets are not monotone");
     if (b == a) continue;  // empty queries have no target (src/workflow.rs:129-132)
+    check_letters(queries + a, b - a, ("query " + std::to_string(i)).c_str());
     t.letters.insert(t.letters.end(), queries + a, queries + b);
     t.offs.push_back((uint32_t)t.letters.size());
     t.query_of.push_back(i);
@@ -354,6 +387,7 @@ int dmr_seed_build(dmr_ctx* ctx, const uint8_t* seqs, const uint64_t* offsets, u
   return guard([&]() -> int {
     std::vector<uint64_t> k;
     std::vector<uint32_t> s, p;
+    if (nseq) check_letters(seqs, offsets[nseq], "sequences");
     ctx->eng->seed_build(seqs, offsets, nseq, sh, red, base, complexity_cut, &k, &s, &p);
     *n_out = k.size();
     if (k.size() > cap) return fail(DMR_E_CAPACITY, "output capacity too small");
@@ -388,6 +422,8 @@ int dmr_xdrop_batch(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uin
   if (!ctx || (n && (!qa || !sa || !out))) return fail(DMR_E_ARG, "NULL argument");
   return guard([&]() -> int {
     std::vector<dmr::SegRec> s(n);
+    check_letters(query, qlen, "query");
+    check_letters(subject, slen, "subject");
     ctx->eng->xdrop_batch(query, qlen, subject, slen, qa, sa, n, xdrop, s.data());
     for (uint64_t i = 0; i < n; ++i) out[i] = dmr_segment{s[i].q_start, s[i].s_start, s[i].len, s[i].score, 0};
     return DMR_OK;
@@ -408,6 +444,8 @@ int dmr_extend_span_batch(dmr_ctx* ctx, const uint8_t* qbuf, uint64_t qbuf_len, 
   dmr::GapRow gr;
   if (!dmr::blosum62_gap_row(go, ge, &gr)) return fail(DMR_E_ARG, "unsupported gap costs");
   return guard([&]() -> int {
+    check_letters(qbuf, qbuf_len, "query buffer");
+    check_letters(sbuf, sbuf_len, "subject buffer");
     std::vector<dmr::Engine::SpanJob> sj(njobs);
     for (uint64_t i = 0; i < njobs; ++i)
       sj[i] = {jobs[i].q_off, jobs[i].q_len, jobs[i].s_off, jobs[i].s_len, jobs[i].d0, jobs[i].d1};
@@ -461,12 +499,205 @@ int dmr_extend(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t*
 
 int dmr_db_load(dmr_ctx* ctx, const uint8_t* letters, const uint64_t* offsets, uint32_t nseq) {
   if (!ctx || !offsets || (offsets[nseq] && !letters)) return fail(DMR_E_ARG, "NULL argument");
+  if (offsets[0] != 0) return fail(DMR_E_ARG, "offsets[0] must be 0");
   for (uint32_t i = 0; i < nseq; ++i)
     if (offsets[i + 1] < offsets[i]) return fail(DMR_E_ARG, "offsets are not monotone");
   return guard([&]() -> int {
+    check_letters(letters, offsets[nseq], "database");
     ctx->eng->load_db(letters, offsets, nseq);
+    ctx->db_ids.clear();
+    ctx->db_has_ids = false;
     return DMR_OK;
   });
+}
+
+namespace {
+
+// DRDMND01 layout (src/db.rs:24-29, 117-131): a 72-byte header (magic,
+// version u32, flags u32, then seq_count, total_letters, defs_len,
+// seq_off_start, def_off_start, letters_start, defs_start as u64), then
+// seq_count + 1 u64 letter offsets, seq_count + 1 u64 header offsets, the
+// encoded letters and the newline-free FASTA headers. Little-endian.
+constexpr size_t kDbHeaderLen = 72;
+constexpr uint32_t kDbVersion = 1;
+
+uint64_t le64(const uint8_t* p) {
+  uint64_t v;
+  std::memcpy(&v, p, 8);
+  return v;
+}
+
+bool path_exists(const std::string& p) {
+  struct stat st;
+  return ::stat(p.c_str(), &st) == 0;
+}
+
+// db::resolve (src/db.rs:39-60): the path as given, else with ".dmnd"
+// appended when its file name has no extension.
+bool resolve_db_path(const std::string& path, std::string* out) {
+  if (path_exists(path)) {
+    *out = path;
+    return true;
+  }
+  std::string name = path;
+  while (name.size() > 1 && name.back() == '/') name.pop_back();
+  const size_t slash = name.rfind('/');
+  if (slash != std::string::npos) name = name.substr(slash + 1);
+  const size_t dot = name.rfind('.');
+  const bool has_ext = name != ".." && dot != std::string::npos && dot > 0;
+  if (has_ext) return false;
+  *out = path + ".dmnd";
+  return path_exists(*out);
+}
+
+// std::str::from_utf8 acceptance: well-formed UTF-8, no overlong forms,
+// surrogates or code points above U+10FFFF.
+bool valid_utf8(const uint8_t* s, size_t n) {
+  size_t i = 0;
+  while (i < n) {
+    const uint8_t c = s[i];
+    if (c < 0x80) {
+      ++i;
+      continue;
+    }
+    size_t len;
+    uint32_t cp;
+    if (c >= 0xC2 && c <= 0xDF) {
+      len = 2;
+      cp = c & 0x1F;
+    } else if (c >= 0xE0 && c <= 0xEF) {
+      len = 3;
+      cp = c & 0x0F;
+    } else if (c >= 0xF0 && c <= 0xF4) {
+      len = 4;
+      cp = c & 0x07;
+    } else {
+      return false;
+    }
+    if (n - i < len) return false;
+    for (size_t k = 1; k < len; ++k) {
+      if ((s[i + k] & 0xC0) != 0x80) return false;
+      cp = cp << 6 | (s[i + k] & 0x3F);
+    }
+    if (len == 3 && (cp < 0x800 || (cp >= 0xD800 && cp <= 0xDFFF))) return false;
+    if (len == 4 && (cp < 0x10000 || cp > 0x10FFFF)) return false;
+    i += len;
+  }
+  return true;
+}
+
+// Database::id (src/db.rs:262-265) over alphabet::seq_id (src/alphabet.rs:87-93):
+// the header (empty when it is not UTF-8) without leading '>' and ' ', up to
+// the first ASCII whitespace.
+void append_seq_id(const uint8_t* h, size_t n, std::string* out) {
+  if (valid_utf8(h, n)) {
+    size_t a = 0;
+    while (a < n && (h[a] == '>' || h[a] == ' ')) ++a;
+    size_t b = a;
+    while (b < n && h[b] != ' ' && h[b] != '\t' && h[b] != '\n' && h[b] != '\f' && h[b] != '\r') ++b;
+    out->append(reinterpret_cast<const char*>(h + a), b - a);
+  }
+  out->push_back('\n');
+}
+
+// Read-only mapping of a whole file, unmapped on scope exit.
+struct Mapped {
+  const uint8_t* p = nullptr;
+  size_t n = 0;
+  ~Mapped() {
+    if (p && n) ::munmap(const_cast<uint8_t*>(p), n);
+  }
+};
+
+}  // namespace
+
+int dmr_db_load_file(dmr_ctx* ctx, const char* path, uint32_t* nseq) {
+  if (!ctx || !path) return fail(DMR_E_ARG, "NULL argument");
+  std::string file;
+  if (!resolve_db_path(path, &file)) return fail(DMR_E_ARG, std::string("Database file not found: ") + path);
+  return guard([&]() -> int {
+    Mapped m;
+    {
+      const int fd = ::open(file.c_str(), O_RDONLY);
+      struct stat st;
+      if (fd < 0 || ::fstat(fd, &st) != 0) {
+        const std::string why = std::strerror(errno);
+        if (fd >= 0) ::close(fd);
+        return fail(DMR_E_ARG, "cannot read " + file + ": " + why);
+      }
+      m.n = (size_t)st.st_size;
+      if (m.n) {
+        void* a = ::mmap(nullptr, m.n, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (a == MAP_FAILED) {
+          const std::string why = std::strerror(errno);
+          ::close(fd);
+          m.n = 0;
+          return fail(DMR_E_ARG, "cannot read " + file + ": " + why);
+        }
+        m.p = static_cast<const uint8_t*>(a);
+      }
+      ::close(fd);
+    }
+    const uint8_t* d = m.p;
+    const uint64_t len = m.n;
+    // Database::open (src/db.rs:172-222).
+    if (len < kDbHeaderLen) return fail(DMR_E_ARG, file + ": not a diamond-rust database (file too short)");
+    if (std::memcmp(d, "DRDMND01", 8) != 0)
+      return fail(DMR_E_ARG, file + ": not a diamond-rust database (bad magic). Build one with `diamond-rust makedb`.");
+    uint32_t version;
+    std::memcpy(&version, d + 8, 4);
+    if (version != kDbVersion)
+      return fail(DMR_E_ARG, file + ": database version " + std::to_string(version) +
+                                 " is not supported (expected " + std::to_string(kDbVersion) + ")");
+    const uint64_t n = le64(d + 16), defs_len = le64(d + 32);
+    const uint64_t seq_off_start = le64(d + 40), def_off_start = le64(d + 48);
+    const uint64_t letters_start = le64(d + 56), defs_start = le64(d + 64);
+    if (defs_start > len || len - defs_start != defs_len)
+      return fail(DMR_E_ARG, file + ": truncated or corrupt database");
+    auto table_fits = [&](uint64_t start) { return n < len / 8 && start <= len && (n + 1) * 8 <= len - start; };
+    if (!table_fits(seq_off_start) || !table_fits(def_off_start))
+      return fail(DMR_E_ARG, file + ": corrupt offset table");
+    // Offsets the reference reads only when a sequence or header is used
+    // (Database::seq / header, src/db.rs:243-259) are checked here once:
+    // monotone and inside the file.
+    if (n > UINT32_MAX) return fail(DMR_E_UNSUPPORTED, file + ": more than 2^32 - 1 sequences");
+    std::vector<uint64_t> so(n + 1);
+    std::memcpy(so.data(), d + seq_off_start, (n + 1) * 8);
+    const uint8_t* dofs = d + def_off_start;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (so[i + 1] < so[i] || le64(dofs + 8 * (i + 1)) < le64(dofs + 8 * i))
+        return fail(DMR_E_ARG, file + ": corrupt offset table");
+    }
+    if (letters_start > len || so[n] > len - letters_start || le64(dofs + 8 * n) > len - defs_start)
+      return fail(DMR_E_ARG, file + ": corrupt offset table");
+    if (so[0] != 0) return fail(DMR_E_ARG, file + ": corrupt offset table");
+    const uint8_t* letters = d + letters_start;
+    const uint64_t bad = first_bad_letter(letters, so[n]);
+    if (bad < so[n])
+      return fail(DMR_E_ARG, file + ": corrupt letters section (code " + std::to_string(letters[bad]) +
+                                 " at letter " + std::to_string(bad) + " is outside the 25-letter alphabet)");
+    std::string ids;
+    ids.reserve(n * 16);
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t a = le64(dofs + 8 * i), b = le64(dofs + 8 * (i + 1));
+      append_seq_id(d + defs_start + a, b - a, &ids);
+    }
+    ctx->eng->load_db(letters, so.data(), (uint32_t)n);
+    ctx->db_ids = std::move(ids);
+    ctx->db_has_ids = true;
+    if (nseq) *nseq = (uint32_t)n;
+    return DMR_OK;
+  });
+}
+
+int dmr_db_ids(const dmr_ctx* ctx, char* buf, uint64_t cap, uint64_t* len) {
+  if (!ctx || !len) return fail(DMR_E_ARG, "NULL argument");
+  if (!ctx->db_has_ids) return fail(DMR_E_ARG, "the resident database was not loaded from a file");
+  *len = ctx->db_ids.size();
+  if (cap < ctx->db_ids.size() || (ctx->db_ids.size() && !buf))
+    return fail(DMR_E_CAPACITY, "output capacity too small");
+  if (!ctx->db_ids.empty()) std::memcpy(buf, ctx->db_ids.data(), ctx->db_ids.size());
+  return DMR_OK;
 }
 
 int dmr_db_build_index(dmr_ctx* ctx, const char* sens, int32_t index_chunks, double seed_cut,

@@ -160,6 +160,21 @@ int dmr_extend_span_batch(dmr_ctx* ctx, const uint8_t* qbuf, uint64_t qbuf_len,
  * encoded letters and nseq + 1 offsets. Replaces any previous database. */
 int dmr_db_load(dmr_ctx* ctx, const uint8_t* letters, const uint64_t* offsets, uint32_t nseq);
 
+/* Database::open (src/db.rs:159-224, with db::resolve, src/db.rs:39-60) of a
+ * DRDMND01 file made by `diamond-rust makedb`, loaded as the resident
+ * database as by dmr_db_load; *nseq (may be NULL) receives its sequence
+ * count. Errors carry the reference's messages (file not found, file too
+ * short, bad magic, unsupported version, truncated, corrupt offset table);
+ * offsets and letter codes the reference reads only on use are checked at
+ * load. On error the previous database stays resident. */
+int dmr_db_load_file(dmr_ctx* ctx, const char* path, uint32_t* nseq);
+
+/* Database::id of every sequence of the database that dmr_db_load_file
+ * loaded (src/db.rs:262-265, alphabet::seq_id src/alphabet.rs:87-93), each
+ * followed by '\n': the subject_ids of dmr_result_format. *len receives the
+ * byte length; the text (not NUL-terminated) is written when cap >= *len. */
+int dmr_db_ids(const dmr_ctx* ctx, char* buf, uint64_t cap, uint64_t* len);
+
 /* Builds (or keeps) the resident seed index for a sensitivity preset:
  * seed::chunks(n, index_chunks) slices x all shapes (src/search.rs:121-133). */
 int dmr_db_build_index(dmr_ctx* ctx, const char* sensitivity, int32_t index_chunks,
