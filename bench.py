@@ -60,6 +60,39 @@ def hbm_traffic(kernel: str, workload: str):
     return d.get("kernels", {}).get(kernel)
 
 
+def as_fasta(letters, offsets, ids, width=60):
+    """FASTA text of encoded sequences (NCBI-order symbols, 60 per line)."""
+    sym = np.frombuffer(b"ARNDCQEGHILKMFPSTWYVBJZX*", np.uint8)[letters].tobytes()
+    off = offsets.astype(np.int64)
+    parts = []
+    for k, rid in enumerate(ids):
+        s = sym[off[k]:off[k + 1]]
+        parts.append(b">" + rid.encode() + b" synthetic\n" +
+                     b"".join(s[i:i + width] + b"\n" for i in range(0, len(s), width)))
+    return b"".join(parts)
+
+
+def fasta_ingress(dmr, w):
+    """dmr_fasta_parse + encode (host) on the workload's queries and database
+    written as FASTA; not part of the timed step."""
+    res = {}
+    for name, letters, offsets, ids in (("queries", w.q_letters, w.q_offsets, w.q_ids),
+                                        ("database", w.db_letters, w.db_offsets, w.db_ids)):
+        text = as_fasta(letters, offsets, ids)
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            f = dmr.parse_fasta(text)
+            times.append(time.perf_counter() - t0)
+        same = (np.array_equal(f.letters, letters) and np.array_equal(f.offsets, offsets)
+                and f.ids == list(ids))
+        ms = float(np.median(times)) * 1e3
+        res[name] = {"bytes": len(text), "ms": round(ms, 3), "mb_per_s": round(len(text) / ms / 1e3, 1),
+                     "matches_workload": bool(same)}
+    log(f"[bench] FASTA ingress: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +198,8 @@ def main():
             "basis": "all banded-DP cells x 11 16-bit ops / score-pass wall time (all classes, "
                      "concurrent streams joined)"},
     }
+
+    out["fasta_ingress"] = fasta_ingress(dmr, w)
 
     if not a.no_plain and a.plain_sample > 0:
         sub = w.subset_queries(list(range(a.plain_sample)))

@@ -35,6 +35,7 @@ EXPORTS = [
     "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
     "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy", "dmr_score_layout",
+    "dmr_fasta_parse", "dmr_fasta_sizes", "dmr_fasta_residues", "dmr_fasta_text", "dmr_fasta_destroy",
 ]
 
 # Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
@@ -169,6 +170,11 @@ def lib():
             "dmr_last_class_profile": ([P, P, P, P], C.c_int),
             "dmr_score_layout": ([P, P, P], C.c_int),
             "dmr_result_destroy": ([P], None), "dmr_search_config_default": ([P], None),
+            "dmr_fasta_parse": ([C.c_char_p, U64, P], C.c_int),
+            "dmr_fasta_sizes": ([P, P, P, P, P, P], C.c_int),
+            "dmr_fasta_residues": ([P, P, P, P], C.c_int),
+            "dmr_fasta_text": ([P, P, P], C.c_int),
+            "dmr_fasta_destroy": ([P], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -206,6 +212,41 @@ def pack(seqs):
         np.cumsum([len(s) for s in seqs], out=off[1:])
     letters = np.concatenate(seqs) if seqs else np.zeros(0, np.uint8)
     return np.ascontiguousarray(letters, dtype=np.uint8), off
+
+
+@dataclasses.dataclass
+class FastaRecords:
+    """io::parse_fasta output (src/io.rs:40-102) in the library's layout:
+    raw residues (upper-cased, layout characters dropped), the same encoded
+    (alphabet::encode), n + 1 offsets into either, ids and headers."""
+    raw: bytes
+    letters: np.ndarray
+    offsets: np.ndarray
+    ids: list
+    headers: list
+    looks_like_dna: bool
+
+
+def parse_fasta(data: bytes) -> FastaRecords:
+    """dmr_fasta_parse of FASTA text held in memory (host only, no device)."""
+    L = lib()
+    h = C.c_void_p()
+    _check(L.dmr_fasta_parse(data, len(data), C.byref(h)))
+    try:
+        nrec, nres, nid, nhd, dna = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_int()
+        _check(L.dmr_fasta_sizes(h, C.byref(nrec), C.byref(nres), C.byref(nid), C.byref(nhd), C.byref(dna)))
+        raw = C.create_string_buffer(max(1, nres.value))
+        letters = np.zeros(nres.value, np.uint8)
+        offsets = np.zeros(nrec.value + 1, np.uint64)
+        _check(L.dmr_fasta_residues(h, raw, _ptr(letters), offsets.ctypes.data))
+        ids = C.create_string_buffer(max(1, nid.value))
+        heads = C.create_string_buffer(max(1, nhd.value))
+        _check(L.dmr_fasta_text(h, ids, heads))
+    finally:
+        L.dmr_fasta_destroy(h)
+    split = lambda b, n: b.raw[:n].decode().split("\n")[:-1] if n else []
+    return FastaRecords(raw.raw[:nres.value], letters, offsets, split(ids, nid.value),
+                        split(heads, nhd.value), bool(dna.value))
 
 
 class Result:

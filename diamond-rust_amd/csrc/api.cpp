@@ -324,6 +324,10 @@ void set_aligned(const dmr::SwOut& o, dmr_aligned* a, int64_t q_shift) {
 
 }  // namespace
 
+namespace dmr {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace dmr
+
 extern "C" {
 
 const char* dmr_last_error(void) { return g_err.c_str(); }

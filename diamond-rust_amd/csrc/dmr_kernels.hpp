@@ -111,8 +111,8 @@ void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, Se
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st);
 // Packed score pass of width class cls (k_sw16.hip): split = sw16_split_lanes
 // lanes per DP pair (part bands each) for classes 0-4; classes 5-6 (merged bands)
-// always run split (sw16_split_lanes: 16 lanes per pair for widths to 193,
-// 8 above) with per-DP walls; f16 = half-precision
+// always run split (sw16_split_lanes: 8 lanes per pair) with per-DP walls;
+// f16 = half-precision
 // arithmetic (pair_tab must then be the f16 table).
 void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
                  SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,

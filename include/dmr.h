@@ -235,7 +235,7 @@ int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
  * milliseconds. Diagnostics only; no counterpart in the reference. */
 #define DMR_SCORE_CLASSES 14
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
-/* Layout of the packed classes 0-4 on ctx (5-6 always split, 16/8 lanes):
+/* Layout of the packed classes 0-4 on ctx (5-6 always split, 8 lanes):
  * bit c of *split_mask set = class c runs split over several lanes per DP
  * pair (k_sw16p: two for width 81, four otherwise), clear = one lane
  * (k_sw16); *f16 = 1 when the packed kernels compute in f16 pairs, 0 in
@@ -243,6 +243,31 @@ int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double
  * counterpart in the reference. */
 int dmr_score_layout(dmr_ctx* ctx, uint32_t* split_mask, uint32_t* f16);
 void dmr_result_destroy(dmr_result* r);
+
+/* ---- FASTA ingress (host) -------------------------------------------------
+ * io::parse_fasta (src/io.rs:40-102) of a FASTA text held in memory: one
+ * record per '>' line, the header from that line (lossy UTF-8, trimmed),
+ * Record::id (its first ASCII-whitespace token, "seq<len>" when empty), and
+ * the residues of the lines up to the next header, upper-cased, with ASCII
+ * whitespace, '-' and '.' dropped. Lines before the first header may be
+ * empty or start with ';' or '#'. Fails with DMR_E_ARG and the reference's
+ * message for sequence data before the first header and for non-empty text
+ * without a header. Needs no context or device. */
+typedef struct dmr_fasta dmr_fasta;
+int dmr_fasta_parse(const char* data, uint64_t len, dmr_fasta** out);
+/* Record count, residue count, byte lengths of the ids and headers texts,
+ * and io::looks_like_dna (src/io.rs:104-121). Any may be NULL. */
+int dmr_fasta_sizes(const dmr_fasta* f, uint64_t* nrec, uint64_t* nres, uint64_t* ids_len,
+                    uint64_t* headers_len, int* looks_like_dna);
+/* Residues as parsed (raw: the blastx reads of dmr_blastx), encoded with
+ * alphabet::encode (letters: src/alphabet.rs:54-63, the blastp queries of
+ * dmr_blastp and the database of dmr_db_load), and nrec + 1 offsets into
+ * either. Buffers hold nres bytes and nrec + 1 offsets; any may be NULL. */
+int dmr_fasta_residues(const dmr_fasta* f, char* raw, uint8_t* letters, uint64_t* offsets);
+/* The ids and headers, each followed by '\n' (the query_ids and subject_ids
+ * of dmr_result_format), not NUL-terminated; either may be NULL. */
+int dmr_fasta_text(const dmr_fasta* f, char* ids, char* headers);
+void dmr_fasta_destroy(dmr_fasta* f);
 
 #ifdef __cplusplus
 }

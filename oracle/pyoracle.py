@@ -266,3 +266,70 @@ def search(db_letters, db_offsets, q_letters, q_offsets, sensitivity="default", 
                          len(qo) - 1, sensitivity.encode(), threads, evalue, max_target_seqs,
                          max_hsps, index_chunks, seed_cut, 1 if translated else 0, min_orf)
     return SearchResult(h)
+
+
+# ---- FASTA ingress (pure Python) -------------------------------------------
+
+# char::is_whitespace, which str::trim uses: the Unicode White_Space property
+# (PropList.txt). Python's str.strip differs (it also strips U+001C..U+001F).
+WHITE_SPACE = frozenset([*range(0x09, 0x0E), 0x20, 0x85, 0xA0, 0x1680, *range(0x2000, 0x200B), 0x2028,
+                         0x2029, 0x202F, 0x205F, 0x3000])
+_UPPER = bytes.maketrans(b"abcdefghijklmnopqrstuvwxyz", b"ABCDEFGHIJKLMNOPQRSTUVWXYZ")
+_LAYOUT = b" \t\n\x0c\r-."  # u8::is_ascii_whitespace, '-' and '.'
+
+
+def rust_trim(s: str) -> str:
+    a, b = 0, len(s)
+    while a < b and ord(s[a]) in WHITE_SPACE:
+        a += 1
+    while b > a and ord(s[b - 1]) in WHITE_SPACE:
+        b -= 1
+    return s[a:b]
+
+
+def _finish(header: str, seq: bytearray):
+    """io::finish (src/io.rs:76-94): id = first ASCII-whitespace token, else seq<len>."""
+    n = 0
+    while n < len(header) and header[n] not in " \t\n\x0c\r":
+        n += 1
+    rid = header[:n] if n else f"seq{max(len(seq), 1)}"
+    return header, rid, bytes(seq)
+
+
+def parse_fasta(data: bytes):
+    """io::parse_fasta (src/io.rs:40-74) with split_lines (src/io.rs:96-102):
+    a list of (header, id, raw residues); raises ValueError with the
+    reference's message. from_utf8_lossy is Python's 'replace' decoding (both
+    substitute maximal invalid subparts)."""
+    out = []
+    header, seq, seen = None, bytearray(), False
+    for line in data.split(b"\n"):
+        if line.endswith(b"\r"):
+            line = line[:-1]
+        if line[:1] == b">":
+            if header is not None:
+                out.append(_finish(header, seq))
+                seq = bytearray()
+            seen = True
+            header = rust_trim(line[1:].decode("utf-8", "replace"))
+        elif header is not None:
+            seq += line.translate(_UPPER, _LAYOUT)
+        elif line and line[:1] not in (b";", b"#"):
+            raise ValueError("malformed FASTA: sequence data found before the first '>' header")
+    if header is not None:
+        out.append(_finish(header, seq))
+    if not seen and data:
+        raise ValueError("malformed FASTA: no '>' header found")
+    return out
+
+
+def looks_like_dna(records) -> bool:
+    """io::looks_like_dna (src/io.rs:104-121) over parse_fasta's records."""
+    canonical = total = 0
+    for _, _, raw in records:
+        part = raw[:20000]
+        total += len(part)
+        canonical += sum(part.count(c) for c in b"ACGTUN")
+        if total >= 20000:
+            break
+    return total > 0 and canonical * 100 >= total * 85
