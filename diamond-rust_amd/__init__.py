@@ -249,6 +249,15 @@ def parse_fasta(data: bytes) -> FastaRecords:
                         split(heads, nhd.value), bool(dna.value))
 
 
+def ids_text(ids) -> bytes:
+    """Newline-separated id text for dmr_result_format from a list or a str/bytes."""
+    if isinstance(ids, bytes):
+        return ids
+    if isinstance(ids, str):
+        return ids.encode()
+    return "\n".join(ids).encode()
+
+
 class Result:
     def __init__(self, handle):
         self._h = C.c_void_p(handle)
@@ -282,8 +291,10 @@ class Result:
         return buf[: n.value].copy()
 
     def format(self, query_ids, subject_ids, fields="") -> str:
-        q = "\n".join(query_ids).encode()
-        s = "\n".join(subject_ids).encode()
+        """outfmt-6 text; ids as lists of str, or as one newline-separated
+        str/bytes (as dmr_db_ids gives them)."""
+        q = ids_text(query_ids)
+        s = ids_text(subject_ids)
         f = fields.encode() if isinstance(fields, str) else " ".join(fields).encode()
         n = C.c_uint64()
         _check(lib().dmr_result_format(self._h, q, s, f, None, 0, C.byref(n)))

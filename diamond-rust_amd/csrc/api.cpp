@@ -264,25 +264,106 @@ dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets
   return t;
 }
 
-// src/output.rs:235-241
+// {:.1} of 0 <= x < 100 by exact integer rounding of x * 10 (x = m / 2^s
+// exactly; round half to even on the exact value, as the exact decimal
+// formatting does). False outside that range.
+bool fixed1_fast(double x, char* b, char** end) {
+  if (!(x >= 0.0 && x < 100.0)) return false;
+  uint64_t q = 0;
+  if (x != 0.0) {
+    int e2;
+    const uint64_t m = (uint64_t)std::ldexp(std::frexp(x, &e2), 53);  // exact
+    const int sh = 53 - e2;
+    if (sh <= 0) {
+      q = (m << -sh) * 10;
+    } else if (sh <= 60) {  // otherwise x < 2^-7, below 0.05: q = 0
+      const uint64_t t = m * 10;  // < 2^57
+      q = t >> sh;
+      const uint64_t rem = t & ((1ull << sh) - 1), half = 1ull << (sh - 1);
+      if (rem > half || (rem == half && (q & 1))) ++q;
+    }
+  }
+  char* p = std::to_chars(b, b + 8, q / 10).ptr;
+  *p++ = '.';
+  *p++ = (char)('0' + q % 10);
+  *end = p;
+  return true;
+}
+
+// Correctly rounded 10^n, n in [0, 308].
+const double* pow10_table() {
+  static const std::vector<double> t = [] {
+    std::vector<double> v(309);
+    for (int n = 0; n <= 308; ++n) v[n] = std::strtod(("1e" + std::to_string(n)).c_str(), nullptr);
+    return v;
+  }();
+  return t.data();
+}
+
+// {:.2e} of e when its three significant digits are clear of a rounding tie
+// (and of the 100 / 1000 bounds) by far more than the error of e * 10^n
+// (two roundings); false otherwise, and the caller formats exactly.
+bool sci2_fast(double e, char* b, char** end) {
+  if (!(e >= 1e-290 && e < 1e290)) return false;
+  const double* p10 = pow10_table();
+  int e2;
+  std::frexp(e, &e2);  // e in [2^(e2-1), 2^e2)
+  int k = (int)std::floor((e2 - 1) * 0.30102999566398120);  // floor(log10 e) or one less
+  auto scaled = [&](int kk) { return 2 - kk >= 0 ? e * p10[2 - kk] : e / p10[kk - 2]; };
+  double s = scaled(k);
+  if (s >= 1000.0) s = scaled(++k);
+  constexpr double kMargin = 1e-6;
+  if (s < 100.0 + kMargin || s >= 1000.0 - kMargin) return false;
+  const double q = std::floor(s), frac = s - q;
+  if (std::fabs(frac - 0.5) < kMargin) return false;
+  unsigned d = (unsigned)q + (frac > 0.5 ? 1u : 0u);
+  if (d == 1000) {
+    d = 100;
+    ++k;
+  }
+  char* p = b;
+  *p++ = (char)('0' + d / 100);
+  *p++ = '.';
+  *p++ = (char)('0' + d / 10 % 10);
+  *p++ = (char)('0' + d % 10);
+  *p++ = 'e';
+  p = std::to_chars(p, p + 8, k).ptr;
+  *end = p;
+  return true;
+}
+
 // src/output.rs:235-242: values >= 100 print as their floor, others with one
 // decimal ({:.1}, the exact binary value rounded half to even, as printf's
 // %.1f; std::to_chars with a precision is specified to match printf).
-std::string fmt_double(double x) {
+void put_double(std::string& out, double x) {
   char b[64];
+  char* end;
+  if (fixed1_fast(x, b, &end)) {
+    out.append(b, end);
+    return;
+  }
   std::to_chars_result r;
   if (x >= 100.0) r = std::to_chars(b, b + sizeof b, (long long)std::floor(x));
   else r = std::to_chars(b, b + sizeof b, x, std::chars_format::fixed, 1);
-  return std::string(b, r.ptr);
+  out.append(b, r.ptr);
 }
+
 // src/output.rs:244-250 (Rust {:.2e}: no exponent sign padding)
-std::string fmt_evalue(double e) {
-  if (!std::isfinite(e) || e <= 0.0) return "0.0";
+void put_evalue(std::string& out, double e) {
+  if (!std::isfinite(e) || e <= 0.0) {
+    out += "0.0";
+    return;
+  }
   char b[64];
+  char* fe;
+  if (sci2_fast(e, b, &fe)) {
+    out.append(b, fe);
+    return;
+  }
   const auto r = std::to_chars(b, b + sizeof b, e, std::chars_format::scientific, 2);
   const char* end = r.ptr;
   const char* p = std::find(static_cast<const char*>(b), end, 'e');
-  std::string out(b, (size_t)(p - b));
+  out.append(b, (size_t)(p - b));
   out += 'e';
   const char* x = p + 1;
   const bool neg = *x == '-';
@@ -290,7 +371,6 @@ std::string fmt_evalue(double e) {
   while (x + 1 < end && *x == '0') ++x;
   if (neg) out += '-';
   out.append(x, (size_t)(end - x));
-  return out;
 }
 
 std::vector<std::string> split(const char* s, char sep) {
@@ -914,10 +994,11 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
   for (size_t i = 0; i < r->hsps.size(); ++i)
     if (r->target_of[i] >= 0 && (r->hsps[i].query >= qid.size() || r->hsps[i].subject >= sid.size()))
       return fail(DMR_E_ARG, "id list shorter than the query or database");
-  // Lines are independent: one block of lines per host worker (at least 64
-  // lines each) renders into its own string; the blocks are joined in order.
+  // Lines are independent: blocks of at least 64 lines (up to four per host
+  // thread, claimed as threads come free) render into their own strings; the
+  // blocks are joined in order.
   const size_t nh = r->hsps.size();
-  const size_t nblocks = std::max<size_t>(1, std::min<size_t>(dmr::host_threads(), (nh + 63) / 64));
+  const size_t nblocks = std::max<size_t>(1, std::min<size_t>(4 * dmr::host_threads(), (nh + 63) / 64));
   const size_t block = (nh + nblocks - 1) / nblocks;
   std::vector<std::string>& parts = *parts_out;
   parts.assign(nblocks, std::string());
@@ -988,24 +1069,25 @@ void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, 
         case kQend: num((long long)qe); break;
         case kSstart: num((long long)h.s_begin + 1); break;
         case kSend: num((long long)h.s_end); break;
-        case kEvalue: out += fmt_evalue(h.evalue); break;
-        case kBitscore: out += fmt_double(h.bits); break;
+        case kEvalue: put_evalue(out, h.evalue); break;
+        case kBitscore: put_double(out, h.bits); break;
         case kScore: num(h.score); break;
         case kLength: num(h.length); break;
-        case kPident: out += fmt_double(h.length ? h.identities * 100.0 / h.length : 0.0); break;
+        case kPident: put_double(out, h.length ? h.identities * 100.0 / h.length : 0.0); break;
         case kNident: num(h.identities); break;
         case kMismatch: num(h.mismatches); break;
         case kPositive: num(h.positives); break;
         case kGapopen: num(h.gap_openings); break;
         case kGaps: num(h.gaps); break;
-        case kPpos: out += fmt_double(h.length ? h.positives * 100.0 / h.length : 0.0); break;
+        case kPpos: put_double(out, h.length ? h.positives * 100.0 / h.length : 0.0); break;
         case kQframe: num(h.frame < 3 ? (int)h.frame + 1 : 2 - (int)h.frame); break;
         case kQcovhsp:
-          out += qlen == 0 ? std::string("0.0") : fmt_double((double)(qe - qb) * 100.0 / (double)qlen);
+          if (qlen == 0) out += "0.0";
+          else put_double(out, (double)(qe - qb) * 100.0 / (double)qlen);
           break;
         case kScovhsp:
-          out += slen == 0 ? std::string("0.0")
-                           : fmt_double((double)(h.s_end - h.s_begin) * 100.0 / (double)slen);
+          if (slen == 0) out += "0.0";
+          else put_double(out, (double)(h.s_end - h.s_begin) * 100.0 / (double)slen);
           break;
         case kCigar:
           for (uint32_t op : h.ops) {

@@ -21,13 +21,17 @@ class HostPool {
     {
       std::lock_guard<std::mutex> g(m_);
       stop_ = true;
+      stop_flag_.store(true);
     }
     wake_.notify_all();
     for (auto& w : workers_) w.join();
   }
   unsigned threads() const { return (unsigned)workers_.size() + 1; }
 
-  // f(part) for part = 0..parts-1; the caller takes parts too.
+  // f(part) for part = 0..parts-1; the caller takes parts too. Parts are
+  // claimed one at a time from a shared counter, and the call waits only for
+  // the workers that joined this run, so a worker that wakes late (after
+  // every part was claimed) delays nothing.
   void run(unsigned parts, const std::function<void(unsigned)>& f) {
     std::lock_guard<std::mutex> one(run_m_);
     {
@@ -35,43 +39,52 @@ class HostPool {
       f_ = &f;
       parts_ = parts;
       next_.store(0);
-      busy_ = (unsigned)workers_.size();
-      ++generation_;
+      active_ = true;
+      generation_.fetch_add(1);
     }
     wake_.notify_all();
-    take();
+    for (unsigned p; (p = next_.fetch_add(1)) < parts;) f(p);
     std::unique_lock<std::mutex> g(m_);
-    done_.wait(g, [this] { return busy_ == 0; });
+    done_.wait(g, [this] { return inflight_ == 0; });
+    active_ = false;
     f_ = nullptr;
   }
 
  private:
-  void take() {
-    for (unsigned p; (p = next_.fetch_add(1)) < parts_;) (*f_)(p);
-  }
   void work() {
     uint64_t seen = 0;
     for (;;) {
+      // A short spin first: runs often come in quick succession.
+      for (int k = 0; k < kSpin && generation_.load() == seen && !stop_flag_.load(); ++k)
+        std::this_thread::yield();
+      const std::function<void(unsigned)>* f;
+      unsigned parts;
       {
         std::unique_lock<std::mutex> g(m_);
-        wake_.wait(g, [&] { return stop_ || generation_ != seen; });
+        wake_.wait(g, [&] { return stop_ || generation_.load() != seen; });
         if (stop_) return;
-        seen = generation_;
+        seen = generation_.load();
+        if (!active_) continue;
+        f = f_;
+        parts = parts_;
+        ++inflight_;
       }
-      take();
+      for (unsigned p; (p = next_.fetch_add(1)) < parts;) (*f)(p);
       std::lock_guard<std::mutex> g(m_);
-      if (--busy_ == 0) done_.notify_one();
+      if (--inflight_ == 0) done_.notify_one();
     }
   }
 
+  static constexpr int kSpin = 500;
   std::vector<std::thread> workers_;
   std::mutex m_, run_m_;
   std::condition_variable wake_, done_;
   const std::function<void(unsigned)>* f_ = nullptr;
-  unsigned parts_ = 0, busy_ = 0;
+  unsigned parts_ = 0, inflight_ = 0;
   std::atomic<unsigned> next_{0};
-  uint64_t generation_ = 0;
-  bool stop_ = false;
+  std::atomic<uint64_t> generation_{0};
+  bool active_ = false, stop_ = false;
+  std::atomic<bool> stop_flag_{false};
 };
 
 HostPool& pool() {
@@ -92,7 +105,8 @@ void host_parallel_for(uint64_t n, uint64_t min_per_part,
     return;
   }
   HostPool& p = pool();
-  const unsigned parts = (unsigned)std::min<uint64_t>(want, p.threads());
+  // Up to four parts per thread, claimed as threads come free.
+  const unsigned parts = (unsigned)std::min<uint64_t>(want, 4ull * p.threads());
   if (parts <= 1) {
     f(0, n);
     return;

@@ -45,16 +45,32 @@ __device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* total) {
   return wave_off + x - v;
 }
 
+// LDS slot of tile element e: one padding word after every 8, so a thread's
+// 8 consecutive elements (reads at stride 9 words of 8 bytes) and the
+// wave's consecutive elements both spread over the banks.
+__device__ __forceinline__ uint32_t scan_slot(uint32_t e) { return e + (e >> 3); }
+
 // Exclusive scan of one tile per block; adds block_off[blockIdx] if given.
+// The tile moves between global memory and LDS in coalesced order (element
+// k * kScanThreads + thread); each thread scans 8 consecutive elements from
+// LDS. in may equal out: a block reads its whole tile before writing it.
 template <class T>
-__global__ void k_scan_apply(const T* in, uint64_t* out, uint64_t n, const uint64_t* block_off) {
-  uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const T* in, uint64_t* out, uint64_t n,
+                                                            const uint64_t* block_off) {
+  __shared__ uint64_t tile[kScanTile + kScanTile / 8];
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanTile;
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t e = (uint32_t)k * kScanThreads + t;
+    tile[scan_slot(e)] = b0 + e < n ? (uint64_t)in[b0 + e] : 0;
+  }
+  __syncthreads();
   uint64_t v[kScanItems];
   uint64_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    uint64_t i = base + k;
-    v[k] = i < n ? (uint64_t)in[i] : 0;
+    v[k] = tile[scan_slot(t * kScanItems + k)];
     s += v[k];
   }
   uint64_t tot;
@@ -62,19 +78,25 @@ __global__ void k_scan_apply(const T* in, uint64_t* out, uint64_t n, const uint6
   if (block_off) run += block_off[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    uint64_t i = base + k;
-    if (i < n) out[i] = run;
+    tile[scan_slot(t * kScanItems + k)] = run;
     run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t e = (uint32_t)k * kScanThreads + t;
+    if (b0 + e < n) out[b0 + e] = tile[scan_slot(e)];
   }
 }
 
+// Sum of each tile (order-free, so read in coalesced order).
 template <class T>
-__global__ void k_scan_reduce_blocked(const T* in, uint64_t n, uint64_t* sums) {
-  uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce_blocked(const T* in, uint64_t n, uint64_t* sums) {
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanTile;
   uint64_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    uint64_t i = base + k;
+    const uint64_t i = b0 + (uint64_t)k * kScanThreads + threadIdx.x;
     if (i < n) s += (uint64_t)in[i];
   }
   uint64_t tot;
