@@ -255,6 +255,7 @@ dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets
     uint64_t a = q_offsets[i], b = q_offsets[i + 1];
     if (b < a) throw dmr::Error(DMR_E_ARG, "query offsets are not monotone");
     if (b == a) continue;  // empty queries have no target (src/workflow.rs:129-132)
+    if (!queries) throw dmr::Error(DMR_E_ARG, "NULL argument");
     check_letters(queries + a, b - a, ("query " + std::to_string(i)).c_str());
     t.letters.insert(t.letters.end(), queries + a, queries + b);
     t.offs.push_back((uint32_t)t.letters.size());
@@ -463,7 +464,8 @@ int dmr_evalue(int32_t go, int32_t ge, int32_t raw, uint64_t qlen, uint64_t slen
 int dmr_seed_build(dmr_ctx* ctx, const uint8_t* seqs, const uint64_t* offsets, uint32_t nseq,
                    const char* shape_code, const char* reduction_def, uint32_t base,
                    double complexity_cut, dmr_seed_entry* out, uint64_t cap, uint64_t* n_out) {
-  if (!ctx || !offsets || !shape_code || !reduction_def || !n_out) return fail(DMR_E_ARG, "NULL argument");
+  if (!ctx || !offsets || !shape_code || !reduction_def || !n_out || (nseq && offsets[nseq] && !seqs))
+    return fail(DMR_E_ARG, "NULL argument");
   dmr::ShapeDef sh;
   if (!dmr::parse_shape(shape_code, &sh)) return fail(DMR_E_ARG, "invalid seed shape");
   dmr::ReductionDef red = dmr::parse_reduction(reduction_def);
@@ -503,7 +505,8 @@ int dmr_seed_join(dmr_ctx* ctx, const dmr_seed_entry* q, uint64_t nq, const dmr_
 int dmr_xdrop_batch(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
                     uint64_t slen, const uint32_t* qa, const uint32_t* sa, uint64_t n, int32_t xdrop,
                     dmr_segment* out) {
-  if (!ctx || (n && (!qa || !sa || !out))) return fail(DMR_E_ARG, "NULL argument");
+  if (!ctx || (n && (!qa || !sa || !out)) || (qlen && !query) || (slen && !subject))
+    return fail(DMR_E_ARG, "NULL argument");
   return guard([&]() -> int {
     std::vector<dmr::SegRec> s(n);
     check_letters(query, qlen, "query");
@@ -524,7 +527,8 @@ int dmr_extend_span_batch(dmr_ctx* ctx, const uint8_t* qbuf, uint64_t qbuf_len, 
                           uint64_t sbuf_len, const dmr_span_job* jobs, uint64_t njobs, int32_t go,
                           int32_t ge, int traceback, dmr_aligned* out, uint32_t* ops_pool,
                           uint64_t ops_cap, uint64_t* ops_offsets, uint64_t* ops_needed) {
-  if (!ctx || (njobs && (!jobs || !out))) return fail(DMR_E_ARG, "NULL argument");
+  if (!ctx || (njobs && (!jobs || !out)) || (qbuf_len && !qbuf) || (sbuf_len && !sbuf))
+    return fail(DMR_E_ARG, "NULL argument");
   dmr::GapRow gr;
   if (!dmr::blosum62_gap_row(go, ge, &gr)) return fail(DMR_E_ARG, "unsupported gap costs");
   return guard([&]() -> int {

@@ -19,6 +19,7 @@ constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kTotalStripes = 16;  // copies of the 256 digit totals per radix pass
 
 // Block-wide exclusive scan of one value per thread; returns the exclusive
 // prefix and writes the block total to *total.
@@ -123,7 +124,7 @@ void scan_impl(const T* in, uint64_t* out, uint64_t n, uint64_t* tmp, hipStream_
 // ---- radix sort -----------------------------------------------------------
 
 __global__ void k_radix_hist(const uint64_t* keys, uint64_t n, int shift, uint32_t* hist,
-                             uint32_t nblocks) {
+                             uint32_t nblocks, uint32_t* totals) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -135,6 +136,39 @@ __global__ void k_radix_hist(const uint64_t* keys, uint64_t n, int shift, uint32
   }
   __syncthreads();
   hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+  // Digit totals, spread over kTotalStripes copies so that few blocks add
+  // to the same word.
+  if (h[threadIdx.x]) atomicAdd(&totals[(blockIdx.x % kTotalStripes) * 256 + threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Scatter offsets of one pass: block d (one per digit) writes
+// offs[d * nblocks + b] = (keys of digits below d) + (keys of digit d in
+// tiles before b), the exclusive scan of the digit-major histogram.
+__global__ __launch_bounds__(kScanThreads) void k_radix_offsets(const uint32_t* hist, uint32_t nblocks,
+                                                               const uint32_t* totals, uint64_t* offs) {
+  __shared__ uint64_t base_s;
+  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  uint64_t tot, mine = 0;
+#pragma unroll
+  for (int k = 0; k < kTotalStripes; ++k) mine += totals[k * 256 + t];
+  const uint64_t below = block_exclusive_scan(mine, &tot);
+  if (t == d) base_s = below;
+  __syncthreads();
+  uint64_t carry = base_s;
+  const uint32_t* row = hist + (uint64_t)d * nblocks;
+  uint64_t* out = offs + (uint64_t)d * nblocks;
+  for (uint32_t c = 0; c < nblocks; c += kScanThreads) {
+    const uint32_t b = c + t;
+    const uint64_t v = b < nblocks ? row[b] : 0;
+    const uint64_t ex = block_exclusive_scan(v, &tot);
+    if (b < nblocks) out[b] = carry + ex;
+    carry += tot;
+  }
 }
 
 // One tile of kSortTile keys per block. Keys are ranked stably by digit in
@@ -224,15 +258,18 @@ void radix_impl(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t
                 uint32_t* hist, uint64_t* scan_tmp, hipStream_t st) {
   if (n <= 1 || bits <= 0) return;
   uint32_t nb = (uint32_t)((n + kSortTile - 1) / kSortTile);
-  uint64_t* offs = scan_tmp;                  // 256 * nb
-  uint64_t* tmp = scan_tmp + 256ull * nb;     // scan scratch
+  uint64_t* offs = scan_tmp;                                        // 256 * nb
+  uint32_t* totals = reinterpret_cast<uint32_t*>(scan_tmp + 256ull * nb);  // stripes x 256 per pass
   int passes = (bits + 7) / 8;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)(passes * kTotalStripes)), dim3(256), 0, st, totals,
+                     (uint32_t)(256 * kTotalStripes * passes));
   uint64_t *ka = k0, *kb = k1;
   uint32_t *va = v0, *vb = v1;
   for (int p = 0; p < passes; ++p) {
     int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist, nb);
-    scan_impl<uint32_t>(hist, offs, 256ull * nb, tmp, st);
+    uint32_t* pt = totals + 256 * kTotalStripes * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist, nb, pt);
+    hipLaunchKernelGGL(k_radix_offsets, dim3(256), dim3(kScanThreads), 0, st, hist, nb, pt, offs);
     hipLaunchKernelGGL(k_radix_scatter<kValues>, dim3(nb), dim3(kSortThreads), 0, st, ka, va, kb,
                        vb, n, shift, offs, nb);
     std::swap(ka, kb);
@@ -258,8 +295,10 @@ uint64_t scan_tmp_elems(uint64_t n) {
 uint64_t radix_hist_elems(uint64_t n) { return 256ull * ((n + kSortTile - 1) / kSortTile) + 256; }
 
 uint64_t radix_scan_tmp_elems(uint64_t n) {
-  uint64_t h = 256ull * ((n + kSortTile - 1) / kSortTile);
-  return h + scan_tmp_elems(h) + 8;
+  // Scatter offsets (256 per tile), then the striped digit totals of up to 8
+  // passes (u32).
+  const uint64_t h = 256ull * ((n + kSortTile - 1) / kSortTile);
+  return h + 8 * kTotalStripes * 256 / 2 + 8;
 }
 
 void scan_exclusive_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* tmp,

@@ -138,3 +138,17 @@ def test_letters_outside_alphabet_rejected(dmr, fresh, e2e):
     q[-1] = 200
     with pytest.raises(dmr.DmrError, match="outside the 25-letter alphabet"):
         fresh.blastp(q, e2e["q_offsets"], dmr.SearchConfig())
+
+
+def test_drop_in_calls_refuse_codes_outside_alphabet(dmr, fresh):
+    """The per-call entry points check their letters the same way."""
+    q = np.array([0, 1, 2, 3, 4, 5, 6, 7], np.uint8)
+    bad = q.copy()
+    bad[3] = 31
+    with pytest.raises(dmr.DmrError, match="outside the 25-letter alphabet"):
+        fresh.xdrop(bad, q, 2, 2, 20)
+    with pytest.raises(dmr.DmrError, match="outside the 25-letter alphabet"):
+        fresh.extend_span(q, bad, -2, 2)
+    with pytest.raises(dmr.DmrError, match="outside the 25-letter alphabet"):
+        fresh.seed_build([bad], "111101011", "A KR EDNQ C G H ILVM FYW P ST")
+    assert fresh.extend_span(q, q, -2, 2)["score"] > 0
