@@ -1,6 +1,7 @@
 // Prefix scans and LSD radix sort for the seed and hit arrays.
 //
-// Radix sort: 8-bit digits, one (histogram, scan, scatter) triple per pass.
+// Radix sort: 8-bit digits, three kernels per pass: per-tile digit histogram
+// (with the digit totals), scatter offsets (one block per digit), scatter.
 // The scatter ranks keys stably inside a 256-thread tile with 64-lane ballot
 // matching (8 ballots give each lane the mask of lanes sharing its digit), so
 // every pass is stable and the final order equals a stable sort by key; the
