@@ -152,3 +152,21 @@ def test_drop_in_calls_refuse_codes_outside_alphabet(dmr, fresh):
     with pytest.raises(dmr.DmrError, match="outside the 25-letter alphabet"):
         fresh.seed_build([bad], "111101011", "A KR EDNQ C G H ILVM FYW P ST")
     assert fresh.extend_span(q, q, -2, 2)["score"] > 0
+
+
+@pytest.mark.parametrize("sens", ["default", "sensitive"])
+def test_fasta_ingress_search(dmr, fresh, orc, e2e, sens):
+    """FASTA text -> dmr_fasta_parse -> database and queries -> dmr_blastp ->
+    outfmt 6 with the parsed ids equals the oracle's search of the same files."""
+    import gzip
+    import pathlib
+    g = pathlib.Path(__file__).resolve().parent / "golden" / "e2e"
+    db = dmr.parse_fasta(gzip.open(g / "db_prot.fa.gz", "rb").read())
+    qs = dmr.parse_fasta(gzip.open(g / "query.fa.gz", "rb").read())
+    assert np.array_equal(db.letters, e2e["db_letters"]) and np.array_equal(qs.letters, e2e["q_letters"])
+    fresh.db_load(db.letters, db.offsets)
+    got = fresh.blastp(qs.letters, qs.offsets, dmr.SearchConfig(sensitivity=sens)).format(qs.ids, db.ids, EXTENDED)
+    ref = orc.search(e2e["db_letters"], e2e["db_offsets"], e2e["q_letters"], e2e["q_offsets"],
+                     sensitivity=sens, threads=8)
+    assert got == ref.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
+    assert got.count("\n") == {"default": 4, "sensitive": 7}[sens]
