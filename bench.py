@@ -201,6 +201,18 @@ def main():
                      "concurrent streams joined)"},
     }
 
+    # The timed batch's text against the oracle's search of the same whole
+    # batch, stored as a fixture (tests/golden/make_search_golden.py): its
+    # first 12 columns are the default fields the step renders.
+    gold = ROOT / "tests" / "golden" / "search" / f"{w.name}.tsv.gz"
+    if gold.exists():
+        import gzip
+        exp = "".join("\t".join(l.split("\t")[:12]) + "\n"
+                      for l in gzip.open(gold, "rb").read().decode().splitlines())
+        out["bit_exact_full_batch"] = {"outfmt6_vs_oracle": text == exp, "lines": exp.count("\n"),
+                                       "fixture": str(gold.relative_to(ROOT))}
+        log(f"[bench] timed batch == oracle fixture: {text == exp}")
+
     out["fasta_ingress"] = fasta_ingress(dmr, w)
 
     if not a.no_plain and a.plain_sample > 0:

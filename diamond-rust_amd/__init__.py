@@ -31,7 +31,7 @@ EXPORTS = [
     "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend",
-    "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
+    "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_headers", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
     "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
     "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy", "dmr_score_layout",
@@ -157,6 +157,7 @@ def lib():
             "dmr_db_load": ([P, P, P, U32], C.c_int),
             "dmr_db_load_file": ([P, C.c_char_p, P], C.c_int),
             "dmr_db_ids": ([P, P, U64, P], C.c_int),
+            "dmr_db_headers": ([P, P, U64, P], C.c_int),
             "dmr_db_build_index": ([P, C.c_char_p, I32, D, P], C.c_int),
             "dmr_blastp": ([P, P, P, U32, P, P], C.c_int),
             "dmr_blastx": ([P, P, P, U32, P, P], C.c_int),
@@ -165,7 +166,8 @@ def lib():
             "dmr_blastp_plain": ([P, P, P, U32, P, P], C.c_int),
             "dmr_result_count": ([P], I64), "dmr_result_hsps": ([P, P, P], C.c_int),
             "dmr_result_ops": ([P, I64, P, U32, P], C.c_int),
-            "dmr_result_format": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, U64, P], C.c_int),
+            "dmr_result_format": ([P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, P, U64,
+                                   P], C.c_int),
             "dmr_result_stats": ([P, P], C.c_int), "dmr_last_dp_cells": ([P, P], C.c_int),
             "dmr_last_class_profile": ([P, P, P, P], C.c_int),
             "dmr_score_layout": ([P, P, P], C.c_int),
@@ -250,12 +252,13 @@ def parse_fasta(data: bytes) -> FastaRecords:
 
 
 def ids_text(ids) -> bytes:
-    """Newline-separated id text for dmr_result_format from a list or a str/bytes."""
+    """Id or header text for dmr_result_format (each line ended by a newline,
+    so empty entries keep their place) from a list, or a str/bytes as is."""
     if isinstance(ids, bytes):
         return ids
     if isinstance(ids, str):
         return ids.encode()
-    return "\n".join(ids).encode()
+    return "".join(i + "\n" for i in ids).encode()
 
 
 class Result:
@@ -290,16 +293,19 @@ class Result:
         _check(lib().dmr_result_ops(self._h, i, buf.ctypes.data, len(buf), C.byref(n)))
         return buf[: n.value].copy()
 
-    def format(self, query_ids, subject_ids, fields="") -> str:
-        """outfmt-6 text; ids as lists of str, or as one newline-separated
-        str/bytes (as dmr_db_ids gives them)."""
+    def format(self, query_ids, subject_ids, fields="", query_titles=None, subject_titles=None) -> str:
+        """outfmt-6 text; ids and titles (full headers, for qtitle/stitle) as
+        lists of str, or as one newline-separated str/bytes (as dmr_db_ids and
+        dmr_db_headers give them)."""
         q = ids_text(query_ids)
         s = ids_text(subject_ids)
+        qt = ids_text(query_titles) if query_titles is not None else None
+        st = ids_text(subject_titles) if subject_titles is not None else None
         f = fields.encode() if isinstance(fields, str) else " ".join(fields).encode()
         n = C.c_uint64()
-        _check(lib().dmr_result_format(self._h, q, s, f, None, 0, C.byref(n)))
+        _check(lib().dmr_result_format(self._h, q, s, qt, st, f, None, 0, C.byref(n)))
         buf = C.create_string_buffer(n.value + 1)
-        _check(lib().dmr_result_format(self._h, q, s, f, buf, n.value + 1, C.byref(n)))
+        _check(lib().dmr_result_format(self._h, q, s, qt, st, f, buf, n.value + 1, C.byref(n)))
         return C.string_at(buf, n.value).decode()
 
     def stats(self):
@@ -349,6 +355,16 @@ class Context:
             _check(rc)
         buf = C.create_string_buffer(max(1, need.value))
         _check(lib().dmr_db_ids(self._h, buf, need.value, C.byref(need)))
+        return buf.raw[:need.value].decode()
+
+    def db_headers(self) -> str:
+        """Headers of the file-loaded database, one per line (the stitle text)."""
+        need = C.c_uint64()
+        rc = lib().dmr_db_headers(self._h, None, 0, C.byref(need))
+        if rc not in (DMR_OK, DMR_E_CAPACITY):
+            _check(rc)
+        buf = C.create_string_buffer(max(1, need.value))
+        _check(lib().dmr_db_headers(self._h, buf, need.value, C.byref(need)))
         return buf.raw[:need.value].decode()
 
     def build_index(self, sensitivity="default", index_chunks=-1, seed_cut=-1.0) -> float:

@@ -26,9 +26,10 @@ struct dmr_ctx {
   dmr::TargetSet loaded;  // resident query batch (dmr_queries_load)
   uint32_t loaded_nq = 0;
   bool has_loaded = false;
-  // Ids of the resident database, one per line (dmr_db_load_file only).
-  std::string db_ids;
-  bool db_has_ids = false;
+  // Ids and headers of the resident database, one per line
+  // (dmr_db_load_file only); db_header_nl: some header holds a newline.
+  std::string db_ids, db_headers;
+  bool db_has_ids = false, db_header_nl = false;
 };
 
 struct dmr_result {
@@ -45,7 +46,7 @@ struct dmr_result {
   // so the usual size-then-fill call pair renders once.
   // Last rendered text, as one block of lines per host thread, and the
   // inputs it was rendered from (id lists and fields, compared on the next call).
-  mutable std::string fmt_q, fmt_s, fmt_f;
+  mutable std::string fmt_q, fmt_s, fmt_qt, fmt_st, fmt_f;
   mutable std::vector<std::string> fmt_parts;
   mutable uint64_t fmt_len = 0;
   mutable bool fmt_valid = false;
@@ -248,6 +249,15 @@ void finish(dmr_ctx* ctx, dmr_result* r, const dmr::SearchOptions& o, size_t nqu
   assemble(ctx, r, per_target, o, nqueries);
 }
 
+// Closes the target just appended to t.letters: its end offset is 32-bit in
+// the device layout (with one padding letter per target and two more), so
+// batches past that are refused before the offset is narrowed.
+void end_target(dmr::TargetSet& t) {
+  if ((uint64_t)t.letters.size() + t.offs.size() + 2 >= (1ull << 32))
+    throw dmr::Error(DMR_E_UNSUPPORTED, "query batch larger than 4G letters");
+  t.offs.push_back((uint32_t)t.letters.size());
+}
+
 dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets, uint32_t nq) {
   dmr::TargetSet t;
   t.offs.push_back(0);
@@ -258,7 +268,7 @@ dmr::TargetSet protein_targets(const uint8_t* queries, const uint64_t* q_offsets
     if (!queries) throw dmr::Error(DMR_E_ARG, "NULL argument");
     check_letters(queries + a, b - a, ("query " + std::to_string(i)).c_str());
     t.letters.insert(t.letters.end(), queries + a, queries + b);
-    t.offs.push_back((uint32_t)t.letters.size());
+    end_target(t);
     t.query_of.push_back(i);
     t.frame_of.push_back(0);
   }
@@ -737,7 +747,7 @@ int dmr_db_load_file(dmr_ctx* ctx, const char* path, uint32_t* nseq) {
     if (version != kDbVersion)
       return fail(DMR_E_ARG, file + ": database version " + std::to_string(version) +
                                  " is not supported (expected " + std::to_string(kDbVersion) + ")");
-    const uint64_t n = le64(d + 16), defs_len = le64(d + 32);
+    const uint64_t n = le64(d + 16), total_letters = le64(d + 24), defs_len = le64(d + 32);
     const uint64_t seq_off_start = le64(d + 40), def_off_start = le64(d + 48);
     const uint64_t letters_start = le64(d + 56), defs_start = le64(d + 64);
     if (defs_start > len || len - defs_start != defs_len)
@@ -758,20 +768,37 @@ int dmr_db_load_file(dmr_ctx* ctx, const char* path, uint32_t* nseq) {
     }
     if (letters_start > len || so[n] > len - letters_start || le64(dofs + 8 * n) > len - defs_start)
       return fail(DMR_E_ARG, file + ": corrupt offset table");
-    if (so[0] != 0) return fail(DMR_E_ARG, file + ": corrupt offset table");
-    const uint8_t* letters = d + letters_start;
+    // Sequence i is letters[so[i]..so[i + 1]] (Database::seq, src/db.rs:256-260);
+    // letters before so[0] belong to no sequence, so the offsets are rebased.
+    const uint8_t* letters = d + letters_start + so[0];
+    const uint64_t base = so[0];
+    for (auto& o : so) o -= base;
     const uint64_t bad = first_bad_letter(letters, so[n]);
     if (bad < so[n])
       return fail(DMR_E_ARG, file + ": corrupt letters section (code " + std::to_string(letters[bad]) +
-                                 " at letter " + std::to_string(bad) + " is outside the 25-letter alphabet)");
-    std::string ids;
+                                 " at letter " + std::to_string(base + bad) + " is outside the 25-letter alphabet)");
+    std::string ids, heads;
     ids.reserve(n * 16);
+    heads.reserve(defs_len + n);
+    bool nl = false;
     for (uint64_t i = 0; i < n; ++i) {
       const uint64_t a = le64(dofs + 8 * i), b = le64(dofs + 8 * (i + 1));
-      append_seq_id(d + defs_start + a, b - a, &ids);
+      const uint8_t* h = d + defs_start + a;
+      append_seq_id(h, b - a, &ids);
+      // Database::header (src/db.rs:262-266): the bytes as text, "" when not UTF-8.
+      if (valid_utf8(h, b - a)) {
+        nl |= std::memchr(h, '\n', b - a) != nullptr;
+        heads.append(reinterpret_cast<const char*>(h), b - a);
+      }
+      heads += '\n';
     }
-    ctx->eng->load_db(letters, so.data(), (uint32_t)n);
+    // E-values use the header's letter count (Database::total_letters,
+    // src/db.rs:199,236-238 and src/search.rs:158), which a file may state
+    // differently from its offset table.
+    ctx->eng->load_db(letters, so.data(), (uint32_t)n, &total_letters);
     ctx->db_ids = std::move(ids);
+    ctx->db_headers = std::move(heads);
+    ctx->db_header_nl = nl;
     ctx->db_has_ids = true;
     if (nseq) *nseq = (uint32_t)n;
     return DMR_OK;
@@ -785,6 +812,18 @@ int dmr_db_ids(const dmr_ctx* ctx, char* buf, uint64_t cap, uint64_t* len) {
   if (cap < ctx->db_ids.size() || (ctx->db_ids.size() && !buf))
     return fail(DMR_E_CAPACITY, "output capacity too small");
   if (!ctx->db_ids.empty()) std::memcpy(buf, ctx->db_ids.data(), ctx->db_ids.size());
+  return DMR_OK;
+}
+
+int dmr_db_headers(const dmr_ctx* ctx, char* buf, uint64_t cap, uint64_t* len) {
+  if (!ctx || !len) return fail(DMR_E_ARG, "NULL argument");
+  if (!ctx->db_has_ids) return fail(DMR_E_ARG, "the resident database was not loaded from a file");
+  if (ctx->db_header_nl)
+    return fail(DMR_E_UNSUPPORTED, "a database header holds a newline; the header list cannot be newline-separated");
+  *len = ctx->db_headers.size();
+  if (cap < ctx->db_headers.size() || (ctx->db_headers.size() && !buf))
+    return fail(DMR_E_CAPACITY, "output capacity too small");
+  if (!ctx->db_headers.empty()) std::memcpy(buf, ctx->db_headers.data(), ctx->db_headers.size());
   return DMR_OK;
 }
 
@@ -888,7 +927,7 @@ int dmr_blastx(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint3
         for (auto c : fr) any |= c < 23;
         if (!any) continue;
         r->targets.letters.insert(r->targets.letters.end(), fr.begin(), fr.end());
-        r->targets.offs.push_back((uint32_t)r->targets.letters.size());
+        end_target(r->targets);
         r->targets.query_of.push_back(i);
         r->targets.frame_of.push_back((uint8_t)f);
       }
@@ -977,13 +1016,16 @@ std::vector<std::pair<const char*, size_t>> lines_of(const char* s) {
   return out;
 }
 
-void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, size_t>>& qid,
-                  const std::vector<std::pair<const char*, size_t>>& sid, const std::vector<int>& f,
-                  size_t i0, size_t i1, std::string* text);
+using Lines = std::vector<std::pair<const char*, size_t>>;
 
-int render(const dmr_result* r, const char* qids, const char* sids, const char* fields,
-           std::vector<std::string>* parts_out) {
+void render_lines(const dmr_result* r, const Lines& qid, const Lines& sid, const Lines& qtl,
+                  const Lines& stl, const std::vector<int>& f, size_t i0, size_t i1,
+                  std::string* text);
+
+int render(const dmr_result* r, const char* qids, const char* sids, const char* qtitles,
+           const char* stitles, const char* fields, std::vector<std::string>* parts_out) {
   const auto qid = lines_of(qids), sid = lines_of(sids);
+  const auto qtl = lines_of(qtitles), stl = lines_of(stitles);
   std::vector<int> f;
   for (const std::string& n : split(fields, ' ')) {
     int code = -1;
@@ -995,9 +1037,22 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
   if (f.empty())
     f = {kQseqid, kSseqid, kPident, kLength, kMismatch, kGapopen,
          kQstart, kQend, kSstart, kSend, kEvalue, kBitscore};
-  for (size_t i = 0; i < r->hsps.size(); ++i)
-    if (r->target_of[i] >= 0 && (r->hsps[i].query >= qid.size() || r->hsps[i].subject >= sid.size()))
+  bool want_qt = false, want_st = false;
+  for (int code : f) {
+    want_qt |= code == kQtitle;
+    want_st |= code == kStitle;
+  }
+  // qtitle / stitle print the full headers (src/output.rs:201-202).
+  if (want_qt && !qtitles) return fail(DMR_E_ARG, "qtitle needs the query headers");
+  if (want_st && !stitles) return fail(DMR_E_ARG, "stitle needs the database headers");
+  for (size_t i = 0; i < r->hsps.size(); ++i) {
+    if (r->target_of[i] < 0) continue;
+    const HspRec& h = r->hsps[i];
+    if (h.query >= qid.size() || h.subject >= sid.size())
       return fail(DMR_E_ARG, "id list shorter than the query or database");
+    if ((want_qt && h.query >= qtl.size()) || (want_st && h.subject >= stl.size()))
+      return fail(DMR_E_ARG, "header list shorter than the query or database");
+  }
   // Lines are independent: blocks of at least 64 lines (up to four per host
   // thread, claimed as threads come free) render into their own strings; the
   // blocks are joined in order.
@@ -1008,16 +1063,16 @@ int render(const dmr_result* r, const char* qids, const char* sids, const char* 
   parts.assign(nblocks, std::string());
   dmr::host_parallel_for(nblocks, 1, [&](uint64_t b0, uint64_t b1) {
     for (uint64_t blk = b0; blk < b1; ++blk)
-      render_lines(r, qid, sid, f, std::min(nh, blk * block), std::min(nh, (size_t)(blk + 1) * block),
-                   &parts[blk]);
+      render_lines(r, qid, sid, qtl, stl, f, std::min(nh, blk * block),
+                   std::min(nh, (size_t)(blk + 1) * block), &parts[blk]);
   });
   return DMR_OK;
 }
 
 // outfmt-6 lines of HSPs [i0, i1) (ids already checked) appended to out.
-void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, size_t>>& qid,
-                  const std::vector<std::pair<const char*, size_t>>& sid, const std::vector<int>& f,
-                  size_t i0, size_t i1, std::string* text) {
+void render_lines(const dmr_result* r, const Lines& qid, const Lines& sid, const Lines& qtl,
+                  const Lines& stl, const std::vector<int>& f, size_t i0, size_t i1,
+                  std::string* text) {
   std::string& out = *text;
   out.reserve((i1 - i0) * 96);
   char nb[32];
@@ -1065,9 +1120,11 @@ void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, 
     for (size_t k = 0; k < f.size(); ++k) {
       if (k) out += '\t';
       switch (f[k]) {
-        case kQseqid: case kQtitle: out.append(qid[h.query].first, qid[h.query].second); break;
+        case kQseqid: out.append(qid[h.query].first, qid[h.query].second); break;
+        case kQtitle: out.append(qtl[h.query].first, qtl[h.query].second); break;
         case kQlen: num((long long)qlen); break;
-        case kSseqid: case kStitle: out.append(sid[h.subject].first, sid[h.subject].second); break;
+        case kSseqid: out.append(sid[h.subject].first, sid[h.subject].second); break;
+        case kStitle: out.append(stl[h.subject].first, stl[h.subject].second); break;
         case kSlen: num((long long)slen); break;
         case kQstart: num((long long)qb + 1); break;
         case kQend: num((long long)qe); break;
@@ -1153,20 +1210,25 @@ void render_lines(const dmr_result* r, const std::vector<std::pair<const char*, 
 
 extern "C" {
 
-int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, const char* fields,
-                      char* buf, uint64_t cap, uint64_t* len) {
+int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, const char* qtitles,
+                      const char* stitles, const char* fields, char* buf, uint64_t cap, uint64_t* len) {
   if (!r || !len) return fail(DMR_E_ARG, "NULL argument");
   return guard([&]() -> int {
     // The text is rendered once per distinct request (the size query and
-    // the fill of one request render once) and kept as its blocks.
-    auto same = [](const std::string& kept, const char* s) { return kept == (s ? s : ""); };
-    if (!r->fmt_valid || !same(r->fmt_q, qids) || !same(r->fmt_s, sids) || !same(r->fmt_f, fields)) {
+    // the fill of one request render once) and kept as its blocks. A NULL
+    // title list is kept as "\x01", which no list of lines equals.
+    auto kept_as = [](const char* s) { return std::string(s ? s : "\x01"); };
+    auto same = [&](const std::string& kept, const char* s) { return kept == kept_as(s); };
+    if (!r->fmt_valid || !same(r->fmt_q, qids) || !same(r->fmt_s, sids) || !same(r->fmt_qt, qtitles) ||
+        !same(r->fmt_st, stitles) || !same(r->fmt_f, fields)) {
       r->fmt_valid = false;
-      const int rc = render(r, qids, sids, fields, &r->fmt_parts);
+      const int rc = render(r, qids, sids, qtitles, stitles, fields, &r->fmt_parts);
       if (rc != DMR_OK) return rc;
-      r->fmt_q = qids ? qids : "";
-      r->fmt_s = sids ? sids : "";
-      r->fmt_f = fields ? fields : "";
+      r->fmt_q = kept_as(qids);
+      r->fmt_s = kept_as(sids);
+      r->fmt_qt = kept_as(qtitles);
+      r->fmt_st = kept_as(stitles);
+      r->fmt_f = kept_as(fields);
       r->fmt_len = 0;
       for (const auto& pt : r->fmt_parts) r->fmt_len += pt.size();
       r->fmt_valid = true;

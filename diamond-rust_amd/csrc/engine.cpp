@@ -175,7 +175,8 @@ AlpParams Engine::alp(int go, int ge) const {
                   kB62UngappedAlphaV);
 }
 
-void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n) {
+void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n,
+                     const uint64_t* stat_letters) {
   DMR_HIP(hipSetDevice(device_));
   uint64_t total = offsets[n];
   if (total + n + 2 >= (1ull << 32)) throw Error(-5, "database larger than 4G letters");
@@ -201,7 +202,7 @@ void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n
                      st_);
   DMR_HIP(hipStreamSynchronize(st_));
   db_n_ = n;
-  db_letters_ = total;
+  db_letters_ = stat_letters ? *stat_letters : total;
   db_max_len_ = maxl;
   chunks_.clear();
   idx_sens_.clear();

@@ -185,8 +185,11 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return st_; }
 
-  // Resident database (codes 0..24, offsets n+1).
-  void load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n);
+  // Resident database (codes 0..24, offsets n+1). stat_letters: the letter
+  // count E-values use (Database::total_letters, src/db.rs:236-238, read
+  // from a file's header); NULL = the letters loaded.
+  void load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n,
+               const uint64_t* stat_letters = nullptr);
   uint32_t db_size() const { return db_n_; }
   uint64_t db_letters() const { return db_letters_; }
   const std::vector<uint32_t>& db_offsets_host() const { return h_db_offs_; }

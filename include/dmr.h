@@ -175,6 +175,13 @@ int dmr_db_load_file(dmr_ctx* ctx, const char* path, uint32_t* nseq);
  * byte length; the text (not NUL-terminated) is written when cap >= *len. */
 int dmr_db_ids(const dmr_ctx* ctx, char* buf, uint64_t cap, uint64_t* len);
 
+/* Database::header of every sequence of the database that dmr_db_load_file
+ * loaded (src/db.rs:262-266: the header bytes as text, empty when they are
+ * not UTF-8), each followed by '\n': the subject_titles of dmr_result_format
+ * (stitle). *len receives the byte length; the text is written when
+ * cap >= *len. DMR_E_UNSUPPORTED when a header holds a newline. */
+int dmr_db_headers(const dmr_ctx* ctx, char* buf, uint64_t cap, uint64_t* len);
+
 /* Builds (or keeps) the resident seed index for a sensitivity preset:
  * seed::chunks(n, index_chunks) slices x all shapes (src/search.rs:121-133). */
 int dmr_db_build_index(dmr_ctx* ctx, const char* sensitivity, int32_t index_chunks,
@@ -214,11 +221,16 @@ int dmr_result_hsps(const dmr_result* r, int64_t* out16, double* out2);
 /* Alignment ops of HSP i (op | count << 2). */
 int dmr_result_ops(const dmr_result* r, int64_t i, uint32_t* ops, uint32_t cap, uint32_t* n);
 /* outfmt 6 text (Tabular::write, src/output.rs:150-223). query_ids and
- * subject_ids are newline-separated; fields is a space-separated list of
- * outfmt-6 field names (NULL or "" = the 12 defaults). *len receives the
- * byte length; the text is written when buf holds cap >= *len bytes. */
+ * subject_ids are newline-separated (Record::id / Database::id);
+ * query_titles and subject_titles are the full headers, newline-separated
+ * (Record::header / Database::header: qtitle and stitle,
+ * src/output.rs:201-202), and may be NULL when neither field is asked for.
+ * fields is a space-separated list of outfmt-6 field names (NULL or "" = the
+ * 12 defaults). *len receives the byte length; the text is written when buf
+ * holds cap >= *len bytes. */
 int dmr_result_format(const dmr_result* r, const char* query_ids, const char* subject_ids,
-                      const char* fields, char* buf, uint64_t cap, uint64_t* len);
+                      const char* query_titles, const char* subject_titles, const char* fields,
+                      char* buf, uint64_t cap, uint64_t* len);
 /* Stage counters and GPU times: query_seeds, runs, hits, hit_runs,
  * segments, groups, jobs, maybe, selected, traceback_cells, ms_seed,
  * ms_join, ms_sort_hits, ms_ungapped, ms_bands, ms_sw, ms_select,

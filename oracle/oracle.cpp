@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 
 #include "oracle_data.hpp"
@@ -973,6 +974,31 @@ std::string fmt_evalue(double e) {
   return mant + "e" + std::to_string(ex);
 }
 
+// src/hsp.rs:84-90 — qframe column: +1..+3 forward, -1..-3 reverse.
+int q_frame(uint8_t frame) { return frame < 3 ? (int)frame + 1 : 2 - (int)frame; }
+
+// src/hsp.rs:97-110 — nucleotide range on the forward strand (0-based,
+// half-open) of a frame's amino-acid range.
+std::pair<size_t, size_t> nt_range(const Hsp& h, size_t dna_len) {
+  const size_t f = h.frame;
+  if (f < 3) return {f + 3 * h.q_begin, f + 3 * h.q_end};
+  const size_t o = f - 3;
+  return {dna_len - (o + 3 * h.q_end), dna_len - (o + 3 * h.q_begin)};
+}
+
+// src/hsp.rs:164-192 — gapped query and subject strings of the aligned region.
+std::pair<std::string, std::string> aligned_strings(const Hsp& h, const uint8_t* q, const uint8_t* s) {
+  std::string qs, ss;
+  size_t qi = h.q_begin, si = h.s_begin;
+  for (auto& [op, n] : h.ops)
+    for (uint32_t t = 0; t < n; ++t) {
+      if (op == OP_M) { qs += decode_byte(q[qi++]); ss += decode_byte(s[si++]); }
+      else if (op == OP_I) { qs += decode_byte(q[qi++]); ss += '-'; }
+      else { qs += '-'; ss += decode_byte(s[si++]); }
+    }
+  return {qs, ss};
+}
+
 // src/hsp.rs:152-159
 std::string cigar(const Hsp& h) {
   std::string s;
@@ -1238,14 +1264,48 @@ int orc_six_frames(const char* nt, uint8_t* out, int64_t cap, int64_t* lens6, lo
   return (int)o;
 }
 
+int orc_q_frame(int frame) { return q_frame((uint8_t)frame); }
+
+void orc_nt_range(int frame, int64_t q_begin, int64_t q_end, int64_t dna_len, int64_t* out2) {
+  Hsp h;
+  h.frame = (uint8_t)frame;
+  h.q_begin = (size_t)q_begin;
+  h.q_end = (size_t)q_end;
+  auto r = nt_range(h, (size_t)dna_len);
+  out2[0] = (int64_t)r.first;
+  out2[1] = (int64_t)r.second;
+}
+
+// Text of one alignment given by its ops (op | count << 2) starting at
+// (q_begin, s_begin) of the encoded query q and subject s: which = 0 CIGAR,
+// 1 BTOP, 2 qseq, 3 sseq. Returns the byte length; writes up to cap.
+int64_t orc_alignment_text(int which, const uint32_t* ops, int64_t nops, const uint8_t* q,
+                           int64_t q_begin, const uint8_t* s, int64_t s_begin, char* buf, int64_t cap) {
+  Hsp h;
+  h.q_begin = (size_t)q_begin;
+  h.s_begin = (size_t)s_begin;
+  for (int64_t i = 0; i < nops; ++i) h.ops.push_back({(Op)(ops[i] & 3), ops[i] >> 2});
+  std::string t;
+  if (which == 0) t = cigar(h);
+  else if (which == 1) t = btop(h, q, s);
+  else {
+    auto al = aligned_strings(h, q, s);
+    t = which == 2 ? al.first : al.second;
+  }
+  if (buf && cap > 0) std::memcpy(buf, t.data(), (size_t)std::min<int64_t>(cap, (int64_t)t.size()));
+  return (int64_t)t.size();
+}
+
 // Full search. Queries: encoded residues (blastp) or nucleotide ASCII (blastx).
+// header_letters: the database's header letter count (0 = its letters).
 void* orc_search(const uint8_t* dbl, const uint64_t* db_off, uint32_t ndb, const uint8_t* ql,
                  const uint64_t* q_off, uint32_t nq, const char* sens, int threads, double evalue,
                  int max_target_seqs, int max_hsps, long index_chunks, double seed_cut,
-                 int translated, long min_orf) {
+                 int translated, long min_orf, uint64_t header_letters) {
   Result* r = new Result();
   r->db_copy.letters.assign(dbl, dbl + db_off[ndb]);
   r->db_copy.offsets.assign(db_off, db_off + ndb + 1);
+  r->db_copy.header_letters = header_letters;
   r->db = &r->db_copy;
   r->translated = translated != 0;
   r->nqueries = nq;
@@ -1323,11 +1383,15 @@ void orc_result_counters(void* h, double* out) {
 }
 
 // outfmt-6 text for the given field names (space separated, empty = defaults)
-// following src/output.rs:150-223. Returns the byte length; writes up to cap.
-int64_t orc_result_outfmt6(void* h, const char* qids, const char* sids, const char* fields,
-                           char* buf, int64_t cap) {
+// following src/output.rs:150-223. Ids and titles (the FASTA headers of the
+// queries, the database headers) are newline-separated lists; titles may be
+// NULL when no title field is asked for. Returns the byte length; writes up
+// to cap.
+int64_t orc_result_outfmt6(void* h, const char* qids, const char* sids, const char* qtitles,
+                           const char* stitles, const char* fields, char* buf, int64_t cap) {
   Result* r = (Result*)h;
   auto qid = split_lines(qids), sid = split_lines(sids);
+  auto qtl = split_lines(qtitles), stl = split_lines(stitles);
   std::vector<std::string> f;
   {
     std::string cur;
@@ -1350,19 +1414,17 @@ int64_t orc_result_outfmt6(void* h, const char* qids, const char* sids, const ch
     size_t slen = r->db->seq_len(x.subject);
     size_t dna_len = r->translated ? r->raw_nt[x.query].size() : letters.size();
     size_t qb = x.q_begin, qe = x.q_end;
-    if (r->translated) {
-      size_t fr = x.frame;
-      if (fr < 3) { qb = fr + 3 * x.q_begin; qe = fr + 3 * x.q_end; }
-      else { size_t o = fr - 3; qb = dna_len - (o + 3 * x.q_end); qe = dna_len - (o + 3 * x.q_begin); }
-    }
+    if (r->translated) std::tie(qb, qe) = nt_range(x, dna_len);
     size_t query_len = r->translated ? dna_len : letters.size();
     std::string row;
     for (size_t k = 0; k < f.size(); ++k) {
       const std::string& n = f[k];
       std::string v;
-      if (n == "qseqid" || n == "qtitle") v = qid[x.query];
+      if (n == "qseqid") v = qid[x.query];
+      else if (n == "qtitle") v = x.query < qtl.size() ? qtl[x.query] : "?";
       else if (n == "qlen") v = std::to_string(query_len);
-      else if (n == "sseqid" || n == "stitle") v = sid[x.subject];
+      else if (n == "sseqid") v = sid[x.subject];
+      else if (n == "stitle") v = x.subject < stl.size() ? stl[x.subject] : "?";
       else if (n == "slen") v = std::to_string(slen);
       else if (n == "qstart") v = std::to_string(qb + 1);
       else if (n == "qend") v = std::to_string(qe);
@@ -1379,7 +1441,7 @@ int64_t orc_result_outfmt6(void* h, const char* qids, const char* sids, const ch
       else if (n == "gapopen") v = std::to_string(x.stats.gap_openings);
       else if (n == "gaps") v = std::to_string(x.stats.gaps);
       else if (n == "ppos") v = fmt_double(x.stats.length ? x.stats.positives * 100.0 / x.stats.length : 0.0);
-      else if (n == "qframe") v = std::to_string(x.frame < 3 ? (int)x.frame + 1 : 2 - (int)x.frame);
+      else if (n == "qframe") v = std::to_string(q_frame(x.frame));
       else if (n == "qcovhsp") v = query_len == 0 ? "0.0" : fmt_double((double)(qe - qb) * 100.0 / (double)query_len);
       else if (n == "scovhsp") v = slen == 0 ? "0.0" : fmt_double((double)(x.s_end - x.s_begin) * 100.0 / (double)slen);
       else if (n == "cigar") v = cigar(x);
@@ -1390,15 +1452,8 @@ int64_t orc_result_outfmt6(void* h, const char* qids, const char* sids, const ch
         v = aa_string(letters, b, e);
       } else if (n == "qstrand") v = x.frame >= 3 ? "-" : "+";
       else if (n == "qseq" || n == "sseq") {
-        std::string qs, ss;
-        size_t qi = x.q_begin, si = x.s_begin;
-        for (auto& [op, cnt] : x.ops)
-          for (uint32_t t = 0; t < cnt; ++t) {
-            if (op == OP_M) { qs += decode_byte(letters[qi++]); ss += decode_byte(subj[si++]); }
-            else if (op == OP_I) { qs += decode_byte(letters[qi++]); ss += '-'; }
-            else { qs += '-'; ss += decode_byte(subj[si++]); }
-          }
-        v = n == "qseq" ? qs : ss;
+        auto al = aligned_strings(x, letters.data(), subj);
+        v = n == "qseq" ? al.first : al.second;
       } else v = "?";
       if (k) row += '\t';
       row += v;

@@ -138,7 +138,10 @@ struct Database {
   size_t size() const { return offsets.empty() ? 0 : offsets.size() - 1; }
   const uint8_t* seq(size_t i) const { return letters.data() + offsets[i]; }
   size_t seq_len(size_t i) const { return offsets[i + 1] - offsets[i]; }
-  uint64_t total_letters() const { return letters.size(); }
+  // src/db.rs:199,236-238 — the letter count of the file header (offset 24),
+  // which a file may state differently from its offset table; 0 = letters.
+  uint64_t header_letters = 0;
+  uint64_t total_letters() const { return header_letters ? header_letters : letters.size(); }
 };
 
 std::vector<std::vector<Hsp>> search_run(const Database& db, const std::vector<Target>& targets,
@@ -155,7 +158,10 @@ size_t min_orf_len(size_t dna_len, long user);
 // ---- output (src/output.rs:150-250, src/hsp.rs:113-241) -------------------
 std::string fmt_double(double x);
 std::string fmt_evalue(double e);
+int q_frame(uint8_t frame);
+std::pair<size_t, size_t> nt_range(const Hsp& h, size_t dna_len);
 std::string cigar(const Hsp& h);
 std::string btop(const Hsp& h, const uint8_t* q, const uint8_t* s);
+std::pair<std::string, std::string> aligned_strings(const Hsp& h, const uint8_t* q, const uint8_t* s);
 
 }  // namespace orc

@@ -51,12 +51,16 @@ def lib():
             "orc_fmt_double": ([D, P, C.c_int], C.c_int),
             "orc_fmt_evalue": ([D, P, C.c_int], C.c_int),
             "orc_six_frames": ([C.c_char_p, P, I64, P, C.c_long], C.c_int),
+            "orc_q_frame": ([C.c_int], C.c_int),
+            "orc_nt_range": ([C.c_int, I64, I64, I64, P], None),
+            "orc_alignment_text": ([C.c_int, P, I64, P, I64, P, I64, P, I64], I64),
             "orc_search": ([P, P, U32, P, P, U32, C.c_char_p, C.c_int, D, C.c_int, C.c_int,
-                            C.c_long, D, C.c_int, C.c_long], P),
+                            C.c_long, D, C.c_int, C.c_long, U64], P),
             "orc_result_count": ([P], I64),
             "orc_result_hsps": ([P, P, P], None),
             "orc_result_counters": ([P, P], None),
-            "orc_result_outfmt6": ([P, C.c_char_p, C.c_char_p, C.c_char_p, P, I64], I64),
+            "orc_result_outfmt6": ([P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                    P, I64], I64),
             "orc_result_free": ([P], None),
         }
         for name, (args, res) in sig.items():
@@ -245,27 +249,57 @@ class SearchResult:
         lib().orc_result_counters(self._h, v.ctypes.data)
         return dict(zip(COUNTER_FIELDS, v.tolist()))
 
-    def format(self, qids, sids, fields=""):
-        q = "\n".join(qids).encode()
-        s = "\n".join(sids).encode()
+    def format(self, qids, sids, fields="", qtitles=None, stitles=None):
+        """outfmt-6 text; qtitles/stitles (lists of headers) feed qtitle/stitle."""
+        lines = lambda v: "".join(x + "\n" for x in v).encode()
+        q, s = lines(qids), lines(sids)
+        qt = lines(qtitles) if qtitles is not None else None
+        st = lines(stitles) if stitles is not None else None
         f = fields.encode() if isinstance(fields, str) else " ".join(fields).encode()
-        n = lib().orc_result_outfmt6(self._h, q, s, f, None, 0)
+        n = lib().orc_result_outfmt6(self._h, q, s, qt, st, f, None, 0)
         buf = C.create_string_buffer(n + 1)
-        lib().orc_result_outfmt6(self._h, q, s, f, buf, n + 1)
+        lib().orc_result_outfmt6(self._h, q, s, qt, st, f, buf, n + 1)
         return buf.raw[:n].decode()
 
 
 def search(db_letters, db_offsets, q_letters, q_offsets, sensitivity="default", threads=1,
            evalue=1e-3, max_target_seqs=25, max_hsps=1, index_chunks=-1, seed_cut=-1.0,
-           translated=False, min_orf=-1):
+           translated=False, min_orf=-1, header_letters=0):
     dbl = np.ascontiguousarray(db_letters, dtype=np.uint8)
     dbo = np.ascontiguousarray(db_offsets, dtype=np.uint64)
     ql = np.ascontiguousarray(q_letters, dtype=np.uint8)
     qo = np.ascontiguousarray(q_offsets, dtype=np.uint64)
     h = lib().orc_search(_p(dbl), dbo.ctypes.data, len(dbo) - 1, _p(ql), qo.ctypes.data,
                          len(qo) - 1, sensitivity.encode(), threads, evalue, max_target_seqs,
-                         max_hsps, index_chunks, seed_cut, 1 if translated else 0, min_orf)
+                         max_hsps, index_chunks, seed_cut, 1 if translated else 0, min_orf,
+                         header_letters)
     return SearchResult(h)
+
+
+def q_frame(frame):
+    """Hsp::q_frame (src/hsp.rs:84-90)."""
+    return int(lib().orc_q_frame(frame))
+
+
+def nt_range(frame, q_begin, q_end, dna_len):
+    """Hsp::nt_range (src/hsp.rs:97-110)."""
+    out = np.zeros(2, np.int64)
+    lib().orc_nt_range(frame, q_begin, q_end, dna_len, out.ctypes.data)
+    return int(out[0]), int(out[1])
+
+
+def alignment_text(which, ops, q, q_begin, s, s_begin):
+    """CIGAR / BTOP / qseq / sseq (src/hsp.rs:152-241) of ops (op | count << 2,
+    op 0 M, 1 I, 2 D) over encoded q and s from (q_begin, s_begin)."""
+    code = {"cigar": 0, "btop": 1, "qseq": 2, "sseq": 3}[which]
+    o = np.ascontiguousarray(ops, dtype=np.uint32)
+    qa = np.ascontiguousarray(q, dtype=np.uint8)
+    sa = np.ascontiguousarray(s, dtype=np.uint8)
+    L = lib()
+    n = L.orc_alignment_text(code, _p(o), len(o), _p(qa), q_begin, _p(sa), s_begin, None, 0)
+    buf = C.create_string_buffer(n + 1)
+    L.orc_alignment_text(code, _p(o), len(o), _p(qa), q_begin, _p(sa), s_begin, buf, n + 1)
+    return buf.raw[:n].decode()
 
 
 # ---- FASTA ingress (pure Python) -------------------------------------------

@@ -97,9 +97,12 @@ def e2e(orc):
     np.cumsum([len(x) for x in dbs], out=off[1:])
     qo = np.zeros(len(qs) + 1, np.uint64)
     np.cumsum([len(x) for x in qs], out=qo[1:])
+    # Full headers (qtitle / stitle) as io::parse_fasta keeps them.
+    heads = lambda name: [h for h, _, _ in orc.parse_fasta(gzip.open(g / name, "rb").read())]
     return dict(db_letters=np.concatenate(dbs), db_offsets=off, q_letters=np.concatenate(qs),
                 q_offsets=qo, db_ids=[h.split()[0] for h, _ in db],
-                q_ids=[h.split()[0] for h, _ in q])
+                q_ids=[h.split()[0] for h, _ in q], db_headers=heads("db_prot.fa.gz"),
+                q_headers=heads("query.fa.gz"))
 
 
 def rand_seq(rng, n, canonical_only=True):

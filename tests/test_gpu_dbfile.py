@@ -15,10 +15,14 @@ from test_gpu_search import EXTENDED
 pytestmark = pytest.mark.gpu
 
 
-def write_db(path, headers, seqs, magic=b"DRDMND01", version=1):
+def write_db(path, headers, seqs, magic=b"DRDMND01", version=1, total_letters=None, lead=b""):
+    """total_letters: the header's letter count (default: the letters
+    written); lead: bytes before the first sequence in the letters section
+    (the letter offsets then start at len(lead))."""
     n = len(seqs)
     so = np.zeros(n + 1, np.uint64)
     np.cumsum([len(s) for s in seqs], out=so[1:])
+    so += len(lead)
     hb = [h.encode() if isinstance(h, str) else h for h in headers]
     do = np.zeros(n + 1, np.uint64)
     np.cumsum([len(h) for h in hb], out=do[1:])
@@ -26,9 +30,11 @@ def write_db(path, headers, seqs, magic=b"DRDMND01", version=1):
     def_off_start = seq_off_start + 8 * (n + 1)
     letters_start = def_off_start + 8 * (n + 1)
     defs_start = letters_start + int(so[-1])
-    head = magic + struct.pack("<II7Q", version, 1, n, int(so[-1]), int(do[-1]), seq_off_start,
+    tl = int(so[-1]) - len(lead) if total_letters is None else total_letters
+    head = magic + struct.pack("<II7Q", version, 1, n, tl, int(do[-1]), seq_off_start,
                                def_off_start, letters_start, defs_start)
-    body = so.tobytes() + do.tobytes() + b"".join(np.asarray(s, np.uint8).tobytes() for s in seqs)
+    body = (so.tobytes() + do.tobytes() + lead +
+            b"".join(np.asarray(s, np.uint8).tobytes() for s in seqs))
     data = head + body + b"".join(hb)
     with open(path, "wb") as f:
         f.write(data)
@@ -170,3 +176,52 @@ def test_fasta_ingress_search(dmr, fresh, orc, e2e, sens):
                      sensitivity=sens, threads=8)
     assert got == ref.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
     assert got.count("\n") == {"default": 4, "sensitive": 7}[sens]
+
+
+def test_headers_and_titles(dmr, fresh, orc, e2e, tmp_path):
+    """dmr_db_headers gives Database::header of each sequence (the bytes as
+    text, empty when not UTF-8); qtitle / stitle print the full headers
+    (src/output.rs:201-202) as the oracle does from the reference's files."""
+    write_db(tmp_path / "db.dmnd", e2e["db_headers"], split(e2e))
+    fresh.db_load_file(tmp_path / "db.dmnd")
+    heads = fresh.db_headers()
+    assert heads == "".join(h + "\n" for h in e2e["db_headers"])
+    fields = "qseqid qtitle sseqid stitle score evalue"
+    got = fresh.blastp(e2e["q_letters"], e2e["q_offsets"], dmr.SearchConfig(sensitivity="sensitive")).format(
+        e2e["q_ids"], fresh.db_ids(), fields, e2e["q_headers"], heads)
+    ref = orc.search(e2e["db_letters"], e2e["db_offsets"], e2e["q_letters"], e2e["q_offsets"],
+                     sensitivity="sensitive", threads=8)
+    exp = ref.format(e2e["q_ids"], e2e["db_ids"], fields, e2e["q_headers"], e2e["db_headers"])
+    assert got == exp and got.count("\n") == 7
+    r = fresh.blastp(e2e["q_letters"], e2e["q_offsets"], dmr.SearchConfig(sensitivity="sensitive"))
+    with pytest.raises(dmr.DmrError, match="qtitle needs the query headers"):
+        r.format(e2e["q_ids"], fresh.db_ids(), "qseqid qtitle")
+    with pytest.raises(dmr.DmrError, match="stitle needs the database headers"):
+        r.format(e2e["q_ids"], fresh.db_ids(), "stitle", e2e["q_headers"])
+    # Headers that are not UTF-8 read as empty; others keep every byte.
+    hs = [b"a b\tc", b"\xff bad", b"", "été x".encode()]
+    write_db(tmp_path / "h.dmnd", hs, [np.array([1, 2, 3], np.uint8)] * 4)
+    fresh.db_load_file(tmp_path / "h.dmnd")
+    assert fresh.db_headers() == "a b\tc\n\n\nété x\n"
+    write_db(tmp_path / "nl.dmnd", [b"x\ny", b"z"], [np.array([1, 2, 3], np.uint8)] * 2)
+    fresh.db_load_file(tmp_path / "nl.dmnd")
+    with pytest.raises(dmr.DmrError, match="holds a newline"):
+        fresh.db_headers()
+
+
+def test_header_letter_count_sets_evalues(dmr, fresh, orc, e2e, tmp_path):
+    """E-values use Database::total_letters, the header's count
+    (src/db.rs:199,236-238; src/search.rs:158), even where it differs from
+    the letters the offsets hold; letters before the first offset belong to
+    no sequence (Database::seq)."""
+    n_letters = int(e2e["db_offsets"][-1])
+    cfg = dmr.SearchConfig(sensitivity="sensitive", evalue=10.0)
+    for total, lead in [(n_letters * 10, b""), (n_letters // 3, b""), (n_letters, b"\x05\x06\x07")]:
+        write_db(tmp_path / "t.dmnd", e2e["db_ids"], split(e2e), total_letters=total, lead=lead)
+        fresh.db_load_file(tmp_path / "t.dmnd")
+        got = fresh.blastp(e2e["q_letters"], e2e["q_offsets"], cfg).format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
+        ref = orc.search(e2e["db_letters"], e2e["db_offsets"], e2e["q_letters"], e2e["q_offsets"],
+                         sensitivity="sensitive", threads=8, evalue=10.0, header_letters=total)
+        exp = ref.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
+        assert got == exp and len(ref) > 0, (total, lead)
+    assert len(ref) > 0

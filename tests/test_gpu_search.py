@@ -52,6 +52,11 @@ def test_reference_e2e_data(dmr, ctx, orc, e2e, sens):
     got = res.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
     assert got == ref.format(e2e["q_ids"], e2e["db_ids"], EXTENDED)
     assert len(res) == {"default": 4, "fast": 1, "sensitive": 7}[sens]
+    # qtitle / stitle: the full FASTA headers (src/output.rs:201-202).
+    f = "qseqid qtitle sseqid stitle qlen slen"
+    titled = res.format(e2e["q_ids"], e2e["db_ids"], f, e2e["q_headers"], e2e["db_headers"])
+    assert titled == ref.format(e2e["q_ids"], e2e["db_ids"], f, e2e["q_headers"], e2e["db_headers"])
+    assert all(len(l.split("\t")[3]) > len(l.split("\t")[2]) for l in titled.splitlines())
 
 
 def test_sensitive_batch_subset(dmr, ctx, orc, synth):
@@ -106,3 +111,23 @@ def test_results_keep_their_ops(dmr, synth):
     assert fmt(b) == text_b and fmt(c2) == text_c
     assert [list(b.ops(i)) for i in range(len(b))] == ops_b
     assert all(len(o) > 0 for o in ops_b)
+
+
+def test_full_sensitive_batch_matches_oracle(dmr, ctx, synth):
+    """The whole 1024-query --sensitive workload the benchmark times, as one
+    batch (the cap counts every query of it, src/search.rs:109-131), equals
+    the oracle's search of the same batch, stored by
+    tests/golden/make_search_golden.py (12 default fields, score, CIGAR)."""
+    import gzip
+    import pathlib
+    g = pathlib.Path(__file__).resolve().parent / "golden" / "search" / "blastp-sensitive-1024.tsv.gz"
+    exp = gzip.open(g, "rb").read().decode()
+    w = synth.make("blastp-sensitive-1024")
+    ctx.db_load(w.db_letters, w.db_offsets)
+    ctx.queries_load(w.q_letters, w.q_offsets)
+    res = ctx.search_loaded(dmr.SearchConfig(sensitivity="sensitive"))
+    fields = ("qseqid sseqid pident length mismatch gapopen qstart qend sstart send evalue bitscore "
+              "score cigar")
+    got = res.format(w.q_ids, w.db_ids, fields)
+    assert got.count("\n") == exp.count("\n") == 4735
+    assert got == exp

@@ -25,7 +25,7 @@ def test_alphabet_roundtrip(orc):  # src/alphabet.rs:99-112
     assert list(orc.encode("AC-DE\nFG")) == [0, 4, 3, 6, 13, 7]
 
 
-def test_reductions(orc):  # src/reduction.rs:205-235
+def test_reductions(orc):  # src/reduction.rs:86-116
     m = np.zeros(25, np.uint8)
     assert orc.lib().orc_reduction_map(b"default", m.ctypes.data) == 10
     enc = {c: int(orc.encode(c)[0]) for c in "AKRLM"}
@@ -46,7 +46,7 @@ def test_shape_keys(orc):  # src/shape.rs:231-265
     assert orc.shape_key("11", orc.encode("AX"), 0, NONE20) is None
 
 
-def test_shape_presets(orc):  # src/shape.rs:268-299
+def test_shape_presets(orc):  # src/shape.rs:267-299
     d = orc.shapes_for("default")
     assert d == ["111101110111", "111011010010111"]
     assert all(c.count("1") == 10 for c in d)
@@ -85,7 +85,7 @@ def test_chunks(orc):  # src/seed.rs:211-216
     assert orc.chunks(0, 4) == []
 
 
-def test_xdrop(orc):  # src/ungapped.rs:316-346
+def test_xdrop(orc):  # src/ungapped.rs:99-129
     q = orc.encode("MKWVTFISLLFLFSSAYSR")
     qs, ss, ln, sc = orc.xdrop(q, q, 0, 0, 40)
     assert qs == 0 and ln == len(q)
@@ -147,7 +147,7 @@ def test_alp_evalues(orc):  # src/alp.rs:398-428
     assert e > 0 and math.isfinite(e) and abs(math.log(e / blast)) < 3
 
 
-def test_formatting(orc):  # src/output.rs:570-584
+def test_formatting(orc):  # src/output.rs:282-296
     assert orc.fmt_evalue(0.0) == "0.0"
     assert orc.fmt_evalue(0.001) == "1.00e-3"
     assert orc.fmt_evalue(4.7e-45) == "4.70e-45"
