@@ -30,7 +30,7 @@ DMR_E_UNSUPPORTED = -5
 EXPORTS = [
     "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
-    "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend",
+    "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend_span_noalloc", "dmr_extend",
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_headers", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
     "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
     "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
@@ -152,6 +152,7 @@ def lib():
             "dmr_xdrop_batch": ([P, P, U64, P, U64, P, P, U64, I32, P], C.c_int),
             "dmr_extend_span": ([P, P, U64, P, U64, I64, I64, I32, I32, P, P, U32], C.c_int),
             "dmr_extend": ([P, P, U64, P, U64, U64, U64, U64, I32, I32, P, P, U32], C.c_int),
+            "dmr_extend_span_noalloc": ([P, P, U64, P, U64, I64, I64, I32, I32, P, P, U32], C.c_int),
             "dmr_extend_span_batch": ([P, P, U64, P, U64, P, U64, I32, I32, C.c_int, P, P, U64, P, P],
                                       C.c_int),
             "dmr_db_load": ([P, P, P, U32], C.c_int),
@@ -505,6 +506,16 @@ class Context:
         ops = np.zeros(len(q) + len(s) + 1, dtype=np.uint32)
         _check(lib().dmr_extend_span(self._h, _ptr(q), len(q), _ptr(s), len(s), d0, d1, gap_open,
                                      gap_extend, C.byref(out), ops.ctypes.data, len(ops)))
+        return _aligned_dict(out, ops[: out.n_ops])
+
+    def extend_span_noalloc(self, query, subject, d0, d1, gap_open=11, gap_extend=1):
+        """dmr_extend_span_noalloc: extend_span without device allocation."""
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        s = np.ascontiguousarray(subject, dtype=np.uint8)
+        out = Aligned()
+        ops = np.zeros(len(q) + len(s) + 1, dtype=np.uint32)
+        _check(lib().dmr_extend_span_noalloc(self._h, _ptr(q), len(q), _ptr(s), len(s), d0, d1, gap_open,
+                                             gap_extend, C.byref(out), ops.ctypes.data, len(ops)))
         return _aligned_dict(out, ops[: out.n_ops])
 
     def extend(self, query, subject, anchor, band, gap_open=11, gap_extend=1):

@@ -587,6 +587,24 @@ int dmr_extend_span(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uin
   return DMR_OK;
 }
 
+int dmr_extend_span_noalloc(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+                            uint64_t slen, int64_t d0, int64_t d1, int32_t go, int32_t ge, dmr_aligned* out,
+                            uint32_t* ops, uint32_t ops_cap) {
+  if (!ctx || !out || (qlen && !query) || (slen && !subject)) return fail(DMR_E_ARG, "NULL argument");
+  dmr::GapRow gr;
+  if (!dmr::blosum62_gap_row(go, ge, &gr)) return fail(DMR_E_ARG, "unsupported gap costs");
+  return guard([&]() -> int {
+    check_letters(query, qlen, "query");
+    check_letters(subject, slen, "subject");
+    dmr::SwOut o;
+    bool fit = false;
+    ctx->eng->extend_span_noalloc(query, qlen, subject, slen, d0, d1, go, ge, &o, ops, ops_cap, &fit);
+    set_aligned(o, out, 0);
+    if (!fit) return fail(DMR_E_CAPACITY, "ops capacity too small");
+    return DMR_OK;
+  });
+}
+
 int dmr_extend(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
                uint64_t slen, uint64_t aq, uint64_t as, uint64_t band, int32_t go, int32_t ge,
                dmr_aligned* out, uint32_t* ops, uint32_t ops_cap) {

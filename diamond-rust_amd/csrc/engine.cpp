@@ -105,6 +105,8 @@ Engine::Engine(int device) : device_(device) {
   hipDeviceProp_t prop;
   DMR_HIP(hipGetDeviceProperties(&prop, device));
   cu_count_ = prop.multiProcessorCount;
+  // The no-allocation entry point's array: its address on this device.
+  noalloc_ = noalloc_scratch_address();
   const char* fg = std::getenv("DMR_FORCE_GENERIC");
   force_generic_ = fg && fg[0] == '1';
   // Width classes whose packed kernel runs split (two lanes per DP pair):

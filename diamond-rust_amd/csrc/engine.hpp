@@ -268,7 +268,19 @@ class Engine {
 
   AlpParams alp(int gap_open, int gap_extend) const;
 
+  // gapped::extend_span with traceback for one sequence pair, allocating
+  // nothing: its device memory comes from a static per-device array of
+  // kNoAllocScratchBytes (noalloc.hip). *ops_fit = false when the ops did not
+  // fit ops_cap (out->n_ops holds the count needed; no ops are copied).
+  static constexpr uint64_t kNoAllocScratchBytes = 32ull << 20;
+  void extend_span_noalloc(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint64_t slen, int64_t d0,
+                           int64_t d1, int gap_open, int gap_extend, SwOut* out, uint32_t* ops,
+                           uint32_t ops_cap, bool* ops_fit);
+
  private:
+  // Device address of the no-allocation array on the current device.
+  static void* noalloc_scratch_address();
+  void* noalloc_ = nullptr;
   struct IndexChunk {
     DevBuf keys, locs, table;
     uint64_t n = 0, nbuckets = 0;

@@ -141,6 +141,20 @@ int dmr_extend_span(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uin
                     uint64_t slen, int64_t d0, int64_t d1, int32_t gap_open, int32_t gap_extend,
                     dmr_aligned* out, uint32_t* ops, uint32_t ops_cap);
 
+/* No counterpart in the reference: dmr_extend_span for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results; it calls no
+ * hipMalloc, hipFree or other allocating routine. Its device memory (the
+ * sequences, the DP rows, the trace and the ops) comes from one static
+ * array of 32 MiB per GPU, held until the process ends; a DP needing more
+ * (about qlen x band width trace bytes plus 16 bytes per band cell of a row
+ * and 4 bytes per letter of both sequences) fails with DMR_E_UNSUPPORTED.
+ * All calls on one GPU share that array: calls that overlap on one GPU (from
+ * two threads, or two contexts of the same device) race. */
+int dmr_extend_span_noalloc(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
+                            uint64_t slen, int64_t d0, int64_t d1, int32_t gap_open, int32_t gap_extend,
+                            dmr_aligned* out, uint32_t* ops, uint32_t ops_cap);
+
 /* gapped::extend (src/gapped.rs:50-60): band around the anchor diagonal. */
 int dmr_extend(dmr_ctx* ctx, const uint8_t* query, uint64_t qlen, const uint8_t* subject,
                uint64_t slen, uint64_t anchor_q, uint64_t anchor_s, uint64_t band,
