@@ -25,17 +25,20 @@ DMR_E_DEVICE = -2
 DMR_E_OOM = -3
 DMR_E_CAPACITY = -4
 DMR_E_UNSUPPORTED = -5
+DMR_E_TIMEOUT = -6
 
 # Symbols declared by include/dmr.h.
 EXPORTS = [
     "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
+    "dmr_ctx_set_tuning", "dmr_ctx_get_tuning",
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend_span_noalloc", "dmr_extend",
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_headers", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
-    "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
+    "dmr_six_frames", "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
     "dmr_result_count", "dmr_result_hsps", "dmr_result_ops", "dmr_result_format",
     "dmr_result_stats", "dmr_last_dp_cells", "dmr_result_destroy", "dmr_score_layout",
     "dmr_fasta_parse", "dmr_fasta_sizes", "dmr_fasta_residues", "dmr_fasta_text", "dmr_fasta_destroy",
+    "dmr_server_create", "dmr_server_submit", "dmr_server_wait", "dmr_server_stats", "dmr_server_destroy",
 ]
 
 # Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
@@ -64,7 +67,7 @@ HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end
 STAT_FIELDS = ["query_seeds", "runs", "hits", "hit_runs", "segments", "groups", "jobs", "maybe",
                "selected", "traceback_cells", "ms_seed", "ms_join", "ms_sort_hits",
                "ms_ungapped", "ms_bands", "ms_sw", "ms_select", "ms_traceback", "ms_total",
-               "ms_host"]
+               "ms_host", "ms_translate"]
 
 
 class DmrError(RuntimeError):
@@ -145,6 +148,8 @@ def lib():
             "dmr_last_error": ([], C.c_char_p), "dmr_version": ([], C.c_char_p),
             "dmr_ctx_create": ([C.c_int, P], C.c_int), "dmr_ctx_destroy": ([P], None),
             "dmr_device_count": ([P], C.c_int), "dmr_band_for": ([U64], U32),
+            "dmr_ctx_set_tuning": ([P, C.c_char_p, D], C.c_int),
+            "dmr_ctx_get_tuning": ([P, C.c_char_p, P], C.c_int),
             "dmr_evalue": ([I32, I32, I32, U64, U64, U64, P, P], C.c_int),
             "dmr_seed_build": ([P, P, P, U32, C.c_char_p, C.c_char_p, U32, D, P, U64, P], C.c_int),
             "dmr_seed_join": ([P, P, U64, P, U64, U64, P, U64, P], C.c_int),
@@ -162,6 +167,7 @@ def lib():
             "dmr_db_build_index": ([P, C.c_char_p, I32, D, P], C.c_int),
             "dmr_blastp": ([P, P, P, U32, P, P], C.c_int),
             "dmr_blastx": ([P, P, P, U32, P, P], C.c_int),
+            "dmr_six_frames": ([P, P, P, U32, I32, P, U64, P, P], C.c_int),
             "dmr_queries_load": ([P, P, P, U32], C.c_int),
             "dmr_search_loaded": ([P, P, P], C.c_int),
             "dmr_blastp_plain": ([P, P, P, U32, P, P], C.c_int),
@@ -178,6 +184,11 @@ def lib():
             "dmr_fasta_residues": ([P, P, P, P], C.c_int),
             "dmr_fasta_text": ([P, P, P], C.c_int),
             "dmr_fasta_destroy": ([P], None),
+            "dmr_server_create": ([P, U32, U64, C.c_char_p, P], C.c_int),
+            "dmr_server_submit": ([P, P, U64, P, P], C.c_int),
+            "dmr_server_wait": ([P, U64, I64, P, P], C.c_int),
+            "dmr_server_stats": ([P, P], C.c_int),
+            "dmr_server_destroy": ([P], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -310,9 +321,58 @@ class Result:
         return C.string_at(buf, n.value).decode()
 
     def stats(self):
-        v = np.zeros(20, dtype=np.float64)
+        v = np.zeros(len(STAT_FIELDS), dtype=np.float64)
         _check(lib().dmr_result_stats(self._h, v.ctypes.data))
         return dict(zip(STAT_FIELDS, v.tolist()))
+
+
+SERVER_STATS = ["requests", "batches", "mean_batch", "max_batch", "peak_admitted_bytes", "max_queue",
+                "mean_queue_wait_ms", "budget_bytes"]
+
+
+class Server:
+    """dmr_server: a serving loop over a context's resident database
+    (continuous batching, device-memory admission, per-request cap)."""
+
+    def __init__(self, ctx: "Context", workers=2, hbm_budget=8 << 30, presets="default,sensitive,very-sensitive"):
+        h = C.c_void_p()
+        _check(lib().dmr_server_create(ctx._h, workers, hbm_budget, presets.encode() if presets else None,
+                                       C.byref(h)))
+        self._h = h
+        self._ctx = ctx  # the context outlives the server
+        self._cfgs = {}
+
+    def submit(self, letters, cfg: "SearchConfig | None" = None) -> int:
+        cfg = cfg or SearchConfig()
+        q = np.ascontiguousarray(letters, dtype=np.uint8)
+        c = cfg._c()
+        t = C.c_uint64()
+        _check(lib().dmr_server_submit(self._h, _ptr(q), len(q), C.byref(c), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(Result, latency_ms) of a request; raises DmrError (DMR_E_TIMEOUT
+        when it is not done within timeout_ms)."""
+        h = C.c_void_p()
+        lat = C.c_double()
+        _check(lib().dmr_server_wait(self._h, ticket, timeout_ms, C.byref(h), C.byref(lat)))
+        return Result(h.value), lat.value
+
+    def stats(self):
+        v = np.zeros(8, dtype=np.float64)
+        _check(lib().dmr_server_stats(self._h, v.ctypes.data))
+        return dict(zip(SERVER_STATS, v.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dmr_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -333,6 +393,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_tuning(self, key: str, value: float):
+        """dmr_ctx_set_tuning (slice_hits, profile, split_mask, score_plan, f16)."""
+        _check(lib().dmr_ctx_set_tuning(self._h, key.encode(), float(value)))
+
+    def get_tuning(self, key: str) -> float:
+        v = C.c_double()
+        _check(lib().dmr_ctx_get_tuning(self._h, key.encode(), C.byref(v)))
+        return v.value
 
     # -- database / search ------------------------------------------------
     def db_load(self, letters, offsets):
@@ -418,6 +487,24 @@ class Context:
         buf = C.create_string_buffer(data, len(data) + 1)
         _check(lib().dmr_blastx(self._h, buf, off.ctypes.data, len(reads), C.byref(c), C.byref(h)))
         return Result(h.value)
+
+    def six_frames(self, reads, min_orf=-1):
+        """dmr_six_frames: the six masked frames of every read (a list of
+        6 * len(reads) uint8 arrays, frame k of read r at 6r + k)."""
+        data = "".join(reads).encode()
+        off = np.zeros(len(reads) + 1, dtype=np.uint64)
+        np.cumsum([len(r) for r in reads], out=off[1:])
+        fo = np.zeros(6 * len(reads) + 1, dtype=np.uint64)
+        n = C.c_uint64()
+        buf = C.create_string_buffer(data, len(data) + 1)
+        rc = lib().dmr_six_frames(self._h, buf, off.ctypes.data, len(reads), min_orf, None, 0,
+                                  fo.ctypes.data, C.byref(n))
+        if rc not in (DMR_OK, DMR_E_CAPACITY):
+            _check(rc)
+        letters = np.zeros(max(1, n.value), np.uint8)
+        _check(lib().dmr_six_frames(self._h, buf, off.ctypes.data, len(reads), min_orf, letters.ctypes.data,
+                                    n.value, fo.ctypes.data, C.byref(n)))
+        return [letters[fo[k]:fo[k + 1]].copy() for k in range(6 * len(reads))]
 
     def class_profile(self):
         """Per score-kernel class of the last search: packed 16-bit by band width

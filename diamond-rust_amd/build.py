@@ -19,7 +19,8 @@ LIB = HERE / "libdmr.so"
 ARCH = os.environ.get("DMR_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["k_sort.hip", "k_seed.hip", "k_extend.hip", "k_sw.hip", "k_sw16.hip", "k_swwave.hip", "plain.cpp",
-           "engine.cpp", "api.cpp", "host_pool.cpp", "fasta.cpp", "noalloc.hip"]
+           "engine.cpp", "api.cpp", "host_pool.cpp", "fasta.cpp", "noalloc.hip", "k_translate.hip",
+           "serve.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",

@@ -19,60 +19,15 @@
 #include <string>
 #include <vector>
 
-#include "engine.hpp"
+#include "api_internal.hpp"
 
-struct dmr_ctx {
-  std::unique_ptr<dmr::Engine> eng;
-  dmr::TargetSet loaded;  // resident query batch (dmr_queries_load)
-  uint32_t loaded_nq = 0;
-  bool has_loaded = false;
-  // Ids and headers of the resident database, one per line
-  // (dmr_db_load_file only); db_header_nl: some header holds a newline.
-  std::string db_ids, db_headers;
-  bool db_has_ids = false, db_header_nl = false;
-};
-
-struct dmr_result {
-  std::vector<dmr::HspRec> hsps;     // output order
-  std::vector<std::shared_ptr<const dmr::OpsStore>> ops_stores;  // where the HSPs' ops live
-  std::vector<int> target_of;        // target index of each HSP
-  dmr::TargetSet targets;
-  std::vector<uint64_t> dna_len;     // per query record (blastx)
-  bool translated = false;
-  std::shared_ptr<const std::vector<uint32_t>> db_offs;  // host copy of the database
-  std::shared_ptr<const std::vector<uint8_t>> db_res;
-  dmr::StageStats stats;
-  // Text of the last dmr_result_format call and the inputs it was made from,
-  // so the usual size-then-fill call pair renders once.
-  // Last rendered text, as one block of lines per host thread, and the
-  // inputs it was rendered from (id lists and fields, compared on the next call).
-  mutable std::string fmt_q, fmt_s, fmt_qt, fmt_st, fmt_f;
-  mutable std::vector<std::string> fmt_parts;
-  mutable uint64_t fmt_len = 0;
-  mutable bool fmt_valid = false;
-};
-
-namespace {
+namespace dmr_api {
 
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
-}
-
-// Runs f and maps exceptions to status codes (no exception crosses the ABI).
-template <class F>
-int guard(F&& f) {
-  try {
-    return f();
-  } catch (const dmr::Error& e) {
-    return fail(e.code, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(DMR_E_OOM, "host allocation failed");
-  } catch (const std::exception& e) {
-    return fail(DMR_E_DEVICE, e.what());
-  }
 }
 
 // Index of the first letter code outside the 25-letter alphabet
@@ -161,48 +116,16 @@ std::vector<std::vector<HspRec>> merge_frames(std::vector<std::vector<HspRec>>& 
   return per;
 }
 
-// translate.rs: six frames (0-2 forward offsets, 3-5 reverse complement) with
-// the standard code; codons with ambiguous bases become X.
-uint8_t nt_code(char b) {
-  switch (b) {
-    case 'T': case 't': case 'U': case 'u': return 0;
-    case 'C': case 'c': return 1;
-    case 'A': case 'a': return 2;
-    case 'G': case 'g': return 3;
-    default: return 4;
+// translate::min_orf_len (src/translate.rs:124-137) of every read: the
+// user's --min-orf, else 1 below 30 bases, 20 below 100, then 40.
+std::vector<uint32_t> min_orf_lengths(const uint64_t* r_offsets, uint32_t n, int32_t user) {
+  std::vector<uint32_t> mo(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t len = r_offsets[i + 1] - r_offsets[i];
+    mo[i] = user >= 0 ? (uint32_t)user : (len < 30 ? 1u : (len < 100 ? 20u : 40u));
   }
+  return mo;
 }
-char nt_comp(char b) {
-  switch (b) {
-    case 'A': return 'T'; case 'T': return 'A'; case 'C': return 'G'; case 'G': return 'C';
-    case 'U': return 'A'; case 'R': return 'Y'; case 'Y': return 'R'; case 'S': return 'S';
-    case 'W': return 'W'; case 'K': return 'M'; case 'M': return 'K'; case 'B': return 'V';
-    case 'V': return 'B'; case 'D': return 'H'; case 'H': return 'D'; case 'N': return 'N';
-    default: return b;
-  }
-}
-std::vector<uint8_t> translate(const std::string& nt, size_t off) {
-  static const char code[] = "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
-  std::vector<uint8_t> out;
-  for (size_t i = off; i + 3 <= nt.size(); i += 3) {
-    uint8_t a = nt_code(nt[i]), b = nt_code(nt[i + 1]), c = nt_code(nt[i + 2]);
-    out.push_back((a > 3 || b > 3 || c > 3) ? 23 : dmr::encode_letter((uint8_t)code[a * 16 + b * 4 + c]));
-  }
-  return out;
-}
-// --min-orf masking (src/translate.rs:101-121): stop-free runs shorter than
-// min_len become X.
-void mask_orfs(std::vector<uint8_t>& f, size_t min_len) {
-  if (min_len <= 1) return;
-  size_t begin = 0;
-  for (size_t i = 0; i < f.size(); ++i)
-    if (f[i] == 24) {
-      if (i - begin < min_len) std::fill(f.begin() + begin, f.begin() + i, (uint8_t)23);
-      begin = i + 1;
-    }
-  if (f.size() - begin < min_len) std::fill(f.begin() + begin, f.end(), (uint8_t)23);
-}
-size_t default_min_orf(size_t len) { return len < 30 ? 1 : (len < 100 ? 20 : 40); }
 
 dmr::SearchOptions options(const dmr_search_config* c) {
   dmr::SearchOptions o;
@@ -243,9 +166,7 @@ void assemble(dmr_ctx* ctx, dmr_result* r, std::vector<std::vector<HspRec>>& per
 void finish(dmr_ctx* ctx, dmr_result* r, const dmr::SearchOptions& o, size_t nqueries,
             bool upload = true) {
   if (upload) ctx->eng->load_targets(r->targets);
-  std::shared_ptr<const dmr::OpsStore> store;
-  auto per_target = ctx->eng->search_targets(o, &r->stats, &store);
-  r->ops_stores.push_back(std::move(store));
+  auto per_target = ctx->eng->search_targets(o, &r->stats, &r->ops_stores);
   assemble(ctx, r, per_target, o, nqueries);
 }
 
@@ -413,7 +334,9 @@ void set_aligned(const dmr::SwOut& o, dmr_aligned* a, int64_t q_shift) {
   a->n_ops = o.n_ops;
 }
 
-}  // namespace
+}  // namespace dmr_api
+
+using namespace dmr_api;
 
 namespace dmr {
 void set_last_error(const std::string& msg) { g_err = msg; }
@@ -455,6 +378,42 @@ int dmr_ctx_create(int device, dmr_ctx** out) {
 }
 
 void dmr_ctx_destroy(dmr_ctx* ctx) { delete ctx; }
+
+int dmr_ctx_set_tuning(dmr_ctx* ctx, const char* key, double value) {
+  if (!ctx || !key) return fail(DMR_E_ARG, "NULL argument");
+  const std::string k = key;
+  dmr::Engine& e = *ctx->eng;
+  if (k == "slice_hits") {
+    if (!(value >= 1)) return fail(DMR_E_ARG, "slice_hits must be at least 1");
+    e.set_slice_hits((uint64_t)value);
+  } else if (k == "profile") {
+    e.set_profile(value != 0);
+  } else if (k == "split_mask") {
+    if (value < 0 || value > 31) return fail(DMR_E_ARG, "split_mask is a 5-bit mask");
+    e.set_split_mask((unsigned)value);
+  } else if (k == "score_plan") {
+    if (value != 0 && value != 1 && value != 2) return fail(DMR_E_ARG, "score_plan is 0, 1 or 2");
+    e.set_score_plan((int)value);
+  } else if (k == "f16") {
+    e.set_f16(value != 0);
+  } else {
+    return fail(DMR_E_ARG, "unknown tuning key: " + k);
+  }
+  return DMR_OK;
+}
+
+int dmr_ctx_get_tuning(const dmr_ctx* ctx, const char* key, double* value) {
+  if (!ctx || !key || !value) return fail(DMR_E_ARG, "NULL argument");
+  const std::string k = key;
+  const dmr::Engine& e = *ctx->eng;
+  if (k == "slice_hits") *value = (double)e.slice_hits();
+  else if (k == "split_mask") *value = e.split_mask();
+  else if (k == "score_plan") *value = e.score_plan();
+  else if (k == "f16") *value = e.packed_f16() ? 1 : 0;
+  else if (k == "last_slices") *value = e.last_slices();
+  else return fail(DMR_E_ARG, "unknown tuning key: " + k);
+  return DMR_OK;
+}
 
 uint32_t dmr_band_for(uint64_t len) { return dmr::band_for(len); }
 
@@ -927,31 +886,40 @@ int dmr_blastx(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint3
   dmr::GapRow gr;
   if (!dmr::blosum62_gap_row(o.gap_open, o.gap_extend, &gr))
     return fail(DMR_E_ARG, "Gap penalty settings are outside the supported range for BLOSUM62");
+  if (nreads && !reads && r_offsets[nreads] > r_offsets[0]) return fail(DMR_E_ARG, "NULL argument");
+  for (uint32_t i = 0; i < nreads; ++i)
+    if (r_offsets[i + 1] < r_offsets[i]) return fail(DMR_E_ARG, "read offsets are not monotone");
   return guard([&]() -> int {
     auto r = std::make_unique<dmr_result>();
     r->translated = true;
-    r->targets.offs.push_back(0);
     r->dna_len.resize(nreads);
-    for (uint32_t i = 0; i < nreads; ++i) {
-      std::string nt(reads + r_offsets[i], reads + r_offsets[i + 1]);
-      r->dna_len[i] = nt.size();
-      size_t mo = (cfg && cfg->min_orf >= 0) ? (size_t)cfg->min_orf : default_min_orf(nt.size());
-      std::string rc(nt.rbegin(), nt.rend());
-      for (auto& ch : rc) ch = nt_comp(ch);
-      for (int f = 0; f < 6; ++f) {
-        auto fr = translate(f < 3 ? nt : rc, (size_t)(f % 3));
-        mask_orfs(fr, mo);
-        bool any = false;
-        for (auto c : fr) any |= c < 23;
-        if (!any) continue;
-        r->targets.letters.insert(r->targets.letters.end(), fr.begin(), fr.end());
-        end_target(r->targets);
-        r->targets.query_of.push_back(i);
-        r->targets.frame_of.push_back((uint8_t)f);
-      }
-    }
-    finish(ctx, r.get(), o, nreads);
+    const auto mo = min_orf_lengths(r_offsets, nreads, cfg ? cfg->min_orf : -1);
+    for (uint32_t i = 0; i < nreads; ++i) r->dna_len[i] = r_offsets[i + 1] - r_offsets[i];
+    // Six frames, masking and the target set on the device
+    // (src/workflow.rs:112-127); the result keeps the host copy.
+    ctx->eng->load_reads(reads, r_offsets, nreads, mo.data(), &r->targets);
+    finish(ctx, r.get(), o, nreads, false);
+    r->stats.ms_translate = ctx->eng->last_translate_ms();
     *out = r.release();
+    return DMR_OK;
+  });
+}
+
+int dmr_six_frames(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint32_t nreads, int32_t min_orf,
+                   uint8_t* letters, uint64_t cap, uint64_t* frame_offsets, uint64_t* n_letters) {
+  if (!ctx || !r_offsets || !n_letters) return fail(DMR_E_ARG, "NULL argument");
+  if (nreads && !reads && r_offsets[nreads] > r_offsets[0]) return fail(DMR_E_ARG, "NULL argument");
+  for (uint32_t i = 0; i < nreads; ++i)
+    if (r_offsets[i + 1] < r_offsets[i]) return fail(DMR_E_ARG, "read offsets are not monotone");
+  return guard([&]() -> int {
+    const auto mo = min_orf_lengths(r_offsets, nreads, min_orf);
+    std::vector<uint8_t> v;
+    std::vector<uint64_t> off;
+    ctx->eng->six_frames(reads, r_offsets, nreads, mo.data(), &v, &off);
+    *n_letters = v.size();
+    if (frame_offsets) std::copy(off.begin(), off.end(), frame_offsets);
+    if (v.size() > cap || (!v.empty() && !letters)) return fail(DMR_E_CAPACITY, "letter capacity too small");
+    if (!v.empty()) std::memcpy(letters, v.data(), v.size());
     return DMR_OK;
   });
 }
@@ -985,11 +953,11 @@ int dmr_result_ops(const dmr_result* r, int64_t i, uint32_t* ops, uint32_t cap, 
 int dmr_result_stats(const dmr_result* r, double* o) {
   if (!r) return fail(DMR_E_ARG, "NULL result");
   const dmr::StageStats& s = r->stats;
-  double v[20] = {(double)s.query_seeds, (double)s.runs, (double)s.hits, (double)s.hit_runs,
+  double v[21] = {(double)s.query_seeds, (double)s.runs, (double)s.hits, (double)s.hit_runs,
                   (double)s.segments, (double)s.groups, (double)s.jobs, (double)s.maybe,
                   (double)s.selected, (double)s.traceback_cells, s.ms_seed, s.ms_join,
                   s.ms_sort_hits, s.ms_ungapped, s.ms_bands, s.ms_sw, s.ms_select, s.ms_traceback,
-                  s.ms_total, s.ms_host};
+                  s.ms_total, s.ms_host, s.ms_translate};
   std::memcpy(o, v, sizeof v);
   return DMR_OK;
 }

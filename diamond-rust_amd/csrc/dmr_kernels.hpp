@@ -11,7 +11,7 @@ namespace dmr {
 struct RefChunkDev {
   const uint64_t* keys;   // tagged keys, sorted
   const uint32_t* locs;   // global residue index of each seed
-  const uint32_t* table;  // bucket lower bounds, nbuckets + 1 entries
+  const uint64_t* table;  // bucket lower bounds, nbuckets + 1 entries
   uint64_t n;
   uint64_t nbuckets;
   int shift;              // bucket = key >> shift
@@ -23,20 +23,42 @@ void launch_seed_enumerate(const uint8_t* res, const uint32_t* seq_of, const uin
                            uint64_t r0, uint64_t r1, const SeedParams* P, uint32_t nshapes,
                            int key_bits, uint32_t* block_counts, uint64_t* block_off,
                            uint64_t* scan_tmp, uint64_t* keys, uint32_t* locs, hipStream_t st);
+// The two halves of launch_seed_enumerate: per-block counts and their scan
+// (the total is block_off[nblocks * nshapes]), then the emission.
+void launch_seed_count(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs, uint64_t r0,
+                       uint64_t r1, const SeedParams* P, uint32_t nshapes, int key_bits, uint32_t* block_counts,
+                       uint64_t* block_off, uint64_t* scan_tmp, hipStream_t st);
+void launch_seed_emit(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs, uint64_t r0,
+                      uint64_t r1, const SeedParams* P, uint32_t nshapes, int key_bits, const uint64_t* block_off,
+                      uint64_t* keys, uint32_t* locs, hipStream_t st);
 uint32_t seed_enum_blocks(uint64_t r0, uint64_t r1);
 void launch_bucket_table(const uint64_t* keys, uint64_t n, int shift, uint64_t nbuckets,
-                         uint32_t* table, hipStream_t st);
+                         uint64_t* table, hipStream_t st);
 void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st);
 void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n, uint32_t* out,
                           hipStream_t st);
 void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_t nruns,
                        uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                       uint64_t max_expand, uint32_t* counts, uint32_t* rlo, hipStream_t st);
+                       uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st);
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                      const uint32_t* counts, const uint32_t* rlo, const uint64_t* hit_off,
+                      const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off,
                       const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
-                      const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st);
+                      const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st,
+                      const uint32_t* slice_start = nullptr, const uint32_t* slice_nq = nullptr);
+// Run flags that also split a key's entries by record (group[target]).
+void launch_run_flags_grouped(const uint64_t* keys, const uint32_t* locs, const uint32_t* seq_of,
+                              const uint32_t* group, uint64_t n, uint32_t* flags, hipStream_t st);
+// thits[target] += hits of every kept (run, chunk) pair (thits zeroed first).
+void launch_target_hits(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
+                        uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint64_t* thits,
+                        hipStream_t st);
+// Per (run, chunk): the hits, first entry and entry count of the part whose
+// targets lie in [t0, t1).
+void launch_join_slice(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
+                       uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint32_t t0,
+                       uint32_t t1, uint32_t* slice_counts, uint32_t* slice_start, uint32_t* slice_nq,
+                       hipStream_t st);
 
 // ---- ungapped + bands ----------------------------------------------------
 struct SeqSet {
@@ -56,6 +78,21 @@ inline uint64_t pad_letters_bytes(uint64_t n, uint64_t total) { return total + n
 // Builds that layout from s.res/s.offs (offs non-decreasing from 0): pad_offs[k]
 // = offs[k] + k + 2.
 void launch_pad_letters(SeqSet s, uint32_t n, uint8_t* pad, uint32_t* pad_offs, hipStream_t st);
+
+// Six-frame translation and --min-orf masking (k_translate.hip). Reads are
+// ASCII bases at nt[roff[r]..roff[r+1]]; frame k of read r is slot 6r + k
+// at foff[6r + k] (6n + 1 offsets); min_orf holds one length per read.
+void launch_translate_frames(const uint8_t* nt, const uint64_t* roff, uint32_t n, const uint64_t* foff,
+                             uint8_t* out, hipStream_t st);
+void launch_mask_orfs(const uint64_t* foff, uint32_t nframes, const uint32_t* min_orf, const uint8_t* in,
+                      uint8_t* out, uint32_t* good, hipStream_t st);
+void launch_kept_lengths(const uint64_t* foff, const uint32_t* good, uint32_t nframes, uint32_t* keep,
+                         uint32_t* klen, hipStream_t st);
+void launch_compact_frames(const uint64_t* foff, const uint32_t* good, const uint64_t* kept_before,
+                           const uint64_t* tletters, const uint8_t* in, uint8_t* letters, uint32_t* toffs,
+                           uint32_t* query_of, uint8_t* frame_of, uint32_t nframes, hipStream_t st);
+// seq_of[p] = k for every letter p of sequence k (offs: n + 1 entries).
+void launch_seq_of(const uint32_t* offs, uint32_t n, uint32_t* seq_of, hipStream_t st);
 
 // Key flags over packed hits: 1 where (hit >> shift) changes.
 void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st);

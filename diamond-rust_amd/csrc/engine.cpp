@@ -59,7 +59,7 @@ void launch_xdrop_pairs(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint6
                         int xdrop, SegRec* out, hipStream_t st);
 void launch_join_emit_entries(const uint32_t* qs, const uint32_t* qp, const uint32_t* run_start,
                               uint64_t nruns, uint64_t nq_total, const uint32_t* rs,
-                              const uint32_t* rp, const uint32_t* counts, const uint32_t* rlo,
+                              const uint32_t* rp, const uint32_t* counts, const uint64_t* rlo,
                               const uint64_t* hit_off, HitGeom g, uint64_t* hits, hipStream_t st);
 
 namespace {
@@ -126,6 +126,10 @@ Engine::Engine(int device) : device_(device) {
   use_f16_ = !(hf && hf[0] == '0');
   const char* sp = std::getenv("DMR_SW16_SPLIT");
   split_mask_ = sp ? (unsigned)std::strtoul(sp, nullptr, 0) : 0x18u;
+  // Hits per slice of targets (device memory of the per-hit and per-DP
+  // buffers grows with it: ~52 bytes per hit and ~150 per DP job).
+  const char* sh = std::getenv("DMR_SLICE_HITS");
+  slice_hits_ = sh ? std::strtoull(sh, nullptr, 0) : (300ull << 20);
 }
 
 Engine::~Engine() {
@@ -177,40 +181,46 @@ AlpParams Engine::alp(int go, int ge) const {
                   kB62UngappedAlphaV);
 }
 
+void Engine::share_db(const Engine& other) {
+  if (other.device_ != device_) throw Error(-1, "engines of different devices cannot share a database");
+  db_ = other.db_;
+  idx_.reset();
+}
+
 void Engine::load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n,
                      const uint64_t* stat_letters) {
   DMR_HIP(hipSetDevice(device_));
   uint64_t total = offsets[n];
   if (total + n + 2 >= (1ull << 32)) throw Error(-5, "database larger than 4G letters");
-  h_db_offs_.resize(n + 1);
+  // A new resident database: engines sharing the previous one keep it.
+  auto d = std::make_shared<DbState>();
+  d->h_offs.resize(n + 1);
   uint32_t maxl = 0;
   std::vector<uint32_t> seq_of(total);
   for (uint32_t i = 0; i < n; ++i) {
-    h_db_offs_[i] = (uint32_t)offsets[i];
+    d->h_offs[i] = (uint32_t)offsets[i];
     uint32_t l = (uint32_t)(offsets[i + 1] - offsets[i]);
     maxl = std::max(maxl, l);
-    for (uint64_t k = offsets[i]; k < offsets[i + 1]; ++k) seq_of[k] = i;
+    std::fill(seq_of.begin() + (long)offsets[i], seq_of.begin() + (long)offsets[i + 1], i);
   }
-  h_db_offs_[n] = (uint32_t)total;
-  h_db_offs_sp_ = std::make_shared<const std::vector<uint32_t>>(h_db_offs_);
-  h_db_res_sp_ = std::make_shared<const std::vector<uint8_t>>(letters, letters + total);
-  DMR_HIP(hipMemcpy(db_res_.ensure<uint8_t>(total + 64), letters, total, hipMemcpyHostToDevice));
-  DMR_HIP(hipMemcpy(db_offs_.ensure<uint32_t>(n + 1), h_db_offs_.data(), (n + 1) * 4,
-                    hipMemcpyHostToDevice));
-  DMR_HIP(hipMemcpy(db_seq_of_.ensure<uint32_t>(total + 1), seq_of.data(), total * 4,
-                    hipMemcpyHostToDevice));
-  launch_pad_letters(SeqSet{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()}, n,
-                     db_pad_.ensure<uint8_t>(pad_letters_bytes(n, total)), db_pad_offs_.ensure<uint32_t>(n + 1),
-                     st_);
+  d->h_offs[n] = (uint32_t)total;
+  d->h_offs_sp = std::make_shared<const std::vector<uint32_t>>(d->h_offs);
+  d->h_res_sp = std::make_shared<const std::vector<uint8_t>>(letters, letters + total);
+  DMR_HIP(hipMemcpy(d->res.ensure<uint8_t>(total + 64), letters, total, hipMemcpyHostToDevice));
+  DMR_HIP(hipMemcpy(d->offs.ensure<uint32_t>(n + 1), d->h_offs.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+  DMR_HIP(hipMemcpy(d->seq_of.ensure<uint32_t>(total + 1), seq_of.data(), total * 4, hipMemcpyHostToDevice));
+  launch_pad_letters(SeqSet{d->res.as<uint8_t>(), d->offs.as<uint32_t>()}, n,
+                     d->pad.ensure<uint8_t>(pad_letters_bytes(n, total)), d->pad_offs.ensure<uint32_t>(n + 1), st_);
   DMR_HIP(hipStreamSynchronize(st_));
-  db_n_ = n;
-  db_letters_ = stat_letters ? *stat_letters : total;
-  db_max_len_ = maxl;
-  chunks_.clear();
-  idx_sens_.clear();
+  d->n = n;
+  d->letters = total;
+  d->stat_letters = stat_letters ? *stat_letters : total;
+  d->max_len = maxl;
+  db_ = std::move(d);
+  idx_.reset();
 }
 
-void Engine::upload_seed_params(const std::string& sens, double cut, uint32_t* nshapes,
+void Engine::upload_seed_params(const std::string& sens, double cut, DevBuf* params, uint32_t* nshapes,
                                 int* key_bits, int* shape_bits) {
   const auto& codes = shape_codes(sens);
   ReductionDef red = parse_reduction(reduction_def(sens));
@@ -238,17 +248,20 @@ void Engine::upload_seed_params(const std::string& sens, double cut, uint32_t* n
   *key_bits = kb;
   *shape_bits = P.nshapes > 1 ? bits_for(P.nshapes - 1) : 0;
   if (kb + *shape_bits > 64) throw Error(-5, "tagged seed key does not fit 64 bits");
-  DMR_HIP(hipMemcpyAsync(seed_params_.ensure<SeedParams>(1), &P, sizeof P, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(params->ensure<SeedParams>(1), &P, sizeof P, hipMemcpyHostToDevice, st_));
 }
 
 uint64_t Engine::index_entries() const {
   uint64_t n = 0;
-  for (auto& c : chunks_) n += c.n;
+  if (idx_)
+    for (auto& c : idx_->chunks) n += c.n;
   return n;
 }
 
 // Resident index: per DB chunk (src/seed.rs:146-164), all shapes enumerated
-// and sorted by tagged key, plus a bucket table over the top key bits.
+// and sorted by tagged key, plus a bucket table over the top key bits. One
+// index per preset is kept with the database; the first engine to ask for a
+// preset builds it (the others wait on the database's lock).
 void Engine::build_index(const std::string& sens, int index_chunks, double seed_cut) {
   DMR_HIP(hipSetDevice(device_));
   double cutv = (seed_cut >= 0 ? seed_cut : preset_seed_cut(sens));
@@ -258,14 +271,24 @@ void Engine::build_index(const std::string& sens, int index_chunks, double seed_
   parse_shape(codes[0], &s0);
   // cfg.seed_complexity_cut = seed_cut * ln 2 * shapes[0].weight (src/search.rs:102-105)
   double cut = cutv * 0.6931471805599453 * (double)s0.weight;
-  if (!chunks_.empty() && idx_sens_ == sens && idx_chunks_ == nch && idx_cut_ == cut) return;
+  char keybuf[128];
+  std::snprintf(keybuf, sizeof keybuf, "%s|%d|%.17g", sens.c_str(), nch, cut);
+  const std::string key = keybuf;
+  DbState& D = *db_;
+  std::lock_guard<std::mutex> lock(D.m);
+  for (auto& kv : D.indexes)
+    if (kv.first == key) {
+      idx_ = kv.second;
+      return;
+    }
   double t0 = now_ms();
-  upload_seed_params(sens, cut, &nshapes_, &key_bits_, &shape_bits_);
-  const int total_bits = key_bits_ + shape_bits_;
+  auto I = std::make_shared<IndexState>();
+  upload_seed_params(sens, cut, &I->seed_params, &I->nshapes, &I->key_bits, &I->shape_bits);
+  const int total_bits = I->key_bits + I->shape_bits;
   // chunks(n, parts)
   std::vector<std::pair<uint64_t, uint64_t>> ranges;
   {
-    uint64_t n = db_n_, parts = std::min<uint64_t>(std::max(nch, 1), std::max<uint64_t>(n, 1));
+    uint64_t n = D.n, parts = std::min<uint64_t>(std::max(nch, 1), std::max<uint64_t>(n, 1));
     uint64_t start = 0;
     for (uint64_t p = 0; p < parts && n > 0; ++p) {
       uint64_t end = (p + 1 == parts) ? n : start + n / parts + (p < n % parts ? 1 : 0);
@@ -273,27 +296,26 @@ void Engine::build_index(const std::string& sens, int index_chunks, double seed_
       start = end;
     }
   }
-  chunks_.clear();
-  chunks_.resize(ranges.size());
+  I->chunks.resize(ranges.size());
   std::vector<RefChunkDev> desc(ranges.size());
   for (size_t c = 0; c < ranges.size(); ++c) {
-    uint64_t r0 = h_db_offs_[ranges[c].first], r1 = h_db_offs_[ranges[c].second];
+    uint64_t r0 = D.h_offs[ranges[c].first], r1 = D.h_offs[ranges[c].second];
     uint32_t nb = seed_enum_blocks(r0, r1);
-    uint64_t nbk = (uint64_t)nb * nshapes_ + 1;
+    uint64_t nbk = (uint64_t)nb * I->nshapes + 1;
     uint32_t* bc = b_counts_.ensure<uint32_t>(nbk);
     launch_fill_u32(bc + nbk - 1, 1, 0, st_);
     uint64_t* bo = b_off_.ensure<uint64_t>(nbk);
     uint64_t* tmp = b_scan_.ensure<uint64_t>(scan_tmp_elems(nbk));
-    // Upper bound on entries: one per residue and shape.
-    uint64_t cap = (r1 - r0) * nshapes_ + 1;
-    IndexChunk& ch = chunks_[c];
-    uint64_t* keys = ch.keys.ensure<uint64_t>(cap);
-    uint32_t* locs = ch.locs.ensure<uint32_t>(cap);
-    launch_seed_enumerate(db_res_.as<uint8_t>(), db_seq_of_.as<uint32_t>(), db_offs_.as<uint32_t>(),
-                          r0, r1, seed_params_.as<SeedParams>(), nshapes_, key_bits_, bc, bo, tmp,
-                          keys, locs, st_);
+    // Count first, then allocate exactly (an index can be tens of GB).
+    IndexChunk& ch = I->chunks[c];
+    launch_seed_count(D.res.as<uint8_t>(), D.seq_of.as<uint32_t>(), D.offs.as<uint32_t>(), r0, r1,
+                      I->seed_params.as<SeedParams>(), I->nshapes, I->key_bits, bc, bo, tmp, st_);
     uint64_t n = read1(bo + nbk - 1);
     ch.n = n;
+    uint64_t* keys = ch.keys.ensure<uint64_t>(n + 1);
+    uint32_t* locs = ch.locs.ensure<uint32_t>(n + 1);
+    launch_seed_emit(D.res.as<uint8_t>(), D.seq_of.as<uint32_t>(), D.offs.as<uint32_t>(), r0, r1,
+                     I->seed_params.as<SeedParams>(), I->nshapes, I->key_bits, bo, keys, locs, st_);
     uint64_t* k2 = qkeys2_.ensure<uint64_t>(n + 1);
     uint32_t* l2 = qlocs2_.ensure<uint32_t>(n + 1);
     uint32_t* hist = b_hist_.ensure<uint32_t>(radix_hist_elems(n));
@@ -301,18 +323,22 @@ void Engine::build_index(const std::string& sens, int index_chunks, double seed_
     radix_sort_pairs(keys, locs, k2, l2, n, total_bits, hist, stmp, st_);
     int tb = std::min(total_bits, 22);
     ch.shift = total_bits - tb;
-    uint64_t maxkey = (((uint64_t)(nshapes_ - 1)) << key_bits_) | ((key_bits_ >= 64) ? ~0ull : ((1ull << key_bits_) - 1));
+    uint64_t maxkey = (((uint64_t)(I->nshapes - 1)) << I->key_bits) |
+                      ((I->key_bits >= 64) ? ~0ull : ((1ull << I->key_bits) - 1));
     ch.nbuckets = (maxkey >> ch.shift) + 1;
-    launch_bucket_table(keys, n, ch.shift, ch.nbuckets, ch.table.ensure<uint32_t>(ch.nbuckets + 1), st_);
-    desc[c] = RefChunkDev{keys, locs, ch.table.as<uint32_t>(), n, ch.nbuckets, ch.shift, 0};
+    launch_bucket_table(keys, n, ch.shift, ch.nbuckets, ch.table.ensure<uint64_t>(ch.nbuckets + 1), st_);
+    desc[c] = RefChunkDev{keys, locs, ch.table.as<uint64_t>(), n, ch.nbuckets, ch.shift, 0};
   }
-  DMR_HIP(hipMemcpyAsync(chunk_desc_.ensure<RefChunkDev>(desc.size() + 1), desc.data(),
+  DMR_HIP(hipMemcpyAsync(I->chunk_desc.ensure<RefChunkDev>(desc.size() + 1), desc.data(),
                          desc.size() * sizeof(RefChunkDev), hipMemcpyHostToDevice, st_));
   DMR_HIP(hipStreamSynchronize(st_));
-  idx_sens_ = sens;
-  idx_chunks_ = nch;
-  idx_cut_ = cut;
-  index_ms_ = now_ms() - t0;
+  // The sort's second buffers were sized for a whole chunk; searches size
+  // them for their query seeds again.
+  qkeys2_.release();
+  qlocs2_.release();
+  I->build_ms = now_ms() - t0;
+  D.indexes.push_back({key, I});
+  idx_ = I;
 }
 
 void Engine::load_targets(const TargetSet& t) {
@@ -333,44 +359,158 @@ void Engine::load_targets(const TargetSet& t) {
                          hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(q_seq_of_.ensure<uint32_t>(q_total_ + 1), seq_of.data(), q_total_ * 4,
                          hipMemcpyHostToDevice, st_));
+  // The record of each target (frames of one read share it).
+  if (t.query_of.size() != q_n_) throw Error(-1, "target set without a record per target");
+  DMR_HIP(hipMemcpyAsync(q_group_.ensure<uint32_t>(q_n_ + 1), t.query_of.data(), q_n_ * 4ull,
+                         hipMemcpyHostToDevice, st_));
   launch_pad_letters(SeqSet{q_res_.as<uint8_t>(), q_offs_.as<uint32_t>()}, q_n_,
                      q_pad_.ensure<uint8_t>(pad_letters_bytes(q_n_, q_total_)),
                      q_pad_offs_.ensure<uint32_t>(q_n_ + 1), st_);
   DMR_HIP(hipStreamSynchronize(st_));
 }
 
-std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt, StageStats* stats,
-                                                        std::shared_ptr<const OpsStore>* ops_store) {
+// Reads to the device, their six frames translated (src/translate.rs:54-82)
+// and masked (src/translate.rs:101-121) into r_masked_, with each frame's
+// count of letters below X in r_good_; *foff receives the 6n + 1 frame
+// offsets. Returns the number of frame letters.
+uint64_t Engine::frames_dev(const char* reads, const uint64_t* roffs, uint32_t n, const uint32_t* min_orf,
+                            std::vector<uint64_t>* foff) {
+  const uint64_t base = roffs[0], nt = roffs[n] - base;
+  std::vector<uint64_t> ro(n + 1);
+  foff->assign(6ull * n + 1, 0);
+  for (uint32_t r = 0; r <= n; ++r) ro[r] = roffs[r] - base;
+  for (uint32_t r = 0; r < n; ++r) {
+    if (ro[r + 1] < ro[r]) throw Error(-1, "read offsets are not monotone");
+    const uint64_t L = ro[r + 1] - ro[r];
+    for (int k = 0; k < 6; ++k) {
+      const uint64_t o = (uint64_t)(k % 3);
+      (*foff)[6ull * r + k + 1] = (*foff)[6ull * r + k] + (L >= o ? (L - o) / 3 : 0);
+    }
+  }
+  const uint64_t total = foff->back();
+  DMR_HIP(hipMemcpyAsync(r_nt_.ensure<uint8_t>(nt + 1), reads + base, nt, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(r_off_.ensure<uint64_t>(n + 1), ro.data(), (n + 1) * 8, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(r_minorf_.ensure<uint32_t>(n + 1), min_orf, (uint64_t)n * 4, hipMemcpyHostToDevice, st_));
+  DMR_HIP(hipMemcpyAsync(r_foff_.ensure<uint64_t>(6ull * n + 1), foff->data(), (6ull * n + 1) * 8,
+                         hipMemcpyHostToDevice, st_));
+  uint8_t* fr = r_frames_.ensure<uint8_t>(total + 1);
+  launch_translate_frames(r_nt_.as<uint8_t>(), r_off_.as<uint64_t>(), n, r_foff_.as<uint64_t>(), fr, st_);
+  launch_mask_orfs(r_foff_.as<uint64_t>(), 6 * n, r_minorf_.as<uint32_t>(), fr, r_masked_.ensure<uint8_t>(total + 1),
+                   r_good_.ensure<uint32_t>(6ull * n + 1), st_);
+  return total;
+}
+
+void Engine::six_frames(const char* reads, const uint64_t* roffs, uint32_t n, const uint32_t* min_orf,
+                        std::vector<uint8_t>* letters, std::vector<uint64_t>* offsets) {
   DMR_HIP(hipSetDevice(device_));
-  *ops_store = std::make_shared<OpsStore>();
+  letters->clear();
+  offsets->assign(1, 0);
+  if (!n) return;
+  const uint64_t total = frames_dev(reads, roffs, n, min_orf, offsets);
+  letters->resize(total);
+  if (total)
+    DMR_HIP(hipMemcpyAsync(letters->data(), r_masked_.as<uint8_t>(), total, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+}
+
+void Engine::load_reads(const char* reads, const uint64_t* roffs, uint32_t n, const uint32_t* min_orf,
+                        TargetSet* host) {
+  DMR_HIP(hipSetDevice(device_));
+  const double t0 = now_ms();
+  *host = TargetSet{};
+  host->offs.push_back(0);
+  std::vector<uint64_t> foff;
+  const uint64_t total = n ? frames_dev(reads, roffs, n, min_orf, &foff) : 0;
+  const uint32_t nf = 6 * n;
+  uint64_t ntargets = 0, nletters = 0;
+  if (nf) {
+    // Kept frames (a letter below X), their target numbers and letter offsets.
+    uint32_t* keep = r_keep_.ensure<uint32_t>(nf + 1);
+    uint32_t* klen = r_klen_.ensure<uint32_t>(nf + 1);
+    launch_kept_lengths(r_foff_.as<uint64_t>(), r_good_.as<uint32_t>(), nf, keep, klen, st_);
+    launch_fill_u32(keep + nf, 1, 0, st_);
+    launch_fill_u32(klen + nf, 1, 0, st_);
+    uint64_t* kept = r_kept_.ensure<uint64_t>(nf + 1);
+    uint64_t* tlet = r_tlet_.ensure<uint64_t>(nf + 1);
+    scan_exclusive_u32_to_u64(keep, kept, nf + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nf + 1)), st_);
+    scan_exclusive_u32_to_u64(klen, tlet, nf + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nf + 1)), st_);
+    uint64_t tail[2];
+    DMR_HIP(hipMemcpyAsync(&tail[0], kept + nf, 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(&tail[1], tlet + nf, 8, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    ntargets = tail[0];
+    nletters = tail[1];
+    if (nletters + ntargets + 2 >= (1ull << 32)) throw Error(-5, "query batch larger than 4G letters");
+    uint32_t* toffs = q_offs_.ensure<uint32_t>(ntargets + 1);
+    launch_compact_frames(r_foff_.as<uint64_t>(), r_good_.as<uint32_t>(), kept, tlet, r_masked_.as<uint8_t>(),
+                          q_res_.ensure<uint8_t>(nletters + 64), toffs, r_qof_.ensure<uint32_t>(ntargets + 1),
+                          r_fof_.ensure<uint8_t>(ntargets + 1), nf, st_);
+    const uint32_t end = (uint32_t)nletters;
+    DMR_HIP(hipMemcpyAsync(toffs + ntargets, &end, 4, hipMemcpyHostToDevice, st_));
+    host->letters.resize(nletters);
+    host->offs.resize(ntargets + 1);
+    host->query_of.resize(ntargets);
+    host->frame_of.resize(ntargets);
+    if (nletters)
+      DMR_HIP(hipMemcpyAsync(host->letters.data(), q_res_.as<uint8_t>(), nletters, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(host->offs.data(), toffs, (ntargets + 1) * 4, hipMemcpyDeviceToHost, st_));
+    if (ntargets) {
+      DMR_HIP(hipMemcpyAsync(host->query_of.data(), r_qof_.as<uint32_t>(), ntargets * 4, hipMemcpyDeviceToHost, st_));
+      DMR_HIP(hipMemcpyAsync(host->frame_of.data(), r_fof_.as<uint8_t>(), ntargets, hipMemcpyDeviceToHost, st_));
+    }
+    DMR_HIP(hipStreamSynchronize(st_));
+  }
+  (void)total;
+  // The device copies load_targets would make, from the device.
+  tset_ = *host;
+  q_n_ = (uint32_t)ntargets;
+  q_total_ = nletters;
+  q_max_len_ = 0;
+  for (uint64_t t = 0; t < ntargets; ++t) q_max_len_ = std::max(q_max_len_, host->offs[t + 1] - host->offs[t]);
+  if (!ntargets) launch_fill_u32(q_offs_.ensure<uint32_t>(1), 1, 0, st_);
+  if (ntargets)
+    DMR_HIP(hipMemcpyAsync(q_group_.ensure<uint32_t>(ntargets + 1), r_qof_.as<uint32_t>(), ntargets * 4,
+                           hipMemcpyDeviceToDevice, st_));
+  launch_seq_of(q_offs_.as<uint32_t>(), q_n_, q_seq_of_.ensure<uint32_t>(q_total_ + 1), st_);
+  launch_pad_letters(SeqSet{q_res_.ensure<uint8_t>(q_total_ + 64), q_offs_.as<uint32_t>()}, q_n_,
+                     q_pad_.ensure<uint8_t>(pad_letters_bytes(q_n_, q_total_)),
+                     q_pad_offs_.ensure<uint32_t>(q_n_ + 1), st_);
+  DMR_HIP(hipStreamSynchronize(st_));
+  translate_ms_ = now_ms() - t0;
+}
+
+std::vector<std::vector<HspRec>> Engine::search_targets(
+    const SearchOptions& opt, StageStats* stats, std::vector<std::shared_ptr<const OpsStore>>* ops_stores) {
+  DMR_HIP(hipSetDevice(device_));
   StageStats S;
   double t_start = now_ms();
   std::vector<std::vector<HspRec>> result(q_n_);
-  if (q_n_ == 0 || db_n_ == 0 || q_total_ == 0) {
+  slices_ = 0;
+  last_njobs_ = 0;
+  last_slices_ = 0;
+  if (q_n_ == 0 || db_->n == 0 || q_total_ == 0) {
     if (stats) *stats = S;
     return result;
   }
   build_index(opt.sensitivity, opt.index_chunks, opt.seed_cut);
   const AlpParams P = alp(opt.gap_open, opt.gap_extend);
-  const int xdrop = raw_score(P, 12.3);  // --xdrop 12.3 bits (src/search.rs:51-53)
-  const int cutoff = 19;                  // src/search.rs:164
-  const int total_bits = key_bits_ + shape_bits_;
+  const int total_bits = idx_->key_bits + idx_->shape_bits;
   SeqSet qs{q_res_.as<uint8_t>(), q_offs_.as<uint32_t>(), q_pad_.as<uint8_t>(), q_pad_offs_.as<uint32_t>()};
-  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>(), db_pad_.as<uint8_t>(),
-             db_pad_offs_.as<uint32_t>()};
+  SeqSet dbs{db_->res.as<uint8_t>(), db_->offs.as<uint32_t>(), db_->pad.as<uint8_t>(),
+             db_->pad_offs.as<uint32_t>()};
   stage_event(0);
 
   // ---- query seeds (all shapes, tagged, sorted) ---------------------------
   uint32_t nb = seed_enum_blocks(0, q_total_);
-  uint64_t nbk = (uint64_t)nb * nshapes_ + 1;
+  uint64_t nbk = (uint64_t)nb * idx_->nshapes + 1;
   uint32_t* bc = b_counts_.ensure<uint32_t>(nbk);
   launch_fill_u32(bc + nbk - 1, 1, 0, st_);
   uint64_t* bo = b_off_.ensure<uint64_t>(nbk);
-  uint64_t cap = q_total_ * nshapes_ + 1;
+  uint64_t cap = q_total_ * idx_->nshapes + 1;
   uint64_t* qk = qkeys_.ensure<uint64_t>(cap);
   uint32_t* ql = qlocs_.ensure<uint32_t>(cap);
   launch_seed_enumerate(q_res_.as<uint8_t>(), q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), 0,
-                        q_total_, seed_params_.as<SeedParams>(), nshapes_, key_bits_, bc, bo,
+                        q_total_, idx_->seed_params.as<SeedParams>(), idx_->nshapes, idx_->key_bits, bc, bo,
                         b_scan_.ensure<uint64_t>(scan_tmp_elems(nbk)), qk, ql, st_);
   const uint64_t nQ = read1(bo + nbk - 1);
   S.query_seeds = nQ;
@@ -383,53 +523,133 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   stage_event(1);
 
   // ---- join: runs of equal query keys x DB chunks --------------------------
+  // nq counts a key's entries over the whole batch (src/search.rs:109-131),
+  // or over one record when opt.per_record (a record searched alone).
   uint32_t* fl = flags_.ensure<uint32_t>(nQ + 1);
-  launch_run_flags(qk, nQ, fl, st_);
+  if (opt.per_record)
+    launch_run_flags_grouped(qk, ql, q_seq_of_.as<uint32_t>(), q_group_.as<uint32_t>(), nQ, fl, st_);
+  else
+    launch_run_flags(qk, nQ, fl, st_);
   launch_fill_u32(fl + nQ, 1, 0, st_);
   uint64_t* ps = pos_.ensure<uint64_t>(nQ + 1);
   scan_exclusive_u32_to_u64(fl, ps, nQ + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nQ + 1)), st_);
   const uint64_t nruns = read1(ps + nQ);
-  uint32_t* rs = runs_.ensure<uint32_t>(nruns + 1);
+  // Query runs stay in their own buffer: every slice reads them.
+  uint32_t* rs = qruns_.ensure<uint32_t>(nruns + 1);
   launch_compact_flags(fl, ps, nQ, rs, st_);
   S.runs = nruns;
-  const uint32_t nch = (uint32_t)chunks_.size();
+  const uint32_t nch = (uint32_t)idx_->chunks.size();
   const uint64_t npairs = nruns * nch;
   uint32_t* cnt = jc_counts_.ensure<uint32_t>(npairs + 1);
-  uint32_t* rlo = jc_rlo_.ensure<uint32_t>(npairs + 1);
-  launch_join_count(qk, rs, nruns, nQ, chunk_desc_.as<RefChunkDev>(), nch, opt.max_expand, cnt, rlo,
-                    st_);
+  uint64_t* rlo = jc_rlo_.ensure<uint64_t>(npairs + 1);
+  launch_join_count(qk, rs, nruns, nQ, idx_->chunk_desc.as<RefChunkDev>(), nch, opt.max_expand, cnt, rlo, st_);
   launch_fill_u32(cnt + npairs, 1, 0, st_);
+  // All hits at once when they fit the slice budget; else slices of targets
+  // by their hit counts (bounded device memory; the cap decisions above are
+  // the batch's, so slicing changes no result).
   uint64_t* hoff = jc_off_.ensure<uint64_t>(npairs + 1);
-  scan_exclusive_u32_to_u64(cnt, hoff, npairs + 1,
-                            b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
-  const uint64_t nhits = read1(hoff + npairs);
-  S.hits = nhits;
+  scan_exclusive_u32_to_u64(cnt, hoff, npairs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
+  const uint64_t all_hits = read1(hoff + npairs);
+  const uint64_t budget = opt.slice_hits ? opt.slice_hits : slice_hits_;
+  std::vector<std::pair<uint32_t, uint32_t>> slices;
+  std::vector<uint64_t> slice_total;
+  if (all_hits <= budget) {
+    slices.push_back({0, q_n_});
+    slice_total.push_back(all_hits);
+  } else {
+    uint64_t* thits = thits_.ensure<uint64_t>(q_n_ + 1);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(thits), 2ull * q_n_, 0, st_);
+    launch_target_hits(ql, rs, nruns, nQ, nch, cnt, q_seq_of_.as<uint32_t>(), thits, st_);
+    uint64_t* h_thits = thits_pin_.ensure<uint64_t>(q_n_ + 1);
+    DMR_HIP(hipMemcpyAsync(h_thits, thits, q_n_ * 8ull, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    uint32_t t0 = 0;
+    uint64_t acc = 0;
+    for (uint32_t t = 0; t < q_n_; ++t) {
+      if (t > t0 && acc + h_thits[t] > budget) {
+        slices.push_back({t0, t});
+        slice_total.push_back(acc);
+        t0 = t;
+        acc = 0;
+      }
+      acc += h_thits[t];
+    }
+    slices.push_back({t0, q_n_});
+    slice_total.push_back(acc);
+  }
+  stage_event(2);
+  DMR_HIP(hipStreamSynchronize(st_));
+  S.ms_seed = stage_ms(0, 1);
+  S.ms_join = stage_ms(1, 2);
+  slices_ = (uint32_t)slices.size();
+  for (uint64_t h : slice_total) S.hits += h;
   HitGeom g;
   g.bt = bits_for(q_n_ > 0 ? q_n_ - 1 : 0);
-  g.bs = bits_for(db_n_ > 0 ? db_n_ - 1 : 0);
-  g.bp = bits_for(db_max_len_);
-  g.bd = bits_for((uint64_t)q_max_len_ + db_max_len_);
-  g.diag_bias = db_max_len_;
+  g.bs = bits_for(db_->n > 0 ? db_->n - 1 : 0);
+  g.bp = bits_for(db_->max_len);
+  g.bd = bits_for((uint64_t)q_max_len_ + db_->max_len);
+  g.diag_bias = db_->max_len;
   const int hit_bits = (int)(g.bt + g.bs + g.bd + g.bp);
   if (hit_bits > 64) throw Error(-5, "hit key does not fit 64 bits for this batch");
-  uint64_t* hits = hits_.ensure<uint64_t>(nhits + 1);
-  launch_join_emit(ql, rs, nruns, nQ, chunk_desc_.as<RefChunkDev>(), nch, cnt, rlo, hoff,
-                   q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), db_seq_of_.as<uint32_t>(),
-                   db_offs_.as<uint32_t>(), g, hits, st_);
-  stage_event(2);
-  radix_sort_keys(hits, hits2_.ensure<uint64_t>(nhits + 1), nhits, hit_bits,
-                  b_hist_.ensure<uint32_t>(radix_hist_elems(nhits)),
-                  b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nhits)), st_);
-  stage_event(3);
+  for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = 0;
+  if (profile_) {
+    for (int c = 0; c < kClasses; ++c) prof_jobs_[c] = prof_cells_[c] = 0;
+    prof_dp_cells_ = 0;
+  }
+  for (size_t sl = 0; sl < slices.size(); ++sl) {
+    const uint32_t t0 = slices[sl].first, t1 = slices[sl].second;
+    const uint64_t nhits = slice_total[sl];
+    stage_event(2);
+    uint64_t* hits = hits_.ensure<uint64_t>(nhits + 1);
+    // The emission writes exactly the hits its offsets' scan totals; a
+    // slice's total is checked against its per-target count before any write.
+    const bool whole = slices.size() == 1;
+    uint32_t* scnt = whole ? cnt : jc_scnt_.ensure<uint32_t>(npairs + 1);
+    uint32_t* sst = whole ? nullptr : jc_sst_.ensure<uint32_t>(npairs + 1);
+    uint32_t* snq = whole ? nullptr : jc_snq_.ensure<uint32_t>(npairs + 1);
+    if (!whole) {
+      launch_join_slice(ql, rs, nruns, nQ, nch, cnt, q_seq_of_.as<uint32_t>(), t0, t1, scnt, sst, snq, st_);
+      launch_fill_u32(scnt + npairs, 1, 0, st_);
+      scan_exclusive_u32_to_u64(scnt, hoff, npairs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
+      if (read1(hoff + npairs) != nhits) throw Error(-2, "join offsets disagree with the per-target hit counts");
+    }
+    launch_join_emit(ql, rs, nruns, nQ, idx_->chunk_desc.as<RefChunkDev>(), nch, scnt, rlo, hoff,
+                     q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), db_->seq_of.as<uint32_t>(),
+                     db_->offs.as<uint32_t>(), g, hits, st_, sst, snq);
+    stage_event(11);
+    radix_sort_keys(hits, hits2_.ensure<uint64_t>(nhits + 1), nhits, hit_bits,
+                    b_hist_.ensure<uint32_t>(radix_hist_elems(nhits)),
+                    b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nhits)), st_);
+    stage_event(3);
+    search_slice(opt, P, g, hits, nhits, qs, dbs, &S, &result, ops_stores);
+    S.ms_join += stage_ms(2, 11);
+    S.ms_sort_hits += stage_ms(11, 3);
+  }
+  last_slices_ = slices_;
+  prof_valid_ = profile_;
+  S.ms_total = now_ms() - t_start;
+  if (stats) *stats = S;
+  return result;
+}
 
+// One slice of targets, from its sorted hits to its HSPs in result[target]:
+// ungapped extension, bands, the score pass, selection, traceback.
+void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const HitGeom& g, const uint64_t* hits,
+                          uint64_t nhits, SeqSet qs, SeqSet dbs, StageStats* Sp,
+                          std::vector<std::vector<HspRec>>* result_out,
+                          std::vector<std::shared_ptr<const OpsStore>>* ops_stores) {
+  StageStats& S = *Sp;
+  std::vector<std::vector<HspRec>>& result = *result_out;
+  const int xdrop = raw_score(P, 12.3);  // --xdrop 12.3 bits (src/search.rs:51-53)
+  const int cutoff = 19;                  // src/search.rs:164
   // ---- ungapped: runs of (target, subject, diagonal) -----------------------
-  fl = flags_.ensure<uint32_t>(nhits + 1);
+  uint32_t* fl = flags_.ensure<uint32_t>(nhits + 1);
   launch_key_flags(hits, nhits, (int)g.bp, fl, st_);
   launch_fill_u32(fl + nhits, 1, 0, st_);
-  ps = pos_.ensure<uint64_t>(nhits + 1);
+  uint64_t* ps = pos_.ensure<uint64_t>(nhits + 1);
   scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
   const uint64_t nhr = read1(ps + nhits);
-  S.hit_runs = nhr;
+  S.hit_runs += nhr;
   uint32_t* hrs = runs_.ensure<uint32_t>(nhr + 1);
   launch_compact_flags(fl, ps, nhits, hrs, st_);
   SegRec* sraw = segs_raw_.ensure<SegRec>(nhits + 1);
@@ -440,7 +660,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   ps = pos_.ensure<uint64_t>(nhits + 1);
   scan_exclusive_u32_to_u64(keep, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
   const uint64_t nsegs = read1(ps + nhits);
-  S.segments = nsegs;
+  S.segments += nsegs;
   SegRec* segs = segs_.ensure<SegRec>(nsegs + 1);
   uint64_t* gk = gkeys_.ensure<uint64_t>(nsegs + 1);
   launch_compact_segments(sraw, hits, keep, ps, nhits, (int)(g.bd + g.bp), segs, gk, st_);
@@ -453,7 +673,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   ps = pos_.ensure<uint64_t>(nsegs + 1);
   scan_exclusive_u32_to_u64(fl, ps, nsegs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nsegs + 1)), st_);
   const uint64_t ngroups = read1(ps + nsegs);
-  S.groups = ngroups;
+  S.groups += ngroups;
   uint32_t* gs = gstart_.ensure<uint32_t>(ngroups + 1);
   launch_compact_flags(fl, ps, nsegs, gs, st_);
   uint32_t* nbd = nbands_.ensure<uint32_t>(ngroups + 1);
@@ -464,7 +684,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   uint64_t* jo = joboff_.ensure<uint64_t>(ngroups + 1);
   scan_exclusive_u32_to_u64(nbd, jo, ngroups + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(ngroups + 1)), st_);
   const uint64_t njobs = read1(jo + ngroups);
-  S.jobs = njobs;
+  S.jobs += njobs;
   SwJob* jobs = jobs_.ensure<SwJob>(njobs + 1);
   launch_jobs_compact(tj, gs, nbd, jo, ngroups, jobs, st_);
   stage_event(5);
@@ -472,18 +692,27 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
   // ---- banded DP, score pass -----------------------------------------------
   SwOut* outs = outs_.ensure<SwOut>(njobs + 1);
   stage_event(6);
-  score_pass(jobs, njobs, (int)g.bt, qs, dbs, opt, outs, &S);
+  StageStats SS;
+  score_pass(jobs, njobs, (int)g.bt, qs, dbs, opt, outs, &SS);
   stage_event(7);
+  S.sw16_lanes += SS.sw16_lanes;
+  S.sw_generic += SS.sw_generic;
+  S.sw_redo += SS.sw_redo;
+  for (int c = 0; c < kClasses; ++c) {
+    S.ms_class[c] += SS.ms_class[c];
+    last_ms_class_[c] += SS.ms_class[c];
+  }
   last_njobs_ = njobs;
+  if (profile_) accumulate_profile(njobs);
 
   // ---- selection by E-value -------------------------------------------------
   fl = flags_.ensure<uint32_t>(njobs + 1);
-  launch_evalue_filter(jobs, outs, njobs, qs, dbs, P, db_letters_, opt.evalue, fl, st_);
+  launch_evalue_filter(jobs, outs, njobs, qs, dbs, P, db_->stat_letters, opt.evalue, fl, st_);
   launch_fill_u32(fl + njobs, 1, 0, st_);
   ps = pos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
   const uint64_t nmaybe = read1(ps + njobs);
-  S.maybe = nmaybe;
+  S.maybe += nmaybe;
   uint32_t* midx = sel_idx_.ensure<uint32_t>(nmaybe + 1);
   launch_compact_flags(fl, ps, njobs, midx, st_);
   // The candidates (job index, job, score-pass output) come back through
@@ -519,7 +748,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
     tb_pos[k] = tb_list.size();
     tb_list.push_back(h_midx[k]);
     tb_jobs.push_back(h_mjobs[k]);
-    tb_slen.push_back(h_db_offs_[h_mjobs[k].subject + 1] - h_db_offs_[h_mjobs[k].subject]);
+    tb_slen.push_back(db_->h_offs[h_mjobs[k].subject + 1] - db_->h_offs[h_mjobs[k].subject]);
   }
   stage_event(9);
   TbRun tb_run;
@@ -535,8 +764,8 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
       const int32_t sc = h_mouts[k].score;
       if (sc <= 0) continue;
       const uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
-      const uint64_t slen = h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
-      h_ev[k] = alp_evalue(P, sc, qlen, slen, db_letters_);
+      const uint64_t slen = db_->h_offs[j.subject + 1] - db_->h_offs[j.subject];
+      h_ev[k] = alp_evalue(P, sc, qlen, slen, db_->stat_letters);
     }
   });
   // Per group: bands in rank order; the first max_hsps whose exact E-value
@@ -546,6 +775,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
     HspRec h;
   };
   std::vector<std::vector<std::vector<Cand>>> per_target(q_n_);
+  std::vector<uint32_t> touched;
   {
     uint64_t k = 0;
     while (k < nmaybe) {
@@ -572,12 +802,15 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
       if (!hs.empty()) {
         std::stable_sort(hs.begin(), hs.end(),
                          [](const Cand& a, const Cand& b) { return hsp_less(a.h, b.h); });
-        per_target[hs[0].h.query].push_back(std::move(hs));
+        const uint32_t tq = hs[0].h.query;
+        if (per_target[tq].empty()) touched.push_back(tq);
+        per_target[tq].push_back(std::move(hs));
       }
     }
   }
-  std::vector<uint64_t> sel;  // indices into the maybe list
-  for (auto& found : per_target) {
+  uint64_t nsel = 0;
+  for (uint32_t tq : touched) {
+    auto& found = per_target[tq];
     std::stable_sort(found.begin(), found.end(),
                      [](const std::vector<Cand>& a, const std::vector<Cand>& b) {
                        return hsp_less(a[0].h, b[0].h);
@@ -585,58 +818,52 @@ std::vector<std::vector<HspRec>> Engine::search_targets(const SearchOptions& opt
     if (found.size() > opt.max_target_seqs) found.resize(opt.max_target_seqs);
     for (auto& hs : found) {
       if (hs.size() > opt.max_hsps) hs.resize(opt.max_hsps);
-      for (auto& c : hs) sel.push_back(c.job);
+      nsel += hs.size();
     }
   }
-  S.ms_host = now_ms() - th0;
-  S.selected = sel.size();
+  S.ms_host += now_ms() - th0;
+  S.selected += nsel;
 
   std::vector<SwOut> h_touts;
   std::vector<OpsRef> h_tops;
   std::shared_ptr<OpsStore> tb_store;
-  traceback_collect(&tb_run, &h_touts, &h_tops, &tb_store, &S.traceback_cells);
-  *ops_store = tb_store;
+  uint64_t tcells = 0;
+  traceback_collect(&tb_run, &h_touts, &h_tops, &tb_store, &tcells);
+  S.traceback_cells += tcells;
+  ops_stores->push_back(tb_store);
   stage_event(10);
   DMR_HIP(hipStreamSynchronize(st_));
 
   // ---- assemble HSPs in align_target order ----------------------------------
-  {
-    for (auto& found : per_target) {
-      for (auto& hs : found) {
-        for (auto& c : hs) {
-          const SwJob& j = h_mjobs[c.job];
-          const uint64_t k = tb_pos[c.job];  // selected DPs have a positive score
-          const SwOut& o = h_touts[k];
-          HspRec h = c.h;
-          h.q_begin = j.q0 + o.q_begin;
-          h.q_end = j.q0 + o.best_i;
-          h.s_begin = o.s_begin;
-          h.s_end = o.best_j;
-          h.identities = o.identities;
-          h.positives = o.positives;
-          h.mismatches = o.mismatches;
-          h.gap_openings = o.gap_openings;
-          h.gaps = o.gaps;
-          h.length = o.length;
-          h.ops = h_tops[k];
-          if (o.score != c.h.score)
-            throw Error(-2, "traceback score differs from the score pass");
-          result[h.query].push_back(std::move(h));
-        }
+  for (uint32_t tq : touched) {
+    for (auto& hs : per_target[tq]) {
+      for (auto& c : hs) {
+        const SwJob& j = h_mjobs[c.job];
+        const uint64_t k = tb_pos[c.job];  // selected DPs have a positive score
+        const SwOut& o = h_touts[k];
+        HspRec h = c.h;
+        h.q_begin = j.q0 + o.q_begin;
+        h.q_end = j.q0 + o.best_i;
+        h.s_begin = o.s_begin;
+        h.s_end = o.best_j;
+        h.identities = o.identities;
+        h.positives = o.positives;
+        h.mismatches = o.mismatches;
+        h.gap_openings = o.gap_openings;
+        h.gaps = o.gaps;
+        h.length = o.length;
+        h.ops = h_tops[k];
+        if (o.score != c.h.score)
+          throw Error(-2, "traceback score differs from the score pass");
+        result[h.query].push_back(std::move(h));
       }
     }
   }
-  S.ms_seed = stage_ms(0, 1);
-  S.ms_join = stage_ms(1, 2);
-  S.ms_sort_hits = stage_ms(2, 3);
-  S.ms_ungapped = stage_ms(3, 4);
-  S.ms_bands = stage_ms(4, 5);
-  S.ms_sw = stage_ms(6, 7);
-  S.ms_select = stage_ms(7, 8);
-  S.ms_traceback = stage_ms(9, 10);
-  S.ms_total = now_ms() - t_start;
-  if (stats) *stats = S;
-  return result;
+  S.ms_ungapped += stage_ms(3, 4);
+  S.ms_bands += stage_ms(4, 5);
+  S.ms_sw += stage_ms(6, 7);
+  S.ms_select += stage_ms(7, 8);
+  S.ms_traceback += stage_ms(9, 10);
 }
 
 // Score pass over all DP jobs: jobs of the standard band widths go to the
@@ -835,7 +1062,6 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     DMR_HIP(hipEventSynchronize(ev_[13 + 2 * kRowExact]));
     S->ms_class[kRowExact] = stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
   }
-  for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S->ms_class[c];
 }
 
 // Diagnostics (DMR_LANE_REPORT=1): per packed class, the band cells of its
@@ -852,7 +1078,7 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
   auto rows = [&](uint32_t k, int64_t* lo, int64_t* hi) {
     const SwJob& j = hj[k];
     const int64_t n = (int64_t)j.q1 - j.q0;
-    const int64_t m = (int64_t)h_db_offs_[j.subject + 1] - h_db_offs_[j.subject];
+    const int64_t m = (int64_t)db_->h_offs[j.subject + 1] - db_->h_offs[j.subject];
     *lo = std::max<int64_t>(1, (int64_t)j.d0 + 1);
     *hi = std::min<int64_t>(n, m + j.d1);
   };
@@ -1165,15 +1391,43 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector
   if (cells_out) *cells_out = cells;
 }
 
+// Jobs and DP cells per score class of the slice just scored (its jobs in
+// the class order of sw_idx_), added to the profile of the search.
+void Engine::accumulate_profile(uint64_t njobs) {
+  if (!njobs || last_cbeg_.size() != (size_t)kClasses) return;
+  SeqSet dbs{db_->res.as<uint8_t>(), db_->offs.as<uint32_t>()};
+  uint64_t* c = cells_.ensure<uint64_t>(njobs + 1);
+  launch_job_geometry(jobs_.as<SwJob>(), sw_idx_.as<uint32_t>(), njobs, dbs, c, nullptr, nullptr, st_);
+  std::vector<uint64_t> hc(njobs);
+  DMR_HIP(hipMemcpyAsync(hc.data(), c, njobs * 8, hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  for (int k = 0; k < kClasses; ++k)
+    for (uint64_t p = last_cbeg_[k]; p < last_cend_[k]; ++p) {
+      prof_jobs_[k]++;
+      prof_cells_[k] += hc[p];
+      prof_dp_cells_ += hc[p];
+    }
+}
+
 void Engine::last_class_profile(uint64_t jobs[kClasses], uint64_t cells[kClasses],
                                 double ms[kClasses]) {
   for (int c = 0; c < kClasses; ++c) {
     jobs[c] = cells[c] = 0;
     ms[c] = last_ms_class_[c];
   }
+  if (last_slices_ > 1) {
+    if (!prof_valid_)
+      throw Error(-5, "the last search ran in several slices: its class profile needs profiling on "
+                      "(dmr_ctx_set_tuning \"profile\" 1) before the search");
+    for (int c = 0; c < kClasses; ++c) {
+      jobs[c] = prof_jobs_[c];
+      cells[c] = prof_cells_[c];
+    }
+    return;
+  }
   const uint64_t n = last_njobs_;
   if (!n || last_cbeg_.size() != (size_t)kClasses) return;
-  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
+  SeqSet dbs{db_->res.as<uint8_t>(), db_->offs.as<uint32_t>()};
   uint64_t* c = cells_.ensure<uint64_t>(n + 1);
   launch_job_geometry(jobs_.as<SwJob>(), sw_idx_.as<uint32_t>(), n, dbs, c, nullptr, nullptr, st_);
   std::vector<uint64_t> hc(n);
@@ -1187,9 +1441,15 @@ void Engine::last_class_profile(uint64_t jobs[kClasses], uint64_t cells[kClasses
 }
 
 uint64_t Engine::count_last_dp_cells() {
+  if (last_slices_ > 1) {
+    if (!prof_valid_)
+      throw Error(-5, "the last search ran in several slices: its DP cells need profiling on "
+                      "(dmr_ctx_set_tuning \"profile\" 1) before the search");
+    return prof_dp_cells_;
+  }
   uint64_t n = last_njobs_;
   if (!n) return 0;
-  SeqSet dbs{db_res_.as<uint8_t>(), db_offs_.as<uint32_t>()};
+  SeqSet dbs{db_->res.as<uint8_t>(), db_->offs.as<uint32_t>()};
   uint64_t* cells = cells_.ensure<uint64_t>(n + 1);
   launch_job_geometry(jobs_.as<SwJob>(), nullptr, n, dbs, cells, nullptr, nullptr, st_);
   std::vector<uint64_t> h(n);
@@ -1295,9 +1555,9 @@ void Engine::seed_join(const uint64_t* qk, const uint32_t* qs, const uint32_t* q
   ch.n = nr;
   ch.shift = kb - tb;
   ch.nbuckets = (maxkey >> ch.shift) + 1;
-  ch.table = tab.ensure<uint32_t>(ch.nbuckets + 1);
+  ch.table = tab.ensure<uint64_t>(ch.nbuckets + 1);
   ch.pad = 0;
-  launch_bucket_table(ch.keys, nr, ch.shift, ch.nbuckets, tab.as<uint32_t>(), st_);
+  launch_bucket_table(ch.keys, nr, ch.shift, ch.nbuckets, tab.as<uint64_t>(), st_);
   DMR_HIP(hipMemcpyAsync(desc.ensure<RefChunkDev>(1), &ch, sizeof ch, hipMemcpyHostToDevice, st_));
   uint32_t* fl = dfl.ensure<uint32_t>(nq + 1);
   launch_run_flags(dqk.as<uint64_t>(), nq, fl, st_);
@@ -1308,7 +1568,7 @@ void Engine::seed_join(const uint64_t* qk, const uint32_t* qs, const uint32_t* q
   uint32_t* runs = druns.ensure<uint32_t>(nruns + 1);
   launch_compact_flags(fl, ps, nq, runs, st_);
   uint32_t* cnt = dcnt.ensure<uint32_t>(nruns + 1);
-  uint32_t* rlo = drlo.ensure<uint32_t>(nruns + 1);
+  uint64_t* rlo = drlo.ensure<uint64_t>(nruns + 1);
   launch_join_count(dqk.as<uint64_t>(), runs, nruns, nq, desc.as<RefChunkDev>(), 1, max_expand, cnt,
                     rlo, st_);
   launch_fill_u32(cnt + nruns, 1, 0, st_);
@@ -1469,6 +1729,8 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
     while ((1ull << tb) <= 2 * njobs) ++tb;
     score_pass(djobs.as<SwJob>(), njobs, tb, qs, ss, so, o, &S);
     last_dropin_stats_ = S;
+    for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S.ms_class[c];
+    last_slices_ = 1;
   }
   DMR_HIP(hipMemcpyAsync(outs->data(), o, njobs * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
   std::vector<uint32_t> hops(ooff[njobs]);

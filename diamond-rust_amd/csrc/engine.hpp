@@ -99,6 +99,11 @@ struct SearchOptions {
   double seed_cut = -1;   // <0: preset
   int gap_open = 11, gap_extend = 1;
   uint64_t max_expand = 4096;
+  // The cap counts a key's query entries per record instead of over the
+  // batch: each record's hits are those of searching it alone.
+  bool per_record = false;
+  // Hits per slice of targets (0 = the engine's setting).
+  uint64_t slice_hits = 0;
 };
 
 // Run-length alignment ops of one HSP (op | count << 2; 0 = M, 1 = I, 2 = D),
@@ -167,6 +172,7 @@ struct StageStats {
   double ms_class[kPackedClasses + kWaveKinds + 1] = {};
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;
+  double ms_translate = 0;  // blastx: six frames, masking and targets on the device
 };
 
 // A batch of query targets (one per protein query, or <= 6 frames per read).
@@ -177,6 +183,42 @@ struct TargetSet {
   std::vector<uint8_t> frame_of;
 };
 
+// One DB chunk of a resident seed index: tagged keys sorted, the global
+// letter index of each seed, and a bucket table of lower bounds over the
+// top key bits.
+struct IndexChunk {
+  DevBuf keys, locs, table;
+  uint64_t n = 0, nbuckets = 0;
+  int shift = 0;
+};
+
+// The seed index of one preset (shapes, reduction, cut) over the chunks of
+// the database (src/search.rs:121-133), resident until the database goes.
+struct IndexState {
+  std::vector<IndexChunk> chunks;
+  DevBuf chunk_desc, seed_params;
+  int key_bits = 0, shape_bits = 0;
+  uint32_t nshapes = 0;
+  double build_ms = 0;
+};
+
+// The resident database of a device and its seed indexes, shared by the
+// engines that search it (each with its own streams and scratch): letters,
+// offsets, owner of each letter, padded letters, host copies, and one index
+// per preset built so far (built once, under the lock).
+struct DbState {
+  DevBuf res, offs, seq_of, pad, pad_offs;
+  uint32_t n = 0;
+  uint64_t letters = 0;       // letters loaded
+  uint64_t stat_letters = 0;  // letters E-values use (Database::total_letters)
+  uint32_t max_len = 0;
+  std::vector<uint32_t> h_offs;
+  std::shared_ptr<const std::vector<uint32_t>> h_offs_sp;
+  std::shared_ptr<const std::vector<uint8_t>> h_res_sp;
+  std::mutex m;
+  std::vector<std::pair<std::string, std::shared_ptr<IndexState>>> indexes;
+};
+
 class Engine {
  public:
   explicit Engine(int device);
@@ -185,30 +227,55 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return st_; }
 
+  // Searches the resident database (and indexes) of another engine of the
+  // same device: engines of one device share it in HBM.
+  void share_db(const Engine& other);
   // Resident database (codes 0..24, offsets n+1). stat_letters: the letter
   // count E-values use (Database::total_letters, src/db.rs:236-238, read
   // from a file's header); NULL = the letters loaded.
   void load_db(const uint8_t* letters, const uint64_t* offsets, uint32_t n,
                const uint64_t* stat_letters = nullptr);
-  uint32_t db_size() const { return db_n_; }
-  uint64_t db_letters() const { return db_letters_; }
-  const std::vector<uint32_t>& db_offsets_host() const { return h_db_offs_; }
-  const std::vector<uint8_t>& db_letters_host() const { return *h_db_res_sp_; }
-  std::shared_ptr<const std::vector<uint32_t>> db_offsets_shared() const { return h_db_offs_sp_; }
-  std::shared_ptr<const std::vector<uint8_t>> db_letters_shared() const { return h_db_res_sp_; }
+  uint32_t db_size() const { return db_->n; }
+  uint64_t db_letters() const { return db_->stat_letters; }
+  const std::vector<uint32_t>& db_offsets_host() const { return db_->h_offs; }
+  const std::vector<uint8_t>& db_letters_host() const { return *db_->h_res_sp; }
+  std::shared_ptr<const std::vector<uint32_t>> db_offsets_shared() const { return db_->h_offs_sp; }
+  std::shared_ptr<const std::vector<uint8_t>> db_letters_shared() const { return db_->h_res_sp; }
 
   // Resident seed index for a preset (rebuilt only when the preset changes).
   void build_index(const std::string& sensitivity, int index_chunks, double seed_cut);
-  double last_index_ms() const { return index_ms_; }
+  double last_index_ms() const { return idx_ ? idx_->build_ms : 0.0; }
   uint64_t index_entries() const;
 
   // Upload a target batch (kept resident until the next call).
   void load_targets(const TargetSet& t);
+  // blastx targets made on the device (src/workflow.rs:112-127): the six
+  // frames of every read (ASCII bases reads[roffs[r]..roffs[r+1]]), masked
+  // with min_orf[r], frames without a letter below X dropped; kept resident
+  // as load_targets keeps them, and returned in *host (the result's copy).
+  void load_reads(const char* reads, const uint64_t* roffs, uint32_t n, const uint32_t* min_orf,
+                  TargetSet* host);
+  // translate::six_frames + mask_short_orfs of every read: all 6n frames in
+  // slot order (letters and 6n + 1 offsets).
+  void six_frames(const char* reads, const uint64_t* roffs, uint32_t n, const uint32_t* min_orf,
+                  std::vector<uint8_t>* letters, std::vector<uint64_t>* offsets);
+  double last_translate_ms() const { return translate_ms_; }
   // Search the resident targets; HSPs per target, ordered as align_target
   // returns them (src/search.rs:262-270).
   // The HSPs' ops live in *ops_store, which the caller keeps with them.
   std::vector<std::vector<HspRec>> search_targets(const SearchOptions& opt, StageStats* stats,
-                                                  std::shared_ptr<const OpsStore>* ops_store);
+                                                  std::vector<std::shared_ptr<const OpsStore>>* ops_stores);
+  // Slices of targets the last search ran in (hits bounded per slice).
+  uint32_t last_slices() const { return last_slices_; }
+  uint64_t slice_hits() const { return slice_hits_; }
+  void set_slice_hits(uint64_t v) { slice_hits_ = v ? v : 1; }
+  // Profiling: per-class jobs and DP cells of every slice of a search
+  // (dmr_last_class_profile / dmr_last_dp_cells of sliced searches).
+  void set_profile(bool on) { profile_ = on; }
+  void set_split_mask(unsigned m) { split_mask_ = m & 0x1fu; }
+  void set_score_plan(int p) { score_plan_ = p; }
+  void set_f16(bool on) { use_f16_ = on; }
+  int score_plan() const { return score_plan_; }
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
   unsigned split_mask() const { return split_mask_; }
@@ -281,16 +348,23 @@ class Engine {
   // Device address of the no-allocation array on the current device.
   static void* noalloc_scratch_address();
   void* noalloc_ = nullptr;
-  struct IndexChunk {
-    DevBuf keys, locs, table;
-    uint64_t n = 0, nbuckets = 0;
-    int shift = 0;
-  };
-  void upload_seed_params(const std::string& sens, double cut, uint32_t* nshapes, int* key_bits,
+  void upload_seed_params(const std::string& sens, double cut, DevBuf* params, uint32_t* nshapes, int* key_bits,
                           int* shape_bits);
   bool swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, bool fast) const;
   void score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
                   const SearchOptions& opt, SwOut* outs, StageStats* S);
+  void search_slice(const SearchOptions& opt, const AlpParams& P, const HitGeom& g, const uint64_t* hits,
+                    uint64_t nhits, SeqSet qs, SeqSet dbs, StageStats* S,
+                    std::vector<std::vector<HspRec>>* result,
+                    std::vector<std::shared_ptr<const OpsStore>>* ops_stores);
+  void accumulate_profile(uint64_t njobs);
+  uint32_t slices_ = 0, last_slices_ = 0;
+  uint64_t slice_hits_ = 300ull << 20;
+  bool profile_ = false, prof_valid_ = false;
+  uint64_t prof_jobs_[kPackedClasses + kWaveKinds + 1] = {}, prof_cells_[kPackedClasses + kWaveKinds + 1] = {};
+  uint64_t prof_dp_cells_ = 0;
+  DevBuf thits_, jc_scnt_, jc_sst_, jc_snq_, q_group_, qruns_;
+  PinBuf thits_pin_;
   void stage_event(int k);
   float stage_ms(int a, int b);
   template <class T>
@@ -311,25 +385,9 @@ class Engine {
   void fork();
   void join();
 
-  // database
-  DevBuf db_res_, db_offs_, db_seq_of_;
-  DevBuf db_pad_, db_pad_offs_;  // padded letters of the packed score kernels
-  uint32_t db_n_ = 0;
-  uint64_t db_letters_ = 0;
-  uint32_t db_max_len_ = 0;
-  std::vector<uint32_t> h_db_offs_;
-  std::shared_ptr<const std::vector<uint32_t>> h_db_offs_sp_;
-  std::shared_ptr<const std::vector<uint8_t>> h_db_res_sp_;
-
-  // index
-  std::string idx_sens_;
-  int idx_chunks_ = 0;
-  double idx_cut_ = 0;
-  std::vector<IndexChunk> chunks_;
-  DevBuf chunk_desc_, seed_params_, ref_params_;
-  int key_bits_ = 0, shape_bits_ = 0;
-  uint32_t nshapes_ = 0;
-  double index_ms_ = 0;
+  // database and the index of the current preset (shared)
+  std::shared_ptr<DbState> db_ = std::make_shared<DbState>();
+  std::shared_ptr<IndexState> idx_;
 
   // matrix (32 x 32 int8, padded letters score -64)
   DevBuf matrix32_;
@@ -340,6 +398,14 @@ class Engine {
   DevBuf q_pad_, q_pad_offs_;
   uint32_t q_n_ = 0, q_max_len_ = 0;
   uint64_t q_total_ = 0;
+
+  // blastx reads: bases, offsets, min_orf, frame offsets, translated and
+  // masked frames, per-frame counts and the compaction's scans.
+  DevBuf r_nt_, r_off_, r_minorf_, r_foff_, r_frames_, r_masked_, r_good_, r_keep_, r_klen_, r_kept_,
+      r_tlet_, r_qof_, r_fof_;
+  uint64_t frames_dev(const char* reads, const uint64_t* roffs, uint32_t n, const uint32_t* min_orf,
+                      std::vector<uint64_t>* foff);
+  double translate_ms_ = 0;
 
   // per-search scratch
   DevBuf b_counts_, b_off_, b_scan_, b_hist_;

@@ -102,7 +102,7 @@ __global__ void k_seed_emit(const uint8_t* __restrict__ res, const uint32_t* __r
 
 // table[b] = first index whose key >= b << shift (lower bound per bucket).
 __global__ void k_bucket_table(const uint64_t* __restrict__ keys, uint64_t n, int shift,
-                               uint64_t nbuckets, uint32_t* __restrict__ table) {
+                               uint64_t nbuckets, uint64_t* __restrict__ table) {
   uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b > nbuckets) return;
   uint64_t target = b << shift;
@@ -112,13 +112,26 @@ __global__ void k_bucket_table(const uint64_t* __restrict__ keys, uint64_t n, in
     if (keys[mid] < target) lo = mid + 1;
     else hi = mid;
   }
-  table[b] = (uint32_t)lo;
+  table[b] = lo;
 }
 
 __global__ void k_run_flags(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flags) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// Runs of equal keys within one record: a run also ends where the record of
+// its entries' target changes (entries of a key are in letter order, and a
+// record's targets are contiguous), so each record's seeds meet the cap
+// alone, as when the record is searched by itself.
+__global__ void k_run_flags_grouped(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ locs,
+                                    const uint32_t* __restrict__ seq_of, const uint32_t* __restrict__ group,
+                                    uint64_t n, uint32_t* __restrict__ flags) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = (i == 0 || keys[i] != keys[i - 1] || group[seq_of[locs[i]]] != group[seq_of[locs[i - 1]]]) ? 1u
+                                                                                                      : 0u;
 }
 
 __global__ void k_compact_flags(const uint32_t* __restrict__ flags, const uint64_t* __restrict__ pos,
@@ -129,21 +142,21 @@ __global__ void k_compact_flags(const uint32_t* __restrict__ flags, const uint64
 }
 
 __device__ __forceinline__ void ref_range(const uint64_t* __restrict__ rkeys,
-                                          const uint32_t* __restrict__ table, int shift,
-                                          uint64_t key, uint32_t* lo_out, uint32_t* hi_out) {
+                                          const uint64_t* __restrict__ table, int shift,
+                                          uint64_t key, uint64_t* lo_out, uint64_t* hi_out) {
   uint64_t b = key >> shift;
-  uint32_t lo = table[b], hi = table[b + 1];
+  uint64_t lo = table[b], hi = table[b + 1];
   // lower_bound(key) and upper_bound(key) inside the bucket.
-  uint32_t a = lo, z = hi;
+  uint64_t a = lo, z = hi;
   while (a < z) {
-    uint32_t mid = (a + z) >> 1;
+    uint64_t mid = (a + z) >> 1;
     if (rkeys[mid] < key) a = mid + 1;
     else z = mid;
   }
-  uint32_t first = a;
+  uint64_t first = a;
   z = hi;
   while (a < z) {
-    uint32_t mid = (a + z) >> 1;
+    uint64_t mid = (a + z) >> 1;
     if (rkeys[mid] <= key) a = mid + 1;
     else z = mid;
   }
@@ -155,7 +168,7 @@ __device__ __forceinline__ void ref_range(const uint64_t* __restrict__ rkeys,
 __global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t* __restrict__ run_start,
                              uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
                              uint32_t nchunks, uint64_t max_expand, uint32_t* __restrict__ counts,
-                             uint32_t* __restrict__ rlo) {
+                             uint64_t* __restrict__ rlo) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nruns * nchunks) return;
   uint64_t r = t / nchunks;
@@ -164,7 +177,7 @@ __global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t*
   uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
   uint64_t key = qkeys[s0];
   const RefChunkDev ch = chunks[c];
-  uint32_t lo = 0, hi = 0;
+  uint64_t lo = 0, hi = 0;
   if (ch.n > 0 && (key >> ch.shift) < ch.nbuckets) ref_range(ch.keys, ch.table, ch.shift, key, &lo, &hi);
   uint64_t nq = s1 - s0, nr = hi - lo;
   uint64_t cnt = nq * nr;
@@ -172,14 +185,67 @@ __global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t*
   rlo[t] = lo;
 }
 
+// Hits per target: every entry of a kept (run, chunk) pair contributes its
+// nr reference entries to its target.
+__global__ void k_target_hits(const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start,
+                              uint64_t nruns, uint64_t nq_total, uint32_t nchunks,
+                              const uint32_t* __restrict__ counts, const uint32_t* __restrict__ q_seq_of,
+                              unsigned long long* __restrict__ thits) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nruns * nchunks) return;
+  const uint32_t cnt = counts[t];
+  if (cnt == 0) return;
+  const uint64_t r = t / nchunks;
+  const uint64_t s0 = run_start[r], s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
+  const uint32_t nr = cnt / (uint32_t)(s1 - s0);
+  for (uint64_t a = s0; a < s1; ++a) atomicAdd(&thits[q_seq_of[qlocs[a]]], (unsigned long long)nr);
+}
+
+// The part of each kept (run, chunk) pair whose entries' targets lie in
+// [t0, t1): its first entry, entry count and hit count (entries of a run are
+// in letter order, so the part is one range, found by binary search).
+__global__ void k_join_slice(const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start,
+                             uint64_t nruns, uint64_t nq_total, uint32_t nchunks,
+                             const uint32_t* __restrict__ counts, const uint32_t* __restrict__ q_seq_of,
+                             uint32_t t0, uint32_t t1, uint32_t* __restrict__ slice_counts,
+                             uint32_t* __restrict__ slice_start, uint32_t* __restrict__ slice_nq) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nruns * nchunks) return;
+  const uint32_t cnt = counts[t];
+  const uint64_t r = t / nchunks;
+  const uint64_t s0 = run_start[r], s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
+  uint32_t out = 0, e0 = (uint32_t)s0, ne = 0;
+  if (cnt) {
+    const uint32_t nr = cnt / (uint32_t)(s1 - s0);
+    auto first_at_least = [&](uint32_t tg) {
+      uint64_t lo = s0, hi = s1;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (q_seq_of[qlocs[mid]] < tg) lo = mid + 1;
+        else hi = mid;
+      }
+      return lo;
+    };
+    const uint64_t a = first_at_least(t0), b = first_at_least(t1);
+    e0 = (uint32_t)a;
+    ne = (uint32_t)(b - a);
+    out = ne * nr;
+  }
+  slice_counts[t] = out;
+  slice_start[t] = e0;
+  slice_nq[t] = ne;
+}
+
 // One thread per (run, chunk) pair; writes its nq*nr hits as packed keys.
+// With slice_start / slice_nq, only the pair's entries in that range.
 __global__ void k_join_emit(const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start,
                             uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
                             uint32_t nchunks, const uint32_t* __restrict__ counts,
-                            const uint32_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
+                            const uint64_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
                             const uint32_t* __restrict__ q_seq_of, const uint32_t* __restrict__ q_offs,
                             const uint32_t* __restrict__ db_seq_of, const uint32_t* __restrict__ db_offs,
-                            HitGeom g, uint64_t* __restrict__ hits) {
+                            HitGeom g, uint64_t* __restrict__ hits, const uint32_t* __restrict__ slice_start,
+                            const uint32_t* __restrict__ slice_nq) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nruns * nchunks) return;
   uint32_t cnt = counts[t];
@@ -188,7 +254,12 @@ __global__ void k_join_emit(const uint32_t* __restrict__ qlocs, const uint32_t* 
   uint32_t c = (uint32_t)(t - r * nchunks);
   uint64_t s0 = run_start[r];
   uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
-  uint32_t nq = (uint32_t)(s1 - s0), nr = cnt / nq;
+  uint32_t nq = (uint32_t)(s1 - s0);
+  if (slice_start) {  // cnt counts the slice's entries only
+    s0 = slice_start[t];
+    nq = slice_nq[t];
+  }
+  const uint32_t nr = cnt / nq;
   const uint32_t* rl = chunks[c].locs + rlo[t];
   uint64_t o = hit_off[t];
   for (uint32_t a = 0; a < nq; ++a) {
@@ -211,7 +282,7 @@ __global__ void k_join_emit_entries(const uint32_t* __restrict__ qs, const uint3
                                     const uint32_t* __restrict__ run_start, uint64_t nruns,
                                     uint64_t nq_total, const uint32_t* __restrict__ rs,
                                     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ counts,
-                                    const uint32_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
+                                    const uint64_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
                                     HitGeom g, uint64_t* __restrict__ hits) {
   uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nruns) return;
@@ -235,25 +306,40 @@ __global__ void k_join_emit_entries(const uint32_t* __restrict__ qs, const uint3
 
 void launch_join_emit_entries(const uint32_t* qs, const uint32_t* qp, const uint32_t* run_start,
                               uint64_t nruns, uint64_t nq_total, const uint32_t* rs,
-                              const uint32_t* rp, const uint32_t* counts, const uint32_t* rlo,
+                              const uint32_t* rp, const uint32_t* counts, const uint64_t* rlo,
                               const uint64_t* hit_off, HitGeom g, uint64_t* hits, hipStream_t st) {
   if (!nruns) return;
   hipLaunchKernelGGL(k_join_emit_entries, dim3((unsigned)((nruns + 255) / 256)), dim3(256), 0, st, qs,
                      qp, run_start, nruns, nq_total, rs, rp, counts, rlo, hit_off, g, hits);
 }
 
-void launch_seed_enumerate(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs,
-                           uint64_t r0, uint64_t r1, const SeedParams* P, uint32_t nshapes,
-                           int key_bits, uint32_t* block_counts, uint64_t* block_off,
-                           uint64_t* scan_tmp, uint64_t* keys, uint32_t* locs, hipStream_t st) {
+void launch_seed_count(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs, uint64_t r0,
+                       uint64_t r1, const SeedParams* P, uint32_t nshapes, int key_bits, uint32_t* block_counts,
+                       uint64_t* block_off, uint64_t* scan_tmp, hipStream_t st) {
   if (r1 <= r0) return;
   uint32_t nb = (uint32_t)((r1 - r0 + kSeedThreads - 1) / kSeedThreads);
   dim3 grid(nb, nshapes);
   hipLaunchKernelGGL(k_seed_count, grid, dim3(kSeedThreads), 0, st, res, seq_of, offs, r0, r1, P,
                      key_bits, block_counts);
   scan_exclusive_u32_to_u64(block_counts, block_off, (uint64_t)nb * nshapes + 1, scan_tmp, st);
+}
+
+void launch_seed_emit(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs, uint64_t r0,
+                      uint64_t r1, const SeedParams* P, uint32_t nshapes, int key_bits, const uint64_t* block_off,
+                      uint64_t* keys, uint32_t* locs, hipStream_t st) {
+  if (r1 <= r0) return;
+  uint32_t nb = (uint32_t)((r1 - r0 + kSeedThreads - 1) / kSeedThreads);
+  dim3 grid(nb, nshapes);
   hipLaunchKernelGGL(k_seed_emit, grid, dim3(kSeedThreads), 0, st, res, seq_of, offs, r0, r1, P,
                      key_bits, block_off, keys, locs);
+}
+
+void launch_seed_enumerate(const uint8_t* res, const uint32_t* seq_of, const uint32_t* offs,
+                           uint64_t r0, uint64_t r1, const SeedParams* P, uint32_t nshapes,
+                           int key_bits, uint32_t* block_counts, uint64_t* block_off,
+                           uint64_t* scan_tmp, uint64_t* keys, uint32_t* locs, hipStream_t st) {
+  launch_seed_count(res, seq_of, offs, r0, r1, P, nshapes, key_bits, block_counts, block_off, scan_tmp, st);
+  launch_seed_emit(res, seq_of, offs, r0, r1, P, nshapes, key_bits, block_off, keys, locs, st);
 }
 
 uint32_t seed_enum_blocks(uint64_t r0, uint64_t r1) {
@@ -261,7 +347,7 @@ uint32_t seed_enum_blocks(uint64_t r0, uint64_t r1) {
 }
 
 void launch_bucket_table(const uint64_t* keys, uint64_t n, int shift, uint64_t nbuckets,
-                         uint32_t* table, hipStream_t st) {
+                         uint64_t* table, hipStream_t st) {
   uint64_t threads = nbuckets + 1;
   hipLaunchKernelGGL(k_bucket_table, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, keys,
                      n, shift, nbuckets, table);
@@ -281,7 +367,7 @@ void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n
 
 void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_t nruns,
                        uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                       uint64_t max_expand, uint32_t* counts, uint32_t* rlo, hipStream_t st) {
+                       uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st) {
   uint64_t n = nruns * nchunks;
   if (!n) return;
   hipLaunchKernelGGL(k_join_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qkeys,
@@ -290,14 +376,41 @@ void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_
 
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                      const uint32_t* counts, const uint32_t* rlo, const uint64_t* hit_off,
+                      const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off,
                       const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
-                      const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st) {
+                      const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st,
+                      const uint32_t* slice_start, const uint32_t* slice_nq) {
   uint64_t n = nruns * nchunks;
   if (!n) return;
   hipLaunchKernelGGL(k_join_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs,
                      run_start, nruns, nq_total, chunks, nchunks, counts, rlo, hit_off, q_seq_of,
-                     q_offs, db_seq_of, db_offs, g, hits);
+                     q_offs, db_seq_of, db_offs, g, hits, slice_start, slice_nq);
+}
+
+void launch_run_flags_grouped(const uint64_t* keys, const uint32_t* locs, const uint32_t* seq_of,
+                              const uint32_t* group, uint64_t n, uint32_t* flags, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_run_flags_grouped, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, locs,
+                     seq_of, group, n, flags);
+}
+
+void launch_target_hits(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
+                        uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint64_t* thits,
+                        hipStream_t st) {
+  uint64_t n = nruns * nchunks;
+  if (!n) return;
+  hipLaunchKernelGGL(k_target_hits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs, run_start,
+                     nruns, nq_total, nchunks, counts, q_seq_of, reinterpret_cast<unsigned long long*>(thits));
+}
+
+void launch_join_slice(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
+                       uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint32_t t0,
+                       uint32_t t1, uint32_t* slice_counts, uint32_t* slice_start, uint32_t* slice_nq,
+                       hipStream_t st) {
+  uint64_t n = nruns * nchunks;
+  if (!n) return;
+  hipLaunchKernelGGL(k_join_slice, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs, run_start, nruns,
+                     nq_total, nchunks, counts, q_seq_of, t0, t1, slice_counts, slice_start, slice_nq);
 }
 
 }  // namespace dmr

@@ -186,6 +186,33 @@ def protein_db_2p26(seed=20240601, n_queries=1024):
     return db, ids, q1, qs
 
 
+def protein_db_60m(seed=20240601, n_queries=4096, n_db=200_000):
+    """The 200,000-protein (~60 M letters) DB of the --very-sensitive and
+    blastx workloads: background lengths log-normal (median 250, sigma 0.57,
+    clipped 30..5000), 4096 queries (median 300, clipped 50..1000), each with
+    0-10 remote homologs planted at 20-35% identity with indels at 0.04 per
+    site."""
+    rng = np.random.default_rng(seed + 60)
+    qlens = _lengths(rng, n_queries, 300, 0.57, 50, 1000)
+    q_all = _residues(rng, int(qlens.sum()))
+    q_off = np.concatenate([[0], np.cumsum(qlens)])
+    qs = [q_all[q_off[i]:q_off[i + 1]] for i in range(n_queries)]
+    planted = []
+    for qi in range(n_queries):
+        for _ in range(int(rng.integers(0, 11))):
+            planted.append((qi, mutate(rng, qs[qi], rng.uniform(0.20, 0.35), 0.04)))
+    n_bg = n_db - len(planted)
+    bg_lens = _lengths(rng, n_bg, 250, 0.57, 30, 5000)
+    bg_all = _residues(rng, int(bg_lens.sum()))
+    bg_off = np.concatenate([[0], np.cumsum(bg_lens)])
+    db = [bg_all[bg_off[i]:bg_off[i + 1]] for i in range(n_bg)] + [p[1] for p in planted]
+    tags = ["bg"] * n_bg + [f"hq{p[0]}" for p in planted]
+    perm = rng.permutation(len(db))
+    db = [db[i] for i in perm]
+    ids = [f"v{k:06d}_{tags[i]}" for k, i in enumerate(perm)]
+    return db, ids, qs
+
+
 # Standard genetic code, amino acid -> synonymous codons (reverse translation
 # of planted proteins into reads).
 _CODONS = {
@@ -198,6 +225,73 @@ _CODONS = {
     "W": ["TGG"], "Y": ["TAT", "TAC"], "V": ["GTT", "GTC", "GTA", "GTG"],
 }
 _COMP = str.maketrans("ACGT", "TGCA")
+
+# Approximate E. coli K-12 codon usage (per thousand codons), the weights the
+# reads' reverse translation draws synonymous codons with.
+_ECOLI_USAGE = {
+    "GCG": 33.7, "GCC": 25.8, "GCA": 20.1, "GCT": 15.3, "CGT": 20.9, "CGC": 22.0, "CGA": 3.6,
+    "CGG": 5.4, "AGA": 2.1, "AGG": 1.2, "AAC": 21.7, "AAT": 17.7, "GAT": 32.1, "GAC": 19.1,
+    "TGC": 6.5, "TGT": 5.2, "CAG": 28.8, "CAA": 15.3, "GAA": 39.4, "GAG": 17.8, "GGC": 29.6,
+    "GGT": 24.7, "GGG": 11.1, "GGA": 8.0, "CAT": 12.9, "CAC": 9.7, "ATT": 30.3, "ATC": 25.1,
+    "ATA": 4.4, "CTG": 52.6, "TTA": 13.9, "TTG": 13.7, "CTT": 11.0, "CTC": 11.0, "CTA": 3.9,
+    "AAA": 33.6, "AAG": 10.3, "ATG": 27.9, "TTT": 22.3, "TTC": 16.6, "CCG": 23.2, "CCA": 8.4,
+    "CCT": 7.0, "CCC": 5.5, "AGC": 16.1, "TCT": 8.5, "TCC": 8.6, "AGT": 8.8, "TCG": 8.9,
+    "TCA": 7.2, "ACC": 23.4, "ACG": 14.4, "ACT": 8.9, "ACA": 7.1, "TGG": 15.2, "TAT": 16.2,
+    "TAC": 12.2, "GTG": 26.4, "GTT": 18.3, "GTC": 15.3, "GTA": 10.9,
+}
+# Per letter code 0..19: its codons (padded to 6) and cumulative usage.
+_CODON_BYTES = np.zeros((20, 6, 3), np.uint8)
+_CODON_CUM = np.ones((20, 6))
+for _a in range(20):
+    _cs = _CODONS[AA[_a]]
+    _w = np.array([_ECOLI_USAGE[c] for c in _cs])
+    _CODON_CUM[_a, :len(_cs)] = np.cumsum(_w) / _w.sum()
+    for _k, _c in enumerate(_cs):
+        _CODON_BYTES[_a, _k] = np.frombuffer(_c.encode(), np.uint8)
+
+
+def reverse_translate(rng, prot):
+    """DNA of a protein (codes 0..19) with E. coli-like synonymous codons."""
+    u = rng.random(len(prot))
+    k = np.minimum((_CODON_CUM[prot] < u[:, None]).sum(axis=1), 5)
+    return _CODON_BYTES[prot, k].reshape(-1).tobytes().decode()
+
+
+def sequencing_errors(rng, seq, sub=0.01, indel=0.001):
+    """1% base substitutions, 0.1% single-base insertions or deletions."""
+    b = np.frombuffer(seq.encode(), np.uint8).copy()
+    m = rng.random(len(b)) < sub
+    b[m] = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, size=int(m.sum()))]
+    ev = np.nonzero(rng.random(len(b)) < indel)[0]
+    if len(ev) == 0:
+        return b.tobytes().decode()
+    out, last = [], 0
+    for p in ev:
+        out.append(b[last:p].tobytes())
+        if rng.random() < 0.5:
+            out.append(b"ACGT"[int(rng.integers(0, 4))].to_bytes(1, "little"))
+            last = p
+        else:
+            last = p + 1
+    out.append(b[last:].tobytes())
+    return b"".join(out).decode()
+
+
+def _planted_reads(rng, db_seqs, n_reads, lo_nt, hi_nt):
+    """Reads of lo_nt..hi_nt bases (uniform): a segment of a random DB
+    protein reverse-translated with E. coli codon usage, with sequencing
+    errors, reverse-complemented half of the time."""
+    reads = []
+    for _ in range(n_reads):
+        n_nt = int(rng.integers(lo_nt, hi_nt + 1))
+        prot = db_seqs[int(rng.integers(0, len(db_seqs)))]
+        n_aa = min(len(prot), max(1, n_nt // 3))
+        a = int(rng.integers(0, len(prot) - n_aa + 1))
+        seq = sequencing_errors(rng, reverse_translate(rng, np.asarray(prot[a:a + n_aa], np.int64)))
+        if rng.random() < 0.5:
+            seq = seq.translate(_COMP)[::-1]
+        reads.append(seq)
+    return reads
 
 
 class ReadWorkload:
@@ -244,7 +338,9 @@ def _reads(rng, db_seqs, n_reads, lo_nt, hi_nt, planted_frac):
 def make_reads(name: str, seed: int = 20240601) -> ReadWorkload:
     """Named blastx workloads:
     'blastx-tiny'  : 40 reads of 60..900 bases vs the 'tiny' 400-protein DB
-    'blastx-2048'  : 2048 reads of 150..3000 bases vs the 2.26 Maa DB
+    'blastx-2048'  : 2048 reads of 150..3000 bases (uniform) vs the 60 M letter
+                     DB: reverse-translated DB proteins (E. coli codon usage),
+                     1% substitutions, 0.1% indels, either strand
     """
     rng = np.random.default_rng(seed + 7)
     if name == "blastx-tiny":
@@ -252,9 +348,9 @@ def make_reads(name: str, seed: int = 20240601) -> ReadWorkload:
         db_seqs = [db.db_letters[db.db_offsets[i]:db.db_offsets[i + 1]] for i in range(db.n_db)]
         reads = _reads(rng, db_seqs, 40, 60, 900, 0.7)
     elif name == "blastx-2048":
-        db = make("blastp-sensitive-1024", seed)
+        db = make("blastp-very-sensitive-4096", seed)
         db_seqs = [db.db_letters[db.db_offsets[i]:db.db_offsets[i + 1]] for i in range(db.n_db)]
-        reads = _reads(rng, db_seqs, 2048, 150, 3000, 0.5)
+        reads = _planted_reads(rng, db_seqs, 2048, 150, 3000)
     else:
         raise ValueError(f"unknown read workload {name}")
     return ReadWorkload(name, db, reads, [f"r{i:05d}" for i in range(len(reads))],
@@ -265,6 +361,8 @@ def make(name: str, seed: int = 20240601) -> Workload:
     """Named workloads:
     'blastp-default-1x497'   : 1 query of 497 aa vs the 2.26 Maa DB
     'blastp-sensitive-1024'  : 1024 queries (median 300, 50..1000) vs the same DB
+    'blastp-very-sensitive-4096': 4096 queries vs 200,000 proteins (~60 M letters),
+                               remote homologs (20-35% identity, indels 0.04)
     'tiny'                   : 24 queries vs 400 proteins (fast parity cases)
     """
     if name in ("blastp-default-1x497", "blastp-sensitive-1024"):
@@ -274,6 +372,10 @@ def make(name: str, seed: int = 20240601) -> Workload:
                             dict(sensitivity="default", seed=seed, queries=1))
         return Workload(name, db, ids, qs, [f"q{i:04d}" for i in range(len(qs))],
                         dict(sensitivity="sensitive", seed=seed, queries=len(qs)))
+    if name == "blastp-very-sensitive-4096":
+        db, ids, qs = protein_db_60m(seed)
+        return Workload(name, db, ids, qs, [f"v{i:04d}" for i in range(len(qs))],
+                        dict(sensitivity="very-sensitive", seed=seed, queries=len(qs)))
     if name.startswith("tiny"):
         rng = np.random.default_rng(seed)
         n_q = 24

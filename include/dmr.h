@@ -29,6 +29,7 @@ extern "C" {
 #define DMR_E_OOM (-3)         /* device allocation failed */
 #define DMR_E_CAPACITY (-4)    /* output buffer too small; *n_out holds the size needed */
 #define DMR_E_UNSUPPORTED (-5) /* input outside what the library handles */
+#define DMR_E_TIMEOUT (-6)     /* dmr_server_wait: the request is not done yet */
 
 typedef struct dmr_ctx dmr_ctx;
 typedef struct dmr_result dmr_result;
@@ -103,6 +104,22 @@ const char* dmr_version(void);
 int dmr_ctx_create(int device, dmr_ctx** out);
 void dmr_ctx_destroy(dmr_ctx* ctx);
 int dmr_device_count(int* n);
+
+/* Engine settings of a context; no counterpart in the reference (none of
+ * them changes a result). Keys:
+ *   "slice_hits"  seed hits per slice of query targets: a search runs its
+ *                 targets in slices whose hits stay under this (device
+ *                 memory grows with it, ~52 bytes per hit and ~150 per DP);
+ *   "profile"     1 = keep per-class DP jobs and cells over every slice of a
+ *                 search (for dmr_last_class_profile / dmr_last_dp_cells);
+ *   "split_mask"  packed score classes 0-4 that run split over several lanes
+ *                 per DP pair (bit c = class c);
+ *   "score_plan"  how the score-class kernels share the streams (0, 1, 2);
+ *   "f16"         1 = packed score pass in f16 pairs, 0 = saturating int16.
+ * dmr_ctx_get_tuning reads them back, plus "last_slices" (slices of the
+ * last search). */
+int dmr_ctx_set_tuning(dmr_ctx* ctx, const char* key, double value);
+int dmr_ctx_get_tuning(const dmr_ctx* ctx, const char* key, double* value);
 
 /* gapped::band_for (src/gapped.rs:27-32) */
 uint32_t dmr_band_for(uint64_t len);
@@ -207,9 +224,20 @@ int dmr_blastp(dmr_ctx* ctx, const uint8_t* queries, const uint64_t* q_offsets, 
                const dmr_search_config* cfg, dmr_result** out);
 
 /* blastx: queries are nucleotide ASCII; six frames with --min-orf masking
- * (src/translate.rs:54-137, src/workflow.rs:112-127). */
+ * (src/translate.rs:54-137, src/workflow.rs:112-127), translated on the
+ * device; frames without a letter below X are not searched. */
 int dmr_blastx(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint32_t nreads,
                const dmr_search_config* cfg, dmr_result** out);
+
+/* translate::six_frames + mask_short_orfs (src/translate.rs:54-121) of every
+ * read, on the device: frame k of read r (0-2 forward offsets 0-2, 3-5 the
+ * reverse complement's) at letters[frame_offsets[6r + k] ..
+ * frame_offsets[6r + k + 1]], as letter codes (X for ambiguous codons, * for
+ * stops). min_orf < 0 = translate::min_orf_len's default per read length.
+ * frame_offsets holds 6 * nreads + 1 entries (may be NULL); *n_letters
+ * receives the letter count; the letters are written when cap >= it. */
+int dmr_six_frames(dmr_ctx* ctx, const char* reads, const uint64_t* r_offsets, uint32_t nreads, int32_t min_orf,
+                   uint8_t* letters, uint64_t cap, uint64_t* frame_offsets, uint64_t* n_letters);
 
 /* Uploads a blastp query batch and keeps it resident on the device, so
  * dmr_search_loaded can run without any host-to-device copy of the inputs
@@ -248,8 +276,9 @@ int dmr_result_format(const dmr_result* r, const char* query_ids, const char* su
 /* Stage counters and GPU times: query_seeds, runs, hits, hit_runs,
  * segments, groups, jobs, maybe, selected, traceback_cells, ms_seed,
  * ms_join, ms_sort_hits, ms_ungapped, ms_bands, ms_sw, ms_select,
- * ms_traceback, ms_total, ms_host (20 doubles). */
-int dmr_result_stats(const dmr_result* r, double* out20);
+ * ms_traceback, ms_total, ms_host, ms_translate (blastx: six frames, masking
+ * and the target set on the device) (21 doubles). */
+int dmr_result_stats(const dmr_result* r, double* out21);
 /* Cells of every banded DP of the last search on ctx (for GCUPS). */
 int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
 /* Score-pass profile of the last search on ctx, per kernel class
@@ -269,6 +298,40 @@ int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double
  * counterpart in the reference. */
 int dmr_score_layout(dmr_ctx* ctx, uint32_t* split_mask, uint32_t* f16);
 void dmr_result_destroy(dmr_result* r);
+
+/* ---- Serving loop ----------------------------------------------------------
+ * No counterpart in the reference (its CLI searches one batch per process):
+ * a request loop over the context's resident database and seed indexes.
+ * Requests (one encoded protein query each, with its own config: preset,
+ * evalue, limits) are submitted at any time; a dispatcher thread forms
+ * batches of queued requests with equal configs, oldest first, as workers
+ * come free and the device-memory budget admits them, and each worker
+ * searches its batch on its own streams, so batches overlap on the GPU.
+ * The max_expand cap (src/search.rs:109-131, src/seed.rs:123-125) counts a
+ * key's query entries per request, so every result equals that of
+ * dmr_blastp on the request alone. The context must stay alive and keep its
+ * database while the server runs. */
+typedef struct dmr_server dmr_server;
+/* workers: 1..8 batches in flight; hbm_budget: device bytes the admitted
+ * requests' estimates may total (at least 64 MiB); presets: comma-separated
+ * sensitivities whose seed indexes are built now (may be NULL). */
+int dmr_server_create(dmr_ctx* ctx, uint32_t workers, uint64_t hbm_budget, const char* presets,
+                      dmr_server** out);
+/* Queues one query; *ticket identifies it for dmr_server_wait. */
+int dmr_server_submit(dmr_server* s, const uint8_t* query, uint64_t len, const dmr_search_config* cfg,
+                      uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: without limit) for a request; on success
+ * *out receives its result (one query record, index 0; free it with
+ * dmr_result_destroy) and *latency_ms (may be NULL) the time from submit to
+ * its result. DMR_E_TIMEOUT leaves the request pending; any other return
+ * collects the ticket. */
+int dmr_server_wait(dmr_server* s, uint64_t ticket, int64_t timeout_ms, dmr_result** out, double* latency_ms);
+/* requests done, batches, mean and largest batch, peak admitted bytes,
+ * longest queue, mean queue wait (ms), budget (8 doubles). */
+int dmr_server_stats(dmr_server* s, double* out8);
+/* Finishes the queued and running requests, then stops; results not
+ * collected are freed. */
+void dmr_server_destroy(dmr_server* s);
 
 /* ---- FASTA ingress (host) -------------------------------------------------
  * io::parse_fasta (src/io.rs:40-102) of a FASTA text held in memory: one

@@ -1250,6 +1250,8 @@ int orc_fmt_evalue(double x, char* buf, int cap) {
   return std::snprintf(buf, cap, "%s", fmt_evalue(x).c_str());
 }
 
+// Letters written up to cap; returns the count of all six frames' letters
+// (more than cap means the output was cut).
 int orc_six_frames(const char* nt, uint8_t* out, int64_t cap, int64_t* lens6, long min_orf) {
   std::string s(nt);
   auto f = six_frames(s);
@@ -1258,8 +1260,10 @@ int orc_six_frames(const char* nt, uint8_t* out, int64_t cap, int64_t* lens6, lo
   for (int k = 0; k < 6; ++k) {
     mask_short_orfs(f[k], mo);
     lens6[k] = (int64_t)f[k].size();
-    for (auto c : f[k])
-      if (o < cap) out[o++] = c;
+    for (auto c : f[k]) {
+      if (o < cap) out[o] = c;
+      ++o;
+    }
   }
   return (int)o;
 }

@@ -203,10 +203,12 @@ def fmt_evalue(x):
 
 
 def six_frames(nt, min_orf=-1):
-    cap = len(nt) + 8
+    cap = 2 * len(nt) + 8  # six frames of about len / 3 letters each
     out = np.zeros(cap, np.uint8)
     lens = np.zeros(6, np.int64)
-    lib().orc_six_frames(nt.encode(), out.ctypes.data, cap, lens.ctypes.data, min_orf)
+    n = lib().orc_six_frames(nt.encode(), out.ctypes.data, cap, lens.ctypes.data, min_orf)
+    if n > cap:
+        raise RuntimeError("six_frames output larger than its buffer")
     frames, o = [], 0
     for L in lens:
         frames.append(out[o:o + L].copy())

@@ -198,3 +198,27 @@ def test_int16_saturation_reruns_exact(ctx, orc):
     ref = [orc.extend_span(*c)["score"] for c in cases]
     assert ref[0] > 32767
     assert got == ref
+
+
+def test_six_frames_match_oracle(ctx, orc):
+    """dmr_six_frames (device translation + --min-orf masking) against the
+    oracle's translate::six_frames + mask_short_orfs (src/translate.rs:54-137)
+    on reads with both cases, U, IUPAC and other bytes, lengths 0..3000, and
+    default, zero, one, small and huge min_orf values."""
+    rng = np.random.default_rng(11)
+    alphabet = list("ACGTACGTACGTACGTacgtuUNRYKMSWBDHVnx-*")
+    reads = []
+    for L in list(range(0, 12)) + [29, 30, 31, 99, 100, 101] + list(rng.integers(100, 3000, size=40)):
+        p = rng.random()
+        src = alphabet if p < 0.3 else list("ACGT")
+        reads.append("".join(rng.choice(src, size=int(L))))
+    # Reads of stops and short ORFs only, and one long ORF.
+    reads.append("TAA" * 60)
+    reads.append(("ATG" * 7 + "TAG") * 20)
+    reads.append("ATG" * 300)
+    for min_orf in [-1, 0, 1, 5, 40, 1000]:
+        got = ctx.six_frames(reads, min_orf)
+        for r, nt in enumerate(reads):
+            exp = orc.six_frames(nt, min_orf=min_orf)
+            for k in range(6):
+                assert list(got[6 * r + k]) == list(exp[k]), (min_orf, r, k)

@@ -131,3 +131,53 @@ def test_full_sensitive_batch_matches_oracle(dmr, ctx, synth):
     got = res.format(w.q_ids, w.db_ids, fields)
     assert got.count("\n") == exp.count("\n") == 4735
     assert got == exp
+
+
+@pytest.mark.parametrize("slice_hits", [1, 300])
+def test_sliced_search_matches_oracle(dmr, orc, synth, slice_hits):
+    """A search run in slices of targets (each slice's hits under slice_hits;
+    1 = one target per slice) prints what the oracle prints for the whole
+    batch: the cap decisions are the batch's, slices only bound memory."""
+    w = workload_dict(synth.make("tiny"))
+    c = dmr.Context(0)
+    c.set_tuning("slice_hits", slice_hits)
+    c.db_load(w["db_letters"], w["db_offsets"])
+    for sens in ["sensitive", "default"]:
+        res = c.blastp(w["q_letters"], w["q_offsets"], dmr.SearchConfig(sensitivity=sens))
+        assert c.get_tuning("last_slices") >= 2
+        ref = orc.search(w["db_letters"], w["db_offsets"], w["q_letters"], w["q_offsets"], sensitivity=sens,
+                         threads=8)
+        assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+        assert res.stats()["hits"] == ref.counters()["hits"]
+    c.close()
+
+
+def test_sliced_full_batch_and_profile(dmr, synth):
+    """The 1024-query batch in about ten slices equals the stored oracle text;
+    with profiling on, the DP cells and class jobs summed over the slices
+    equal those of the same search in one slice."""
+    import gzip
+    import pathlib
+    g = pathlib.Path(__file__).resolve().parent / "golden" / "search" / "blastp-sensitive-1024.tsv.gz"
+    exp = gzip.open(g, "rb").read().decode()
+    fields = ("qseqid sseqid pident length mismatch gapopen qstart qend sstart send evalue bitscore "
+              "score cigar")
+    w = synth.make("blastp-sensitive-1024")
+    c = dmr.Context(0)
+    c.db_load(w.db_letters, w.db_offsets)
+    c.queries_load(w.q_letters, w.q_offsets)
+    cfg = dmr.SearchConfig(sensitivity="sensitive")
+    c.search_loaded(cfg)
+    assert c.get_tuning("last_slices") == 1
+    cells1 = c.last_dp_cells()
+    prof1 = c.class_profile()
+    c.set_tuning("slice_hits", 300000)
+    c.set_tuning("profile", 1)
+    res = c.search_loaded(cfg)
+    assert c.get_tuning("last_slices") >= 8
+    assert res.format(w.q_ids, w.db_ids, fields) == exp
+    assert c.last_dp_cells() == cells1
+    prof = c.class_profile()
+    assert {k: (v["jobs"], v["cells"]) for k, v in prof.items()} == {k: (v["jobs"], v["cells"])
+                                                                     for k, v in prof1.items()}
+    c.close()
