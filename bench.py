@@ -93,6 +93,212 @@ def fasta_ingress(dmr, w):
     return res
 
 
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def stage_rooflines(stats, nshapes, q_letters):
+    """Seeds -> bands stages against the 8 TB/s HBM peak, from algorithmic
+    bytes: seed enumeration reads each query letter once per shape and writes
+    12 B per seed (8 B tagged key, 4 B letter index), the sort reads and
+    writes each seed once (24 B); the join reads 12 B per query seed and
+    writes 8 B per hit plus 4 B of reference letter index read per hit; the
+    hit sort reads and writes 8 B per hit (16 B); the ungapped stage reads
+    8 B per hit and writes 16 B per segment (residue reads excluded: a lower
+    bound)."""
+    E, H, S = stats["query_seeds"], stats["hits"], stats["segments"]
+    rows = {
+        "seed_and_sort": (q_letters * nshapes + E * 12 + E * 24, stats["ms_seed"]),
+        "join": (E * 12 + H * 12, stats["ms_join"]),
+        "sort_hits": (H * 16, stats["ms_sort_hits"]),
+        "ungapped": (H * 8 + S * 16, stats["ms_ungapped"]),
+    }
+    out = {}
+    for k, (b, ms) in rows.items():
+        gbs = b / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        out[k] = {"bytes": int(b), "ms": round(ms, 4), "achieved_gbs": round(gbs, 1),
+                  "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def fixture(name):
+    import gzip
+    g = ROOT / "tests" / "golden" / "search" / f"{name}.tsv.gz"
+    return gzip.open(g, "rb").read().decode() if g.exists() else None
+
+
+FIX_FIELDS = ("qseqid sseqid pident length mismatch gapopen qstart qend sstart send evalue bitscore "
+              "score cigar")
+
+
+def leg_very_sensitive(dmr, synth, steps):
+    """--very-sensitive, 4096 queries vs 200,000 proteins (60 M letters),
+    one batch (sliced by seed hits)."""
+    t = time.time()
+    w = synth.make("blastp-very-sensitive-4096")
+    gen_s = time.time() - t
+    ctx = dmr.Context(0)
+    ctx.db_load(w.db_letters, w.db_offsets)
+    index_ms = ctx.build_index("very-sensitive")
+    ctx.queries_load(w.q_letters, w.q_offsets)
+    cfg = dmr.SearchConfig(sensitivity="very-sensitive")
+    qids, sids = dmr.ids_text(w.q_ids), dmr.ids_text(w.db_ids)
+    r = ctx.search_loaded(cfg)  # warmup
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        r = ctx.search_loaded(cfg)
+        text = r.format(qids, sids)
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] very-sensitive step: {times[-1] * 1e3:.0f} ms")
+    st = r.stats()
+    slices = int(ctx.get_tuning("last_slices"))
+    # DP cells of every slice (profiling adds work, so its own run).
+    ctx.set_tuning("profile", 1)
+    rp = ctx.search_loaded(cfg)
+    cells = ctx.last_dp_cells()
+    sw_ms = rp.stats()["ms_sw"]
+    ctx.set_tuning("profile", 0)
+    # Parity: 64 of the queries as their own batch against the oracle's text.
+    exp = fixture("blastp-very-sensitive-4096-q64")
+    sub = w.subset_queries(list(range(0, 4096, 64)))
+    got = ctx.blastp(sub.q_letters, sub.q_offsets, cfg).format(sub.q_ids, w.db_ids, FIX_FIELDS)
+    ctx.close()
+    ms = float(np.mean(times)) * 1e3
+    return {"workload": f"{w.name}: blastp --very-sensitive, {w.n_q} queries vs {w.n_db} proteins / "
+                        f"{int(w.db_offsets[-1])} letters, one batch",
+            "value": round(w.n_q / (ms / 1e3), 2), "unit": "queries/s", "ms_per_step": round(ms, 1),
+            "steps": steps, "index_build_ms": round(index_ms, 1), "generate_s": round(gen_s, 1),
+            "slices": slices, "hsps": text.count("\n"), "dp_cells": int(cells),
+            "gcups": round(cells / (sw_ms / 1e3) / 1e9, 1) if sw_ms > 0 else None,
+            "stage_ms": {k[3:]: round(v, 2) for k, v in st.items() if k.startswith("ms_")},
+            "counts": {k: int(v) for k, v in st.items() if not k.startswith("ms_")},
+            "stage_rooflines": stage_rooflines(st, 16, int(w.q_offsets[-1])),
+            "bit_exact_subset": {"outfmt6_vs_oracle": got == exp, "lines": exp.count("\n") if exp else 0,
+                                 "sample": "queries 0, 64, ... (64) as one batch"}}
+
+
+def leg_blastx(dmr, synth, steps):
+    """blastx: 2048 reads of 150-3000 bases vs the 60 M letter DB, --default;
+    frames translated and masked on the device."""
+    t = time.time()
+    w = synth.make_reads("blastx-2048")
+    gen_s = time.time() - t
+    ctx = dmr.Context(0)
+    ctx.db_load(w.db_letters, w.db_offsets)
+    index_ms = ctx.build_index("default")
+    cfg = dmr.SearchConfig(sensitivity="default")
+    rids, sids = dmr.ids_text(w.read_ids), dmr.ids_text(w.db_ids)
+    r = ctx.blastx(w.reads, cfg)  # warmup
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        r = ctx.blastx(w.reads, cfg)
+        text = r.format(rids, sids)
+        times.append(time.perf_counter() - t0)
+    st = r.stats()
+    import importlib
+    sys.path.insert(0, str(ROOT / "tests" / "golden"))
+    mg = importlib.import_module("make_search_golden")
+    exp = fixture("blastx-2048-r64")
+    _, idx, _ = mg.READ_SUBSETS["blastx-2048-r64"]
+    got = ctx.blastx([w.reads[i] for i in idx], cfg).format([w.read_ids[i] for i in idx], w.db_ids,
+                                                            mg.READ_FIELDS)
+    ctx.close()
+    ms = float(np.mean(times)) * 1e3
+    nt = sum(len(x) for x in w.reads)
+    return {"workload": f"{w.name}: blastx --default, {len(w.reads)} reads ({nt} bases) vs {w.n_db} proteins / "
+                        f"{int(w.db_offsets[-1])} letters",
+            "value": round(len(w.reads) / (ms / 1e3), 2), "unit": "reads/s", "ms_per_step": round(ms, 2),
+            "steps": steps, "index_build_ms": round(index_ms, 1), "generate_s": round(gen_s, 1),
+            "translate_ms": round(st["ms_translate"], 3), "hsps": text.count("\n"),
+            "note": "the step includes the reads' upload (host buffers in) and the device translation",
+            "stage_ms": {k[3:]: round(v, 2) for k, v in st.items() if k.startswith("ms_")},
+            "counts": {k: int(v) for k, v in st.items() if not k.startswith("ms_")},
+            "bit_exact_subset": {"outfmt6_vs_oracle": got == exp, "lines": exp.count("\n") if exp else 0,
+                                 "sample": "reads 0, 32, ... (64) as one batch"}}
+
+
+def leg_serving(dmr, synth, duration_s=10.0):
+    """Serving loop: 1,000,000 proteins (~300 M letters) resident with the
+    default, sensitive and very-sensitive indexes; requests of the pooled
+    queries (lengths log-normal median 350, 30..5000), presets 50/40/10.
+    First a burst (all submitted at once) gives the saturated throughput;
+    then a Poisson stream at half of it gives per-request latency."""
+    import threading
+    t = time.time()
+    w = synth.make_serving("serve-1m")
+    gen_s = time.time() - t
+    ctx = dmr.Context(0)
+    t0 = time.perf_counter()
+    ctx.db_load(w.db_letters, w.db_offsets)
+    load_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    srv = dmr.Server(ctx, workers=2, hbm_budget=32 << 30, presets="default,sensitive,very-sensitive")
+    index_s = time.perf_counter() - t0
+    log(f"[bench] serving: DB loaded in {load_s:.1f} s, three indexes in {index_s:.1f} s")
+    cfgs = {p: dmr.SearchConfig(sensitivity=p) for p in ("default", "sensitive", "very-sensitive")}
+
+    def run(stream):
+        lat = [None] * len(stream)
+        tickets = [None] * len(stream)
+        ready = threading.Semaphore(0)
+
+        def collect():
+            for k in range(len(stream)):
+                ready.acquire()
+                _, lat[k] = srv.wait(tickets[k])
+
+        th = threading.Thread(target=collect)
+        th.start()
+        start = time.perf_counter()
+        for k, (at, qi, preset) in enumerate(stream):
+            d = start + at - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[k] = srv.submit(w.pool[qi], cfgs[preset])
+            ready.release()
+        th.join()
+        return lat, time.perf_counter() - start
+
+    # Warm every preset's kernels and buffers.
+    run(w.stream(64, 1e9, seed=1))
+    burst = w.stream(1024, 1e9, seed=2)
+    lat_b, wall_b = run(burst)
+    capacity = len(burst) / wall_b
+    qps = max(10.0, round(capacity * 0.5))
+    n = int(min(4000, max(200, qps * duration_s)))
+    stream = w.stream(n, qps, seed=3)
+    lat, wall = run(stream)
+    st = srv.stats()
+    # A sample of served requests equals the same queries searched alone.
+    sample = w.stream(24, 1e9, seed=4)
+    served = []
+    for _, qi, preset in sample:
+        served.append((qi, preset, srv.submit(w.pool[qi], cfgs[preset])))
+    same = True
+    sids = dmr.ids_text(w.db_ids)
+    for qi, preset, tk in served:
+        res, _ = srv.wait(tk)
+        q = w.pool[qi]
+        alone = ctx.blastp(q, np.array([0, len(q)], np.uint64), cfgs[preset])
+        same &= res.format(["q"], sids) == alone.format(["q"], sids)
+    srv.close()
+    ctx.close()
+    lat = np.array(lat)
+    return {"workload": f"{w.name}: {w.n_db} proteins / {int(w.db_offsets[-1])} letters resident "
+                        "(default + sensitive + very-sensitive indexes), pooled queries of log-normal "
+                        "length (median 350, 30..5000), presets 50/40/10, Poisson arrivals",
+            "qps_offered": qps, "requests": n, "value": round(n / wall, 2), "unit": "queries/s",
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2),
+                           "mean": round(float(lat.mean()), 2), "of": "arrival to result, per request"},
+            "saturated": {"queries_per_s": round(capacity, 1), "requests": len(burst),
+                          "latency_p50_ms": round(float(np.percentile(lat_b, 50)), 1)},
+            "server": st, "db_load_s": round(load_s, 1), "index_build_s": round(index_s, 1),
+            "generate_s": round(gen_s, 1),
+            "served_equals_alone_on_sample": bool(same), "sample": len(sample)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +309,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=4, help="queries timed on the plain form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--legs", default="very-sensitive,blastx,serving",
+                    help="other configurations to run after the headline one ('' for none)")
+    ap.add_argument("--leg-steps", type=int, default=2)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported "
@@ -213,6 +422,7 @@ def main():
                                        "fixture": str(gold.relative_to(ROOT))}
         log(f"[bench] timed batch == oracle fixture: {text == exp}")
 
+    out["stage_rooflines"] = stage_rooflines(stats, 16 if sens == "sensitive" else 2, int(w.q_offsets[-1]))
     out["fasta_ingress"] = fasta_ingress(dmr, w)
 
     if not a.no_plain and a.plain_sample > 0:
@@ -248,6 +458,21 @@ def main():
         out["bit_exact"] = {"outfmt6_gpu_vs_cpu_on_sample": got == exp, "lines": exp.count("\n")}
         log(f"[bench] CPU oracle: {sub.n_q / tc:.2f} queries/s on {threads} threads; "
             f"GPU == CPU outfmt6: {got == exp}")
+    ctx.close()
+    legs = {}
+    runners = {"very-sensitive": lambda: leg_very_sensitive(dmr, synth, a.leg_steps),
+               "blastx": lambda: leg_blastx(dmr, synth, a.leg_steps),
+               "serving": lambda: leg_serving(dmr, synth)}
+    for name in [x for x in a.legs.split(",") if x]:
+        t0 = time.time()
+        try:
+            legs[name] = runners[name]()
+        except Exception as e:  # a failing leg is reported, the headline line still prints
+            legs[name] = {"error": f"{type(e).__name__}: {e}"}
+        legs[name]["wall_s"] = round(time.time() - t0, 1)
+        log(f"[bench] leg {name}: {json.dumps(legs[name])[:400]}")
+    if legs:
+        out["legs"] = legs
     print(json.dumps(out), flush=True)
 
 

@@ -158,6 +158,10 @@ void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, 
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);
 void launch_u32_times(const uint32_t* in, uint64_t n, uint32_t mul, uint64_t* out, hipStream_t st);
+// sums[j] += v[ranges[2j] .. ranges[2j + 1]) for j < k; longest = the
+// largest range length (sums are not cleared).
+void launch_range_sums(const uint64_t* v, const uint64_t* ranges, uint32_t k, uint64_t longest, uint64_t* sums,
+                       hipStream_t st);
 void launch_gather_u64(const uint64_t* in, const uint32_t* idx, uint64_t n, uint64_t* out, hipStream_t st);
 
 // Persistent kernels (k_sw16, k_sw_wave) launch as many blocks as can be

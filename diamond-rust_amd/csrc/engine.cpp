@@ -1398,15 +1398,26 @@ void Engine::accumulate_profile(uint64_t njobs) {
   SeqSet dbs{db_->res.as<uint8_t>(), db_->offs.as<uint32_t>()};
   uint64_t* c = cells_.ensure<uint64_t>(njobs + 1);
   launch_job_geometry(jobs_.as<SwJob>(), sw_idx_.as<uint32_t>(), njobs, dbs, c, nullptr, nullptr, st_);
-  std::vector<uint64_t> hc(njobs);
-  DMR_HIP(hipMemcpyAsync(hc.data(), c, njobs * 8, hipMemcpyDeviceToHost, st_));
+  // Per-class sums on the device: only kClasses words come back.
+  uint64_t* rng = prof_ranges_.ensure<uint64_t>(3 * kClasses);
+  uint64_t* sums = rng + 2 * kClasses;
+  uint64_t* h = prof_pin_.ensure<uint64_t>(3 * kClasses);
+  uint64_t longest = 0;
+  for (int k = 0; k < kClasses; ++k) {
+    h[2 * k] = last_cbeg_[k];
+    h[2 * k + 1] = std::max(last_cbeg_[k], last_cend_[k]);
+    h[2 * kClasses + k] = 0;
+    longest = std::max(longest, h[2 * k + 1] - h[2 * k]);
+  }
+  DMR_HIP(hipMemcpyAsync(rng, h, 3 * kClasses * 8, hipMemcpyHostToDevice, st_));
+  launch_range_sums(c, rng, kClasses, longest, sums, st_);
+  DMR_HIP(hipMemcpyAsync(h + 2 * kClasses, sums, kClasses * 8, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
-  for (int k = 0; k < kClasses; ++k)
-    for (uint64_t p = last_cbeg_[k]; p < last_cend_[k]; ++p) {
-      prof_jobs_[k]++;
-      prof_cells_[k] += hc[p];
-      prof_dp_cells_ += hc[p];
-    }
+  for (int k = 0; k < kClasses; ++k) {
+    prof_jobs_[k] += h[2 * k + 1] - h[2 * k];
+    prof_cells_[k] += h[2 * kClasses + k];
+    prof_dp_cells_ += h[2 * kClasses + k];
+  }
 }
 
 void Engine::last_class_profile(uint64_t jobs[kClasses], uint64_t cells[kClasses],

@@ -363,7 +363,8 @@ class Engine {
   bool profile_ = false, prof_valid_ = false;
   uint64_t prof_jobs_[kPackedClasses + kWaveKinds + 1] = {}, prof_cells_[kPackedClasses + kWaveKinds + 1] = {};
   uint64_t prof_dp_cells_ = 0;
-  DevBuf thits_, jc_scnt_, jc_sst_, jc_snq_, q_group_, qruns_;
+  DevBuf thits_, jc_scnt_, jc_sst_, jc_snq_, q_group_, qruns_, prof_ranges_;
+  PinBuf prof_pin_;
   PinBuf thits_pin_;
   void stage_event(int k);
   float stage_ms(int a, int b);

@@ -267,9 +267,36 @@ __global__ void k_u32_times(const uint32_t* __restrict__ in, uint64_t n, uint32_
   if (i < n) out[i] = (uint64_t)in[i] * mul;
 }
 
+// sums[k] += v[b_k .. e_k) for the ranges of ranges[2k], ranges[2k + 1]:
+// grid (blocks over the longest range, k); each block adds its part.
+__global__ void k_range_sums(const uint64_t* __restrict__ v, const uint64_t* __restrict__ ranges,
+                             unsigned long long* __restrict__ sums) {
+  __shared__ unsigned long long part[4];
+  const uint32_t k = blockIdx.y;
+  const uint64_t b = ranges[2 * k], e = ranges[2 * k + 1];
+  unsigned long long x = 0;
+  for (uint64_t i = b + (uint64_t)blockIdx.x * 1024 + threadIdx.x; i < e && i < b + (uint64_t)(blockIdx.x + 1) * 1024;
+       i += 256)
+    x += v[i];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(&sums[k], t);
+  }
+}
+
 inline unsigned nblk(uint64_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
+
+void launch_range_sums(const uint64_t* v, const uint64_t* ranges, uint32_t k, uint64_t longest, uint64_t* sums,
+                       hipStream_t st) {
+  if (!k || !longest) return;
+  hipLaunchKernelGGL(k_range_sums, dim3(nblk(longest, 1024), k), dim3(256), 0, st, v, ranges,
+                     reinterpret_cast<unsigned long long*>(sums));
+}
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
   if (!n) return;

@@ -213,6 +213,73 @@ def protein_db_60m(seed=20240601, n_queries=4096, n_db=200_000):
     return db, ids, qs
 
 
+# 16-bit lookup of the background distribution (for the largest DB, where
+# drawing doubles per letter costs seconds).
+_LUT16 = np.minimum(np.searchsorted(_BG_CUM, (np.arange(65536) + 0.5) / 65536.0, side="right"),
+                    19).astype(np.uint8)
+
+
+def protein_db_serving(seed=20240601, n_db=1_000_000, n_pool=4096):
+    """The serving workload: 1,000,000 proteins (~300 M letters; lengths
+    log-normal, median 255, sigma 0.57, clipped 30..5000, letters from the
+    background through a 16-bit table) and a pool of 4096 queries with
+    heavy-tailed lengths (log-normal median 350, sigma 0.8, clipped
+    30..5000), each with 0-5 homologs planted at 25-60% identity (indels
+    0.02). Returns (db letters, db offsets, db ids, query list)."""
+    rng = np.random.default_rng(seed + 1000)
+    qlens = _lengths(rng, n_pool, 350, 0.8, 30, 5000)
+    qs = [_residues(rng, int(L)) for L in qlens]
+    planted = []
+    for qi in range(n_pool):
+        for _ in range(int(rng.integers(0, 6))):
+            planted.append(mutate(rng, qs[qi], rng.uniform(0.25, 0.60), 0.02))
+    n_bg = n_db - len(planted)
+    lens = np.concatenate([_lengths(rng, n_bg, 255, 0.57, 30, 5000),
+                           np.array([len(p) for p in planted], np.int64)])
+    perm = rng.permutation(n_db)  # DB slot -> source (background first, then planted)
+    slot_lens = lens[perm]
+    off = np.zeros(n_db + 1, np.uint64)
+    np.cumsum(slot_lens, out=off[1:])
+    letters = _LUT16[rng.integers(0, 65536, size=int(off[-1]), dtype=np.uint16)]
+    for k in np.nonzero(perm >= n_bg)[0]:
+        letters[int(off[k]):int(off[k + 1])] = planted[int(perm[k]) - n_bg]
+    ids = [f"u{k:07d}" for k in range(n_db)]
+    return letters, off, ids, qs
+
+
+class ServingWorkload:
+    """A resident DB and a query pool for the serving loop."""
+
+    def __init__(self, name, db_letters, db_offsets, db_ids, pool, meta):
+        self.name = name
+        self.db_letters, self.db_offsets, self.db_ids = db_letters, db_offsets, db_ids
+        self.pool = pool
+        self.meta = meta
+
+    @property
+    def n_db(self):
+        return len(self.db_offsets) - 1
+
+    def stream(self, n, qps, seed=7):
+        """n requests of a Poisson arrival stream at qps: (arrival s, pool
+        index, preset), presets 50% default, 40% sensitive, 10%
+        very-sensitive."""
+        rng = np.random.default_rng(seed)
+        t = np.cumsum(rng.exponential(1.0 / qps, size=n))
+        q = rng.integers(0, len(self.pool), size=n)
+        u = rng.random(n)
+        presets = np.where(u < 0.5, "default", np.where(u < 0.9, "sensitive", "very-sensitive"))
+        return list(zip(t.tolist(), q.tolist(), presets.tolist()))
+
+
+def make_serving(name="serve-1m", seed=20240601) -> ServingWorkload:
+    """'serve-1m': 1,000,000 proteins (~300 M letters), 4096 pooled queries;
+    'serve-tiny': the same recipe at 20,000 proteins (tests)."""
+    n_db = {"serve-1m": 1_000_000, "serve-tiny": 20_000}[name]
+    letters, off, ids, pool = protein_db_serving(seed, n_db=n_db, n_pool=4096 if n_db > 100_000 else 256)
+    return ServingWorkload(name, letters, off, ids, pool, dict(seed=seed, proteins=n_db))
+
+
 # Standard genetic code, amino acid -> synonymous codons (reverse translation
 # of planted proteins into reads).
 _CODONS = {
