@@ -19,7 +19,7 @@ LIB = HERE / "libdmr.so"
 ARCH = os.environ.get("DMR_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["k_sort.hip", "k_seed.hip", "k_extend.hip", "k_sw.hip", "k_sw16.hip", "k_swwave.hip", "plain.cpp",
-           "engine.cpp", "api.cpp", "host_pool.cpp", "fasta.cpp", "noalloc.hip", "k_translate.hip",
+           "engine.cpp", "api.cpp", "host_pool.cpp", "fasta.cpp", "noalloc.hip", "k_translate.hip", "k_swblock.hip",
            "serve.cpp"]
 
 COMMON = [

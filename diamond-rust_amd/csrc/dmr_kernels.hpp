@@ -183,6 +183,20 @@ constexpr int kRowKeyBits = 18;
 // Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width
 // <= 64*K, K a power of two up to kMaxWaveK (0 = wider than 64*kMaxWaveK).
 constexpr int kMaxWaveK = 32;
+// Block-per-DP int32 kernel (k_swblock.hip): 16 waves, K offsets per thread,
+// widths up to 1024*K for K = 4, 8, 16 (0 = wider).
+constexpr int kMaxBlockK = 16;
+int swblock_k_for(int64_t width);
+uint64_t swblock_row_bytes(int K);  // trace bytes of one row
+// Every DP of list whose width lies in (2048, 16384] (only_k: one K's range,
+// 0 = all); score mode writes outs[job].score, trace mode outs[task] and the
+// task's ops, and needs the score pass's outputs (known, by job). Wider DPs
+// and DPs whose best may lie in the clamped rows go to the fallback list (job
+// ids in score mode, tasks in trace mode).
+void launch_sw_block(bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs, SeqSet q, SeqSet db,
+                     const int8_t* m32, int gap_open, int gap_extend, SwOut* outs, uint8_t* trace,
+                     const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback,
+                     uint32_t* nfallback, const SwOut* known, int only_k, hipStream_t st);
 constexpr int kWaveKinds = 6;  // K = 1, 2, 4, 8, 16, 32
 constexpr int kPackedClasses = 7;  // packed 16-bit score classes (k_sw16.hip)
 int swwave_k_for(int64_t width);

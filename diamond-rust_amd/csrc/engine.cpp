@@ -1020,6 +1020,18 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     for (int c = W0 + 2; c >= W0; --c) launch_wave(c, aux_[1]);  // K = 4, 2, 1
   }
   join();
+  // Bands wider than a wave covers: one block of 16 waves per DP (widths up
+  // to 16384; wider ones, and DPs whose best may lie in the clamped rows, go
+  // on to the fallback list of the row-exact kernel).
+  const uint64_t nwide = cend[kRowExact] > cbeg[kRowExact] ? cend[kRowExact] - cbeg[kRowExact] : 0;
+  float wide_ms = 0;
+  if (nwide) {
+    DMR_HIP(hipEventRecord(ev_[12 + 2 * kRowExact], st_));
+    launch_sw_block(false, jobs, idx + cbeg[kRowExact], (uint32_t)nwide, qs, dbs, matrix32_.as<int8_t>(),
+                    opt.gap_open, opt.gap_extend, outs, nullptr, nullptr, nullptr, nullptr, fallback, cnt + 16,
+                    nullptr, 0, st_);
+    DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
+  }
   // DPs at the 16-bit limit (f16 1900, int16 32000, or a best below 11 with
   // clamped rows that may score): re-scored by the int32 wave kernel (widths
   // <= 256). It reads their count on the device, so no host read precedes it;
@@ -1035,10 +1047,12 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     if ((c < kPackedClasses && nl && class_lanes[c]) ||
         (c >= kPackedClasses && swk_launched(c, cbeg, cend, fast)))
       S->ms_class[c] = stage_ms(12 + 2 * c, 13 + 2 * c);
-  // Row-exact int32 kernel: bands wider than 64 * kMaxWaveK and the DPs whose
-  // band never reaches the last in-band row but whose clamped rows may score.
-  const uint64_t nwide = cend[kRowExact] > cbeg[kRowExact] ? cend[kRowExact] - cbeg[kRowExact] : 0;
-  const uint64_t nlist = nwide + nfb;
+  if (nwide) wide_ms = stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
+  S->ms_class[kRowExact] = wide_ms;
+  // Row-exact int32 kernel: the DPs the other kernels handed on (bands wider
+  // than the block kernel's reach, and DPs whose band never reaches the last
+  // in-band row but whose clamped rows may score).
+  const uint64_t nlist = nfb;
   S->sw16_lanes = nl;
   S->sw_redo = nr16;
   S->sw_generic = nlist;
@@ -1046,8 +1060,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   last_cend_.assign(cend, cend + kClasses);
   if (nlist) {
     uint32_t* list = sw_list_.ensure<uint32_t>(nlist + 1);
-    if (nwide) DMR_HIP(hipMemcpyAsync(list, idx + cbeg[kRowExact], nwide * 4, hipMemcpyDeviceToDevice, st_));
-    if (nfb) DMR_HIP(hipMemcpyAsync(list + nwide, fallback, nfb * 4, hipMemcpyDeviceToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(list, fallback, nfb * 4, hipMemcpyDeviceToDevice, st_));
     uint32_t* stride = stride_.ensure<uint32_t>(nlist + 1);
     launch_job_geometry(jobs, list, nlist, dbs, nullptr, nullptr, stride, st_);
     uint64_t* soff = scr_off_.ensure<uint64_t>(nlist + 1);
@@ -1060,7 +1073,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
                       nullptr, nullptr, nullptr, outs, false, st_, /*scatter=*/true);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
     DMR_HIP(hipEventSynchronize(ev_[13 + 2 * kRowExact]));
-    S->ms_class[kRowExact] = stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
+    S->ms_class[kRowExact] = wide_ms + stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
   }
 }
 
@@ -1158,6 +1171,7 @@ void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<
   run->dbs = dbs;
   run->go = go;
   run->ge = ge;
+  run->known = known;
   run->ns = ns;
   if (!ns) return;
   // Group g < kWaveKinds: wave kernel with K = 2^g; the last group: row-exact.
@@ -1341,7 +1355,65 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector
       (*ops)[items[t]] = OpsRef{pin_ops + pin_poff[T], o.n_ops};
     }
   }
-  for (uint64_t k : group[kWaveKinds]) fallback_items.push_back(k);
+  // Bands wider than a wave covers: the block kernel (with the score pass's
+  // best scores); what it hands on joins the row-exact kernel's list.
+  std::vector<uint64_t> block_items;
+  for (uint64_t k : group[kWaveKinds]) {
+    const SwJob& j = hjobs[k];
+    if (run->known && swblock_k_for((int64_t)j.d1 - j.d0 + 1)) block_items.push_back(k);
+    else fallback_items.push_back(k);
+  }
+  if (!block_items.empty()) {
+    const uint64_t nb = block_items.size();
+    std::vector<uint32_t> jl(nb);
+    std::vector<uint64_t> toff(nb + 1, 0), ooff(nb + 1, 0);
+    for (uint64_t t = 0; t < nb; ++t) {
+      const uint64_t k = block_items[t];
+      const SwJob& j = hjobs[k];
+      jl[t] = list[k];
+      const int64_t n = (int64_t)j.q1 - j.q0, m = slen[k];
+      const int64_t rlo = std::max<int64_t>(1, (int64_t)j.d0 + 1), rhi = std::min<int64_t>(n, m + j.d1);
+      const uint64_t rows = rhi >= rlo ? (uint64_t)(rhi - rlo + 1) : 0;
+      toff[t + 1] = toff[t] + rows * swblock_row_bytes(swblock_k_for((int64_t)j.d1 - j.d0 + 1));
+      ooff[t + 1] = ooff[t] + (uint64_t)(n + m + 1);
+      cells += rows * (uint64_t)((int64_t)j.d1 - j.d0 + 1);
+    }
+    uint32_t* dl = bk_list_.ensure<uint32_t>(nb + 1);
+    uint64_t* dto = bk_toff_.ensure<uint64_t>(nb + 1);
+    uint64_t* doo = bk_ooff_.ensure<uint64_t>(nb + 1);
+    DMR_HIP(hipMemcpyAsync(dl, jl.data(), nb * 4, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(dto, toff.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st_));
+    DMR_HIP(hipMemcpyAsync(doo, ooff.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st_));
+    uint32_t* dcnt = bk_cnt_.ensure<uint32_t>(1);
+    launch_fill_u32(dcnt, 1, 0, st_);
+    uint32_t* dfb = bk_fb_.ensure<uint32_t>(nb + 1);
+    SwOut* dout = bk_out_.ensure<SwOut>(nb + 1);
+    uint32_t* dops = bk_ops_.ensure<uint32_t>(ooff[nb] + 1);
+    launch_sw_block(true, djobs, dl, (uint32_t)nb, qs, dbs, matrix32_.as<int8_t>(), go, ge, dout,
+                    bk_trace_.ensure<uint8_t>(toff[nb] + 4), dto, dops, doo, dfb, dcnt, run->known, 0, st_);
+    std::vector<SwOut> ho(nb);
+    std::vector<uint32_t> hfb(nb + 1), hops(ooff[nb]);
+    uint32_t hn = 0;
+    DMR_HIP(hipMemcpyAsync(&hn, dcnt, 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(hfb.data(), dfb, nb * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(ho.data(), dout, nb * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    if (ooff[nb]) DMR_HIP(hipMemcpyAsync(hops.data(), dops, ooff[nb] * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    std::vector<char> is_fb(nb, 0);
+    if (hn > nb) throw Error(-2, "block-kernel fallback count exceeds its tasks");
+    for (uint32_t f = 0; f < hn; ++f) {
+      if (hfb[f] >= nb) throw Error(-2, "block-kernel fallback task out of range");
+      is_fb[hfb[f]] = 1;
+      fallback_items.push_back(block_items[hfb[f]]);
+    }
+    store->parts.push_back(std::move(hops));
+    const uint32_t* bops = store->parts.back().data();
+    for (uint64_t t = 0; t < nb; ++t) {
+      if (is_fb[t]) continue;
+      (*outs)[block_items[t]] = ho[t];
+      (*ops)[block_items[t]] = OpsRef{bops + ooff[t], ho[t].n_ops};
+    }
+  }
   if (!fallback_items.empty()) {
     const uint64_t nf = fallback_items.size();
     std::vector<uint32_t> jl(nf);

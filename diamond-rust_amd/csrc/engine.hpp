@@ -299,6 +299,7 @@ class Engine {
     const std::vector<uint32_t>* slen = nullptr;
     SeqSet qs{}, dbs{};
     int go = 0, ge = 0;
+    const SwOut* known = nullptr;  // score-pass outputs by job (device)
     uint64_t ns = 0, cells = 0;
     std::vector<std::vector<uint64_t>> group;
     // Per wave-kernel group: its block in the pinned input and its first
@@ -429,6 +430,7 @@ class Engine {
   PinBuf tb_pin_in_, tb_pin_out_, tb_pin_poff_, tb_pin_fb_;
   std::shared_ptr<PinPool> ops_pool_ = std::make_shared<PinPool>();
   DevBuf tb_out_all_, tb_ops_all_, tb_poff_all_, tb_pack_all_;
+  DevBuf bk_list_, bk_toff_, bk_ooff_, bk_trace_, bk_ops_, bk_out_, bk_fb_, bk_cnt_;
   PinBuf sel_pin_;  // selection candidates (index, job, score-pass output)
   // Host copies of each group's job list and offsets; they stay alive until
   // the copies queued from them have run.

@@ -64,3 +64,20 @@ def test_clipped_window_matches_oracle(dmr, ctx, orc, synth, sens):
     ctx.db_load(w["db_letters"], w["db_offsets"])
     got = ctx.blastp(w["q_letters"], w["q_offsets"], dmr.SearchConfig(sensitivity=sens))
     assert got.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+
+
+@pytest.mark.parametrize("copies", [16, 26, 44, 60])
+def test_wide_merged_bands_match_oracle(dmr, ctx, orc, synth, copies):
+    """Merged bands 2,000-15,000 diagonals wide (the block-per-DP kernel in
+    the score pass and, with the score pass's best, in the traceback),
+    clipped or not, against the oracle."""
+    w = clipping_workload(synth, copies=copies, seed=31 + copies)
+    ref = orc.search(w["db_letters"], w["db_offsets"], w["q_letters"], w["q_offsets"], sensitivity="default",
+                     threads=8)
+    c = ref.counters()
+    assert c["anchor_tie_events"] == 0 and c["band_tie_events"] == 0 and len(ref) >= 1
+    ctx.db_load(w["db_letters"], w["db_offsets"])
+    got = ctx.blastp(w["q_letters"], w["q_offsets"], dmr.SearchConfig(sensitivity="default"))
+    prof = ctx.class_profile()
+    assert prof["rowexact"]["jobs"] >= 1  # the class of bands wider than 2048
+    assert got.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)

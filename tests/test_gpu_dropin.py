@@ -222,3 +222,33 @@ def test_six_frames_match_oracle(ctx, orc):
             exp = orc.six_frames(nt, min_orf=min_orf)
             for k in range(6):
                 assert list(got[6 * r + k]) == list(exp[k]), (min_orf, r, k)
+
+
+def test_wide_bands_block_kernel(ctx, orc):
+    """Bands wider than one wave covers (2049 .. 16384 offsets: the
+    block-per-DP kernel, K = 4, 8, 16) and past it (the row-exact kernel):
+    scores of long related and unrelated pairs, bands inside, across and
+    outside the matrix, against the oracle."""
+    rng = np.random.default_rng(23)
+    cases = []
+    for w in [1100, 1500, 2100, 3000, 4500, 7000, 8300]:
+        for kind in range(3):
+            n = int(rng.integers(600, 2600))
+            q = rand_seq(rng, n)
+            s = q.copy() if kind == 0 else (mutate(rng, q, 0.3, 0.03) if kind == 1 else rand_seq(rng, int(rng.integers(400, 3000))))
+            c = int(rng.integers(-len(s), n))
+            cases.append((q, s, c - w, c + w + int(rng.integers(0, 3))))
+    # Outside the matrix on both sides, and one very long unrelated pair.
+    q, s = rand_seq(rng, 900), rand_seq(rng, 700)
+    cases += [(q, s, 1000, 1000 + 3000), (q, s, -5000, -1500), (q, s, -2500, 2500)]
+    qbuf = np.concatenate([c[0] for c in cases])
+    sbuf = np.concatenate([c[1] for c in cases])
+    jobs, qo, so = [], 0, 0
+    for q, s, d0, d1 in cases:
+        jobs.append((qo, len(q), so, len(s), d0, d1))
+        qo += len(q)
+        so += len(s)
+    got = ctx.extend_span_batch(qbuf, sbuf, jobs, traceback=False)
+    for (q, s, d0, d1), g in zip(cases, got):
+        ref = orc.extend_span(q, s, d0, d1)
+        assert g == (ref["score"] if ref else None), (len(q), len(s), d0, d1)
