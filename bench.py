@@ -378,7 +378,8 @@ def main():
         "metric": METRIC, "value": round(qps, 2), "unit": "queries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": ("f16" if "F16" in kname else "int16") if kname.startswith("k_sw16") else "int32",
+        "dtype": ("f16-held exact integers (< 2048; int32 re-run from 1900)" if "F16" in kname else "int16")
+                 if kname.startswith("k_sw16") else "int32",
         "data": "synthetic (seeded generator diamond-rust_amd/synth.py)",
         "config": {"workload": f"{w.name}: blastp --sensitive, {w.n_q} queries vs {w.n_db} "
                                f"proteins / {int(w.db_offsets[-1])} letters",
