@@ -30,7 +30,7 @@ DMR_E_TIMEOUT = -6
 # Symbols declared by include/dmr.h.
 EXPORTS = [
     "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
-    "dmr_ctx_set_tuning", "dmr_ctx_get_tuning",
+    "dmr_ctx_set_tuning", "dmr_ctx_get_tuning", "dmr_ctx_load_tuning", "dmr_sw16_variants",
     "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend_span_noalloc", "dmr_extend",
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_headers", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
@@ -41,24 +41,26 @@ EXPORTS = [
     "dmr_server_create", "dmr_server_submit", "dmr_server_wait", "dmr_server_stats", "dmr_server_destroy",
 ]
 
+# Packed score classes (tuning keys lanes_<c> / threads_<c>): band_for widths
+# and merged-band width bounds.
+PACKED_CLASSES = ["w25", "w33", "w61", "w81", "w129", "m97", "m129", "m161", "m193", "m225", "m257"]
 # Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
-SCORE_CLASSES = ["w25", "w33", "w61", "w81", "w129", "m193", "m256", "wave64", "wave128", "wave256",
-                 "wave512", "wave1024", "wave2048", "rowexact"]
+SCORE_CLASSES = PACKED_CLASSES + ["wave64", "wave128", "wave256", "wave512", "wave1024", "wave2048", "rowexact"]
 
-def _class_kernel(i: int, name: str, split_mask: int, f16: bool) -> str:
-    """Kernel that scores class i: its template instance as rocprofv3 names it,
-    without the dmr namespaces."""
+
+def _class_kernel(name: str, lanes: int, threads: int, f16: bool) -> str:
+    """Kernel that scores class `name` with the given lanes per DP pair and
+    block size: its template instance as rocprofv3 names it, without the dmr
+    namespaces."""
     if name.startswith("wave"):
         return f"k_sw_wave<{int(name[4:]) // 64}, false>"
     if name == "rowexact":
-        return "k_sw_generic<false>"
+        return "k_sw_block / k_sw_generic<false>"
     tr = "F16" if f16 else "I16"
-    if name.startswith("m"):  # merged bands: 8 lanes per DP pair, per-DP walls
-        return f"k_sw16p<257, 8, 512, {tr}, true>" if name == "m256" else f"k_sw16p<193, 8, 512, {tr}, true>"
     w = int(name[1:])
-    lanes = 2 if w == 81 else 4
-    threads = 768 if w == 129 else 512
-    return f"k_sw16p<{w}, {lanes}, {threads}, {tr}, false>" if (split_mask >> i) & 1 else f"k_sw16<{w}, {tr}>"
+    if name.startswith("m"):  # merged bands: per-DP walls
+        return f"k_sw16p<{w}, {lanes}, {threads}, {tr}, true>"
+    return f"k_sw16p<{w}, {lanes}, {threads}, {tr}, false>" if lanes > 1 else f"k_sw16<{w}, {tr}>"
 
 
 HSP_FIELDS = ["query", "subject", "frame", "q_begin", "q_end", "s_begin", "s_end", "score",
@@ -68,6 +70,17 @@ STAT_FIELDS = ["query_seeds", "runs", "hits", "hit_runs", "segments", "groups", 
                "selected", "traceback_cells", "ms_seed", "ms_join", "ms_sort_hits",
                "ms_ungapped", "ms_bands", "ms_sw", "ms_select", "ms_traceback", "ms_total",
                "ms_host", "ms_translate"]
+
+
+def sw16_variants():
+    """Compiled packed score kernels (dmr_sw16_variants): dicts of class name,
+    width (bound), lanes per DP pair and threads per block."""
+    n = C.c_uint32()
+    _check(lib().dmr_sw16_variants(None, 0, C.byref(n)))
+    out = np.zeros(4 * n.value, np.int32)
+    _check(lib().dmr_sw16_variants(out.ctypes.data, n.value, C.byref(n)))
+    return [dict(cls=PACKED_CLASSES[int(r[0])], width=int(r[1]), lanes=int(r[2]), threads=int(r[3]))
+            for r in out.reshape(-1, 4)]
 
 
 class DmrError(RuntimeError):
@@ -150,6 +163,8 @@ def lib():
             "dmr_device_count": ([P], C.c_int), "dmr_band_for": ([U64], U32),
             "dmr_ctx_set_tuning": ([P, C.c_char_p, D], C.c_int),
             "dmr_ctx_get_tuning": ([P, C.c_char_p, P], C.c_int),
+            "dmr_ctx_load_tuning": ([P, C.c_char_p], C.c_int),
+            "dmr_sw16_variants": ([P, U32, P], C.c_int),
             "dmr_evalue": ([I32, I32, I32, U64, U64, U64, P, P], C.c_int),
             "dmr_seed_build": ([P, P, P, U32, C.c_char_p, C.c_char_p, U32, D, P, U64, P], C.c_int),
             "dmr_seed_join": ([P, P, U64, P, U64, U64, P, U64, P], C.c_int),
@@ -395,8 +410,19 @@ class Context:
             pass
 
     def set_tuning(self, key: str, value: float):
-        """dmr_ctx_set_tuning (slice_hits, profile, split_mask, score_plan, f16)."""
+        """dmr_ctx_set_tuning: slice_hits, profile, and the score-pass kernels
+        (lanes_<class>, threads_<class>, merged_mask, f16, score_plan)."""
         _check(lib().dmr_ctx_set_tuning(self._h, key.encode(), float(value)))
+
+    def load_tuning(self, path):
+        """dmr_ctx_load_tuning: the tuning file (None = none)."""
+        _check(lib().dmr_ctx_load_tuning(self._h, None if path is None else str(path).encode()))
+
+    def layout(self) -> dict:
+        """Score-pass settings of the last search (every layout key)."""
+        keys = [f"{k}_{c}" for c in PACKED_CLASSES for k in ("lanes", "threads")]
+        keys += ["merged_mask", "f16", "score_plan"]
+        return {k: int(self.get_tuning(k)) for k in keys}
 
     def get_tuning(self, key: str) -> float:
         v = C.c_double()
@@ -516,9 +542,11 @@ class Context:
         c = np.zeros(len(names), np.uint64)
         m = np.zeros(len(names), np.float64)
         _check(lib().dmr_last_class_profile(self._h, j.ctypes.data, c.ctypes.data, m.ctypes.data))
-        mask, f16 = self.score_layout()
+        lay = self.layout()
         return {n: dict(jobs=int(j[i]), cells=int(c[i]), ms=float(m[i]),
-                        kernel=_class_kernel(i, n, mask, f16)) for i, n in enumerate(names)}
+                        kernel=_class_kernel(n, lay.get(f"lanes_{n}", 0), lay.get(f"threads_{n}", 0),
+                                             bool(lay["f16"])))
+                for i, n in enumerate(names)}
 
     def score_layout(self):
         """(split mask, f16) of the packed score classes: bit c set = class c

@@ -1,0 +1,188 @@
+"""Autotuner of the score-pass kernels: picks, on the device, the packed
+kernel of every width class (lanes per DP pair and threads per block among
+the compiled ones), the merged-band width buckets in use, the arithmetic
+(f16 or int16 pairs) and the stream plan, for a search preset on a workload,
+and writes the choice with every trial's timing to the tuning file that the
+library loads at context creation (tuning_<arch>.json beside libdmr.so).
+
+None of these settings changes a result: every trial's outfmt-6 text must
+equal the first one's, or the tuner stops.
+
+Search: coordinate descent from the built-in defaults. Each step tries the
+values of one setting with the others held, and keeps a value only when its
+median search time beats the best so far by more than --min-gain (noise
+guard): f16; the merged-band kernels in use (one bucket bound added or
+removed at a time, until no change helps); the kernel of each class that
+has DPs in the workload; the stream plan; the merged buckets once more.
+
+Usage (on a GPU box; the workloads are the seeded synthetic ones):
+  python diamond-rust_amd/autotune.py                       # every preset below
+  python diamond-rust_amd/autotune.py --preset sensitive --reps 7
+"""
+from __future__ import annotations
+
+import argparse
+import datetime
+import importlib.util
+import json
+import os
+import pathlib
+import statistics
+import sys
+import time
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+# preset -> (synthetic workload, queries used: the first n of its batch)
+PRESETS = {
+    "sensitive": ("blastp-sensitive-1024", 1024),
+    "default": ("blastp-sensitive-1024", 1024),
+    "very-sensitive": ("blastp-very-sensitive-4096", 512),
+}
+MERGED = ["m97", "m129", "m161", "m193", "m225", "m257"]
+
+
+def _load(name, path, pkg=False):
+    spec = importlib.util.spec_from_file_location(
+        name, path, submodule_search_locations=[str(path.parent)] if pkg else None)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print("[autotune]", *a, flush=True)
+
+
+class Tuner:
+    def __init__(self, dmr, ctx, cfg, ids, reps, min_gain):
+        self.dmr, self.ctx, self.cfg, self.ids = dmr, ctx, cfg, ids
+        self.reps, self.min_gain = reps, min_gain
+        self.trials = []
+        self.ref_text = None
+
+    def apply(self, settings):
+        # Lanes before threads: a lanes_ key resets its class's block size.
+        for k in sorted(settings, key=lambda k: (not k.startswith("lanes_"), k)):
+            self.ctx.set_tuning(k, settings[k])
+
+    def measure(self, settings, changed):
+        self.apply(settings)
+        r = self.ctx.search_loaded(self.cfg)  # warm-up (and the result check)
+        text = r.format(self.ids[0], self.ids[1])
+        if self.ref_text is None:
+            self.ref_text = text
+        elif text != self.ref_text:
+            raise SystemExit(f"autotune: the output changed under {changed}; no setting may change a result")
+        times, sw = [], []
+        for _ in range(self.reps):
+            t0 = time.perf_counter()
+            r = self.ctx.search_loaded(self.cfg)
+            times.append((time.perf_counter() - t0) * 1e3)
+            sw.append(r.stats()["ms_sw"])
+        ms = statistics.median(times)
+        self.trials.append(dict(changed=changed, ms=round(ms, 3), score_pass_ms=round(statistics.median(sw), 3)))
+        log(f"{json.dumps(changed)} {ms:.2f} ms (score pass {statistics.median(sw):.2f} ms)")
+        return ms
+
+    def run(self, variants):
+        cur = self.ctx.layout()
+        best = self.measure(cur, {"defaults": True})
+        default_ms = best
+
+        def consider(cand, changed):
+            nonlocal cur, best
+            ms = self.measure(cand, changed)
+            won = ms < best * (1 - self.min_gain)
+            self.trials[-1]["accepted"] = won
+            if won:
+                cur, best = cand, ms
+            self.apply(cur)
+            return won
+
+        consider({**cur, "f16": 1 - cur["f16"]}, {"f16": 1 - cur["f16"]})
+
+        def merged_pass():
+            changed = True
+            while changed:
+                changed = False
+                for i in range(len(MERGED) - 1):
+                    m = cur["merged_mask"] ^ (1 << i)
+                    if consider({**cur, "merged_mask": m}, {"merged_mask": m}):
+                        changed = True
+
+        merged_pass()
+        # Classes with DPs under the current buckets.
+        self.ctx.set_tuning("profile", 1)
+        self.ctx.search_loaded(self.cfg)
+        prof = self.ctx.class_profile()
+        self.ctx.set_tuning("profile", 0)
+        for name in self.dmr.PACKED_CLASSES:
+            if prof[name]["jobs"] == 0:
+                continue
+            for v in variants:
+                if v["cls"] != name or (v["lanes"], v["threads"]) == (cur[f"lanes_{name}"], cur[f"threads_{name}"]):
+                    continue
+                ch = {f"lanes_{name}": v["lanes"], f"threads_{name}": v["threads"]}
+                consider({**cur, **ch}, ch)
+        for p in range(3):
+            if p != cur["score_plan"]:
+                consider({**cur, "score_plan": p}, {"score_plan": p})
+        merged_pass()
+        return cur, best, default_ms
+
+
+def tune_preset(dmr, synth, preset, reps, min_gain, queries=None):
+    wl, nq = PRESETS[preset]
+    nq = queries or nq
+    log(f"preset {preset}: workload {wl}, {nq} queries")
+    w = synth.make(wl)
+    if nq < w.n_q:
+        w = w.subset_queries(list(range(nq)))
+    ctx = dmr.Context(0)
+    ctx.load_tuning(None)  # trials start from the built-in defaults
+    ctx.db_load(w.db_letters, w.db_offsets)
+    ctx.queries_load(w.q_letters, w.q_offsets)
+    cfg = dmr.SearchConfig(sensitivity=preset)
+    t = Tuner(dmr, ctx, cfg, (w.q_ids, w.db_ids), reps, min_gain)
+    layout, ms, default_ms = t.run(dmr.sw16_variants())
+    ctx.close()
+    log(f"preset {preset}: {default_ms:.2f} ms with the defaults -> {ms:.2f} ms")
+    return dict(settings=layout, workload=wl, queries=nq, reps=reps, min_gain=min_gain, ms=round(ms, 3),
+                default_ms=round(default_ms, 3), trials=t.trials,
+                tuned_at=datetime.datetime.now(datetime.timezone.utc).strftime("%Y-%m-%dT%H:%M:%SZ"))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--preset", action="append", choices=sorted(PRESETS), help="repeatable; default: all")
+    ap.add_argument("--queries", type=int, default=None, help="queries of the workload used (default per preset)")
+    ap.add_argument("--reps", type=int, default=5, help="timed searches per trial (median)")
+    ap.add_argument("--min-gain", type=float, default=0.01, help="fraction a trial must gain to be kept")
+    ap.add_argument("--arch", default=os.environ.get("DMR_OFFLOAD_ARCH", "gfx950"))
+    ap.add_argument("--out", default=None, help="tuning file (default: tuning_<arch>.json beside libdmr.so)")
+    a = ap.parse_args(argv)
+    dmr = _load("diamond_rust_amd", HERE / "__init__.py", pkg=True)
+    synth = _load("diamond_rust_amd.synth", HERE / "synth.py")
+    out = pathlib.Path(a.out) if a.out else HERE / f"tuning_{a.arch}.json"
+    doc = json.loads(out.read_text()) if out.exists() else {}
+    doc.setdefault("presets", {})
+    doc["arch"] = a.arch
+    doc["library"] = dmr.lib().dmr_version().decode()
+    doc["variants"] = [[v["cls"], v["width"], v["lanes"], v["threads"]] for v in dmr.sw16_variants()]
+    for p in a.preset or list(PRESETS):
+        doc["presets"][p] = tune_preset(dmr, synth, p, a.reps, a.min_gain, a.queries)
+        tmp = out.with_suffix(".json.tmp")
+        tmp.write_text(json.dumps(doc, indent=1) + "\n")
+        os.replace(tmp, out)
+        log(f"wrote {out}")
+    # The file must load: a context created now takes it.
+    ctx = dmr.Context(0)
+    ctx.load_tuning(str(out))
+    assert ctx.get_tuning("tuned_presets") == len(doc["presets"])
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

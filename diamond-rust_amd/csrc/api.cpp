@@ -381,37 +381,54 @@ void dmr_ctx_destroy(dmr_ctx* ctx) { delete ctx; }
 
 int dmr_ctx_set_tuning(dmr_ctx* ctx, const char* key, double value) {
   if (!ctx || !key) return fail(DMR_E_ARG, "NULL argument");
-  const std::string k = key;
-  dmr::Engine& e = *ctx->eng;
-  if (k == "slice_hits") {
-    if (!(value >= 1)) return fail(DMR_E_ARG, "slice_hits must be at least 1");
-    e.set_slice_hits((uint64_t)value);
-  } else if (k == "profile") {
-    e.set_profile(value != 0);
-  } else if (k == "split_mask") {
-    if (value < 0 || value > 31) return fail(DMR_E_ARG, "split_mask is a 5-bit mask");
-    e.set_split_mask((unsigned)value);
-  } else if (k == "score_plan") {
-    if (value != 0 && value != 1 && value != 2) return fail(DMR_E_ARG, "score_plan is 0, 1 or 2");
-    e.set_score_plan((int)value);
-  } else if (k == "f16") {
-    e.set_f16(value != 0);
-  } else {
-    return fail(DMR_E_ARG, "unknown tuning key: " + k);
-  }
-  return DMR_OK;
+  return guard([&] {
+    const std::string k = key;
+    dmr::Engine& e = *ctx->eng;
+    if (k == "slice_hits") {
+      if (!(value >= 1)) return fail(DMR_E_ARG, "slice_hits must be at least 1");
+      e.set_slice_hits((uint64_t)value);
+    } else if (k == "profile") {
+      e.set_profile(value != 0);
+    } else {
+      e.set_layout_key(k, value);
+    }
+    return DMR_OK;
+  });
 }
 
 int dmr_ctx_get_tuning(const dmr_ctx* ctx, const char* key, double* value) {
   if (!ctx || !key || !value) return fail(DMR_E_ARG, "NULL argument");
-  const std::string k = key;
-  const dmr::Engine& e = *ctx->eng;
-  if (k == "slice_hits") *value = (double)e.slice_hits();
-  else if (k == "split_mask") *value = e.split_mask();
-  else if (k == "score_plan") *value = e.score_plan();
-  else if (k == "f16") *value = e.packed_f16() ? 1 : 0;
-  else if (k == "last_slices") *value = e.last_slices();
-  else return fail(DMR_E_ARG, "unknown tuning key: " + k);
+  return guard([&] {
+    const std::string k = key;
+    const dmr::Engine& e = *ctx->eng;
+    if (k == "slice_hits") *value = (double)e.slice_hits();
+    else if (k == "last_slices") *value = e.last_slices();
+    else if (k == "tuned_presets") *value = (double)e.tuned_presets();
+    else *value = e.layout_key(k);
+    return DMR_OK;
+  });
+}
+
+int dmr_ctx_load_tuning(dmr_ctx* ctx, const char* path) {
+  if (!ctx) return fail(DMR_E_ARG, "NULL argument");
+  return guard([&] {
+    ctx->eng->load_tuning(path ? path : "");
+    return DMR_OK;
+  });
+}
+
+int dmr_sw16_variants(int32_t* out, uint32_t cap, uint32_t* n) {
+  if (!n) return fail(DMR_E_ARG, "NULL argument");
+  const auto& v = dmr::sw16_variants();
+  *n = (uint32_t)v.size();
+  if (!out) return DMR_OK;
+  if (cap < v.size()) return fail(DMR_E_CAPACITY, "variant buffer too small");
+  for (size_t k = 0; k < v.size(); ++k) {
+    out[4 * k] = v[k].cls;
+    out[4 * k + 1] = v[k].width;
+    out[4 * k + 2] = v[k].lanes;
+    out[4 * k + 3] = v[k].threads;
+  }
   return DMR_OK;
 }
 
@@ -1244,8 +1261,13 @@ int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double
 
 int dmr_score_layout(dmr_ctx* ctx, uint32_t* split_mask, uint32_t* f16) {
   if (!ctx) return fail(DMR_E_ARG, "NULL argument");
-  if (split_mask) *split_mask = ctx->eng->split_mask();
-  if (f16) *f16 = ctx->eng->packed_f16() ? 1u : 0u;
+  const dmr::ScoreLayout& L = ctx->eng->layout();
+  if (split_mask) {
+    *split_mask = 0;
+    for (int c = 0; c < dmr::kExactClasses; ++c)
+      if (L.lanes[c] > 1) *split_mask |= 1u << c;
+  }
+  if (f16) *f16 = L.f16 ? 1u : 0u;
   return DMR_OK;
 }
 

@@ -124,7 +124,28 @@ int pair_table_entries();
 // Pair-score table of the packed kernels: entry (qa, sA, sB) holds
 // (score(qa, sA) + o, score(qa, sB) + o) as two int16 or, f16, two halves.
 void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out);
-void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits,
+// Packed score classes: 0-4 the band_for widths (2 * band + 1 = 25, 33, 61,
+// 81, 129); kExactClasses + i for merged bands (widths 62..256 other than 81
+// and 129) up to width 97 + 32 i, i < kMergedClasses (per-DP walls).
+constexpr int kExactClasses = 5;
+constexpr int kMergedClasses = 6;
+constexpr int kPackedClasses = kExactClasses + kMergedClasses;
+// Width of class cls (0-4) or the width bound of a merged class.
+int sw16_width(int cls);
+// Class table of the merged widths for the merged kernels in use (bit i of
+// mask = class kExactClasses + i; the widest is always in use): a merged
+// width goes to the narrowest kernel in use that holds it (4 bits per bucket
+// of 32 widths).
+uint32_t sw16_merged_map(unsigned mask);
+// A compiled packed kernel: class, its width (bound), lanes per DP pair (1 =
+// k_sw16, the whole band in one lane; otherwise k_sw16p split over that many
+// lanes) and threads per block.
+struct Sw16Variant {
+  int cls, width, lanes, threads;
+};
+const std::vector<Sw16Variant>& sw16_variants();
+bool sw16_has_variant(int cls, int lanes, int threads);
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits, uint32_t merged_map,
                      uint64_t* keys, uint32_t* idx, hipStream_t st);
 // Lane pairing of the sorted score-pass jobs (see k_lane_heads): head flags
 // and class ranges, then lanes at the heads' scan positions, then the lanes
@@ -136,24 +157,23 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const 
                        const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st);
 void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
                          hipStream_t st);
-int sw16_width(int cls);
-// Lanes per DP pair of class cls when it runs split (always for 5-6).
-int sw16_split_lanes(int cls);
 // Lane order of the packed classes: key = class << kLaneSpanBits | (kLaneSpanMax
 // - row span), so a sort puts each class's longest lanes first.
 constexpr int kLaneSpanBits = 13;
 constexpr uint32_t kLaneSpanMax = (1u << kLaneSpanBits) - 1;
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                      uint64_t* keys, uint32_t* idx, hipStream_t st);
+                      uint32_t merged_map, uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st);
-// Packed score pass of width class cls (k_sw16.hip): split = sw16_split_lanes
-// lanes per DP pair (part bands each) for classes 0-4; classes 5-6 (merged bands)
-// always run split (sw16_split_lanes: 8 lanes per pair) with per-DP walls;
-// f16 = half-precision
+// Packed score pass of class cls over nlanes lanes (k_sw16.hip, merged
+// classes k_sw16m.hip) with the compiled kernel (cls, lanes_per_pair,
+// threads) of sw16_variants(); false when there is none. f16 = half-precision
 // arithmetic (pair_tab must then be the f16 table).
-void launch_sw16(int cls, bool split, bool f16, const uint2* lanes, uint32_t nlanes, const SwJob* jobs,
-                 SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
+bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
+                 const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
                  SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
+bool launch_sw16_merged(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
+                        const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter, SwOut* outs,
+                        uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);
@@ -198,7 +218,6 @@ void launch_sw_block(bool traceback, const SwJob* jobs, const uint32_t* list, ui
                      const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback,
                      uint32_t* nfallback, const SwOut* known, int only_k, hipStream_t st);
 constexpr int kWaveKinds = 6;  // K = 1, 2, 4, 8, 16, 32
-constexpr int kPackedClasses = 7;  // packed 16-bit score classes (k_sw16.hip)
 int swwave_k_for(int64_t width);
 // Packs the run-length ops of n traceback DPs (capacity slots at ops_off)
 // into packed, at pack_off (n + 1 offsets; pack_off[n] = total words).

@@ -3,6 +3,8 @@
 
 #include "host_pool.hpp"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -109,27 +111,89 @@ Engine::Engine(int device) : device_(device) {
   noalloc_ = noalloc_scratch_address();
   const char* fg = std::getenv("DMR_FORCE_GENERIC");
   force_generic_ = fg && fg[0] == '1';
-  // Width classes whose packed kernel runs split (two lanes per DP pair):
-  // bit c for class c; DMR_SW16_SPLIT overrides the default (81 and 129).
   const char* lr = std::getenv("DMR_LANE_REPORT");
   lane_report_ = lr && lr[0] == '1';
   // DMR_SCORE_SERIAL=1 runs the score classes one after another on the engine
   // stream (per-class kernel times without sharing the chip).
   const char* ss = std::getenv("DMR_SCORE_SERIAL");
   score_serial_ = ss && ss[0] == '1';
-  // DMR_SCORE_PLAN (0-2, default 0) selects how the class kernels share the
-  // streams (see score_pass).
-  const char* plan = std::getenv("DMR_SCORE_PLAN");
-  score_plan_ = plan ? std::atoi(plan) : 0;
-  // Packed score pass in f16 (default) or int16: DMR_SW16_F16=0 selects int16.
-  const char* hf = std::getenv("DMR_SW16_F16");
-  use_f16_ = !(hf && hf[0] == '0');
-  const char* sp = std::getenv("DMR_SW16_SPLIT");
-  split_mask_ = sp ? (unsigned)std::strtoul(sp, nullptr, 0) : 0x18u;
+  // Score-pass kernel settings: the tuning file DMR_TUNING names (empty:
+  // none), else tuning_<arch>.json beside this library when it exists.
+  const char* tf = std::getenv("DMR_TUNING");
+  if (tf) {
+    load_tuning(tf);
+  } else {
+    Dl_info info{};
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    if (dladdr(reinterpret_cast<void*>(&default_layout), &info) && info.dli_fname) {
+      std::string lib = info.dli_fname;
+      const std::string path = lib.substr(0, lib.rfind('/') + 1) + "tuning_" + arch + ".json";
+      if (FILE* f = std::fopen(path.c_str(), "rb")) {
+        std::fclose(f);
+        load_tuning(path);
+      }
+    }
+  }
   // Hits per slice of targets (device memory of the per-hit and per-DP
   // buffers grows with it: ~52 bytes per hit and ~150 per DP job).
   const char* sh = std::getenv("DMR_SLICE_HITS");
   slice_hits_ = sh ? std::strtoull(sh, nullptr, 0) : (300ull << 20);
+}
+
+void Engine::load_tuning(const std::string& path) {
+  std::map<std::string, Settings> t;
+  if (!path.empty()) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw Error(-1, "cannot open tuning file " + path);
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+    std::fclose(f);
+    try {
+      t = parse_tuning(text);
+    } catch (const Error& e) {
+      throw Error(-1, path + ": " + e.what());
+    }
+  }
+  std::swap(tuned_, t);
+  try {
+    layout_ = resolve_layout(layout_preset_);
+  } catch (...) {
+    std::swap(tuned_, t);
+    throw;
+  }
+  tuning_file_ = path;
+}
+
+ScoreLayout Engine::resolve_layout(const std::string& preset, const Settings* pinned) const {
+  ScoreLayout L = default_layout();
+  auto any = tuned_.find("*");
+  if (any != tuned_.end()) apply_settings(&L, any->second);
+  auto own = tuned_.find(preset);
+  if (own != tuned_.end() && preset != "*") apply_settings(&L, own->second);
+  apply_settings(&L, pinned ? *pinned : pinned_);
+  return L;
+}
+
+void Engine::set_layout_key(const std::string& key, double value) {
+  if (!is_layout_key(key)) throw Error(-1, "unknown tuning key: " + key);
+  Settings p = pinned_;
+  p[key] = value;
+  // A lane count sets its class's block size too: a block size pinned
+  // before it goes.
+  if (key.compare(0, 6, "lanes_") == 0) p.erase("threads_" + key.substr(6));
+  layout_ = resolve_layout(layout_preset_, &p);
+  pinned_ = std::move(p);
+}
+
+void Engine::copy_tuning(const Engine& o) {
+  tuned_ = o.tuned_;
+  pinned_ = o.pinned_;
+  tuning_file_ = o.tuning_file_;
+  layout_ = o.layout_;
+  layout_preset_ = o.layout_preset_;
 }
 
 Engine::~Engine() {
@@ -875,13 +939,18 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   for (int c = 0; c < kClasses; ++c) S->ms_class[c] = 0;
   if (!njobs) return;
   last_target_bits_ = target_bits;
+  layout_ = resolve_layout(opt.sensitivity);
+  layout_preset_ = opt.sensitivity;
+  const ScoreLayout L = layout_;
+  const uint32_t mmap = sw16_merged_map(L.merged_mask);
   const bool fast = opt.gap_open == 11 && opt.gap_extend == 1 && !force_generic_;
   if (fast && (!qs.pad || !qs.pad_offs || !dbs.pad || !dbs.pad_offs))
     throw Error(-2, "score pass without padded letters");
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
-  launch_classify(jobs, njobs, qs, dbs, target_bits, keys, idx, st_);
-  const int key_bits = 4 + target_bits + kRowKeyBits;
+  launch_classify(jobs, njobs, qs, dbs, target_bits, mmap, keys, idx, st_);
+  static_assert(kClasses <= 32, "the class takes 5 key bits");
+  const int key_bits = 5 + target_bits + kRowKeyBits;
   radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
                    njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
                    b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_);
@@ -924,8 +993,13 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     class_lanes[c] = h_summ[2 * kClasses + npacked + c];
   }
   const uint32_t nl = h_summ[2 * (kClasses + npacked)];
-  uint32_t* cnt = sw_counters_.ensure<uint32_t>(32);
-  launch_fill_u32(cnt, 32, 0, st_);
+  // Counters of the score pass: the redo list's length, one task counter per
+  // packed and wave class, the fallback list's length, the redo kernel's
+  // task counter.
+  constexpr int kCntRedo = 0, kCntPacked = 1, kCntWave = kCntPacked + kPackedClasses,
+                kCntFallback = kCntWave + kWaveKinds, kCntRedoWave = kCntFallback + 1, kCntWords = kCntRedoWave + 1;
+  uint32_t* cnt = sw_counters_.ensure<uint32_t>(kCntWords);
+  launch_fill_u32(cnt, kCntWords, 0, st_);
   uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
   uint32_t* fallback = sw_fallback_.ensure<uint32_t>(njobs + 1);
   const int cus = cu_count_;
@@ -937,14 +1011,15 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     // key), so class_lane0/class_lanes still hold.
     uint64_t* lk = sw_keys_.as<uint64_t>();
     uint32_t* li = sw_idx2_.as<uint32_t>();
-    launch_lane_keys(paired, nl, jobs, qs, dbs, lk, li, st_);
+    launch_lane_keys(paired, nl, jobs, qs, dbs, mmap, lk, li, st_);
+    static_assert(kPackedClasses <= 16, "the lane class takes 4 key bits");
     radix_sort_pairs(lk, li, sw_keys2_.as<uint64_t>(), sw_lane_idx_.ensure<uint32_t>(nl + 1), nl,
-                     3 + kLaneSpanBits, b_hist_.ensure<uint32_t>(radix_hist_elems(nl)),
+                     4 + kLaneSpanBits, b_hist_.ensure<uint32_t>(radix_hist_elems(nl)),
                      b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nl)), st_);
     lanes = sw_lanes_.ensure<uint2>(nl + 1);
     launch_gather_lanes(paired, li, nl, lanes, st_);
     if (lane_report_ && qs.offs == q_offs_.as<uint32_t>())
-      report_lanes(jobs, njobs, lanes, nl, class_lane0, class_lanes);
+      report_lanes(jobs, njobs, lanes, nl, class_lane0, class_lanes, L);
   }
   // The class kernels are independent (distinct jobs and counters; the redo
   // and fallback lists are filled through atomics): they run side by side on
@@ -956,9 +1031,10 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   auto launch_packed = [&](int c, hipStream_t s) {
     if (!nl || !class_lanes[c]) return;
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
-    launch_sw16(c, ((split_mask_ >> c) & 1u) != 0, use_f16_, lanes + class_lane0[c], class_lanes[c], jobs,
-                qs, dbs, (use_f16_ ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(),
-                cnt + 1 + c, outs, redo, cnt, cus, s);
+    if (!launch_sw16(c, L.lanes[c], L.threads[c], L.f16, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs,
+                     (L.f16 ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(), cnt + kCntPacked + c,
+                     outs, redo, cnt + kCntRedo, cus, s))
+      throw Error(-1, "no compiled packed kernel for class " + packed_class_name(c));
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   };
   // Wave-per-DP int32 kernel: classes 7-12 (and every width class when the
@@ -987,33 +1063,34 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     }
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
-                   opt.gap_open, opt.gap_extend, cnt + 8 + (c - kPackedClasses), outs, nullptr, nullptr, nullptr,
-                   nullptr, fallback, cnt + 16, cus, s);
+                   opt.gap_open, opt.gap_extend, cnt + kCntWave + (c - kPackedClasses), outs, nullptr, nullptr,
+                   nullptr, nullptr, fallback, cnt + kCntFallback, cus, s);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   };
-  static_assert(kPackedClasses == 7 && kWaveKinds == 6, "the launch plan names every class");
+  static_assert(kExactClasses == 5 && kWaveKinds == 6, "the launch plan names every class");
   const int W0 = kPackedClasses;  // first wave class (K = 1)
+  auto launch_merged = [&](hipStream_t s) {  // widest first
+    for (int c = kPackedClasses - 1; c >= kExactClasses; --c) launch_packed(c, s);
+  };
   if (score_serial_) {
     for (int c = kPackedClasses - 1; c >= 0; --c) launch_packed(c, st_);
     for (int c = W0; c < W0 + kWaveKinds; ++c) launch_wave(c, st_);
-  } else if (score_plan_ == 2) {
+  } else if (L.plan == 2) {
     // Every throughput class on a stream of its own beside the others.
     for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
-    launch_packed(6, aux_[2]);
-    launch_packed(5, aux_[2]);
+    launch_merged(aux_[2]);
     launch_packed(4, aux_[0]);
     launch_packed(3, aux_[1]);
     for (int c = 2; c >= 0; --c) launch_packed(c, aux_[2]);
     for (int c = W0 + 2; c >= W0; --c) launch_wave(c, aux_[1]);  // K = 4, 2, 1
   } else {
-    // The throughput-bound packed classes one after another on one stream
-    // (plan 0, the default: the smaller classes first, then the widest, which
-    // then has the chip nearly to itself; plan 1: widest first); the
+    // The throughput-bound band_for classes one after another on one stream
+    // (plan 0: the smaller classes first, then the widest, which then has the
+    // chip nearly to itself; plan 1: widest first); the merged bands and the
     // latency-bound classes (few, long DPs) beside them on the other streams.
     for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
-    launch_packed(6, aux_[2]);
-    launch_packed(5, aux_[2]);
-    if (score_plan_ == 1)
+    launch_merged(aux_[2]);
+    if (L.plan == 1)
       for (int c = 4; c >= 0; --c) launch_packed(c, aux_[0]);
     else
       for (int c : {3, 2, 1, 0, 4}) launch_packed(c, aux_[0]);
@@ -1028,7 +1105,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   if (nwide) {
     DMR_HIP(hipEventRecord(ev_[12 + 2 * kRowExact], st_));
     launch_sw_block(false, jobs, idx + cbeg[kRowExact], (uint32_t)nwide, qs, dbs, matrix32_.as<int8_t>(),
-                    opt.gap_open, opt.gap_extend, outs, nullptr, nullptr, nullptr, nullptr, fallback, cnt + 16,
+                    opt.gap_open, opt.gap_extend, outs, nullptr, nullptr, nullptr, nullptr, fallback, cnt + kCntFallback,
                     nullptr, 0, st_);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
   }
@@ -1037,12 +1114,12 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   // <= 256). It reads their count on the device, so no host read precedes it;
   // one read afterwards brings both counts.
   launch_sw_wave(4, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
-                 opt.gap_extend, cnt + 17, outs, nullptr, nullptr, nullptr, nullptr, fallback, cnt + 16, cus,
-                 st_, nullptr, cnt);
-  uint32_t* h_cnt = sw_pin_summary_.ensure<uint32_t>(32);
-  DMR_HIP(hipMemcpyAsync(h_cnt, cnt, 17 * 4, hipMemcpyDeviceToHost, st_));
+                 opt.gap_extend, cnt + kCntRedoWave, outs, nullptr, nullptr, nullptr, nullptr, fallback,
+                 cnt + kCntFallback, cus, st_, nullptr, cnt + kCntRedo);
+  uint32_t* h_cnt = sw_pin_summary_.ensure<uint32_t>(kCntWords);
+  DMR_HIP(hipMemcpyAsync(h_cnt, cnt, kCntWords * 4, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
-  const uint32_t nr16 = h_cnt[0], nfb = h_cnt[16];
+  const uint32_t nr16 = h_cnt[kCntRedo], nfb = h_cnt[kCntFallback];
   for (int c = 0; c < kRowExact; ++c)
     if ((c < kPackedClasses && nl && class_lanes[c]) ||
         (c >= kPackedClasses && swk_launched(c, cbeg, cend, fast)))
@@ -1082,7 +1159,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
 // the union of its two DPs' rows; a wave runs as many rows as its longest
 // lane or lane group).
 void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
-                          const uint32_t* class_lane0, const uint32_t* class_lanes) {
+                          const uint32_t* class_lane0, const uint32_t* class_lanes, const ScoreLayout& L) {
   std::vector<SwJob> hj(njobs);
   std::vector<uint2> hl(nl);
   DMR_HIP(hipMemcpyAsync(hj.data(), jobs, njobs * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
@@ -1098,7 +1175,7 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
   for (int c = 0; c < kPackedClasses; ++c) {
     if (!class_lanes[c]) continue;
     const int64_t W = sw16_width(c);
-    const int P = (c >= 5 || ((split_mask_ >> c) & 1u)) ? sw16_split_lanes(c) : 1;
+    const int P = L.lanes[c];
     const int per_wave = 64 / P;
     uint64_t band = 0, pair_slots = 0, wave_slots = 0;
     int64_t wave_max = 0, in_wave = 0;
@@ -1812,7 +1889,9 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
     while ((1ull << tb) <= 2 * njobs) ++tb;
     score_pass(djobs.as<SwJob>(), njobs, tb, qs, ss, so, o, &S);
     last_dropin_stats_ = S;
+    // The class profile of a drop-in batch: kernel times only.
     for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S.ms_class[c];
+    last_njobs_ = 0;
     last_slices_ = 1;
   }
   DMR_HIP(hipMemcpyAsync(outs->data(), o, njobs * sizeof(SwOut), hipMemcpyDeviceToHost, st_));

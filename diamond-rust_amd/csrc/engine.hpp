@@ -17,6 +17,7 @@
 #include "dmr_device.hpp"
 #include "dmr_kernels.hpp"
 #include "dmr_tables.hpp"
+#include "tuning.hpp"
 
 namespace dmr {
 
@@ -272,14 +273,20 @@ class Engine {
   // Profiling: per-class jobs and DP cells of every slice of a search
   // (dmr_last_class_profile / dmr_last_dp_cells of sliced searches).
   void set_profile(bool on) { profile_ = on; }
-  void set_split_mask(unsigned m) { split_mask_ = m & 0x1fu; }
-  void set_score_plan(int p) { score_plan_ = p; }
-  void set_f16(bool on) { use_f16_ = on; }
-  int score_plan() const { return score_plan_; }
+  // Score-pass kernel settings (tuning.hpp): the tuning file's settings per
+  // preset ("" clears them), keys pinned over them (validated against the
+  // compiled kernels; a bad key leaves the settings as they were), and the
+  // layout of the last search (before the first: the default preset's).
+  void load_tuning(const std::string& path);
+  void set_layout_key(const std::string& key, double value);
+  double layout_key(const std::string& key) const { return layout_value(layout_, key); }
+  const ScoreLayout& layout() const { return layout_; }
+  size_t tuned_presets() const { return tuned_.size(); }
+  const std::string& tuning_file() const { return tuning_file_; }
+  // The tuning file and pinned keys of another engine (serving workers).
+  void copy_tuning(const Engine& o);
   // Cells of every DP of the last search (one extra pass, for GCUPS).
   uint64_t count_last_dp_cells();
-  unsigned split_mask() const { return split_mask_; }
-  bool packed_f16() const { return use_f16_; }
   static constexpr int kClasses = kPackedClasses + kWaveKinds + 1;  // k_classify in k_sw16.hip
   static constexpr int kRowExact = kClasses - 1;
   // Per score-kernel class of the last search (see StageStats::ms_class):
@@ -437,13 +444,18 @@ class Engine {
   std::vector<uint64_t> last_cbeg_, last_cend_;
   int cu_count_ = 256;
   bool force_generic_ = false;
-  unsigned split_mask_ = 0x18u;
   bool lane_report_ = false;
   bool score_serial_ = false;
-  int score_plan_ = 0;
-  bool use_f16_ = true;
+  std::map<std::string, Settings> tuned_;
+  Settings pinned_;
+  std::string tuning_file_;
+  ScoreLayout layout_ = default_layout();
+  std::string layout_preset_ = "default";
+  // Layout for a search preset: defaults, the file's "*" and preset entries,
+  // then the pinned keys (pinned: in place of pinned_).
+  ScoreLayout resolve_layout(const std::string& preset, const Settings* pinned = nullptr) const;
   void report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
-                    const uint32_t* class_lane0, const uint32_t* class_lanes);
+                    const uint32_t* class_lane0, const uint32_t* class_lanes, const ScoreLayout& L);
   uint64_t last_njobs_ = 0;
   int last_target_bits_ = 0;
   double last_ms_class_[kClasses] = {};

@@ -266,6 +266,7 @@ int dmr_server_create(dmr_ctx* ctx, uint32_t workers, uint64_t hbm_budget, const
       auto w = std::make_unique<Worker>();
       w->eng = std::make_unique<dmr::Engine>(ctx->eng->device());
       w->eng->share_db(*ctx->eng);
+      w->eng->copy_tuning(*ctx->eng);
       s->workers.push_back(std::move(w));
     }
     dmr_server* sp = s.get();

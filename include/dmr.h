@@ -112,14 +112,36 @@ int dmr_device_count(int* n);
  *                 memory grows with it, ~52 bytes per hit and ~150 per DP);
  *   "profile"     1 = keep per-class DP jobs and cells over every slice of a
  *                 search (for dmr_last_class_profile / dmr_last_dp_cells);
- *   "split_mask"  packed score classes 0-4 that run split over several lanes
- *                 per DP pair (bit c = class c);
- *   "score_plan"  how the score-class kernels share the streams (0, 1, 2);
- *   "f16"         1 = packed score pass in f16 pairs, 0 = saturating int16.
- * dmr_ctx_get_tuning reads them back, plus "last_slices" (slices of the
- * last search). */
+ * and the score-pass kernels, per packed class <c> (w25 w33 w61 w81 w129:
+ * band_for widths; m97 m129 m161 m193 m225 m257: merged bands up to that
+ * width):
+ *   "lanes_<c>"   lanes per DP pair (1 = the whole band in one lane, w*
+ *                 only; else a compiled split kernel, dmr_sw16_variants);
+ *                 also sets the class's first compiled block size for it;
+ *   "threads_<c>" threads per block of a compiled kernel for those lanes;
+ *   "merged_mask" merged kernels in use (bit i = m97 + 32 i; m257 always):
+ *                 a merged band goes to the narrowest one that holds it;
+ *   "f16"         1 = packed pairs in f16, 0 = saturating int16;
+ *   "score_plan"  how the score-class kernels share the streams (0, 1, 2).
+ * Score-pass settings come from built-in defaults, then the tuning file (the
+ * file DMR_TUNING names at context creation, empty for none; else
+ * tuning_<arch>.json beside the library when it exists; settings per search
+ * preset, written by the autotuner), then the keys set here, which stay
+ * pinned over every preset. A key with no compiled kernel is refused
+ * (DMR_E_ARG) and changes nothing. dmr_ctx_get_tuning reads the settings of
+ * the last search (before the first: those of the default preset), plus
+ * "last_slices" (slices of the last search) and "tuned_presets" (presets in
+ * the loaded tuning file). */
 int dmr_ctx_set_tuning(dmr_ctx* ctx, const char* key, double value);
 int dmr_ctx_get_tuning(const dmr_ctx* ctx, const char* key, double* value);
+/* Replaces the context's tuning file (NULL or "" = none); the pinned keys
+ * stay. A malformed file, or one naming an unknown key or a kernel that is
+ * not compiled, is refused (DMR_E_ARG) and changes nothing. */
+int dmr_ctx_load_tuning(dmr_ctx* ctx, const char* path);
+/* Compiled packed score kernels: *n entries of (class, width or width bound,
+ * lanes per DP pair, threads per block) into out (4 * cap int32). out NULL:
+ * *n only; cap too small: DMR_E_CAPACITY. */
+int dmr_sw16_variants(int32_t* out, uint32_t cap, uint32_t* n);
 
 /* gapped::band_for (src/gapped.rs:27-32) */
 uint32_t dmr_band_for(uint64_t len);
@@ -283,19 +305,19 @@ int dmr_result_stats(const dmr_result* r, double* out21);
 int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
 /* Score-pass profile of the last search on ctx, per kernel class
  * (DMR_SCORE_CLASSES entries): 0-4 packed 16-bit kernel for band widths
- * 25/33/61/81/129, 5-6 packed kernel for merged bands of widths 62..193 and
- * 194..256, 7-12 wave-per-DP int32 kernel for widths up to
- * 64/128/256/512/1024/2048, 13 row-exact int32 kernel (wider bands, and
- * bands that end before their clamped rows): DP jobs, DP cells and kernel
- * milliseconds. Diagnostics only; no counterpart in the reference. */
-#define DMR_SCORE_CLASSES 14
+ * 25/33/61/81/129, 5-10 packed kernel for merged bands of widths up to
+ * 97/129/161/193/225/257 (those in use; see "merged_mask"), 11-16
+ * wave-per-DP int32 kernel for widths up to 64/128/256/512/1024/2048, 17
+ * block-per-DP and row-exact int32 kernels (wider bands, and bands that end
+ * before their clamped rows): DP jobs, DP cells and kernel milliseconds.
+ * Diagnostics only; no counterpart in the reference. */
+#define DMR_SCORE_CLASSES 18
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
-/* Layout of the packed classes 0-4 on ctx (5-6 always split, 8 lanes):
- * bit c of *split_mask set = class c runs split over several lanes per DP
- * pair (k_sw16p: two for width 81, four otherwise), clear = one lane
- * (k_sw16); *f16 = 1 when the packed kernels compute in f16 pairs, 0 in
- * saturating int16 pairs. Either may be NULL. Diagnostics only; no
- * counterpart in the reference. */
+/* Layout of the packed classes 0-4 in the last search on ctx: bit c of
+ * *split_mask set = class c ran split over several lanes per DP pair
+ * (k_sw16p), clear = one lane (k_sw16); *f16 = 1 when the packed kernels
+ * computed in f16 pairs, 0 in saturating int16 pairs. Either may be NULL.
+ * Diagnostics only; no counterpart in the reference. */
 int dmr_score_layout(dmr_ctx* ctx, uint32_t* split_mask, uint32_t* f16);
 void dmr_result_destroy(dmr_result* r);
 

@@ -59,3 +59,24 @@ def test_product_tables_match_reference(dmr, orc, synth):
     row = [p for p in g["params"] if p[0] == 11 and p[1] == 1][0]
     e, b = dmr.evalue(100, 300, 300, 10 ** 6)
     assert abs(b - (row[2] * 100 - math.log(row[3])) / math.log(2)) < 1e-12
+
+
+def test_committed_tuning_file_names_compiled_kernels(dmr):
+    """The autotuner's file beside the library (loaded by every context):
+    every preset's settings name known keys and compiled kernels, and the
+    tuned time is no worse than the defaults' (dmr_sw16_variants is host
+    code: no GPU needed)."""
+    import json
+    path = dmr.LIB_PATH.parent / "tuning_gfx950.json"
+    doc = json.loads(path.read_text())
+    compiled = {(v["cls"], v["lanes"], v["threads"]) for v in dmr.sw16_variants()}
+    assert len(compiled) >= 26 and doc["arch"] == "gfx950"
+    keys = {f"{k}_{c}" for c in dmr.PACKED_CLASSES for k in ("lanes", "threads")} | {
+        "merged_mask", "f16", "score_plan"}
+    for name, entry in doc["presets"].items():
+        s = entry["settings"]
+        assert set(s) <= keys, name
+        for c in dmr.PACKED_CLASSES:
+            assert (c, s[f"lanes_{c}"], s[f"threads_{c}"]) in compiled, (name, c)
+        assert 0 <= s["merged_mask"] < 64 and s["score_plan"] in (0, 1, 2)
+        assert entry["ms"] <= entry["default_ms"] and len(entry["trials"]) > 10

@@ -1,15 +1,17 @@
 """The packed 16-bit score pass in both layouts and both arithmetics against
-the oracle, and the merged-band classes (16 or 8 lanes per pair, per-DP walls).
+the oracle, the merged-band classes (per-DP walls), and every compiled
+packed kernel.
 
-Each width class runs either with one lane per DP pair (the whole band in a
-lane's registers) or split over two or four lanes (part bands, with the
-horizontal-gap correction across lanes). DMR_SW16_SPLIT selects the layout
-per class when a context is created (bit c = class c). The pairs compute in
-f16 (exact for integers up to 2048; DPs scoring 1900 or more are re-scored in
-int32) or, with DMR_SW16_F16=0, in saturating int16 (re-scored from 32000).
-Every combination is checked on drop-in batches (one DP per lane), on scores
-around the f16 limit, and on a batched search (two DPs of the same query per
-lane).
+Each band_for width class runs either with one lane per DP pair (the whole
+band in a lane's registers) or split over two or more lanes (part bands,
+with the horizontal-gap correction across lanes); dmr_ctx_set_tuning selects
+the kernel per class ("lanes_<class>", "threads_<class>") and the merged-band
+kernels in use ("merged_mask"). The pairs compute in f16 (exact for integers
+up to 2048; DPs scoring 1900 or more are re-scored in int32) or, with "f16"
+0, in saturating int16 (re-scored from 32000). Every combination is checked
+on drop-in batches (one DP per lane), on scores around the f16 limit, and on
+a batched search (two DPs of the same query per lane); every compiled kernel
+on the widths of its class.
 """
 import numpy as np
 import pytest
@@ -20,17 +22,25 @@ from test_gpu_search import EXTENDED, run_both, workload_dict
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [(0x00, "1"), (0x1F, "1"), (0x00, "0"), (0x1F, "0")]
+EXACT = ["w25", "w33", "w61", "w81", "w129"]
+LAYOUTS = [(False, True), (True, True), (False, False), (True, False)]
+
+
+def first_split(dmr, name):
+    return next(v for v in dmr.sw16_variants() if v["cls"] == name and v["lanes"] > 1)
 
 
 @pytest.fixture(params=LAYOUTS, ids=["full-f16", "split-f16", "full-i16", "split-i16"])
-def layout_ctx(request, dmr, monkeypatch):
-    mask, f16 = request.param
-    monkeypatch.setenv("DMR_SW16_SPLIT", str(mask))
-    monkeypatch.setenv("DMR_SW16_F16", f16)
+def layout_ctx(request, dmr):
+    split, f16 = request.param
     c = dmr.Context(0)
+    c.load_tuning(None)
+    for name in EXACT:
+        c.set_tuning(f"lanes_{name}", first_split(dmr, name)["lanes"] if split else 1)
+    c.set_tuning("f16", int(f16))
+    mask = sum(1 << k for k, n in enumerate(EXACT) if split)
     got = c.score_layout()
-    assert got == (mask, f16 == "1"), got
+    assert got == (mask, f16), got
     yield c
     c.close()
 
@@ -78,16 +88,14 @@ def test_scores_around_f16_limit(layout_ctx, orc):
         assert g == r, (len(c[0]), len(c[1]), c[2], c[3], r)
 
 
-def test_merged_band_widths(layout_ctx, orc):
-    """Bands of widths 62..256 other than 81 and 129 (merged bands) run the
-    split kernels with a wall at each DP's own width (classes m193/m256);
-    61, 81, 129 and 257+ stay on their own kernels. Every width near a class
-    edge is included."""
-    rng = np.random.default_rng(91)
-    edges = [60, 61, 62, 63, 64, 65, 80, 81, 82, 100, 127, 128, 129, 130, 131, 192, 193, 194, 195, 255,
-             256, 257, 258]
+def merged_cases(seed=91, count=420):
+    """Bands of widths 62..256 (merged bands), with every width near a
+    merged-class edge and the band_for widths between them."""
+    rng = np.random.default_rng(seed)
+    edges = [60, 61, 62, 63, 64, 65, 80, 81, 82, 96, 97, 98, 100, 127, 128, 129, 130, 131, 160, 161, 162,
+             192, 193, 194, 195, 224, 225, 226, 255, 256, 257, 258]
     cases = []
-    for k in range(420):
+    for k in range(count):
         w = edges[k] if k < len(edges) else int(rng.integers(62, 257))
         n = int(rng.integers(1, 1000 if rng.random() < 0.2 else 400))
         q = rand_seq(rng, n, canonical_only=rng.random() < 0.9)
@@ -98,13 +106,25 @@ def test_merged_band_widths(layout_ctx, orc):
             s = rand_seq(rng, 2)
         d0 = int(rng.integers(-len(s) - w, n + 1))
         cases.append((q, s, d0, d0 + w - 1))
+    return cases
+
+
+def test_merged_band_widths(layout_ctx, orc):
+    """Bands of widths 62..256 other than 81 and 129 (merged bands) run the
+    split kernels with a wall at each DP's own width (the default merged
+    kernels, m193 and m257, and then all six); 61, 81, 129 and 257+ stay on
+    their own kernels."""
+    cases = merged_cases()
     assert sum(61 < c[3] - c[2] + 1 <= 128 for c in cases) > 50
     assert sum(129 < c[3] - c[2] + 1 <= 193 for c in cases) > 50
     assert sum(193 < c[3] - c[2] + 1 <= 256 for c in cases) > 50
-    got = _score_batch(layout_ctx, cases)
-    for (q, s, d0, d1), g in zip(cases, got):
-        ref = orc.extend_span(q, s, d0, d1)
-        assert g == (ref["score"] if ref else None), (len(q), len(s), d0, d1)
+    ref = [orc.extend_span(*c) for c in cases]
+    ref = [r["score"] if r else None for r in ref]
+    for mask in (0x28, 0x3F):
+        layout_ctx.set_tuning("merged_mask", mask)
+        got = _score_batch(layout_ctx, cases)
+        for c, g, r in zip(cases, got, ref):
+            assert g == r, (mask, len(c[0]), len(c[1]), c[2], c[3])
 
 
 def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
@@ -115,5 +135,57 @@ def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
     res, ref = run_both(dmr, layout_ctx, orc, w, "sensitive")
     assert len(ref) > 0
     prof = layout_ctx.class_profile()
-    assert prof["m193"]["jobs"] > 0 and prof["m256"]["jobs"] > 0
+    assert prof["m193"]["jobs"] > 0 and prof["m257"]["jobs"] > 0
     assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+
+
+@pytest.fixture(scope="module")
+def variant_cases(orc):
+    """Per packed class, DPs of its widths with the oracle's scores."""
+    rng = np.random.default_rng(404)
+    merged = merged_cases(seed=405, count=900)
+    out = {}
+    bounds = {"m97": (62, 97), "m129": (98, 129), "m161": (130, 161), "m193": (162, 193), "m225": (194, 225),
+              "m257": (226, 256)}
+    for name, (lo, hi) in bounds.items():
+        out[name] = [c for c in merged if lo <= c[3] - c[2] + 1 <= hi and c[3] - c[2] + 1 not in (81, 129)]
+    for name in EXACT:
+        w = int(name[1:])
+        cases = []
+        for _ in range(150):
+            n = int(rng.integers(1, 900 if rng.random() < 0.2 else 300))
+            q = rand_seq(rng, n, canonical_only=rng.random() < 0.9)
+            s = q.copy() if rng.random() < 0.3 else mutate(rng, q, float(rng.uniform(0.1, 0.6)), 0.06)
+            if len(s) == 0:
+                s = rand_seq(rng, 2)
+            d0 = int(rng.integers(-len(s) - w, n + 1))
+            cases.append((q, s, d0, d0 + w - 1))
+        out[name] = cases
+    for name, cases in out.items():
+        assert len(cases) >= 40, name
+        out[name] = (cases, [(r["score"] if r else None) for r in (orc.extend_span(*c) for c in cases)])
+    return out
+
+
+@pytest.mark.parametrize("f16", [1, 0], ids=["f16", "i16"])
+def test_every_compiled_kernel(dmr, variant_cases, f16):
+    """Each compiled packed kernel (class, lanes per DP pair, threads), pinned
+    for its class with every merged kernel in use, scores the widths of its
+    class as the oracle does."""
+    c = dmr.Context(0)
+    c.load_tuning(None)
+    c.set_tuning("f16", f16)
+    c.set_tuning("merged_mask", 0x3F)
+    seen = 0
+    for v in dmr.sw16_variants():
+        c.set_tuning(f"lanes_{v['cls']}", v["lanes"])
+        c.set_tuning(f"threads_{v['cls']}", v["threads"])
+        cases, ref = variant_cases[v["cls"]]
+        got = _score_batch(c, cases)
+        assert c.get_tuning(f"lanes_{v['cls']}") == v["lanes"]
+        assert c.class_profile()[v["cls"]]["ms"] > 0, v  # the class's kernel ran
+        for cs, g, r in zip(cases, got, ref):
+            assert g == r, (v, len(cs[0]), len(cs[1]), cs[2], cs[3])
+        seen += 1
+    assert seen == len(dmr.sw16_variants()) >= 26
+    c.close()
