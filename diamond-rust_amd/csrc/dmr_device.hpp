@@ -71,10 +71,12 @@ void scan_exclusive_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, ui
 
 // LSD radix sort of (key, value) pairs on the low `bits` bits of key.
 // Sorted result lands in (k0, v0); (k1, v1) is scratch of the same size.
+// n_dev: sort the first *n_dev of a buffer of n (a bound; the grid), the
+// count read on the device.
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n, int bits,
-                      uint32_t* hist, uint64_t* scan_tmp, hipStream_t st);
+                      uint32_t* hist, uint64_t* scan_tmp, hipStream_t st, const uint64_t* n_dev = nullptr);
 void radix_sort_keys(uint64_t* k0, uint64_t* k1, uint64_t n, int bits, uint32_t* hist,
-                     uint64_t* scan_tmp, hipStream_t st);
+                     uint64_t* scan_tmp, hipStream_t st, const uint64_t* n_dev = nullptr);
 uint64_t radix_hist_elems(uint64_t n);
 uint64_t radix_scan_tmp_elems(uint64_t n);
 

@@ -34,31 +34,40 @@ void launch_seed_emit(const uint8_t* res, const uint32_t* seq_of, const uint32_t
 uint32_t seed_enum_blocks(uint64_t r0, uint64_t r1);
 void launch_bucket_table(const uint64_t* keys, uint64_t n, int shift, uint64_t nbuckets,
                          uint64_t* table, hipStream_t st);
-void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st);
+// Device counts: launchers that take n_dev (or nruns_dev / nq_dev) run over
+// the host count as a bound (the grid) and take the device value as the
+// count; their flags and counts past it are 0, so a scan over the bound
+// totals the same. No host read of the count is needed.
+void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st,
+                      const uint64_t* n_dev = nullptr);
 void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n, uint32_t* out,
                           hipStream_t st);
 void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_t nruns,
                        uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                       uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st);
+                       uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st,
+                       const uint64_t* nruns_dev = nullptr, const uint64_t* nq_dev = nullptr);
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
                       const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off,
                       const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
                       const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st,
-                      const uint32_t* slice_start = nullptr, const uint32_t* slice_nq = nullptr);
+                      const uint32_t* slice_start = nullptr, const uint32_t* slice_nq = nullptr,
+                      const uint64_t* nruns_dev = nullptr, const uint64_t* nq_dev = nullptr);
 // Run flags that also split a key's entries by record (group[target]).
 void launch_run_flags_grouped(const uint64_t* keys, const uint32_t* locs, const uint32_t* seq_of,
-                              const uint32_t* group, uint64_t n, uint32_t* flags, hipStream_t st);
+                              const uint32_t* group, uint64_t n, uint32_t* flags, hipStream_t st,
+                              const uint64_t* n_dev = nullptr);
 // thits[target] += hits of every kept (run, chunk) pair (thits zeroed first).
 void launch_target_hits(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
                         uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint64_t* thits,
-                        hipStream_t st);
+                        hipStream_t st, const uint64_t* nruns_dev = nullptr, const uint64_t* nq_dev = nullptr);
 // Per (run, chunk): the hits, first entry and entry count of the part whose
 // targets lie in [t0, t1).
 void launch_join_slice(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
                        uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint32_t t0,
                        uint32_t t1, uint32_t* slice_counts, uint32_t* slice_start, uint32_t* slice_nq,
-                       hipStream_t st);
+                       hipStream_t st, const uint64_t* nruns_dev = nullptr,
+                       const uint64_t* nq_dev = nullptr);
 
 // ---- ungapped + bands ----------------------------------------------------
 struct SeqSet {
@@ -95,19 +104,25 @@ void launch_compact_frames(const uint64_t* foff, const uint32_t* good, const uin
 void launch_seq_of(const uint32_t* offs, uint32_t n, uint32_t* seq_of, hipStream_t st);
 
 // Key flags over packed hits: 1 where (hit >> shift) changes.
-void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st);
+void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st,
+                      const uint64_t* n_dev = nullptr);
+// dst[k] = src[k], k < n <= 64 (one small kernel, for counts).
+void launch_copy_u64(const uint64_t* src, uint64_t* dst, uint32_t n, hipStream_t st);
 void launch_xdrop_runs(const uint64_t* hits, const uint32_t* run_start, uint64_t nruns,
                        uint64_t nhits, HitGeom g, SeqSet q, SeqSet db, const int8_t* matrix32,
-                       int xdrop, int cutoff, SegRec* segs, uint32_t* keep, hipStream_t st);
+                       int xdrop, int cutoff, SegRec* segs, uint32_t* keep, hipStream_t st,
+                       const uint64_t* nruns_dev = nullptr);
 void launch_compact_segments(const SegRec* segs, const uint64_t* hits, const uint32_t* keep,
                              const uint64_t* pos, uint64_t nhits, int group_shift, SegRec* out,
                              uint64_t* out_gkey, hipStream_t st);
 void launch_bands(const SegRec* segs, const uint64_t* gkey, const uint32_t* group_start,
                   uint64_t ngroups, uint64_t nsegs, uint32_t bs, SeqSet q, SeqSet db,
                   int32_t* tmp_d0, int32_t* tmp_d1, uint32_t* tmp_best, SwJob* tmp_jobs,
-                  uint32_t* nbands, hipStream_t st);
+                  uint32_t* nbands, hipStream_t st, const uint64_t* ngroups_dev = nullptr,
+                  const uint64_t* nsegs_dev = nullptr);
 void launch_jobs_compact(const SwJob* tmp_jobs, const uint32_t* group_start, const uint32_t* nbands,
-                         const uint64_t* job_off, uint64_t ngroups, SwJob* jobs, hipStream_t st);
+                         const uint64_t* job_off, uint64_t ngroups, SwJob* jobs, hipStream_t st,
+                         const uint64_t* ngroups_dev = nullptr);
 
 // ---- gapped --------------------------------------------------------------
 // Per job: DP cells (sum of row widths, rows 1..n) and the int32 row stride.
@@ -146,15 +161,16 @@ struct Sw16Variant {
 const std::vector<Sw16Variant>& sw16_variants();
 bool sw16_has_variant(int cls, int lanes, int threads);
 void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits, uint32_t merged_map,
-                     uint64_t* keys, uint32_t* idx, hipStream_t st);
+                     uint64_t* keys, uint32_t* idx, hipStream_t st, const uint64_t* njobs_dev = nullptr);
 // Lane pairing of the sorted score-pass jobs (see k_lane_heads): head flags
 // and class ranges, then lanes at the heads' scan positions, then the lanes
 // per class. summary holds 2 * (nclasses + npacked) + 1 words.
 void launch_lane_heads(const uint64_t* keys, const uint32_t* flags, const uint64_t* pos, const uint32_t* run_start,
                        uint64_t n, int cls_shift, int npacked, int nclasses, uint32_t* heads, uint32_t* summary,
-                       hipStream_t st);
+                       hipStream_t st, const uint64_t* n_dev = nullptr);
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const uint32_t* heads,
-                       const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st);
+                       const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st,
+                       const uint64_t* n_dev = nullptr);
 void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
                          hipStream_t st);
 // Lane order of the packed classes: key = class << kLaneSpanBits | (kLaneSpanMax

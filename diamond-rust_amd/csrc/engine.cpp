@@ -565,55 +565,67 @@ std::vector<std::vector<HspRec>> Engine::search_targets(
   stage_event(0);
 
   // ---- query seeds (all shapes, tagged, sorted) ---------------------------
+  // Counts stay on the device up to the join's hit total, the one host read
+  // before the hits (it sizes their buffer): the seeds and runs are sized by
+  // their bound (every letter of every shape seeds) and the kernels read the
+  // counts where they need them.
+  uint64_t* dc = counts_dev_.ensure<uint64_t>(kCountSlots);
   uint32_t nb = seed_enum_blocks(0, q_total_);
   uint64_t nbk = (uint64_t)nb * idx_->nshapes + 1;
   uint32_t* bc = b_counts_.ensure<uint32_t>(nbk);
   launch_fill_u32(bc + nbk - 1, 1, 0, st_);
   uint64_t* bo = b_off_.ensure<uint64_t>(nbk);
-  uint64_t cap = q_total_ * idx_->nshapes + 1;
-  uint64_t* qk = qkeys_.ensure<uint64_t>(cap);
-  uint32_t* ql = qlocs_.ensure<uint32_t>(cap);
+  const uint64_t nQb = q_total_ * idx_->nshapes;  // bound of the query seeds
+  uint64_t* qk = qkeys_.ensure<uint64_t>(nQb + 1);
+  uint32_t* ql = qlocs_.ensure<uint32_t>(nQb + 1);
   launch_seed_enumerate(q_res_.as<uint8_t>(), q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), 0,
                         q_total_, idx_->seed_params.as<SeedParams>(), idx_->nshapes, idx_->key_bits, bc, bo,
                         b_scan_.ensure<uint64_t>(scan_tmp_elems(nbk)), qk, ql, st_);
-  const uint64_t nQ = read1(bo + nbk - 1);
-  S.query_seeds = nQ;
+  launch_copy_u64(bo + nbk - 1, dc + kCountSeeds, 1, st_);
+  const uint64_t* nQd = dc + kCountSeeds;
   {
-    uint64_t* k2 = qkeys2_.ensure<uint64_t>(nQ + 1);
-    uint32_t* l2 = qlocs2_.ensure<uint32_t>(nQ + 1);
-    radix_sort_pairs(qk, ql, k2, l2, nQ, total_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(nQ)),
-                     b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nQ)), st_);
+    uint64_t* k2 = qkeys2_.ensure<uint64_t>(nQb + 1);
+    uint32_t* l2 = qlocs2_.ensure<uint32_t>(nQb + 1);
+    radix_sort_pairs(qk, ql, k2, l2, nQb, total_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(nQb)),
+                     b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nQb)), st_, nQd);
   }
   stage_event(1);
 
   // ---- join: runs of equal query keys x DB chunks --------------------------
   // nq counts a key's entries over the whole batch (src/search.rs:109-131),
   // or over one record when opt.per_record (a record searched alone).
-  uint32_t* fl = flags_.ensure<uint32_t>(nQ + 1);
+  uint32_t* fl = flags_.ensure<uint32_t>(nQb + 1);
   if (opt.per_record)
-    launch_run_flags_grouped(qk, ql, q_seq_of_.as<uint32_t>(), q_group_.as<uint32_t>(), nQ, fl, st_);
+    launch_run_flags_grouped(qk, ql, q_seq_of_.as<uint32_t>(), q_group_.as<uint32_t>(), nQb, fl, st_, nQd);
   else
-    launch_run_flags(qk, nQ, fl, st_);
-  launch_fill_u32(fl + nQ, 1, 0, st_);
-  uint64_t* ps = pos_.ensure<uint64_t>(nQ + 1);
-  scan_exclusive_u32_to_u64(fl, ps, nQ + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nQ + 1)), st_);
-  const uint64_t nruns = read1(ps + nQ);
+    launch_run_flags(qk, nQb, fl, st_, nQd);
+  launch_fill_u32(fl + nQb, 1, 0, st_);
+  uint64_t* ps = pos_.ensure<uint64_t>(nQb + 1);
+  scan_exclusive_u32_to_u64(fl, ps, nQb + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nQb + 1)), st_);
+  launch_copy_u64(ps + nQb, dc + kCountRuns, 1, st_);
+  const uint64_t* nrd = dc + kCountRuns;
   // Query runs stay in their own buffer: every slice reads them.
-  uint32_t* rs = qruns_.ensure<uint32_t>(nruns + 1);
-  launch_compact_flags(fl, ps, nQ, rs, st_);
-  S.runs = nruns;
+  uint32_t* rs = qruns_.ensure<uint32_t>(nQb + 1);
+  launch_compact_flags(fl, ps, nQb, rs, st_);
   const uint32_t nch = (uint32_t)idx_->chunks.size();
-  const uint64_t npairs = nruns * nch;
+  const uint64_t npairs = nQb * nch;  // bound of the (run, chunk) pairs
   uint32_t* cnt = jc_counts_.ensure<uint32_t>(npairs + 1);
   uint64_t* rlo = jc_rlo_.ensure<uint64_t>(npairs + 1);
-  launch_join_count(qk, rs, nruns, nQ, idx_->chunk_desc.as<RefChunkDev>(), nch, opt.max_expand, cnt, rlo, st_);
+  launch_join_count(qk, rs, nQb, nQb, idx_->chunk_desc.as<RefChunkDev>(), nch, opt.max_expand, cnt, rlo, st_,
+                    nrd, nQd);
   launch_fill_u32(cnt + npairs, 1, 0, st_);
   // All hits at once when they fit the slice budget; else slices of targets
   // by their hit counts (bounded device memory; the cap decisions above are
   // the batch's, so slicing changes no result).
   uint64_t* hoff = jc_off_.ensure<uint64_t>(npairs + 1);
   scan_exclusive_u32_to_u64(cnt, hoff, npairs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
-  const uint64_t all_hits = read1(hoff + npairs);
+  launch_copy_u64(hoff + npairs, dc + kCountHits, 1, st_);
+  uint64_t* hc = counts_pin_.ensure<uint64_t>(kCountSlots);
+  DMR_HIP(hipMemcpyAsync(hc, dc, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+  DMR_HIP(hipStreamSynchronize(st_));
+  const uint64_t nQ = hc[kCountSeeds], nruns = hc[kCountRuns], all_hits = hc[kCountHits];
+  S.query_seeds = nQ;
+  S.runs = nruns;
   const uint64_t budget = opt.slice_hits ? opt.slice_hits : slice_hits_;
   std::vector<std::pair<uint32_t, uint32_t>> slices;
   std::vector<uint64_t> slice_total;
@@ -623,7 +635,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(
   } else {
     uint64_t* thits = thits_.ensure<uint64_t>(q_n_ + 1);
     launch_fill_u32(reinterpret_cast<uint32_t*>(thits), 2ull * q_n_, 0, st_);
-    launch_target_hits(ql, rs, nruns, nQ, nch, cnt, q_seq_of_.as<uint32_t>(), thits, st_);
+    launch_target_hits(ql, rs, nQb, nQb, nch, cnt, q_seq_of_.as<uint32_t>(), thits, st_, nrd, nQd);
     uint64_t* h_thits = thits_pin_.ensure<uint64_t>(q_n_ + 1);
     DMR_HIP(hipMemcpyAsync(h_thits, thits, q_n_ * 8ull, hipMemcpyDeviceToHost, st_));
     DMR_HIP(hipStreamSynchronize(st_));
@@ -672,14 +684,15 @@ std::vector<std::vector<HspRec>> Engine::search_targets(
     uint32_t* sst = whole ? nullptr : jc_sst_.ensure<uint32_t>(npairs + 1);
     uint32_t* snq = whole ? nullptr : jc_snq_.ensure<uint32_t>(npairs + 1);
     if (!whole) {
-      launch_join_slice(ql, rs, nruns, nQ, nch, cnt, q_seq_of_.as<uint32_t>(), t0, t1, scnt, sst, snq, st_);
+      launch_join_slice(ql, rs, nQb, nQb, nch, cnt, q_seq_of_.as<uint32_t>(), t0, t1, scnt, sst, snq, st_, nrd,
+                        nQd);
       launch_fill_u32(scnt + npairs, 1, 0, st_);
       scan_exclusive_u32_to_u64(scnt, hoff, npairs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
       if (read1(hoff + npairs) != nhits) throw Error(-2, "join offsets disagree with the per-target hit counts");
     }
-    launch_join_emit(ql, rs, nruns, nQ, idx_->chunk_desc.as<RefChunkDev>(), nch, scnt, rlo, hoff,
+    launch_join_emit(ql, rs, nQb, nQb, idx_->chunk_desc.as<RefChunkDev>(), nch, scnt, rlo, hoff,
                      q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), db_->seq_of.as<uint32_t>(),
-                     db_->offs.as<uint32_t>(), g, hits, st_, sst, snq);
+                     db_->offs.as<uint32_t>(), g, hits, st_, sst, snq, nrd, nQd);
     stage_event(11);
     radix_sort_keys(hits, hits2_.ensure<uint64_t>(nhits + 1), nhits, hit_bits,
                     b_hist_.ensure<uint32_t>(radix_hist_elems(nhits)),
@@ -707,58 +720,66 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   const int xdrop = raw_score(P, 12.3);  // --xdrop 12.3 bits (src/search.rs:51-53)
   const int cutoff = 19;                  // src/search.rs:164
   // ---- ungapped: runs of (target, subject, diagonal) -----------------------
+  // Every later count (hit runs, segments, groups, jobs) is at most the
+  // hits': buffers are sized by nhits and the counts stay on the device until
+  // the score pass's host read.
+  uint64_t* dc = counts_dev_.ensure<uint64_t>(kCountSlots);
   uint32_t* fl = flags_.ensure<uint32_t>(nhits + 1);
   launch_key_flags(hits, nhits, (int)g.bp, fl, st_);
   launch_fill_u32(fl + nhits, 1, 0, st_);
   uint64_t* ps = pos_.ensure<uint64_t>(nhits + 1);
   scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
-  const uint64_t nhr = read1(ps + nhits);
-  S.hit_runs += nhr;
-  uint32_t* hrs = runs_.ensure<uint32_t>(nhr + 1);
+  launch_copy_u64(ps + nhits, dc + kCountHitRuns, 1, st_);
+  uint32_t* hrs = runs_.ensure<uint32_t>(nhits + 1);
   launch_compact_flags(fl, ps, nhits, hrs, st_);
   SegRec* sraw = segs_raw_.ensure<SegRec>(nhits + 1);
   uint32_t* keep = keep_.ensure<uint32_t>(nhits + 1);
-  launch_xdrop_runs(hits, hrs, nhr, nhits, g, qs, dbs, matrix32_.as<int8_t>(), xdrop, cutoff, sraw,
-                    keep, st_);
+  launch_xdrop_runs(hits, hrs, nhits, nhits, g, qs, dbs, matrix32_.as<int8_t>(), xdrop, cutoff, sraw, keep, st_,
+                    dc + kCountHitRuns);
   launch_fill_u32(keep + nhits, 1, 0, st_);
-  ps = pos_.ensure<uint64_t>(nhits + 1);
   scan_exclusive_u32_to_u64(keep, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
-  const uint64_t nsegs = read1(ps + nhits);
-  S.segments += nsegs;
-  SegRec* segs = segs_.ensure<SegRec>(nsegs + 1);
-  uint64_t* gk = gkeys_.ensure<uint64_t>(nsegs + 1);
+  launch_copy_u64(ps + nhits, dc + kCountSegs, 1, st_);
+  SegRec* segs = segs_.ensure<SegRec>(nhits + 1);
+  uint64_t* gk = gkeys_.ensure<uint64_t>(nhits + 1);
   launch_compact_segments(sraw, hits, keep, ps, nhits, (int)(g.bd + g.bp), segs, gk, st_);
   stage_event(4);
 
   // ---- bands per (target, subject) -----------------------------------------
-  fl = flags_.ensure<uint32_t>(nsegs + 1);
-  launch_key_flags(gk, nsegs, 0, fl, st_);
-  launch_fill_u32(fl + nsegs, 1, 0, st_);
-  ps = pos_.ensure<uint64_t>(nsegs + 1);
-  scan_exclusive_u32_to_u64(fl, ps, nsegs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nsegs + 1)), st_);
-  const uint64_t ngroups = read1(ps + nsegs);
-  S.groups += ngroups;
-  uint32_t* gs = gstart_.ensure<uint32_t>(ngroups + 1);
-  launch_compact_flags(fl, ps, nsegs, gs, st_);
-  uint32_t* nbd = nbands_.ensure<uint32_t>(ngroups + 1);
-  SwJob* tj = tmpjobs_.ensure<SwJob>(nsegs + 1);
-  launch_bands(segs, gk, gs, ngroups, nsegs, g.bs, qs, dbs, bd0_.ensure<int32_t>(nsegs + 1),
-               bd1_.ensure<int32_t>(nsegs + 1), bbest_.ensure<uint32_t>(nsegs + 1), tj, nbd, st_);
-  launch_fill_u32(nbd + ngroups, 1, 0, st_);
-  uint64_t* jo = joboff_.ensure<uint64_t>(ngroups + 1);
-  scan_exclusive_u32_to_u64(nbd, jo, ngroups + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(ngroups + 1)), st_);
-  const uint64_t njobs = read1(jo + ngroups);
-  S.jobs += njobs;
-  SwJob* jobs = jobs_.ensure<SwJob>(njobs + 1);
-  launch_jobs_compact(tj, gs, nbd, jo, ngroups, jobs, st_);
+  launch_key_flags(gk, nhits, 0, fl, st_, dc + kCountSegs);
+  launch_fill_u32(fl + nhits, 1, 0, st_);
+  scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+  launch_copy_u64(ps + nhits, dc + kCountGroups, 1, st_);
+  uint32_t* gs = gstart_.ensure<uint32_t>(nhits + 1);
+  launch_compact_flags(fl, ps, nhits, gs, st_);
+  uint32_t* nbd = nbands_.ensure<uint32_t>(nhits + 1);
+  SwJob* tj = tmpjobs_.ensure<SwJob>(nhits + 1);
+  launch_bands(segs, gk, gs, nhits, nhits, g.bs, qs, dbs, bd0_.ensure<int32_t>(nhits + 1),
+               bd1_.ensure<int32_t>(nhits + 1), bbest_.ensure<uint32_t>(nhits + 1), tj, nbd, st_,
+               dc + kCountGroups, dc + kCountSegs);
+  launch_fill_u32(nbd + nhits, 1, 0, st_);
+  uint64_t* jo = joboff_.ensure<uint64_t>(nhits + 1);
+  scan_exclusive_u32_to_u64(nbd, jo, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+  launch_copy_u64(jo + nhits, dc + kCountJobs, 1, st_);
+  SwJob* jobs = jobs_.ensure<SwJob>(nhits + 1);
+  launch_jobs_compact(tj, gs, nbd, jo, nhits, jobs, st_, dc + kCountGroups);
+  uint64_t* hc = counts_pin_.ensure<uint64_t>(kCountSlots);
+  DMR_HIP(hipMemcpyAsync(hc + kCountHitRuns, dc + kCountHitRuns, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         st_));
   stage_event(5);
 
   // ---- banded DP, score pass -----------------------------------------------
-  SwOut* outs = outs_.ensure<SwOut>(njobs + 1);
+  SwOut* outs = outs_.ensure<SwOut>(nhits + 1);
   stage_event(6);
   StageStats SS;
-  score_pass(jobs, njobs, (int)g.bt, qs, dbs, opt, outs, &SS);
+  const uint64_t njobs = score_pass(jobs, nhits, (int)g.bt, qs, dbs, opt, outs, &SS, dc + kCountJobs);
+  if (!nhits) DMR_HIP(hipStreamSynchronize(st_));
   stage_event(7);
+  const uint64_t nsegs = hc[kCountSegs];
+  S.hit_runs += hc[kCountHitRuns];
+  S.segments += nsegs;
+  S.groups += hc[kCountGroups];
+  S.jobs += njobs;
+  if (hc[kCountJobs] != njobs) throw Error(-2, "score pass job count disagrees with the bands'");
   S.sw16_lanes += SS.sw16_lanes;
   S.sw_generic += SS.sw_generic;
   S.sw_redo += SS.sw_redo;
@@ -934,10 +955,10 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
 // packed int16 kernel (two DPs of one query per lane), the rest (merged or
 // clipped bands, other gap costs) and any int16-saturated DP to the exact
 // int32 kernel.
-void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
-                        const SearchOptions& opt, SwOut* outs, StageStats* S) {
+uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
+                            const SearchOptions& opt, SwOut* outs, StageStats* S, const uint64_t* njobs_dev) {
   for (int c = 0; c < kClasses; ++c) S->ms_class[c] = 0;
-  if (!njobs) return;
+  if (!njobs) return 0;
   last_target_bits_ = target_bits;
   layout_ = resolve_layout(opt.sensitivity);
   layout_preset_ = opt.sensitivity;
@@ -948,18 +969,19 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     throw Error(-2, "score pass without padded letters");
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
-  launch_classify(jobs, njobs, qs, dbs, target_bits, mmap, keys, idx, st_);
+  // With njobs_dev, njobs bounds the count until the summary read below.
+  launch_classify(jobs, njobs, qs, dbs, target_bits, mmap, keys, idx, st_, njobs_dev);
   static_assert(kClasses <= 32, "the class takes 5 key bits");
   const int key_bits = 5 + target_bits + kRowKeyBits;
   radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
                    njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
-                   b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_);
+                   b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_, njobs_dev);
   // Runs of equal (class, target) and their starts; lane heads (even
   // positions of a run in a packed class), their scan, the lanes and the
   // per-class summary, all on the device; one small read brings the class
   // ranges and lane counts back.
   uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
-  launch_key_flags(keys, njobs, kRowKeyBits, fl, st_);
+  launch_key_flags(keys, njobs, kRowKeyBits, fl, st_, njobs_dev);
   launch_fill_u32(fl + njobs, 1, 0, st_);
   uint64_t* ps = pos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
@@ -971,16 +993,23 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
   uint32_t* summ = sw_summary_.ensure<uint32_t>(kSummary);
   launch_fill_u32(summ, kSummary, 0, st_);
   uint32_t* heads = sw_heads_.ensure<uint32_t>(njobs + 1);
-  launch_lane_heads(keys, fl, ps, rs, njobs, cls_shift, npacked, kClasses, heads, summ, st_);
+  launch_lane_heads(keys, fl, ps, rs, njobs, cls_shift, npacked, kClasses, heads, summ, st_, njobs_dev);
   launch_fill_u32(heads + njobs, 1, 0, st_);
   uint64_t* hpos = sw_hpos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(heads, hpos, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
   uint2* paired = sw_lanes2_.ensure<uint2>(njobs + 1);
-  launch_make_lanes(idx, fl, heads, hpos, njobs, paired, st_);
+  launch_make_lanes(idx, fl, heads, hpos, njobs, paired, st_, njobs_dev);
   launch_lane_summary(hpos, njobs, npacked, kClasses, summ, st_);
   uint32_t* h_summ = sw_pin_summary_.ensure<uint32_t>(kSummary);
   DMR_HIP(hipMemcpyAsync(h_summ, summ, kSummary * 4, hipMemcpyDeviceToHost, st_));
+  uint64_t* h_njobs = sw_pin_count_.ensure<uint64_t>(1);
+  if (njobs_dev) DMR_HIP(hipMemcpyAsync(h_njobs, njobs_dev, 8, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
+  if (njobs_dev) {
+    if (*h_njobs > njobs) throw Error(-2, "score pass: job count past its bound");
+    njobs = *h_njobs;
+    if (!njobs) return 0;
+  }
   // Class ranges in the sorted order; lanes of the packed classes.
   uint64_t cbeg[kClasses], cend[kClasses];
   for (int c = 0; c < kClasses; ++c) {
@@ -1152,6 +1181,7 @@ void Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqS
     DMR_HIP(hipEventSynchronize(ev_[13 + 2 * kRowExact]));
     S->ms_class[kRowExact] = wide_ms + stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
   }
+  return njobs;
 }
 
 // Diagnostics (DMR_LANE_REPORT=1): per packed class, the band cells of its

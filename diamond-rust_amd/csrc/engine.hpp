@@ -359,8 +359,11 @@ class Engine {
   void upload_seed_params(const std::string& sens, double cut, DevBuf* params, uint32_t* nshapes, int* key_bits,
                           int* shape_bits);
   bool swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, bool fast) const;
-  void score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
-                  const SearchOptions& opt, SwOut* outs, StageStats* S);
+  // Score pass over the jobs; njobs_dev: njobs is a bound and the count is
+  // read on the device (the pass's one host read brings it back). Returns the
+  // job count.
+  uint64_t score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
+                      const SearchOptions& opt, SwOut* outs, StageStats* S, const uint64_t* njobs_dev = nullptr);
   void search_slice(const SearchOptions& opt, const AlpParams& P, const HitGeom& g, const uint64_t* hits,
                     uint64_t nhits, SeqSet qs, SeqSet dbs, StageStats* S,
                     std::vector<std::vector<HspRec>>* result,
@@ -439,6 +442,12 @@ class Engine {
   DevBuf tb_out_all_, tb_ops_all_, tb_poff_all_, tb_pack_all_;
   DevBuf bk_list_, bk_toff_, bk_ooff_, bk_trace_, bk_ops_, bk_out_, bk_fb_, bk_cnt_;
   PinBuf sel_pin_;  // selection candidates (index, job, score-pass output)
+  // Stage counts of a search kept on the device (kCount* slots) and their
+  // pinned copies, read at the chain's host syncs.
+  static constexpr int kCountSeeds = 0, kCountRuns = 1, kCountHits = 2, kCountHitRuns = 3, kCountSegs = 4,
+                       kCountGroups = 5, kCountJobs = 6, kCountSlots = 8;
+  DevBuf counts_dev_;
+  PinBuf counts_pin_, sw_pin_count_;
   // Host copies of each group's job list and offsets; they stay alive until
   // the copies queued from them have run.
   std::vector<uint64_t> last_cbeg_, last_cend_;

@@ -96,18 +96,33 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Device counts: a kernel given n_dev runs over its host count as a bound
+// (the grid) and takes *n_dev as the count; its flags and counts past the
+// count are 0, so a scan over the bound totals the same.
+__device__ __forceinline__ uint64_t dcount(uint64_t bound, const uint64_t* n_dev) {
+  return n_dev ? *n_dev : bound;
+}
+
 __global__ void k_key_flags(const uint64_t* __restrict__ keys, uint64_t n, int shift,
-                            uint32_t* __restrict__ flags) {
+                            uint32_t* __restrict__ flags, const uint64_t* __restrict__ n_dev) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  flags[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
+  flags[i] = i < dcount(n, n_dev) && (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
+}
+
+// dst[k] = src[k] for k < n (counts kept past the buffers they came from).
+__global__ void k_copy_u64(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint32_t n) {
+  if (threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
 }
 
 __global__ void k_xdrop_runs(const uint64_t* __restrict__ hits, const uint32_t* __restrict__ run_start,
                              uint64_t nruns, uint64_t nhits, HitGeom g, SeqSet q, SeqSet db,
                              const int8_t* __restrict__ m32, int xdrop, int cutoff,
-                             SegRec* __restrict__ segs, uint32_t* __restrict__ keep) {
+                             SegRec* __restrict__ segs, uint32_t* __restrict__ keep,
+                             const uint64_t* __restrict__ nruns_dev) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
+  nruns = dcount(nruns, nruns_dev);
+  if ((uint64_t)blockIdx.x * blockDim.x >= nruns) return;  // the whole block lies past the runs
   load_matrix(m32, M);
   uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nruns) return;
@@ -165,9 +180,16 @@ __global__ void k_bands(const SegRec* __restrict__ segs, const uint64_t* __restr
                         const uint32_t* __restrict__ group_start, uint64_t ngroups, uint64_t nsegs,
                         uint32_t bs, SeqSet q, SeqSet db, int32_t* __restrict__ bd0,
                         int32_t* __restrict__ bd1, uint32_t* __restrict__ bbest,
-                        SwJob* __restrict__ tmp_jobs, uint32_t* __restrict__ nbands) {
+                        SwJob* __restrict__ tmp_jobs, uint32_t* __restrict__ nbands,
+                        const uint64_t* __restrict__ ngroups_dev, const uint64_t* __restrict__ nsegs_dev) {
   uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= ngroups) return;
+  ngroups = dcount(ngroups, ngroups_dev);
+  nsegs = dcount(nsegs, nsegs_dev);
+  if (gi >= ngroups) {
+    nbands[gi] = 0;
+    return;
+  }
   uint64_t s0 = group_start[gi], s1 = gi + 1 < ngroups ? group_start[gi + 1] : nsegs;
   uint64_t key = gkey[s0];
   uint32_t sj = (uint32_t)(key & ((1ull << bs) - 1));
@@ -235,9 +257,9 @@ __global__ void k_bands(const SegRec* __restrict__ segs, const uint64_t* __restr
 
 __global__ void k_jobs_compact(const SwJob* __restrict__ tmp_jobs, const uint32_t* __restrict__ group_start,
                                const uint32_t* __restrict__ nbands, const uint64_t* __restrict__ job_off,
-                               uint64_t ngroups, SwJob* __restrict__ jobs) {
+                               uint64_t ngroups, SwJob* __restrict__ jobs, const uint64_t* __restrict__ ngroups_dev) {
   uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= ngroups) return;
+  if (gi >= dcount(ngroups, ngroups_dev)) return;
   uint64_t s0 = group_start[gi], o = job_off[gi];
   for (uint32_t r = 0; r < nbands[gi]; ++r) jobs[o + r] = tmp_jobs[s0 + r];
 }
@@ -323,17 +345,24 @@ void launch_xdrop_pairs(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint6
                      m32, xdrop, out);
 }
 
-void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st) {
+void launch_key_flags(const uint64_t* keys, uint64_t n, int shift, uint32_t* flags, hipStream_t st,
+                      const uint64_t* n_dev) {
   if (!n) return;
-  hipLaunchKernelGGL(k_key_flags, dim3(nblk(n)), dim3(256), 0, st, keys, n, shift, flags);
+  hipLaunchKernelGGL(k_key_flags, dim3(nblk(n)), dim3(256), 0, st, keys, n, shift, flags, n_dev);
+}
+
+void launch_copy_u64(const uint64_t* src, uint64_t* dst, uint32_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(64), 0, st, src, dst, n);
 }
 
 void launch_xdrop_runs(const uint64_t* hits, const uint32_t* run_start, uint64_t nruns,
                        uint64_t nhits, HitGeom g, SeqSet q, SeqSet db, const int8_t* matrix32,
-                       int xdrop, int cutoff, SegRec* segs, uint32_t* keep, hipStream_t st) {
+                       int xdrop, int cutoff, SegRec* segs, uint32_t* keep, hipStream_t st,
+                       const uint64_t* nruns_dev) {
   if (!nruns) return;
   hipLaunchKernelGGL(k_xdrop_runs, dim3(nblk(nruns, 128)), dim3(128), 0, st, hits, run_start, nruns,
-                     nhits, g, q, db, matrix32, xdrop, cutoff, segs, keep);
+                     nhits, g, q, db, matrix32, xdrop, cutoff, segs, keep, nruns_dev);
 }
 
 void launch_compact_segments(const SegRec* segs, const uint64_t* hits, const uint32_t* keep,
@@ -347,17 +376,19 @@ void launch_compact_segments(const SegRec* segs, const uint64_t* hits, const uin
 void launch_bands(const SegRec* segs, const uint64_t* gkey, const uint32_t* group_start,
                   uint64_t ngroups, uint64_t nsegs, uint32_t bs, SeqSet q, SeqSet db,
                   int32_t* tmp_d0, int32_t* tmp_d1, uint32_t* tmp_best, SwJob* tmp_jobs,
-                  uint32_t* nbands, hipStream_t st) {
+                  uint32_t* nbands, hipStream_t st, const uint64_t* ngroups_dev, const uint64_t* nsegs_dev) {
   if (!ngroups) return;
   hipLaunchKernelGGL(k_bands, dim3(nblk(ngroups, 128)), dim3(128), 0, st, segs, gkey, group_start,
-                     ngroups, nsegs, bs, q, db, tmp_d0, tmp_d1, tmp_best, tmp_jobs, nbands);
+                     ngroups, nsegs, bs, q, db, tmp_d0, tmp_d1, tmp_best, tmp_jobs, nbands, ngroups_dev,
+                     nsegs_dev);
 }
 
 void launch_jobs_compact(const SwJob* tmp_jobs, const uint32_t* group_start, const uint32_t* nbands,
-                         const uint64_t* job_off, uint64_t ngroups, SwJob* jobs, hipStream_t st) {
+                         const uint64_t* job_off, uint64_t ngroups, SwJob* jobs, hipStream_t st,
+                         const uint64_t* ngroups_dev) {
   if (!ngroups) return;
   hipLaunchKernelGGL(k_jobs_compact, dim3(nblk(ngroups)), dim3(256), 0, st, tmp_jobs, group_start,
-                     nbands, job_off, ngroups, jobs);
+                     nbands, job_off, ngroups, jobs, ngroups_dev);
 }
 
 void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,

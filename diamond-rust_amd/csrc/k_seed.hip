@@ -115,10 +115,18 @@ __global__ void k_bucket_table(const uint64_t* __restrict__ keys, uint64_t n, in
   table[b] = lo;
 }
 
-__global__ void k_run_flags(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flags) {
+// Device counts: a kernel given n_dev runs over n entries (a bound, the
+// grid) and takes *n_dev as the count; flags and counts past it are 0, so a
+// scan over the bound totals the same.
+__device__ __forceinline__ uint64_t dcount(uint64_t bound, const uint64_t* n_dev) {
+  return n_dev ? *n_dev : bound;
+}
+
+__global__ void k_run_flags(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flags,
+                            const uint64_t* __restrict__ n_dev) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+  flags[i] = i < dcount(n, n_dev) && (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
 // Runs of equal keys within one record: a run also ends where the record of
@@ -127,9 +135,13 @@ __global__ void k_run_flags(const uint64_t* __restrict__ keys, uint64_t n, uint3
 // alone, as when the record is searched by itself.
 __global__ void k_run_flags_grouped(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ locs,
                                     const uint32_t* __restrict__ seq_of, const uint32_t* __restrict__ group,
-                                    uint64_t n, uint32_t* __restrict__ flags) {
+                                    uint64_t n, uint32_t* __restrict__ flags, const uint64_t* __restrict__ n_dev) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (i >= dcount(n, n_dev)) {
+    flags[i] = 0;
+    return;
+  }
   flags[i] = (i == 0 || keys[i] != keys[i - 1] || group[seq_of[locs[i]]] != group[seq_of[locs[i - 1]]]) ? 1u
                                                                                                       : 0u;
 }
@@ -168,11 +180,19 @@ __device__ __forceinline__ void ref_range(const uint64_t* __restrict__ rkeys,
 __global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t* __restrict__ run_start,
                              uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
                              uint32_t nchunks, uint64_t max_expand, uint32_t* __restrict__ counts,
-                             uint64_t* __restrict__ rlo) {
+                             uint64_t* __restrict__ rlo, const uint64_t* __restrict__ nruns_dev,
+                             const uint64_t* __restrict__ nq_dev) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nruns * nchunks) return;
   uint64_t r = t / nchunks;
   uint32_t c = (uint32_t)(t - r * nchunks);
+  nruns = dcount(nruns, nruns_dev);
+  nq_total = dcount(nq_total, nq_dev);
+  if (r >= nruns) {
+    counts[t] = 0;
+    rlo[t] = 0;
+    return;
+  }
   uint64_t s0 = run_start[r];
   uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
   uint64_t key = qkeys[s0];
@@ -190,9 +210,12 @@ __global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t*
 __global__ void k_target_hits(const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start,
                               uint64_t nruns, uint64_t nq_total, uint32_t nchunks,
                               const uint32_t* __restrict__ counts, const uint32_t* __restrict__ q_seq_of,
-                              unsigned long long* __restrict__ thits) {
+                              unsigned long long* __restrict__ thits, const uint64_t* __restrict__ nruns_dev,
+                              const uint64_t* __restrict__ nq_dev) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nruns * nchunks) return;
+  nruns = dcount(nruns, nruns_dev);
+  nq_total = dcount(nq_total, nq_dev);
   const uint32_t cnt = counts[t];
   if (cnt == 0) return;
   const uint64_t r = t / nchunks;
@@ -208,13 +231,16 @@ __global__ void k_join_slice(const uint32_t* __restrict__ qlocs, const uint32_t*
                              uint64_t nruns, uint64_t nq_total, uint32_t nchunks,
                              const uint32_t* __restrict__ counts, const uint32_t* __restrict__ q_seq_of,
                              uint32_t t0, uint32_t t1, uint32_t* __restrict__ slice_counts,
-                             uint32_t* __restrict__ slice_start, uint32_t* __restrict__ slice_nq) {
+                             uint32_t* __restrict__ slice_start, uint32_t* __restrict__ slice_nq,
+                             const uint64_t* __restrict__ nruns_dev, const uint64_t* __restrict__ nq_dev) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nruns * nchunks) return;
-  const uint32_t cnt = counts[t];
+  nruns = dcount(nruns, nruns_dev);
+  nq_total = dcount(nq_total, nq_dev);
+  const uint32_t cnt = counts[t];  // 0 past the runs
   const uint64_t r = t / nchunks;
   const uint64_t s0 = run_start[r], s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
-  uint32_t out = 0, e0 = (uint32_t)s0, ne = 0;
+  uint32_t out = 0, e0 = 0, ne = 0;
   if (cnt) {
     const uint32_t nr = cnt / (uint32_t)(s1 - s0);
     auto first_at_least = [&](uint32_t tg) {
@@ -245,10 +271,13 @@ __global__ void k_join_emit(const uint32_t* __restrict__ qlocs, const uint32_t* 
                             const uint32_t* __restrict__ q_seq_of, const uint32_t* __restrict__ q_offs,
                             const uint32_t* __restrict__ db_seq_of, const uint32_t* __restrict__ db_offs,
                             HitGeom g, uint64_t* __restrict__ hits, const uint32_t* __restrict__ slice_start,
-                            const uint32_t* __restrict__ slice_nq) {
+                            const uint32_t* __restrict__ slice_nq, const uint64_t* __restrict__ nruns_dev,
+                            const uint64_t* __restrict__ nq_dev) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nruns * nchunks) return;
-  uint32_t cnt = counts[t];
+  nruns = dcount(nruns, nruns_dev);
+  nq_total = dcount(nq_total, nq_dev);
+  uint32_t cnt = counts[t];  // 0 past the runs
   if (cnt == 0) return;
   uint64_t r = t / nchunks;
   uint32_t c = (uint32_t)(t - r * nchunks);
@@ -353,9 +382,9 @@ void launch_bucket_table(const uint64_t* keys, uint64_t n, int shift, uint64_t n
                      n, shift, nbuckets, table);
 }
 
-void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st) {
+void launch_run_flags(const uint64_t* keys, uint64_t n, uint32_t* flags, hipStream_t st, const uint64_t* n_dev) {
   if (!n) return;
-  hipLaunchKernelGGL(k_run_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, flags);
+  hipLaunchKernelGGL(k_run_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, flags, n_dev);
 }
 
 void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n, uint32_t* out,
@@ -367,11 +396,12 @@ void launch_compact_flags(const uint32_t* flags, const uint64_t* pos, uint64_t n
 
 void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_t nruns,
                        uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                       uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st) {
+                       uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st,
+                       const uint64_t* nruns_dev, const uint64_t* nq_dev) {
   uint64_t n = nruns * nchunks;
   if (!n) return;
   hipLaunchKernelGGL(k_join_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qkeys,
-                     run_start, nruns, nq_total, chunks, nchunks, max_expand, counts, rlo);
+                     run_start, nruns, nq_total, chunks, nchunks, max_expand, counts, rlo, nruns_dev, nq_dev);
 }
 
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
@@ -379,38 +409,42 @@ void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t
                       const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off,
                       const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
                       const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st,
-                      const uint32_t* slice_start, const uint32_t* slice_nq) {
+                      const uint32_t* slice_start, const uint32_t* slice_nq, const uint64_t* nruns_dev,
+                      const uint64_t* nq_dev) {
   uint64_t n = nruns * nchunks;
   if (!n) return;
   hipLaunchKernelGGL(k_join_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs,
                      run_start, nruns, nq_total, chunks, nchunks, counts, rlo, hit_off, q_seq_of,
-                     q_offs, db_seq_of, db_offs, g, hits, slice_start, slice_nq);
+                     q_offs, db_seq_of, db_offs, g, hits, slice_start, slice_nq, nruns_dev, nq_dev);
 }
 
 void launch_run_flags_grouped(const uint64_t* keys, const uint32_t* locs, const uint32_t* seq_of,
-                              const uint32_t* group, uint64_t n, uint32_t* flags, hipStream_t st) {
+                              const uint32_t* group, uint64_t n, uint32_t* flags, hipStream_t st,
+                              const uint64_t* n_dev) {
   if (!n) return;
   hipLaunchKernelGGL(k_run_flags_grouped, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, locs,
-                     seq_of, group, n, flags);
+                     seq_of, group, n, flags, n_dev);
 }
 
 void launch_target_hits(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
                         uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint64_t* thits,
-                        hipStream_t st) {
+                        hipStream_t st, const uint64_t* nruns_dev, const uint64_t* nq_dev) {
   uint64_t n = nruns * nchunks;
   if (!n) return;
   hipLaunchKernelGGL(k_target_hits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs, run_start,
-                     nruns, nq_total, nchunks, counts, q_seq_of, reinterpret_cast<unsigned long long*>(thits));
+                     nruns, nq_total, nchunks, counts, q_seq_of, reinterpret_cast<unsigned long long*>(thits),
+                     nruns_dev, nq_dev);
 }
 
 void launch_join_slice(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns, uint64_t nq_total,
                        uint32_t nchunks, const uint32_t* counts, const uint32_t* q_seq_of, uint32_t t0,
                        uint32_t t1, uint32_t* slice_counts, uint32_t* slice_start, uint32_t* slice_nq,
-                       hipStream_t st) {
+                       hipStream_t st, const uint64_t* nruns_dev, const uint64_t* nq_dev) {
   uint64_t n = nruns * nchunks;
   if (!n) return;
   hipLaunchKernelGGL(k_join_slice, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs, run_start, nruns,
-                     nq_total, nchunks, counts, q_seq_of, t0, t1, slice_counts, slice_start, slice_nq);
+                     nq_total, nchunks, counts, q_seq_of, t0, t1, slice_counts, slice_start, slice_nq, nruns_dev,
+                     nq_dev);
 }
 
 }  // namespace dmr

@@ -125,8 +125,9 @@ void scan_impl(const T* in, uint64_t* out, uint64_t n, uint64_t* tmp, hipStream_
 // ---- radix sort -----------------------------------------------------------
 
 __global__ void k_radix_hist(const uint64_t* keys, uint64_t n, int shift, uint32_t* hist,
-                             uint32_t nblocks, uint32_t* totals) {
+                             uint32_t nblocks, uint32_t* totals, const uint64_t* n_dev) {
   __shared__ uint32_t h[256];
+  if (n_dev) n = *n_dev;  // a device count within the bound n (tiles past it count nothing)
   h[threadIdx.x] = 0;
   __syncthreads();
   uint64_t base = (uint64_t)blockIdx.x * kSortTile;
@@ -183,7 +184,7 @@ template <bool kValues>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin,
                                                                 uint64_t* kout, uint32_t* vout, uint64_t n,
                                                                 int shift, const uint64_t* offs,
-                                                                uint32_t nblocks) {
+                                                                uint32_t nblocks, const uint64_t* n_dev) {
   __shared__ uint32_t wave_hist[kSortWaves][256];
   __shared__ uint32_t count[256];       // keys of each digit ranked so far
   __shared__ uint32_t local_base[256];  // first LDS slot of each digit
@@ -191,6 +192,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
   __shared__ uint64_t skey[kSortTile];
   __shared__ uint32_t sval[kValues ? kSortTile : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (n_dev) n = *n_dev;
+  if ((uint64_t)blockIdx.x * kSortTile >= n) return;  // a tile past the device count
   run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
   count[threadIdx.x] = 0;
 #pragma unroll
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
 
 template <bool kValues>
 void radix_impl(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n, int bits,
-                uint32_t* hist, uint64_t* scan_tmp, hipStream_t st) {
+                uint32_t* hist, uint64_t* scan_tmp, hipStream_t st, const uint64_t* n_dev) {
   if (n <= 1 || bits <= 0) return;
   uint32_t nb = (uint32_t)((n + kSortTile - 1) / kSortTile);
   uint64_t* offs = scan_tmp;                                        // 256 * nb
@@ -269,10 +272,10 @@ void radix_impl(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t
   for (int p = 0; p < passes; ++p) {
     int shift = 8 * p;
     uint32_t* pt = totals + 256 * kTotalStripes * p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist, nb, pt);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist, nb, pt, n_dev);
     hipLaunchKernelGGL(k_radix_offsets, dim3(256), dim3(kScanThreads), 0, st, hist, nb, pt, offs);
     hipLaunchKernelGGL(k_radix_scatter<kValues>, dim3(nb), dim3(kSortThreads), 0, st, ka, va, kb,
-                       vb, n, shift, offs, nb);
+                       vb, n, shift, offs, nb, n_dev);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
@@ -313,13 +316,13 @@ void scan_exclusive_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, ui
 }
 
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t n, int bits,
-                      uint32_t* hist, uint64_t* scan_tmp, hipStream_t st) {
-  radix_impl<true>(k0, v0, k1, v1, n, bits, hist, scan_tmp, st);
+                      uint32_t* hist, uint64_t* scan_tmp, hipStream_t st, const uint64_t* n_dev) {
+  radix_impl<true>(k0, v0, k1, v1, n, bits, hist, scan_tmp, st, n_dev);
 }
 
 void radix_sort_keys(uint64_t* k0, uint64_t* k1, uint64_t n, int bits, uint32_t* hist,
-                     uint64_t* scan_tmp, hipStream_t st) {
-  radix_impl<false>(k0, nullptr, k1, nullptr, n, bits, hist, scan_tmp, st);
+                     uint64_t* scan_tmp, hipStream_t st, const uint64_t* n_dev) {
+  radix_impl<false>(k0, nullptr, k1, nullptr, n, bits, hist, scan_tmp, st, n_dev);
 }
 
 }  // namespace dmr

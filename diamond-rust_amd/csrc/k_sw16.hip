@@ -33,9 +33,9 @@ __device__ __forceinline__ int packed_class(int w, uint32_t merged) {
 // rows as its longest lane, so neighbours should start and end alike.
 __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSet q, SeqSet db,
                            int target_bits, uint32_t merged, uint64_t* __restrict__ keys,
-                           uint32_t* __restrict__ idx) {
+                           uint32_t* __restrict__ idx, const uint64_t* __restrict__ njobs_dev) {
   uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= njobs) return;
+  if (k >= (njobs_dev ? *njobs_dev : njobs)) return;
   const SwJob j = jobs[k];
   const int64_t qlen = q.offs[j.target + 1] - q.offs[j.target];
   const int64_t m = db.offs[j.subject + 1] - db.offs[j.subject];
@@ -70,9 +70,17 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
 __global__ void k_lane_heads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
                              const uint64_t* __restrict__ pos, const uint32_t* __restrict__ run_start,
                              uint64_t n, int cls_shift, int npacked, int nclasses,
-                             uint32_t* __restrict__ heads, uint32_t* __restrict__ summary) {
+                             uint32_t* __restrict__ heads, uint32_t* __restrict__ summary,
+                             const uint64_t* __restrict__ n_dev) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (n_dev) {  // a device count within the bound n: no heads past it
+    n = *n_dev;
+    if (i >= n) {
+      heads[i] = 0;
+      return;
+    }
+  }
   auto cls_of = [&](uint64_t k) { return (int)min<uint64_t>(keys[k] >> cls_shift, (uint64_t)(nclasses - 1)); };
   const int c = cls_of(i);
   const uint64_t r = pos[i] + flags[i] - 1;
@@ -85,9 +93,10 @@ __global__ void k_lane_heads(const uint64_t* __restrict__ keys, const uint32_t* 
 // its run (none at the run's end) into lane hpos[i].
 __global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint32_t* __restrict__ flags,
                              const uint32_t* __restrict__ heads, const uint64_t* __restrict__ hpos, uint64_t n,
-                             uint2* __restrict__ lanes) {
+                             uint2* __restrict__ lanes, const uint64_t* __restrict__ n_dev) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !heads[i]) return;
+  if (n_dev) n = *n_dev;  // the job after the last is no partner
   lanes[hpos[i]] = make_uint2(sorted_idx[i], i + 1 < n && flags[i + 1] == 0 ? sorted_idx[i + 1] : ~0u);
 }
 
@@ -182,25 +191,26 @@ void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out)
 }
 
 void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits, uint32_t merged_map,
-                     uint64_t* keys, uint32_t* idx, hipStream_t st) {
+                     uint64_t* keys, uint32_t* idx, hipStream_t st, const uint64_t* njobs_dev) {
   if (!njobs) return;
   hipLaunchKernelGGL(k_classify, dim3(nblk(njobs, 256)), dim3(256), 0, st, jobs, njobs, q, db,
-                     target_bits, merged_map, keys, idx);
+                     target_bits, merged_map, keys, idx, njobs_dev);
 }
 
 void launch_lane_heads(const uint64_t* keys, const uint32_t* flags, const uint64_t* pos, const uint32_t* run_start,
                        uint64_t n, int cls_shift, int npacked, int nclasses, uint32_t* heads, uint32_t* summary,
-                       hipStream_t st) {
+                       hipStream_t st, const uint64_t* n_dev) {
   if (!n) return;
   hipLaunchKernelGGL(k_lane_heads, dim3(nblk(n, 256)), dim3(256), 0, st, keys, flags, pos, run_start, n,
-                     cls_shift, npacked, nclasses, heads, summary);
+                     cls_shift, npacked, nclasses, heads, summary, n_dev);
 }
 
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const uint32_t* heads,
-                       const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st) {
+                       const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st,
+                       const uint64_t* n_dev) {
   if (!n) return;
   hipLaunchKernelGGL(k_make_lanes, dim3(nblk(n, 256)), dim3(256), 0, st, sorted_idx, flags, heads, hpos, n,
-                     lanes);
+                     lanes, n_dev);
 }
 
 void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
