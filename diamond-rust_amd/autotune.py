@@ -1,7 +1,8 @@
 """Autotuner of the score-pass kernels: picks, on the device, the packed
 kernel of every width class (lanes per DP pair and threads per block among
 the compiled ones), the merged-band width buckets in use, the arithmetic
-(f16 or int16 pairs) and the stream plan, for a search preset on a workload,
+(f16 or int16 pairs), the stream plan and whether the chain before the
+score pass runs as one hipGraph, for a search preset on a workload,
 and writes the choice with every trial's timing to the tuning file that the
 library loads at context creation (tuning_<arch>.json beside libdmr.so).
 
@@ -11,7 +12,7 @@ equal the first one's, or the tuner stops.
 Search: coordinate descent from the built-in defaults. Each step tries the
 values of one setting with the others held, and keeps a value only when its
 median search time beats the best so far by more than --min-gain (noise
-guard): f16; the merged-band kernels in use (one bucket bound added or
+guard): f16; the graph; the merged-band kernels in use (one bucket bound added or
 removed at a time, until no change helps); the kernel of each class that
 has DPs in the workload; the stream plan; the merged buckets once more.
 
@@ -102,6 +103,7 @@ class Tuner:
             return won
 
         consider({**cur, "f16": 1 - cur["f16"]}, {"f16": 1 - cur["f16"]})
+        consider({**cur, "graph": 1 - cur["graph"]}, {"graph": 1 - cur["graph"]})
 
         def merged_pass():
             changed = True

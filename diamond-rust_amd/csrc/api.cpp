@@ -404,6 +404,8 @@ int dmr_ctx_get_tuning(const dmr_ctx* ctx, const char* key, double* value) {
     if (k == "slice_hits") *value = (double)e.slice_hits();
     else if (k == "last_slices") *value = e.last_slices();
     else if (k == "tuned_presets") *value = (double)e.tuned_presets();
+    else if (k == "graph_launches") *value = (double)e.graph_launches();
+    else if (k == "graph_builds") *value = (double)e.graph_builds();
     else *value = e.layout_key(k);
     return DMR_OK;
   });

@@ -208,6 +208,7 @@ Engine::~Engine() {
   if (fork_ev_) (void)hipEventDestroy(fork_ev_);
   for (auto& s : aux_)
     if (s) (void)hipStreamDestroy(s);
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -231,7 +232,41 @@ T Engine::read1(const T* dptr) {
   return v;
 }
 
-void Engine::stage_event(int k) { DMR_HIP(hipEventRecord(ev_[k], st_)); }
+void Engine::stage_event(int k) {
+  if (capturing_) return;  // recorded after the graph's launch instead
+  DMR_HIP(hipEventRecord(ev_[k], st_));
+}
+
+thread_local bool g_capture_active = false;
+
+void Engine::end_capture_and_launch() {
+  g_capture_active = false;
+  capturing_ = false;
+  hipGraph_t gr = nullptr;
+  DMR_HIP(hipStreamEndCapture(st_, &gr));
+  bool updated = false;
+  if (gexec_) {
+    hipGraphNode_t err = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    updated = hipGraphExecUpdate(gexec_, gr, &err, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(gexec_);
+      gexec_ = nullptr;
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (!gexec_) {
+    e = hipGraphInstantiate(&gexec_, gr, nullptr, nullptr, 0);
+    ++graph_builds_;
+  }
+  (void)hipGraphDestroy(gr);
+  DMR_HIP(e);
+  DMR_HIP(hipGraphLaunch(gexec_, st_));
+  ++graph_launches_;
+  // The chain's stage events: one span for the whole graph.
+  for (int k : {4, 5, 6}) DMR_HIP(hipEventRecord(ev_[k], st_));
+}
 float Engine::stage_ms(int a, int b) {
   float ms = 0;
   DMR_HIP(hipEventElapsedTime(&ms, ev_[a], ev_[b]));
@@ -719,59 +754,97 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   std::vector<std::vector<HspRec>>& result = *result_out;
   const int xdrop = raw_score(P, 12.3);  // --xdrop 12.3 bits (src/search.rs:51-53)
   const int cutoff = 19;                  // src/search.rs:164
-  // ---- ungapped: runs of (target, subject, diagonal) -----------------------
-  // Every later count (hit runs, segments, groups, jobs) is at most the
-  // hits': buffers are sized by nhits and the counts stay on the device until
-  // the score pass's host read.
-  uint64_t* dc = counts_dev_.ensure<uint64_t>(kCountSlots);
-  uint32_t* fl = flags_.ensure<uint32_t>(nhits + 1);
-  launch_key_flags(hits, nhits, (int)g.bp, fl, st_);
-  launch_fill_u32(fl + nhits, 1, 0, st_);
-  uint64_t* ps = pos_.ensure<uint64_t>(nhits + 1);
-  scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
-  launch_copy_u64(ps + nhits, dc + kCountHitRuns, 1, st_);
-  uint32_t* hrs = runs_.ensure<uint32_t>(nhits + 1);
-  launch_compact_flags(fl, ps, nhits, hrs, st_);
-  SegRec* sraw = segs_raw_.ensure<SegRec>(nhits + 1);
-  uint32_t* keep = keep_.ensure<uint32_t>(nhits + 1);
-  launch_xdrop_runs(hits, hrs, nhits, nhits, g, qs, dbs, matrix32_.as<int8_t>(), xdrop, cutoff, sraw, keep, st_,
-                    dc + kCountHitRuns);
-  launch_fill_u32(keep + nhits, 1, 0, st_);
-  scan_exclusive_u32_to_u64(keep, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
-  launch_copy_u64(ps + nhits, dc + kCountSegs, 1, st_);
-  SegRec* segs = segs_.ensure<SegRec>(nhits + 1);
-  uint64_t* gk = gkeys_.ensure<uint64_t>(nhits + 1);
-  launch_compact_segments(sraw, hits, keep, ps, nhits, (int)(g.bd + g.bp), segs, gk, st_);
-  stage_event(4);
-
-  // ---- bands per (target, subject) -----------------------------------------
-  launch_key_flags(gk, nhits, 0, fl, st_, dc + kCountSegs);
-  launch_fill_u32(fl + nhits, 1, 0, st_);
-  scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
-  launch_copy_u64(ps + nhits, dc + kCountGroups, 1, st_);
-  uint32_t* gs = gstart_.ensure<uint32_t>(nhits + 1);
-  launch_compact_flags(fl, ps, nhits, gs, st_);
-  uint32_t* nbd = nbands_.ensure<uint32_t>(nhits + 1);
-  SwJob* tj = tmpjobs_.ensure<SwJob>(nhits + 1);
-  launch_bands(segs, gk, gs, nhits, nhits, g.bs, qs, dbs, bd0_.ensure<int32_t>(nhits + 1),
-               bd1_.ensure<int32_t>(nhits + 1), bbest_.ensure<uint32_t>(nhits + 1), tj, nbd, st_,
-               dc + kCountGroups, dc + kCountSegs);
-  launch_fill_u32(nbd + nhits, 1, 0, st_);
-  uint64_t* jo = joboff_.ensure<uint64_t>(nhits + 1);
-  scan_exclusive_u32_to_u64(nbd, jo, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
-  launch_copy_u64(jo + nhits, dc + kCountJobs, 1, st_);
-  SwJob* jobs = jobs_.ensure<SwJob>(nhits + 1);
-  launch_jobs_compact(tj, gs, nbd, jo, nhits, jobs, st_, dc + kCountGroups);
-  uint64_t* hc = counts_pin_.ensure<uint64_t>(kCountSlots);
-  DMR_HIP(hipMemcpyAsync(hc + kCountHitRuns, dc + kCountHitRuns, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                         st_));
-  stage_event(5);
-
-  // ---- banded DP, score pass -----------------------------------------------
-  SwOut* outs = outs_.ensure<SwOut>(nhits + 1);
-  stage_event(6);
+  // The chain from the sorted hits to the score pass's host read, run
+  // directly or (layout key "graph") captured and replayed as one graph.
+  SwJob* jobs = nullptr;
+  SwOut* outs = nullptr;
+  uint64_t* hc = nullptr;
   StageStats SS;
-  const uint64_t njobs = score_pass(jobs, nhits, (int)g.bt, qs, dbs, opt, outs, &SS, dc + kCountJobs);
+  auto chain = [&]() -> uint64_t {
+    // ---- ungapped: runs of (target, subject, diagonal) -----------------------
+    // Every later count (hit runs, segments, groups, jobs) is at most the
+    // hits': buffers are sized by nhits and the counts stay on the device until
+    // the score pass's host read.
+    uint64_t* dc = counts_dev_.ensure<uint64_t>(kCountSlots);
+    uint32_t* fl = flags_.ensure<uint32_t>(nhits + 1);
+    launch_key_flags(hits, nhits, (int)g.bp, fl, st_);
+    launch_fill_u32(fl + nhits, 1, 0, st_);
+    uint64_t* ps = pos_.ensure<uint64_t>(nhits + 1);
+    scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+    launch_copy_u64(ps + nhits, dc + kCountHitRuns, 1, st_);
+    uint32_t* hrs = runs_.ensure<uint32_t>(nhits + 1);
+    launch_compact_flags(fl, ps, nhits, hrs, st_);
+    SegRec* sraw = segs_raw_.ensure<SegRec>(nhits + 1);
+    uint32_t* keep = keep_.ensure<uint32_t>(nhits + 1);
+    launch_xdrop_runs(hits, hrs, nhits, nhits, g, qs, dbs, matrix32_.as<int8_t>(), xdrop, cutoff, sraw, keep, st_,
+                      dc + kCountHitRuns);
+    launch_fill_u32(keep + nhits, 1, 0, st_);
+    scan_exclusive_u32_to_u64(keep, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+    launch_copy_u64(ps + nhits, dc + kCountSegs, 1, st_);
+    SegRec* segs = segs_.ensure<SegRec>(nhits + 1);
+    uint64_t* gk = gkeys_.ensure<uint64_t>(nhits + 1);
+    launch_compact_segments(sraw, hits, keep, ps, nhits, (int)(g.bd + g.bp), segs, gk, st_);
+    stage_event(4);
+
+    // ---- bands per (target, subject) -----------------------------------------
+    launch_key_flags(gk, nhits, 0, fl, st_, dc + kCountSegs);
+    launch_fill_u32(fl + nhits, 1, 0, st_);
+    scan_exclusive_u32_to_u64(fl, ps, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+    launch_copy_u64(ps + nhits, dc + kCountGroups, 1, st_);
+    uint32_t* gs = gstart_.ensure<uint32_t>(nhits + 1);
+    launch_compact_flags(fl, ps, nhits, gs, st_);
+    uint32_t* nbd = nbands_.ensure<uint32_t>(nhits + 1);
+    SwJob* tj = tmpjobs_.ensure<SwJob>(nhits + 1);
+    launch_bands(segs, gk, gs, nhits, nhits, g.bs, qs, dbs, bd0_.ensure<int32_t>(nhits + 1),
+                 bd1_.ensure<int32_t>(nhits + 1), bbest_.ensure<uint32_t>(nhits + 1), tj, nbd, st_,
+                 dc + kCountGroups, dc + kCountSegs);
+    launch_fill_u32(nbd + nhits, 1, 0, st_);
+    uint64_t* jo = joboff_.ensure<uint64_t>(nhits + 1);
+    scan_exclusive_u32_to_u64(nbd, jo, nhits + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(nhits + 1)), st_);
+    launch_copy_u64(jo + nhits, dc + kCountJobs, 1, st_);
+    jobs = jobs_.ensure<SwJob>(nhits + 1);
+    launch_jobs_compact(tj, gs, nbd, jo, nhits, jobs, st_, dc + kCountGroups);
+    hc = counts_pin_.ensure<uint64_t>(kCountSlots);
+    DMR_HIP(hipMemcpyAsync(hc + kCountHitRuns, dc + kCountHitRuns, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           st_));
+    stage_event(5);
+
+    // ---- banded DP, score pass -----------------------------------------------
+    outs = outs_.ensure<SwOut>(nhits + 1);
+    stage_event(6);
+    SS = StageStats();
+    return score_pass(jobs, nhits, (int)g.bt, qs, dbs, opt, outs, &SS, dc + kCountJobs);
+  };
+  uint64_t njobs = 0;
+  bool direct = true;
+  if (nhits > 0 && nhits <= graph_cap_ && resolve_layout(opt.sensitivity).graph) {
+    DMR_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    capturing_ = true;
+    g_capture_active = true;
+    auto abandon = [&] {
+      g_capture_active = false;
+      if (!capturing_) return;
+      capturing_ = false;
+      hipGraph_t gr = nullptr;
+      (void)hipStreamEndCapture(st_, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+    };
+    try {
+      njobs = chain();  // the score pass ends the capture and launches the graph
+      direct = false;
+    } catch (const CaptureGrow&) {  // a buffer must grow: run the chain directly
+      abandon();
+      graph_cap_ = 0;
+    } catch (...) {
+      abandon();
+      throw;
+    }
+  }
+  if (direct) {
+    njobs = chain();
+    graph_cap_ = std::max(graph_cap_, nhits);
+  }
   if (!nhits) DMR_HIP(hipStreamSynchronize(st_));
   stage_event(7);
   const uint64_t nsegs = hc[kCountSegs];
@@ -791,10 +864,10 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   if (profile_) accumulate_profile(njobs);
 
   // ---- selection by E-value -------------------------------------------------
-  fl = flags_.ensure<uint32_t>(njobs + 1);
+  uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
   launch_evalue_filter(jobs, outs, njobs, qs, dbs, P, db_->stat_letters, opt.evalue, fl, st_);
   launch_fill_u32(fl + njobs, 1, 0, st_);
-  ps = pos_.ensure<uint64_t>(njobs + 1);
+  uint64_t* ps = pos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
   const uint64_t nmaybe = read1(ps + njobs);
   S.maybe += nmaybe;
@@ -1004,6 +1077,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
   DMR_HIP(hipMemcpyAsync(h_summ, summ, kSummary * 4, hipMemcpyDeviceToHost, st_));
   uint64_t* h_njobs = sw_pin_count_.ensure<uint64_t>(1);
   if (njobs_dev) DMR_HIP(hipMemcpyAsync(h_njobs, njobs_dev, 8, hipMemcpyDeviceToHost, st_));
+  if (capturing_) end_capture_and_launch();
   DMR_HIP(hipStreamSynchronize(st_));
   if (njobs_dev) {
     if (*h_njobs > njobs) throw Error(-2, "score pass: job count past its bound");

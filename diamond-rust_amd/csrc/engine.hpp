@@ -35,6 +35,12 @@ struct Error : std::runtime_error {
   } while (0)
 
 // Growable device buffer (reallocated only when a larger size is requested).
+// Set while a stream capture records the chain into a graph: a buffer that
+// would have to grow then throws CaptureGrow (allocation is no stream work),
+// and the chain runs again without the graph.
+extern thread_local bool g_capture_active;
+struct CaptureGrow {};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -54,6 +60,7 @@ struct DevBuf {
   T* ensure(size_t n) {
     size_t bytes = n * sizeof(T) + 256;
     if (bytes > cap) {
+      if (g_capture_active) throw CaptureGrow{};
       release();
       size_t c = bytes + bytes / 4;
       DMR_HIP(hipMalloc(&p, c));
@@ -81,6 +88,7 @@ struct PinBuf {
   T* ensure(size_t n) {
     size_t bytes = n * sizeof(T) + 256;
     if (bytes > cap) {
+      if (g_capture_active) throw CaptureGrow{};
       release();
       size_t c = bytes + bytes / 4;
       DMR_HIP(hipHostMalloc(&p, c, hipHostMallocDefault));
@@ -282,6 +290,8 @@ class Engine {
   double layout_key(const std::string& key) const { return layout_value(layout_, key); }
   const ScoreLayout& layout() const { return layout_; }
   size_t tuned_presets() const { return tuned_.size(); }
+  uint64_t graph_launches() const { return graph_launches_; }
+  uint64_t graph_builds() const { return graph_builds_; }
   const std::string& tuning_file() const { return tuning_file_; }
   // The tuning file and pinned keys of another engine (serving workers).
   void copy_tuning(const Engine& o);
@@ -448,6 +458,14 @@ class Engine {
                        kCountGroups = 5, kCountJobs = 6, kCountSlots = 8;
   DevBuf counts_dev_;
   PinBuf counts_pin_, sw_pin_count_;
+  // The ungapped .. score-pass preprocessing chain as a graph (layout key
+  // "graph"): captured per slice, updated into one executable graph (built
+  // again when its shape changes) and launched; graph_cap_ = the most hits a
+  // run without the graph has sized every buffer of the chain for.
+  hipGraphExec_t gexec_ = nullptr;
+  bool capturing_ = false;
+  uint64_t graph_cap_ = 0, graph_launches_ = 0, graph_builds_ = 0;
+  void end_capture_and_launch();
   // Host copies of each group's job list and offsets; they stay alive until
   // the copies queued from them have run.
   std::vector<uint64_t> last_cbeg_, last_cend_;

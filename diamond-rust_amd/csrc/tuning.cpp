@@ -165,11 +165,12 @@ ScoreLayout default_layout() {
   L.merged_mask = 0x28u;  // width bounds 193 and 257
   L.f16 = true;
   L.plan = 0;
+  L.graph = false;
   return L;
 }
 
 bool is_layout_key(const std::string& k) {
-  if (k == "merged_mask" || k == "f16" || k == "score_plan") return true;
+  if (k == "merged_mask" || k == "f16" || k == "score_plan" || k == "graph") return true;
   for (const char* pre : {"lanes_", "threads_"}) {
     const size_t n = std::strlen(pre);
     if (k.compare(0, n, pre) == 0 && class_of(k.substr(n)) >= 0) return true;
@@ -205,6 +206,8 @@ void apply_settings(ScoreLayout* L, const Settings& s) {
       L->merged_mask = (unsigned)m | (1u << (kMergedClasses - 1));
     } else if (k == "f16") {
       L->f16 = v != 0;
+    } else if (k == "graph") {
+      L->graph = v != 0;
     } else if (k == "score_plan") {
       const int p = as_int(k, v);
       if (p < 0 || p > 2) bad("score_plan is 0, 1 or 2");
@@ -219,6 +222,7 @@ double layout_value(const ScoreLayout& L, const std::string& k) {
   if (k == "merged_mask") return L.merged_mask;
   if (k == "f16") return L.f16 ? 1 : 0;
   if (k == "score_plan") return L.plan;
+  if (k == "graph") return L.graph ? 1 : 0;
   if (k.compare(0, 6, "lanes_") == 0 && class_of(k.substr(6)) >= 0) return L.lanes[class_of(k.substr(6))];
   if (k.compare(0, 8, "threads_") == 0 && class_of(k.substr(8)) >= 0) return L.threads[class_of(k.substr(8))];
   bad("unknown tuning key: " + k);

@@ -19,6 +19,7 @@ struct ScoreLayout {
   unsigned merged_mask;         // merged kernels in use: bit i = class kExactClasses + i
   bool f16;                     // packed pairs in f16 (else saturating int16)
   int plan;                     // how the class kernels share the streams (0, 1, 2)
+  bool graph;                   // ungapped .. score-pass preprocessing replayed as one hipGraph
 };
 
 // Built-in defaults.
@@ -32,8 +33,9 @@ using Settings = std::map<std::string, double>;
 // Applies settings to L: "lanes_<class>" (a compiled lane count; also sets
 // the class's default block size for it), then "threads_<class>" (a compiled
 // block size for the class's lanes), "merged_mask" (6 bits; the widest merged
-// kernel is always in use), "f16" (0/1), "score_plan" (0-2). Throws Error(-1)
-// naming the key on an unknown key or a value with no compiled kernel.
+// kernel is always in use), "f16" (0/1), "score_plan" (0-2), "graph" (0/1).
+// Throws Error(-1) naming the key on an unknown key or a value with no
+// compiled kernel.
 void apply_settings(ScoreLayout* L, const Settings& s);
 // Value of a layout key in L (throws Error(-1) on an unknown key).
 double layout_value(const ScoreLayout& L, const std::string& key);

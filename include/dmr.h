@@ -122,7 +122,11 @@ int dmr_device_count(int* n);
  *   "merged_mask" merged kernels in use (bit i = m97 + 32 i; m257 always):
  *                 a merged band goes to the narrowest one that holds it;
  *   "f16"         1 = packed pairs in f16, 0 = saturating int16;
- *   "score_plan"  how the score-class kernels share the streams (0, 1, 2).
+ *   "score_plan"  how the score-class kernels share the streams (0, 1, 2);
+ *   "graph"       1 = the chain from the sorted hits to the score pass's
+ *                 class summary (ungapped extension, bands, job keys, sort,
+ *                 lane pairing) is captured and launched as one hipGraph,
+ *                 its executable graph updated in place from batch to batch.
  * Score-pass settings come from built-in defaults, then the tuning file (the
  * file DMR_TUNING names at context creation, empty for none; else
  * tuning_<arch>.json beside the library when it exists; settings per search
@@ -130,8 +134,9 @@ int dmr_device_count(int* n);
  * pinned over every preset. A key with no compiled kernel is refused
  * (DMR_E_ARG) and changes nothing. dmr_ctx_get_tuning reads the settings of
  * the last search (before the first: those of the default preset), plus
- * "last_slices" (slices of the last search) and "tuned_presets" (presets in
- * the loaded tuning file). */
+ * "last_slices" (slices of the last search), "tuned_presets" (presets in
+ * the loaded tuning file), "graph_launches" and "graph_builds" (graph
+ * launches and instantiations so far). */
 int dmr_ctx_set_tuning(dmr_ctx* ctx, const char* key, double value);
 int dmr_ctx_get_tuning(const dmr_ctx* ctx, const char* key, double* value);
 /* Replaces the context's tuning file (NULL or "" = none); the pinned keys

@@ -72,7 +72,7 @@ def test_committed_tuning_file_names_compiled_kernels(dmr):
     compiled = {(v["cls"], v["lanes"], v["threads"]) for v in dmr.sw16_variants()}
     assert len(compiled) >= 26 and doc["arch"] == "gfx950"
     keys = {f"{k}_{c}" for c in dmr.PACKED_CLASSES for k in ("lanes", "threads")} | {
-        "merged_mask", "f16", "score_plan"}
+        "merged_mask", "f16", "score_plan", "graph"}
     for name, entry in doc["presets"].items():
         s = entry["settings"]
         assert set(s) <= keys, name
