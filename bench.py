@@ -96,7 +96,17 @@ def fasta_ingress(dmr, w):
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
-def stage_rooflines(stats, nshapes, q_letters):
+def stage_traffic(stage: str, workload: str):
+    """Measured HBM bytes of a search stage (rocprofv3 FETCH_SIZE/WRITE_SIZE,
+    profiles/hbm_traffic.json), or None when not measured on this workload."""
+    try:
+        d = json.loads((ROOT / "profiles" / "hbm_traffic.json").read_text())
+    except (OSError, ValueError):
+        return None
+    return d.get("stages", {}).get(stage) if d.get("workload") == workload else None
+
+
+def stage_rooflines(stats, nshapes, q_letters, workload=None):
     """Seeds -> bands stages against the 8 TB/s HBM peak, from algorithmic
     bytes: seed enumeration reads each query letter once per shape and writes
     12 B per seed (8 B tagged key, 4 B letter index), the sort reads and
@@ -104,19 +114,23 @@ def stage_rooflines(stats, nshapes, q_letters):
     writes 8 B per hit plus 4 B of reference letter index read per hit; the
     hit sort reads and writes 8 B per hit (16 B); the ungapped stage reads
     8 B per hit and writes 16 B per segment (residue reads excluded: a lower
-    bound)."""
-    E, H, S = stats["query_seeds"], stats["hits"], stats["segments"]
+    bound); the bands read 24 B per segment and write a 32-byte job. `traffic`
+    is the stage's measured HBM bytes (rocprofv3 FETCH_SIZE/WRITE_SIZE) where
+    profiled on this workload."""
+    E, H, S, J = stats["query_seeds"], stats["hits"], stats["segments"], stats["jobs"]
     rows = {
-        "seed_and_sort": (q_letters * nshapes + E * 12 + E * 24, stats["ms_seed"]),
-        "join": (E * 12 + H * 12, stats["ms_join"]),
-        "sort_hits": (H * 16, stats["ms_sort_hits"]),
-        "ungapped": (H * 8 + S * 16, stats["ms_ungapped"]),
+        "seed_and_sort": (q_letters * nshapes + E * 12 + E * 24, stats["ms_seed"], "seed"),
+        "join": (E * 12 + H * 12, stats["ms_join"], "join"),
+        "sort_hits": (H * 16, stats["ms_sort_hits"], "sort_hits"),
+        "ungapped": (H * 8 + S * 16, stats["ms_ungapped"], "ungapped"),
+        "bands": (S * 24 + J * 32, stats["ms_bands"], "bands"),
     }
     out = {}
-    for k, (b, ms) in rows.items():
+    for k, (b, ms, pmc) in rows.items():
         gbs = b / (ms / 1e3) / 1e9 if ms > 0 else 0.0
         out[k] = {"bytes": int(b), "ms": round(ms, 4), "achieved_gbs": round(gbs, 1),
-                  "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                  "frac": round(gbs / HBM_PEAK_GBS, 4),
+                  "traffic": stage_traffic(pmc, workload) if workload else None}
     return out
 
 
@@ -312,7 +326,13 @@ def main():
     ap.add_argument("--legs", default="very-sensitive,blastx,serving",
                     help="other configurations to run after the headline one ('' for none)")
     ap.add_argument("--leg-steps", type=int, default=2)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="pin a score-pass setting (dmr_ctx_set_tuning) over the tuning file, e.g. merged_mask=63")
     a = ap.parse_args()
+    pins = {}
+    for t in a.tune:
+        k, _, v = t.partition("=")
+        pins[k] = float(v)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported "
               "(the multi-GPU split is a design only, not run).", file=sys.stderr)
@@ -331,6 +351,8 @@ def main():
     cfg = dmr.SearchConfig(sensitivity=sens)
 
     ctx = dmr.Context(0)
+    for k in sorted(pins, key=lambda k: (not k.startswith("lanes_"), k)):
+        ctx.set_tuning(k, pins[k])
     ctx.db_load(w.db_letters, w.db_offsets)
     index_ms = ctx.build_index(sens)
     log(f"[bench] resident seed index built in {index_ms:.1f} ms")
@@ -383,7 +405,8 @@ def main():
         "data": "synthetic (seeded generator diamond-rust_amd/synth.py)",
         "config": {"workload": f"{w.name}: blastp --sensitive, {w.n_q} queries vs {w.n_db} "
                                f"proteins / {int(w.db_offsets[-1])} letters",
-                   "global_batch": w.n_q, "parallelism": "single GPU"},
+                   "global_batch": w.n_q, "parallelism": "single GPU",
+                   **({"tuning_pins": pins} if pins else {})},
         "gcups": round(gcups, 2),
         "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3), "of": "one 1024-query batch"},
         "index_build_ms": round(index_ms, 2),
@@ -423,7 +446,8 @@ def main():
                                        "fixture": str(gold.relative_to(ROOT))}
         log(f"[bench] timed batch == oracle fixture: {text == exp}")
 
-    out["stage_rooflines"] = stage_rooflines(stats, 16 if sens == "sensitive" else 2, int(w.q_offsets[-1]))
+    out["stage_rooflines"] = stage_rooflines(stats, 16 if sens == "sensitive" else 2, int(w.q_offsets[-1]),
+                                             w.name)
     out["fasta_ingress"] = fasta_ingress(dmr, w)
 
     if not a.no_plain and a.plain_sample > 0:
