@@ -1415,7 +1415,7 @@ void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<
     uint32_t* jl = reinterpret_cast<uint32_t*>(pin_in + in_off[g]);
     uint64_t* toff = pin_in + in_off[g] + (ni + 1) / 2;
     toff[0] = 0;
-    const int64_t stride = 64ll << g;  // trace row: 64 * K bytes
+    const int64_t stride = 32ll << g;  // trace row: 32 * K bytes, 4 bits per cell
     for (uint64_t t = 0; t < ni; ++t) {
       items[t] = order[t].second;
       const SwJob& j = hjobs[items[t]];

@@ -1,5 +1,5 @@
 // Banded SW with one DP per wavefront: exact int32, any band width up to
-// 64*K (K = 1..32, so up to 2048), optional trace bytes and traceback.
+// 64*K (K = 1..32, so up to 2048), optional trace and traceback.
 //
 // Lane l owns band offsets t = l*K .. l*K+K-1 (column j = i - d1 + t of row
 // i). Rows are processed in order; within a row, D and E only read the
@@ -10,9 +10,9 @@
 // Using H' instead of H gives the same F and the same extension bits as the
 // cell-by-cell recurrence of gapped::extend_span (src/gapped.rs:151-169): a gap
 // opened from a cell whose H came from F is never better than extending F.
-// Trace bytes are choice | e_ext << 2 | f_ext << 3 with the reference's
-// strict D > E > F priority (src/gapped.rs:171-188); the best cell is the
-// first strict maximum in row-major order.
+// Trace cells are 4 bits, choice | e_ext << 2 | f_ext << 3, with the
+// reference's strict D > E > F priority (src/gapped.rs:171-188), two per
+// byte; the best cell is the first strict maximum in row-major order.
 //
 // Row ranges follow the padding scheme of the packed kernel: cells with
 // j <= 0 read a padding letter (-64) and stay at H = 0; cells with j > m are
@@ -86,7 +86,7 @@ __device__ __forceinline__ int letter_at(const uint8_t* ss, int m, int j) {
 
 // Traceback walk of one DP by a whole wave (gapped::traceback,
 // src/gapped.rs:205-279): every lane runs the same state machine on
-// wave-uniform values, and the trace bytes, query letters and subject letters
+// wave-uniform values, and the trace cells, query letters and subject letters
 // it reads come from a wave-private LDS cache filled a block of kRows rows at
 // a time by all lanes (coalesced), so a step costs one LDS read instead of a
 // dependent global load. Lane 0 writes the run-length ops (op | count << 2,
@@ -96,8 +96,8 @@ template <int K>
 __device__ void trace_walk(int lane, int wave, const uint8_t* tr, int rlo, int d0, int d1, int m,
                            const uint8_t* qs, const uint8_t* ss, const int8_t* M, int bi, int bj,
                            uint32_t* op, SwOut& o) {
-  constexpr int kRowBytes = 64 * K;       // one trace row
-  constexpr int kRows = 64 / K;           // rows per block: 4 KB of trace
+  constexpr int kRowBytes = 32 * K;       // one trace row (4 bits per cell)
+  constexpr int kRows = K == 1 ? 64 : 128 / K;  // rows per block: 2-4 KB of trace
   constexpr int kCols = kRows + 64 * K;   // subject letters those rows' bands cover
   __shared__ uint32_t tcache[4][kRows * kRowBytes / 4];
   __shared__ uint8_t qcache[4][kRows];
@@ -145,7 +145,8 @@ __device__ void trace_walk(int lane, int wave, const uint8_t* tr, int rlo, int d
   while (i > 0 && j > 0) {
     if (j < R.lo(i) || j > R.hi(i) || i < rlo) break;
     if (i < r0) refill(i);
-    const int tv = uniform((int)tc[(i - r0) * kRowBytes + (j - i + d1)]);
+    const int t = j - i + d1;  // band offset: nibble t % 2 of byte t / 2
+    const int tv = uniform((int)(tc[(i - r0) * kRowBytes + (t >> 1)] >> (4 * (t & 1))) & 15);
     uint32_t code;
     if (state == 0) {
       const int c = tv & 3;
@@ -337,9 +338,9 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         fext |= (uint32_t)(fe > fo);
       }
       int rowbest = 0;
-      uint32_t tw[(K + 3) / 4];  // trace bytes of the lane's offsets, packed
+      uint32_t tw[(K + 7) / 8];  // trace nibbles of the lane's offsets (offset k at bit 4k)
 #pragma unroll
-      for (int w = 0; w < (K + 3) / 4; ++w) tw[w] = 0;
+      for (int w = 0; w < (K + 7) / 8; ++w) tw[w] = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         int hv;
@@ -359,8 +360,8 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
             hv = F[k];
             choice = 3;
           }
-          const uint32_t byte = (uint32_t)choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3);
-          tw[k / 4] |= byte << (8 * (k % 4));
+          const uint32_t nib = (uint32_t)choice | (((eext >> k) & 1u) << 2) | (((fext >> k) & 1u) << 3);
+          tw[k / 8] |= nib << (4 * (k % 8));
         } else {
           hv = max(Hq[k], F[k]);
         }
@@ -369,14 +370,22 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         rowbest = max(rowbest, Hp[k]);
       }
       if (kTrace) {
-        // Row i of the trace is 64K bytes (offset t at byte t): every lane
-        // stores its K bytes at once, and a wave writes whole lines.
-        uint8_t* row = tr + (uint64_t)(i - rlo) * (64 * K) + lane * K;
-        if (K == 1) *row = (uint8_t)tw[0];
-        else if (K == 2) *reinterpret_cast<uint16_t*>(row) = (uint16_t)tw[0];
-        else
+        // Row i of the trace is 32K bytes, 4 bits per cell: offset t in the
+        // low (t even) or high nibble of byte t / 2. Every lane stores its
+        // K / 2 bytes at once (with one lane, an even lane stores its own
+        // nibble and its neighbour's), and a wave writes whole lines.
+        uint8_t* row = tr + (uint64_t)(i - rlo) * (32 * K);
+        if (K == 1) {
+          const uint32_t pair = tw[0] | ((uint32_t)from_next((int)tw[0], 0) << 4);
+          if ((lane & 1) == 0) row[lane >> 1] = (uint8_t)pair;
+        } else if (K == 2) {
+          row[lane] = (uint8_t)tw[0];
+        } else if (K == 4) {
+          reinterpret_cast<uint16_t*>(row)[lane] = (uint16_t)tw[0];
+        } else {
 #pragma unroll
-          for (int w = 0; w < (K + 3) / 4; ++w) reinterpret_cast<uint32_t*>(row)[w] = tw[w];
+          for (int w = 0; w < K / 8; ++w) reinterpret_cast<uint32_t*>(row + lane * (K / 2))[w] = tw[w];
+        }
       }
       // Subject letters for the next row (the next lane's first letter is
       // read before the shift overwrites it).
