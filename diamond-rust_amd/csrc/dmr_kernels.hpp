@@ -132,7 +132,8 @@ void launch_sw_generic(const SwJob* jobs, const uint32_t* job_idx, uint64_t njob
                        SeqSet db, const int8_t* matrix32, int gap_open, int gap_extend,
                        int32_t* scratch, const uint64_t* scratch_off, uint8_t* trace,
                        const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off,
-                       SwOut* out, bool traceback, hipStream_t st, bool scatter = false);
+                       SwOut* out, bool traceback, hipStream_t st, bool scatter = false,
+                       int32_t* scores = nullptr);  // scores: score mode writes scores[job] only
 
 // Packed int16 score pass (k_sw16.hip).
 int pair_table_entries();
@@ -186,11 +187,11 @@ void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2
 // arithmetic (pair_tab must then be the f16 table).
 bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
                  const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
-                 SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
+                 int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 bool launch_sw16_merged(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
-                        const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter, SwOut* outs,
+                        const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter, int32_t* scores,
                         uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
-void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
+void launch_evalue_filter(const SwJob* jobs, const int32_t* scores, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);
 void launch_u32_times(const uint32_t* in, uint64_t n, uint32_t mul, uint64_t* out, hipStream_t st);
@@ -225,14 +226,14 @@ constexpr int kMaxBlockK = 16;
 int swblock_k_for(int64_t width);
 uint64_t swblock_row_bytes(int K);  // trace bytes of one row
 // Every DP of list whose width lies in (2048, 16384] (only_k: one K's range,
-// 0 = all); score mode writes outs[job].score, trace mode outs[task] and the
-// task's ops, and needs the score pass's outputs (known, by job). Wider DPs
+// 0 = all); score mode writes scores[job], trace mode outs[task] and the
+// task's ops, and needs the score pass's scores (known, by job). Wider DPs
 // and DPs whose best may lie in the clamped rows go to the fallback list (job
 // ids in score mode, tasks in trace mode).
 void launch_sw_block(bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs, SeqSet q, SeqSet db,
-                     const int8_t* m32, int gap_open, int gap_extend, SwOut* outs, uint8_t* trace,
-                     const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback,
-                     uint32_t* nfallback, const SwOut* known, int only_k, hipStream_t st);
+                     const int8_t* m32, int gap_open, int gap_extend, SwOut* outs, int32_t* scores,
+                     uint8_t* trace, const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off,
+                     uint32_t* fallback, uint32_t* nfallback, const int32_t* known, int only_k, hipStream_t st);
 constexpr int kWaveKinds = 6;  // K = 1, 2, 4, 8, 16, 32
 int swwave_k_for(int64_t width);
 // Packs the run-length ops of n traceback DPs (capacity slots at ops_off)
@@ -241,11 +242,14 @@ int swwave_k_for(int64_t width);
 void launch_ops_pack(const SwOut* outs, const uint32_t* ops, const uint64_t* ops_off, uint32_t n,
                      uint32_t* sizes, uint64_t* pack_off, uint64_t* scan_tmp, uint32_t* packed,
                      hipStream_t st);
+// Wave-per-DP int32 kernel over jobs[list[0..njobs)]: score mode writes
+// scores[job]; trace mode writes outs[task] and the task's ops, stopping each
+// DP at the first cell that reaches known[job] (the score pass's scores).
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
-                    uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
+                    uint32_t* counter, SwOut* outs, int32_t* scores, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int cu_count, hipStream_t st, const SwOut* known = nullptr,
+                    int cu_count, hipStream_t st, const int32_t* known = nullptr,
                     const uint32_t* njobs_dev = nullptr);  // njobs_dev: a device-side count (njobs bounds it)
 
 }  // namespace dmr

@@ -54,8 +54,8 @@ OpsStore::~OpsStore() {
 }
 
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
-void launch_gather(const SwJob* jobs, const SwOut* outs, const uint32_t* idx, uint64_t n,
-                   SwJob* jout, SwOut* oout, hipStream_t st);
+void launch_gather(const SwJob* jobs, const int32_t* scores, const uint32_t* idx, uint64_t n,
+                   SwJob* jout, int32_t* sout, hipStream_t st);
 void launch_xdrop_pairs(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint64_t slen,
                         const uint32_t* qa, const uint32_t* sa, uint64_t n, const int8_t* m32,
                         int xdrop, SegRec* out, hipStream_t st);
@@ -757,7 +757,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   // The chain from the sorted hits to the score pass's host read, run
   // directly or (layout key "graph") captured and replayed as one graph.
   SwJob* jobs = nullptr;
-  SwOut* outs = nullptr;
+  int32_t* scores = nullptr;
   uint64_t* hc = nullptr;
   StageStats SS;
   auto chain = [&]() -> uint64_t {
@@ -810,10 +810,10 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
     stage_event(5);
 
     // ---- banded DP, score pass -----------------------------------------------
-    outs = outs_.ensure<SwOut>(nhits + 1);
+    scores = scores_.ensure<int32_t>(nhits + 1);
     stage_event(6);
     SS = StageStats();
-    return score_pass(jobs, nhits, (int)g.bt, qs, dbs, opt, outs, &SS, dc + kCountJobs);
+    return score_pass(jobs, nhits, (int)g.bt, qs, dbs, opt, scores, &SS, dc + kCountJobs);
   };
   uint64_t njobs = 0;
   bool direct = true;
@@ -865,7 +865,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
 
   // ---- selection by E-value -------------------------------------------------
   uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
-  launch_evalue_filter(jobs, outs, njobs, qs, dbs, P, db_->stat_letters, opt.evalue, fl, st_);
+  launch_evalue_filter(jobs, scores, njobs, qs, dbs, P, db_->stat_letters, opt.evalue, fl, st_);
   launch_fill_u32(fl + njobs, 1, 0, st_);
   uint64_t* ps = pos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
@@ -873,21 +873,21 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   S.maybe += nmaybe;
   uint32_t* midx = sel_idx_.ensure<uint32_t>(nmaybe + 1);
   launch_compact_flags(fl, ps, njobs, midx, st_);
-  // The candidates (job index, job, score-pass output) come back through
-  // pinned memory and are read in place (valid until the next search).
+  // The candidates (job index, job, score) come back through pinned memory
+  // and are read in place (valid until the next search).
   const uint64_t pin_job = (nmaybe * 4 + 15) / 16 * 16;
-  const uint64_t pin_out = pin_job + (nmaybe * sizeof(SwJob) + 15) / 16 * 16;
-  uint8_t* spin = sel_pin_.ensure<uint8_t>(pin_out + nmaybe * sizeof(SwOut) + 16);
+  const uint64_t pin_sc = pin_job + (nmaybe * sizeof(SwJob) + 15) / 16 * 16;
+  uint8_t* spin = sel_pin_.ensure<uint8_t>(pin_sc + nmaybe * 4 + 16);
   uint32_t* h_midx = reinterpret_cast<uint32_t*>(spin);
   SwJob* h_mjobs = reinterpret_cast<SwJob*>(spin + pin_job);
-  SwOut* h_mouts = reinterpret_cast<SwOut*>(spin + pin_out);
+  int32_t* h_mscore = reinterpret_cast<int32_t*>(spin + pin_sc);
   if (nmaybe) {
     SwJob* gj = tmpjobs_.ensure<SwJob>(std::max<uint64_t>(nmaybe, nsegs) + 1);
-    SwOut* go = sel_outs_.ensure<SwOut>(nmaybe + 1);
-    launch_gather(jobs, outs, midx, nmaybe, gj, go, st_);
+    int32_t* gs = sel_scores_.ensure<int32_t>(nmaybe + 1);
+    launch_gather(jobs, scores, midx, nmaybe, gj, gs, st_);
     DMR_HIP(hipMemcpyAsync(h_midx, midx, nmaybe * 4, hipMemcpyDeviceToHost, st_));
     DMR_HIP(hipMemcpyAsync(h_mjobs, gj, nmaybe * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
-    DMR_HIP(hipMemcpyAsync(h_mouts, go, nmaybe * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipMemcpyAsync(h_mscore, gs, nmaybe * 4, hipMemcpyDeviceToHost, st_));
   }
   stage_event(8);
   DMR_HIP(hipStreamSynchronize(st_));
@@ -896,13 +896,13 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   // The reported DPs are among the candidates with a positive score (H6), so
   // all of those are traced while the host computes the exact E-values and
   // selects; the few that are not selected are dropped at assembly. The
-  // score pass's best scores (outs, by job) let each traceback DP stop at the
+  // score pass's best scores (by job) let each traceback DP stop at the
   // row of its first maximum.
   std::vector<uint32_t> tb_list, tb_slen;
   std::vector<SwJob> tb_jobs;
   std::vector<uint64_t> tb_pos(nmaybe, ~0ull);  // maybe index -> traced index
   for (uint64_t k = 0; k < nmaybe; ++k) {
-    if (h_mouts[k].score <= 0) continue;
+    if (h_mscore[k] <= 0) continue;
     tb_pos[k] = tb_list.size();
     tb_list.push_back(h_midx[k]);
     tb_jobs.push_back(h_mjobs[k]);
@@ -910,7 +910,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   }
   stage_event(9);
   TbRun tb_run;
-  traceback_launch(&tb_run, jobs, tb_list, tb_jobs, tb_slen, qs, dbs, opt.gap_open, opt.gap_extend, outs);
+  traceback_launch(&tb_run, jobs, tb_list, tb_jobs, tb_slen, qs, dbs, opt.gap_open, opt.gap_extend, scores);
 
   double th0 = now_ms();
   // Exact E-values of the candidates (glibc f64, as the reference), computed
@@ -919,7 +919,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
   host_parallel_for(nmaybe, 256, [&](uint64_t b, uint64_t e) {
     for (uint64_t k = b; k < e; ++k) {
       const SwJob& j = h_mjobs[k];
-      const int32_t sc = h_mouts[k].score;
+      const int32_t sc = h_mscore[k];
       if (sc <= 0) continue;
       const uint64_t qlen = tset_.offs[j.target + 1] - tset_.offs[j.target];
       const uint64_t slen = db_->h_offs[j.subject + 1] - db_->h_offs[j.subject];
@@ -942,7 +942,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
       for (; k < nmaybe && h_mjobs[k].group == grp; ++k) {
         if (hs.size() >= opt.max_hsps) continue;
         const SwJob& j = h_mjobs[k];
-        int32_t sc = h_mouts[k].score;
+        int32_t sc = h_mscore[k];
         if (sc <= 0) continue;
         const double e = h_ev[k];
         if (e <= opt.evalue) {
@@ -1029,7 +1029,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
 // clipped bands, other gap costs) and any int16-saturated DP to the exact
 // int32 kernel.
 uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
-                            const SearchOptions& opt, SwOut* outs, StageStats* S, const uint64_t* njobs_dev) {
+                            const SearchOptions& opt, int32_t* scores, StageStats* S, const uint64_t* njobs_dev) {
   for (int c = 0; c < kClasses; ++c) S->ms_class[c] = 0;
   if (!njobs) return 0;
   last_target_bits_ = target_bits;
@@ -1136,7 +1136,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     if (!launch_sw16(c, L.lanes[c], L.threads[c], L.f16, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs,
                      (L.f16 ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(), cnt + kCntPacked + c,
-                     outs, redo, cnt + kCntRedo, cus, s))
+                     scores, redo, cnt + kCntRedo, cus, s))
       throw Error(-1, "no compiled packed kernel for class " + packed_class_name(c));
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   };
@@ -1166,8 +1166,8 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
     }
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     launch_sw_wave(K, false, jobs, list, (uint32_t)total, qs, dbs, matrix32_.as<int8_t>(),
-                   opt.gap_open, opt.gap_extend, cnt + kCntWave + (c - kPackedClasses), outs, nullptr, nullptr,
-                   nullptr, nullptr, fallback, cnt + kCntFallback, cus, s);
+                   opt.gap_open, opt.gap_extend, cnt + kCntWave + (c - kPackedClasses), nullptr, scores, nullptr,
+                   nullptr, nullptr, nullptr, fallback, cnt + kCntFallback, cus, s);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   };
   static_assert(kExactClasses == 5 && kWaveKinds == 6, "the launch plan names every class");
@@ -1208,7 +1208,8 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
   if (nwide) {
     DMR_HIP(hipEventRecord(ev_[12 + 2 * kRowExact], st_));
     launch_sw_block(false, jobs, idx + cbeg[kRowExact], (uint32_t)nwide, qs, dbs, matrix32_.as<int8_t>(),
-                    opt.gap_open, opt.gap_extend, outs, nullptr, nullptr, nullptr, nullptr, fallback, cnt + kCntFallback,
+                    opt.gap_open, opt.gap_extend, nullptr, scores, nullptr, nullptr, nullptr, nullptr, fallback,
+                    cnt + kCntFallback,
                     nullptr, 0, st_);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
   }
@@ -1217,7 +1218,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
   // <= 256). It reads their count on the device, so no host read precedes it;
   // one read afterwards brings both counts.
   launch_sw_wave(4, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
-                 opt.gap_extend, cnt + kCntRedoWave, outs, nullptr, nullptr, nullptr, nullptr, fallback,
+                 opt.gap_extend, cnt + kCntRedoWave, nullptr, scores, nullptr, nullptr, nullptr, nullptr, fallback,
                  cnt + kCntFallback, cus, st_, nullptr, cnt + kCntRedo);
   uint32_t* h_cnt = sw_pin_summary_.ensure<uint32_t>(kCntWords);
   DMR_HIP(hipMemcpyAsync(h_cnt, cnt, kCntWords * 4, hipMemcpyDeviceToHost, st_));
@@ -1250,7 +1251,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
     DMR_HIP(hipEventRecord(ev_[12 + 2 * kRowExact], st_));
     launch_sw_generic(jobs, list, nlist, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                       opt.gap_extend, scratch_.ensure<int32_t>(scratch_total + 1), soff, nullptr,
-                      nullptr, nullptr, nullptr, outs, false, st_, /*scatter=*/true);
+                      nullptr, nullptr, nullptr, nullptr, false, st_, /*scatter=*/true, scores);
     DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
     DMR_HIP(hipEventSynchronize(ev_[13 + 2 * kRowExact]));
     S->ms_class[kRowExact] = wide_ms + stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
@@ -1320,13 +1321,13 @@ bool Engine::swk_launched(int c, const uint64_t* cbeg, const uint64_t* cend, boo
 // Traceback DPs for jobs[list[k]] (hjobs[k] is the host copy, slen[k] the
 // subject length). Wave-per-DP kernel by band width; the row-exact kernel for
 // bands wider than 512 and for DPs whose best cell may sit in the clamped
-// rows below the band. known (optional, device): the score pass's outputs by
+// rows below the band. known (optional, device): the score pass's scores by
 // job; with them each wave-kernel DP stops at the row of its first maximum.
 void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& list,
                             const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen,
                             SeqSet qs, SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
                             std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out,
-                            const SwOut* known) {
+                            const int32_t* known) {
   TbRun run;
   traceback_launch(&run, djobs, list, hjobs, slen, qs, dbs, go, ge, known);
   std::vector<OpsRef> refs;
@@ -1341,7 +1342,7 @@ void Engine::traceback_pass(const SwJob* djobs, const std::vector<uint32_t>& lis
 // traceback_collect; list, hjobs and slen must stay alive until then.
 void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<uint32_t>& list,
                               const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen,
-                              SeqSet qs, SeqSet dbs, int go, int ge, const SwOut* known) {
+                              SeqSet qs, SeqSet dbs, int go, int ge, const int32_t* known) {
   const uint64_t ns = list.size();
   *run = TbRun{};
   run->djobs = djobs;
@@ -1450,7 +1451,7 @@ void Engine::traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<
     uint8_t* tr = tb_trace_[g].ensure<uint8_t>(htoff[ni] + 1);
     uint32_t* fb = tb_fb_[g].ensure<uint32_t>(ni + 1);
     launch_sw_wave(1 << g, true, djobs, dl, (uint32_t)ni, qs, dbs, matrix32_.as<int8_t>(), go, ge,
-                   cnt + g, dout_all + out_base[g], tr, dt, dops_all, dooff + out_base[g], fb,
+                   cnt + g, dout_all + out_base[g], nullptr, tr, dt, dops_all, dooff + out_base[g], fb,
                    cnt + kWaveKinds + g, cus, s, known);
   }
   join();
@@ -1570,7 +1571,7 @@ void Engine::traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector
     uint32_t* dfb = bk_fb_.ensure<uint32_t>(nb + 1);
     SwOut* dout = bk_out_.ensure<SwOut>(nb + 1);
     uint32_t* dops = bk_ops_.ensure<uint32_t>(ooff[nb] + 1);
-    launch_sw_block(true, djobs, dl, (uint32_t)nb, qs, dbs, matrix32_.as<int8_t>(), go, ge, dout,
+    launch_sw_block(true, djobs, dl, (uint32_t)nb, qs, dbs, matrix32_.as<int8_t>(), go, ge, dout, nullptr,
                     bk_trace_.ensure<uint8_t>(toff[nb] + 4), dto, dops, doo, dfb, dcnt, run->known, 0, st_);
     std::vector<SwOut> ho(nb);
     std::vector<uint32_t> hfb(nb + 1), hops(ooff[nb]);
@@ -1914,7 +1915,7 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
     dj[i] = SwJob{(uint32_t)(2 * i), (uint32_t)(2 * i), 0, (uint32_t)J.qlen, (int32_t)d0, (int32_t)d1,
                   (uint32_t)i, 0};
   }
-  DevBuf dq, ds, dqo, dso, djobs, dst, dsoff, dscr, dcells, dw0, dtoff, dtr, doo, dops, dout;
+  DevBuf dq, ds, dqo, dso, djobs, dsc;
   DevBuf dqp, dsp, dqpo, dspo;
   DMR_HIP(hipMemcpyAsync(dq.ensure<uint8_t>(qbuf_len + 64), qbuf, qbuf_len, hipMemcpyHostToDevice, st_));
   DMR_HIP(hipMemcpyAsync(ds.ensure<uint8_t>(sbuf_len + 64), sbuf, sbuf_len, hipMemcpyHostToDevice, st_));
@@ -1949,27 +1950,6 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
     ss.pad = dsp.as<uint8_t>();
     ss.pad_offs = dspo.as<uint32_t>();
   }
-  uint64_t* cells = dcells.ensure<uint64_t>(njobs);
-  uint64_t* w0 = dw0.ensure<uint64_t>(njobs);
-  uint32_t* stv = dst.ensure<uint32_t>(njobs);
-  launch_job_geometry(djobs.as<SwJob>(), nullptr, njobs, ss, cells, w0, stv, st_);
-  std::vector<uint64_t> hc(njobs), hw(njobs);
-  std::vector<uint32_t> hs(njobs);
-  DMR_HIP(hipMemcpyAsync(hc.data(), cells, njobs * 8, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipMemcpyAsync(hw.data(), w0, njobs * 8, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipMemcpyAsync(hs.data(), stv, njobs * 4, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipStreamSynchronize(st_));
-  std::vector<uint64_t> soff(njobs + 1, 0), toff(njobs + 1, 0), ooff(njobs + 1, 0);
-  for (uint64_t i = 0; i < njobs; ++i) {
-    soff[i + 1] = soff[i] + 4ull * hs[i];
-    toff[i + 1] = toff[i] + (traceback ? hc[i] + hw[i] : 0);
-    ooff[i + 1] = ooff[i] + (traceback ? jobs[i].qlen + jobs[i].slen + 1 : 0);
-  }
-  DMR_HIP(hipMemcpyAsync(dsoff.ensure<uint64_t>(njobs + 1), soff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
-  DMR_HIP(hipMemcpyAsync(dtoff.ensure<uint64_t>(njobs + 1), toff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
-  DMR_HIP(hipMemcpyAsync(doo.ensure<uint64_t>(njobs + 1), ooff.data(), (njobs + 1) * 8, hipMemcpyHostToDevice, st_));
-  SwOut* o = dout.ensure<SwOut>(njobs);
-  uint32_t* opsd = dops.ensure<uint32_t>(ooff[njobs] + 1);
   if (traceback) {
     std::vector<uint32_t> list(njobs), slen(njobs);
     for (uint64_t i = 0; i < njobs; ++i) {
@@ -1991,21 +1971,18 @@ void Engine::extend_span_batch(const uint8_t* qbuf, uint64_t qbuf_len, const uin
     StageStats S;
     int tb = 1;
     while ((1ull << tb) <= 2 * njobs) ++tb;
-    score_pass(djobs.as<SwJob>(), njobs, tb, qs, ss, so, o, &S);
+    int32_t* sc = dsc.ensure<int32_t>(njobs);
+    score_pass(djobs.as<SwJob>(), njobs, tb, qs, ss, so, sc, &S);
     last_dropin_stats_ = S;
     // The class profile of a drop-in batch: kernel times only.
     for (int c = 0; c < kClasses; ++c) last_ms_class_[c] = S.ms_class[c];
     last_njobs_ = 0;
     last_slices_ = 1;
+    std::vector<int32_t> hsc(njobs);
+    DMR_HIP(hipMemcpyAsync(hsc.data(), sc, njobs * 4, hipMemcpyDeviceToHost, st_));
+    DMR_HIP(hipStreamSynchronize(st_));
+    for (uint64_t i = 0; i < njobs; ++i) (*outs)[i].score = hsc[i];
   }
-  DMR_HIP(hipMemcpyAsync(outs->data(), o, njobs * sizeof(SwOut), hipMemcpyDeviceToHost, st_));
-  std::vector<uint32_t> hops(ooff[njobs]);
-  if (traceback && ooff[njobs])
-    DMR_HIP(hipMemcpyAsync(hops.data(), opsd, ooff[njobs] * 4, hipMemcpyDeviceToHost, st_));
-  DMR_HIP(hipStreamSynchronize(st_));
-  if (ops && traceback)
-    for (uint64_t i = 0; i < njobs; ++i)
-      (*ops)[i].assign(hops.begin() + ooff[i], hops.begin() + ooff[i] + (*outs)[i].n_ops);
 }
 
 }  // namespace dmr

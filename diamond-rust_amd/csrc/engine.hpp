@@ -307,7 +307,7 @@ class Engine {
                       const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
                       SeqSet dbs, int go, int ge, std::vector<SwOut>* outs,
                       std::vector<std::vector<uint32_t>>* ops, uint64_t* cells_out,
-                      const SwOut* known = nullptr);
+                      const int32_t* known = nullptr);
   // A traceback between its launch and the collection of its results.
   struct TbRun {
     const SwJob* djobs = nullptr;
@@ -316,7 +316,7 @@ class Engine {
     const std::vector<uint32_t>* slen = nullptr;
     SeqSet qs{}, dbs{};
     int go = 0, ge = 0;
-    const SwOut* known = nullptr;  // score-pass outputs by job (device)
+    const int32_t* known = nullptr;  // score-pass scores by job (device)
     uint64_t ns = 0, cells = 0;
     std::vector<std::vector<uint64_t>> group;
     // Per wave-kernel group: its block in the pinned input and its first
@@ -328,7 +328,7 @@ class Engine {
   };
   void traceback_launch(TbRun* run, const SwJob* djobs, const std::vector<uint32_t>& list,
                         const std::vector<SwJob>& hjobs, const std::vector<uint32_t>& slen, SeqSet qs,
-                        SeqSet dbs, int go, int ge, const SwOut* known);
+                        SeqSet dbs, int go, int ge, const int32_t* known);
   void traceback_collect(TbRun* run, std::vector<SwOut>* outs, std::vector<OpsRef>* ops,
                          std::shared_ptr<OpsStore>* store,
                          uint64_t* cells_out);
@@ -373,7 +373,7 @@ class Engine {
   // read on the device (the pass's one host read brings it back). Returns the
   // job count.
   uint64_t score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
-                      const SearchOptions& opt, SwOut* outs, StageStats* S, const uint64_t* njobs_dev = nullptr);
+                      const SearchOptions& opt, int32_t* scores, StageStats* S, const uint64_t* njobs_dev = nullptr);
   void search_slice(const SearchOptions& opt, const AlpParams& P, const HitGeom& g, const uint64_t* hits,
                     uint64_t nhits, SeqSet qs, SeqSet dbs, StageStats* S,
                     std::vector<std::vector<HspRec>>* result,
@@ -436,8 +436,8 @@ class Engine {
   DevBuf hits_, hits2_;
   DevBuf segs_raw_, keep_, segs_, gkeys_, gstart_;
   DevBuf bd0_, bd1_, bbest_, tmpjobs_, nbands_, joboff_, jobs_;
-  DevBuf stride_, scr_off_, scratch_, outs_;
-  DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_outs_;
+  DevBuf stride_, scr_off_, scratch_, scores_;
+  DevBuf sel_idx_, cells_, width0_, trace_off_, trace_, ops_off_, ops_, sel_scores_;
   DevBuf sw_keys_, sw_idx_, sw_keys2_, sw_idx2_, sw_lanes_, sw_lanes2_, sw_lane_idx_, sw_counters_,
       sw_redo_, sw_list_, sw_summary_, sw_heads_, sw_hpos_;
   PinBuf sw_pin_summary_;

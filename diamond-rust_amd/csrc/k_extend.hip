@@ -76,13 +76,13 @@ __global__ void k_xdrop_pairs(const uint8_t* __restrict__ q, uint64_t qlen, cons
   out[i] = xdrop_dev(M, q, (int64_t)qlen, s, (int64_t)slen, qa[i], sa[i], xdrop);
 }
 
-__global__ void k_gather(const SwJob* __restrict__ jobs, const SwOut* __restrict__ outs,
+__global__ void k_gather(const SwJob* __restrict__ jobs, const int32_t* __restrict__ scores,
                          const uint32_t* __restrict__ idx, uint64_t n, SwJob* __restrict__ jout,
-                         SwOut* __restrict__ oout) {
+                         int32_t* __restrict__ sout) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   jout[i] = jobs[idx[i]];
-  oout[i] = outs[idx[i]];
+  sout[i] = scores[idx[i]];
 }
 
 __global__ void k_gather_u64(const uint64_t* __restrict__ in, const uint32_t* __restrict__ idx,
@@ -264,12 +264,12 @@ __global__ void k_jobs_compact(const SwJob* __restrict__ tmp_jobs, const uint32_
   for (uint32_t r = 0; r < nbands[gi]; ++r) jobs[o + r] = tmp_jobs[s0 + r];
 }
 
-__global__ void k_evalue_filter(const SwJob* __restrict__ jobs, const SwOut* __restrict__ outs,
+__global__ void k_evalue_filter(const SwJob* __restrict__ jobs, const int32_t* __restrict__ scores,
                                 uint64_t njobs, SeqSet q, SeqSet db, AlpParams P, uint64_t db_letters,
                                 double threshold, uint32_t* __restrict__ flags) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= njobs) return;
-  int32_t s = outs[i].score;
+  int32_t s = scores[i];
   uint32_t f = 0;
   if (s > 0) {
     const SwJob j = jobs[i];
@@ -331,10 +331,10 @@ void launch_gather_u64(const uint64_t* in, const uint32_t* idx, uint64_t n, uint
   hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(256), 0, st, in, idx, n, out);
 }
 
-void launch_gather(const SwJob* jobs, const SwOut* outs, const uint32_t* idx, uint64_t n,
-                   SwJob* jout, SwOut* oout, hipStream_t st) {
+void launch_gather(const SwJob* jobs, const int32_t* scores, const uint32_t* idx, uint64_t n,
+                   SwJob* jout, int32_t* sout, hipStream_t st) {
   if (!n) return;
-  hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(256), 0, st, jobs, outs, idx, n, jout, oout);
+  hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(256), 0, st, jobs, scores, idx, n, jout, sout);
 }
 
 void launch_xdrop_pairs(const uint8_t* q, uint64_t qlen, const uint8_t* s, uint64_t slen,
@@ -391,11 +391,11 @@ void launch_jobs_compact(const SwJob* tmp_jobs, const uint32_t* group_start, con
                      nbands, job_off, ngroups, jobs, ngroups_dev);
 }
 
-void launch_evalue_filter(const SwJob* jobs, const SwOut* outs, uint64_t njobs, SeqSet q,
+void launch_evalue_filter(const SwJob* jobs, const int32_t* scores, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st) {
   if (!njobs) return;
-  hipLaunchKernelGGL(k_evalue_filter, dim3(nblk(njobs)), dim3(256), 0, st, jobs, outs, njobs, q, db,
+  hipLaunchKernelGGL(k_evalue_filter, dim3(nblk(njobs)), dim3(256), 0, st, jobs, scores, njobs, q, db,
                      P, db_letters, threshold, flags);
 }
 

@@ -62,7 +62,7 @@ __global__ void k_sw_generic(const SwJob* __restrict__ jobs, const uint32_t* __r
                              const uint64_t* __restrict__ scratch_off, uint8_t* __restrict__ trace,
                              const uint64_t* __restrict__ trace_off, uint32_t* __restrict__ ops,
                              const uint64_t* __restrict__ ops_off, SwOut* __restrict__ out,
-                             int scatter) {
+                             int scatter, int32_t* __restrict__ scores) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
   for (int i = threadIdx.x; i < kScoreStride * kScoreStride; i += blockDim.x) M[i] = m32[i];
   __syncthreads();
@@ -74,9 +74,12 @@ __global__ void k_sw_generic(const SwJob* __restrict__ jobs, const uint32_t* __r
   const uint8_t* ss = db.res + db.offs[J.subject];
   const int64_t m = db.offs[J.subject + 1] - db.offs[J.subject];
   SwOut o = {};
-  SwOut* dst = out + (scatter ? job_idx[t] : t);
+  // Score mode with scores: only the score, at the job's index.
+  SwOut* dst = scores ? nullptr : out + (scatter ? job_idx[t] : t);
+  int32_t* sdst = scores ? scores + (job_idx ? job_idx[t] : t) : nullptr;
   if (n <= 0 || m <= 0) {
-    *dst = o;
+    if (sdst) *sdst = 0;
+    else *dst = o;
     return;
   }
   const RowRange R{J.d0, J.d1, m};
@@ -242,7 +245,8 @@ __global__ void k_sw_generic(const SwJob* __restrict__ jobs, const uint32_t* __r
     o.s_begin = (uint32_t)j;
     if (nruns == 0) o.score = 0;
   }
-  *dst = o;
+  if (sdst) *sdst = o.score;
+  else *dst = o;
 }
 
 inline unsigned nblk(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -260,17 +264,17 @@ void launch_sw_generic(const SwJob* jobs, const uint32_t* job_idx, uint64_t njob
                        SeqSet db, const int8_t* matrix32, int gap_open, int gap_extend,
                        int32_t* scratch, const uint64_t* scratch_off, uint8_t* trace,
                        const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off,
-                       SwOut* out, bool traceback, hipStream_t st, bool scatter) {
+                       SwOut* out, bool traceback, hipStream_t st, bool scatter, int32_t* scores) {
   if (!njobs) return;
   const int sc = (scatter && job_idx) ? 1 : 0;
   if (traceback)
     hipLaunchKernelGGL(k_sw_generic<true>, dim3(nblk(njobs, 64)), dim3(64), 0, st, jobs, job_idx,
                        njobs, q, db, matrix32, gap_open, gap_extend, scratch, scratch_off, trace,
-                       trace_off, ops, ops_off, out, sc);
+                       trace_off, ops, ops_off, out, sc, traceback ? nullptr : scores);
   else
     hipLaunchKernelGGL(k_sw_generic<false>, dim3(nblk(njobs, 64)), dim3(64), 0, st, jobs, job_idx,
                        njobs, q, db, matrix32, gap_open, gap_extend, scratch, scratch_off, trace,
-                       trace_off, ops, ops_off, out, sc);
+                       trace_off, ops, ops_off, out, sc, traceback ? nullptr : scores);
 }
 
 }  // namespace dmr

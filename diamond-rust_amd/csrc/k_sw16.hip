@@ -272,11 +272,11 @@ bool sw16_has_variant(int cls, int lanes, int threads) {
 // split kernel of DMR_SW16_EXACT_SPLIT.
 template <class Tr>
 bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                  const int* pair_tab, const int8_t* m32, uint32_t* counter, SwOut* outs, uint32_t* redo,
+                  const int* pair_tab, const int8_t* m32, uint32_t* counter, int32_t* scores, uint32_t* redo,
                   uint32_t* nredo, int cu_count, hipStream_t st) {
 #define DMR_SW16_SPLIT_CASE(c, w, p, t)                                                                       \
   if (cls == c && P == p && T == t) {                                                                          \
-    launch_split<w, p, t, Tr, false>(lanes, nlanes, jobs, q, db, pair_tab, counter, outs, redo, nredo,       \
+    launch_split<w, p, t, Tr, false>(lanes, nlanes, jobs, q, db, pair_tab, counter, scores, redo, nredo,       \
                                      cu_count, st);                                                          \
     return true;                                                                                             \
   }
@@ -288,7 +288,7 @@ bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, co
     static const unsigned resident = resident_blocks(k_sw16<w, Tr>, 256, cu_count);                        \
     const unsigned blocks = std::min<unsigned>(resident, nblk(nlanes, 4 * 64));                               \
     hipLaunchKernelGGL((k_sw16<w, Tr>), dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db, pair_tab, \
-                       m32, counter, outs, redo, nredo);                                                      \
+                       m32, counter, scores, redo, nredo);                                                      \
     return true;                                                                                              \
   }
   switch (cls) {
@@ -305,16 +305,16 @@ bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, co
 
 bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
                  const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
-                 SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
+                 int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!sw16_has_variant(cls, lanes_per_pair, threads)) return false;
   if (!nlanes) return true;
   if (cls >= kExactClasses)
-    return launch_sw16_merged(cls, lanes_per_pair, threads, f16, lanes, nlanes, jobs, q, db, pair_tab, counter, outs,
+    return launch_sw16_merged(cls, lanes_per_pair, threads, f16, lanes, nlanes, jobs, q, db, pair_tab, counter, scores,
                               redo, nredo, cu_count, st);
   return f16 ? launch_exact<F16>(cls, lanes_per_pair, threads, lanes, nlanes, jobs, q, db, pair_tab, m32, counter,
-                                 outs, redo, nredo, cu_count, st)
+                                 scores, redo, nredo, cu_count, st)
              : launch_exact<I16>(cls, lanes_per_pair, threads, lanes, nlanes, jobs, q, db, pair_tab, m32, counter,
-                                 outs, redo, nredo, cu_count, st);
+                                 scores, redo, nredo, cu_count, st);
 }
 
 }  // namespace dmr

@@ -63,11 +63,12 @@ template <int K, bool kTrace>
 __global__ __launch_bounds__(kBT) void k_sw_block(const SwJob* __restrict__ jobs, const uint32_t* __restrict__ list,
                                                  uint32_t njobs, int wmin, SeqSet q, SeqSet db,
                                                  const int8_t* __restrict__ m32, int gap_open, int gap_extend,
-                                                 SwOut* __restrict__ outs, uint8_t* __restrict__ trace,
+                                                 SwOut* __restrict__ outs, int32_t* __restrict__ scores,
+                                                 uint8_t* __restrict__ trace,
                                                  const uint64_t* __restrict__ trace_off, uint32_t* __restrict__ ops,
                                                  const uint64_t* __restrict__ ops_off,
                                                  uint32_t* __restrict__ fallback, uint32_t* __restrict__ nfallback,
-                                                 const SwOut* __restrict__ known) {
+                                                 const int32_t* __restrict__ known) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
   __shared__ int nbH[2][kBW + 1], nbE[2][kBW + 1];  // first offset of each wave, previous row
   __shared__ int tot[2][kBW];                        // each wave's F prefix maximum
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(kBT) void k_sw_block(const SwJob* __restrict__ jobs
     hit[tid] = INT32_MAX;
   }
   __syncthreads();
-  const int stop_at = (kTrace && known) ? known[jid].score : INT32_MAX;
+  const int stop_at = (kTrace && known) ? known[jid] : INT32_MAX;
   const bool rows = rlo <= rhi && m >= 1;
   int lbest = 0, bi = 0, bj = 0;
   bool found = false;
@@ -264,8 +265,7 @@ __global__ __launch_bounds__(kBT) void k_sw_block(const SwJob* __restrict__ jobs
       if (need_exact) {
         fallback[atomicAdd(nfallback, 1u)] = jid;
       } else {
-        outs[jid].score = best;
-        outs[jid].flags = 0;
+        scores[jid] = best;
       }
     }
   } else {
@@ -360,16 +360,16 @@ int swblock_k_for(int64_t width) {
 uint64_t swblock_row_bytes(int K) { return (uint64_t)kBT * K; }
 
 void launch_sw_block(bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs, SeqSet q, SeqSet db,
-                     const int8_t* m32, int gap_open, int gap_extend, SwOut* outs, uint8_t* trace,
-                     const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback,
-                     uint32_t* nfallback, const SwOut* known, int only_k, hipStream_t st) {
+                     const int8_t* m32, int gap_open, int gap_extend, SwOut* outs, int32_t* scores,
+                     uint8_t* trace, const uint64_t* trace_off, uint32_t* ops, const uint64_t* ops_off,
+                     uint32_t* fallback, uint32_t* nfallback, const int32_t* known, int only_k, hipStream_t st) {
   if (!njobs) return;
   // One launch per K: a block serves its DP only if the width falls in that
   // K's range ((1024 * K/2, 1024 * K], the smallest from 2048), so every DP
   // runs once, with the fewest idle offsets.
 #define DMR_SWB(k, tr, lo)                                                                                    \
   hipLaunchKernelGGL((k_sw_block<k, tr>), dim3(njobs), dim3(kBT), 0, st, jobs, list, njobs, lo, q, db, m32,    \
-                     gap_open, gap_extend, outs, trace, trace_off, ops, ops_off, fallback, nfallback, known)
+                     gap_open, gap_extend, outs, scores, trace, trace_off, ops, ops_off, fallback, nfallback, known)
   if (traceback) {
     if (only_k == 0 || only_k == 4) DMR_SWB(4, true, 64 * kMaxWaveK);
     if (only_k == 0 || only_k == 8) DMR_SWB(8, true, kBT * 4);

@@ -213,13 +213,14 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
                                                  SeqSet q, SeqSet db, const int8_t* __restrict__ m32,
                                                  int gap_open, int gap_extend,
                                                  uint32_t* __restrict__ counter,
-                                                 SwOut* __restrict__ outs, uint8_t* __restrict__ trace,
+                                                 SwOut* __restrict__ outs, int32_t* __restrict__ scores,
+                                                 uint8_t* __restrict__ trace,
                                                  const uint64_t* __restrict__ trace_off,
                                                  uint32_t* __restrict__ ops,
                                                  const uint64_t* __restrict__ ops_off,
                                                  uint32_t* __restrict__ fallback,
                                                  uint32_t* __restrict__ nfallback,
-                                                 const SwOut* __restrict__ known,
+                                                 const int32_t* __restrict__ known,
                                                  const uint32_t* __restrict__ njobs_dev) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
   for (int k = threadIdx.x; k < kScoreStride * kScoreStride; k += blockDim.x) M[k] = m32[k];
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     // Trace mode with the DP's best score known (from the score pass): the
     // first cell that reaches it is the first maximum in row-major order, so
     // the rows after its row cannot change the result and are not run.
-    const int stop_at = (kTrace && known) ? uniform(known[jid].score) : INT32_MAX;
+    const int stop_at = (kTrace && known) ? uniform(known[jid]) : INT32_MAX;
     bool stop = false;
     for (int blk = rlo; rows && blk <= rhi && !stop; blk += 64) {
      const int qv = dpp<kQuadIdentity>(0, qn), ev = dpp<kQuadIdentity>(0, en);
@@ -458,8 +459,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
         fallback[atomicAdd(nfallback, 1u)] = kTrace ? task : jid;
         if (kTrace) outs[task] = SwOut{};  // no ops: the row-exact kernel redoes this DP
       } else if (!kTrace) {
-        outs[jid].score = best;
-        outs[jid].flags = 0;
+        scores[jid] = best;
       }
     }
   }
@@ -514,16 +514,16 @@ void launch_ops_pack(const SwOut* outs, const uint32_t* ops, const uint64_t* ops
 
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
-                    uint32_t* counter, SwOut* outs, uint8_t* trace, const uint64_t* trace_off,
+                    uint32_t* counter, SwOut* outs, int32_t* scores, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int cu_count, hipStream_t st, const SwOut* known, const uint32_t* njobs_dev) {
+                    int cu_count, hipStream_t st, const int32_t* known, const uint32_t* njobs_dev) {
   if (!njobs) return;
 #define DMR_SWW(k, tr)                                                                             \
   do {                                                                                             \
     static const unsigned resident = resident_blocks(k_sw_wave<k, tr>, 256, cu_count);            \
     const unsigned blocks = std::min<unsigned>(resident, nblk(njobs, 4));                          \
     hipLaunchKernelGGL((k_sw_wave<k, tr>), dim3(blocks), dim3(256), 0, st, jobs, list, njobs, q,   \
-                       db, m32, gap_open, gap_extend, counter, outs, trace, trace_off, ops,        \
+                       db, m32, gap_open, gap_extend, counter, outs, scores, trace, trace_off, ops,\
                        ops_off, fallback, nfallback, known, njobs_dev);                            \
   } while (0)
   if (traceback) {

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
                                               const int* __restrict__ pair_tab,
                                               const int8_t* __restrict__ m32,
                                               uint32_t* __restrict__ counter,
-                                              SwOut* __restrict__ outs,
+                                              int32_t* __restrict__ scores,
                                               uint32_t* __restrict__ redo, uint32_t* __restrict__ nredo) {
   __shared__ int T[kPairTab];
   for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
@@ -319,16 +319,14 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
       if (bA >= Tr::kRedo || (bA < 11 && mA + d1A < n)) {
         redo[atomicAdd(nredo, 1u)] = ln.x;
       } else {
-        outs[ln.x].score = bA;
-        outs[ln.x].flags = 0;
+        scores[ln.x] = bA;
       }
     }
     if (hasB) {
       if (bB >= Tr::kRedo || (bB < 11 && mB + d1B < n)) {
         redo[atomicAdd(nredo, 1u)] = ln.y;
       } else {
-        outs[ln.y].score = bB;
-        outs[ln.y].flags = 0;
+        scores[ln.y] = bB;
       }
     }
   }
@@ -384,7 +382,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
                                                    const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                                    const int* __restrict__ pair_tab,
                                                    uint32_t* __restrict__ counter,
-                                                   SwOut* __restrict__ outs,
+                                                   int32_t* __restrict__ scores,
                                                    uint32_t* __restrict__ redo,
                                                    uint32_t* __restrict__ nredo) {
   static_assert(W % 2 == 1, "band widths are odd");
@@ -649,16 +647,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
         if (bA >= Tr::kRedo || (bA < 11 && mA + d1A < n)) {
           redo[atomicAdd(nredo, 1u)] = ln.x;
         } else {
-          outs[ln.x].score = bA;
-          outs[ln.x].flags = 0;
+          scores[ln.x] = bA;
         }
       }
       if (hasB) {
         if (bB >= Tr::kRedo || (bB < 11 && mB + d1B < n)) {
           redo[atomicAdd(nredo, 1u)] = ln.y;
         } else {
-          outs[ln.y].score = bB;
-          outs[ln.y].flags = 0;
+          scores[ln.y] = bB;
         }
       }
     }
@@ -669,12 +665,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // resident blocks as the work needs.
 template <int W, int P, int T, class Tr, bool kWall>
 void launch_split(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab,
-                  uint32_t* counter, SwOut* outs, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
+                  uint32_t* counter, int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   const unsigned tasks = (nlanes + 64 / P - 1) / (64 / P);
   static const unsigned resident = resident_blocks(k_sw16p<W, P, T, Tr, kWall>, T, cu_count);
   const unsigned blocks = std::min<unsigned>(resident, (tasks + T / 64 - 1) / (T / 64));
   hipLaunchKernelGGL((k_sw16p<W, P, T, Tr, kWall>), dim3(blocks), dim3(T), 0, st, lanes, nlanes, jobs, q, db,
-                     pair_tab, counter, outs, redo, nredo);
+                     pair_tab, counter, scores, redo, nredo);
 }
 
 }  // namespace
