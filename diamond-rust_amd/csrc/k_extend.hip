@@ -25,38 +25,59 @@ __device__ __forceinline__ void load_matrix(const int8_t* __restrict__ m32, int8
   __syncthreads();
 }
 
-// ungapped::xdrop: left from (qa-1, sa-1), then right from (qa, sa) starting
-// at the left-hand best score; stop once best - running >= xdrop.
-__device__ __forceinline__ SegRec xdrop_dev(const int8_t* M, const uint8_t* __restrict__ qs,
-                                            int64_t qlen, const uint8_t* __restrict__ ss,
-                                            int64_t slen, int64_t qa, int64_t sa, int xdrop) {
-  int32_t score = 0, st = 0;
-  int64_t n = 1, delta = 0, len = 0;
-  int64_t qi = qa - 1, si = sa - 1;
-  while (score - st < xdrop && qi >= 0 && si >= 0) {
-    st += M[qs[qi] * kScoreStride + ss[si]];
+// ungapped::xdrop (src/ungapped.rs:38-88), one lane per extension: left
+// from (qa-1, sa-1), then right from (qa, sa) starting at the left-hand best
+// score; each direction stops once best - running >= xdrop. Steps go in
+// batches of 16 whose 32 letter loads are issued together (one memory
+// latency per batch instead of one per step); the last steps before a
+// sequence end go one at a time, so no load leaves the two sequences.
+constexpr int kXBatch = 16;
+
+// One direction: steps k = 0 .. limit-1 read q[dir*k], s[dir*k] (q and s
+// point at step 0). (score, st) carry the best and running scores in and
+// out; `at` becomes 1 + the step where the final best was first reached
+// (unchanged if no step improves it).
+template <int kDir>
+__device__ __forceinline__ void xdrop_dir(const int8_t* M, const uint8_t* __restrict__ q,
+                                          const uint8_t* __restrict__ s, int64_t limit, int xdrop,
+                                          int& score, int& st, int64_t& at) {
+  if (score - st >= xdrop) return;  // the loop test comes before every step
+  int64_t k = 0;
+  for (; k + kXBatch <= limit; k += kXBatch) {
+    uint32_t a[kXBatch], b[kXBatch];
+#pragma unroll
+    for (int j = 0; j < kXBatch; ++j) {
+      a[j] = q[kDir * (k + j)];
+      b[j] = s[kDir * (k + j)];
+    }
+#pragma unroll
+    for (int j = 0; j < kXBatch; ++j) {
+      st += M[a[j] * kScoreStride + b[j]];
+      if (st > score) {
+        score = st;
+        at = k + j + 1;
+      }
+      if (score - st >= xdrop) return;
+    }
+  }
+  for (; k < limit; ++k) {
+    st += M[q[kDir * k] * kScoreStride + s[kDir * k]];
     if (st > score) {
       score = st;
-      delta = n;
+      at = k + 1;
     }
-    --qi;
-    --si;
-    ++n;
+    if (score - st >= xdrop) return;
   }
-  qi = qa;
-  si = sa;
+}
+
+__device__ __forceinline__ SegRec xdrop_dev(const int8_t* M, const uint8_t* __restrict__ qs, int64_t qlen,
+                                            const uint8_t* __restrict__ ss, int64_t slen, int64_t qa, int64_t sa,
+                                            int xdrop) {
+  int score = 0, st = 0;
+  int64_t delta = 0, len = 0;
+  xdrop_dir<-1>(M, qs + qa - 1, ss + sa - 1, min(qa, sa), xdrop, score, st, delta);
   st = score;
-  n = 1;
-  while (score - st < xdrop && qi < qlen && si < slen) {
-    st += M[qs[qi] * kScoreStride + ss[si]];
-    if (st > score) {
-      score = st;
-      len = n;
-    }
-    ++qi;
-    ++si;
-    ++n;
-  }
+  xdrop_dir<1>(M, qs + qa, ss + sa, min(qlen - qa, slen - sa), xdrop, score, st, len);
   SegRec sg;
   sg.q_start = (uint32_t)(qa - delta);
   sg.s_start = (uint32_t)(sa - delta);
@@ -115,6 +136,8 @@ __global__ void k_copy_u64(const uint64_t* __restrict__ src, uint64_t* __restric
   if (threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
 }
 
+// Segments of the sorted hits: one lane per (target, subject, diagonal) run,
+// walking the run's hits in order as one thread of the reference does.
 __global__ void k_xdrop_runs(const uint64_t* __restrict__ hits, const uint32_t* __restrict__ run_start,
                              uint64_t nruns, uint64_t nhits, HitGeom g, SeqSet q, SeqSet db,
                              const int8_t* __restrict__ m32, int xdrop, int cutoff,
@@ -144,10 +167,8 @@ __global__ void k_xdrop_runs(const uint64_t* __restrict__ hits, const uint32_t* 
       keep[h] = 0;
       continue;
     }
-    int64_t qa = diag + spos, sa = spos;
-    SegRec sg = xdrop_dev(M, qs, qlen, ss, slen, qa, sa, xdrop);
-    int32_t score = sg.score;
-    if (score >= cutoff) {
+    SegRec sg = xdrop_dev(M, qs, qlen, ss, slen, diag + spos, spos, xdrop);
+    if (sg.score >= cutoff) {
       segs[h] = sg;
       keep[h] = 1;
       has_last = true;

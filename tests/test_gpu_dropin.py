@@ -91,8 +91,19 @@ def test_xdrop_batch(ctx, orc):
         got = ctx.xdrop_batch(q, s, qa, sa, xd)
         for k in range(50):
             assert tuple(int(x) for x in got[k]) == orc.xdrop(q, s, int(qa[k]), int(sa[k]), xd)
-    q = orc.encode("WWWWWWWWWWCCCCCCCCCC")  # src/ungapped.rs:328-337
+    q = orc.encode("WWWWWWWWWWCCCCCCCCCC")  # src/ungapped.rs:99-129
     assert ctx.xdrop(q, orc.encode("WWWWWWWWWWDDDDDDDDDD"), 0, 0, 5)[2:] == (10, 110)
+    # Long extensions (many 16-step batches, then single steps up to both
+    # sequence ends), anchors at and next to the ends, and an X-drop of 0 or
+    # less, where the reference's loop test fails before the first step.
+    q = rand_seq(rng, 1500)
+    s = np.concatenate([rand_seq(rng, 37), mutate(rng, q, 0.1, 0.0), rand_seq(rng, 21)])
+    qa = np.array([0, 1, 15, 16, 17, 700, 1499, 1500, 3, 1000], np.uint32)
+    sa = qa + 37
+    for xd in (-3, 0, 1, 20, 60, 400):
+        got = ctx.xdrop_batch(q, s, qa, sa, xd)
+        for k in range(len(qa)):
+            assert tuple(int(x) for x in got[k]) == orc.xdrop(q, s, int(qa[k]), int(sa[k]), xd)
 
 
 @pytest.mark.parametrize("code,red,cut", [
