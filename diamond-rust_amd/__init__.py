@@ -31,7 +31,7 @@ DMR_E_TIMEOUT = -6
 EXPORTS = [
     "dmr_search_config_default", "dmr_last_error", "dmr_version", "dmr_ctx_create",
     "dmr_ctx_set_tuning", "dmr_ctx_get_tuning", "dmr_ctx_load_tuning", "dmr_sw16_variants",
-    "dmr_ctx_destroy", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
+    "dmr_ctx_destroy", "dmr_ctx_share_db", "dmr_device_count", "dmr_band_for", "dmr_evalue", "dmr_seed_build",
     "dmr_seed_join", "dmr_xdrop", "dmr_xdrop_batch", "dmr_extend_span", "dmr_extend_span_noalloc", "dmr_extend",
     "dmr_extend_span_batch", "dmr_db_load", "dmr_db_load_file", "dmr_db_ids", "dmr_db_headers", "dmr_db_build_index", "dmr_blastp", "dmr_blastx",
     "dmr_six_frames", "dmr_queries_load", "dmr_search_loaded", "dmr_blastp_plain", "dmr_last_class_profile",
@@ -160,6 +160,7 @@ def lib():
         sig = {
             "dmr_last_error": ([], C.c_char_p), "dmr_version": ([], C.c_char_p),
             "dmr_ctx_create": ([C.c_int, P], C.c_int), "dmr_ctx_destroy": ([P], None),
+            "dmr_ctx_share_db": ([P, P], C.c_int),
             "dmr_device_count": ([P], C.c_int), "dmr_band_for": ([U64], U32),
             "dmr_ctx_set_tuning": ([P, C.c_char_p, D], C.c_int),
             "dmr_ctx_get_tuning": ([P, C.c_char_p, P], C.c_int),
@@ -430,6 +431,12 @@ class Context:
         return v.value
 
     # -- database / search ------------------------------------------------
+    def share_db(self, other: "Context"):
+        """dmr_ctx_share_db: use other's resident database and seed indexes
+        (no copy); the two contexts may then search concurrently."""
+        _check(lib().dmr_ctx_share_db(self._h, other._h))
+        self._db = getattr(other, "_db", None)
+
     def db_load(self, letters, offsets):
         letters = np.ascontiguousarray(letters, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)

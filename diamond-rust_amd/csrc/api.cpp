@@ -379,6 +379,19 @@ int dmr_ctx_create(int device, dmr_ctx** out) {
 
 void dmr_ctx_destroy(dmr_ctx* ctx) { delete ctx; }
 
+int dmr_ctx_share_db(dmr_ctx* ctx, const dmr_ctx* from) {
+  if (!ctx || !from) return fail(DMR_E_ARG, "NULL context");
+  if (ctx == from) return DMR_OK;
+  return guard([&]() -> int {
+    ctx->eng->share_db(*from->eng);
+    ctx->db_ids = from->db_ids;
+    ctx->db_headers = from->db_headers;
+    ctx->db_has_ids = from->db_has_ids;
+    ctx->db_header_nl = from->db_header_nl;
+    return DMR_OK;
+  });
+}
+
 int dmr_ctx_set_tuning(dmr_ctx* ctx, const char* key, double value) {
   if (!ctx || !key) return fail(DMR_E_ARG, "NULL argument");
   return guard([&] {

@@ -103,6 +103,16 @@ const char* dmr_version(void);
 /* Context: one HIP device, one stream, the resident database and seed index. */
 int dmr_ctx_create(int device, dmr_ctx** out);
 void dmr_ctx_destroy(dmr_ctx* ctx);
+/* ctx takes the resident database (letters, ids, headers) and seed indexes
+ * of `from` (same device) in place of its own, with no copy; no counterpart
+ * in the reference, whose rayon workers share one &Database
+ * (src/search.rs:135-145). Contexts sharing a database may search
+ * concurrently from different host threads, each with its own query batch
+ * and stream, so one batch's host work and latency-bound kernels overlap
+ * another's score pass; an index missing for a preset is built once, under
+ * the database's lock. The database lives while some context holds it; a
+ * later dmr_db_load* on either context replaces only that context's. */
+int dmr_ctx_share_db(dmr_ctx* ctx, const dmr_ctx* from);
 int dmr_device_count(int* n);
 
 /* Engine settings of a context; no counterpart in the reference (none of

@@ -100,3 +100,62 @@ def test_cap_counts_each_request(dmr, orc):
         srv.wait(10**9, 0)
     srv.close()
     ctx.close()
+
+
+def test_shared_database_concurrent_searches(dmr, synth):
+    """Contexts sharing one resident database (dmr_ctx_share_db) search their
+    own query batches from concurrent host threads; every result equals the
+    same batch searched on a context with its own copy of the database. The
+    owner builds only the --sensitive index, so the sharers build the other
+    two presets' indexes themselves, concurrently (once each, under the
+    database's lock). After the owner loads another database the sharers
+    keep searching theirs."""
+    import threading
+    w = synth.make("tiny")
+    ref = dmr.Context(0)
+    ref.db_load(w.db_letters, w.db_offsets)
+    owner = dmr.Context(0)
+    owner.db_load(w.db_letters, w.db_offsets)
+    owner.build_index("sensitive")
+    subs = [w.subset_queries(list(range(k, w.n_q, 3))) for k in range(3)]
+    cfgs = [dmr.SearchConfig(sensitivity=s) for s in ("sensitive", "default", "very-sensitive")]
+    alone = [[ref.blastp(s.q_letters, s.q_offsets, c).format(s.q_ids, w.db_ids, EXTENDED) for c in cfgs]
+             for s in subs]
+    ref.close()
+    sharers = []
+    for s in subs:
+        c = dmr.Context(0)
+        c.share_db(owner)
+        c.queries_load(s.q_letters, s.q_offsets)
+        sharers.append(c)
+    got = [[None] * len(cfgs) for _ in subs]
+    errors = []
+
+    def run(k):
+        order = list(range(len(cfgs)))
+        if k % 2:
+            order.reverse()
+        try:
+            for rep in range(3):
+                for ci in order:
+                    t = sharers[k].search_loaded(cfgs[ci]).format(subs[k].q_ids, w.db_ids, EXTENDED)
+                    if got[k][ci] is None:
+                        got[k][ci] = t
+                    elif got[k][ci] != t:
+                        errors.append((k, ci, rep))
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(len(subs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors
+    assert got == alone
+    assert all(any(t for t in row) for row in alone)
+    owner.db_load(w.db_letters[:1000], np.array([0, 500, 1000], np.uint64))
+    for k, c in enumerate(sharers):
+        assert c.search_loaded(cfgs[0]).format(subs[k].q_ids, w.db_ids, EXTENDED) == alone[k][0]
+        c.close()
+    owner.close()
