@@ -1,8 +1,9 @@
 """Autotuner of the score-pass kernels: picks, on the device, the packed
 kernel of every width class (lanes per DP pair and threads per block among
 the compiled ones), the merged-band width buckets in use, the arithmetic
-(f16 or int16 pairs), the stream plan and whether the chain before the
-score pass runs as one hipGraph, for a search preset on a workload,
+(f16 or int16 pairs), which sequence runs along a packed DP's rows, the
+stream plan and whether the chain before the score pass runs as one
+hipGraph, for a search preset on a workload,
 and writes the choice with every trial's timing to the tuning file that the
 library loads at context creation (tuning_<arch>.json beside libdmr.so).
 
@@ -12,9 +13,10 @@ equal the first one's, or the tuner stops.
 Search: coordinate descent from the built-in defaults. Each step tries the
 values of one setting with the others held, and keeps a value only when its
 median search time beats the best so far by more than --min-gain (noise
-guard): f16; the graph; the merged-band kernels in use (one bucket bound added or
-removed at a time, until no change helps); the kernel of each class that
-has DPs in the workload; the stream plan; the merged buckets once more.
+guard): f16; the graph; the merged-band kernels in use (one bucket bound
+added or removed at a time, until no change helps); the row sequence
+(transpose_<class>) and the kernel of each class that has DPs in the
+workload; the stream plan; the merged buckets once more.
 
 Usage (on a GPU box; the workloads are the seeded synthetic ones):
   python diamond-rust_amd/autotune.py                       # every preset below
@@ -123,6 +125,10 @@ class Tuner:
         for name in self.dmr.PACKED_CLASSES:
             if prof[name]["jobs"] == 0:
                 continue
+            key = f"transpose_{name}"
+            for t in range(3):
+                if t != cur[key]:
+                    consider({**cur, key: t}, {key: t})
             for v in variants:
                 if v["cls"] != name or (v["lanes"], v["threads"]) == (cur[f"lanes_{name}"], cur[f"threads_{name}"]):
                     continue

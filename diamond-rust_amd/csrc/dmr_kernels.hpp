@@ -161,36 +161,47 @@ struct Sw16Variant {
 };
 const std::vector<Sw16Variant>& sw16_variants();
 bool sw16_has_variant(int cls, int lanes, int threads);
-void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits, uint32_t merged_map,
-                     uint64_t* keys, uint32_t* idx, hipStream_t st, const uint64_t* njobs_dev = nullptr);
+// Score-pass sort keys (class | orientation | row sequence | rows, see
+// k_classify). seq_bits: bits of the larger of a target and a subject index;
+// transpose: per packed class c, bits 2c..2c+1 say which sequence runs along
+// the rows of its DPs: 0 the query, 1 the one giving fewer rows, 2 the
+// subject (every result is the same).
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int seq_bits, uint32_t merged_map,
+                     uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st,
+                     const uint64_t* njobs_dev = nullptr);
 // Lane pairing of the sorted score-pass jobs (see k_lane_heads): head flags
 // and class ranges, then lanes at the heads' scan positions, then the lanes
-// per class. summary holds 2 * (nclasses + npacked) + 1 words.
+// per class. summary holds 2 * (nclasses + npacked) + 1 words, then at
+// tfirst and trow npacked words each (first job with the subject on the rows,
+// lanes with the query on the rows).
 void launch_lane_heads(const uint64_t* keys, const uint32_t* flags, const uint64_t* pos, const uint32_t* run_start,
-                       uint64_t n, int cls_shift, int npacked, int nclasses, uint32_t* heads, uint32_t* summary,
-                       hipStream_t st, const uint64_t* n_dev = nullptr);
+                       uint64_t n, int cls_shift, int npacked, int nclasses, int tfirst, uint32_t* heads,
+                       uint32_t* summary, hipStream_t st, const uint64_t* n_dev = nullptr);
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const uint32_t* heads,
                        const uint64_t* hpos, uint64_t n, uint2* lanes, hipStream_t st,
                        const uint64_t* n_dev = nullptr);
-void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
-                         hipStream_t st);
-// Lane order of the packed classes: key = class << kLaneSpanBits | (kLaneSpanMax
-// - row span), so a sort puts each class's longest lanes first.
+void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, int tfirst, int trow,
+                         uint32_t* summary, hipStream_t st);
+// Lane order of the packed classes: key = class << (kLaneSpanBits + 1) |
+// orientation << kLaneSpanBits | (kLaneSpanMax - row span), so a sort keeps
+// each class's query-row lanes first and puts the longest lanes of each
+// orientation first.
 constexpr int kLaneSpanBits = 13;
 constexpr uint32_t kLaneSpanMax = (1u << kLaneSpanBits) - 1;
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                      uint32_t merged_map, uint64_t* keys, uint32_t* idx, hipStream_t st);
+                      uint32_t merged_map, uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st);
 // Packed score pass of class cls over nlanes lanes (k_sw16.hip, merged
 // classes k_sw16m.hip) with the compiled kernel (cls, lanes_per_pair,
-// threads) of sw16_variants(); false when there is none. f16 = half-precision
-// arithmetic (pair_tab must then be the f16 table).
+// threads) of sw16_variants(); false when there is none or nrow > nlanes.
+// Lanes [0, nrow) have the query on the rows, the rest the subject. f16 =
+// half-precision arithmetic (pair_tab must then be the f16 table).
 bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
-                 const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
-                 int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
+                 uint32_t nrow, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32,
+                 uint32_t* counter, int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 bool launch_sw16_merged(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
-                        const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter, int32_t* scores,
-                        uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
+                        uint32_t nrow, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter,
+                        int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
 void launch_evalue_filter(const SwJob* jobs, const int32_t* scores, uint64_t njobs, SeqSet q,
                           SeqSet db, AlpParams P, uint64_t db_letters, double threshold,
                           uint32_t* flags, hipStream_t st);
@@ -213,8 +224,8 @@ unsigned resident_blocks(Kernel kernel, int threads, int cu_count) {
   return (unsigned)per_cu * (unsigned)(cu_count > 0 ? cu_count : 1);
 }
 
-// Score-pass sort key of a DP job: class | target | row field, the row field
-// being (row span / 32, first row / 32), 9 bits each.
+// Score-pass sort key of a DP job: class | orientation | row sequence | row
+// field, the row field being (row span / 32, first row / 32), 9 bits each.
 constexpr int kRowKeyBits = 18;
 
 // Wave-per-DP int32 kernel (k_swwave.hip): K offsets per lane, band width

@@ -813,7 +813,7 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
     scores = scores_.ensure<int32_t>(nhits + 1);
     stage_event(6);
     SS = StageStats();
-    return score_pass(jobs, nhits, (int)g.bt, qs, dbs, opt, scores, &SS, dc + kCountJobs);
+    return score_pass(jobs, nhits, (int)std::max(g.bt, g.bs), qs, dbs, opt, scores, &SS, dc + kCountJobs);
   };
   uint64_t njobs = 0;
   bool direct = true;
@@ -1028,11 +1028,10 @@ void Engine::search_slice(const SearchOptions& opt, const AlpParams& P, const Hi
 // packed int16 kernel (two DPs of one query per lane), the rest (merged or
 // clipped bands, other gap costs) and any int16-saturated DP to the exact
 // int32 kernel.
-uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
+uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, SeqSet qs, SeqSet dbs,
                             const SearchOptions& opt, int32_t* scores, StageStats* S, const uint64_t* njobs_dev) {
   for (int c = 0; c < kClasses; ++c) S->ms_class[c] = 0;
   if (!njobs) return 0;
-  last_target_bits_ = target_bits;
   layout_ = resolve_layout(opt.sensitivity);
   layout_preset_ = opt.sensitivity;
   const ScoreLayout L = layout_;
@@ -1043,16 +1042,17 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
   uint64_t* keys = sw_keys_.ensure<uint64_t>(njobs + 1);
   uint32_t* idx = sw_idx_.ensure<uint32_t>(njobs + 1);
   // With njobs_dev, njobs bounds the count until the summary read below.
-  launch_classify(jobs, njobs, qs, dbs, target_bits, mmap, keys, idx, st_, njobs_dev);
+  const uint32_t transpose = fast ? transpose_modes(L) : 0u;
+  launch_classify(jobs, njobs, qs, dbs, seq_bits, mmap, transpose, keys, idx, st_, njobs_dev);
   static_assert(kClasses <= 32, "the class takes 5 key bits");
-  const int key_bits = 5 + target_bits + kRowKeyBits;
+  const int key_bits = 5 + 1 + seq_bits + kRowKeyBits;
   radix_sort_pairs(keys, idx, sw_keys2_.ensure<uint64_t>(njobs + 1), sw_idx2_.ensure<uint32_t>(njobs + 1),
                    njobs, key_bits, b_hist_.ensure<uint32_t>(radix_hist_elems(njobs)),
                    b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(njobs)), st_, njobs_dev);
-  // Runs of equal (class, target) and their starts; lane heads (even
-  // positions of a run in a packed class), their scan, the lanes and the
-  // per-class summary, all on the device; one small read brings the class
-  // ranges and lane counts back.
+  // Runs of equal (class, orientation, row sequence) and their starts; lane
+  // heads (even positions of a run in a packed class), their scan, the lanes
+  // and the per-class summary, all on the device; one small read brings the
+  // class ranges and lane counts back.
   uint32_t* fl = flags_.ensure<uint32_t>(njobs + 1);
   launch_key_flags(keys, njobs, kRowKeyBits, fl, st_, njobs_dev);
   launch_fill_u32(fl + njobs, 1, 0, st_);
@@ -1060,19 +1060,23 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
   scan_exclusive_u32_to_u64(fl, ps, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
   uint32_t* rs = runs_.ensure<uint32_t>(njobs + 1);
   launch_compact_flags(fl, ps, njobs, rs, st_);
-  const int cls_shift = target_bits + kRowKeyBits;
+  const int cls_shift = seq_bits + 1 + kRowKeyBits;
   const int npacked = fast ? kPackedClasses : 0;
-  constexpr int kSummary = 2 * (kClasses + kPackedClasses) + 1;
+  // Summary: class ranges, lane ranges and the lane total, then per packed
+  // class the query-row lane count (kSumRow) and the first subject-row job
+  // (kSumTr).
+  constexpr int kSumRow = 2 * (kClasses + kPackedClasses) + 1, kSumTr = kSumRow + kPackedClasses;
+  constexpr int kSummary = kSumTr + kPackedClasses;
   uint32_t* summ = sw_summary_.ensure<uint32_t>(kSummary);
   launch_fill_u32(summ, kSummary, 0, st_);
   uint32_t* heads = sw_heads_.ensure<uint32_t>(njobs + 1);
-  launch_lane_heads(keys, fl, ps, rs, njobs, cls_shift, npacked, kClasses, heads, summ, st_, njobs_dev);
+  launch_lane_heads(keys, fl, ps, rs, njobs, cls_shift, npacked, kClasses, kSumTr, heads, summ, st_, njobs_dev);
   launch_fill_u32(heads + njobs, 1, 0, st_);
   uint64_t* hpos = sw_hpos_.ensure<uint64_t>(njobs + 1);
   scan_exclusive_u32_to_u64(heads, hpos, njobs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(njobs + 1)), st_);
   uint2* paired = sw_lanes2_.ensure<uint2>(njobs + 1);
   launch_make_lanes(idx, fl, heads, hpos, njobs, paired, st_, njobs_dev);
-  launch_lane_summary(hpos, njobs, npacked, kClasses, summ, st_);
+  launch_lane_summary(hpos, njobs, npacked, kClasses, kSumTr, kSumRow, summ, st_);
   uint32_t* h_summ = sw_pin_summary_.ensure<uint32_t>(kSummary);
   DMR_HIP(hipMemcpyAsync(h_summ, summ, kSummary * 4, hipMemcpyDeviceToHost, st_));
   uint64_t* h_njobs = sw_pin_count_.ensure<uint64_t>(1);
@@ -1090,10 +1094,11 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
     cbeg[c] = h_summ[c];
     cend[c] = h_summ[kClasses + c];
   }
-  uint32_t class_lanes[kPackedClasses] = {}, class_lane0[kPackedClasses] = {};
+  uint32_t class_lanes[kPackedClasses] = {}, class_lane0[kPackedClasses] = {}, class_rows[kPackedClasses] = {};
   for (int c = 0; c < npacked; ++c) {
     class_lane0[c] = h_summ[2 * kClasses + c];
     class_lanes[c] = h_summ[2 * kClasses + npacked + c];
+    class_rows[c] = h_summ[kSumRow + c];
   }
   const uint32_t nl = h_summ[2 * (kClasses + npacked)];
   // Counters of the score pass: the redo list's length, one task counter per
@@ -1114,15 +1119,15 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
     // key), so class_lane0/class_lanes still hold.
     uint64_t* lk = sw_keys_.as<uint64_t>();
     uint32_t* li = sw_idx2_.as<uint32_t>();
-    launch_lane_keys(paired, nl, jobs, qs, dbs, mmap, lk, li, st_);
+    launch_lane_keys(paired, nl, jobs, qs, dbs, mmap, transpose, lk, li, st_);
     static_assert(kPackedClasses <= 16, "the lane class takes 4 key bits");
     radix_sort_pairs(lk, li, sw_keys2_.as<uint64_t>(), sw_lane_idx_.ensure<uint32_t>(nl + 1), nl,
-                     4 + kLaneSpanBits, b_hist_.ensure<uint32_t>(radix_hist_elems(nl)),
+                     4 + 1 + kLaneSpanBits, b_hist_.ensure<uint32_t>(radix_hist_elems(nl)),
                      b_scan_.ensure<uint64_t>(radix_scan_tmp_elems(nl)), st_);
     lanes = sw_lanes_.ensure<uint2>(nl + 1);
     launch_gather_lanes(paired, li, nl, lanes, st_);
     if (lane_report_ && qs.offs == q_offs_.as<uint32_t>())
-      report_lanes(jobs, njobs, lanes, nl, class_lane0, class_lanes, L);
+      report_lanes(jobs, njobs, lanes, nl, class_lane0, class_lanes, class_rows, L);
   }
   // The class kernels are independent (distinct jobs and counters; the redo
   // and fallback lists are filled through atomics): they run side by side on
@@ -1134,9 +1139,9 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
   auto launch_packed = [&](int c, hipStream_t s) {
     if (!nl || !class_lanes[c]) return;
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
-    if (!launch_sw16(c, L.lanes[c], L.threads[c], L.f16, lanes + class_lane0[c], class_lanes[c], jobs, qs, dbs,
-                     (L.f16 ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(), cnt + kCntPacked + c,
-                     scores, redo, cnt + kCntRedo, cus, s))
+    if (!launch_sw16(c, L.lanes[c], L.threads[c], L.f16, lanes + class_lane0[c], class_lanes[c], class_rows[c],
+                     jobs, qs, dbs, (L.f16 ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(),
+                     cnt + kCntPacked + c, scores, redo, cnt + kCntRedo, cus, s))
       throw Error(-1, "no compiled packed kernel for class " + packed_class_name(c));
     DMR_HIP(hipEventRecord(ev_[13 + 2 * c], s));
   };
@@ -1262,20 +1267,22 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, 
 // Diagnostics (DMR_LANE_REPORT=1): per packed class, the band cells of its
 // DPs against the cell slots its lanes and waves step through (a lane runs
 // the union of its two DPs' rows; a wave runs as many rows as its longest
-// lane or lane group).
+// lane or lane group; the first class_rows lanes of a class have the query
+// on the rows, the others the subject, and no wave mixes the two).
 void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
-                          const uint32_t* class_lane0, const uint32_t* class_lanes, const ScoreLayout& L) {
+                          const uint32_t* class_lane0, const uint32_t* class_lanes, const uint32_t* class_rows,
+                          const ScoreLayout& L) {
   std::vector<SwJob> hj(njobs);
   std::vector<uint2> hl(nl);
   DMR_HIP(hipMemcpyAsync(hj.data(), jobs, njobs * sizeof(SwJob), hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipMemcpyAsync(hl.data(), lanes, nl * sizeof(uint2), hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));
-  auto rows = [&](uint32_t k, int64_t* lo, int64_t* hi) {
+  auto rows = [&](uint32_t k, bool tr, int64_t* lo, int64_t* hi) {
     const SwJob& j = hj[k];
     const int64_t n = (int64_t)j.q1 - j.q0;
     const int64_t m = (int64_t)db_->h_offs[j.subject + 1] - db_->h_offs[j.subject];
-    *lo = std::max<int64_t>(1, (int64_t)j.d0 + 1);
-    *hi = std::min<int64_t>(n, m + j.d1);
+    *lo = tr ? std::max<int64_t>(1, 1 - (int64_t)j.d1) : std::max<int64_t>(1, (int64_t)j.d0 + 1);
+    *hi = tr ? std::min<int64_t>(m, n - (int64_t)j.d0) : std::min<int64_t>(n, m + j.d1);
   };
   for (int c = 0; c < kPackedClasses; ++c) {
     if (!class_lanes[c]) continue;
@@ -1286,10 +1293,16 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
     int64_t wave_max = 0, in_wave = 0;
     for (uint32_t t = 0; t < class_lanes[c]; ++t) {
       const uint2 ln = hl[class_lane0[c] + t];
+      const bool tr = t >= class_rows[c];
+      if (t == class_rows[c] && in_wave) {  // the subject-row lanes start a wave
+        wave_slots += (uint64_t)wave_max * W * 2 * per_wave;
+        wave_max = 0;
+        in_wave = 0;
+      }
       int64_t lo = INT64_MAX, hi = INT64_MIN, a, b;
       for (uint32_t k : {ln.x, ln.y}) {
         if (k == ~0u) continue;
-        rows(k, &a, &b);
+        rows(k, tr, &a, &b);
         band += (uint64_t)std::max<int64_t>(0, b - a + 1) * W;
         lo = std::min(lo, a);
         hi = std::max(hi, b);
@@ -1303,8 +1316,9 @@ void Engine::report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes,
         in_wave = 0;
       }
     }
-    std::fprintf(stderr, "[lanes] class %d (W %lld, %d lanes per pair): %u pairs, band cells %.3g, pair slots %.3g (x%.2f), "
-                 "wave slots %.3g (x%.2f)\n", c, (long long)W, P, class_lanes[c],
+    std::fprintf(stderr, "[lanes] class %d (W %lld, %d lanes per pair): %u pairs (%u subject rows), band cells %.3g, "
+                 "pair slots %.3g (x%.2f), wave slots %.3g (x%.2f)\n", c, (long long)W, P, class_lanes[c],
+                 class_lanes[c] - class_rows[c],
                  (double)band, (double)pair_slots, (double)pair_slots / std::max<double>(1, band),
                  (double)wave_slots, (double)wave_slots / std::max<double>(1, band));
   }

@@ -372,7 +372,7 @@ class Engine {
   // Score pass over the jobs; njobs_dev: njobs is a bound and the count is
   // read on the device (the pass's one host read brings it back). Returns the
   // job count.
-  uint64_t score_pass(const SwJob* jobs, uint64_t njobs, int target_bits, SeqSet qs, SeqSet dbs,
+  uint64_t score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, SeqSet qs, SeqSet dbs,
                       const SearchOptions& opt, int32_t* scores, StageStats* S, const uint64_t* njobs_dev = nullptr);
   void search_slice(const SearchOptions& opt, const AlpParams& P, const HitGeom& g, const uint64_t* hits,
                     uint64_t nhits, SeqSet qs, SeqSet dbs, StageStats* S,
@@ -482,9 +482,9 @@ class Engine {
   // then the pinned keys (pinned: in place of pinned_).
   ScoreLayout resolve_layout(const std::string& preset, const Settings* pinned = nullptr) const;
   void report_lanes(const SwJob* jobs, uint64_t njobs, const uint2* lanes, uint32_t nl,
-                    const uint32_t* class_lane0, const uint32_t* class_lanes, const ScoreLayout& L);
+                    const uint32_t* class_lane0, const uint32_t* class_lanes, const uint32_t* class_rows,
+                    const ScoreLayout& L);
   uint64_t last_njobs_ = 0;
-  int last_target_bits_ = 0;
   double last_ms_class_[kClasses] = {};
 
  public:

@@ -24,15 +24,21 @@ __device__ __forceinline__ int packed_class(int w, uint32_t merged) {
   }
 }
 
-// Sort key per job: class | target | rows. Classes below kPackedClasses:
-// packed 16-bit kernels (packed_class); then kWaveKinds wave-per-DP int32
-// classes with K = 1..32 offsets per lane (widths up to 64 * K); the last:
-// wider bands (block-per-DP and row-exact int32 kernels). Within a target the jobs are ordered by row span,
-// then first row (32-row buckets): the packed kernels pair consecutive jobs
-// in a lane, which runs the union of their rows, and a wave runs as many
-// rows as its longest lane, so neighbours should start and end alike.
+// Sort key per job: class | orientation | row sequence | rows. Classes below
+// kPackedClasses: packed 16-bit kernels (packed_class); then kWaveKinds
+// wave-per-DP int32 classes with K = 1..32 offsets per lane (widths up to
+// 64 * K); the last: wider bands (block-per-DP and row-exact int32 kernels).
+// A packed job runs with the query or the subject on the rows (subject_rows
+// with its class's mode, bits 2c..2c+1 of transpose; the other classes
+// always with the query), and the
+// packed kernels pair two jobs with the same row sequence in a lane, so the
+// key groups the jobs of one class and orientation by their row sequence
+// (target or subject, seq_bits bits). Within a run they are ordered by row
+// span and first row (32-row buckets): a lane runs the union of its two
+// DPs' rows, and a wave as many rows as its longest lane, so neighbours
+// should start and end alike.
 __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSet q, SeqSet db,
-                           int target_bits, uint32_t merged, uint64_t* __restrict__ keys,
+                           int seq_bits, uint32_t merged, uint32_t transpose, uint64_t* __restrict__ keys,
                            uint32_t* __restrict__ idx, const uint64_t* __restrict__ njobs_dev) {
   uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (njobs_dev ? *njobs_dev : njobs)) return;
@@ -54,22 +60,34 @@ __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSe
         break;
       }
   }
-  const int64_t span = rhi - rlo + 1;
+  const bool tr = cls < kPackedClasses && subject_rows(n, m, j.d0, j.d1, (transpose >> (2 * cls)) & 3u);
+  // Rows in the job's orientation.
+  const int64_t lo = tr ? max((int64_t)1, (int64_t)(1 - j.d1)) : rlo;
+  const int64_t hi = tr ? min(m, n - (int64_t)j.d0) : rhi;
+  const int64_t span = hi - lo + 1;
   const uint64_t sb = (uint64_t)(span < 0 ? 0 : (span > 16383 ? 16383 : span)) >> 5;
-  const uint64_t lb = (uint64_t)(rlo > 16383 ? 16383 : rlo) >> 5;
-  keys[k] = ((uint64_t)cls << (target_bits + kRowKeyBits)) | ((uint64_t)j.target << kRowKeyBits) |
-            (sb << 9) | lb;
+  const uint64_t lb = (uint64_t)(lo > 16383 ? 16383 : lo) >> 5;
+  const uint64_t seq = tr ? j.subject : j.target;
+  // With the subject on the rows, two DPs of one subject (from different
+  // queries) may cover distant parts of it: those runs are ordered by first
+  // row, then span, so a lane's two row ranges overlap.
+  const uint64_t rows = tr ? (lb << 9) | sb : (sb << 9) | lb;
+  keys[k] = ((uint64_t)cls << (seq_bits + 1 + kRowKeyBits)) | ((uint64_t)tr << (seq_bits + kRowKeyBits)) |
+            (seq << kRowKeyBits) | rows;
   idx[k] = (uint32_t)k;
 }
 
 // One thread per sorted job. A job heads a lane when it sits at an even
-// position of its (class, target) run (run r of job i is pos[i] + flags[i] - 1,
-// starting at run_start[r]) in a packed class (class < npacked); the class
-// ranges of the sorted order go to the summary: summary[c] = first job of
-// class c, summary[nclasses + c] = one past its last (absent classes keep 0).
+// position of its (class, orientation, row sequence) run (run r of job i is
+// pos[i] + flags[i] - 1, starting at run_start[r]) in a packed class (class <
+// npacked); the class ranges of the sorted order go to the summary:
+// summary[c] = first job of class c, summary[nclasses + c] = one past its
+// last (absent classes keep 0), and summary[tfirst + c] = 1 + the first job
+// of class c with the subject on the rows (the orientation bit lies right
+// below the class; 0: none).
 __global__ void k_lane_heads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flags,
                              const uint64_t* __restrict__ pos, const uint32_t* __restrict__ run_start,
-                             uint64_t n, int cls_shift, int npacked, int nclasses,
+                             uint64_t n, int cls_shift, int npacked, int nclasses, int tfirst,
                              uint32_t* __restrict__ heads, uint32_t* __restrict__ summary,
                              const uint64_t* __restrict__ n_dev) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -82,11 +100,14 @@ __global__ void k_lane_heads(const uint64_t* __restrict__ keys, const uint32_t* 
     }
   }
   auto cls_of = [&](uint64_t k) { return (int)min<uint64_t>(keys[k] >> cls_shift, (uint64_t)(nclasses - 1)); };
+  auto tr_of = [&](uint64_t k) { return (int)((keys[k] >> (cls_shift - 1)) & 1u); };
   const int c = cls_of(i);
   const uint64_t r = pos[i] + flags[i] - 1;
   heads[i] = (c < npacked && ((i - run_start[r]) & 1) == 0) ? 1u : 0u;
   if (i == 0 || cls_of(i - 1) != c) summary[c] = (uint32_t)i;
   if (i + 1 == n || cls_of(i + 1) != c) summary[nclasses + c] = (uint32_t)(i + 1);
+  if (c < npacked && tr_of(i) && (i == 0 || cls_of(i - 1) != c || !tr_of(i - 1)))
+    summary[tfirst + c] = (uint32_t)(i + 1);
 }
 
 // One thread per sorted job: a lane head pairs itself with the next job of
@@ -102,40 +123,48 @@ __global__ void k_make_lanes(const uint32_t* __restrict__ sorted_idx, const uint
 
 // Lanes per packed class from the class ranges: summary[2 * nclasses + c] =
 // first lane of class c, summary[2 * nclasses + npacked + c] = its lane
-// count, summary[2 * (nclasses + npacked)] = all lanes.
+// count, summary[2 * (nclasses + npacked)] = all lanes, summary[trow + c] =
+// the lanes of class c with the query on the rows (they come first).
 __global__ void k_lane_summary(const uint64_t* __restrict__ hpos, uint64_t n, int npacked, int nclasses,
-                               uint32_t* __restrict__ summary) {
+                               int tfirst, int trow, uint32_t* __restrict__ summary) {
   const int c = threadIdx.x;
   if (c < npacked) {
     const uint32_t b = summary[c], e = summary[nclasses + c];
     const bool any = e > b;
+    const uint32_t t = summary[tfirst + c];
     summary[2 * nclasses + c] = any ? (uint32_t)hpos[b] : 0u;
     summary[2 * nclasses + npacked + c] = any ? (uint32_t)(hpos[e] - hpos[b]) : 0u;
+    summary[trow + c] = any ? (uint32_t)(hpos[t ? t - 1 : e] - hpos[b]) : 0u;
   }
   if (c == 0) summary[2 * (nclasses + npacked)] = (uint32_t)hpos[n];
 }
 
-// Sort key per lane: class | (row span, longest first). The rows a lane runs
-// are the union of its two DPs' row ranges, as in the packed kernels.
+// Sort key per lane: class | orientation | (row span, longest first). The
+// rows a lane runs are the union of its two DPs' row ranges in their
+// orientation (subject_rows with the classification's modes), as in the
+// packed kernels; the lanes of a class with the query on the rows stay first.
 __global__ void k_lane_keys(const uint2* __restrict__ lanes, uint32_t nlanes, const SwJob* __restrict__ jobs,
-                            SeqSet q, SeqSet db, uint32_t merged, uint64_t* __restrict__ keys,
+                            SeqSet q, SeqSet db, uint32_t merged, uint32_t transpose, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ idx) {
   const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
   if (L >= nlanes) return;
   const uint2 ln = lanes[L];
   int lo = INT32_MAX, hi = 0, cls = 0;
+  bool tr = false;
   for (int h = 0; h < 2; ++h) {
     const uint32_t k = h ? ln.y : ln.x;
     if (k == ~0u) continue;
     const SwJob j = jobs[k];
     const int n = (int)(q.offs[j.target + 1] - q.offs[j.target]);
     const int m = (int)(db.offs[j.subject + 1] - db.offs[j.subject]);
-    lo = min(lo, max(1, j.d0 + 1));
-    hi = max(hi, min(n, m + j.d1));
     cls = packed_class(j.d1 - j.d0 + 1, merged);
+    tr = subject_rows(n, m, j.d0, j.d1, (transpose >> (2 * cls)) & 3u);
+    lo = min(lo, tr ? max(1, 1 - j.d1) : max(1, j.d0 + 1));
+    hi = max(hi, tr ? min(m, n - j.d0) : min(n, m + j.d1));
   }
   const int span = min(max(hi - lo + 1, 0), (int)kLaneSpanMax);
-  keys[L] = ((uint64_t)cls << kLaneSpanBits) | (uint64_t)(kLaneSpanMax - span);
+  keys[L] = ((uint64_t)cls << (kLaneSpanBits + 1)) | ((uint64_t)tr << kLaneSpanBits) |
+            (uint64_t)(kLaneSpanMax - span);
   idx[L] = L;
 }
 
@@ -167,6 +196,9 @@ int pair_table_entries() { return kPairTab; }
 
 // Score table for the packed kernel: entry (qa, sA, sB) holds
 // (score(qa, sA) + o, score(qa, sB) + o) as two int16; letter 25 is padding.
+// With the subject on the rows the kernels read entry (s, qA, qB) for
+// (score(qA, s), score(qB, s)): BLOSUM62 is symmetric, so one table serves
+// both orientations.
 // IEEE half-precision bits of a small integer (|v| < 2048, exact).
 static uint16_t half_bits(int v) {
   if (v == 0) return 0;
@@ -190,19 +222,19 @@ void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out)
       }
 }
 
-void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int target_bits, uint32_t merged_map,
-                     uint64_t* keys, uint32_t* idx, hipStream_t st, const uint64_t* njobs_dev) {
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int seq_bits, uint32_t merged_map,
+                     uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st, const uint64_t* njobs_dev) {
   if (!njobs) return;
   hipLaunchKernelGGL(k_classify, dim3(nblk(njobs, 256)), dim3(256), 0, st, jobs, njobs, q, db,
-                     target_bits, merged_map, keys, idx, njobs_dev);
+                     seq_bits, merged_map, transpose, keys, idx, njobs_dev);
 }
 
 void launch_lane_heads(const uint64_t* keys, const uint32_t* flags, const uint64_t* pos, const uint32_t* run_start,
-                       uint64_t n, int cls_shift, int npacked, int nclasses, uint32_t* heads, uint32_t* summary,
-                       hipStream_t st, const uint64_t* n_dev) {
+                       uint64_t n, int cls_shift, int npacked, int nclasses, int tfirst, uint32_t* heads,
+                       uint32_t* summary, hipStream_t st, const uint64_t* n_dev) {
   if (!n) return;
   hipLaunchKernelGGL(k_lane_heads, dim3(nblk(n, 256)), dim3(256), 0, st, keys, flags, pos, run_start, n,
-                     cls_shift, npacked, nclasses, heads, summary, n_dev);
+                     cls_shift, npacked, nclasses, tfirst, heads, summary, n_dev);
 }
 
 void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const uint32_t* heads,
@@ -213,16 +245,16 @@ void launch_make_lanes(const uint32_t* sorted_idx, const uint32_t* flags, const 
                      lanes, n_dev);
 }
 
-void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, uint32_t* summary,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_lane_summary, dim3(1), dim3(64), 0, st, hpos, n, npacked, nclasses, summary);
+void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int nclasses, int tfirst, int trow,
+                         uint32_t* summary, hipStream_t st) {
+  hipLaunchKernelGGL(k_lane_summary, dim3(1), dim3(64), 0, st, hpos, n, npacked, nclasses, tfirst, trow, summary);
 }
 
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                      uint32_t merged_map, uint64_t* keys, uint32_t* idx, hipStream_t st) {
+                      uint32_t merged_map, uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st) {
   if (!nlanes) return;
   hipLaunchKernelGGL(k_lane_keys, dim3(nblk(nlanes, 256)), dim3(256), 0, st, lanes, nlanes, jobs, q, db,
-                     merged_map, keys, idx);
+                     merged_map, transpose, keys, idx);
 }
 
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st) {
@@ -271,12 +303,13 @@ bool sw16_has_variant(int cls, int lanes, int threads) {
 // Band_for width classes: k_sw16 (one lane per DP pair, 256 threads) or a
 // split kernel of DMR_SW16_EXACT_SPLIT.
 template <class Tr>
-bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
+bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, uint32_t nrow, const SwJob* jobs,
+                  SeqSet q, SeqSet db,
                   const int* pair_tab, const int8_t* m32, uint32_t* counter, int32_t* scores, uint32_t* redo,
                   uint32_t* nredo, int cu_count, hipStream_t st) {
 #define DMR_SW16_SPLIT_CASE(c, w, p, t)                                                                       \
   if (cls == c && P == p && T == t) {                                                                          \
-    launch_split<w, p, t, Tr, false>(lanes, nlanes, jobs, q, db, pair_tab, counter, scores, redo, nredo,       \
+    launch_split<w, p, t, Tr, false>(lanes, nlanes, nrow, jobs, q, db, pair_tab, counter, scores, redo, nredo, \
                                      cu_count, st);                                                          \
     return true;                                                                                             \
   }
@@ -286,9 +319,10 @@ bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, co
 #define DMR_SW16_FULL(c, w)                                                                                   \
   case c: {                                                                                                   \
     static const unsigned resident = resident_blocks(k_sw16<w, Tr>, 256, cu_count);                        \
-    const unsigned blocks = std::min<unsigned>(resident, nblk(nlanes, 4 * 64));                               \
-    hipLaunchKernelGGL((k_sw16<w, Tr>), dim3(blocks), dim3(256), 0, st, lanes, nlanes, jobs, q, db, pair_tab, \
-                       m32, counter, scores, redo, nredo);                                                      \
+    const unsigned tasks = nblk(nrow, 64) + nblk(nlanes - nrow, 64);                                          \
+    const unsigned blocks = std::min<unsigned>(resident, nblk(tasks, 4));                                     \
+    hipLaunchKernelGGL((k_sw16<w, Tr>), dim3(blocks), dim3(256), 0, st, lanes, nlanes, nrow, jobs, q, db,     \
+                       pair_tab, m32, counter, scores, redo, nredo);                                            \
     return true;                                                                                              \
   }
   switch (cls) {
@@ -304,17 +338,18 @@ bool launch_exact(int cls, int P, int T, const uint2* lanes, uint32_t nlanes, co
 }
 
 bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
-                 const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32, uint32_t* counter,
-                 int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
+                 uint32_t nrow, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, const int8_t* m32,
+                 uint32_t* counter, int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
   if (!sw16_has_variant(cls, lanes_per_pair, threads)) return false;
   if (!nlanes) return true;
+  if (nrow > nlanes) return false;
   if (cls >= kExactClasses)
-    return launch_sw16_merged(cls, lanes_per_pair, threads, f16, lanes, nlanes, jobs, q, db, pair_tab, counter, scores,
-                              redo, nredo, cu_count, st);
-  return f16 ? launch_exact<F16>(cls, lanes_per_pair, threads, lanes, nlanes, jobs, q, db, pair_tab, m32, counter,
-                                 scores, redo, nredo, cu_count, st)
-             : launch_exact<I16>(cls, lanes_per_pair, threads, lanes, nlanes, jobs, q, db, pair_tab, m32, counter,
-                                 scores, redo, nredo, cu_count, st);
+    return launch_sw16_merged(cls, lanes_per_pair, threads, f16, lanes, nlanes, nrow, jobs, q, db, pair_tab, counter,
+                              scores, redo, nredo, cu_count, st);
+  return f16 ? launch_exact<F16>(cls, lanes_per_pair, threads, lanes, nlanes, nrow, jobs, q, db, pair_tab, m32,
+                                 counter, scores, redo, nredo, cu_count, st)
+             : launch_exact<I16>(cls, lanes_per_pair, threads, lanes, nlanes, nrow, jobs, q, db, pair_tab, m32,
+                                 counter, scores, redo, nredo, cu_count, st);
 }
 
 }  // namespace dmr

@@ -5,11 +5,13 @@
 
 // Banded SW score pass in packed int16: two DPs per lane, band in registers.
 //
-// Each lane carries two DPs of the same query (lo/hi halves of every VGPR)
-// with the same band width W (a compile-time constant), so every recurrence
-// step is one packed instruction for two cells. The band of a row lives in
-// registers indexed by the in-row offset t (column j = i - d1 + t), and the
-// dependencies of gapped::extend_span (src/gapped.rs:110-199) become
+// Each lane carries two DPs with the same row sequence (lo/hi halves of every
+// VGPR: two DPs of one query, or, with the subject on the rows, of one
+// subject; see subject_rows) and the same band width W (a compile-time
+// constant), so every recurrence step is one packed instruction for two
+// cells. The band of a row lives in registers indexed by the in-row offset t
+// (column j = i - d1 + t), and the dependencies of gapped::extend_span
+// (src/gapped.rs:110-199) become
 //   D(i,t) = H(i-1,t) + s,  E(i,t) = max(H(i-1,t+1) - o, E(i-1,t+1) - e),
 //   F(i,t) = max(H(i,t-1) - o, F(i,t-1) - e),  H = max(0, D, E, F),
 // with o = open + extend. G = H - o is stored instead of H (the score table
@@ -149,8 +151,67 @@ __device__ __forceinline__ int settle(int v) {
   return v;
 }
 
+// Orientation of a packed DP. The score pass needs only a band's best score,
+// and that does not depend on which of the two sequences runs along the rows:
+// the band holds the same cells, and the recurrence and its boundary are
+// symmetric in the two sequences (E and F are the two gap kinds with the same
+// costs; outside the band or the matrix a D neighbour is 0 and an E or F
+// neighbour absent). The packed kernels step rows and keep a row's W band
+// offsets in registers, so a DP costs rows x W cells: with the query on the
+// rows the rows are max(1, d0 + 1) .. min(n, m + d1), with the subject on the
+// rows (diagonals -d1 .. -d0) max(1, 1 - d1) .. min(m, n - d0). Band cells
+// past either end of the column sequence are computed as padding, so the
+// orientation with fewer rows computes fewer of them. (The first maximum in
+// row-major order, where the traceback starts, does depend on it; the
+// traceback runs in the reference's orientation.)
+// mode: 0 query rows, 1 the orientation with fewer rows (ties: query rows),
+// 2 subject rows.
+__device__ __forceinline__ bool subject_rows(int64_t n, int64_t m, int64_t d0, int64_t d1, int mode) {
+  if (mode != 1) return mode == 2;
+  const int64_t rq = min(n, m + d1) - max((int64_t)1, d0 + 1);
+  const int64_t rs = min(m, n - d0) - max((int64_t)1, 1 - d1);
+  return rs < rq;
+}
+
+// A lane's two DPs in the orientation of its task: n the length of the row
+// sequence (one for both DPs: the query, or with tr the subject), mA/mB the
+// lengths of the column sequences, rp/ap/bp their padded-letter bases (base
+// 1: an empty sequence for an absent DP), the bands in that orientation, and
+// whether the reference's clamped rows exist (rows past m + d1 with the query
+// on the rows; one cell at the subject's end each), which the 16-bit pass
+// leaves to the exact re-run.
+struct LanePair {
+  int n, mA, mB;
+  uint32_t rp, ap, bp;
+  int d0A, d1A, d0B, d1B;
+  bool clampA, clampB;
+};
+__device__ __forceinline__ LanePair lane_pair(bool tr, bool hasA, bool hasB, const SwJob& ja, const SwJob& jb,
+                                              const SeqSet& q, const SeqSet& db) {
+  const SeqSet& R = tr ? db : q;
+  const SeqSet& C = tr ? q : db;
+  const uint32_t rA = tr ? ja.subject : ja.target;
+  const uint32_t cA = tr ? ja.target : ja.subject, cB = tr ? jb.target : jb.subject;
+  LanePair lp;
+  lp.n = hasA ? (int)(R.offs[rA + 1] - R.offs[rA]) : 0;
+  lp.mA = hasA ? (int)(C.offs[cA + 1] - C.offs[cA]) : 0;
+  lp.mB = hasB ? (int)(C.offs[cB + 1] - C.offs[cB]) : 0;
+  lp.rp = hasA ? R.pad_offs[rA] : 1u;
+  lp.ap = hasA ? C.pad_offs[cA] : 1u;
+  lp.bp = hasB ? C.pad_offs[cB] : 1u;
+  lp.d0A = tr ? -ja.d1 : ja.d0;
+  lp.d1A = tr ? -ja.d0 : ja.d1;
+  lp.d0B = hasB ? (tr ? -jb.d1 : jb.d0) : lp.d0A;
+  lp.d1B = hasB ? (tr ? -jb.d0 : jb.d1) : lp.d1A;
+  // Query rows: m + d1 < n. Subject rows: the query is the column sequence
+  // and d1 = -d0 of this orientation.
+  lp.clampA = tr ? lp.n - lp.d0A < lp.mA : lp.mA + lp.d1A < lp.n;
+  lp.clampB = tr ? lp.n - lp.d0B < lp.mB : lp.mB + lp.d1B < lp.n;
+  return lp;
+}
+
 template <int W, class Tr>
-__global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, uint32_t nlanes,
+__global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, uint32_t nlanes, uint32_t nrow,
                                               const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                               const int* __restrict__ pair_tab,
                                               const int8_t* __restrict__ m32,
@@ -167,27 +228,29 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
   const V GOFF = Tr::splat(1 - kOpen);
   const V FRAME2 = Tr::splat(kFrame);
   const V ZERO2 = Tr::splat(0);
+  // Lanes [0, nrow) run with the query on the rows, lanes [nrow, nlanes) with
+  // the subject (subject_rows); no task mixes the two.
+  const uint32_t trow = (nrow + 63) / 64, ntasks = trow + (nlanes - nrow + 63) / 64;
   for (;;) {
     // Branch-free task fetch (every lane joins the atomic, lane 0 adds 1):
     // the task lives in a scalar register and the loop exit is uniform.
     const uint32_t task =
         (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
-    if ((uint64_t)task * 64 >= nlanes) break;
-    const uint32_t L = task * 64 + lane;
-    const bool active = L < nlanes;
+    if (task >= ntasks) break;
+    const bool tr = task >= trow;
+    const uint32_t L = (tr ? nrow + (task - trow) * 64 : task * 64) + lane;
+    const bool active = L < (tr ? nlanes : nrow);
     uint2 ln = active ? lanes[L] : make_uint2(~0u, ~0u);
     const bool hasA = ln.x != ~0u, hasB = ln.y != ~0u;
     SwJob ja = {}, jb = {};
     if (hasA) ja = jobs[ln.x];
     if (hasB) jb = jobs[ln.y];
-    const int n = hasA ? (int)(q.offs[ja.target + 1] - q.offs[ja.target]) : 0;
-    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - db.offs[ja.subject]) : 0;
-    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - db.offs[jb.subject]) : 0;
-    // Padded-letter bases (base 1: an empty sequence for an absent DP).
-    const uint32_t qp = hasA ? q.pad_offs[ja.target] : 1u;
-    const uint32_t ap = hasA ? db.pad_offs[ja.subject] : 1u, bp = hasB ? db.pad_offs[jb.subject] : 1u;
-    const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
-    const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
+    const LanePair lp = lane_pair(tr, hasA, hasB, ja, jb, q, db);
+    const uint8_t* rpad = tr ? db.pad : q.pad;  // row and column letters
+    const uint8_t* cpad = tr ? q.pad : db.pad;
+    const int n = lp.n, mA = lp.mA, mB = lp.mB;
+    const uint32_t qp = lp.rp, ap = lp.ap, bp = lp.bp;
+    const int d1A = lp.d1A, d1B = lp.d1B, d0A = lp.d0A, d0B = lp.d0B;
     // Rows of this lane: the union of its two DPs' row ranges, rows 1..m+d1
     // (the clamped rows below, one cell at column m each, are left to the
     // exact re-run when they can matter). Each lane runs its own rows (row
@@ -219,19 +282,19 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-      uint32_t lo16 = pair_off4(pad_at(db.pad, ap, mA, lo - d1A + 2 * k - 1),
-                                pad_at(db.pad, bp, mB, lo - d1B + 2 * k - 1));
-      uint32_t hi16 = 2 * k + 1 < W ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + 2 * k),
-                                                pad_at(db.pad, bp, mB, lo - d1B + 2 * k))
+      uint32_t lo16 = pair_off4(pad_at(cpad, ap, mA, lo - d1A + 2 * k - 1),
+                                pad_at(cpad, bp, mB, lo - d1B + 2 * k - 1));
+      uint32_t hi16 = 2 * k + 1 < W ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + 2 * k),
+                                                pad_at(cpad, bp, mB, lo - d1B + 2 * k))
                                     : 0u;
       R[k] = lo16 | (hi16 << 16);
     }
     // Per-row letters: the query letter of row i and the letter pair entering
     // the window at t = W-1 (subject positions i - d0 - 1).
     // Query letter of row i (padding past n), times 4.
-    auto qletter = [&](int i) { return (int)q.pad[qp + (uint32_t)min(i - 1, n)]; };
+    auto qletter = [&](int i) { return (int)rpad[qp + (uint32_t)min(i - 1, n)]; };
     auto pair_at = [&](int i) {
-      return pair_off4(pad_at(db.pad, ap, mA, i - d0A - 1), pad_at(db.pad, bp, mB, i - d0B - 1));
+      return pair_off4(pad_at(cpad, ap, mA, i - d0A - 1), pad_at(cpad, bp, mB, i - d0B - 1));
     };
     // Widths up to 81 load them one row ahead (the clamped addresses are
     // always in bounds); the raw bytes are only read by a VALU copy at the
@@ -260,8 +323,8 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
       int rq = 0, ra = 0, rb = 0;
       if (kPrefetch) {  // raw bytes of row i + 1
         rq = qletter(i + 1);
-        ra = pad_at(db.pad, ap, mA, i - d0A);
-        rb = pad_at(db.pad, bp, mB, i - d0B);
+        ra = pad_at(cpad, ap, mA, i - d0A);
+        rb = pad_at(cpad, bp, mB, i - d0B);
       }
       const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
       auto score = [&](int t) {
@@ -316,14 +379,14 @@ __global__ __launch_bounds__(256) void k_sw16(const uint2* __restrict__ lanes, u
     // there could then hold the maximum.
     const int bA = Tr::lo(best), bB = Tr::hi(best);
     if (hasA) {
-      if (bA >= Tr::kRedo || (bA < 11 && mA + d1A < n)) {
+      if (bA >= Tr::kRedo || (bA < 11 && lp.clampA)) {
         redo[atomicAdd(nredo, 1u)] = ln.x;
       } else {
         scores[ln.x] = bA;
       }
     }
     if (hasB) {
-      if (bB >= Tr::kRedo || (bB < 11 && mB + d1B < n)) {
+      if (bB >= Tr::kRedo || (bB < 11 && lp.clampB)) {
         redo[atomicAdd(nredo, 1u)] = ln.y;
       } else {
         scores[ln.y] = bB;
@@ -378,7 +441,7 @@ template <int kThreads>
 constexpr int kMinWaves = kThreads / 256;
 
 template <int W, int P, int kThreads, class Tr, bool kWall>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<kThreads>))) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<kThreads>))) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes, uint32_t nrow,
                                                    const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
                                                    const int* __restrict__ pair_tab,
                                                    uint32_t* __restrict__ counter,
@@ -427,25 +490,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
   using SwapC = std::integral_constant<int, 0xB1>;    // [1,0,3,2]
   using Swap2C = std::integral_constant<int, 0x4E>;   // [2,3,0,1]
   using MirrorC = std::integral_constant<int, 0x141>; // row_half_mirror
+  // Lanes [0, nrow) run with the query on the rows, lanes [nrow, nlanes) with
+  // the subject (subject_rows); no task mixes the two.
+  constexpr uint32_t kPer = 64 / P;  // lanes (DP pairs) per wave task
+  const uint32_t trow = (nrow + kPer - 1) / kPer, ntasks = trow + (nlanes - nrow + kPer - 1) / kPer;
   for (;;) {
     const uint32_t task =
         (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
-    if ((uint64_t)task * (64 / P) >= nlanes) break;
-    const uint32_t L = task * (64 / P) + lane / P;
-    const bool active = L < nlanes;
+    if (task >= ntasks) break;
+    const bool tr = task >= trow;
+    const uint32_t L = (tr ? nrow + (task - trow) * kPer : task * kPer) + lane / P;
+    const bool active = L < (tr ? nlanes : nrow);
     uint2 ln = active ? lanes[L] : make_uint2(~0u, ~0u);
     const bool hasA = ln.x != ~0u, hasB = ln.y != ~0u;
     SwJob ja = {}, jb = {};
     if (hasA) ja = jobs[ln.x];
     if (hasB) jb = jobs[ln.y];
-    const int n = hasA ? (int)(q.offs[ja.target + 1] - q.offs[ja.target]) : 0;
-    const int mA = hasA ? (int)(db.offs[ja.subject + 1] - db.offs[ja.subject]) : 0;
-    const int mB = hasB ? (int)(db.offs[jb.subject + 1] - db.offs[jb.subject]) : 0;
-    // Padded-letter bases (base 1: an empty sequence for an absent DP).
-    const uint32_t qp = hasA ? q.pad_offs[ja.target] : 1u;
-    const uint32_t ap = hasA ? db.pad_offs[ja.subject] : 1u, bp = hasB ? db.pad_offs[jb.subject] : 1u;
-    const int d1A = ja.d1, d1B = hasB ? jb.d1 : ja.d1;
-    const int d0A = ja.d0, d0B = hasB ? jb.d0 : ja.d0;
+    const LanePair lp = lane_pair(tr, hasA, hasB, ja, jb, q, db);
+    const uint8_t* rpad = tr ? db.pad : q.pad;  // row and column letters
+    const uint8_t* cpad = tr ? q.pad : db.pad;
+    const int n = lp.n, mA = lp.mA, mB = lp.mB;
+    const uint32_t qp = lp.rp, ap = lp.ap, bp = lp.bp;
+    const int d1A = lp.d1A, d1B = lp.d1B, d0A = lp.d0A, d0B = lp.d0B;
     // Rows as in k_sw16 (all lanes of a group have the same range).
     int lo = 0, hi = -1;
     if (hasA) {
@@ -487,18 +553,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int t = t0 + 2 * k;
-      uint32_t lo16 = 2 * k < H ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t - 1),
-                                            pad_at(db.pad, bp, mB, lo - d1B + t - 1))
+      uint32_t lo16 = 2 * k < H ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + t - 1),
+                                            pad_at(cpad, bp, mB, lo - d1B + t - 1))
                                 : 0u;
-      uint32_t hi16 = 2 * k + 1 < H ? pair_off4(pad_at(db.pad, ap, mA, lo - d1A + t),
-                                                pad_at(db.pad, bp, mB, lo - d1B + t))
+      uint32_t hi16 = 2 * k + 1 < H ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + t),
+                                                pad_at(cpad, bp, mB, lo - d1B + t))
                                     : 0u;
       R[k] = lo16 | (hi16 << 16);
     }
     // Letters of the next row, loaded one row ahead (clamped, unconditional
     // loads, settled at the end of the row): the query letter and the pair
     // entering the last lane at offset P*H-1.
-    int qcur = (int)q.pad[qp + (uint32_t)min(lo - 1, n)];
+    int qcur = (int)rpad[qp + (uint32_t)min(lo - 1, n)];
     uint32_t pnext = 0;
     V best = ZERO2;  // in the frame of the current row
     V Zr = ZERO2;    // frame offset of the current row
@@ -511,8 +577,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
       const int i = lo + r;
       // Raw letters of row i + 1: subject positions i + 1 - d1 + P*H - 2.
       const int pos = i - 1 + P * H;
-      const int rq = (int)q.pad[qp + (uint32_t)min(i, n)];
-      const int ra = pad_at(db.pad, ap, mA, pos - d1A), rb = pad_at(db.pad, bp, mB, pos - d1B);
+      const int rq = (int)rpad[qp + (uint32_t)min(i, n)];
+      const int ra = pad_at(cpad, ap, mA, pos - d1A), rb = pad_at(cpad, bp, mB, pos - d1B);
       {
         // E source past the lane: the next lane's first offset, previous row.
         const V g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
@@ -644,14 +710,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
     const int bA = Tr::lo(best), bB = Tr::hi(best);
     if (p == 0) {
       if (hasA) {
-        if (bA >= Tr::kRedo || (bA < 11 && mA + d1A < n)) {
+        if (bA >= Tr::kRedo || (bA < 11 && lp.clampA)) {
           redo[atomicAdd(nredo, 1u)] = ln.x;
         } else {
           scores[ln.x] = bA;
         }
       }
       if (hasB) {
-        if (bB >= Tr::kRedo || (bB < 11 && mB + d1B < n)) {
+        if (bB >= Tr::kRedo || (bB < 11 && lp.clampB)) {
           redo[atomicAdd(nredo, 1u)] = ln.y;
         } else {
           scores[ln.y] = bB;
@@ -664,12 +730,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // Persistent launch of a split kernel: 64 / P lanes per wave task, as many
 // resident blocks as the work needs.
 template <int W, int P, int T, class Tr, bool kWall>
-void launch_split(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab,
-                  uint32_t* counter, int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st) {
-  const unsigned tasks = (nlanes + 64 / P - 1) / (64 / P);
+void launch_split(const uint2* lanes, uint32_t nlanes, uint32_t nrow, const SwJob* jobs, SeqSet q, SeqSet db,
+                  const int* pair_tab, uint32_t* counter, int32_t* scores, uint32_t* redo, uint32_t* nredo,
+                  int cu_count, hipStream_t st) {
+  constexpr unsigned kPer = 64 / P;
+  const unsigned tasks = (nrow + kPer - 1) / kPer + (nlanes - nrow + kPer - 1) / kPer;
   static const unsigned resident = resident_blocks(k_sw16p<W, P, T, Tr, kWall>, T, cu_count);
   const unsigned blocks = std::min<unsigned>(resident, (tasks + T / 64 - 1) / (T / 64));
-  hipLaunchKernelGGL((k_sw16p<W, P, T, Tr, kWall>), dim3(blocks), dim3(T), 0, st, lanes, nlanes, jobs, q, db,
+  hipLaunchKernelGGL((k_sw16p<W, P, T, Tr, kWall>), dim3(blocks), dim3(T), 0, st, lanes, nlanes, nrow, jobs, q, db,
                      pair_tab, counter, scores, redo, nredo);
 }
 

@@ -166,12 +166,22 @@ ScoreLayout default_layout() {
   L.f16 = true;
   L.plan = 0;
   L.graph = false;
+  // Row sequence: the subject for the widest band_for class (its long
+  // queries meet shorter subjects), the query elsewhere.
+  for (int c = 0; c < kPackedClasses; ++c) L.transpose[c] = 0;
+  L.transpose[4] = 2;
   return L;
 }
 
+uint32_t transpose_modes(const ScoreLayout& L) {
+  uint32_t m = 0;
+  for (int c = 0; c < kPackedClasses; ++c) m |= (uint32_t)L.transpose[c] << (2 * c);
+  return m;
+}
+
 bool is_layout_key(const std::string& k) {
-  if (k == "merged_mask" || k == "f16" || k == "score_plan" || k == "graph") return true;
-  for (const char* pre : {"lanes_", "threads_"}) {
+  if (k == "merged_mask" || k == "f16" || k == "score_plan" || k == "graph" || k == "transpose") return true;
+  for (const char* pre : {"lanes_", "threads_", "transpose_"}) {
     const size_t n = std::strlen(pre);
     if (k.compare(0, n, pre) == 0 && class_of(k.substr(n)) >= 0) return true;
   }
@@ -208,6 +218,13 @@ void apply_settings(ScoreLayout* L, const Settings& s) {
       L->f16 = v != 0;
     } else if (k == "graph") {
       L->graph = v != 0;
+    } else if (k == "transpose" || k.compare(0, 10, "transpose_") == 0) {
+      const int c = k == "transpose" ? -1 : class_of(k.substr(10));
+      if (c < 0 && k != "transpose") bad("unknown tuning key: " + k);
+      const int t = as_int(k, v);
+      if (t < 0 || t > 2) bad(k + " is 0, 1 or 2");
+      for (int w = 0; w < kPackedClasses; ++w)
+        if (c < 0 || w == c) L->transpose[w] = t;
     } else if (k == "score_plan") {
       const int p = as_int(k, v);
       if (p < 0 || p > 2) bad("score_plan is 0, 1 or 2");
@@ -223,6 +240,12 @@ double layout_value(const ScoreLayout& L, const std::string& k) {
   if (k == "f16") return L.f16 ? 1 : 0;
   if (k == "score_plan") return L.plan;
   if (k == "graph") return L.graph ? 1 : 0;
+  if (k == "transpose") {
+    for (int c = 1; c < kPackedClasses; ++c)
+      if (L.transpose[c] != L.transpose[0]) return -1;
+    return L.transpose[0];
+  }
+  if (k.compare(0, 10, "transpose_") == 0 && class_of(k.substr(10)) >= 0) return L.transpose[class_of(k.substr(10))];
   if (k.compare(0, 6, "lanes_") == 0 && class_of(k.substr(6)) >= 0) return L.lanes[class_of(k.substr(6))];
   if (k.compare(0, 8, "threads_") == 0 && class_of(k.substr(8)) >= 0) return L.threads[class_of(k.substr(8))];
   bad("unknown tuning key: " + k);

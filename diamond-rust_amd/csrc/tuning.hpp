@@ -20,7 +20,10 @@ struct ScoreLayout {
   bool f16;                     // packed pairs in f16 (else saturating int16)
   int plan;                     // how the class kernels share the streams (0, 1, 2)
   bool graph;                   // ungapped .. score-pass preprocessing replayed as one hipGraph
+  int transpose[kPackedClasses];  // row sequence of a packed DP: 0 query, 1 fewer rows, 2 subject
 };
+// Row-sequence modes of the packed classes, 2 bits per class (k_classify).
+uint32_t transpose_modes(const ScoreLayout& L);
 
 // Built-in defaults.
 ScoreLayout default_layout();
@@ -33,7 +36,9 @@ using Settings = std::map<std::string, double>;
 // Applies settings to L: "lanes_<class>" (a compiled lane count; also sets
 // the class's default block size for it), then "threads_<class>" (a compiled
 // block size for the class's lanes), "merged_mask" (6 bits; the widest merged
-// kernel is always in use), "f16" (0/1), "score_plan" (0-2), "graph" (0/1).
+// kernel is always in use), "f16" (0/1), "score_plan" (0-2), "graph" (0/1),
+// "transpose_<class>" (0-2: which sequence runs along the rows of the
+// class's DPs), "transpose" (every class's; reads -1 when they differ).
 // Throws Error(-1) naming the key on an unknown key or a value with no
 // compiled kernel.
 void apply_settings(ScoreLayout* L, const Settings& s);

@@ -136,7 +136,12 @@ int dmr_device_count(int* n);
  *   "graph"       1 = the chain from the sorted hits to the score pass's
  *                 class summary (ungapped extension, bands, job keys, sort,
  *                 lane pairing) is captured and launched as one hipGraph,
- *                 its executable graph updated in place from batch to batch.
+ *                 its executable graph updated in place from batch to batch;
+ *   "transpose_<c>" which sequence runs along the rows of the class's DPs:
+ *                 0 the query, 1 the one giving fewer rows (fewer band cells
+ *                 past the matrix), 2 the subject; the best score of a band
+ *                 is the same either way ("transpose" sets every class and
+ *                 reads -1 when they differ).
  * Score-pass settings come from built-in defaults, then the tuning file (the
  * file DMR_TUNING names at context creation, empty for none; else
  * tuning_<arch>.json beside the library when it exists; settings per search

@@ -2,6 +2,10 @@
 the oracle, the merged-band classes (per-DP walls), and every compiled
 packed kernel.
 
+A packed DP runs with the query or the subject on the rows ("transpose": 0
+query, 1 the one with fewer rows, 2 subject; lanes pair two DPs with the same
+row sequence), which changes no score.
+
 Each band_for width class runs either with one lane per DP pair (the whole
 band in a lane's registers) or split over two or more lanes (part bands,
 with the horizontal-gap correction across lanes); dmr_ctx_set_tuning selects
@@ -139,6 +143,23 @@ def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
     assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
 
 
+@pytest.mark.parametrize("transpose", [0, 1, 2])
+def test_batched_search_orientations(dmr, orc, synth, transpose):
+    """The 24-query subset with every packed DP's query on the rows (lanes
+    pair DPs of one query) and with every packed DP's subject on the rows
+    (lanes pair DPs of one subject, from different queries)."""
+    w = synth.make("blastp-sensitive-1024").subset_queries(list(range(5, 1024, 43)))
+    w = workload_dict(w)
+    c = dmr.Context(0)
+    c.load_tuning(None)
+    c.set_tuning("transpose", transpose)
+    assert c.get_tuning("transpose") == transpose
+    res, ref = run_both(dmr, c, orc, w, "sensitive")
+    assert len(ref) > 0
+    assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
+    c.close()
+
+
 @pytest.fixture(scope="module")
 def variant_cases(orc):
     """Per packed class, DPs of its widths with the oracle's scores."""
@@ -167,14 +188,16 @@ def variant_cases(orc):
     return out
 
 
-@pytest.mark.parametrize("f16", [1, 0], ids=["f16", "i16"])
-def test_every_compiled_kernel(dmr, variant_cases, f16):
+@pytest.mark.parametrize("f16,transpose", [(1, 1), (0, 1), (1, 0), (1, 2)],
+                         ids=["f16", "i16", "f16-query-rows", "f16-subject-rows"])
+def test_every_compiled_kernel(dmr, variant_cases, f16, transpose):
     """Each compiled packed kernel (class, lanes per DP pair, threads), pinned
     for its class with every merged kernel in use, scores the widths of its
-    class as the oracle does."""
+    class as the oracle does, with either sequence on the rows."""
     c = dmr.Context(0)
     c.load_tuning(None)
     c.set_tuning("f16", f16)
+    c.set_tuning("transpose", transpose)
     c.set_tuning("merged_mask", 0x3F)
     seen = 0
     for v in dmr.sw16_variants():

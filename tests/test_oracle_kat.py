@@ -130,6 +130,14 @@ def test_matrix_table_matches_reference(orc):
             assert orc.lib().orc_matrix_score(a, b) == g["scores"][a][b]
 
 
+def test_matrix_is_symmetric():
+    """The packed score kernels may run a DP with the subject on the rows
+    (the best score of a band is the same either way when the score matrix
+    is symmetric, which the reference's BLOSUM62 is)."""
+    g = json.loads((GOLD / "blosum62.json").read_text())["scores"]
+    assert all(g[a][b] == g[b][a] for a in range(25) for b in range(25))
+
+
 def test_log_erfc(orc):  # src/alp.rs:387-396
     assert orc.log_erfc(0.0) == 0.0
     assert abs(orc.log_erfc(1.0) - (-1.8496055099332485)) < 1e-12
