@@ -142,55 +142,79 @@ __device__ void trace_walk(int lane, int wave, const uint8_t* tr, int rlo, int d
     if (lane == 0) op[nr] = v;
     ++nr;
   };
+  auto push = [&](uint32_t code, uint32_t n) {
+    if (code == cur) {
+      cnt += n;
+    } else {
+      if (cnt) emit(cur | (cnt << 2));
+      cur = code;
+      cnt = n;
+    }
+  };
+  // A cached trace cell (row ik >= r0 of the cache, in the band).
+  auto cell = [&](int ik, int jk) {
+    const int t = jk - ik + d1;  // band offset: nibble t % 2 of byte t / 2
+    return (int)(tc[(ik - r0) * kRowBytes + (t >> 1)] >> (4 * (t & 1))) & 15;
+  };
+  // The reference's loop (src/gapped.rs:222-265), a run of steps at a time:
+  // lane k looks at step k of the current state's direction (a diagonal run
+  // of matches in state 0, a vertical or horizontal gap in states 1 and 2),
+  // each lane checks the loop condition and the row range at its own cell,
+  // and one ballot gives the run's length. Lane 0's cell is the current one,
+  // already checked.
   while (i > 0 && j > 0) {
     if (j < R.lo(i) || j > R.hi(i) || i < rlo) break;
     if (i < r0) refill(i);
-    const int t = j - i + d1;  // band offset: nibble t % 2 of byte t / 2
-    const int tv = uniform((int)(tc[(i - r0) * kRowBytes + (t >> 1)] >> (4 * (t & 1))) & 15);
-    uint32_t code;
+    const int di = state == 2 ? 0 : 1, dj = state == 1 ? 0 : 1;
+    // Steps that stay in the cached rows (a horizontal gap stays in row i).
+    const int span = state == 2 ? 64 : min(64, i - r0 + 1);
+    const int ik = i - lane * di, jk = j - lane * dj;
+    const bool ok = lane < span && ik > 0 && jk > 0 && jk >= R.lo(ik) && jk <= R.hi(ik) && ik >= rlo;
+    const int tv = ok ? cell(ik, jk) : 0;
+    const uint64_t bad = __ballot(!ok);  // bit k: the loop stops before step k
     if (state == 0) {
-      const int c = tv & 3;
+      const int c = uniform(tv) & 3;  // lane 0's cell
       if (c == 0) break;
-      if (c == 2) {
-        state = 1;
+      if (c == 2 || c == 3) {
+        state = c - 1;
         continue;
       }
-      if (c == 3) {
-        state = 2;
-        continue;
+      // Matches while the cells are in range and choose D.
+      const uint64_t stopm = bad | __ballot(ok && (tv & 3) != 1);
+      const int run = stopm ? __builtin_ctzll(stopm) : 64;
+      const bool in_run = lane < run;
+      int x = 0, y = 0;
+      if (in_run) {
+        x = (int)qcache[wave][ik - r0];
+        y = (int)scache[wave][jk - c0];
       }
-      const int x = uniform((int)qcache[wave][i - r0]), y = uniform((int)scache[wave][j - c0]);
-      o.length++;
-      if (x == y) o.identities++;
-      else o.mismatches++;
-      if (M[x * kScoreStride + y] > 0) o.positives++;
+      const int idn = __builtin_popcountll(__ballot(in_run && x == y));
+      o.length += run;
+      o.identities += idn;
+      o.mismatches += run - idn;
+      o.positives += __builtin_popcountll(__ballot(in_run && M[x * kScoreStride + y] > 0));
       in_gap = false;
-      code = 0;
-      --i;
-      --j;
+      push(0, (uint32_t)run);
+      i -= run;
+      j -= run;
     } else {
-      o.length++;
-      o.gaps++;
+      // Gap steps: each runs (its cell in range), and the first whose
+      // extension bit is clear is the last of the gap.
+      const int bit = state == 1 ? 2 : 3;
+      const uint64_t last = __ballot(ok && ((tv >> bit) & 1) == 0);
+      const int f = bad ? __builtin_ctzll(bad) : 64;     // steps before the loop stops
+      const int z = last ? __builtin_ctzll(last) : 64;   // the gap's last step
+      const int g = z < f ? z + 1 : f;
+      o.length += g;
+      o.gaps += g;
       if (!in_gap) {
         o.gap_openings++;
         in_gap = true;
       }
-      if (state == 1) {
-        code = 1;
-        if (((tv >> 2) & 1) == 0) state = 0;
-        --i;
-      } else {
-        code = 2;
-        if (((tv >> 3) & 1) == 0) state = 0;
-        --j;
-      }
-    }
-    if (code == cur) {
-      ++cnt;
-    } else {
-      if (cnt) emit(cur | (cnt << 2));
-      cur = code;
-      cnt = 1;
+      push(state == 1 ? 1u : 2u, (uint32_t)g);
+      if (state == 1) i -= g;
+      else j -= g;
+      if (z < f) state = 0;
     }
   }
   if (cnt) emit(cur | (cnt << 2));
@@ -264,7 +288,10 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
       Ep[k] = kNeg;
       S[k] = letter_at(ss, m, rlo - d1 + t);
     }
-    int best = 0, lbest = 0, bi = 0, bj = 0;
+    // lbest: the lane's largest H so far; in trace mode li/lt hold the
+    // first cell (row, band offset) where it occurs (rows in order, offsets
+    // in order within a row).
+    int best = 0, lbest = 0, li = 0, lt = 0, bi = 0, bj = 0;
     uint8_t* tr = kTrace ? trace + trace_off[task] : nullptr;
     // Per-row letters in blocks of 64 rows, one row per lane: the query
     // letter of row i and the subject letter entering the window after row i
@@ -395,20 +422,20 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
       for (int k = 0; k + 1 < K; ++k) S[k] = S[k + 1];
       S[K - 1] = sn;
       if (kTrace) {
-        // The wave-wide first maximum is only updated on rows where some
-        // lane beats the current best.
-        if (__ballot(rowbest > best) != 0) {
-          const int rb = wmax(rowbest);
-          int tf = 1 << 30;
+        // Each lane keeps its own first maximum (no cross-lane work per
+        // row); the wave's first maximum in row-major order is reduced from
+        // them after the last row. Once some lane holds the known best, the
+        // first maximum lies in this row or an earlier one.
+        if (rowbest > lbest) {
+          int tf = 0;
 #pragma unroll
           for (int k = K - 1; k >= 0; --k)
-            if (valid[k] && Hp[k] == rb) tf = lane * K + k;
-          tf = wmin(tf);
-          best = rb;
-          bi = i;
-          bj = i - d1 + tf;
+            if (valid[k] && Hp[k] == rowbest) tf = k;
+          lbest = rowbest;
+          li = i;
+          lt = lane * K + tf;
         }
-        if (best >= stop_at) {
+        if (known && __ballot(lbest >= stop_at) != 0) {
           stop = true;
           break;
         }
@@ -424,10 +451,15 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     // below 11, the largest matrix entry. Score mode finishes them here from
     // H and E of (istar, m), held by offset 0 after row istar; trace mode and
     // bands that never reach row istar go to the row-exact kernel.
+    best = wmax(lbest);
+    if (kTrace && best > 0) {
+      // First maximum in row-major order: the smallest (row, offset) among
+      // the lanes holding the wave's largest H.
+      bi = wmin(lbest == best ? li : INT32_MAX);
+      bj = bi - d1 + wmin(lbest == best && li == bi ? lt : INT32_MAX);
+    }
     bool need_exact = best < 11 && istar < n;
     if (!kTrace) {
-      best = wmax(lbest);
-      need_exact = best < 11 && istar < n;
       if (need_exact && m >= 1 && istar >= 1 && rlo <= rhi) {
         int h = uniform(Hp[0]), e = uniform(Ep[0]);
         const int sl = ss[m - 1];
