@@ -46,9 +46,11 @@ void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_
                        uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
                        uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st,
                        const uint64_t* nruns_dev = nullptr, const uint64_t* nq_dev = nullptr);
+// hit_off: npairs + 1 = nruns * nchunks + 1 exclusive offsets of counts, the
+// last the total nhits (which the host knows).
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                      const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off,
+                      const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off, uint64_t nhits,
                       const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
                       const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st,
                       const uint32_t* slice_start = nullptr, const uint32_t* slice_nq = nullptr,

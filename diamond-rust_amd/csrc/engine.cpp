@@ -725,7 +725,7 @@ std::vector<std::vector<HspRec>> Engine::search_targets(
       scan_exclusive_u32_to_u64(scnt, hoff, npairs + 1, b_scan_.ensure<uint64_t>(scan_tmp_elems(npairs + 1)), st_);
       if (read1(hoff + npairs) != nhits) throw Error(-2, "join offsets disagree with the per-target hit counts");
     }
-    launch_join_emit(ql, rs, nQb, nQb, idx_->chunk_desc.as<RefChunkDev>(), nch, scnt, rlo, hoff,
+    launch_join_emit(ql, rs, nQb, nQb, idx_->chunk_desc.as<RefChunkDev>(), nch, scnt, rlo, hoff, nhits,
                      q_seq_of_.as<uint32_t>(), q_offs_.as<uint32_t>(), db_->seq_of.as<uint32_t>(),
                      db_->offs.as<uint32_t>(), g, hits, st_, sst, snq, nrd, nQd);
     stage_event(11);

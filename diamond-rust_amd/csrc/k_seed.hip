@@ -262,48 +262,62 @@ __global__ void k_join_slice(const uint32_t* __restrict__ qlocs, const uint32_t*
   slice_nq[t] = ne;
 }
 
-// One thread per (run, chunk) pair; writes its nq*nr hits as packed keys.
-// With slice_start / slice_nq, only the pair's entries in that range.
-__global__ void k_join_emit(const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start,
-                            uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
-                            uint32_t nchunks, const uint32_t* __restrict__ counts,
-                            const uint64_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off,
-                            const uint32_t* __restrict__ q_seq_of, const uint32_t* __restrict__ q_offs,
-                            const uint32_t* __restrict__ db_seq_of, const uint32_t* __restrict__ db_offs,
-                            HitGeom g, uint64_t* __restrict__ hits, const uint32_t* __restrict__ slice_start,
-                            const uint32_t* __restrict__ slice_nq, const uint64_t* __restrict__ nruns_dev,
-                            const uint64_t* __restrict__ nq_dev) {
-  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nruns * nchunks) return;
+// Join emission, one thread per hit: hit o belongs to the (run, chunk) pair
+// t with hit_off[t] <= o < hit_off[t + 1] (an empty pair never holds one),
+// and is entry a = l / nr of the query run times entry b = l % nr of the
+// reference run, l = o - hit_off[t]: the order a pair-by-pair double loop
+// writes, so consecutive threads write consecutive hits. Each block finds
+// the pair range of its hits by binary search over hit_off (npairs + 1
+// offsets, the last the total), each thread its pair inside that range.
+// With slice_start / slice_nq, a pair's entries are the slice's part of it.
+__global__ __launch_bounds__(256) void k_join_emit(
+    const uint32_t* __restrict__ qlocs, const uint32_t* __restrict__ run_start, uint64_t nruns,
+    uint64_t nq_total, const RefChunkDev* __restrict__ chunks, uint32_t nchunks, const uint32_t* __restrict__ counts,
+    const uint64_t* __restrict__ rlo, const uint64_t* __restrict__ hit_off, uint64_t npairs, uint64_t nhits,
+    const uint32_t* __restrict__ q_seq_of, const uint32_t* __restrict__ q_offs,
+    const uint32_t* __restrict__ db_seq_of, const uint32_t* __restrict__ db_offs, HitGeom g,
+    uint64_t* __restrict__ hits, const uint32_t* __restrict__ slice_start, const uint32_t* __restrict__ slice_nq,
+    const uint64_t* __restrict__ nruns_dev, const uint64_t* __restrict__ nq_dev) {
+  __shared__ uint64_t range[2];
+  const uint64_t o0 = (uint64_t)blockIdx.x * blockDim.x;
+  // Last pair whose offset is <= x (x < nhits = hit_off[npairs]).
+  auto last_at_most = [&](uint64_t x, uint64_t lo, uint64_t hi) {  // hit_off[lo] <= x < hit_off[hi]
+    while (hi - lo > 1) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (hit_off[mid] <= x) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  };
+  if (threadIdx.x < 2) {
+    const uint64_t x = threadIdx.x == 0 ? o0 : min(o0 + blockDim.x, nhits) - 1;
+    range[threadIdx.x] = last_at_most(x, 0, npairs);
+  }
+  __syncthreads();
+  const uint64_t o = o0 + threadIdx.x;
+  if (o >= nhits) return;
   nruns = dcount(nruns, nruns_dev);
   nq_total = dcount(nq_total, nq_dev);
-  uint32_t cnt = counts[t];  // 0 past the runs
-  if (cnt == 0) return;
-  uint64_t r = t / nchunks;
-  uint32_t c = (uint32_t)(t - r * nchunks);
+  const uint64_t t = last_at_most(o, range[0], range[1] + 1);
+  const uint64_t r = t / nchunks;
+  const uint32_t c = (uint32_t)(t - r * nchunks);
   uint64_t s0 = run_start[r];
-  uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
-  uint32_t nq = (uint32_t)(s1 - s0);
-  if (slice_start) {  // cnt counts the slice's entries only
+  uint32_t nq = (uint32_t)(((r + 1 < nruns) ? run_start[r + 1] : nq_total) - s0);
+  if (slice_start) {  // the pair's count covers the slice's entries only
     s0 = slice_start[t];
     nq = slice_nq[t];
   }
-  const uint32_t nr = cnt / nq;
-  const uint32_t* rl = chunks[c].locs + rlo[t];
-  uint64_t o = hit_off[t];
-  for (uint32_t a = 0; a < nq; ++a) {
-    uint32_t ql = qlocs[s0 + a];
-    uint32_t tg = q_seq_of[ql];
-    uint32_t qpos = ql - q_offs[tg];
-    for (uint32_t b = 0; b < nr; ++b) {
-      uint32_t rloc = rl[b];
-      uint32_t sj = db_seq_of[rloc];
-      uint32_t spos = rloc - db_offs[sj];
-      uint64_t diag = (uint64_t)((int64_t)qpos - (int64_t)spos + g.diag_bias);
-      uint64_t key = ((((uint64_t)tg << g.bs) | sj) << g.bd | diag) << g.bp | spos;
-      hits[o++] = key;
-    }
-  }
+  const uint32_t nr = counts[t] / nq;
+  const uint32_t l = (uint32_t)(o - hit_off[t]);
+  const uint32_t a = l / nr, b = l - a * nr;
+  const uint32_t ql = qlocs[s0 + a];
+  const uint32_t tg = q_seq_of[ql];
+  const uint32_t qpos = ql - q_offs[tg];
+  const uint32_t rloc = chunks[c].locs[rlo[t] + b];
+  const uint32_t sj = db_seq_of[rloc];
+  const uint32_t spos = rloc - db_offs[sj];
+  const uint64_t diag = (uint64_t)((int64_t)qpos - (int64_t)spos + g.diag_bias);
+  hits[o] = ((((uint64_t)tg << g.bs) | sj) << g.bd | diag) << g.bp | spos;
 }
 
 // Join emission for explicit (seq, pos) entry arrays (drop-in seed::join_into).
@@ -406,16 +420,16 @@ void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_
 
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,
                       uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
-                      const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off,
+                      const uint32_t* counts, const uint64_t* rlo, const uint64_t* hit_off, uint64_t nhits,
                       const uint32_t* q_seq_of, const uint32_t* q_offs, const uint32_t* db_seq_of,
                       const uint32_t* db_offs, HitGeom g, uint64_t* hits, hipStream_t st,
                       const uint32_t* slice_start, const uint32_t* slice_nq, const uint64_t* nruns_dev,
                       const uint64_t* nq_dev) {
-  uint64_t n = nruns * nchunks;
-  if (!n) return;
-  hipLaunchKernelGGL(k_join_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qlocs,
-                     run_start, nruns, nq_total, chunks, nchunks, counts, rlo, hit_off, q_seq_of,
-                     q_offs, db_seq_of, db_offs, g, hits, slice_start, slice_nq, nruns_dev, nq_dev);
+  const uint64_t npairs = nruns * nchunks;
+  if (!npairs || !nhits) return;
+  hipLaunchKernelGGL(k_join_emit, dim3((unsigned)((nhits + 255) / 256)), dim3(256), 0, st, qlocs, run_start,
+                     nruns, nq_total, chunks, nchunks, counts, rlo, hit_off, npairs, nhits, q_seq_of, q_offs,
+                     db_seq_of, db_offs, g, hits, slice_start, slice_nq, nruns_dev, nq_dev);
 }
 
 void launch_run_flags_grouped(const uint64_t* keys, const uint32_t* locs, const uint32_t* seq_of,
