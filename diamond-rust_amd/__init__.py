@@ -413,7 +413,7 @@ class Context:
     def set_tuning(self, key: str, value: float):
         """dmr_ctx_set_tuning: slice_hits, profile, and the score-pass kernels
         (lanes_<class>, threads_<class>, transpose_<class>, transpose,
-        merged_mask, f16, score_plan, graph)."""
+        merged_mask, f16, score_plan, graph, fuse)."""
         _check(lib().dmr_ctx_set_tuning(self._h, key.encode(), float(value)))
 
     def load_tuning(self, path):
@@ -423,7 +423,7 @@ class Context:
     def layout(self) -> dict:
         """Score-pass settings of the last search (every layout key)."""
         keys = [f"{k}_{c}" for c in PACKED_CLASSES for k in ("lanes", "threads", "transpose")]
-        keys += ["merged_mask", "f16", "score_plan", "graph"]
+        keys += ["merged_mask", "f16", "score_plan", "graph", "fuse"]
         return {k: int(self.get_tuning(k)) for k in keys}
 
     def get_tuning(self, key: str) -> float:

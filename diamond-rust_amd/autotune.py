@@ -13,7 +13,7 @@ equal the first one's, or the tuner stops.
 Search: coordinate descent from the built-in defaults. Each step tries the
 values of one setting with the others held, and keeps a value only when its
 median search time beats the best so far by more than --min-gain (noise
-guard): f16; the graph; the merged-band kernels in use (one bucket bound
+guard): f16; the graph; the fused launch; the merged-band kernels in use (one bucket bound
 added or removed at a time, until no change helps); the row sequence
 (transpose_<class>) and the kernel of each class that has DPs in the
 workload; the stream plan; the merged buckets once more.
@@ -106,6 +106,7 @@ class Tuner:
 
         consider({**cur, "f16": 1 - cur["f16"]}, {"f16": 1 - cur["f16"]})
         consider({**cur, "graph": 1 - cur["graph"]}, {"graph": 1 - cur["graph"]})
+        consider({**cur, "fuse": 1 - cur["fuse"]}, {"fuse": 1 - cur["fuse"]})
 
         def merged_pass():
             changed = True

@@ -20,7 +20,7 @@ ARCH = os.environ.get("DMR_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["k_sort.hip", "k_seed.hip", "k_extend.hip", "k_sw.hip", "k_sw16.hip", "k_sw16m.hip", "k_swwave.hip", "plain.cpp",
            "engine.cpp", "tuning.cpp", "api.cpp", "host_pool.cpp", "fasta.cpp", "noalloc.hip", "k_translate.hip", "k_swblock.hip",
-           "serve.cpp"]
+           "serve.cpp", "k_sw16f.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",

@@ -1136,8 +1136,33 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
   // first on streams of their own; the VALU-bound packed classes follow one
   // another on a third (DMR_SCORE_PLAN selects other arrangements).
   fork();
+  // Fused launch (tuning key `fuse`): the classes of DMR_SW16_FUSED in one
+  // persistent grid; they are not launched on their own. Its time counts for
+  // each of them.
+  bool fused[kPackedClasses] = {};
+  const bool fuse = L.fuse && nl && !score_serial_;
+  if (fuse)
+    for (int k = 0; k < kFusedKinds; ++k) fused[sw16_fused_class(k)] = true;
+  const int fused_slot = sw16_fused_class(0);  // its events and task counter
+  auto launch_fused = [&](hipStream_t s) {
+    FusedSections fs = {};
+    bool any = false;
+    for (int k = 0; k < kFusedKinds; ++k) {
+      const int c = sw16_fused_class(k);
+      fs.lanes[k] = lanes + class_lane0[c];
+      fs.nlanes[k] = class_lanes[c];
+      fs.nrow[k] = class_rows[c];
+      any |= class_lanes[c] > 0;
+    }
+    if (!any) return;
+    DMR_HIP(hipEventRecord(ev_[12 + 2 * fused_slot], s));
+    if (!launch_sw16_fused(L.f16, fs, jobs, qs, dbs, (L.f16 ? pair_tab16_ : pair_tab_).as<int>(),
+                           cnt + kCntPacked + fused_slot, scores, redo, cnt + kCntRedo, cus, s))
+      throw Error(-1, "fused packed launch: lane counts out of range");
+    DMR_HIP(hipEventRecord(ev_[13 + 2 * fused_slot], s));
+  };
   auto launch_packed = [&](int c, hipStream_t s) {
-    if (!nl || !class_lanes[c]) return;
+    if (!nl || !class_lanes[c] || fused[c]) return;
     DMR_HIP(hipEventRecord(ev_[12 + 2 * c], s));
     if (!launch_sw16(c, L.lanes[c], L.threads[c], L.f16, lanes + class_lane0[c], class_lanes[c], class_rows[c],
                      jobs, qs, dbs, (L.f16 ? pair_tab16_ : pair_tab_).as<int>(), matrix32_.as<int8_t>(),
@@ -1187,6 +1212,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
     // Every throughput class on a stream of its own beside the others.
     for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
     launch_merged(aux_[2]);
+    if (fuse) launch_fused(aux_[1]);
     launch_packed(4, aux_[0]);
     launch_packed(3, aux_[1]);
     for (int c = 2; c >= 0; --c) launch_packed(c, aux_[2]);
@@ -1198,6 +1224,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
     // latency-bound classes (few, long DPs) beside them on the other streams.
     for (int c = W0 + kWaveKinds - 1; c >= W0 + 3; --c) launch_wave(c, st_);  // K = 32, 16, 8
     launch_merged(aux_[2]);
+    if (fuse) launch_fused(aux_[0]);
     if (L.plan == 1)
       for (int c = 4; c >= 0; --c) launch_packed(c, aux_[0]);
     else
@@ -1230,8 +1257,10 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
   DMR_HIP(hipStreamSynchronize(st_));
   const uint32_t nr16 = h_cnt[kCntRedo], nfb = h_cnt[kCntFallback];
   for (int c = 0; c < kRowExact; ++c)
-    if ((c < kPackedClasses && nl && class_lanes[c]) ||
-        (c >= kPackedClasses && swk_launched(c, cbeg, cend, fast)))
+    if (c < kPackedClasses && fused[c] && class_lanes[c])
+      S->ms_class[c] = stage_ms(12 + 2 * fused_slot, 13 + 2 * fused_slot);
+    else if ((c < kPackedClasses && nl && class_lanes[c]) ||
+             (c >= kPackedClasses && swk_launched(c, cbeg, cend, fast)))
       S->ms_class[c] = stage_ms(12 + 2 * c, 13 + 2 * c);
   if (nwide) wide_ms = stage_ms(12 + 2 * kRowExact, 13 + 2 * kRowExact);
   S->ms_class[kRowExact] = wide_ms;

@@ -153,56 +153,113 @@ __global__ void k_compact_flags(const uint32_t* __restrict__ flags, const uint64
   if (flags[i]) out[pos[i]] = (uint32_t)i;
 }
 
-__device__ __forceinline__ void ref_range(const uint64_t* __restrict__ rkeys,
-                                          const uint64_t* __restrict__ table, int shift,
-                                          uint64_t key, uint64_t* lo_out, uint64_t* hi_out) {
-  uint64_t b = key >> shift;
-  uint64_t lo = table[b], hi = table[b + 1];
-  // lower_bound(key) and upper_bound(key) inside the bucket.
-  uint64_t a = lo, z = hi;
+// First index of a chunk's sorted keys that is >= key (upper: > key),
+// through the bucket table: keys past the last bucket are past every key.
+__device__ __forceinline__ uint64_t ref_bound(const RefChunkDev& ch, uint64_t key, bool upper) {
+  const uint64_t b = key >> ch.shift;
+  if (ch.n == 0 || b >= ch.nbuckets) return ch.n;
+  uint64_t a = ch.table[b], z = ch.table[b + 1];
   while (a < z) {
-    uint64_t mid = (a + z) >> 1;
-    if (rkeys[mid] < key) a = mid + 1;
+    const uint64_t mid = (a + z) >> 1;
+    if (upper ? ch.keys[mid] <= key : ch.keys[mid] < key) a = mid + 1;
     else z = mid;
   }
-  uint64_t first = a;
-  z = hi;
-  while (a < z) {
-    uint64_t mid = (a + z) >> 1;
-    if (rkeys[mid] <= key) a = mid + 1;
-    else z = mid;
-  }
-  *lo_out = first;
-  *hi_out = a;
+  return a;
 }
 
-// grid over runs; one thread per (run, chunk).
-__global__ void k_join_count(const uint64_t* __restrict__ qkeys, const uint32_t* __restrict__ run_start,
-                             uint64_t nruns, uint64_t nq_total, const RefChunkDev* __restrict__ chunks,
-                             uint32_t nchunks, uint64_t max_expand, uint32_t* __restrict__ counts,
-                             uint64_t* __restrict__ rlo, const uint64_t* __restrict__ nruns_dev,
-                             const uint64_t* __restrict__ nq_dev) {
-  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nruns * nchunks) return;
-  uint64_t r = t / nchunks;
-  uint32_t c = (uint32_t)(t - r * nchunks);
+// Per (query-key run, chunk): the run's reference entries (a range of the
+// chunk's sorted keys) and nq * nr, 0 past the cap. One block takes 256
+// consecutive runs, whose keys are ascending. For up to kJoinGroup chunks at
+// a time, one thread per chunk and bound finds the reference keys between
+// the block's first and last run's key (bucket table plus binary search),
+// the block stages them in LDS, and each thread finds its key's range in
+// each staged chunk, so the reference keys are read once, coalesced, and the
+// dependent searches of a block's chunks run side by side. A chunk whose
+// range holds more keys than its share of the stage is searched per thread.
+constexpr int kJoinRuns = 256;
+constexpr int kJoinStage = 4096;  // reference keys staged per block
+constexpr int kJoinGroup = 8;     // chunks staged together
+__global__ __launch_bounds__(kJoinRuns) void k_join_count(
+    const uint64_t* __restrict__ qkeys, const uint32_t* __restrict__ run_start, uint64_t nruns, uint64_t nq_total,
+    const RefChunkDev* __restrict__ chunks, uint32_t nchunks, uint64_t max_expand, uint32_t* __restrict__ counts,
+    uint64_t* __restrict__ rlo, const uint64_t* __restrict__ nruns_dev, const uint64_t* __restrict__ nq_dev) {
+  __shared__ uint64_t stage[kJoinStage];
+  __shared__ uint64_t range[2 * kJoinGroup];
+  const uint64_t bound = nruns;
   nruns = dcount(nruns, nruns_dev);
   nq_total = dcount(nq_total, nq_dev);
-  if (r >= nruns) {
-    counts[t] = 0;
-    rlo[t] = 0;
+  const uint64_t r0 = (uint64_t)blockIdx.x * kJoinRuns, r = r0 + threadIdx.x;
+  if (r0 >= nruns) {  // past the runs: counts 0 (the scan runs over the bound)
+    if (r < bound)
+      for (uint32_t c = 0; c < nchunks; ++c) {
+        counts[r * nchunks + c] = 0;
+        rlo[r * nchunks + c] = 0;
+      }
     return;
   }
-  uint64_t s0 = run_start[r];
-  uint64_t s1 = (r + 1 < nruns) ? run_start[r + 1] : nq_total;
-  uint64_t key = qkeys[s0];
-  const RefChunkDev ch = chunks[c];
-  uint64_t lo = 0, hi = 0;
-  if (ch.n > 0 && (key >> ch.shift) < ch.nbuckets) ref_range(ch.keys, ch.table, ch.shift, key, &lo, &hi);
-  uint64_t nq = s1 - s0, nr = hi - lo;
-  uint64_t cnt = nq * nr;
-  counts[t] = (nr > 0 && cnt <= max_expand) ? (uint32_t)cnt : 0u;
-  rlo[t] = lo;
+  const bool live = r < nruns;
+  const uint64_t rl = min(r0 + kJoinRuns, nruns) - 1;  // the block's last run
+  uint64_t s0 = 0, nq = 0, key = 0;
+  if (live) {
+    s0 = run_start[r];
+    nq = ((r + 1 < nruns) ? run_start[r + 1] : nq_total) - s0;
+    key = qkeys[s0];
+  }
+  for (uint32_t c0 = 0; c0 < nchunks; c0 += kJoinGroup) {
+    const uint32_t g = min(nchunks - c0, (uint32_t)kJoinGroup);
+    const uint32_t share = kJoinStage / g;  // staged keys per chunk
+    if (threadIdx.x < 2 * g) {
+      const uint32_t c = c0 + threadIdx.x / 2;
+      const bool upper = threadIdx.x & 1;
+      range[threadIdx.x] = ref_bound(chunks[c], qkeys[run_start[upper ? rl : r0]], upper);
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < g; ++k) {
+      const uint64_t lo0 = range[2 * k], hi0 = range[2 * k + 1];
+      const uint64_t ns = hi0 > lo0 ? hi0 - lo0 : 0;
+      if (ns <= share) {
+        const uint64_t* src = chunks[c0 + k].keys + lo0;
+        for (uint32_t e = threadIdx.x; e < ns; e += kJoinRuns) stage[k * share + e] = src[e];
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < g; ++k) {
+      const uint32_t c = c0 + k;
+      if (live) {
+        const uint64_t lo0 = range[2 * k], hi0 = range[2 * k + 1];
+        const uint64_t ns = hi0 > lo0 ? hi0 - lo0 : 0;
+        uint64_t lo, hi;
+        if (ns <= share) {  // lower and upper bound of key among the staged keys
+          const uint64_t* st = stage + k * share;
+          uint32_t a = 0, z = (uint32_t)ns;
+          while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (st[mid] < key) a = mid + 1;
+            else z = mid;
+          }
+          lo = lo0 + a;
+          z = (uint32_t)ns;
+          while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (st[mid] <= key) a = mid + 1;
+            else z = mid;
+          }
+          hi = lo0 + a;
+        } else {
+          const RefChunkDev ch = chunks[c];
+          lo = ref_bound(ch, key, false);
+          hi = ref_bound(ch, key, true);
+        }
+        const uint64_t nr = hi - lo, cnt = nq * nr;
+        counts[r * nchunks + c] = (nr > 0 && cnt <= max_expand) ? (uint32_t)cnt : 0u;
+        rlo[r * nchunks + c] = lo;
+      } else if (r < bound) {
+        counts[r * nchunks + c] = 0;
+        rlo[r * nchunks + c] = 0;
+      }
+    }
+    __syncthreads();  // the stage and the ranges are reused by the next group
+  }
 }
 
 // Hits per target: every entry of a kept (run, chunk) pair contributes its
@@ -412,10 +469,9 @@ void launch_join_count(const uint64_t* qkeys, const uint32_t* run_start, uint64_
                        uint64_t nq_total, const RefChunkDev* chunks, uint32_t nchunks,
                        uint64_t max_expand, uint32_t* counts, uint64_t* rlo, hipStream_t st,
                        const uint64_t* nruns_dev, const uint64_t* nq_dev) {
-  uint64_t n = nruns * nchunks;
-  if (!n) return;
-  hipLaunchKernelGGL(k_join_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, qkeys,
-                     run_start, nruns, nq_total, chunks, nchunks, max_expand, counts, rlo, nruns_dev, nq_dev);
+  if (!nruns || !nchunks) return;
+  hipLaunchKernelGGL(k_join_count, dim3((unsigned)((nruns + kJoinRuns - 1) / kJoinRuns)), dim3(kJoinRuns), 0, st,
+                     qkeys, run_start, nruns, nq_total, chunks, nchunks, max_expand, counts, rlo, nruns_dev, nq_dev);
 }
 
 void launch_join_emit(const uint32_t* qlocs, const uint32_t* run_start, uint64_t nruns,

@@ -440,23 +440,20 @@ constexpr int kWallOff = -30000;
 template <int kThreads>
 constexpr int kMinWaves = kThreads / 256;
 
-template <int W, int P, int kThreads, class Tr, bool kWall>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<kThreads>))) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes, uint32_t nrow,
-                                                   const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
-                                                   const int* __restrict__ pair_tab,
-                                                   uint32_t* __restrict__ counter,
-                                                   int32_t* __restrict__ scores,
-                                                   uint32_t* __restrict__ redo,
-                                                   uint32_t* __restrict__ nredo) {
+// One wave task of a split kernel: the kPer = 64 / P lanes (DP pairs) of
+// task `task` of a class, lanes [0, nrow) with the query on the rows and
+// [nrow, nlanes) with the subject; T is the block's score table in LDS.
+template <int W, int P, class Tr, bool kWall>
+__device__ __forceinline__ void sw16p_task(uint32_t task, const uint2* __restrict__ lanes, uint32_t nlanes,
+                                           uint32_t nrow, const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
+                                           const int* T, int32_t* __restrict__ scores, uint32_t* __restrict__ redo,
+                                           uint32_t* __restrict__ nredo) {
   static_assert(W % 2 == 1, "band widths are odd");
   static_assert(P == 2 || P == 4 || P == 8 || P == 16, "lane groups of 2, 4, 8 or 16");
   constexpr int H = SplitShape<W, P>::H;
   constexpr int NR = SplitShape<W, P>::NR;
   constexpr int kDeadFrom = SplitShape<W, P>::kDeadFrom;
   static_assert(kWall || (kDeadFrom >= 1 && kDeadFrom <= H), "the wall lies in the last lane");
-  __shared__ int T[kPairTab];
-  for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int p = lane & (P - 1);
   const bool last = p == P - 1;
@@ -493,236 +490,298 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
   // Lanes [0, nrow) run with the query on the rows, lanes [nrow, nlanes) with
   // the subject (subject_rows); no task mixes the two.
   constexpr uint32_t kPer = 64 / P;  // lanes (DP pairs) per wave task
-  const uint32_t trow = (nrow + kPer - 1) / kPer, ntasks = trow + (nlanes - nrow + kPer - 1) / kPer;
+  const uint32_t trow = (nrow + kPer - 1) / kPer;
+  const bool tr = task >= trow;
+  const uint32_t L = (tr ? nrow + (task - trow) * kPer : task * kPer) + lane / P;
+  const bool active = L < (tr ? nlanes : nrow);
+  uint2 ln = active ? lanes[L] : make_uint2(~0u, ~0u);
+  const bool hasA = ln.x != ~0u, hasB = ln.y != ~0u;
+  SwJob ja = {}, jb = {};
+  if (hasA) ja = jobs[ln.x];
+  if (hasB) jb = jobs[ln.y];
+  const LanePair lp = lane_pair(tr, hasA, hasB, ja, jb, q, db);
+  const uint8_t* rpad = tr ? db.pad : q.pad;  // row and column letters
+  const uint8_t* cpad = tr ? q.pad : db.pad;
+  const int n = lp.n, mA = lp.mA, mB = lp.mB;
+  const uint32_t qp = lp.rp, ap = lp.ap, bp = lp.bp;
+  const int d1A = lp.d1A, d1B = lp.d1B, d0A = lp.d0A, d0B = lp.d0B;
+  // Rows as in k_sw16 (all lanes of a group have the same range).
+  int lo = 0, hi = -1;
+  if (hasA) {
+    lo = max(1, d0A + 1);
+    hi = min(n, mA + d1A);
+  }
+  if (hasB) {
+    lo = min(lo, max(1, d0B + 1));
+    hi = max(hi, min(n, mB + d1B));
+  }
+  const int steps = __builtin_amdgcn_readfirstlane(wave_max(hi - lo + 1));
+
+  // G/E of this lane's offsets; entry H is the E source past the last one
+  // (the next lane's first offset, or the wall for the last lane).
+  V G[H + 1], E[H + 1];
+  uint32_t R[NR];
+  // kWall: per offset, the addend that turns H into G (1 - o in the band,
+  // kWallOff past the DP's own width), as two halves.
+  V GO[kWall ? H : 1];
+  const int liveA = kWall ? (d1A - d0A + 1) - t0 : 0, liveB = kWall ? (d1B - d0B + 1) - t0 : 0;
+  const V LIVE2 = Tr::pair(liveA, liveB);
+  // Without kWall the last lane's offsets from kDeadFrom on lie past the
+  // band: their G takes the addend kWallOff instead of 1 - o, which keeps
+  // them (and the E they feed) far below any score, as a per-DP wall does;
+  // DADJ moves the F_in correction there by the same amount.
+  const V GOD = last ? Tr::splat(kWallOff) : GOFF;
+  const V DADJ = last ? Tr::splat(kWallOff - (1 - kOpen)) : ZERO2;
+#pragma unroll
+  for (int u = 0; u <= H; ++u) {
+    if (kWall && u < H) {
+      GO[u] = Tr::pair(u < liveA ? 1 - kOpen : kWallOff, u < liveB ? 1 - kOpen : kWallOff);
+      G[u] = Tr::pair(u < liveA ? -kOpen : kNeg16, u < liveB ? -kOpen : kNeg16);
+    } else {
+      const bool dead = u == H || (last && u >= kDeadFrom);
+      G[u] = dead ? NEG2 : Tr::splat(-kOpen);
+    }
+    E[u] = NEG2;
+  }
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int t = t0 + 2 * k;
+    uint32_t lo16 = 2 * k < H ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + t - 1),
+                                          pad_at(cpad, bp, mB, lo - d1B + t - 1))
+                              : 0u;
+    uint32_t hi16 = 2 * k + 1 < H ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + t),
+                                              pad_at(cpad, bp, mB, lo - d1B + t))
+                                  : 0u;
+    R[k] = lo16 | (hi16 << 16);
+  }
+  // Letters of the next row, loaded one row ahead (clamped, unconditional
+  // loads, settled at the end of the row): the query letter and the pair
+  // entering the last lane at offset P*H-1.
+  int qcur = (int)rpad[qp + (uint32_t)min(lo - 1, n)];
+  uint32_t pnext = 0;
+  V best = ZERO2;  // in the frame of the current row
+  V Zr = ZERO2;    // frame offset of the current row
+  // One row step. The window holds the entries of an even row (entry e at
+  // R[e / 2], half e % 2) plus, at entry H, the one entering the next row;
+  // the odd row (S = 1) reads entry u as entry u + 1 of that layout, so the
+  // window moves by whole registers once every two rows.
+  auto row = [&](int r, auto Sc) {
+    constexpr int S = decltype(Sc)::value;
+    const int i = lo + r;
+    // Raw letters of row i + 1: subject positions i + 1 - d1 + P*H - 2.
+    const int pos = i - 1 + P * H;
+    const int rq = (int)rpad[qp + (uint32_t)min(i, n)];
+    const int ra = pad_at(cpad, ap, mA, pos - d1A), rb = pad_at(cpad, bp, mB, pos - d1B);
+    {
+      // E source past the lane: the next lane's first offset, previous row.
+      const V g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
+      G[H] = last ? NEG2 : g0;
+      E[H] = last ? NEG2 : e0;
+    }
+    const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
+    auto score = [&](int u) {
+      const int e = u + S;
+      const uint32_t off = (e & 1) ? (R[e >> 1] >> 16) : (R[e >> 1] & 0xFFFFu);
+      return *reinterpret_cast<const int*>(Tq + off);
+    };
+    constexpr int kAhead = H < 4 ? H : 4;
+    int sq[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) sq[u] = score(u);
+    V F = NEG2, Gl = NEG2, Hodd = ZERO2;
+#pragma unroll
+    for (int u = 0; u < H; ++u) {
+      const V sc = Tr::bits(sq[u % kAhead]);
+      if (u + kAhead < H) sq[u % kAhead] = score(u + kAhead);
+      __builtin_amdgcn_sched_barrier(0);
+      // D and E (with the zero) do not depend on F: only max(Gl, F), the
+      // F decrement, the max3 and the G offset sit on the chain along the row.
+      const V D = Tr::add(G[u], sc);
+      const V Ev = Tr::max3(G[u + 1], E[u + 1], Zr);
+      const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
+      const V Hh = Tr::max3(D, Ev, Fv);
+      // The best score takes cells two at a time as they come (one max3
+      // per two cells, and no cell value stays live to the end of the row).
+      if (u & 1) best = Tr::max3(best, Hodd, Hh);
+      else if (u + 1 < H) Hodd = Hh;
+      else best = Tr::max(best, Hh);
+      Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : (u >= kDeadFrom ? GOD : GOFF));
+      G[u] = Gl;
+      E[u] = Ev;
+      F = Fv;
+    }
+    // F entering each lane: exclusive prefix maximum of
+    // F_out(q) - (P-1-q)*H over the group, plus (P-p)*H. (Every operand
+    // stays at or below some F of the row, which the f16 bound needs.)
+    const V fout = Tr::sub(Tr::max(Gl, F), ONE2);
+    V y = Tr::sub(fout, Tr::splat((P - 1 - p) * H));
+    if (P == 2) {  // one source lane: the exclusive prefix is its value
+    } else if (P == 4) {  // quad_perm sources stay inside the quad
+      y = Tr::max(y, mv(y, PrevC{}));
+      y = Tr::max(y, mv(y, Back2C{}));
+    } else if (P == 8) {  // row shifts reach into the other half-row: masked
+      const V s1 = mv(y, PrevC{});
+      y = p >= 1 ? Tr::max(y, s1) : y;
+      const V s2 = mv(y, Shr2C{});
+      y = p >= 2 ? Tr::max(y, s2) : y;
+      const V s4 = mv(y, Shr4C{});
+      y = p >= 4 ? Tr::max(y, s4) : y;
+    } else {  // the group is a whole row: a lane without source keeps its own
+      y = Tr::max(y, mvk(y, PrevC{}));
+      y = Tr::max(y, mvk(y, Shr2C{}));
+      y = Tr::max(y, mvk(y, Shr4C{}));
+      y = Tr::max(y, mvk(y, Shr8C{}));
+    }
+    V fin = Tr::add(mv(y, PrevC{}), Tr::splat((P - p) * H));
+    if (p == 0) fin = NEG2;
+    best = Tr::max(best, fin);
+    // The correction can only raise offsets u < F_in (F_in taken out of the
+    // frame): from there on F_in - o - u is at most -o, the least G. It runs
+    // in chunks of kChunk offsets while some lane of the wave still needs it
+    // (with walls: inside its DPs' bands).
+    {
+      const V fu = Tr::sub(fin, Zr);
+      using Sc16 = typename Tr::S;
+      const Sc16 fl = Tr::hmax(kWall ? Tr::min(fu, LIVE2) : fu);
+      V fc = kWall ? fin : Tr::add(fin, GOFF);
+      constexpr int kChunk = 8;
+#pragma unroll
+      for (int c = 0; c < H; c += kChunk) {
+        if (__ballot(fl > (Sc16)c) == 0) break;
+#pragma unroll
+        for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
+          G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0])
+                                     : (u >= kDeadFrom ? Tr::add(fc, DADJ) : fc));
+          // Settled here: left to itself the compiler applies every chunk's
+          // maxima after the last test, which keeps the whole ramp live
+          // (about 30 registers more).
+          asm volatile("" : "+v"(G[u]));
+          fc = Tr::sub(fc, ONE2);
+        }
+      }
+    }
+    // Letters of row i + 1.
+    qcur = settle(rq);
+    pnext = pair_off4(settle(ra), settle(rb));
+    // Frame of the next row; every kFrame rows the frame moves back (an
+    // even row never ends a frame).
+    best = Tr::add(best, ONE2);
+    Zr = Tr::add(Zr, ONE2);
+    if (S == 1 && (r & (kFrame - 1)) == kFrame - 1) {
+#pragma unroll
+      for (int u = 0; u < H; ++u) {
+        G[u] = Tr::sub(G[u], FRAME2);
+        E[u] = Tr::sub(E[u], FRAME2);
+      }
+      best = Tr::sub(best, FRAME2);
+      Zr = ZERO2;
+    }
+  };
+  // Entry e of the window gets the pair offset x.
+  auto put = [&](int e, uint32_t x) {
+    if (e & 1) R[e >> 1] = (R[e >> 1] & 0xFFFFu) | (x << 16);
+    else R[e >> 1] = x;
+  };
+  static_assert(kFrame % 2 == 0, "frames end on odd rows");
+  for (int r = 0; r < steps; r += 2) {
+    // The next lane's entries 0 and 1 of row r: the entries entering at
+    // H-1 in rows r+1 and r+2 (the last lane takes new pairs instead).
+    const uint32_t nx = (uint32_t)dppi((int)R[0], NextC{});
+    row(r, std::integral_constant<int, 0>{});
+    if (r + 1 >= steps) break;
+    put(H, last ? pnext : (nx & 0xFFFFu));
+    row(r + 1, std::integral_constant<int, 1>{});
+#pragma unroll
+    for (int k = 0; k + 1 < NR; ++k) R[k] = R[k + 1];
+    put(H - 1, last ? pnext : (nx >> 16));
+  }
+  best = Tr::sub(best, Zr);
+  best = Tr::max(best, mv(best, SwapC{}));
+  if (P >= 4) best = Tr::max(best, mv(best, Swap2C{}));
+  if (P >= 8) best = Tr::max(best, mv(best, MirrorC{}));
+  if (P == 16) best = Tr::max(best, mv(best, RowMirrorC{}));
+  const int bA = Tr::lo(best), bB = Tr::hi(best);
+  if (p == 0) {
+    if (hasA) {
+      if (bA >= Tr::kRedo || (bA < 11 && lp.clampA)) {
+        redo[atomicAdd(nredo, 1u)] = ln.x;
+      } else {
+        scores[ln.x] = bA;
+      }
+    }
+    if (hasB) {
+      if (bB >= Tr::kRedo || (bB < 11 && lp.clampB)) {
+        redo[atomicAdd(nredo, 1u)] = ln.y;
+      } else {
+        scores[ln.y] = bB;
+      }
+    }
+  }
+}
+
+template <int W, int P, int kThreads, class Tr, bool kWall>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<kThreads>))) void k_sw16p(const uint2* __restrict__ lanes, uint32_t nlanes, uint32_t nrow,
+                                                   const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
+                                                   const int* __restrict__ pair_tab,
+                                                   uint32_t* __restrict__ counter,
+                                                   int32_t* __restrict__ scores,
+                                                   uint32_t* __restrict__ redo,
+                                                   uint32_t* __restrict__ nredo) {
+  __shared__ int T[kPairTab];
+  for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
+  __syncthreads();
+  constexpr uint32_t kPer = 64 / P;
+  const uint32_t ntasks = (nrow + kPer - 1) / kPer + (nlanes - nrow + kPer - 1) / kPer;
+  const int lane = threadIdx.x & 63;
   for (;;) {
     const uint32_t task =
         (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
     if (task >= ntasks) break;
-    const bool tr = task >= trow;
-    const uint32_t L = (tr ? nrow + (task - trow) * kPer : task * kPer) + lane / P;
-    const bool active = L < (tr ? nlanes : nrow);
-    uint2 ln = active ? lanes[L] : make_uint2(~0u, ~0u);
-    const bool hasA = ln.x != ~0u, hasB = ln.y != ~0u;
-    SwJob ja = {}, jb = {};
-    if (hasA) ja = jobs[ln.x];
-    if (hasB) jb = jobs[ln.y];
-    const LanePair lp = lane_pair(tr, hasA, hasB, ja, jb, q, db);
-    const uint8_t* rpad = tr ? db.pad : q.pad;  // row and column letters
-    const uint8_t* cpad = tr ? q.pad : db.pad;
-    const int n = lp.n, mA = lp.mA, mB = lp.mB;
-    const uint32_t qp = lp.rp, ap = lp.ap, bp = lp.bp;
-    const int d1A = lp.d1A, d1B = lp.d1B, d0A = lp.d0A, d0B = lp.d0B;
-    // Rows as in k_sw16 (all lanes of a group have the same range).
-    int lo = 0, hi = -1;
-    if (hasA) {
-      lo = max(1, d0A + 1);
-      hi = min(n, mA + d1A);
-    }
-    if (hasB) {
-      lo = min(lo, max(1, d0B + 1));
-      hi = max(hi, min(n, mB + d1B));
-    }
-    const int steps = __builtin_amdgcn_readfirstlane(wave_max(hi - lo + 1));
+    sw16p_task<W, P, Tr, kWall>(task, lanes, nlanes, nrow, jobs, q, db, T, scores, redo, nredo);
+  }
+}
 
-    // G/E of this lane's offsets; entry H is the E source past the last one
-    // (the next lane's first offset, or the wall for the last lane).
-    V G[H + 1], E[H + 1];
-    uint32_t R[NR];
-    // kWall: per offset, the addend that turns H into G (1 - o in the band,
-    // kWallOff past the DP's own width), as two halves.
-    V GO[kWall ? H : 1];
-    const int liveA = kWall ? (d1A - d0A + 1) - t0 : 0, liveB = kWall ? (d1B - d0B + 1) - t0 : 0;
-    const V LIVE2 = Tr::pair(liveA, liveB);
-    // Without kWall the last lane's offsets from kDeadFrom on lie past the
-    // band: their G takes the addend kWallOff instead of 1 - o, which keeps
-    // them (and the E they feed) far below any score, as a per-DP wall does;
-    // DADJ moves the F_in correction there by the same amount.
-    const V GOD = last ? Tr::splat(kWallOff) : GOFF;
-    const V DADJ = last ? Tr::splat(kWallOff - (1 - kOpen)) : ZERO2;
-#pragma unroll
-    for (int u = 0; u <= H; ++u) {
-      if (kWall && u < H) {
-        GO[u] = Tr::pair(u < liveA ? 1 - kOpen : kWallOff, u < liveB ? 1 - kOpen : kWallOff);
-        G[u] = Tr::pair(u < liveA ? -kOpen : kNeg16, u < liveB ? -kOpen : kNeg16);
-      } else {
-        const bool dead = u == H || (last && u >= kDeadFrom);
-        G[u] = dead ? NEG2 : Tr::splat(-kOpen);
-      }
-      E[u] = NEG2;
-    }
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      const int t = t0 + 2 * k;
-      uint32_t lo16 = 2 * k < H ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + t - 1),
-                                            pad_at(cpad, bp, mB, lo - d1B + t - 1))
-                                : 0u;
-      uint32_t hi16 = 2 * k + 1 < H ? pair_off4(pad_at(cpad, ap, mA, lo - d1A + t),
-                                                pad_at(cpad, bp, mB, lo - d1B + t))
-                                    : 0u;
-      R[k] = lo16 | (hi16 << 16);
-    }
-    // Letters of the next row, loaded one row ahead (clamped, unconditional
-    // loads, settled at the end of the row): the query letter and the pair
-    // entering the last lane at offset P*H-1.
-    int qcur = (int)rpad[qp + (uint32_t)min(lo - 1, n)];
-    uint32_t pnext = 0;
-    V best = ZERO2;  // in the frame of the current row
-    V Zr = ZERO2;    // frame offset of the current row
-    // One row step. The window holds the entries of an even row (entry e at
-    // R[e / 2], half e % 2) plus, at entry H, the one entering the next row;
-    // the odd row (S = 1) reads entry u as entry u + 1 of that layout, so the
-    // window moves by whole registers once every two rows.
-    auto row = [&](int r, auto Sc) {
-      constexpr int S = decltype(Sc)::value;
-      const int i = lo + r;
-      // Raw letters of row i + 1: subject positions i + 1 - d1 + P*H - 2.
-      const int pos = i - 1 + P * H;
-      const int rq = (int)rpad[qp + (uint32_t)min(i, n)];
-      const int ra = pad_at(cpad, ap, mA, pos - d1A), rb = pad_at(cpad, bp, mB, pos - d1B);
-      {
-        // E source past the lane: the next lane's first offset, previous row.
-        const V g0 = mv(G[0], NextC{}), e0 = mv(E[0], NextC{});
-        G[H] = last ? NEG2 : g0;
-        E[H] = last ? NEG2 : e0;
-      }
-      const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
-      auto score = [&](int u) {
-        const int e = u + S;
-        const uint32_t off = (e & 1) ? (R[e >> 1] >> 16) : (R[e >> 1] & 0xFFFFu);
-        return *reinterpret_cast<const int*>(Tq + off);
-      };
-      constexpr int kAhead = H < 4 ? H : 4;
-      int sq[kAhead];
-#pragma unroll
-      for (int u = 0; u < kAhead; ++u) sq[u] = score(u);
-      V F = NEG2, Gl = NEG2, Hodd = ZERO2;
-#pragma unroll
-      for (int u = 0; u < H; ++u) {
-        const V sc = Tr::bits(sq[u % kAhead]);
-        if (u + kAhead < H) sq[u % kAhead] = score(u + kAhead);
-        __builtin_amdgcn_sched_barrier(0);
-        // D and E (with the zero) do not depend on F: only max(Gl, F), the
-        // F decrement, the max3 and the G offset sit on the chain along the row.
-        const V D = Tr::add(G[u], sc);
-        const V Ev = Tr::max3(G[u + 1], E[u + 1], Zr);
-        const V Fv = Tr::sub(Tr::max(Gl, F), ONE2);
-        const V Hh = Tr::max3(D, Ev, Fv);
-        // The best score takes cells two at a time as they come (one max3
-        // per two cells, and no cell value stays live to the end of the row).
-        if (u & 1) best = Tr::max3(best, Hodd, Hh);
-        else if (u + 1 < H) Hodd = Hh;
-        else best = Tr::max(best, Hh);
-        Gl = Tr::add(Hh, kWall ? GO[kWall ? u : 0] : (u >= kDeadFrom ? GOD : GOFF));
-        G[u] = Gl;
-        E[u] = Ev;
-        F = Fv;
-      }
-      // F entering each lane: exclusive prefix maximum of
-      // F_out(q) - (P-1-q)*H over the group, plus (P-p)*H. (Every operand
-      // stays at or below some F of the row, which the f16 bound needs.)
-      const V fout = Tr::sub(Tr::max(Gl, F), ONE2);
-      V y = Tr::sub(fout, Tr::splat((P - 1 - p) * H));
-      if (P == 2) {  // one source lane: the exclusive prefix is its value
-      } else if (P == 4) {  // quad_perm sources stay inside the quad
-        y = Tr::max(y, mv(y, PrevC{}));
-        y = Tr::max(y, mv(y, Back2C{}));
-      } else if (P == 8) {  // row shifts reach into the other half-row: masked
-        const V s1 = mv(y, PrevC{});
-        y = p >= 1 ? Tr::max(y, s1) : y;
-        const V s2 = mv(y, Shr2C{});
-        y = p >= 2 ? Tr::max(y, s2) : y;
-        const V s4 = mv(y, Shr4C{});
-        y = p >= 4 ? Tr::max(y, s4) : y;
-      } else {  // the group is a whole row: a lane without source keeps its own
-        y = Tr::max(y, mvk(y, PrevC{}));
-        y = Tr::max(y, mvk(y, Shr2C{}));
-        y = Tr::max(y, mvk(y, Shr4C{}));
-        y = Tr::max(y, mvk(y, Shr8C{}));
-      }
-      V fin = Tr::add(mv(y, PrevC{}), Tr::splat((P - p) * H));
-      if (p == 0) fin = NEG2;
-      best = Tr::max(best, fin);
-      // The correction can only raise offsets u < F_in (F_in taken out of the
-      // frame): from there on F_in - o - u is at most -o, the least G. It runs
-      // in chunks of kChunk offsets while some lane of the wave still needs it
-      // (with walls: inside its DPs' bands).
-      {
-        const V fu = Tr::sub(fin, Zr);
-        using Sc16 = typename Tr::S;
-        const Sc16 fl = Tr::hmax(kWall ? Tr::min(fu, LIVE2) : fu);
-        V fc = kWall ? fin : Tr::add(fin, GOFF);
-        constexpr int kChunk = 8;
-#pragma unroll
-        for (int c = 0; c < H; c += kChunk) {
-          if (__ballot(fl > (Sc16)c) == 0) break;
-#pragma unroll
-          for (int u = c; u < (c + kChunk < H ? c + kChunk : H); ++u) {
-            G[u] = Tr::max(G[u], kWall ? Tr::add(fc, GO[kWall ? u : 0])
-                                       : (u >= kDeadFrom ? Tr::add(fc, DADJ) : fc));
-            // Settled here: left to itself the compiler applies every chunk's
-            // maxima after the last test, which keeps the whole ramp live
-            // (about 30 registers more).
-            asm volatile("" : "+v"(G[u]));
-            fc = Tr::sub(fc, ONE2);
-          }
-        }
-      }
-      // Letters of row i + 1.
-      qcur = settle(rq);
-      pnext = pair_off4(settle(ra), settle(rb));
-      // Frame of the next row; every kFrame rows the frame moves back (an
-      // even row never ends a frame).
-      best = Tr::add(best, ONE2);
-      Zr = Tr::add(Zr, ONE2);
-      if (S == 1 && (r & (kFrame - 1)) == kFrame - 1) {
-#pragma unroll
-        for (int u = 0; u < H; ++u) {
-          G[u] = Tr::sub(G[u], FRAME2);
-          E[u] = Tr::sub(E[u], FRAME2);
-        }
-        best = Tr::sub(best, FRAME2);
-        Zr = ZERO2;
-      }
-    };
-    // Entry e of the window gets the pair offset x.
-    auto put = [&](int e, uint32_t x) {
-      if (e & 1) R[e >> 1] = (R[e >> 1] & 0xFFFFu) | (x << 16);
-      else R[e >> 1] = x;
-    };
-    static_assert(kFrame % 2 == 0, "frames end on odd rows");
-    for (int r = 0; r < steps; r += 2) {
-      // The next lane's entries 0 and 1 of row r: the entries entering at
-      // H-1 in rows r+1 and r+2 (the last lane takes new pairs instead).
-      const uint32_t nx = (uint32_t)dppi((int)R[0], NextC{});
-      row(r, std::integral_constant<int, 0>{});
-      if (r + 1 >= steps) break;
-      put(H, last ? pnext : (nx & 0xFFFFu));
-      row(r + 1, std::integral_constant<int, 1>{});
-#pragma unroll
-      for (int k = 0; k + 1 < NR; ++k) R[k] = R[k + 1];
-      put(H - 1, last ? pnext : (nx >> 16));
-    }
-    best = Tr::sub(best, Zr);
-    best = Tr::max(best, mv(best, SwapC{}));
-    if (P >= 4) best = Tr::max(best, mv(best, Swap2C{}));
-    if (P >= 8) best = Tr::max(best, mv(best, MirrorC{}));
-    if (P == 16) best = Tr::max(best, mv(best, RowMirrorC{}));
-    const int bA = Tr::lo(best), bB = Tr::hi(best);
-    if (p == 0) {
-      if (hasA) {
-        if (bA >= Tr::kRedo || (bA < 11 && lp.clampA)) {
-          redo[atomicAdd(nredo, 1u)] = ln.x;
-        } else {
-          scores[ln.x] = bA;
-        }
-      }
-      if (hasB) {
-        if (bB >= Tr::kRedo || (bB < 11 && lp.clampB)) {
-          redo[atomicAdd(nredo, 1u)] = ln.y;
-        } else {
-          scores[ln.y] = bB;
-        }
-      }
+// Classes of the fused launch, as X(kind, class, width, lanes per DP pair,
+// wall), in task order: the merged bands (long DPs) first.
+#define DMR_SW16_FUSED(X) \
+  X(0, 10, 257, 8, true)  \
+  X(1, 8, 193, 8, true)   \
+  X(2, 5, 97, 8, true)    \
+  X(3, 3, 81, 2, false)   \
+  X(4, 2, 61, 2, false)
+// Fused launch of several packed classes (launch_sw16_fused): one persistent
+// grid whose waves take tasks of any of the classes, the merged bands first.
+// The classes share one block size and two waves per SIMD (at most 256
+// registers), so a class with few long tasks no longer holds whole CUs while
+// the others wait for them: its tasks and the others' interleave on the same
+// SIMDs, and the others' short tasks fill its tail.
+template <class Tr>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_sw16p_fused(
+    FusedSections a, const SwJob* __restrict__ jobs, SeqSet q, SeqSet db, const int* __restrict__ pair_tab,
+    uint32_t* __restrict__ counter, int32_t* __restrict__ scores, uint32_t* __restrict__ redo,
+    uint32_t* __restrict__ nredo) {
+  __shared__ int T[kPairTab];
+  for (int k = threadIdx.x; k < kPairTab; k += blockDim.x) T[k] = pair_tab[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    const uint32_t task =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
+    if (task >= a.ntasks) break;
+    int k = 0;
+    while (k + 1 < kFusedKinds && task >= a.task0[k + 1]) ++k;
+    const uint32_t t = task - a.task0[k];
+    const uint2* ln = a.lanes[k];
+    const uint32_t nl = a.nlanes[k], nr = a.nrow[k];
+    switch (k) {
+#define DMR_SW16_FUSED_CASE(i, c, w, p, wall) \
+      case i:                                  \
+        sw16p_task<w, p, Tr, wall>(t, ln, nl, nr, jobs, q, db, T, scores, redo, nredo); \
+        break;
+      DMR_SW16_FUSED(DMR_SW16_FUSED_CASE)
+#undef DMR_SW16_FUSED_CASE
+      default:
+        break;
     }
   }
 }

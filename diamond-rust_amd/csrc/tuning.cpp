@@ -170,6 +170,7 @@ ScoreLayout default_layout() {
   // queries meet shorter subjects), the query elsewhere.
   for (int c = 0; c < kPackedClasses; ++c) L.transpose[c] = 0;
   L.transpose[4] = 2;
+  L.fuse = false;
   return L;
 }
 
@@ -180,7 +181,8 @@ uint32_t transpose_modes(const ScoreLayout& L) {
 }
 
 bool is_layout_key(const std::string& k) {
-  if (k == "merged_mask" || k == "f16" || k == "score_plan" || k == "graph" || k == "transpose") return true;
+  if (k == "merged_mask" || k == "f16" || k == "score_plan" || k == "graph" || k == "transpose" || k == "fuse")
+    return true;
   for (const char* pre : {"lanes_", "threads_", "transpose_"}) {
     const size_t n = std::strlen(pre);
     if (k.compare(0, n, pre) == 0 && class_of(k.substr(n)) >= 0) return true;
@@ -218,6 +220,8 @@ void apply_settings(ScoreLayout* L, const Settings& s) {
       L->f16 = v != 0;
     } else if (k == "graph") {
       L->graph = v != 0;
+    } else if (k == "fuse") {
+      L->fuse = v != 0;
     } else if (k == "transpose" || k.compare(0, 10, "transpose_") == 0) {
       const int c = k == "transpose" ? -1 : class_of(k.substr(10));
       if (c < 0 && k != "transpose") bad("unknown tuning key: " + k);
@@ -240,6 +244,7 @@ double layout_value(const ScoreLayout& L, const std::string& k) {
   if (k == "f16") return L.f16 ? 1 : 0;
   if (k == "score_plan") return L.plan;
   if (k == "graph") return L.graph ? 1 : 0;
+  if (k == "fuse") return L.fuse ? 1 : 0;
   if (k == "transpose") {
     for (int c = 1; c < kPackedClasses; ++c)
       if (L.transpose[c] != L.transpose[0]) return -1;

@@ -21,6 +21,7 @@ struct ScoreLayout {
   int plan;                     // how the class kernels share the streams (0, 1, 2)
   bool graph;                   // ungapped .. score-pass preprocessing replayed as one hipGraph
   int transpose[kPackedClasses];  // row sequence of a packed DP: 0 query, 1 fewer rows, 2 subject
+  bool fuse;                    // m257, m193, m97, w81, w61 in one launch (k_sw16p_fused)
 };
 // Row-sequence modes of the packed classes, 2 bits per class (k_classify).
 uint32_t transpose_modes(const ScoreLayout& L);
@@ -38,7 +39,8 @@ using Settings = std::map<std::string, double>;
 // block size for the class's lanes), "merged_mask" (6 bits; the widest merged
 // kernel is always in use), "f16" (0/1), "score_plan" (0-2), "graph" (0/1),
 // "transpose_<class>" (0-2: which sequence runs along the rows of the
-// class's DPs), "transpose" (every class's; reads -1 when they differ).
+// class's DPs), "transpose" (every class's; reads -1 when they differ),
+// "fuse" (0/1).
 // Throws Error(-1) naming the key on an unknown key or a value with no
 // compiled kernel.
 void apply_settings(ScoreLayout* L, const Settings& s);

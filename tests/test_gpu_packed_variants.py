@@ -113,6 +113,35 @@ def merged_cases(seed=91, count=420):
     return cases
 
 
+@pytest.mark.parametrize("f16", [1, 0], ids=["f16", "i16"])
+def test_fused_launch_band_widths(dmr, orc, f16):
+    """The fused launch (merged bands with walls, widths 81 and 61 split over
+    two lanes) on drop-in DPs of every width it takes and of the others."""
+    cases = merged_cases(seed=93, count=300)
+    rng = np.random.default_rng(94)
+    for _ in range(200):
+        band = int(rng.choice([30, 40]))
+        n = int(rng.integers(1, 700))
+        q = rand_seq(rng, n)
+        s = q.copy() if rng.random() < 0.3 else mutate(rng, q, float(rng.uniform(0.1, 0.6)), 0.06)
+        if len(s) == 0:
+            s = rand_seq(rng, 2)
+        c0 = int(rng.integers(-len(s) - band, n + band))
+        cases.append((q, s, c0 - band, c0 + band))
+    ref = [orc.extend_span(*c) for c in cases]
+    ref = [r["score"] if r else None for r in ref]
+    c = dmr.Context(0)
+    c.load_tuning(None)
+    c.set_tuning("f16", f16)
+    c.set_tuning("fuse", 1)
+    got = _score_batch(c, cases)
+    prof = c.class_profile()
+    assert prof["w81"]["ms"] > 0 and prof["m257"]["ms"] > 0
+    for cs, g, r in zip(cases, got, ref):
+        assert g == r, (len(cs[0]), len(cs[1]), cs[2], cs[3])
+    c.close()
+
+
 def test_merged_band_widths(layout_ctx, orc):
     """Bands of widths 62..256 other than 81 and 129 (merged bands) run the
     split kernels with a wall at each DP's own width (the default merged
@@ -143,17 +172,19 @@ def test_batched_search_both_layouts(dmr, layout_ctx, orc, synth):
     assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
 
 
-@pytest.mark.parametrize("transpose", [0, 1, 2])
-def test_batched_search_orientations(dmr, orc, synth, transpose):
+@pytest.mark.parametrize("transpose,fuse", [(0, 0), (1, 0), (2, 0), (1, 1), (2, 1)])
+def test_batched_search_orientations(dmr, orc, synth, transpose, fuse):
     """The 24-query subset with every packed DP's query on the rows (lanes
     pair DPs of one query) and with every packed DP's subject on the rows
-    (lanes pair DPs of one subject, from different queries)."""
+    (lanes pair DPs of one subject, from different queries); with fuse, the
+    merged-band, width-81 and width-61 classes in one launch."""
     w = synth.make("blastp-sensitive-1024").subset_queries(list(range(5, 1024, 43)))
     w = workload_dict(w)
     c = dmr.Context(0)
     c.load_tuning(None)
     c.set_tuning("transpose", transpose)
-    assert c.get_tuning("transpose") == transpose
+    c.set_tuning("fuse", fuse)
+    assert c.get_tuning("transpose") == transpose and c.get_tuning("fuse") == fuse
     res, ref = run_both(dmr, c, orc, w, "sensitive")
     assert len(ref) > 0
     assert res.format(w["q_ids"], w["db_ids"], EXTENDED) == ref.format(w["q_ids"], w["db_ids"], EXTENDED)
