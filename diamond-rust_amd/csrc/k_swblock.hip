@@ -269,14 +269,22 @@ __global__ __launch_bounds__(kBT) void k_sw_block(const SwJob* __restrict__ jobs
       }
     }
   } else {
-    if (tid != 0) return;
+    if (tid >= 64) return;  // the first wave walks the trace back
+    const int lane = tid;
     const int best = found ? stop_at : 0;
     if (!found || (best < 11 && istar < n)) {
-      fallback[atomicAdd(nfallback, 1u)] = task;
-      outs[task] = SwOut{};
+      if (lane == 0) {
+        fallback[atomicAdd(nfallback, 1u)] = task;
+        outs[task] = SwOut{};
+      }
       return;
     }
-    // Traceback from (bi, bj) through this block's trace rows.
+    // Traceback from (bi, bj) through this block's trace rows: the
+    // reference's loop (src/gapped.rs:222-265) a run of steps at a time, as
+    // in the wave kernel's walk: lane k reads the trace byte of step k in the
+    // current state's direction (the diagonal in state H, the column in state
+    // E, the row in state F), checks the loop condition and the row range
+    // there, and one ballot gives the run's length.
     SwOut o = {};
     o.score = best;
     o.best_i = (uint32_t)bi;
@@ -286,66 +294,83 @@ __global__ __launch_bounds__(kBT) void k_sw_block(const SwJob* __restrict__ jobs
     uint32_t nr = 0, cur = 3, cnt = 0;
     int state = 0, i = bi, j = bj;
     bool in_gap = false;
+    auto push = [&](uint32_t code, uint32_t n) {
+      if (code == cur) {
+        cnt += n;
+      } else {
+        if (cnt) {
+          if (lane == 0) op[nr] = cur | (cnt << 2);
+          ++nr;
+        }
+        cur = code;
+        cnt = n;
+      }
+    };
     while (i > 0 && j > 0) {
       if (j < R.lo(i) || j > R.hi(i) || i < rlo) break;
-      const int tv = tr[(uint64_t)(i - rlo) * (kBT * K) + (uint64_t)(j - i + d1)];
-      uint32_t code;
+      const int di = state == 2 ? 0 : 1, dj = state == 1 ? 0 : 1;
+      const int ik = i - lane * di, jk = j - lane * dj;
+      const bool ok = ik > 0 && jk > 0 && jk >= R.lo(ik) && jk <= R.hi(ik) && ik >= rlo;
+      const int tv = ok ? (int)tr[(uint64_t)(ik - rlo) * (kBT * K) + (uint64_t)(jk - ik + d1)] : 0;
+      const uint64_t bad = __ballot(!ok);  // bit k: the loop stops before step k
       if (state == 0) {
-        const int c = tv & 3;
+        const int c = __builtin_amdgcn_readfirstlane(tv) & 3;  // lane 0's cell
         if (c == 0) break;
-        if (c == 2) {
-          state = 1;
+        if (c == 2 || c == 3) {
+          state = c - 1;
           continue;
         }
-        if (c == 3) {
-          state = 2;
-          continue;
+        const uint64_t stopm = bad | __ballot(ok && (tv & 3) != 1);
+        const int run = stopm ? __builtin_ctzll(stopm) : 64;
+        const bool in_run = lane < run;
+        int x = 0, y = 0;
+        if (in_run) {
+          x = qs[ik - 1];
+          y = ss[jk - 1];
         }
-        const int x = qs[i - 1], y = ss[j - 1];
-        o.length++;
-        if (x == y) o.identities++;
-        else o.mismatches++;
-        if (M[x * kScoreStride + y] > 0) o.positives++;
+        const int idn = __builtin_popcountll(__ballot(in_run && x == y));
+        o.length += run;
+        o.identities += idn;
+        o.mismatches += run - idn;
+        o.positives += __builtin_popcountll(__ballot(in_run && M[x * kScoreStride + y] > 0));
         in_gap = false;
-        code = 0;
-        --i;
-        --j;
+        push(0, (uint32_t)run);
+        i -= run;
+        j -= run;
       } else {
-        o.length++;
-        o.gaps++;
+        const int bit = state == 1 ? 2 : 3;
+        const uint64_t last = __ballot(ok && ((tv >> bit) & 1) == 0);
+        const int f = bad ? __builtin_ctzll(bad) : 64;
+        const int z = last ? __builtin_ctzll(last) : 64;
+        const int g = z < f ? z + 1 : f;
+        o.length += g;
+        o.gaps += g;
         if (!in_gap) {
           o.gap_openings++;
           in_gap = true;
         }
-        if (state == 1) {
-          code = 1;
-          if (((tv >> 2) & 1) == 0) state = 0;
-          --i;
-        } else {
-          code = 2;
-          if (((tv >> 3) & 1) == 0) state = 0;
-          --j;
-        }
-      }
-      if (code == cur) {
-        ++cnt;
-      } else {
-        if (cnt) op[nr++] = cur | (cnt << 2);
-        cur = code;
-        cnt = 1;
+        push(state == 1 ? 1u : 2u, (uint32_t)g);
+        if (state == 1) i -= g;
+        else j -= g;
+        if (z < f) state = 0;
       }
     }
-    if (cnt) op[nr++] = cur | (cnt << 2);
-    for (uint32_t a = 0, b = nr ? nr - 1 : 0; a < b; ++a, --b) {
-      const uint32_t x = op[a];
-      op[a] = op[b];
+    if (cnt) {
+      if (lane == 0) op[nr] = cur | (cnt << 2);
+      ++nr;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");  // lane 0's ops are read by every lane
+    // The walk runs backwards: reverse the ops, one pair per lane.
+    for (uint32_t a = lane; a < nr / 2; a += 64) {
+      const uint32_t b = nr - 1 - a, x = op[a], y = op[b];
+      op[a] = y;
       op[b] = x;
     }
     o.n_ops = nr;
     o.q_begin = (uint32_t)i;
     o.s_begin = (uint32_t)j;
     if (nr == 0) o.score = 0;
-    outs[task] = o;
+    if (lane == 0) outs[task] = o;
   }
 }
 
