@@ -324,8 +324,8 @@ def leg_serving(dmr, synth, duration_s=10.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="blastp-sensitive-1024")
     ap.add_argument("--cpu-sample", type=int, default=96, help="queries timed on the CPU oracle")
     ap.add_argument("--plain-sample", type=int, default=4, help="queries timed on the plain form")
