@@ -13,8 +13,9 @@ equal the first one's, or the tuner stops.
 Search: coordinate descent from the built-in defaults. Each step tries the
 values of one setting with the others held, and keeps a value only when its
 median search time beats the best so far by more than --min-gain (noise
-guard): f16; the graph; the fused launch; the merged-band kernels in use (one bucket bound
-added or removed at a time, until no change helps); the row sequence
+guard): f16; the graph; the fused launch, alone and with every merged-band bucket in
+use; the merged-band kernels in use (one bucket bound added or removed at a time, until
+no change helps); the row sequence
 (transpose_<class>) and the kernel of each class that has DPs in the
 workload; the stream plan; the merged buckets once more.
 
@@ -107,6 +108,12 @@ class Tuner:
         consider({**cur, "f16": 1 - cur["f16"]}, {"f16": 1 - cur["f16"]})
         consider({**cur, "graph": 1 - cur["graph"]}, {"graph": 1 - cur["graph"]})
         consider({**cur, "fuse": 1 - cur["fuse"]}, {"fuse": 1 - cur["fuse"]})
+        # Every merged bucket pays off only inside the fused launch (apart,
+        # each is a launch of its own, with its own tail), which one step of
+        # either setting alone does not show.
+        every = (1 << len(MERGED)) - 1
+        if (cur["fuse"], cur["merged_mask"]) != (1, every):
+            consider({**cur, "fuse": 1, "merged_mask": every}, {"fuse": 1, "merged_mask": every})
 
         def merged_pass():
             changed = True

@@ -204,18 +204,20 @@ bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2
 bool launch_sw16_merged(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
                         uint32_t nrow, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter,
                         int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
-// Fused launch of the packed classes m257, m193, m97 (eight lanes per DP
-// pair, per-DP walls), w81 and w61 (two lanes per pair): one persistent
+// Fused launch of the packed classes m257, m225, m193, m161, m129, m97
+// (eight lanes per DP pair, per-DP walls), w81 and w61 (two lanes per
+// pair): one persistent
 // grid of 512-thread blocks, kinds in that order (k_sw16p_fused). Per kind:
 // its lanes, lane count and query-row lane count; task0 are the first task
 // of each kind (computed by launch_sw16_fused).
-constexpr int kFusedKinds = 5;
+constexpr int kFusedKinds = 8;
 struct FusedSections {
   const uint2* lanes[kFusedKinds];
   uint32_t nlanes[kFusedKinds], nrow[kFusedKinds], task0[kFusedKinds];
   uint32_t ntasks;
 };
-// Packed class of each fused kind (m257 m193 m97 w81 w61).
+// Packed class of each fused kind (the merged bands widest first, then w81
+// and w61).
 int sw16_fused_class(int kind);
 bool launch_sw16_fused(bool f16, FusedSections s, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab,
                        uint32_t* counter, int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count,

@@ -1,5 +1,5 @@
-// Fused packed score pass (k_sw16p_fused in sw16_kernels.hpp): the merged
-// bands m257, m193, m97 and the band_for widths 81 and 61 in one persistent
+// Fused packed score pass (k_sw16p_fused in sw16_kernels.hpp): the six
+// merged-band classes and the band_for widths 81 and 61 in one persistent
 // launch. A translation unit of its own, so it compiles beside the others.
 #include "sw16_kernels.hpp"
 

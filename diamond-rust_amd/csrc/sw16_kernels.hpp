@@ -56,6 +56,13 @@ constexpr int kPairTab = kCodes * kPairRow;
 constexpr short kNeg16 = -16384;
 constexpr int kOpen = 12;                   // gap open + extend of the 16-bit tables
 constexpr int kFrame = 64;                  // rows between frame moves
+// The pair table in LDS, typed so that a task body compiled as a function of
+// its own (the fused kernel) still reads it with LDS instructions.
+typedef const __attribute__((address_space(3))) int LdsInt;
+typedef const __attribute__((address_space(3))) char LdsChar;
+// Letters in global memory, typed likewise (global loads, not flat ones,
+// whose completion the LDS waits of the row would also have to cover).
+typedef const __attribute__((address_space(1))) uint8_t GlobalByte;
 
 // Arithmetic of the packed kernels, two DPs per register. A value in the
 // frame of its row is the true value plus at most kFrame, and no value exceeds
@@ -123,7 +130,8 @@ __device__ __forceinline__ int wave_max(int v) {
 // padding code on either side, so one clamp replaces the range tests.
 // Sequences are addressed by 32-bit offsets from the kernel-argument base,
 // which keeps per-lane state to one register per sequence.
-__device__ __forceinline__ int pad_at(const uint8_t* pad, uint32_t base, int m, int p) {
+template <class Bytes>
+__device__ __forceinline__ int pad_at(Bytes* pad, uint32_t base, int m, int p) {
   return (int)pad[base + (uint32_t)min(max(p, -1), m)];
 }
 // Byte offset of the score-table row of a query letter (times 4). The empty
@@ -446,7 +454,7 @@ constexpr int kMinWaves = kThreads / 256;
 template <int W, int P, class Tr, bool kWall>
 __device__ __forceinline__ void sw16p_task(uint32_t task, const uint2* __restrict__ lanes, uint32_t nlanes,
                                            uint32_t nrow, const SwJob* __restrict__ jobs, SeqSet q, SeqSet db,
-                                           const int* T, int32_t* __restrict__ scores, uint32_t* __restrict__ redo,
+                                           LdsInt* T, int32_t* __restrict__ scores, uint32_t* __restrict__ redo,
                                            uint32_t* __restrict__ nredo) {
   static_assert(W % 2 == 1, "band widths are odd");
   static_assert(P == 2 || P == 4 || P == 8 || P == 16, "lane groups of 2, 4, 8 or 16");
@@ -500,8 +508,8 @@ __device__ __forceinline__ void sw16p_task(uint32_t task, const uint2* __restric
   if (hasA) ja = jobs[ln.x];
   if (hasB) jb = jobs[ln.y];
   const LanePair lp = lane_pair(tr, hasA, hasB, ja, jb, q, db);
-  const uint8_t* rpad = tr ? db.pad : q.pad;  // row and column letters
-  const uint8_t* cpad = tr ? q.pad : db.pad;
+  GlobalByte* rpad = (GlobalByte*)(tr ? db.pad : q.pad);  // row and column letters
+  GlobalByte* cpad = (GlobalByte*)(tr ? q.pad : db.pad);
   const int n = lp.n, mA = lp.mA, mB = lp.mB;
   const uint32_t qp = lp.rp, ap = lp.ap, bp = lp.bp;
   const int d1A = lp.d1A, d1B = lp.d1B, d0A = lp.d0A, d0B = lp.d0B;
@@ -578,11 +586,11 @@ __device__ __forceinline__ void sw16p_task(uint32_t task, const uint2* __restric
       G[H] = last ? NEG2 : g0;
       E[H] = last ? NEG2 : e0;
     }
-    const char* Tq = reinterpret_cast<const char*>(T) + row_base(qcur);
+    LdsChar* Tq = reinterpret_cast<LdsChar*>(T) + row_base(qcur);
     auto score = [&](int u) {
       const int e = u + S;
       const uint32_t off = (e & 1) ? (R[e >> 1] >> 16) : (R[e >> 1] & 0xFFFFu);
-      return *reinterpret_cast<const int*>(Tq + off);
+      return *reinterpret_cast<LdsInt*>(Tq + off);
     };
     constexpr int kAhead = H < 4 ? H : 4;
     int sq[kAhead];
@@ -737,18 +745,32 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
     const uint32_t task =
         (uint32_t)__builtin_amdgcn_readfirstlane((int)atomicAdd(counter, lane == 0 ? 1u : 0u));
     if (task >= ntasks) break;
-    sw16p_task<W, P, Tr, kWall>(task, lanes, nlanes, nrow, jobs, q, db, T, scores, redo, nredo);
+    sw16p_task<W, P, Tr, kWall>(task, lanes, nlanes, nrow, jobs, q, db, (LdsInt*)T, scores, redo, nredo);
   }
+}
+
+// A task of the fused kernel as a function of its own: each class then gets
+// its own register allocation (inlined into one loop, the eight bodies
+// spill).
+template <int W, int P, class Tr, bool kWall>
+__device__ __attribute__((noinline)) void sw16p_task_call(uint32_t task, const uint2* __restrict__ lanes,
+                                                          uint32_t nlanes, uint32_t nrow, const SwJob* __restrict__ jobs,
+                                                          SeqSet q, SeqSet db, LdsInt* T, int32_t* __restrict__ scores,
+                                                          uint32_t* __restrict__ redo, uint32_t* __restrict__ nredo) {
+  sw16p_task<W, P, Tr, kWall>(task, lanes, nlanes, nrow, jobs, q, db, T, scores, redo, nredo);
 }
 
 // Classes of the fused launch, as X(kind, class, width, lanes per DP pair,
 // wall), in task order: the merged bands (long DPs) first.
 #define DMR_SW16_FUSED(X) \
   X(0, 10, 257, 8, true)  \
-  X(1, 8, 193, 8, true)   \
-  X(2, 5, 97, 8, true)    \
-  X(3, 3, 81, 2, false)   \
-  X(4, 2, 61, 2, false)
+  X(1, 9, 225, 8, true)   \
+  X(2, 8, 193, 8, true)   \
+  X(3, 7, 161, 8, true)   \
+  X(4, 6, 129, 8, true)   \
+  X(5, 5, 97, 8, true)    \
+  X(6, 3, 81, 2, false)   \
+  X(7, 2, 61, 2, false)
 // Fused launch of several packed classes (launch_sw16_fused): one persistent
 // grid whose waves take tasks of any of the classes, the merged bands first.
 // The classes share one block size and two waves per SIMD (at most 256
@@ -776,7 +798,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
     switch (k) {
 #define DMR_SW16_FUSED_CASE(i, c, w, p, wall) \
       case i:                                  \
-        sw16p_task<w, p, Tr, wall>(t, ln, nl, nr, jobs, q, db, T, scores, redo, nredo); \
+        sw16p_task_call<w, p, Tr, wall>(t, ln, nl, nr, jobs, q, db, (LdsInt*)T, scores, redo, nredo); \
         break;
       DMR_SW16_FUSED(DMR_SW16_FUSED_CASE)
 #undef DMR_SW16_FUSED_CASE

@@ -113,10 +113,12 @@ def merged_cases(seed=91, count=420):
     return cases
 
 
-@pytest.mark.parametrize("f16", [1, 0], ids=["f16", "i16"])
-def test_fused_launch_band_widths(dmr, orc, f16):
+@pytest.mark.parametrize("f16,mask", [(1, 0x28), (0, 0x28), (1, 0x3F), (0, 0x3F)],
+                         ids=["f16-m193-m257", "i16-m193-m257", "f16-all", "i16-all"])
+def test_fused_launch_band_widths(dmr, orc, f16, mask):
     """The fused launch (merged bands with walls, widths 81 and 61 split over
-    two lanes) on drop-in DPs of every width it takes and of the others."""
+    two lanes) on drop-in DPs of every width it takes and of the others, with
+    two and with all six merged-band classes in use."""
     cases = merged_cases(seed=93, count=300)
     rng = np.random.default_rng(94)
     for _ in range(200):
@@ -134,9 +136,12 @@ def test_fused_launch_band_widths(dmr, orc, f16):
     c.load_tuning(None)
     c.set_tuning("f16", f16)
     c.set_tuning("fuse", 1)
+    c.set_tuning("merged_mask", mask)
     got = _score_batch(c, cases)
     prof = c.class_profile()
     assert prof["w81"]["ms"] > 0 and prof["m257"]["ms"] > 0
+    used = ["m97", "m129", "m161", "m193", "m225", "m257"] if mask == 0x3F else ["m193", "m257"]
+    assert all(prof[k]["ms"] > 0 for k in used)
     for cs, g, r in zip(cases, got, ref):
         assert g == r, (len(cs[0]), len(cs[1]), cs[2], cs[3])
     c.close()
