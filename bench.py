@@ -207,7 +207,11 @@ def leg_blastx(dmr, synth, steps):
     index_ms = ctx.build_index("default")
     cfg = dmr.SearchConfig(sensitivity="default")
     rids, sids = dmr.ids_text(w.read_ids), dmr.ids_text(w.db_ids)
-    r = ctx.blastx(w.reads, cfg)  # warmup
+    # A step is about 10 ms: a few warmups (the first calls also size the
+    # read and frame buffers) and at least 5 timed steps.
+    for _ in range(3):
+        r = ctx.blastx(w.reads, cfg)
+    steps = max(steps, 5)
     times = []
     for _ in range(steps):
         t0 = time.perf_counter()
