@@ -21,7 +21,7 @@ struct ScoreLayout {
   int plan;                     // how the class kernels share the streams (0, 1, 2)
   bool graph;                   // ungapped .. score-pass preprocessing replayed as one hipGraph
   int transpose[kPackedClasses];  // row sequence of a packed DP: 0 query, 1 fewer rows, 2 subject
-  bool fuse;                    // m257, m193, m97, w81, w61 in one launch (k_sw16p_fused)
+  bool fuse;                    // merged classes, w81, w61 in one launch (k_sw16p_fused)
 };
 // Row-sequence modes of the packed classes, 2 bits per class (k_classify).
 uint32_t transpose_modes(const ScoreLayout& L);

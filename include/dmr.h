@@ -142,9 +142,10 @@ int dmr_device_count(int* n);
  *                 past the matrix), 2 the subject; the best score of a band
  *                 is the same either way ("transpose" sets every class and
  *                 reads -1 when they differ);
- *   "fuse"        1 = the classes m257, m193, m97, w81 and w61 run in one
- *                 persistent launch (their lanes_/threads_ keys then do not
- *                 apply), the merged bands' tasks first.
+ *   "fuse"        1 = the merged-band classes in use (eight lanes per
+ *                 pair) and w81, w61 (two lanes) run in one persistent
+ *                 launch (their lanes_/threads_ keys then do not apply), the
+ *                 merged bands' tasks first, widest first.
  * Score-pass settings come from built-in defaults, then the tuning file (the
  * file DMR_TUNING names at context creation, empty for none; else
  * tuning_<arch>.json beside the library when it exists; settings per search
