@@ -43,7 +43,8 @@ EXPORTS = [
 
 # Packed score classes (tuning keys lanes_<c> / threads_<c>): band_for widths
 # and merged-band width bounds.
-PACKED_CLASSES = ["w25", "w33", "w61", "w81", "w129", "m97", "m129", "m161", "m193", "m225", "m257"]
+PACKED_CLASSES = ["w25", "w33", "w61", "w81", "w129", "m97", "m129", "m161", "m193", "m225", "m257", "m385",
+                  "m513"]
 # Score-kernel classes, in the order of dmr_last_class_profile (DMR_SCORE_CLASSES).
 SCORE_CLASSES = PACKED_CLASSES + ["wave64", "wave128", "wave256", "wave512", "wave1024", "wave2048", "rowexact"]
 

@@ -43,7 +43,8 @@ PRESETS = {
     "default": ("blastp-sensitive-1024", 1024),
     "very-sensitive": ("blastp-very-sensitive-4096", 512),
 }
-MERGED = ["m97", "m129", "m161", "m193", "m225", "m257"]
+MERGED = ["m97", "m129", "m161", "m193", "m225", "m257", "m385", "m513"]
+MERGED_ALWAYS = MERGED.index("m257")  # always in use
 
 
 def _load(name, path, pkg=False):
@@ -119,7 +120,9 @@ class Tuner:
             changed = True
             while changed:
                 changed = False
-                for i in range(len(MERGED) - 1):
+                for i in range(len(MERGED)):
+                    if i == MERGED_ALWAYS:
+                        continue
                     m = cur["merged_mask"] ^ (1 << i)
                     if consider({**cur, "merged_mask": m}, {"merged_mask": m}):
                         changed = True

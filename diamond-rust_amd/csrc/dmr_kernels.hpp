@@ -143,18 +143,24 @@ int pair_table_entries();
 // (score(qa, sA) + o, score(qa, sB) + o) as two int16 or, f16, two halves.
 void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out);
 // Packed score classes: 0-4 the band_for widths (2 * band + 1 = 25, 33, 61,
-// 81, 129); kExactClasses + i for merged bands (widths 62..256 other than 81
-// and 129) up to width 97 + 32 i, i < kMergedClasses (per-DP walls).
+// 81, 129); kExactClasses + i for merged bands (widths 62..512 other than 81
+// and 129; per-DP walls): i < 6 up to width 97 + 32 i (m97 .. m225, then
+// m257 for widths up to 256), i = 6, 7 up to 384 and 512 (m385, m513).
 constexpr int kExactClasses = 5;
-constexpr int kMergedClasses = 6;
+constexpr int kMergedClasses = 8;
+// Merged kernel always in use (m257: every merged width up to 256 is
+// packed); m385 and m513 are optional (without them, widths past 256 go to
+// the int32 wave-per-DP kernels, which finish a few long DPs sooner).
+constexpr int kMergedAlways = 5;
 constexpr int kPackedClasses = kExactClasses + kMergedClasses;
 // Width of class cls (0-4) or the width bound of a merged class.
 int sw16_width(int cls);
 // Class table of the merged widths for the merged kernels in use (bit i of
-// mask = class kExactClasses + i; the widest is always in use): a merged
-// width goes to the narrowest kernel in use that holds it (4 bits per bucket
-// of 32 widths).
-uint32_t sw16_merged_map(unsigned mask);
+// mask = class kExactClasses + i; m257 is always in use): a merged width
+// goes to the narrowest kernel in use that holds it (4 bits per width
+// bucket: 62..97, then 32 widths each up to 256, then 64 each up to 512; 15
+// where no kernel in use holds the bucket).
+uint64_t sw16_merged_map(unsigned mask);
 // A compiled packed kernel: class, its width (bound), lanes per DP pair (1 =
 // k_sw16, the whole band in one lane; otherwise k_sw16p split over that many
 // lanes) and threads per block.
@@ -168,7 +174,7 @@ bool sw16_has_variant(int cls, int lanes, int threads);
 // transpose: per packed class c, bits 2c..2c+1 say which sequence runs along
 // the rows of its DPs: 0 the query, 1 the one giving fewer rows, 2 the
 // subject (every result is the same).
-void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int seq_bits, uint32_t merged_map,
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int seq_bits, uint64_t merged_map,
                      uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st,
                      const uint64_t* njobs_dev = nullptr);
 // Lane pairing of the sorted score-pass jobs (see k_lane_heads): head flags
@@ -191,7 +197,7 @@ void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int ncla
 constexpr int kLaneSpanBits = 13;
 constexpr uint32_t kLaneSpanMax = (1u << kLaneSpanBits) - 1;
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                      uint32_t merged_map, uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st);
+                      uint64_t merged_map, uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st);
 void launch_gather_lanes(const uint2* in, const uint32_t* idx, uint32_t n, uint2* out, hipStream_t st);
 // Packed score pass of class cls over nlanes lanes (k_sw16.hip, merged
 // classes k_sw16m.hip) with the compiled kernel (cls, lanes_per_pair,
@@ -204,13 +210,13 @@ bool launch_sw16(int cls, int lanes_per_pair, int threads, bool f16, const uint2
 bool launch_sw16_merged(int cls, int lanes_per_pair, int threads, bool f16, const uint2* lanes, uint32_t nlanes,
                         uint32_t nrow, const SwJob* jobs, SeqSet q, SeqSet db, const int* pair_tab, uint32_t* counter,
                         int32_t* scores, uint32_t* redo, uint32_t* nredo, int cu_count, hipStream_t st);
-// Fused launch of the packed classes m257, m225, m193, m161, m129, m97
-// (eight lanes per DP pair, per-DP walls), w81 and w61 (two lanes per
-// pair): one persistent
+// Fused launch of the packed classes m513, m385 (sixteen lanes per DP pair),
+// m257, m225, m193, m161, m129, m97 (eight lanes; all with per-DP walls),
+// w81 and w61 (two lanes per pair): one persistent
 // grid of 512-thread blocks, kinds in that order (k_sw16p_fused). Per kind:
 // its lanes, lane count and query-row lane count; task0 are the first task
 // of each kind (computed by launch_sw16_fused).
-constexpr int kFusedKinds = 8;
+constexpr int kFusedKinds = 10;
 struct FusedSections {
   const uint2* lanes[kFusedKinds];
   uint32_t nlanes[kFusedKinds], nrow[kFusedKinds], task0[kFusedKinds];

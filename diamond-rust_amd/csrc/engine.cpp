@@ -1035,7 +1035,7 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
   layout_ = resolve_layout(opt.sensitivity);
   layout_preset_ = opt.sensitivity;
   const ScoreLayout L = layout_;
-  const uint32_t mmap = sw16_merged_map(L.merged_mask);
+  const uint64_t mmap = sw16_merged_map(L.merged_mask);
   const bool fast = opt.gap_open == 11 && opt.gap_extend == 1 && !force_generic_;
   if (fast && (!qs.pad || !qs.pad_offs || !dbs.pad || !dbs.pad_offs))
     throw Error(-2, "score pass without padded letters");
@@ -1247,9 +1247,9 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
   }
   // DPs at the 16-bit limit (f16 1900, int16 32000, or a best below 11 with
   // clamped rows that may score): re-scored by the int32 wave kernel (widths
-  // <= 256). It reads their count on the device, so no host read precedes it;
-  // one read afterwards brings both counts.
-  launch_sw_wave(4, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+  // <= 512, the widest packed class). It reads their count on the device, so
+  // no host read precedes it; one read afterwards brings both counts.
+  launch_sw_wave(8, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                  opt.gap_extend, cnt + kCntRedoWave, nullptr, scores, nullptr, nullptr, nullptr, nullptr, fallback,
                  cnt + kCntFallback, cus, st_, nullptr, cnt + kCntRedo);
   uint32_t* h_cnt = sw_pin_summary_.ensure<uint32_t>(kCntWords);

@@ -6,12 +6,23 @@ namespace dmr {
 
 namespace {
 
+// Merged-width bucket of a width in 62..512: 62..97, then 32 widths each
+// (98..129, ..., 226..256), then 64 each (257..320, ..., 449..512).
+__host__ __device__ __forceinline__ int merged_bucket(int w) {
+  return w <= 97 ? 0 : w <= 256 ? (w - 66) >> 5 : 6 + ((w - 257) >> 6);
+}
+constexpr int kMergedBuckets = 10;
+// Largest width of each bucket.
+__host__ __device__ __forceinline__ int merged_bucket_top(int b) {
+  return b < 5 ? 97 + 32 * b : b == 5 ? 256 : 256 + 64 * (b - 5);
+}
+
 // Packed class of a band width: 0-4 the widths 2 * band + 1 of the band_for
-// values (25/33/61/81/129); a merged band (widths 62..256 other than 81 and
-// 129) by its bucket b of 32 widths (62..97, 98..129, ..., 226..256): class
-// (merged >> 4b) & 15, the narrowest merged kernel in use that holds it
-// (sw16_merged_map); -1 for any other width.
-__device__ __forceinline__ int packed_class(int w, uint32_t merged) {
+// values (25/33/61/81/129); a merged band (widths 62..512 other than 81 and
+// 129) by its bucket b: class (merged >> 4b) & 15, the narrowest merged
+// kernel in use that holds it (sw16_merged_map; 15 for none: widths past 256
+// when neither m385 nor m513 is in use); -1 for any other width.
+__device__ __forceinline__ int packed_class(int w, uint64_t merged) {
   switch (w) {
     case 25: return 0;
     case 33: return 1;
@@ -19,8 +30,9 @@ __device__ __forceinline__ int packed_class(int w, uint32_t merged) {
     case 81: return 3;
     case 129: return 4;
     default:
-      if (w <= 61 || w > 256) return -1;
-      return (int)((merged >> (4 * (w <= 97 ? 0 : (w - 66) >> 5))) & 15u);
+      if (w <= 61 || w > 512) return -1;
+      const int c = (int)((merged >> (4 * merged_bucket(w))) & 15u);
+      return c == 15 ? -1 : c;
   }
 }
 
@@ -38,7 +50,7 @@ __device__ __forceinline__ int packed_class(int w, uint32_t merged) {
 // DPs' rows, and a wave as many rows as its longest lane, so neighbours
 // should start and end alike.
 __global__ void k_classify(const SwJob* __restrict__ jobs, uint64_t njobs, SeqSet q, SeqSet db,
-                           int seq_bits, uint32_t merged, uint32_t transpose, uint64_t* __restrict__ keys,
+                           int seq_bits, uint64_t merged, uint32_t transpose, uint64_t* __restrict__ keys,
                            uint32_t* __restrict__ idx, const uint64_t* __restrict__ njobs_dev) {
   uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (njobs_dev ? *njobs_dev : njobs)) return;
@@ -144,7 +156,7 @@ __global__ void k_lane_summary(const uint64_t* __restrict__ hpos, uint64_t n, in
 // orientation (subject_rows with the classification's modes), as in the
 // packed kernels; the lanes of a class with the query on the rows stay first.
 __global__ void k_lane_keys(const uint2* __restrict__ lanes, uint32_t nlanes, const SwJob* __restrict__ jobs,
-                            SeqSet q, SeqSet db, uint32_t merged, uint32_t transpose, uint64_t* __restrict__ keys,
+                            SeqSet q, SeqSet db, uint64_t merged, uint32_t transpose, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ idx) {
   const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
   if (L >= nlanes) return;
@@ -222,7 +234,7 @@ void fill_pair_table(const int8_t (&m)[25][25], bool f16, std::vector<int>* out)
       }
 }
 
-void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int seq_bits, uint32_t merged_map,
+void launch_classify(const SwJob* jobs, uint64_t njobs, SeqSet q, SeqSet db, int seq_bits, uint64_t merged_map,
                      uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st, const uint64_t* njobs_dev) {
   if (!njobs) return;
   hipLaunchKernelGGL(k_classify, dim3(nblk(njobs, 256)), dim3(256), 0, st, jobs, njobs, q, db,
@@ -251,7 +263,7 @@ void launch_lane_summary(const uint64_t* hpos, uint64_t n, int npacked, int ncla
 }
 
 void launch_lane_keys(const uint2* lanes, uint32_t nlanes, const SwJob* jobs, SeqSet q, SeqSet db,
-                      uint32_t merged_map, uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st) {
+                      uint64_t merged_map, uint32_t transpose, uint64_t* keys, uint32_t* idx, hipStream_t st) {
   if (!nlanes) return;
   hipLaunchKernelGGL(k_lane_keys, dim3(nblk(nlanes, 256)), dim3(256), 0, st, lanes, nlanes, jobs, q, db,
                      merged_map, transpose, keys, idx);
@@ -267,16 +279,18 @@ void launch_pad_letters(SeqSet s, uint32_t n, uint8_t* pad, uint32_t* pad_offs, 
   hipLaunchKernelGGL(k_pad_letters, dim3(n), dim3(256), 0, st, s, n, pad, pad_offs);
 }
 
-static const int kWidths[kPackedClasses] = {25, 33, 61, 81, 129, 97, 129, 161, 193, 225, 257};
+static const int kWidths[kPackedClasses] = {25, 33, 61, 81, 129, 97, 129, 161, 193, 225, 257, 385, 513};
 int sw16_width(int cls) { return kWidths[cls]; }
 
-uint32_t sw16_merged_map(unsigned mask) {
-  mask |= 1u << (kMergedClasses - 1);
-  uint32_t map = 0;
-  for (int b = 0; b < kMergedClasses; ++b) {
-    int i = b;
-    while (!((mask >> i) & 1u)) ++i;
-    map |= (uint32_t)(kExactClasses + i) << (4 * b);
+uint64_t sw16_merged_map(unsigned mask) {
+  // Widest width each merged kernel takes (m257 stops at 256).
+  static const int kHolds[kMergedClasses] = {97, 129, 161, 193, 225, 256, 384, 512};
+  mask |= 1u << kMergedAlways;
+  uint64_t map = 0;
+  for (int b = 0; b < kMergedBuckets; ++b) {
+    int i = 0;
+    while (i < kMergedClasses && (!((mask >> i) & 1u) || kHolds[i] < merged_bucket_top(b))) ++i;
+    map |= (uint64_t)(i < kMergedClasses ? kExactClasses + i : 15) << (4 * b);
   }
   return map;
 }

@@ -763,14 +763,16 @@ __device__ __attribute__((noinline)) void sw16p_task_call(uint32_t task, const u
 // Classes of the fused launch, as X(kind, class, width, lanes per DP pair,
 // wall), in task order: the merged bands (long DPs) first.
 #define DMR_SW16_FUSED(X) \
-  X(0, 10, 257, 8, true)  \
-  X(1, 9, 225, 8, true)   \
-  X(2, 8, 193, 8, true)   \
-  X(3, 7, 161, 8, true)   \
-  X(4, 6, 129, 8, true)   \
-  X(5, 5, 97, 8, true)    \
-  X(6, 3, 81, 2, false)   \
-  X(7, 2, 61, 2, false)
+  X(0, 12, 513, 16, true) \
+  X(1, 11, 385, 16, true) \
+  X(2, 10, 257, 8, true)  \
+  X(3, 9, 225, 8, true)   \
+  X(4, 8, 193, 8, true)   \
+  X(5, 7, 161, 8, true)   \
+  X(6, 6, 129, 8, true)   \
+  X(7, 5, 97, 8, true)    \
+  X(8, 3, 81, 2, false)   \
+  X(9, 2, 61, 2, false)
 // Fused launch of several packed classes (launch_sw16_fused): one persistent
 // grid whose waves take tasks of any of the classes, the merged bands first.
 // The classes share one block size and two waves per SIMD (at most 256
@@ -850,6 +852,8 @@ void launch_split(const uint2* lanes, uint32_t nlanes, uint32_t nrow, const SwJo
   X(9, 225, 8, 512)        \
   X(9, 225, 16, 512)       \
   X(10, 257, 8, 512)       \
-  X(10, 257, 16, 512)
+  X(10, 257, 16, 512)      \
+  X(11, 385, 16, 512)      \
+  X(12, 513, 16, 512)
 
 }  // namespace dmr

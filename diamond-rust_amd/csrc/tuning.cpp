@@ -11,8 +11,8 @@ namespace dmr {
 
 namespace {
 
-const char* const kClassNames[kPackedClasses] = {"w25", "w33", "w61", "w81", "w129", "m97",
-                                                  "m129", "m161", "m193", "m225", "m257"};
+const char* const kClassNames[kPackedClasses] = {"w25", "w33", "w61", "w81", "w129", "m97", "m129",
+                                                  "m161", "m193", "m225", "m257", "m385", "m513"};
 
 int class_of(const std::string& name) {
   for (int c = 0; c < kPackedClasses; ++c)
@@ -153,16 +153,17 @@ std::string packed_class_name(int c) { return kClassNames[c]; }
 ScoreLayout default_layout() {
   ScoreLayout L{};
   // Band_for widths: 81 and 129 split (two and four lanes per pair), the
-  // narrower widths whole in one lane; merged bands over eight lanes.
+  // narrower widths whole in one lane; merged bands over eight lanes, those
+  // wider than 256 over sixteen.
   for (int c = 0; c < kPackedClasses; ++c) {
-    L.lanes[c] = c < kExactClasses ? 1 : 8;
+    L.lanes[c] = c < kExactClasses ? 1 : (sw16_width(c) > 257 ? 16 : 8);
     L.threads[c] = c < kExactClasses ? 256 : 512;
   }
   L.lanes[3] = 2;
   L.threads[3] = 512;
   L.lanes[4] = 4;
   L.threads[4] = 768;
-  L.merged_mask = 0x28u;  // width bounds 193 and 257
+  L.merged_mask = 0x28u;  // width bounds 193 and 257 (wider bands: int32 wave kernels)
   L.f16 = true;
   L.plan = 0;
   L.graph = false;
@@ -214,8 +215,8 @@ void apply_settings(ScoreLayout* L, const Settings& s) {
       L->threads[c] = t;
     } else if (k == "merged_mask") {
       const int m = as_int(k, v);
-      if (m < 0 || m >= (1 << kMergedClasses)) bad("merged_mask is a 6-bit mask");
-      L->merged_mask = (unsigned)m | (1u << (kMergedClasses - 1));
+      if (m < 0 || m >= (1 << kMergedClasses)) bad("merged_mask is an 8-bit mask");
+      L->merged_mask = (unsigned)m | (1u << kMergedAlways);
     } else if (k == "f16") {
       L->f16 = v != 0;
     } else if (k == "graph") {

@@ -123,13 +123,15 @@ int dmr_device_count(int* n);
  *   "profile"     1 = keep per-class DP jobs and cells over every slice of a
  *                 search (for dmr_last_class_profile / dmr_last_dp_cells);
  * and the score-pass kernels, per packed class <c> (w25 w33 w61 w81 w129:
- * band_for widths; m97 m129 m161 m193 m225 m257: merged bands up to that
- * width):
+ * band_for widths; m97 m129 m161 m193 m225 m257 m385 m513: merged bands up
+ * to that width, m257 up to 256):
  *   "lanes_<c>"   lanes per DP pair (1 = the whole band in one lane, w*
  *                 only; else a compiled split kernel, dmr_sw16_variants);
  *                 also sets the class's first compiled block size for it;
  *   "threads_<c>" threads per block of a compiled kernel for those lanes;
- *   "merged_mask" merged kernels in use (bit i = m97 + 32 i; m257 always):
+ *   "merged_mask" merged kernels in use (bit i = class m97 .. m513 in that
+ *                 order; m257 always; without m385 and m513, bands wider
+ *                 than 256 run the int32 wave-per-DP kernels):
  *                 a merged band goes to the narrowest one that holds it;
  *   "f16"         1 = packed pairs in f16, 0 = saturating int16;
  *   "score_plan"  how the score-class kernels share the streams (0, 1, 2);
@@ -329,13 +331,14 @@ int dmr_result_stats(const dmr_result* r, double* out21);
 int dmr_last_dp_cells(dmr_ctx* ctx, uint64_t* cells);
 /* Score-pass profile of the last search on ctx, per kernel class
  * (DMR_SCORE_CLASSES entries): 0-4 packed 16-bit kernel for band widths
- * 25/33/61/81/129, 5-10 packed kernel for merged bands of widths up to
- * 97/129/161/193/225/257 (those in use; see "merged_mask"), 11-16
- * wave-per-DP int32 kernel for widths up to 64/128/256/512/1024/2048, 17
- * block-per-DP and row-exact int32 kernels (wider bands, and bands that end
- * before their clamped rows): DP jobs, DP cells and kernel milliseconds.
- * Diagnostics only; no counterpart in the reference. */
-#define DMR_SCORE_CLASSES 18
+ * 25/33/61/81/129, 5-12 packed kernel for merged bands of widths up to
+ * 97/129/161/193/225/256/384/512 (those in use; see "merged_mask"), 13-18
+ * wave-per-DP int32 kernel for widths up to 64/128/256/512/1024/2048 (bands
+ * the packed kernels do not take), 19 block-per-DP and row-exact int32
+ * kernels (wider bands, and bands that end before their clamped rows): DP
+ * jobs, DP cells and kernel milliseconds. Diagnostics only; no counterpart in
+ * the reference. */
+#define DMR_SCORE_CLASSES 20
 int dmr_last_class_profile(dmr_ctx* ctx, uint64_t* jobs, uint64_t* cells, double* ms);
 /* Layout of the packed classes 0-4 in the last search on ctx: bit c of
  * *split_mask set = class c ran split over several lanes per DP pair

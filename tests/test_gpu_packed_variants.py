@@ -92,15 +92,16 @@ def test_scores_around_f16_limit(layout_ctx, orc):
         assert g == r, (len(c[0]), len(c[1]), c[2], c[3], r)
 
 
-def merged_cases(seed=91, count=420):
-    """Bands of widths 62..256 (merged bands), with every width near a
+def merged_cases(seed=91, count=640):
+    """Bands of widths 62..512 (merged bands), with every width near a
     merged-class edge and the band_for widths between them."""
     rng = np.random.default_rng(seed)
     edges = [60, 61, 62, 63, 64, 65, 80, 81, 82, 96, 97, 98, 100, 127, 128, 129, 130, 131, 160, 161, 162,
-             192, 193, 194, 195, 224, 225, 226, 255, 256, 257, 258]
+             192, 193, 194, 195, 224, 225, 226, 255, 256, 257, 258, 320, 321, 383, 384, 385, 386, 448, 449,
+             511, 512, 513, 514]
     cases = []
     for k in range(count):
-        w = edges[k] if k < len(edges) else int(rng.integers(62, 257))
+        w = edges[k] if k < len(edges) else int(rng.integers(62, 513))
         n = int(rng.integers(1, 1000 if rng.random() < 0.2 else 400))
         q = rand_seq(rng, n, canonical_only=rng.random() < 0.9)
         r = rng.random()
@@ -113,12 +114,12 @@ def merged_cases(seed=91, count=420):
     return cases
 
 
-@pytest.mark.parametrize("f16,mask", [(1, 0x28), (0, 0x28), (1, 0x3F), (0, 0x3F)],
+@pytest.mark.parametrize("f16,mask", [(1, 0xA8), (0, 0xA8), (1, 0xFF), (0, 0xFF)],
                          ids=["f16-m193-m257", "i16-m193-m257", "f16-all", "i16-all"])
 def test_fused_launch_band_widths(dmr, orc, f16, mask):
     """The fused launch (merged bands with walls, widths 81 and 61 split over
     two lanes) on drop-in DPs of every width it takes and of the others, with
-    two and with all six merged-band classes in use."""
+    three and with all eight merged-band classes in use."""
     cases = merged_cases(seed=93, count=300)
     rng = np.random.default_rng(94)
     for _ in range(200):
@@ -140,7 +141,8 @@ def test_fused_launch_band_widths(dmr, orc, f16, mask):
     got = _score_batch(c, cases)
     prof = c.class_profile()
     assert prof["w81"]["ms"] > 0 and prof["m257"]["ms"] > 0
-    used = ["m97", "m129", "m161", "m193", "m225", "m257"] if mask == 0x3F else ["m193", "m257"]
+    used = (["m97", "m129", "m161", "m193", "m225", "m257", "m385", "m513"] if mask == 0xFF
+            else ["m193", "m257", "m513"])
     assert all(prof[k]["ms"] > 0 for k in used)
     for cs, g, r in zip(cases, got, ref):
         assert g == r, (len(cs[0]), len(cs[1]), cs[2], cs[3])
@@ -148,17 +150,19 @@ def test_fused_launch_band_widths(dmr, orc, f16, mask):
 
 
 def test_merged_band_widths(layout_ctx, orc):
-    """Bands of widths 62..256 other than 81 and 129 (merged bands) run the
-    split kernels with a wall at each DP's own width (the default merged
-    kernels, m193 and m257, and then all six); 61, 81, 129 and 257+ stay on
-    their own kernels."""
+    """Bands of widths 62..512 other than 81 and 129 (merged bands) run the
+    split kernels with a wall at each DP's own width (m193 and m257, with
+    wider bands on the int32 wave kernels; m193, m257 and m513; then all
+    eight merged kernels); 61, 81, 129 and 513+ stay on their own kernels."""
     cases = merged_cases()
     assert sum(61 < c[3] - c[2] + 1 <= 128 for c in cases) > 50
     assert sum(129 < c[3] - c[2] + 1 <= 193 for c in cases) > 50
     assert sum(193 < c[3] - c[2] + 1 <= 256 for c in cases) > 50
+    assert sum(256 < c[3] - c[2] + 1 <= 384 for c in cases) > 50
+    assert sum(384 < c[3] - c[2] + 1 <= 512 for c in cases) > 50
     ref = [orc.extend_span(*c) for c in cases]
     ref = [r["score"] if r else None for r in ref]
-    for mask in (0x28, 0x3F):
+    for mask in (0x28, 0xA8, 0xFF):
         layout_ctx.set_tuning("merged_mask", mask)
         got = _score_batch(layout_ctx, cases)
         for c, g, r in zip(cases, got, ref):
@@ -203,7 +207,7 @@ def variant_cases(orc):
     merged = merged_cases(seed=405, count=900)
     out = {}
     bounds = {"m97": (62, 97), "m129": (98, 129), "m161": (130, 161), "m193": (162, 193), "m225": (194, 225),
-              "m257": (226, 256)}
+              "m257": (226, 256), "m385": (257, 384), "m513": (385, 512)}
     for name, (lo, hi) in bounds.items():
         out[name] = [c for c in merged if lo <= c[3] - c[2] + 1 <= hi and c[3] - c[2] + 1 not in (81, 129)]
     for name in EXACT:
@@ -234,7 +238,7 @@ def test_every_compiled_kernel(dmr, variant_cases, f16, transpose):
     c.load_tuning(None)
     c.set_tuning("f16", f16)
     c.set_tuning("transpose", transpose)
-    c.set_tuning("merged_mask", 0x3F)
+    c.set_tuning("merged_mask", 0xFF)
     seen = 0
     for v in dmr.sw16_variants():
         c.set_tuning(f"lanes_{v['cls']}", v["lanes"])
@@ -246,5 +250,5 @@ def test_every_compiled_kernel(dmr, variant_cases, f16, transpose):
         for cs, g, r in zip(cases, got, ref):
             assert g == r, (v, len(cs[0]), len(cs[1]), cs[2], cs[3])
         seen += 1
-    assert seen == len(dmr.sw16_variants()) >= 26
+    assert seen == len(dmr.sw16_variants()) >= 28
     c.close()

@@ -55,7 +55,7 @@ def test_refusals_change_nothing(dmr, tmp_path):
     ctx.load_tuning(None)
     before = ctx.layout()
     for key, value, msg in [("lanes_w129", 3, "no compiled kernel"), ("threads_w25", 512, "no compiled kernel"),
-                            ("lanes_m97", 1, "no compiled kernel"), ("merged_mask", 64, "6-bit"),
+                            ("lanes_m97", 1, "no compiled kernel"), ("merged_mask", 256, "8-bit"),
                             ("score_plan", 3, "score_plan"), ("lanes_w130", 4, "unknown tuning key"),
                             ("split_mask", 1, "unknown tuning key")]:
         with pytest.raises(dmr.DmrError, match=msg):
