@@ -17,7 +17,10 @@ guard): f16; the graph; the fused launch, alone and with every merged-band bucke
 use; the merged-band kernels in use (one bucket bound added or removed at a time, until
 no change helps); the row sequence
 (transpose_<class>) and the kernel of each class that has DPs in the
-workload; the stream plan; the merged buckets once more.
+workload; the stream plan; the merged buckets once more. A trial's time is
+the median of --reps searches, or, for the preset served with batches in
+flight (IN_FLIGHT: the headline sensitive preset, two batches), the wall time
+per batch with that many contexts sharing the database searching at once.
 
 Usage (on a GPU box; the workloads are the seeded synthetic ones):
   python diamond-rust_amd/autotune.py                       # every preset below
@@ -43,6 +46,9 @@ PRESETS = {
     "default": ("blastp-sensitive-1024", 1024),
     "very-sensitive": ("blastp-very-sensitive-4096", 512),
 }
+# The headline preset is served with two batches in flight (its throughput is
+# what counts); the others are tuned for a search alone.
+IN_FLIGHT = {"sensitive": 2}
 MERGED = ["m97", "m129", "m161", "m193", "m225", "m257", "m385", "m513"]
 MERGED_ALWAYS = MERGED.index("m257")  # always in use
 
@@ -61,16 +67,45 @@ def log(*a):
 
 
 class Tuner:
-    def __init__(self, dmr, ctx, cfg, ids, reps, min_gain):
-        self.dmr, self.ctx, self.cfg, self.ids = dmr, ctx, cfg, ids
+    """ctxs[0] is the context tuned; with several contexts (sharing its
+    database), a trial's time is the wall time per batch with one batch in
+    flight on each (host threads), the way a serving loop or the benchmark's
+    headline runs them; with one, the time of a search alone."""
+
+    def __init__(self, dmr, ctxs, cfg, ids, reps, min_gain):
+        self.dmr, self.ctxs, self.cfg, self.ids = dmr, ctxs, cfg, ids
+        self.ctx = ctxs[0]
         self.reps, self.min_gain = reps, min_gain
         self.trials = []
         self.ref_text = None
 
     def apply(self, settings):
         # Lanes before threads: a lanes_ key resets its class's block size.
-        for k in sorted(settings, key=lambda k: (not k.startswith("lanes_"), k)):
-            self.ctx.set_tuning(k, settings[k])
+        for c in self.ctxs:
+            for k in sorted(settings, key=lambda k: (not k.startswith("lanes_"), k)):
+                c.set_tuning(k, settings[k])
+
+    def _in_flight(self):
+        import threading
+        errors = []
+
+        def work(c):
+            try:
+                for _ in range(2):
+                    if c.search_loaded(self.cfg).format(self.ids[0], self.ids[1]) != self.ref_text:
+                        errors.append("text differs")
+            except Exception as e:  # reported after the join
+                errors.append(f"{type(e).__name__}: {e}")
+
+        th = [threading.Thread(target=work, args=(c,)) for c in self.ctxs]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if errors:
+            raise SystemExit(f"autotune: concurrent searches failed: {errors[0]}")
+        return (time.perf_counter() - t0) * 1e3 / (2 * len(self.ctxs))
 
     def measure(self, settings, changed):
         self.apply(settings)
@@ -80,11 +115,17 @@ class Tuner:
             self.ref_text = text
         elif text != self.ref_text:
             raise SystemExit(f"autotune: the output changed under {changed}; no setting may change a result")
+        for c in self.ctxs[1:]:
+            c.search_loaded(self.cfg)  # warm-up
         times, sw = [], []
         for _ in range(self.reps):
-            t0 = time.perf_counter()
-            r = self.ctx.search_loaded(self.cfg)
-            times.append((time.perf_counter() - t0) * 1e3)
+            if len(self.ctxs) > 1:
+                times.append(self._in_flight())
+                r = self.ctx.search_loaded(self.cfg)
+            else:
+                t0 = time.perf_counter()
+                r = self.ctx.search_loaded(self.cfg)
+                times.append((time.perf_counter() - t0) * 1e3)
             sw.append(r.stats()["ms_sw"])
         ms = statistics.median(times)
         self.trials.append(dict(changed=changed, ms=round(ms, 3), score_pass_ms=round(statistics.median(sw), 3)))
@@ -152,24 +193,31 @@ class Tuner:
         return cur, best, default_ms
 
 
-def tune_preset(dmr, synth, preset, reps, min_gain, queries=None):
+def tune_preset(dmr, synth, preset, reps, min_gain, queries=None, in_flight=1):
     wl, nq = PRESETS[preset]
     nq = queries or nq
-    log(f"preset {preset}: workload {wl}, {nq} queries")
+    log(f"preset {preset}: workload {wl}, {nq} queries, {in_flight} in flight")
     w = synth.make(wl)
     if nq < w.n_q:
         w = w.subset_queries(list(range(nq)))
-    ctx = dmr.Context(0)
-    ctx.load_tuning(None)  # trials start from the built-in defaults
-    ctx.db_load(w.db_letters, w.db_offsets)
-    ctx.queries_load(w.q_letters, w.q_offsets)
+    ctxs = [dmr.Context(0)]
+    ctxs[0].load_tuning(None)  # trials start from the built-in defaults
+    ctxs[0].db_load(w.db_letters, w.db_offsets)
+    ctxs[0].queries_load(w.q_letters, w.q_offsets)
+    for _ in range(in_flight - 1):
+        c = dmr.Context(0)
+        c.load_tuning(None)
+        c.share_db(ctxs[0])
+        c.queries_load(w.q_letters, w.q_offsets)
+        ctxs.append(c)
     cfg = dmr.SearchConfig(sensitivity=preset)
-    t = Tuner(dmr, ctx, cfg, (w.q_ids, w.db_ids), reps, min_gain)
+    t = Tuner(dmr, ctxs, cfg, (dmr.ids_text(w.q_ids), dmr.ids_text(w.db_ids)), reps, min_gain)
     layout, ms, default_ms = t.run(dmr.sw16_variants())
-    ctx.close()
+    for c in reversed(ctxs):
+        c.close()
     log(f"preset {preset}: {default_ms:.2f} ms with the defaults -> {ms:.2f} ms")
-    return dict(settings=layout, workload=wl, queries=nq, reps=reps, min_gain=min_gain, ms=round(ms, 3),
-                default_ms=round(default_ms, 3), trials=t.trials,
+    return dict(settings=layout, workload=wl, queries=nq, in_flight=in_flight, reps=reps, min_gain=min_gain,
+                ms=round(ms, 3), default_ms=round(default_ms, 3), trials=t.trials,
                 tuned_at=datetime.datetime.now(datetime.timezone.utc).strftime("%Y-%m-%dT%H:%M:%SZ"))
 
 
@@ -179,6 +227,8 @@ def main(argv=None):
     ap.add_argument("--queries", type=int, default=None, help="queries of the workload used (default per preset)")
     ap.add_argument("--reps", type=int, default=5, help="timed searches per trial (median)")
     ap.add_argument("--min-gain", type=float, default=0.01, help="fraction a trial must gain to be kept")
+    ap.add_argument("--in-flight", type=int, default=None,
+                    help="batches in flight per trial (contexts sharing the database; default per preset)")
     ap.add_argument("--arch", default=os.environ.get("DMR_OFFLOAD_ARCH", "gfx950"))
     ap.add_argument("--out", default=None, help="tuning file (default: tuning_<arch>.json beside libdmr.so)")
     a = ap.parse_args(argv)
@@ -191,7 +241,8 @@ def main(argv=None):
     doc["library"] = dmr.lib().dmr_version().decode()
     doc["variants"] = [[v["cls"], v["width"], v["lanes"], v["threads"]] for v in dmr.sw16_variants()]
     for p in a.preset or list(PRESETS):
-        doc["presets"][p] = tune_preset(dmr, synth, p, a.reps, a.min_gain, a.queries)
+        doc["presets"][p] = tune_preset(dmr, synth, p, a.reps, a.min_gain, a.queries,
+                                        a.in_flight or IN_FLIGHT.get(p, 1))
         tmp = out.with_suffix(".json.tmp")
         tmp.write_text(json.dumps(doc, indent=1) + "\n")
         os.replace(tmp, out)
