@@ -18,9 +18,8 @@ use; the merged-band kernels in use (one bucket bound added or removed at a time
 no change helps); the row sequence
 (transpose_<class>) and the kernel of each class that has DPs in the
 workload; the stream plan; the merged buckets once more. A trial's time is
-the median of --reps searches, or, for the preset served with batches in
-flight (IN_FLIGHT: the headline sensitive preset, two batches), the wall time
-per batch with that many contexts sharing the database searching at once.
+the median of --reps searches, or, with --in-flight N, the wall time per
+batch with N contexts sharing the database searching at once.
 
 Usage (on a GPU box; the workloads are the seeded synthetic ones):
   python diamond-rust_amd/autotune.py                       # every preset below
@@ -46,9 +45,11 @@ PRESETS = {
     "default": ("blastp-sensitive-1024", 1024),
     "very-sensitive": ("blastp-very-sensitive-4096", 512),
 }
-# The headline preset is served with two batches in flight (its throughput is
-# what counts); the others are tuned for a search alone.
-IN_FLIGHT = {"sensitive": 2}
+# Batches in flight per trial, by preset (default 1: a search alone). With two
+# in flight the trials of the headline preset differ by less than their
+# spread (about 1 %), so it too is tuned on a search alone; --in-flight
+# measures the other way.
+IN_FLIGHT = {}
 MERGED = ["m97", "m129", "m161", "m193", "m225", "m257", "m385", "m513"]
 MERGED_ALWAYS = MERGED.index("m257")  # always in use
 
