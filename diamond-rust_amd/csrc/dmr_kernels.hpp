@@ -282,11 +282,15 @@ void launch_ops_pack(const SwOut* outs, const uint32_t* ops, const uint64_t* ops
 // Wave-per-DP int32 kernel over jobs[list[0..njobs)]: score mode writes
 // scores[job]; trace mode writes outs[task] and the task's ops, stopping each
 // DP at the first cell that reaches known[job] (the score pass's scores).
+// list_sel 1 / 2: the list holds tagged entries (bit 31 set for bands wider
+// than 256, the packed kernels' redo list) and the kernel takes only the
+// untagged / tagged ones.
 void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* list, uint32_t njobs,
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, int32_t* scores, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
                     int cu_count, hipStream_t st, const int32_t* known = nullptr,
-                    const uint32_t* njobs_dev = nullptr);  // njobs_dev: a device-side count (njobs bounds it)
+                    const uint32_t* njobs_dev = nullptr,  // njobs_dev: a device-side count (njobs bounds it)
+                    uint32_t list_sel = 0);
 
 }  // namespace dmr

@@ -1105,7 +1105,8 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
   // packed and wave class, the fallback list's length, the redo kernel's
   // task counter.
   constexpr int kCntRedo = 0, kCntPacked = 1, kCntWave = kCntPacked + kPackedClasses,
-                kCntFallback = kCntWave + kWaveKinds, kCntRedoWave = kCntFallback + 1, kCntWords = kCntRedoWave + 1;
+                kCntFallback = kCntWave + kWaveKinds, kCntRedoWave = kCntFallback + 1, kCntRedoWide = kCntRedoWave + 1,
+                kCntWords = kCntRedoWide + 1;
   uint32_t* cnt = sw_counters_.ensure<uint32_t>(kCntWords);
   launch_fill_u32(cnt, kCntWords, 0, st_);
   uint32_t* redo = sw_redo_.ensure<uint32_t>(njobs + 1);
@@ -1246,12 +1247,24 @@ uint64_t Engine::score_pass(const SwJob* jobs, uint64_t njobs, int seq_bits, Seq
     DMR_HIP(hipEventRecord(ev_[13 + 2 * kRowExact], st_));
   }
   // DPs at the 16-bit limit (f16 1900, int16 32000, or a best below 11 with
-  // clamped rows that may score): re-scored by the int32 wave kernel (widths
-  // <= 512, the widest packed class). It reads their count on the device, so
-  // no host read precedes it; one read afterwards brings both counts.
-  launch_sw_wave(8, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+  // clamped rows that may score): re-scored by the int32 wave kernel, bands
+  // up to 256 wide with four offsets per lane and, side by side on an
+  // auxiliary stream, the tagged wider ones (the m385 and m513 classes) with
+  // eight. They read the list's length on the device, so no host read
+  // precedes them; one read afterwards brings the counts.
+  // Tagged entries exist only when a wide class ran, and then its re-run
+  // is always launched.
+  const bool wide_redo = nl && (class_lanes[kPackedClasses - 1] || class_lanes[kPackedClasses - 2]);
+  const bool side = wide_redo && !score_serial_;
+  if (side) fork();
+  if (wide_redo)
+    launch_sw_wave(8, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
+                   opt.gap_extend, cnt + kCntRedoWide, nullptr, scores, nullptr, nullptr, nullptr, nullptr, fallback,
+                   cnt + kCntFallback, cus, side ? aux_[0] : st_, nullptr, cnt + kCntRedo, 2);
+  launch_sw_wave(4, false, jobs, redo, (uint32_t)njobs, qs, dbs, matrix32_.as<int8_t>(), opt.gap_open,
                  opt.gap_extend, cnt + kCntRedoWave, nullptr, scores, nullptr, nullptr, nullptr, nullptr, fallback,
-                 cnt + kCntFallback, cus, st_, nullptr, cnt + kCntRedo);
+                 cnt + kCntFallback, cus, st_, nullptr, cnt + kCntRedo, 1);
+  if (side) join();
   uint32_t* h_cnt = sw_pin_summary_.ensure<uint32_t>(kCntWords);
   DMR_HIP(hipMemcpyAsync(h_cnt, cnt, kCntWords * 4, hipMemcpyDeviceToHost, st_));
   DMR_HIP(hipStreamSynchronize(st_));

@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
                                                  uint32_t* __restrict__ fallback,
                                                  uint32_t* __restrict__ nfallback,
                                                  const int32_t* __restrict__ known,
-                                                 const uint32_t* __restrict__ njobs_dev) {
+                                                 const uint32_t* __restrict__ njobs_dev, uint32_t list_sel) {
   __shared__ int8_t M[kScoreStride * kScoreStride];
   for (int k = threadIdx.x; k < kScoreStride * kScoreStride; k += blockDim.x) M[k] = m32[k];
   __syncthreads();
@@ -261,7 +261,12 @@ __global__ __launch_bounds__(256) void k_sw_wave(const SwJob* __restrict__ jobs,
     // iteration into the next while lane 0 is inactive.)
     const uint32_t task = uniform(atomicAdd(counter, lane == 0 ? 1u : 0u));
     if (task >= njobs || (njobs_dev && task >= uniform(*njobs_dev))) break;
-    const uint32_t jid = uniform(list[task]);
+    uint32_t entry = uniform(list[task]);
+    if (list_sel) {  // a list of tagged entries (bit 31): take one kind only
+      if ((entry >> 31) != (list_sel == 2 ? 1u : 0u)) continue;
+      entry &= 0x7FFFFFFFu;
+    }
+    const uint32_t jid = entry;
     const uint32_t tgt = uniform(jobs[jid].target), sub = uniform(jobs[jid].subject);
     const uint32_t q0 = uniform(jobs[jid].q0), q1 = uniform(jobs[jid].q1);
     const int d0 = uniform(jobs[jid].d0), d1 = uniform(jobs[jid].d1);
@@ -548,7 +553,8 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
                     SeqSet q, SeqSet db, const int8_t* m32, int gap_open, int gap_extend,
                     uint32_t* counter, SwOut* outs, int32_t* scores, uint8_t* trace, const uint64_t* trace_off,
                     uint32_t* ops, const uint64_t* ops_off, uint32_t* fallback, uint32_t* nfallback,
-                    int cu_count, hipStream_t st, const int32_t* known, const uint32_t* njobs_dev) {
+                    int cu_count, hipStream_t st, const int32_t* known, const uint32_t* njobs_dev,
+                    uint32_t list_sel) {
   if (!njobs) return;
 #define DMR_SWW(k, tr)                                                                             \
   do {                                                                                             \
@@ -556,7 +562,7 @@ void launch_sw_wave(int K, bool traceback, const SwJob* jobs, const uint32_t* li
     const unsigned blocks = std::min<unsigned>(resident, nblk(njobs, 4));                          \
     hipLaunchKernelGGL((k_sw_wave<k, tr>), dim3(blocks), dim3(256), 0, st, jobs, list, njobs, q,   \
                        db, m32, gap_open, gap_extend, counter, outs, scores, trace, trace_off, ops,\
-                       ops_off, fallback, nfallback, known, njobs_dev);                            \
+                       ops_off, fallback, nfallback, known, njobs_dev, list_sel);                  \
   } while (0)
   if (traceback) {
     switch (K) {

@@ -709,17 +709,20 @@ __device__ __forceinline__ void sw16p_task(uint32_t task, const uint2* __restric
   if (P >= 8) best = Tr::max(best, mv(best, MirrorC{}));
   if (P == 16) best = Tr::max(best, mv(best, RowMirrorC{}));
   const int bA = Tr::lo(best), bB = Tr::hi(best);
+  // Redo entries of bands wider than 256 carry bit 31: the int32 re-run
+  // takes them with a wider wave kernel than the rest.
+  constexpr uint32_t kWide = W > 257 ? 0x80000000u : 0u;
   if (p == 0) {
     if (hasA) {
       if (bA >= Tr::kRedo || (bA < 11 && lp.clampA)) {
-        redo[atomicAdd(nredo, 1u)] = ln.x;
+        redo[atomicAdd(nredo, 1u)] = ln.x | kWide;
       } else {
         scores[ln.x] = bA;
       }
     }
     if (hasB) {
       if (bB >= Tr::kRedo || (bB < 11 && lp.clampB)) {
-        redo[atomicAdd(nredo, 1u)] = ln.y;
+        redo[atomicAdd(nredo, 1u)] = ln.y | kWide;
       } else {
         scores[ln.y] = bB;
       }
