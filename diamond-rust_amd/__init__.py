@@ -543,7 +543,7 @@ class Context:
 
     def class_profile(self):
         """Per score-kernel class of the last search: packed 16-bit by band width
-        (w25..w129, merged bands m193/m256), wave-per-DP int32 by width bound
+        (w25..w129, merged bands m97..m513), wave-per-DP int32 by width bound
         (wave64..wave2048), and the row-exact int32 kernel: jobs, DP cells,
         kernel ms."""
         names = SCORE_CLASSES

@@ -176,8 +176,8 @@ struct StageStats {
   uint64_t jobs = 0, dp_cells = 0, maybe = 0, selected = 0, traceback_cells = 0;
   uint64_t sw16_lanes = 0, sw_generic = 0, sw_redo = 0;
   // Score-kernel time per class: 0-4 packed 16-bit (widths 25/33/61/81/129),
-  // 5-6 packed 16-bit merged bands, 7-12 wave-per-DP int32 (K = 1..32), 13
-  // row-exact int32.
+  // 5-12 packed 16-bit merged bands (m97 .. m513), 13-18 wave-per-DP int32
+  // (K = 1..32), 19 row-exact int32.
   double ms_class[kPackedClasses + kWaveKinds + 1] = {};
   double ms_seed = 0, ms_join = 0, ms_sort_hits = 0, ms_ungapped = 0, ms_bands = 0, ms_sw = 0,
          ms_select = 0, ms_traceback = 0, ms_total = 0, ms_host = 0;

@@ -36,8 +36,8 @@ using Settings = std::map<std::string, double>;
 
 // Applies settings to L: "lanes_<class>" (a compiled lane count; also sets
 // the class's default block size for it), then "threads_<class>" (a compiled
-// block size for the class's lanes), "merged_mask" (6 bits; the widest merged
-// kernel is always in use), "f16" (0/1), "score_plan" (0-2), "graph" (0/1),
+// block size for the class's lanes), "merged_mask" (8 bits, m97 .. m513; m257
+// is always in use), "f16" (0/1), "score_plan" (0-2), "graph" (0/1),
 // "transpose_<class>" (0-2: which sequence runs along the rows of the
 // class's DPs), "transpose" (every class's; reads -1 when they differ),
 // "fuse" (0/1).
