@@ -332,9 +332,15 @@ class Result:
         st = ids_text(subject_titles) if subject_titles is not None else None
         f = fields.encode() if isinstance(fields, str) else " ".join(fields).encode()
         n = C.c_uint64()
-        _check(lib().dmr_result_format(self._h, q, s, qt, st, f, None, 0, C.byref(n)))
-        buf = C.create_string_buffer(n.value + 1)
-        _check(lib().dmr_result_format(self._h, q, s, qt, st, f, buf, n.value + 1, C.byref(n)))
+        # One call when the text fits a buffer of ~160 bytes per line (the
+        # default fields take ~80); else the size, then the fill.
+        guess = 160 * len(self) + 256
+        buf = C.create_string_buffer(guess)
+        rc = lib().dmr_result_format(self._h, q, s, qt, st, f, buf, guess, C.byref(n))
+        if rc == DMR_E_CAPACITY:
+            buf = C.create_string_buffer(n.value + 1)
+            rc = lib().dmr_result_format(self._h, q, s, qt, st, f, buf, n.value + 1, C.byref(n))
+        _check(rc)
         return C.string_at(buf, n.value).decode()
 
     def stats(self):

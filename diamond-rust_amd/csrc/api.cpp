@@ -1261,30 +1261,38 @@ int dmr_result_format(const dmr_result* r, const char* qids, const char* sids, c
     // title list is kept as "\x01", which no list of lines equals.
     auto kept_as = [](const char* s) { return std::string(s ? s : "\x01"); };
     auto same = [&](const std::string& kept, const char* s) { return kept == kept_as(s); };
+    bool fresh = false;
     if (!r->fmt_valid || !same(r->fmt_q, qids) || !same(r->fmt_s, sids) || !same(r->fmt_qt, qtitles) ||
         !same(r->fmt_st, stitles) || !same(r->fmt_f, fields)) {
       r->fmt_valid = false;
       const int rc = render(r, qids, sids, qtitles, stitles, fields, &r->fmt_parts);
       if (rc != DMR_OK) return rc;
-      r->fmt_q = kept_as(qids);
-      r->fmt_s = kept_as(sids);
-      r->fmt_qt = kept_as(qtitles);
-      r->fmt_st = kept_as(stitles);
-      r->fmt_f = kept_as(fields);
       r->fmt_len = 0;
       for (const auto& pt : r->fmt_parts) r->fmt_len += pt.size();
-      r->fmt_valid = true;
+      fresh = true;
     }
     *len = r->fmt_len;
-    if (buf && cap >= r->fmt_len) {
+    const bool fits = buf && cap >= r->fmt_len;
+    if (fits) {
       char* o = buf;
       for (const auto& pt : r->fmt_parts) {
         std::memcpy(o, pt.data(), pt.size());
         o += pt.size();
       }
-    } else if (buf && cap < r->fmt_len) {
-      return fail(DMR_E_CAPACITY, "buffer too small");
     }
+    // A request that did not fit (or asked for the size) keeps the text and
+    // the lists it came from, for the fill that follows; one served in a
+    // single call keeps nothing (the lists of a large database run to
+    // megabytes).
+    if (fresh && !fits) {
+      r->fmt_q = kept_as(qids);
+      r->fmt_s = kept_as(sids);
+      r->fmt_qt = kept_as(qtitles);
+      r->fmt_st = kept_as(stitles);
+      r->fmt_f = kept_as(fields);
+      r->fmt_valid = true;
+    }
+    if (buf && !fits) return fail(DMR_E_CAPACITY, "buffer too small");
     return DMR_OK;
   });
 }
